@@ -1,0 +1,166 @@
+/*
+ * samos_hip.h -- C-ABI of the MI355X (gfx950) SAMoS particle pipeline.
+ *
+ * One context owns one particle system resident in HBM on one GPU: the
+ * structure-of-arrays state, the cell list and neighbour list, the registered
+ * pair potentials / pair aligners / integrators / constraint, and one HIP
+ * stream.  All functions return 0 on success and a negative status on error;
+ * samos_last_error() then returns the message (the reference's
+ * Messenger::ERROR text where one exists).  No function throws.  A context is
+ * not re-entrant: one host thread drives it.
+ *
+ * Host data crosses this boundary as plain arrays in the reference's particle
+ * id order (id = row of the input file).  Indices are 0-based; particle types
+ * are 1-based as in the reference input format.
+ *
+ * Each entry point names the reference interface it replaces.  Paths are
+ * relative to the SAMoS source tree.
+ */
+#ifndef SAMOS_HIP_H
+#define SAMOS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct samos_ctx samos_ctx;
+
+/* Particle columns (src/system/particle.hpp:257-293).  Codes below 32 are
+ * double columns, codes from 32 are int columns. */
+enum samos_field {
+  SAMOS_X = 0, SAMOS_Y, SAMOS_Z, SAMOS_VX, SAMOS_VY, SAMOS_VZ,
+  SAMOS_NX, SAMOS_NY, SAMOS_NZ, SAMOS_FX, SAMOS_FY, SAMOS_FZ,
+  SAMOS_TAUX, SAMOS_TAUY, SAMOS_TAUZ, SAMOS_OMEGA, SAMOS_AGE,
+  SAMOS_RADIUS, SAMOS_MASS,
+  SAMOS_TYPE = 32, SAMOS_IX, SAMOS_IY, SAMOS_IZ, SAMOS_COORD
+};
+
+enum samos_constraint_kind { SAMOS_CONSTRAINT_NONE = 0, SAMOS_CONSTRAINT_PLANE = 1, SAMOS_CONSTRAINT_SPHERE = 2 };
+enum samos_pair_kind { SAMOS_PAIR_SOFT = 0, SAMOS_PAIR_LJ = 1 };
+enum samos_align_kind { SAMOS_ALIGN_POLAR = 0, SAMOS_ALIGN_NEMATIC = 1 };
+enum samos_brownian_kind { SAMOS_BROWNIAN = 0, SAMOS_BROWNIAN_POS = 1, SAMOS_BROWNIAN_ALIGN = 2 };
+enum samos_langevin_method { SAMOS_LANGEVIN_BAOAB = 0, SAMOS_LANGEVIN_SPV = 1, SAMOS_LANGEVIN_BBK = 2 };
+enum samos_value_kind { SAMOS_VALUE_CONSTANT = 0, SAMOS_VALUE_LINEAR = 1 };
+
+/* samos_run flags */
+#define SAMOS_RUN_FAITHFUL   1  /* evaluate forces exactly where the reference does (no fused step) */
+#define SAMOS_RUN_NO_GRAPH   2  /* launch step by step instead of replaying captured chunks */
+#define SAMOS_RUN_FP32_FORCE 4  /* pair arithmetic in FP32, sums and state in FP64 */
+
+/* ---- context and state ---------------------------------------------------
+ * System(input, msg, box) (src/system/system.cpp:64-452) and Box(lx, ly, lz)
+ * (src/system/box.hpp:56-62): a system of n particles in a box centred at the
+ * origin; periodic as set by `box periodic` / `box fixed` (src/samos.cpp:188-226).
+ * New particles have type 1, radius 1, mass 1 and every other column 0. */
+int samos_create(samos_ctx** out, int device, int n, double lx, double ly, double lz, int periodic);
+int samos_destroy(samos_ctx* ctx);
+const char* samos_last_error(const samos_ctx* ctx); /* ctx may be NULL: last create error */
+int samos_num_particles(const samos_ctx* ctx);
+
+/* Upload / download one column in particle-id order.  `host` holds n doubles
+ * or n ints according to the field code.  Replaces direct Particle& access
+ * (System::get_particle, src/system/system.hpp:101). */
+int samos_set_field(samos_ctx* ctx, int field, const void* host);
+int samos_get_field(samos_ctx* ctx, int field, void* host);
+
+/* `timestep` command (src/samos.cpp:1332-1356, System::set_integrator_step). */
+int samos_set_timestep(samos_ctx* ctx, double dt);
+
+/* ---- neighbour list (src/system/neighbour_list.hpp:124-272) -------------- */
+/* NeighbourList(sys, msg, rcut, pad, param): chooses the cell build when every
+ * box side exceeds 2 (rcut + pad) (neighbour_list.hpp:138-150), else N^2,
+ * re-sorts particles along a Morton curve of the cell grid, and builds. */
+int samos_nlist_create(samos_ctx* ctx, double rcut, double pad);
+/* NeighbourList::build (src/system/neighbour_list.cpp:117-149). */
+int samos_nlist_build(samos_ctx* ctx);
+/* any(NeighbourList::need_update(p)) over all particles (src/samos.cpp:944-965). */
+int samos_nlist_need_update(samos_ctx* ctx, int* out);
+/* info[8]: uses_cells, ncx, ncy, ncz, stride, builds, max_cell_count, overflow */
+int samos_nlist_info(samos_ctx* ctx, long long* info);
+/* The half-pair set of NeighbourList::get_neighbours (neighbour_list.hpp:233):
+ * rows (i, j), i < j in id order, sorted ascending.  With out == NULL only
+ * *count is set.  Pairs are kept once per stencil visit, as in the reference. */
+int samos_nlist_pairs(samos_ctx* ctx, int* out, long long cap, long long* count);
+/* Re-sort particle storage along the Morton curve of the cell grid and rebuild. */
+int samos_resort(samos_ctx* ctx);
+
+/* ---- pair potentials (src/potentials/pair/pair_potential.hpp:52-158) ----- */
+/* PairSoftPotential(..., {k; a; use_particle_radii}) (pair_soft_potential.hpp:63-117) */
+int samos_pair_soft(samos_ctx* ctx, double k, double a, int use_radii);
+/* PairLJPotential(..., {epsilon; sigma; rcut; shifted; use_particle_radii})
+ * (pair_lj_potential.hpp:61-145); WCA is resolved by the caller into
+ * rcut = 2^(1/6) sigma and shifted = 1. */
+int samos_pair_lj(samos_ctx* ctx, double eps, double sigma, double rcut, int shifted, int use_radii);
+/* set_pair_parameters (pair_soft_potential.hpp:125-176, pair_lj_potential.hpp:153-232):
+ * soft p0 = k, p1 = a; lj p0 = epsilon, p1 = sigma, p2 = rcut, shifted (0/1, ORed in). */
+int samos_pair_type_param(samos_ctx* ctx, int kind, int type1, int type2, double p0, double p1, double p2, int shifted);
+
+/* ---- pair aligners (src/aligner/pair/pair_aligner.hpp:53-89) ------------- */
+/* PairPolarAlign / PairNematicAlign(..., {J; rcut}); rcut <= 0 selects the
+ * neighbour list cutoff (pair_polar_aligner.hpp:81-90). */
+int samos_align_polar(samos_ctx* ctx, double J, double rcut);
+int samos_align_nematic(samos_ctx* ctx, double J, double rcut);
+/* set_pair_parameters (pair_polar_aligner.hpp:120-163, pair_nematic_aligner.hpp:110-155). */
+int samos_align_type_param(samos_ctx* ctx, int kind, int type1, int type2, double J, double rcut);
+
+/* ExternalSelfPropulsion {alpha} (src/potentials/external/external_self_propulsion.cpp:33-50). */
+int samos_external_self_propulsion(samos_ctx* ctx, double alpha);
+
+/* ---- constraint (src/constraints/constraint.hpp:47-161) ------------------ */
+/* kind NONE; PLANE with param = zpos, flag = unlimited; SPHERE with param = r.
+ * Like the `constraint` command (src/samos.cpp:510-522) it enforces at once. */
+int samos_constraint(samos_ctx* ctx, int kind, double param, int flag);
+/* Constrainer::enforce on every particle (src/samos.cpp:887-888). */
+int samos_enforce_all(samos_ctx* ctx);
+
+/* ---- integrators (src/integrators/integrator.hpp:54-138) ----------------- */
+/* IntegratorBrownian / BrownianPos / BrownianAlign (integrator_brownian.hpp:55-160):
+ * tau is the flip time (the reference stores dt/tau); temperature Value
+ * (src/system/value.hpp) as tkind/tmin/tmax/tsteps. */
+int samos_integrator_brownian(samos_ctx* ctx, int which, double v0, double nu, double mu, double mur, int seed,
+                              int nematic, double tau, int velocity_align, int tkind, double tmin, double tmax,
+                              int tsteps);
+/* IntegratorLangevin {gamma; seed; method} (integrator_langevin.hpp:64-121). */
+int samos_integrator_langevin(samos_ctx* ctx, double gamma, int seed, int method, int tkind, double tmin,
+                              double tmax, int tsteps);
+/* IntegratorFIRE (integrator_fire.hpp:58-154). */
+int samos_integrator_fire(samos_ctx* ctx, double alpha, double alpha_scale, double f_tol, double e_tol,
+                          double dt_inc, double dt_dec, double dt_max, int min_neg_steps);
+/* FIRE state out[8]: dt, alpha, P, |F|, |v|, E, converged, last_neg. */
+int samos_fire_state(samos_ctx* ctx, double* out);
+
+/* ---- one-call components ------------------------------------------------- */
+/* Loop head: reset_forces + reset_torques (system.hpp:136-157), Potential::compute
+ * (potential.cpp:36-52) and Aligner::compute (aligner.cpp:35-45).  energies[4]
+ * (may be NULL): soft, lj, polar, nematic totals, as get_potential_energy(). */
+int samos_compute_forces(samos_ctx* ctx, double* energies);
+/* PairPotential::compute of one potential, added onto the current forces. */
+int samos_pair_compute(samos_ctx* ctx, int kind, double* energy);
+/* PairAlign::compute of one aligner, added onto the current torques. */
+int samos_align_compute(samos_ctx* ctx, int kind, double* energy);
+/* Integrator::integrate of every registered integrator once, in the
+ * reference's map order (type_group key, src/samos.cpp:666-690). */
+int samos_integrate(samos_ctx* ctx);
+/* The `run N` loop (src/samos.cpp:872-970): N+1 iterations of enforce, reset,
+ * forces, torques, integrators, neighbour-list check.  flags: SAMOS_RUN_*. */
+int samos_run(samos_ctx* ctx, int steps, int flags);
+/* Scalar observables: what = 0 max|F|, 1 sum |F|^2, 2 kinetic energy
+ * (sum m v^2 / 2), 3 |sum v| / N, 4 <|v|>, 5 |sum n| / N. */
+int samos_reduce(samos_ctx* ctx, int what, double* out);
+
+/* ---- timing -------------------------------------------------------------- */
+/* When enabled, HIP events bracket every launch of the named kernel classes
+ * on the context's stream ("interact", "fused", "rebuild", "integrate").
+ * Reading returns the summed duration (ms) and launch count since enabling. */
+int samos_profile(samos_ctx* ctx, int enable);
+int samos_profile_read(samos_ctx* ctx, const char* kernel_class, double* ms_total, long long* launches);
+int samos_sync(samos_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SAMOS_HIP_H */

@@ -1,0 +1,1243 @@
+// SAMoS hot-path ORACLE -- test infrastructure only.
+//
+// A strict FP64 CPU restatement of the reference's per-step particle pipeline.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// this library, and only as a checker or as the timed CPU baseline; the product
+// path (samos_amd/) never links or calls it.
+//
+// Built twice (see oracle/Makefile):
+//   liboracle.so       -O2 -ffp-contract=off, no fast-math: the arithmetic order of
+//                      every expression follows the reference source as written.
+//   liboracle_fast.so  -O3 -funroll-loops -ffast-math -march=x86-64-v3: the
+//                      reference's own Release flags (CMakeCFlagsSetup.txt:46),
+//                      used only to time the single-threaded CPU baseline.
+//
+// Layout follows the reference: an array-of-structures particle record and a
+// half neighbour list stored as vector<vector<int>> (src/system/system.hpp:101,
+// src/system/neighbour_list.hpp).  Every function cites the reference lines it
+// restates (paths relative to the reference repository root).
+//
+// Parity status of each part (what pins it) is stated in the test suite header
+// (tests/test_oracle.py) and in the design notes:
+//   * cell list / nlist build: brute-force O(N^2) pair sets + example inputs;
+//   * soft / LJ / polar / nematic: closed forms, Newton's third law,
+//     finite-difference gradients;
+//   * constraints: invariants (|x| = R, n.N = 0, |n| = 1, rotation inverses);
+//   * integrators: single-particle closed forms (free ABP drift, O-step decay,
+//     FIRE two-particle relaxation), equipartition with noise on;
+//   * the noise stream is the project's counter-based Philox4x32-10 (checked
+//     against the Random123 known-answer vectors), NOT the reference's GSL
+//     MT19937 stream: bit-level parity with GSL draws is unpinned.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <stdexcept>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Counter-based RNG (project-defined noise stream, shared definition with the
+// device library; checked against Random123 philox4x32-10 KAT vectors).
+// ---------------------------------------------------------------------------
+struct U4 { uint32_t v[4]; };
+
+inline void mulhilo(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  hi = (uint32_t)(p >> 32);
+  lo = (uint32_t)p;
+}
+
+U4 philox4x32_10(U4 ctr, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) { k0 += W0; k1 += W1; }
+    uint32_t hi0, lo0, hi1, lo1;
+    mulhilo(M0, ctr.v[0], hi0, lo0);
+    mulhilo(M1, ctr.v[2], hi1, lo1);
+    U4 n;
+    n.v[0] = hi1 ^ ctr.v[1] ^ k0;
+    n.v[1] = lo1;
+    n.v[2] = hi0 ^ ctr.v[3] ^ k1;
+    n.v[3] = lo0;
+    ctr = n;
+  }
+  return ctr;
+}
+
+// Uniform in (0,1): (u + 0.5) * 2^-32.
+inline double u01(uint32_t u) { return ((double)u + 0.5) * 2.3283064365386963e-10; }
+
+// Box-Muller pair from two 32-bit words.
+inline void box_muller(uint32_t a, uint32_t b, double& g0, double& g1) {
+  double u1 = u01(a), u2 = u01(b);
+  double r = std::sqrt(-2.0 * std::log(u1));
+  double th = 6.283185307179586 * u2;
+  g0 = r * std::cos(th);
+  g1 = r * std::sin(th);
+}
+
+// Stream tags: one per integrator family, so that two integrators with the
+// same seed do not share draws.
+enum { TAG_BROWNIAN = 1, TAG_LANGEVIN = 2, TAG_BROWNIAN_POS = 3, TAG_BROWNIAN_ALIGN = 4, TAG_BBK_INIT = 5 };
+
+// Draw block for (seed, tag, particle id, call counter, purpose).
+U4 draw(uint32_t seed, uint32_t tag, uint32_t pid, uint64_t call, uint32_t purpose) {
+  U4 c; c.v[0] = pid; c.v[1] = (uint32_t)call; c.v[2] = (uint32_t)(call >> 32); c.v[3] = purpose;
+  return philox4x32_10(c, seed, tag);
+}
+
+// Three Gaussians (translational noise) for one particle and call.
+void gauss3(uint32_t seed, uint32_t tag, uint32_t pid, uint64_t call, double g[3]) {
+  U4 a = draw(seed, tag, pid, call, 0);
+  double g3;
+  box_muller(a.v[0], a.v[1], g[0], g[1]);
+  box_muller(a.v[2], a.v[3], g[2], g3);
+}
+
+// One Gaussian (rotational noise) plus one uniform (nematic flip) per call.
+void gauss1_u1(uint32_t seed, uint32_t tag, uint32_t pid, uint64_t call, double& g, double& u) {
+  U4 a = draw(seed, tag, pid, call, 1);
+  double g1;
+  box_muller(a.v[0], a.v[1], g, g1);
+  u = u01(a.v[2]) - 1.1641532182693481e-10;  // in [0,1)
+}
+
+// ---------------------------------------------------------------------------
+// Data model (src/system/particle.hpp:257-293, src/system/box.hpp:56-62)
+// ---------------------------------------------------------------------------
+struct Particle {
+  int id, type;
+  double radius, mass;
+  double x, y, z, vx, vy, vz, nx, ny, nz;
+  double fx, fy, fz, tau_x, tau_y, tau_z, omega, age;
+  double Nx, Ny, Nz;
+  double s[9];            // stress, zeroed by reset_forces
+  int ix, iy, iz, coordination, boundary;
+};
+
+struct Box {
+  double Lx, Ly, Lz, xlo, xhi, ylo, yhi, zlo, zhi;
+  void set(double lx, double ly, double lz) {
+    Lx = lx; Ly = ly; Lz = lz;
+    xlo = -0.5 * Lx; xhi = 0.5 * Lx;
+    ylo = -0.5 * Ly; yhi = 0.5 * Ly;
+    zlo = -0.5 * Lz; zhi = 0.5 * Lz;
+  }
+};
+
+struct PartPos { double x, y, z; };
+
+struct TypeTable {
+  int ntypes = 1;
+  std::vector<double> p0, p1, p2;  // ntypes*ntypes each
+  void init(int nt, double a, double b, double c) {
+    ntypes = nt; p0.assign(nt * nt, a); p1.assign(nt * nt, b); p2.assign(nt * nt, c);
+  }
+};
+
+struct PairPot {        // soft (kind 0) or lj (kind 1)
+  int kind;
+  double k, a;                         // soft (pair_soft_potential.hpp:63-180)
+  double eps, sigma, rcut;             // lj (pair_lj_potential.hpp:61-248)
+  bool shifted = false, use_radii = false, has_pair_params = false;
+  TypeTable tab;                       // soft: k,a ; lj: eps,sigma,rcut
+  double energy = 0.0;
+};
+
+struct PairAl {         // polar (kind 0) or nematic (kind 1)
+  int kind;
+  double J, rcut;
+  bool has_pair_params = false, exclude_boundary = false;
+  TypeTable tab;                       // J, rcut
+  double energy = 0.0;
+};
+
+struct Constraint {
+  int kind = -1;        // 0 none, 1 plane, 2 sphere
+  double r = 1.0, zpos = 0.0;
+  bool unlimited = false;
+};
+
+struct Integ {
+  std::string name;     // "brownian", "brownian_pos", "brownian_align", "langevin", "fire"
+  double dt = 0.0;
+  // temperature control (src/system/value.hpp:115-170): kind 0 constant, 1 linear
+  int tkind = 0; double tmin = 0.0, tmax = 1.0; int tsteps = 1000;
+  // brownian family (integrator_brownian.hpp:55-160)
+  double v0 = 1.0, nu = 1.0, mu = 1.0, mur = 1.0, tau = 0.0, stoch = 0.0;
+  bool nematic = false, velocity = false;
+  uint32_t seed = 0; uint64_t calls = 0;
+  // langevin (integrator_langevin.hpp:64-121)
+  double gamma = 1.0; int method = 0;  // 0 baoab, 1 spv, 2 bbk
+  std::vector<double> Rx, Ry, Rz;
+  // fire (integrator_fire.hpp:58-154)
+  double alpha = 0.1, alpha_init = 0.1, f_alpha = 0.99, F_tol = 1e-4, E_tol = 1e-6,
+         f_inc = 1.1, f_dec = 0.5, dt_max = 1.0;
+  int N_min = 5, last_neg = 0; bool converged = false; double old_energy = 1e15;
+  double last_P = 0, last_Fnorm = 0, last_Vnorm = 0, last_E = 0;
+  double Tval(int step) const {
+    if (tkind == 0) return tmin;
+    if (step > tsteps) return tmax;
+    return tmin + (double)step * ((tmax - tmin) / (double)tsteps);
+  }
+};
+
+struct Sim {
+  std::vector<Particle> p;
+  Box box;
+  bool periodic = true;
+  double dt = 0.0;
+  int time_step = 0, run_step = 0;
+  // neighbour list (src/system/neighbour_list.hpp:124-217)
+  bool has_nlist = false;
+  double cut = 3.0, pad = 0.5;
+  bool use_cell = false;
+  int cnx = 0, cny = 0, cnz = 0; double cwx = 0, cwy = 0, cwz = 0;
+  std::vector<std::vector<int>> stencil, cells;
+  std::vector<std::vector<int>> list;
+  std::vector<PartPos> old;
+  long builds = 0;
+  // plugins; std::map order fixes evaluation order (src/potentials/potential.cpp:36-52)
+  std::map<std::string, PairPot> pots;
+  std::map<std::string, PairAl> als;
+  std::vector<double> self_prop;   // external self_propulsion alpha (empty: none)
+  bool has_self_prop = false; double sp_alpha = 1.0;
+  Constraint con;
+  std::map<std::string, Integ> integs;   // key: type + "_" + group (src/samos.cpp:666-690)
+  int ntypes = 1;
+};
+
+// System::apply_periodic (src/system/system.cpp:1083-1095)
+inline void apply_periodic(const Sim& s, double& dx, double& dy, double& dz) {
+  if (s.periodic) {
+    const Box& b = s.box;
+    if (dx > b.xhi) dx -= b.Lx; else if (dx < b.xlo) dx += b.Lx;
+    if (dy > b.yhi) dy -= b.Ly; else if (dy < b.ylo) dy += b.Ly;
+    if (dz > b.zhi) dz -= b.Lz; else if (dz < b.zlo) dz += b.Lz;
+  }
+}
+
+// System::enforce_periodic (src/system/system.cpp:1119-1155)
+inline void enforce_periodic(Sim& s, Particle& p) {
+  if (s.periodic) {
+    const Box& b = s.box;
+    if (p.x <= b.xlo) { p.x += b.Lx; p.ix--; } else if (p.x > b.xhi) { p.x -= b.Lx; p.ix++; }
+    if (p.y <= b.ylo) { p.y += b.Ly; p.iy--; } else if (p.y > b.yhi) { p.y -= b.Ly; p.iy++; }
+    if (p.z <= b.zlo) { p.z += b.Lz; p.iz--; } else if (p.z > b.zhi) { p.z -= b.Lz; p.iz++; }
+  }
+}
+
+// System::reset_forces / reset_torques (src/system/system.hpp:136-157)
+void reset_forces(Sim& s) {
+  for (auto& q : s.p) { q.fx = q.fy = q.fz = 0.0; for (double& v : q.s) v = 0.0; }
+}
+void reset_torques(Sim& s) {
+  for (auto& q : s.p) { q.tau_x = q.tau_y = q.tau_z = 0.0; }
+}
+
+// ---------------------------------------------------------------------------
+// Cell list (src/system/cell_list.cpp:37-104)
+// ---------------------------------------------------------------------------
+void cell_list_create(Sim& s, double cutoff) {
+  s.cnx = (int)(s.box.Lx / cutoff);
+  s.cny = (int)(s.box.Ly / cutoff);
+  s.cnz = (int)(s.box.Lz / cutoff);
+  int size = s.cnx * s.cny * s.cnz;
+  s.cwx = s.box.Lx / s.cnx; s.cwy = s.box.Ly / s.cny; s.cwz = s.box.Lz / s.cnz;
+  s.stencil.assign(size, {});
+  s.cells.assign(size, {});
+  for (int i = 0; i < s.cnx; i++)
+    for (int j = 0; j < s.cny; j++)
+      for (int k = 0; k < s.cnz; k++) {
+        int idx = s.cny * s.cnz * i + s.cnz * j + k;
+        for (int ix = -1; ix <= 1; ix++)
+          for (int iy = -1; iy <= 1; iy++)
+            for (int iz = -1; iz <= 1; iz++) {
+              int iix = i + ix, iiy = j + iy, iiz = k + iz;
+              if (iix < 0) iix = s.cnx - 1; else if (iix == s.cnx) iix = 0;
+              if (iiy < 0) iiy = s.cny - 1; else if (iiy == s.cny) iiy = 0;
+              if (iiz < 0) iiz = s.cnz - 1; else if (iiz == s.cnz) iiz = 0;
+              s.stencil[idx].push_back(s.cny * s.cnz * iix + s.cnz * iiy + iiz);
+            }
+      }
+}
+
+// CellList::get_cell_idx (src/system/cell_list.cpp:72-79).  The reference does
+// not clamp; a particle exactly on the high face indexes one past the grid.
+// Clamping to the last cell is the defined behaviour here (and on the device).
+int cell_idx(const Sim& s, const Particle& p) {
+  int i = (int)((p.x - s.box.xlo) / s.cwx);
+  int j = (int)((p.y - s.box.ylo) / s.cwy);
+  int k = (int)((p.z - s.box.zlo) / s.cwz);
+  if (i >= s.cnx) i = s.cnx - 1; if (i < 0) i = 0;
+  if (j >= s.cny) j = s.cny - 1; if (j < 0) j = 0;
+  if (k >= s.cnz) k = s.cnz - 1; if (k < 0) k = 0;
+  return s.cny * s.cnz * i + s.cnz * j + k;
+}
+
+// CellList::populate (src/system/cell_list.cpp:82-104)
+void cell_populate(Sim& s) {
+  for (auto& c : s.cells) c.clear();
+  const Box& b = s.box;
+  for (int i = 0; i < (int)s.p.size(); i++) {
+    Particle& q = s.p[i];
+    if (s.periodic) {
+      if (q.x < b.xlo) q.x += b.Lx; else if (q.x > b.xhi) q.x -= b.Lx;
+      if (q.y < b.ylo) q.y += b.Ly; else if (q.y > b.yhi) q.y -= b.Ly;
+      if (q.z < b.zlo) q.z += b.Lz; else if (q.z > b.zhi) q.z -= b.Lz;
+    }
+    s.cells[cell_idx(s, q)].push_back(i);
+  }
+}
+
+// NeighbourList::build_cell (src/system/neighbour_list.cpp:216-273)
+void build_cell(Sim& s) {
+  int N = (int)s.p.size();
+  double cut = s.cut + s.pad;
+  double cut2 = cut * cut;
+  s.old.clear();
+  cell_populate(s);
+  for (int i = 0; i < N; i++) s.p[i].coordination = 0;
+  for (int i = 0; i < N; i++) {
+    Particle& pi = s.p[i];
+    const std::vector<int>& nc = s.stencil[cell_idx(s, pi)];
+    for (int c : nc) {
+      const std::vector<int>& ids = s.cells[c];
+      for (int jj : ids) {
+        Particle& pj = s.p[jj];
+        if (pj.id > pi.id) {
+          double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+          apply_periodic(s, dx, dy, dz);
+          double d2 = dx * dx + dy * dy + dz * dz;
+          if (d2 < cut2) {
+            s.list[i].push_back(pj.id);
+            double r = pi.radius + pj.radius;
+            if (d2 < r * r) { pi.coordination++; pj.coordination++; }
+          }
+        }
+      }
+    }
+    s.old.push_back({pi.x, pi.y, pi.z});
+  }
+}
+
+// NeighbourList::build_nsq (src/system/neighbour_list.cpp:178-213)
+void build_nsq(Sim& s, int i) {
+  double cut = s.cut + s.pad;
+  double cut2 = cut * cut;
+  Particle& pi = s.p[i];
+  pi.coordination = 0;
+  for (int j = 0; j < pi.id; j++) {
+    Particle& pj = s.p[j];
+    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    apply_periodic(s, dx, dy, dz);
+    double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 < cut2) {
+      s.list[j].push_back(i);
+      double r = pi.radius + pj.radius;
+      if (d2 < r * r) { pi.coordination++; pj.coordination++; }
+    }
+  }
+  s.old.push_back({pi.x, pi.y, pi.z});
+}
+
+// NeighbourList::build (src/system/neighbour_list.cpp:117-149)
+void nlist_build(Sim& s) {
+  s.list.assign(s.p.size(), {});
+  if (s.use_cell) build_cell(s);
+  else { s.old.clear(); for (int i = 0; i < (int)s.p.size(); i++) build_nsq(s, i); }
+  s.builds++;
+}
+
+// NeighbourList constructor selector (src/system/neighbour_list.hpp:138-150)
+void nlist_create(Sim& s, double rcut, double pad) {
+  s.has_nlist = true; s.cut = rcut; s.pad = pad;
+  if (s.box.Lx > 2.0 * (rcut + pad) && s.box.Ly > 2.0 * (rcut + pad) && s.box.Lz > 2.0 * (rcut + pad)) {
+    s.use_cell = true;
+    cell_list_create(s, rcut + pad);
+  } else {
+    s.use_cell = false;
+  }
+  nlist_build(s);
+}
+
+// NeighbourList::need_update (src/system/neighbour_list.cpp:278-302)
+bool need_update(const Sim& s, const Particle& p) {
+  const Box& b = s.box;
+  double dx = s.old[p.id].x - p.x, dy = s.old[p.id].y - p.y, dz = s.old[p.id].z - p.z;
+  if (s.periodic) {
+    if (dx > b.xhi) dx -= b.Lx; else if (dx < b.xlo) dx += b.Lx;
+    if (dy > b.yhi) dy -= b.Ly; else if (dy < b.ylo) dy += b.Ly;
+    if (dz > b.zhi) dz -= b.Lz; else if (dz < b.zlo) dz += b.Lz;
+  }
+  return !(dx * dx + dy * dy + dz * dz < 0.25 * s.pad * s.pad);
+}
+
+// ---------------------------------------------------------------------------
+// Pair potentials
+// ---------------------------------------------------------------------------
+// PairSoftPotential::compute (src/potentials/pair/pair_soft_potential.cpp:33-119);
+// phase_in is not supported (alpha = 1).
+void soft_compute(Sim& s, PairPot& P) {
+  int N = (int)s.p.size();
+  int nt = P.tab.ntypes;
+  double tot = 0.0;
+  for (int i = 0; i < N; i++) {
+    Particle& pi = s.p[i];
+    double ai = pi.radius;
+    for (int j : s.list[i]) {
+      Particle& pj = s.p[j];
+      int tt = (pi.type - 1) * nt + (pj.type - 1);
+      double k = P.tab.p0[tt];
+      double aj = pj.radius;
+      double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+      apply_periodic(s, dx, dy, dz);
+      double r_sq = dx * dx + dy * dy + dz * dz;
+      double r = std::sqrt(r_sq);
+      double a = P.use_radii ? ai + aj : P.tab.p1[tt];
+      if (r < a) {
+        double diff = a - r;
+        double alpha = 1.0;
+        double e = 0.5 * k * alpha * diff * diff;
+        tot += e;
+        double ff = (r > 0.0) ? k * alpha * diff / r : k * diff;
+        pi.fx -= ff * dx; pi.fy -= ff * dy; pi.fz -= ff * dz;
+        pj.fx += ff * dx; pj.fy += ff * dy; pj.fz += ff * dz;
+      }
+    }
+  }
+  P.energy = tot;
+}
+
+// PairLJPotential::compute (src/potentials/pair/pair_lj_potential.cpp:32-124)
+void lj_compute(Sim& s, PairPot& P) {
+  int N = (int)s.p.size();
+  int nt = P.tab.ntypes;
+  double sigma = P.sigma, eps = P.eps, rcut = P.rcut;
+  double sigma_sq = sigma * sigma, rcut_sq = rcut * rcut;
+  P.energy = 0.0;
+  for (int i = 0; i < N; i++) {
+    Particle& pi = s.p[i];
+    double ai = pi.radius;
+    for (int j : s.list[i]) {
+      Particle& pj = s.p[j];
+      int tt = (pi.type - 1) * nt + (pj.type - 1);
+      if (P.has_pair_params) { rcut = P.tab.p2[tt]; rcut_sq = rcut * rcut; }
+      double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      apply_periodic(s, dx, dy, dz);
+      double r_sq = dx * dx + dy * dy + dz * dz;
+      if (r_sq <= rcut_sq) {
+        if (P.has_pair_params) { sigma = P.tab.p1[tt]; eps = P.tab.p0[tt]; sigma_sq = sigma * sigma; }
+        if (P.use_radii) { sigma = ai + pj.radius; sigma_sq = sigma * sigma; }
+        double alpha = 1.0;
+        double inv_r_sq = sigma_sq / r_sq;
+        double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
+        double e = 4.0 * eps * alpha * inv_r_6 * (inv_r_6 - 1.0);
+        if (P.shifted) {
+          double inv_rc_sq = sigma_sq / rcut_sq;
+          double inv_rc_6 = inv_rc_sq * inv_rc_sq * inv_rc_sq;
+          e -= 4.0 * eps * alpha * inv_rc_6 * (inv_rc_6 - 1.0);
+        }
+        P.energy += e;
+        double ff = 48.0 * eps * alpha * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
+        pi.fx += ff * dx; pi.fy += ff * dy; pi.fz += ff * dz;
+        pj.fx -= ff * dx; pj.fy -= ff * dy; pj.fz -= ff * dz;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pair aligners
+// ---------------------------------------------------------------------------
+// PairPolarAlign::compute (src/aligner/pair/pair_polar_aligner.cpp:32-107)
+void polar_compute(Sim& s, PairAl& A) {
+  int N = (int)s.p.size();
+  int nt = A.tab.ntypes;
+  const Box& b = s.box;
+  double J = A.J, rcut = A.rcut;
+  double tot = 0.0;
+  for (int i = 0; i < N; i++) {
+    Particle& pi = s.p[i];
+    for (int j : s.list[i]) {
+      Particle& pj = s.p[j];
+      if (A.has_pair_params) { int tt = (pi.type - 1) * nt + (pj.type - 1); J = A.tab.p0[tt]; rcut = A.tab.p1[tt]; }
+      double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+      if (s.periodic) {
+        if (dx > b.xhi) dx -= b.Lx; else if (dx < b.xlo) dx += b.Lx;
+        if (dy > b.yhi) dy -= b.Ly; else if (dy < b.ylo) dy += b.Ly;
+        if (dz > b.zhi) dz -= b.Lz; else if (dz < b.zlo) dz += b.Lz;
+      }
+      double r_sq = dx * dx + dy * dy + dz * dz;
+      if (r_sq <= rcut * rcut) {
+        double tx = pi.ny * pj.nz - pi.nz * pj.ny;
+        double ty = pi.nz * pj.nx - pi.nx * pj.nz;
+        double tz = pi.nx * pj.ny - pi.ny * pj.nx;
+        if (!(A.exclude_boundary && pi.boundary)) { pi.tau_x += J * tx; pi.tau_y += J * ty; pi.tau_z += J * tz; }
+        if (!(A.exclude_boundary && pj.boundary)) { pj.tau_x += -J * tx; pj.tau_y += -J * ty; pj.tau_z += -J * tz; }
+        tot += -2.0 * J * (pi.nx * pj.nx + pi.ny * pj.ny + pi.nz * pj.nz);
+      }
+    }
+  }
+  A.energy = tot;
+}
+
+// PairNematicAlign::compute (src/aligner/pair/pair_nematic_aligner.cpp:32-96).
+// The global coupling is doubled; per-type couplings are used as given (:37,:57-61).
+void nematic_compute(Sim& s, PairAl& A) {
+  int N = (int)s.p.size();
+  int nt = A.tab.ntypes;
+  const Box& b = s.box;
+  double J = 2.0 * A.J, rcut = A.rcut;
+  A.energy = 0.0;
+  for (int i = 0; i < N; i++) {
+    Particle& pi = s.p[i];
+    for (int j : s.list[i]) {
+      Particle& pj = s.p[j];
+      if (A.has_pair_params) { int tt = (pi.type - 1) * nt + (pj.type - 1); J = A.tab.p0[tt]; rcut = A.tab.p1[tt]; }
+      double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+      if (s.periodic) {
+        if (dx > b.xhi) dx -= b.Lx; else if (dx < b.xlo) dx += b.Lx;
+        if (dy > b.yhi) dy -= b.Ly; else if (dy < b.ylo) dy += b.Ly;
+        if (dz > b.zhi) dz -= b.Lz; else if (dz < b.zlo) dz += b.Lz;
+      }
+      double r_sq = dx * dx + dy * dy + dz * dz;
+      if (r_sq <= rcut * rcut) {
+        double nn = pi.nx * pj.nx + pi.ny * pj.ny + pi.nz * pj.nz;
+        double tx = pi.ny * pj.nz - pi.nz * pj.ny;
+        double ty = pi.nz * pj.nx - pi.nx * pj.nz;
+        double tz = pi.nx * pj.ny - pi.ny * pj.nx;
+        pi.tau_x += J * nn * tx; pi.tau_y += J * nn * ty; pi.tau_z += J * nn * tz;
+        pj.tau_x += -J * nn * tx; pj.tau_y += -J * nn * ty; pj.tau_z += -J * nn * tz;
+        A.energy += -2.0 * J * (2.0 * nn * nn - 1.0);
+      }
+    }
+  }
+}
+
+// ExternalSelfPropulsion::compute (src/potentials/external/external_self_propulsion.cpp:33-50)
+void self_prop_compute(Sim& s) {
+  for (auto& q : s.p) { q.fx += s.sp_alpha * q.nx; q.fy += s.sp_alpha * q.ny; q.fz += s.sp_alpha * q.nz; }
+}
+
+// Potential::compute (src/potentials/potential.cpp:36-52): pair (map order) then external.
+void pot_compute(Sim& s) {
+  for (auto& kv : s.pots) {
+    if (kv.second.kind == 0) soft_compute(s, kv.second); else lj_compute(s, kv.second);
+  }
+  if (s.has_self_prop) self_prop_compute(s);
+}
+// Potential::compute_potential_energy (src/potentials/potential.cpp:57-75)
+double pot_energy(const Sim& s) {
+  double e = 0.0;
+  for (auto& kv : s.pots) e += kv.second.energy;
+  return e;
+}
+// Aligner::compute (src/aligner/aligner.cpp:35-45)
+void align_compute(Sim& s) {
+  for (auto& kv : s.als) {
+    if (kv.second.kind == 0) polar_compute(s, kv.second); else nematic_compute(s, kv.second);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Constraints
+// ---------------------------------------------------------------------------
+// ConstraintSphere::enforce (src/constraints/constraint_sphere.cpp:38-81)
+void sphere_enforce(Sim& s, Particle& p) {
+  double R0 = s.con.r;
+  double x = p.x, y = p.y, z = p.z;
+  double R = std::sqrt(x * x + y * y + z * z);
+  double sc = R0 / R;
+  p.x *= sc; p.y *= sc; p.z *= sc;
+  double Nx = p.x / R0, Ny = p.y / R0, Nz = p.z / R0;
+  double vN = p.vx * Nx + p.vy * Ny + p.vz * Nz;
+  double nN = p.nx * Nx + p.ny * Ny + p.nz * Nz;
+  p.vx -= vN * Nx; p.vy -= vN * Ny; p.vz -= vN * Nz;
+  p.nx -= nN * Nx; p.ny -= nN * Ny; p.nz -= nN * Nz;
+  double inv_len = 1.0 / std::sqrt(p.nx * p.nx + p.ny * p.ny + p.nz * p.nz);
+  p.nx *= inv_len; p.ny *= inv_len; p.nz *= inv_len;
+  p.Nx = Nx; p.Ny = Ny; p.Nz = Nz;
+}
+
+// ConstraintPlane::enforce (src/constraints/constraint_plane.cpp:36-88)
+void plane_enforce(Sim& s, Particle& p) {
+  double Lx = s.box.Lx, Ly = s.box.Ly;
+  double xlo = -0.5 * Lx, xhi = 0.5 * Lx, ylo = -0.5 * Ly, yhi = 0.5 * Ly;
+  p.z = s.con.zpos; p.vz = 0.0; p.fz = 0.0;
+  p.Nx = 0.0; p.Ny = 0.0; p.Nz = 1.0;
+  if (s.periodic) enforce_periodic(s, p);
+  else if (!s.con.unlimited) {
+    if (p.x < xlo) { p.x = xlo; p.vx = -p.vx; } else if (p.x > xhi) { p.x = xhi; p.vx = -p.vx; }
+    if (p.y < ylo) { p.y = ylo; p.vy = -p.vy; } else if (p.y > yhi) { p.y = yhi; p.vy = -p.vy; }
+  }
+  p.nz = 0.0;
+  double len_n = std::sqrt(p.nx * p.nx + p.ny * p.ny);
+  double inv_len = 1.0;
+  if (len_n != 0.0) inv_len = 1.0 / std::sqrt(p.nx * p.nx + p.ny * p.ny);
+  p.nx *= inv_len; p.ny *= inv_len;
+}
+
+// Constrainer::enforce (src/constraints/constrainer.hpp:73-78) with one constraint;
+// ConstraintNone::enforce (src/constraints/constraint_none.hpp:58-62)
+void enforce(Sim& s, Particle& p) {
+  switch (s.con.kind) {
+    case 0: enforce_periodic(s, p); break;
+    case 1: plane_enforce(s, p); break;
+    case 2: sphere_enforce(s, p); break;
+    default: break;
+  }
+}
+
+// project_torque: sphere via Constraint::project_torque (src/constraints/constraint.cpp:158-173),
+// plane (src/constraints/constraint_plane.cpp:146-160), none returns 0.
+double project_torque(Sim& s, Particle& p) {
+  switch (s.con.kind) {
+    case 1: return p.tau_z;
+    case 2: {
+      double Nx = p.x / s.con.r, Ny = p.y / s.con.r, Nz = p.z / s.con.r;
+      p.Nx = Nx; p.Ny = Ny; p.Nz = Nz;
+      return p.tau_x * Nx + p.tau_y * Ny + p.tau_z * Nz;
+    }
+    default: return 0.0;
+  }
+}
+
+// rotate_director: sphere via Constraint::rotate_director (src/constraints/constraint.cpp:96-119),
+// plane (src/constraints/constraint_plane.cpp:95-115), none is a no-op.
+void rotate_director(Sim& s, Particle& p, double phi) {
+  if (s.con.kind == 1) {
+    double c = std::cos(phi), sn = std::sin(phi);
+    double Rxx = c, Rxy = -sn, Ryx = sn, Ryy = c;
+    double nx = Rxx * p.nx + Rxy * p.ny;
+    double ny = Ryx * p.nx + Ryy * p.ny;
+    double len = std::sqrt(nx * nx + ny * ny);
+    p.nx = nx / len; p.ny = ny / len;
+  } else if (s.con.kind == 2) {
+    double U = p.x / s.con.r, V = p.y / s.con.r, W = p.z / s.con.r;
+    p.Nx = U; p.Ny = V; p.Nz = W;
+    double c = std::cos(phi), sn = std::sin(phi);
+    double nx = U * (U * p.nx + V * p.ny + W * p.nz) * (1.0 - c) + p.nx * c + (-W * p.ny + V * p.nz) * sn;
+    double ny = V * (U * p.nx + V * p.ny + W * p.nz) * (1.0 - c) + p.ny * c + (W * p.nx - U * p.nz) * sn;
+    double nz = W * (U * p.nx + V * p.ny + W * p.nz) * (1.0 - c) + p.nz * c + (-V * p.nx + U * p.ny) * sn;
+    double len = std::sqrt(nx * nx + ny * ny + nz * nz);
+    p.nx = nx / len; p.ny = ny / len; p.nz = nz / len;
+  }
+}
+
+// rotate_velocity: sphere (src/constraints/constraint.cpp:128-150), plane (:118-140 of constraint_plane.cpp)
+void rotate_velocity(Sim& s, Particle& p, double phi) {
+  if (s.con.kind == 1) {
+    double c = std::cos(phi), sn = std::sin(phi);
+    double vx = c * p.vx + (-sn) * p.vy;
+    double vy = sn * p.vx + c * p.vy;
+    p.vx = vx; p.vy = vy;
+  } else if (s.con.kind == 2) {
+    double U = p.x / s.con.r, V = p.y / s.con.r, W = p.z / s.con.r;
+    double c = std::cos(phi), sn = std::sin(phi);
+    double vx = U * (U * p.vx + V * p.vy + W * p.vz) * (1.0 - c) + p.vx * c + (-W * p.vy + V * p.vz) * sn;
+    double vy = V * (U * p.vx + V * p.vy + W * p.vz) * (1.0 - c) + p.vy * c + (W * p.vx - U * p.vz) * sn;
+    double vz = W * (U * p.vx + V * p.vy + W * p.vz) * (1.0 - c) + p.vz * c + (-V * p.vx + U * p.vy) * sn;
+    p.vx = vx; p.vy = vy; p.vz = vz;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Integrators
+// ---------------------------------------------------------------------------
+// IntegratorBrownian::integrate (src/integrators/integrator_brownian.cpp:42-124)
+void brownian(Sim& s, Integ& I) {
+  double T = I.Tval(s.run_step);
+  double B = std::sqrt(2.0 * I.mu * T);
+  double sqrt_dt = std::sqrt(I.dt);
+  uint64_t call = I.calls++;
+  int N = (int)s.p.size();
+  if (I.nematic) {
+    for (int i = 0; i < N; i++) {
+      Particle& p = s.p[i];
+      double g, u; gauss1_u1(I.seed, TAG_BROWNIAN, (uint32_t)p.id, call, g, u);
+      if (u < I.tau) {
+        p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz;
+        if (I.velocity) { p.vx = -p.vx; p.vy = -p.vy; p.vz = -p.vz; }
+      }
+    }
+  }
+  for (int i = 0; i < N; i++) {
+    Particle& p = s.p[i];
+    double fdx = I.v0 * p.nx + I.mu * p.fx;
+    double fdy = I.v0 * p.ny + I.mu * p.fy;
+    double fdz = I.v0 * p.nz + I.mu * p.fz;
+    p.vx = fdx; p.vy = fdy; p.vz = fdz;
+    p.x += I.dt * fdx; p.y += I.dt * fdy; p.z += I.dt * fdz;
+    if (T > 0.0) {
+      double g[3]; gauss3(I.seed, TAG_BROWNIAN, (uint32_t)p.id, call, g);
+      double frx = B * g[0], fry = B * g[1], frz = B * g[2];
+      p.vx += frx; p.vy += fry; p.vz += frz;
+      p.x += sqrt_dt * frx; p.y += sqrt_dt * fry; p.z += sqrt_dt * frz;
+    }
+    enforce(s, p);
+    p.omega = I.mur * project_torque(s, p);
+    double g, u; gauss1_u1(I.seed, TAG_BROWNIAN, (uint32_t)p.id, call, g, u);
+    double dtheta = I.dt * p.omega + I.stoch * g;
+    rotate_director(s, p, dtheta);
+    if (I.velocity) rotate_velocity(s, p, dtheta);
+    p.age += I.dt;
+  }
+}
+
+// IntegratorBrownianPos::integrate (src/integrators/integrator_brownian_pos.cpp:37-84)
+void brownian_pos(Sim& s, Integ& I) {
+  double T = I.Tval(s.run_step);
+  double B = std::sqrt(2.0 * I.mu * T);
+  double sqrt_dt = std::sqrt(I.dt);
+  uint64_t call = I.calls++;
+  for (auto& p : s.p) {
+    double fdx = I.mu * p.fx, fdy = I.mu * p.fy, fdz = I.mu * p.fz;
+    p.vx = fdx; p.vy = fdy; p.vz = fdz;
+    p.x += I.dt * p.vx; p.y += I.dt * p.vy; p.z += I.dt * p.vz;
+    if (T > 0.0) {
+      double g[3]; gauss3(I.seed, TAG_BROWNIAN_POS, (uint32_t)p.id, call, g);
+      double frx = B * g[0], fry = B * g[1], frz = B * g[2];
+      p.vx += frx; p.vy += fry; p.vz += frz;
+      p.x += sqrt_dt * frx; p.y += sqrt_dt * fry; p.z += sqrt_dt * frz;
+    }
+    enforce(s, p);
+    p.age += I.dt;
+  }
+}
+
+// IntegratorBrownianAlign::integrate (src/integrators/integrator_brownian_align.cpp:41-76)
+void brownian_align(Sim& s, Integ& I) {
+  uint64_t call = I.calls++;
+  if (I.nematic) {
+    for (auto& p : s.p) {
+      double g, u; gauss1_u1(I.seed, TAG_BROWNIAN_ALIGN, (uint32_t)p.id, call, g, u);
+      if (u < I.tau) { p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz; }
+    }
+  }
+  for (auto& p : s.p) {
+    p.omega = I.mur * project_torque(s, p);
+    double g, u; gauss1_u1(I.seed, TAG_BROWNIAN_ALIGN, (uint32_t)p.id, call, g, u);
+    double dtheta = I.dt * p.omega + I.stoch * g;
+    rotate_director(s, p, dtheta);
+  }
+}
+
+// IntegratorLangevin::integrate_baoab (src/integrators/integrator_langevin.cpp:54-118)
+void langevin_baoab(Sim& s, Integ& I) {
+  double T = I.Tval(s.run_step);
+  double B = std::sqrt(T * (1.0 - std::exp(-2.0 * I.gamma * I.dt)));
+  double exp_dt = std::exp(-I.gamma * I.dt);
+  double dt2 = 0.5 * I.dt;
+  uint64_t call = I.calls++;
+  for (auto& p : s.p) {
+    double fact = dt2 / p.mass;
+    p.vx += fact * p.fx; p.vy += fact * p.fy; p.vz += fact * p.fz;
+    p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+    enforce(s, p);
+    p.vx *= exp_dt; p.vy *= exp_dt; p.vz *= exp_dt;
+    if (B != 0.0) {
+      double sf = B / std::sqrt(p.mass);
+      double g[3]; gauss3(I.seed, TAG_LANGEVIN, (uint32_t)p.id, call, g);
+      p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
+    }
+    p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+    enforce(s, p);
+  }
+  reset_forces(s);
+  pot_compute(s);
+  for (auto& p : s.p) {
+    double fact = dt2 / p.mass;
+    p.vx += fact * p.fx; p.vy += fact * p.fy; p.vz += fact * p.fz;
+    p.age += I.dt;
+  }
+}
+
+// IntegratorLangevin::integrate_spv (src/integrators/integrator_langevin.cpp:128-183)
+void langevin_spv(Sim& s, Integ& I) {
+  double T = I.Tval(s.run_step);
+  double zeta = std::sqrt(T * (1.0 - std::exp(-2.0 * I.gamma * I.dt)));
+  double eta = (I.gamma == 0.0) ? 0.0 : (1.0 - std::exp(-I.dt * I.gamma)) / I.gamma;
+  double exp_dt = std::exp(-I.dt * I.gamma);
+  double dt2 = 0.5 * I.dt;
+  uint64_t call = I.calls++;
+  for (auto& p : s.p) {
+    p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+    enforce(s, p);
+  }
+  reset_forces(s);
+  pot_compute(s);
+  for (auto& p : s.p) {
+    double fact = eta / p.mass;
+    p.vx = exp_dt * p.vx + fact * p.fx;
+    p.vy = exp_dt * p.vy + fact * p.fy;
+    p.vz = exp_dt * p.vz + fact * p.fz;
+    if (zeta != 0.0) {
+      double sf = zeta / std::sqrt(p.mass);
+      double g[3]; gauss3(I.seed, TAG_LANGEVIN, (uint32_t)p.id, call, g);
+      p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
+    }
+    p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+    enforce(s, p);
+    p.age += I.dt;
+  }
+}
+
+// IntegratorLangevin::integrate_bbk (src/integrators/integrator_langevin.cpp:193-253).
+// The stored random vectors R are indexed by particle id; the initial draw
+// (integrator_langevin.hpp:106-116) uses call counter 2^63.
+void langevin_bbk(Sim& s, Integ& I) {
+  double T = I.Tval(s.run_step);
+  double B = std::sqrt(2.0 * T * I.dt * I.gamma);
+  double dt2 = 0.5 * I.dt;
+  double one_m_dt2 = 1.0 - I.gamma * dt2;
+  double one_div = 1.0 / (1.0 + I.gamma * dt2);
+  uint64_t call = I.calls++;
+  for (auto& p : s.p) {
+    double dm = dt2 / p.mass;
+    p.vx = one_m_dt2 * p.vx + dm * p.fx;
+    p.vy = one_m_dt2 * p.vy + dm * p.fy;
+    p.vz = one_m_dt2 * p.vz + dm * p.fz;
+    if (B != 0.0) {
+      double sf = 0.5 * B / std::sqrt(p.mass);
+      p.vx += sf * I.Rx[p.id]; p.vy += sf * I.Ry[p.id]; p.vz += sf * I.Rz[p.id];
+    }
+    p.x += I.dt * p.vx; p.y += I.dt * p.vy; p.z += I.dt * p.vz;
+    enforce(s, p);
+  }
+  reset_forces(s);
+  pot_compute(s);
+  for (auto& p : s.p) {
+    double dm = dt2 / p.mass;
+    p.vx = one_div * (p.vx + dm * p.fx);
+    p.vy = one_div * (p.vy + dm * p.fy);
+    p.vz = one_div * (p.vz + dm * p.fz);
+    if (B != 0.0) {
+      double sf = 0.5 * B * one_div / std::sqrt(p.mass);
+      double g[3]; gauss3(I.seed, TAG_LANGEVIN, (uint32_t)p.id, call, g);
+      I.Rx[p.id] = g[0]; I.Ry[p.id] = g[1]; I.Rz[p.id] = g[2];
+      p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
+      p.age += I.dt;
+    }
+  }
+}
+
+// IntegratorFIRE::integrate (src/integrators/integrator_fire.cpp:34-148).  The
+// per-iteration stdout line is not printed; its three numbers are kept instead.
+void fire(Sim& s, Integ& I) {
+  int N = (int)s.p.size();
+  double sqrt_ndof = std::sqrt(3 * N);
+  double dt_2 = 0.5 * I.dt;
+  double P = 0.0, Fn = 0.0, Vn = 0.0;
+  for (auto& p : s.p) {
+    p.vx += dt_2 * p.fx; p.vy += dt_2 * p.fy; p.vz += dt_2 * p.fz;
+    p.x += I.dt * p.vx; p.y += I.dt * p.vy; p.z += I.dt * p.vz;
+    enforce(s, p);
+    p.omega += dt_2 * project_torque(s, p);
+    double dtheta = I.dt * p.omega;
+    rotate_director(s, p, dtheta);
+  }
+  reset_forces(s);
+  reset_torques(s);
+  pot_compute(s);
+  align_compute(s);
+  for (auto& p : s.p) {
+    p.vx += dt_2 * p.fx; p.vy += dt_2 * p.fy; p.vz += dt_2 * p.fz;
+    P += p.vx * p.fx + p.vy * p.fy + p.vz * p.fz;
+    Fn += p.fx * p.fx + p.fy * p.fy + p.fz * p.fz;
+    Vn += p.vx * p.vx + p.vy * p.vy + p.vz * p.vz;
+    enforce(s, p);
+    p.omega += dt_2 * project_torque(s, p);
+  }
+  Fn = std::sqrt(Fn);
+  Vn = std::sqrt(Vn);
+  double E = pot_energy(s);
+  I.last_P = P; I.last_Fnorm = Fn; I.last_Vnorm = Vn; I.last_E = E;
+  if (Fn / sqrt_ndof < I.F_tol && std::fabs(E - I.old_energy) < I.E_tol) {
+    I.converged = true;
+    return;
+  }
+  double inv_F = 1.0 / Fn;
+  double f1 = 1.0 - I.alpha;
+  double f2 = I.alpha * inv_F * Vn;
+  for (auto& p : s.p) {
+    p.vx = f1 * p.vx + f2 * p.fx;
+    p.vy = f1 * p.vy + f2 * p.fy;
+    p.vz = f1 * p.vz + f2 * p.fz;
+  }
+  if (P > 0.0) {
+    I.last_neg++;
+    if (I.last_neg > I.N_min) { I.dt = std::min(I.dt * I.f_inc, I.dt_max); I.alpha *= I.f_alpha; }
+  } else {
+    I.dt *= I.f_dec;
+    I.alpha = I.alpha_init;
+    I.last_neg = 0;
+    for (auto& p : s.p) { p.vx = 0.0; p.vy = 0.0; p.vz = 0.0; }
+  }
+  I.converged = false;
+  I.old_energy = E;
+}
+
+void integrate(Sim& s, Integ& I) {
+  if (I.name == "brownian") brownian(s, I);
+  else if (I.name == "brownian_pos") brownian_pos(s, I);
+  else if (I.name == "brownian_align") brownian_align(s, I);
+  else if (I.name == "langevin") {
+    if (I.method == 0) langevin_baoab(s, I); else if (I.method == 1) langevin_spv(s, I); else langevin_bbk(s, I);
+  } else if (I.name == "fire") fire(s, I);
+}
+
+bool needs_nlist(const Sim& s) { return s.has_nlist && (!s.pots.empty() || !s.als.empty()); }
+
+// Run loop (src/samos.cpp:872-970): N+1 iterations.
+void run(Sim& s, int steps) {
+  pot_compute(s);
+  align_compute(s);
+  for (int t = 0; t <= steps; t++) {
+    s.run_step = t;
+    for (auto& p : s.p) enforce(s, p);
+    reset_forces(s);
+    reset_torques(s);
+    pot_compute(s);
+    align_compute(s);
+    for (auto& kv : s.integs) integrate(s, kv.second);
+    if (needs_nlist(s)) {
+      bool rebuild = false;
+      for (auto& p : s.p) if (need_update(s, p)) { rebuild = true; break; }
+      if (rebuild) nlist_build(s);
+    }
+    s.time_step++;
+  }
+}
+
+thread_local std::string g_err;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI (ctypes).  Field codes match include/samos_hip.h's samos_field.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+enum { F_X = 0, F_Y, F_Z, F_VX, F_VY, F_VZ, F_NX, F_NY, F_NZ, F_FX, F_FY, F_FZ,
+       F_TX, F_TY, F_TZ, F_OMEGA, F_AGE, F_RADIUS, F_MASS,
+       F_TYPE = 32, F_IX, F_IY, F_IZ, F_COORD };
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+void* orc_create(int n, double lx, double ly, double lz, int periodic) {
+  Sim* s = new Sim();
+  s->p.resize(n);
+  for (int i = 0; i < n; i++) {
+    Particle& q = s->p[i];
+    std::memset(&q, 0, sizeof(Particle));
+    q.id = i; q.type = 1; q.radius = 1.0; q.mass = 1.0;
+  }
+  s->box.set(lx, ly, lz);
+  s->periodic = periodic != 0;
+  return s;
+}
+
+void orc_destroy(void* h) { delete static_cast<Sim*>(h); }
+
+static double* dfield(Particle& q, int f) {
+  switch (f) {
+    case F_X: return &q.x; case F_Y: return &q.y; case F_Z: return &q.z;
+    case F_VX: return &q.vx; case F_VY: return &q.vy; case F_VZ: return &q.vz;
+    case F_NX: return &q.nx; case F_NY: return &q.ny; case F_NZ: return &q.nz;
+    case F_FX: return &q.fx; case F_FY: return &q.fy; case F_FZ: return &q.fz;
+    case F_TX: return &q.tau_x; case F_TY: return &q.tau_y; case F_TZ: return &q.tau_z;
+    case F_OMEGA: return &q.omega; case F_AGE: return &q.age;
+    case F_RADIUS: return &q.radius; case F_MASS: return &q.mass;
+  }
+  return nullptr;
+}
+static int* ifield(Particle& q, int f) {
+  switch (f) {
+    case F_TYPE: return &q.type; case F_IX: return &q.ix; case F_IY: return &q.iy; case F_IZ: return &q.iz;
+    case F_COORD: return &q.coordination;
+  }
+  return nullptr;
+}
+
+int orc_set_field(void* h, int f, const void* data) {
+  Sim* s = static_cast<Sim*>(h);
+  int n = (int)s->p.size();
+  if (f < 32) {
+    const double* d = static_cast<const double*>(data);
+    for (int i = 0; i < n; i++) { double* t = dfield(s->p[i], f); if (!t) return -1; *t = d[i]; }
+  } else {
+    const int* d = static_cast<const int*>(data);
+    for (int i = 0; i < n; i++) { int* t = ifield(s->p[i], f); if (!t) return -1; *t = d[i]; }
+    if (f == F_TYPE) { int mx = 1; for (int i = 0; i < n; i++) mx = std::max(mx, d[i]); s->ntypes = mx; }
+  }
+  return 0;
+}
+
+int orc_get_field(void* h, int f, void* out) {
+  Sim* s = static_cast<Sim*>(h);
+  int n = (int)s->p.size();
+  if (f < 32) {
+    double* d = static_cast<double*>(out);
+    for (int i = 0; i < n; i++) { double* t = dfield(s->p[i], f); if (!t) return -1; d[i] = *t; }
+  } else {
+    int* d = static_cast<int*>(out);
+    for (int i = 0; i < n; i++) { int* t = ifield(s->p[i], f); if (!t) return -1; d[i] = *t; }
+  }
+  return 0;
+}
+
+int orc_set_timestep(void* h, double dt) { static_cast<Sim*>(h)->dt = dt; return 0; }
+
+int orc_nlist_create(void* h, double rcut, double pad) {
+  nlist_create(*static_cast<Sim*>(h), rcut, pad);
+  return 0;
+}
+int orc_nlist_build(void* h) { nlist_build(*static_cast<Sim*>(h)); return 0; }
+int orc_nlist_uses_cells(void* h) { return static_cast<Sim*>(h)->use_cell ? 1 : 0; }
+long orc_nlist_builds(void* h) { return static_cast<Sim*>(h)->builds; }
+int orc_cell_dims(void* h, int* out) {
+  Sim* s = static_cast<Sim*>(h);
+  out[0] = s->cnx; out[1] = s->cny; out[2] = s->cnz;
+  return 0;
+}
+
+// Half pair list as (i, j) rows with i < j, sorted; returns the row count
+// (duplicates, which the reference produces when a grid side has two cells,
+// are kept).  With out == nullptr only the count is returned.
+long orc_nlist_pairs(void* h, int* out, long cap) {
+  Sim* s = static_cast<Sim*>(h);
+  std::vector<std::pair<int, int>> pr;
+  for (int i = 0; i < (int)s->list.size(); i++)
+    for (int j : s->list[i]) pr.push_back({std::min(i, j), std::max(i, j)});
+  std::sort(pr.begin(), pr.end());
+  if (out) {
+    if ((long)pr.size() > cap) return -1;
+    for (size_t k = 0; k < pr.size(); k++) { out[2 * k] = pr[k].first; out[2 * k + 1] = pr[k].second; }
+  }
+  return (long)pr.size();
+}
+
+// Brute-force O(N^2) reference pair set with the same d^2 expression.
+long orc_brute_pairs(void* h, double cut, int* out, long cap) {
+  Sim* s = static_cast<Sim*>(h);
+  double cut2 = cut * cut;
+  long cnt = 0;
+  int n = (int)s->p.size();
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++) {
+      double dx = s->p[i].x - s->p[j].x, dy = s->p[i].y - s->p[j].y, dz = s->p[i].z - s->p[j].z;
+      apply_periodic(*s, dx, dy, dz);
+      double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 < cut2) {
+        if (out) { if (cnt >= cap) return -1; out[2 * cnt] = i; out[2 * cnt + 1] = j; }
+        cnt++;
+      }
+    }
+  return cnt;
+}
+
+int orc_need_update_any(void* h) {
+  Sim* s = static_cast<Sim*>(h);
+  for (auto& p : s->p) if (need_update(*s, p)) return 1;
+  return 0;
+}
+
+int orc_add_pair_soft(void* h, double k, double a, int use_radii) {
+  Sim* s = static_cast<Sim*>(h);
+  PairPot P; P.kind = 0; P.k = k; P.a = a; P.use_radii = use_radii != 0;
+  P.tab.init(s->ntypes, k, a, 0.0);
+  s->pots["soft"] = P;
+  return 0;
+}
+
+int orc_add_pair_lj(void* h, double eps, double sigma, double rcut, int shifted, int use_radii) {
+  Sim* s = static_cast<Sim*>(h);
+  PairPot P; P.kind = 1; P.eps = eps; P.sigma = sigma; P.rcut = rcut; P.shifted = shifted != 0;
+  P.use_radii = use_radii != 0;
+  P.tab.init(s->ntypes, eps, sigma, rcut);
+  s->pots["lj"] = P;
+  return 0;
+}
+
+// Per-type pair parameters (set_pair_parameters, symmetric fill).
+// soft: (k, a); lj: (eps, sigma, rcut, wca flag -> rcut = 2^(1/6) sigma, shifted).
+int orc_pair_type_param(void* h, const char* name, int t1, int t2, double p0, double p1, double p2, int wca) {
+  Sim* s = static_cast<Sim*>(h);
+  auto it = s->pots.find(name);
+  if (it == s->pots.end()) { g_err = "no such pair potential"; return -1; }
+  PairPot& P = it->second;
+  int nt = P.tab.ntypes;
+  if (t1 < 1 || t2 < 1 || t1 > nt || t2 > nt) { g_err = "type out of range"; return -1; }
+  if (P.kind == 1 && wca) { p2 = 1.122462048309373 * p1; P.shifted = true; }
+  int a = (t1 - 1) * nt + (t2 - 1), b = (t2 - 1) * nt + (t1 - 1);
+  P.tab.p0[a] = P.tab.p0[b] = p0;
+  P.tab.p1[a] = P.tab.p1[b] = p1;
+  P.tab.p2[a] = P.tab.p2[b] = p2;
+  P.has_pair_params = true;
+  return 0;
+}
+
+int orc_add_align(void* h, int kind, double J, double rcut) {
+  Sim* s = static_cast<Sim*>(h);
+  PairAl A; A.kind = kind; A.J = J; A.rcut = rcut;
+  A.tab.init(s->ntypes, J, rcut, 0.0);
+  s->als[kind == 0 ? "polar" : "nematic"] = A;
+  return 0;
+}
+
+int orc_align_type_param(void* h, int kind, int t1, int t2, double J, double rcut) {
+  Sim* s = static_cast<Sim*>(h);
+  auto it = s->als.find(kind == 0 ? "polar" : "nematic");
+  if (it == s->als.end()) { g_err = "no such aligner"; return -1; }
+  PairAl& A = it->second;
+  int nt = A.tab.ntypes;
+  int a = (t1 - 1) * nt + (t2 - 1), b = (t2 - 1) * nt + (t1 - 1);
+  A.tab.p0[a] = A.tab.p0[b] = J;
+  A.tab.p1[a] = A.tab.p1[b] = rcut;
+  A.has_pair_params = true;
+  return 0;
+}
+
+int orc_set_self_propulsion(void* h, double alpha) {
+  Sim* s = static_cast<Sim*>(h);
+  s->has_self_prop = true; s->sp_alpha = alpha;
+  return 0;
+}
+
+// kind: 0 none, 1 plane (param = zpos, flag = unlimited), 2 sphere (param = r).
+// Adding a constraint enforces it on every particle (src/samos.cpp:510-522).
+int orc_set_constraint(void* h, int kind, double param, int flag) {
+  Sim* s = static_cast<Sim*>(h);
+  s->con.kind = kind;
+  if (kind == 1) { s->con.zpos = param; s->con.unlimited = flag != 0; }
+  if (kind == 2) s->con.r = param;
+  for (auto& p : s->p) enforce(*s, p);
+  return 0;
+}
+
+int orc_enforce_all(void* h) {
+  Sim* s = static_cast<Sim*>(h);
+  for (auto& p : s->p) enforce(*s, p);
+  return 0;
+}
+
+int orc_rotate_directors(void* h, const double* phi) {
+  Sim* s = static_cast<Sim*>(h);
+  for (size_t i = 0; i < s->p.size(); i++) rotate_director(*s, s->p[i], phi[i]);
+  return 0;
+}
+
+int orc_project_torques(void* h, double* out) {
+  Sim* s = static_cast<Sim*>(h);
+  for (size_t i = 0; i < s->p.size(); i++) out[i] = project_torque(*s, s->p[i]);
+  return 0;
+}
+
+static Integ make_integ(Sim* s, const char* name, double dt) {
+  Integ I; I.name = name; I.dt = dt > 0.0 ? dt : s->dt;
+  return I;
+}
+
+// Brownian family.  which: 0 brownian, 1 brownian_pos, 2 brownian_align.
+// tau here is the flip time (the integrator stores dt/tau).
+int orc_add_brownian(void* h, int which, double v0, double nu, double mu, double mur, int seed,
+                     int nematic, double tau, int velocity_align, int tkind, double tmin, double tmax, int tsteps) {
+  Sim* s = static_cast<Sim*>(h);
+  const char* nm = which == 0 ? "brownian" : which == 1 ? "brownian_pos" : "brownian_align";
+  Integ I = make_integ(s, nm, 0.0);
+  if (I.dt <= 0.0) { g_err = "Integrator step has not been set."; return -1; }
+  I.v0 = v0; I.nu = nu; I.mu = mu; I.mur = mur; I.seed = (uint32_t)seed;
+  I.nematic = nematic != 0; I.tau = I.dt / tau; I.velocity = velocity_align != 0;
+  I.tkind = tkind; I.tmin = tmin; I.tmax = tmax; I.tsteps = tsteps;
+  I.stoch = std::sqrt(I.nu * I.dt);
+  s->integs[std::string(nm) + "_all"] = I;
+  return 0;
+}
+
+int orc_add_langevin(void* h, double gamma, int seed, int method, int tkind, double tmin, double tmax, int tsteps) {
+  Sim* s = static_cast<Sim*>(h);
+  Integ I = make_integ(s, "langevin", 0.0);
+  if (I.dt <= 0.0) { g_err = "Integrator step has not been set."; return -1; }
+  I.gamma = gamma; I.seed = (uint32_t)seed; I.method = method;
+  I.tkind = tkind; I.tmin = tmin; I.tmax = tmax; I.tsteps = tsteps;
+  if (method == 2) {
+    int n = (int)s->p.size();
+    I.Rx.resize(n); I.Ry.resize(n); I.Rz.resize(n);
+    for (int i = 0; i < n; i++) {
+      double g[3]; gauss3(I.seed, TAG_BBK_INIT, (uint32_t)i, 0, g);
+      I.Rx[i] = g[0]; I.Ry[i] = g[1]; I.Rz[i] = g[2];
+    }
+  }
+  s->integs["langevin_all"] = I;
+  return 0;
+}
+
+int orc_add_fire(void* h, double alpha, double alpha_scale, double f_tol, double e_tol,
+                 double dt_inc, double dt_dec, double dt_max, int min_neg) {
+  Sim* s = static_cast<Sim*>(h);
+  Integ I = make_integ(s, "fire", 0.0);
+  if (I.dt <= 0.0) { g_err = "Integrator step has not been set."; return -1; }
+  I.alpha = alpha; I.alpha_init = alpha; I.f_alpha = alpha_scale; I.F_tol = f_tol; I.E_tol = e_tol;
+  I.f_inc = dt_inc; I.f_dec = dt_dec; I.dt_max = dt_max; I.N_min = min_neg;
+  s->integs["fire_all"] = I;
+  return 0;
+}
+
+// FIRE state: dt, alpha, P, |F|, |v|, E, converged, last_neg
+int orc_fire_state(void* h, double* out) {
+  Sim* s = static_cast<Sim*>(h);
+  auto it = s->integs.find("fire_all");
+  if (it == s->integs.end()) return -1;
+  Integ& I = it->second;
+  out[0] = I.dt; out[1] = I.alpha; out[2] = I.last_P; out[3] = I.last_Fnorm; out[4] = I.last_Vnorm;
+  out[5] = I.last_E; out[6] = I.converged ? 1.0 : 0.0; out[7] = I.last_neg;
+  return 0;
+}
+
+// Loop-head force/torque evaluation: reset, Potential::compute, Aligner::compute.
+// energies: [sum of pair energies, sum of aligner energies]
+int orc_compute_forces(void* h, double* energies) {
+  Sim* s = static_cast<Sim*>(h);
+  reset_forces(*s);
+  reset_torques(*s);
+  pot_compute(*s);
+  align_compute(*s);
+  if (energies) {
+    energies[0] = pot_energy(*s);
+    double ea = 0.0; for (auto& kv : s->als) ea += kv.second.energy;
+    energies[1] = ea;
+  }
+  return 0;
+}
+
+int orc_integrate_once(void* h) {
+  Sim* s = static_cast<Sim*>(h);
+  for (auto& kv : s->integs) integrate(*s, kv.second);
+  return 0;
+}
+
+int orc_run(void* h, int steps) {
+  try { run(*static_cast<Sim*>(h), steps); }
+  catch (std::exception& e) { g_err = e.what(); return -1; }
+  return 0;
+}
+
+// Known-answer hook for the counter-based RNG.
+void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  U4 c; for (int i = 0; i < 4; i++) c.v[i] = ctr[i];
+  U4 r = philox4x32_10(c, key[0], key[1]);
+  for (int i = 0; i < 4; i++) out[i] = r.v[i];
+}
+
+void orc_gauss3(uint32_t seed, uint32_t tag, uint32_t pid, uint64_t call, double* g) {
+  gauss3(seed, tag, pid, call, g);
+}
+
+}  // extern "C"

@@ -1,0 +1,1035 @@
+// C-ABI of the device pipeline (include/samos_hip.h): the context object,
+// state transfer in particle-id order, plugin registration with the
+// reference's parameter semantics, the run loop and captured step chunks.
+#include "samos_hip.h"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace samos;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Integ {
+  std::string name;   // brownian, brownian_pos, brownian_align, langevin, fire
+  int slot = 0;       // clock slot
+  double dt = 0.0;
+  TempD temp{0, 0.0, 1.0, 1000};
+  double v0 = 1.0, nu = 1.0, mu = 1.0, mur = 1.0, stoch = 0.0, flip_p = 0.0;
+  int nematic = 0, velocity = 0;
+  uint32_t seed = 0;
+  unsigned long long calls = 0;
+  double gamma = 1.0; int method = 0; double* R = nullptr;
+  double alpha = 0.1, alpha_init = 0.1, f_alpha = 0.99, F_tol = 1e-4, E_tol = 1e-6, f_inc = 1.1, f_dec = 0.5,
+         dt_max = 1.0;
+  int N_min = 5, last_neg = 0; bool converged = false; double old_energy = 1e15;
+  double last[4] = {0, 0, 0, 0};
+};
+
+struct ProfAcc {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  size_t used = 0;
+};
+
+}  // namespace
+
+struct samos_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  int n = 0, np = 0;
+  BoxD box{};
+  std::string err;
+  double* dcol[kNumDoubleFields] = {};
+  int* icol[kNumIntFields] = {};      // type, ix, iy, iz, coord, id
+  double* alt[3] = {};                // second position buffer for fused steps
+  std::vector<int> h_id;              // slot -> id
+  double dt = 0.0;
+  int ntypes = 1;
+  // neighbour list
+  bool has_nlist = false;
+  double rcut = 3.0, pad = 0.5;
+  NListD nl{};
+  int* d_flags = nullptr;
+  int* d_nnb = nullptr;               // neighbour counts, zero while no list exists
+  // sort scratch
+  void* sort_tmp = nullptr; size_t sort_tmp_bytes = 0;
+  unsigned long long *keys = nullptr, *keys2 = nullptr; int *perm = nullptr, *perm2 = nullptr;
+  double* dtmp = nullptr; int* itmp = nullptr;
+  // interactions
+  bool has_soft = false, has_lj = false, has_pol = false, has_nem = false, has_sp = false;
+  double soft_k = 1, soft_a = 2; int soft_radii = 0;
+  double lj_eps = 1, lj_sig = 1, lj_rc = 3; int lj_shifted = 0, lj_radii = 0;
+  double pol_J = 1, pol_rc = 0; double nem_J = 1, nem_rc = 0; bool nem_pp = false;
+  double sp_alpha = 1.0;
+  std::vector<double> soft_tab, lj_tab, pol_tab, nem_tab;  // host copies [3][16][16]
+  double* d_tab = nullptr;                                  // 4 tables x 3 x 256
+  ConstraintD con{-1, 1.0, 0.0, 0};
+  std::map<std::string, Integ> integs;
+  ClockD* d_clock = nullptr;
+  int time_step = 0;
+  double* d_en = nullptr;       // 4 * np per-slot energies
+  double* d_cols = nullptr;     // 3 * np FIRE columns
+  double* d_partial = nullptr;
+  double* d_out = nullptr;      // 8 reduced values
+  double* h_out = nullptr;      // pinned
+  unsigned long long* d_counter = nullptr;
+  bool prof = false;
+  std::map<std::string, ProfAcc> profs;
+};
+
+namespace {
+
+constexpr int kTab = 3 * kMaxTypes * kMaxTypes;
+
+#define HIPCHK(expr)                                                            \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      ctx->err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr; \
+      return -2;                                                                \
+    }                                                                           \
+  } while (0)
+
+int fail(samos_ctx* ctx, const std::string& m) { ctx->err = m; return -1; }
+
+Fields fields(samos_ctx* ctx, int parity = 0) {
+  Fields f;
+  double** d = ctx->dcol;
+  f.x = parity ? ctx->alt[0] : d[0]; f.y = parity ? ctx->alt[1] : d[1]; f.z = parity ? ctx->alt[2] : d[2];
+  f.vx = d[3]; f.vy = d[4]; f.vz = d[5]; f.nx = d[6]; f.ny = d[7]; f.nz = d[8];
+  f.fx = d[9]; f.fy = d[10]; f.fz = d[11]; f.tx = d[12]; f.ty = d[13]; f.tz = d[14];
+  f.omega = d[15]; f.age = d[16]; f.radius = d[17]; f.mass = d[18];
+  f.type = ctx->icol[0]; f.ix = ctx->icol[1]; f.iy = ctx->icol[2]; f.iz = ctx->icol[3];
+  f.coord = ctx->icol[4]; f.id = ctx->icol[5];
+  return f;
+}
+
+int pot_mask(const samos_ctx* c) { return (c->has_soft ? POT_SOFT : 0) | (c->has_lj ? POT_LJ : 0); }
+int al_mask(const samos_ctx* c) { return (c->has_pol ? AL_POLAR : 0) | (c->has_nem ? AL_NEMATIC : 0); }
+
+InteractD interact_params(samos_ctx* ctx) {
+  InteractD ip{};
+  const int nt = kMaxTypes;
+  ip.soft_tab = ctx->d_tab; ip.soft_nt = nt; ip.soft_radii = ctx->soft_radii;
+  ip.lj_tab = ctx->d_tab + kTab; ip.lj_nt = nt; ip.lj_radii = ctx->lj_radii; ip.lj_shifted = ctx->lj_shifted;
+  ip.pol_tab = ctx->d_tab + 2 * kTab; ip.pol_nt = nt;
+  ip.nem_tab = ctx->d_tab + 3 * kTab; ip.nem_nt = nt;
+  ip.has_sp = ctx->has_sp ? 1 : 0; ip.sp_alpha = ctx->sp_alpha;
+  return ip;
+}
+
+int upload_tables(samos_ctx* ctx) {
+  std::vector<double> all(4 * kTab, 0.0);
+  auto put = [&](int k, const std::vector<double>& t) { if (!t.empty()) std::copy(t.begin(), t.end(), all.begin() + k * kTab); };
+  put(0, ctx->soft_tab); put(1, ctx->lj_tab); put(2, ctx->pol_tab);
+  if (!ctx->nem_tab.empty()) {
+    std::vector<double> t = ctx->nem_tab;
+    // PairNematicAlign::compute: global J is doubled, per-type J used as given
+    // (src/aligner/pair/pair_nematic_aligner.cpp:37,57-61).
+    if (!ctx->nem_pp) for (int q = 0; q < kMaxTypes * kMaxTypes; ++q) t[q] = 2.0 * ctx->nem_J;
+    put(3, t);
+  }
+  HIPCHK(hipMemcpyAsync(ctx->d_tab, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+std::vector<double> make_tab(double a, double b, double c) {
+  std::vector<double> t(kTab);
+  const int nn = kMaxTypes * kMaxTypes;
+  for (int q = 0; q < nn; ++q) { t[q] = a; t[nn + q] = b; t[2 * nn + q] = c; }
+  return t;
+}
+
+void set_tab(std::vector<double>& t, int t1, int t2, double a, double b, double c) {
+  const int nn = kMaxTypes * kMaxTypes;
+  int i = (t1 - 1) * kMaxTypes + (t2 - 1), j = (t2 - 1) * kMaxTypes + (t1 - 1);
+  t[i] = t[j] = a; t[nn + i] = t[nn + j] = b; t[2 * nn + i] = t[2 * nn + j] = c;
+}
+
+// ---- profiling ----------------------------------------------------------
+hipEvent_t prof_begin(samos_ctx* ctx, const char* cls) {
+  if (!ctx->prof) return nullptr;
+  ProfAcc& a = ctx->profs[cls];
+  if (a.used == a.ev.size()) {
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return nullptr;
+    a.ev.push_back({e0, e1});
+  }
+  hipEventRecord(a.ev[a.used].first, ctx->st);
+  return a.ev[a.used].first;
+}
+void prof_end(samos_ctx* ctx, const char* cls, hipEvent_t b) {
+  if (!ctx->prof || !b) return;
+  ProfAcc& a = ctx->profs[cls];
+  hipEventRecord(a.ev[a.used].second, ctx->st);
+  a.used++;
+}
+struct ProfScope {
+  samos_ctx* c; const char* cls; hipEvent_t b;
+  ProfScope(samos_ctx* c_, const char* cls_) : c(c_), cls(cls_) { b = prof_begin(c, cls); }
+  ~ProfScope() { prof_end(c, cls, b); }
+};
+
+// ---- neighbour list -------------------------------------------------------
+void free_nlist(samos_ctx* ctx) {
+  NListD& nl = ctx->nl;
+  hipFree(nl.cell_of); hipFree(nl.count); hipFree(nl.start); hipFree(nl.members); hipFree(nl.bsum);
+  hipFree(nl.nbr); hipFree(nl.ox); hipFree(nl.oy); hipFree(nl.oz);
+  nl.cell_of = nl.count = nl.start = nl.members = nl.bsum = nl.nbr = nullptr;
+  nl.ox = nl.oy = nl.oz = nullptr;
+}
+
+int alloc_nbr(samos_ctx* ctx, int stride) {
+  NListD& nl = ctx->nl;
+  hipFree(nl.nbr);
+  nl.nbr = nullptr;
+  nl.stride = stride;
+  HIPCHK(hipMalloc(&nl.nbr, sizeof(int) * (size_t)stride * (size_t)ctx->np + 64));
+  return 0;
+}
+
+int read_flags(samos_ctx* ctx, int* h) {
+  HIPCHK(hipMemcpyAsync(h, ctx->d_flags, 8 * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+// Synchronous build with neighbour capacity growth: rebuilds until no slot
+// overflowed, then keeps 1.5x headroom for the device-side rebuilds of a run.
+int build_sync(samos_ctx* ctx) {
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
+    HIPCHK(hipMemsetAsync(ctx->d_flags + 5, 0, sizeof(int), ctx->st));
+    {
+      ProfScope ps(ctx, "rebuild");
+      launch_rebuild(ctx->st, fields(ctx), ctx->nl, ctx->box, ctx->n, nullptr);
+    }
+    HIPCHK(hipGetLastError());
+    int h[8];
+    if (int r = read_flags(ctx, h)) return r;
+    const int need = h[5];
+    if (h[2] == 0 && ctx->nl.stride >= (int)(need * 1.25) + 8) return 0;
+    if (int r = alloc_nbr(ctx, (int)std::ceil(need * 1.5) + 16)) return r;
+  }
+  return fail(ctx, "neighbour list capacity could not be established");
+}
+
+int resort(samos_ctx* ctx) {
+  if (!ctx->has_nlist || !ctx->nl.use_cell || ctx->n < 2) return 0;
+  const int n = ctx->n;
+  Fields f = fields(ctx);
+  launch_morton_keys(ctx->st, f, ctx->nl, ctx->box, n, ctx->keys, ctx->perm);
+  radix_sort_pairs(ctx->st, ctx->sort_tmp, ctx->sort_tmp_bytes, ctx->keys, ctx->keys2, ctx->perm, ctx->perm2, n);
+  for (int k = 0; k < kNumDoubleFields; ++k) {
+    launch_gather_d(ctx->st, ctx->dtmp, ctx->dcol[k], ctx->perm2, n);
+    HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->dtmp, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
+  }
+  for (int k = 0; k < kNumIntFields; ++k) {
+    launch_gather_i(ctx->st, ctx->itmp, ctx->icol[k], ctx->perm2, n);
+    HIPCHK(hipMemcpyAsync(ctx->icol[k], ctx->itmp, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->st));
+  }
+  HIPCHK(hipMemcpyAsync(ctx->h_id.data(), ctx->icol[5], sizeof(int) * n, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---- forces ---------------------------------------------------------------
+int energies_to_host(samos_ctx* ctx, double* e4) {
+  launch_reduce_sum(ctx->st, ctx->d_en, 4, ctx->np, ctx->n, ctx->d_partial, ctx->d_out);
+  HIPCHK(hipMemcpyAsync(ctx->h_out, ctx->d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  for (int k = 0; k < 4; ++k) e4[k] = 0.5 * ctx->h_out[k];
+  return 0;
+}
+
+void interact(samos_ctx* ctx, int pm, int am, int mode, double* e_out, bool f32 = false) {
+  if (!ctx->has_nlist) { pm = 0; am = 0; }
+  if (pm == 0 && am == 0 && !ctx->has_sp) {
+    if (e_out) hipMemsetAsync(e_out, 0, sizeof(double) * 4 * ctx->np, ctx->st);
+    return;
+  }
+  ProfScope ps(ctx, "interact");
+  if (f32) launch_interact_f32(ctx->st, pm, am, mode, fields(ctx), ctx->nl, ctx->box, interact_params(ctx), ctx->n, e_out);
+  else launch_interact(ctx->st, pm, am, mode, fields(ctx), ctx->nl, ctx->box, interact_params(ctx), ctx->n, e_out);
+}
+
+// Loop head: reset f and tau (System::reset_forces / reset_torques), then the
+// pair potentials and aligners in one pass.  Columns the pass does not write
+// are cleared here.
+void loop_head_forces(samos_ctx* ctx, double* e_out, bool f32 = false) {
+  const int pm = ctx->has_nlist ? pot_mask(ctx) : 0;
+  const int am = ctx->has_nlist ? al_mask(ctx) : 0;
+  if (pm == 0 && !ctx->has_sp)
+    for (int k = 9; k <= 11; ++k) hipMemsetAsync(ctx->dcol[k], 0, sizeof(double) * ctx->n, ctx->st);
+  if (am == 0)
+    for (int k = 12; k <= 14; ++k) hipMemsetAsync(ctx->dcol[k], 0, sizeof(double) * ctx->n, ctx->st);
+  interact(ctx, pm, am, 0, e_out, f32);
+}
+
+bool needs_nlist(const samos_ctx* c) {
+  return c->has_nlist && (c->has_soft || c->has_lj || c->has_pol || c->has_nem);
+}
+
+// Device-side conditional rebuild after a step: flags[0] = any(need_update).
+void check_and_rebuild(samos_ctx* ctx) {
+  if (!needs_nlist(ctx)) return;
+  hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->st);
+  Fields f = fields(ctx);
+  launch_need_update(ctx->st, f, ctx->nl, ctx->box, ctx->n, ctx->d_flags);
+  ProfScope ps(ctx, "rebuild");
+  launch_rebuild(ctx->st, f, ctx->nl, ctx->box, ctx->n, ctx->d_flags);
+}
+
+// ---- integrators ----------------------------------------------------------
+int integrate_one(samos_ctx* ctx, Integ& I, int run_step) {
+  Fields f = fields(ctx);
+  const int n = ctx->n;
+  const unsigned long long call = I.calls++;
+  if (I.name == "brownian" || I.name == "brownian_pos" || I.name == "brownian_align") {
+    int which = I.name == "brownian" ? 0 : (I.name == "brownian_pos" ? 1 : 2);
+    ProfScope ps(ctx, "integrate");
+    launch_brownian(ctx->st, which, f, ctx->box, ctx->con, n, I.dt, I.v0, I.mu, I.mur, I.stoch, I.flip_p, I.nematic,
+                    I.velocity, I.temp, I.seed, call, nullptr, I.slot, run_step);
+  } else if (I.name == "langevin") {
+    {
+      ProfScope ps(ctx, "integrate");
+      launch_langevin_a(ctx->st, I.method, f, ctx->box, ctx->con, n, I.dt, I.gamma, I.temp, I.seed, call, nullptr,
+                        I.slot, run_step, I.R);
+    }
+    interact(ctx, pot_mask(ctx), 0, 0, nullptr);
+    ProfScope ps(ctx, "integrate");
+    launch_langevin_b(ctx->st, I.method, f, n, I.dt, I.gamma, I.temp, I.seed, call, nullptr, I.slot, run_step, I.R,
+                      ctx->box, ctx->con);
+  } else if (I.name == "fire") {
+    {
+      ProfScope ps(ctx, "integrate");
+      launch_fire_a(ctx->st, f, ctx->box, ctx->con, n, I.dt);
+    }
+    // reset forces and torques, Potential::compute, Aligner::compute
+    loop_head_forces(ctx, ctx->d_en);
+    {
+      ProfScope ps(ctx, "integrate");
+      launch_fire_b(ctx->st, f, ctx->box, ctx->con, n, I.dt, ctx->d_cols, ctx->np);
+    }
+    launch_reduce_sum(ctx->st, ctx->d_cols, 3, ctx->np, n, ctx->d_partial, ctx->d_out);
+    launch_reduce_sum(ctx->st, ctx->d_en, 2, ctx->np, n, ctx->d_partial, ctx->d_out + 3);
+    HIPCHK(hipMemcpyAsync(ctx->h_out, ctx->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    // IntegratorFIRE::integrate, scalar part (src/integrators/integrator_fire.cpp:97-147)
+    const double P = ctx->h_out[0];
+    const double Fn = std::sqrt(ctx->h_out[1]);
+    const double Vn = std::sqrt(ctx->h_out[2]);
+    const double E = 0.5 * ctx->h_out[3] + 0.5 * ctx->h_out[4];
+    const double sqrt_ndof = std::sqrt((double)(3 * n));
+    I.last[0] = P; I.last[1] = Fn; I.last[2] = Vn; I.last[3] = E;
+    if (Fn / sqrt_ndof < I.F_tol && std::fabs(E - I.old_energy) < I.E_tol) {
+      I.converged = true;
+      return 0;
+    }
+    const double inv_F = 1.0 / Fn;
+    const double f1 = 1.0 - I.alpha;
+    const double f2 = I.alpha * inv_F * Vn;
+    int zero_v = 0;
+    if (P > 0.0) {
+      I.last_neg++;
+      if (I.last_neg > I.N_min) { I.dt = std::min(I.dt * I.f_inc, I.dt_max); I.alpha *= I.f_alpha; }
+    } else {
+      I.dt *= I.f_dec;
+      I.alpha = I.alpha_init;
+      I.last_neg = 0;
+      zero_v = 1;
+    }
+    launch_fire_mix(ctx->st, f, n, f1, f2, zero_v);
+    I.converged = false;
+    I.old_energy = E;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---- fused Langevin run ---------------------------------------------------
+int upload_clock(samos_ctx* ctx) {
+  ClockD c{};
+  for (auto& kv : ctx->integs) c.calls[kv.second.slot] = kv.second.calls;
+  c.run_step = 0;
+  HIPCHK(hipMemcpyAsync(ctx->d_clock, &c, sizeof(ClockD), hipMemcpyHostToDevice, ctx->st));
+  return 0;
+}
+
+// One fused step t of the Langevin BAOAB schedule with positions in buffer
+// (t % 2), followed by the device-conditional rebuild on the new positions.
+void fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int t_off, int steps, bool f32) {
+  const int par = t & 1;
+  Fields cur = fields(ctx, par), nxt = fields(ctx, par ^ 1);
+  {
+    ProfScope ps(ctx, "fused");
+    launch_baoab_fused(ctx->st, pot_mask(ctx), phase, cur, nxt, ctx->nl, ctx->box, ctx->con, interact_params(ctx),
+                       ctx->n, I.dt, I.gamma, I.temp, I.seed, ctx->d_clock, I.slot, t_off, par, t + 1 <= steps ? 1 : 0,
+                       f32 ? 1 : 0);
+  }
+  if (phase != 2) {
+    ProfScope ps(ctx, "rebuild");
+    launch_rebuild(ctx->st, nxt, ctx->nl, ctx->box, ctx->n, ctx->d_flags + par);
+  }
+}
+
+int run_fused_langevin(samos_ctx* ctx, Integ& I, int steps, int flags) {
+  const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
+  const bool graph = !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
+  if (int r = upload_clock(ctx)) return r;
+  HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
+  HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
+  // loop head of t = 0
+  launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+  fused_step(ctx, I, 0, 0, 0, steps, f32);
+  launch_clock_advance(ctx->st, ctx->d_clock, 1);
+  int t = 1;
+  const int K = 64;  // steps per captured chunk (even, so buffer parity repeats)
+  if (graph && steps - t + 1 >= 2 * K) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < K; ++j) fused_step(ctx, I, 1, t + j, j, steps, f32);
+    launch_clock_advance(ctx->st, ctx->d_clock, K);
+    HIPCHK(hipStreamEndCapture(ctx->st, &g));
+    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    // every chunk must end before the last step, which has no following loop head
+    while (steps - t + 1 > K) {
+      HIPCHK(hipGraphLaunch(ge, ctx->st));
+      t += K;
+    }
+    HIPCHK(hipGraphExecDestroy(ge));
+    HIPCHK(hipGraphDestroy(g));
+  }
+  const int t0 = t;
+  for (; t <= steps; ++t) fused_step(ctx, I, 1, t, t - t0, steps, f32);
+  launch_clock_advance(ctx->st, ctx->d_clock, steps + 1 - t0);
+  fused_step(ctx, I, 2, steps + 1, 0, steps, f32);
+  if ((steps + 1) & 1) {
+    for (int k = 0; k < 3; ++k)
+      HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->alt[k], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
+  }
+  HIPCHK(hipGetLastError());
+  I.calls += (unsigned long long)(steps + 1);
+  return 0;
+}
+
+int check_overflow(samos_ctx* ctx) {
+  int h[8];
+  if (int r = read_flags(ctx, h)) return r;
+  if (h[2]) {
+    return fail(ctx, "Neighbour list overflow during the run: a particle had " + std::to_string(h[5]) +
+                         " neighbours, capacity " + std::to_string(ctx->nl.stride) + ".");
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int samos_create(samos_ctx** out, int device, int n, double lx, double ly, double lz, int periodic) {
+  *out = nullptr;
+  if (n < 0) { g_create_error = "negative particle count"; return -1; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { g_create_error = "no HIP device"; return -2; }
+  if (device < 0 || device >= ndev) { g_create_error = "device index out of range"; return -1; }
+  if (hipSetDevice(device) != hipSuccess) { g_create_error = "hipSetDevice failed"; return -2; }
+  samos_ctx* ctx = new samos_ctx();
+  ctx->device = device;
+  ctx->n = n;
+  ctx->np = ((n + 63) / 64) * 64;
+  if (ctx->np == 0) ctx->np = 64;
+  BoxD& b = ctx->box;
+  b.Lx = lx; b.Ly = ly; b.Lz = lz;
+  b.xlo = -0.5 * lx; b.xhi = 0.5 * lx; b.ylo = -0.5 * ly; b.yhi = 0.5 * ly; b.zlo = -0.5 * lz; b.zhi = 0.5 * lz;
+  b.periodic = periodic ? 1 : 0;
+  auto bad = [&](const char* what) { g_create_error = std::string("allocation failed: ") + what; samos_destroy(ctx); return -2; };
+  if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess) return bad("stream");
+  const size_t np = ctx->np;
+  for (int k = 0; k < kNumDoubleFields; ++k)
+    if (hipMalloc(&ctx->dcol[k], sizeof(double) * np) != hipSuccess) return bad("columns");
+  for (int k = 0; k < kNumIntFields; ++k)
+    if (hipMalloc(&ctx->icol[k], sizeof(int) * np) != hipSuccess) return bad("columns");
+  for (int k = 0; k < 3; ++k)
+    if (hipMalloc(&ctx->alt[k], sizeof(double) * np) != hipSuccess) return bad("columns");
+  if (hipMalloc(&ctx->d_flags, 16 * sizeof(int)) != hipSuccess) return bad("flags");
+  if (hipMalloc(&ctx->d_nnb, sizeof(int) * np) != hipSuccess) return bad("neighbour counts");
+  hipMemsetAsync(ctx->d_nnb, 0, sizeof(int) * np, ctx->st);
+  ctx->nl.nnb = ctx->d_nnb;
+  ctx->nl.np = ctx->np;
+  ctx->nl.flags = ctx->d_flags;
+  if (hipMalloc(&ctx->d_tab, 4 * kTab * sizeof(double)) != hipSuccess) return bad("tables");
+  if (hipMalloc(&ctx->d_clock, sizeof(ClockD)) != hipSuccess) return bad("clock");
+  if (hipMalloc(&ctx->d_en, 4 * np * sizeof(double)) != hipSuccess) return bad("energies");
+  if (hipMalloc(&ctx->d_cols, 3 * np * sizeof(double)) != hipSuccess) return bad("fire columns");
+  if (hipMalloc(&ctx->d_partial, 8 * (size_t)reduce_blocks(ctx->np) * sizeof(double)) != hipSuccess) return bad("partials");
+  if (hipMalloc(&ctx->d_out, 16 * sizeof(double)) != hipSuccess) return bad("out");
+  if (hipHostMalloc(&ctx->h_out, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) return bad("pinned out");
+  if (hipMalloc(&ctx->d_counter, sizeof(unsigned long long)) != hipSuccess) return bad("counter");
+  if (hipMalloc(&ctx->keys, sizeof(unsigned long long) * np) != hipSuccess) return bad("sort");
+  if (hipMalloc(&ctx->keys2, sizeof(unsigned long long) * np) != hipSuccess) return bad("sort");
+  if (hipMalloc(&ctx->perm, sizeof(int) * np) != hipSuccess) return bad("sort");
+  if (hipMalloc(&ctx->perm2, sizeof(int) * np) != hipSuccess) return bad("sort");
+  if (hipMalloc(&ctx->dtmp, sizeof(double) * np) != hipSuccess) return bad("sort");
+  if (hipMalloc(&ctx->itmp, sizeof(int) * np) != hipSuccess) return bad("sort");
+  ctx->sort_tmp_bytes = radix_sort_temp_bytes(ctx->n > 0 ? ctx->n : 1);
+  if (hipMalloc(&ctx->sort_tmp, ctx->sort_tmp_bytes + 256) != hipSuccess) return bad("sort");
+  // defaults: type 1, radius 1, mass 1, everything else zero; id = slot
+  hipMemsetAsync(ctx->d_flags, 0, 16 * sizeof(int), ctx->st);
+  hipMemsetAsync(ctx->d_tab, 0, 4 * kTab * sizeof(double), ctx->st);
+  for (int k = 0; k < kNumDoubleFields; ++k) hipMemsetAsync(ctx->dcol[k], 0, sizeof(double) * np, ctx->st);
+  for (int k = 0; k < kNumIntFields; ++k) hipMemsetAsync(ctx->icol[k], 0, sizeof(int) * np, ctx->st);
+  ctx->h_id.resize(n);
+  for (int i = 0; i < n; ++i) ctx->h_id[i] = i;
+  std::vector<double> ones(n, 1.0);
+  std::vector<int> ione(n, 1);
+  if (n > 0) {
+    hipMemcpyAsync(ctx->dcol[17], ones.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st);
+    hipMemcpyAsync(ctx->dcol[18], ones.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st);
+    hipMemcpyAsync(ctx->icol[0], ione.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->st);
+    hipMemcpyAsync(ctx->icol[5], ctx->h_id.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->st);
+  }
+  if (hipStreamSynchronize(ctx->st) != hipSuccess) return bad("initialisation");
+  *out = ctx;
+  return 0;
+}
+
+int samos_destroy(samos_ctx* ctx) {
+  if (!ctx) return 0;
+  if (ctx->st) hipStreamSynchronize(ctx->st);
+  for (auto& p : ctx->profs)
+    for (auto& e : p.second.ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+  free_nlist(ctx);
+  for (auto& kv : ctx->integs) hipFree(kv.second.R);
+  for (int k = 0; k < kNumDoubleFields; ++k) hipFree(ctx->dcol[k]);
+  for (int k = 0; k < kNumIntFields; ++k) hipFree(ctx->icol[k]);
+  for (int k = 0; k < 3; ++k) hipFree(ctx->alt[k]);
+  hipFree(ctx->d_flags); hipFree(ctx->d_nnb); hipFree(ctx->d_tab); hipFree(ctx->d_clock); hipFree(ctx->d_en); hipFree(ctx->d_cols);
+  hipFree(ctx->d_partial); hipFree(ctx->d_out); hipFree(ctx->d_counter);
+  if (ctx->h_out) hipHostFree(ctx->h_out);
+  hipFree(ctx->keys); hipFree(ctx->keys2); hipFree(ctx->perm); hipFree(ctx->perm2);
+  hipFree(ctx->dtmp); hipFree(ctx->itmp); hipFree(ctx->sort_tmp);
+  if (ctx->st) hipStreamDestroy(ctx->st);
+  delete ctx;
+  return 0;
+}
+
+const char* samos_last_error(const samos_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int samos_num_particles(const samos_ctx* ctx) { return ctx ? ctx->n : -1; }
+
+int samos_set_field(samos_ctx* ctx, int field, const void* host) {
+  const int n = ctx->n;
+  if (n == 0) return 0;
+  if (field >= 0 && field < kNumDoubleFields) {
+    const double* h = static_cast<const double*>(host);
+    std::vector<double> tmp(n);
+    for (int s = 0; s < n; ++s) tmp[s] = h[ctx->h_id[s]];
+    HIPCHK(hipMemcpyAsync(ctx->dcol[field], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    if (field <= 2) HIPCHK(hipMemcpyAsync(ctx->alt[field], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+  } else if (field >= SAMOS_TYPE && field < SAMOS_TYPE + 5) {
+    const int* h = static_cast<const int*>(host);
+    std::vector<int> tmp(n);
+    for (int s = 0; s < n; ++s) tmp[s] = h[ctx->h_id[s]];
+    if (field == SAMOS_TYPE) {
+      int mx = 1;
+      for (int v : tmp) {
+        if (v < 1 || v > kMaxTypes)
+          return fail(ctx, "particle type " + std::to_string(v) + " outside 1.." + std::to_string(kMaxTypes));
+        mx = std::max(mx, v);
+      }
+      ctx->ntypes = mx;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->icol[field - SAMOS_TYPE], tmp.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->st));
+  } else {
+    return fail(ctx, "unknown field code " + std::to_string(field));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int samos_get_field(samos_ctx* ctx, int field, void* host) {
+  const int n = ctx->n;
+  if (n == 0) return 0;
+  if (field >= 0 && field < kNumDoubleFields) {
+    std::vector<double> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), ctx->dcol[field], sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    double* h = static_cast<double*>(host);
+    for (int s = 0; s < n; ++s) h[ctx->h_id[s]] = tmp[s];
+  } else if (field >= SAMOS_TYPE && field < SAMOS_TYPE + 5) {
+    std::vector<int> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), ctx->icol[field - SAMOS_TYPE], sizeof(int) * n, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    int* h = static_cast<int*>(host);
+    for (int s = 0; s < n; ++s) h[ctx->h_id[s]] = tmp[s];
+  } else {
+    return fail(ctx, "unknown field code " + std::to_string(field));
+  }
+  return 0;
+}
+
+int samos_set_timestep(samos_ctx* ctx, double dt) {
+  ctx->dt = dt;
+  return 0;
+}
+
+int samos_nlist_create(samos_ctx* ctx, double rcut, double pad) {
+  free_nlist(ctx);
+  ctx->has_nlist = true;
+  ctx->rcut = rcut;
+  ctx->pad = pad;
+  NListD& nl = ctx->nl;
+  nl = NListD{};
+  const double cut = rcut + pad;
+  nl.cut2 = cut * cut;
+  nl.pad2q = 0.25 * pad * pad;
+  nl.np = ctx->np;
+  nl.flags = ctx->d_flags;
+  const BoxD& b = ctx->box;
+  nl.use_cell = (b.Lx > 2.0 * cut && b.Ly > 2.0 * cut && b.Lz > 2.0 * cut) ? 1 : 0;
+  if (nl.use_cell) {
+    // CellList::CellList (src/system/cell_list.cpp:37-68)
+    nl.ncx = (int)(b.Lx / cut); nl.ncy = (int)(b.Ly / cut); nl.ncz = (int)(b.Lz / cut);
+    const long ncell = (long)nl.ncx * nl.ncy * nl.ncz;
+    if (ncell > 400000000L) return fail(ctx, "cell grid too large");
+    nl.ncell = (int)ncell;
+    nl.wx = b.Lx / nl.ncx; nl.wy = b.Ly / nl.ncy; nl.wz = b.Lz / nl.ncz;
+    HIPCHK(hipMalloc(&nl.count, sizeof(int) * (size_t)nl.ncell));
+    HIPCHK(hipMalloc(&nl.start, sizeof(int) * ((size_t)nl.ncell + 1)));
+    HIPCHK(hipMalloc(&nl.bsum, sizeof(int) * ((size_t)nl.ncell / 4096 + 2)));
+    HIPCHK(hipMemsetAsync(nl.count, 0, sizeof(int) * (size_t)nl.ncell, ctx->st));
+  } else {
+    nl.ncx = nl.ncy = nl.ncz = 0;
+    nl.ncell = 0;
+  }
+  HIPCHK(hipMalloc(&nl.cell_of, sizeof(int) * ctx->np));
+  HIPCHK(hipMalloc(&nl.members, sizeof(int) * ctx->np));
+  nl.nnb = ctx->d_nnb;
+  HIPCHK(hipMalloc(&nl.ox, sizeof(double) * ctx->np));
+  HIPCHK(hipMalloc(&nl.oy, sizeof(double) * ctx->np));
+  HIPCHK(hipMalloc(&nl.oz, sizeof(double) * ctx->np));
+  HIPCHK(hipMemsetAsync(nl.nnb, 0, sizeof(int) * ctx->np, ctx->st));
+  // initial capacity from the mean density, grown by build_sync when needed
+  const double vol = b.Lx * b.Ly * b.Lz;
+  const double dens = vol > 0 ? ctx->n / vol : 0.0;
+  int guess = (int)(dens * 4.0 / 3.0 * M_PI * cut * cut * cut * 2.0) + 32;
+  guess = std::min(std::max(guess, 32), std::max(ctx->n, 32));
+  if (int r = alloc_nbr(ctx, guess)) return r;
+  // the reference's initial build works on the input order; re-sorting first
+  // changes only the storage order
+  if (nl.use_cell) {
+    if (int r = build_sync(ctx)) return r;
+    if (int r = resort(ctx)) return r;
+  }
+  return build_sync(ctx);
+}
+
+int samos_nlist_build(samos_ctx* ctx) {
+  if (!ctx->has_nlist) return fail(ctx, "no neighbour list");
+  return build_sync(ctx);
+}
+
+int samos_resort(samos_ctx* ctx) {
+  if (!ctx->has_nlist) return fail(ctx, "no neighbour list");
+  if (int r = resort(ctx)) return r;
+  return build_sync(ctx);
+}
+
+int samos_nlist_need_update(samos_ctx* ctx, int* out) {
+  if (!ctx->has_nlist) return fail(ctx, "no neighbour list");
+  HIPCHK(hipMemsetAsync(ctx->d_flags + 6, 0, sizeof(int), ctx->st));
+  launch_need_update(ctx->st, fields(ctx), ctx->nl, ctx->box, ctx->n, ctx->d_flags + 6);
+  int h[8];
+  if (int r = read_flags(ctx, h)) return r;
+  *out = h[6] ? 1 : 0;
+  return 0;
+}
+
+int samos_nlist_info(samos_ctx* ctx, long long* info) {
+  int h[8];
+  if (int r = read_flags(ctx, h)) return r;
+  info[0] = ctx->nl.use_cell; info[1] = ctx->nl.ncx; info[2] = ctx->nl.ncy; info[3] = ctx->nl.ncz;
+  info[4] = ctx->nl.stride; info[5] = h[4]; info[6] = h[3]; info[7] = h[2];
+  return 0;
+}
+
+int samos_nlist_pairs(samos_ctx* ctx, int* out, long long cap, long long* count) {
+  if (!ctx->has_nlist) return fail(ctx, "no neighbour list");
+  // total directed entries bound the half-pair count
+  HIPCHK(hipMemsetAsync(ctx->d_counter, 0, sizeof(unsigned long long), ctx->st));
+  std::vector<int> nnb(ctx->n);
+  HIPCHK(hipMemcpyAsync(nnb.data(), ctx->nl.nnb, sizeof(int) * ctx->n, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  long long total = 0;
+  for (int v : nnb) total += v;
+  const long long half_cap = total / 2 + 1;
+  int* d_out = nullptr;
+  HIPCHK(hipMalloc(&d_out, sizeof(int) * 2 * (size_t)half_cap));
+  launch_export_pairs(ctx->st, fields(ctx), ctx->nl, ctx->n, d_out, ctx->d_counter, half_cap);
+  unsigned long long cnt = 0;
+  HIPCHK(hipMemcpyAsync(&cnt, ctx->d_counter, sizeof(cnt), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if ((long long)cnt > half_cap) { hipFree(d_out); return fail(ctx, "pair export overflow"); }
+  std::vector<std::pair<int, int>> pr(cnt);
+  std::vector<int> raw(2 * cnt);
+  if (cnt) HIPCHK(hipMemcpy(raw.data(), d_out, sizeof(int) * 2 * cnt, hipMemcpyDeviceToHost));
+  hipFree(d_out);
+  for (size_t k = 0; k < cnt; ++k) pr[k] = {raw[2 * k], raw[2 * k + 1]};
+  std::sort(pr.begin(), pr.end());
+  *count = (long long)cnt;
+  if (out) {
+    if ((long long)cnt > cap) return fail(ctx, "output capacity too small");
+    for (size_t k = 0; k < cnt; ++k) { out[2 * k] = pr[k].first; out[2 * k + 1] = pr[k].second; }
+  }
+  return 0;
+}
+
+int samos_pair_soft(samos_ctx* ctx, double k, double a, int use_radii) {
+  ctx->has_soft = true; ctx->soft_k = k; ctx->soft_a = a; ctx->soft_radii = use_radii ? 1 : 0;
+  ctx->soft_tab = make_tab(k, a, 0.0);
+  return upload_tables(ctx);
+}
+
+int samos_pair_lj(samos_ctx* ctx, double eps, double sigma, double rcut, int shifted, int use_radii) {
+  ctx->has_lj = true; ctx->lj_eps = eps; ctx->lj_sig = sigma; ctx->lj_rc = rcut;
+  ctx->lj_shifted = shifted ? 1 : 0; ctx->lj_radii = use_radii ? 1 : 0;
+  ctx->lj_tab = make_tab(eps, sigma, rcut);
+  return upload_tables(ctx);
+}
+
+int samos_pair_type_param(samos_ctx* ctx, int kind, int t1, int t2, double p0, double p1, double p2, int shifted) {
+  if (t1 < 1 || t2 < 1 || t1 > kMaxTypes || t2 > kMaxTypes) return fail(ctx, "type out of range");
+  if (kind == SAMOS_PAIR_SOFT) {
+    if (!ctx->has_soft) return fail(ctx, "soft potential not defined");
+    set_tab(ctx->soft_tab, t1, t2, p0, p1, 0.0);
+  } else if (kind == SAMOS_PAIR_LJ) {
+    if (!ctx->has_lj) return fail(ctx, "lj potential not defined");
+    set_tab(ctx->lj_tab, t1, t2, p0, p1, p2);
+    if (shifted) ctx->lj_shifted = 1;
+  } else {
+    return fail(ctx, "unknown pair potential kind");
+  }
+  return upload_tables(ctx);
+}
+
+int samos_align_polar(samos_ctx* ctx, double J, double rcut) {
+  if (rcut <= 0.0) {
+    if (!ctx->has_nlist) return fail(ctx, "polar aligner needs a neighbour list");
+    rcut = ctx->rcut;
+  }
+  ctx->has_pol = true; ctx->pol_J = J; ctx->pol_rc = rcut;
+  ctx->pol_tab = make_tab(J, rcut, 0.0);
+  return upload_tables(ctx);
+}
+
+int samos_align_nematic(samos_ctx* ctx, double J, double rcut) {
+  if (rcut <= 0.0) {
+    if (!ctx->has_nlist) return fail(ctx, "nematic aligner needs a neighbour list");
+    rcut = ctx->rcut;
+  }
+  ctx->has_nem = true; ctx->nem_J = J; ctx->nem_rc = rcut; ctx->nem_pp = false;
+  ctx->nem_tab = make_tab(J, rcut, 0.0);
+  return upload_tables(ctx);
+}
+
+int samos_align_type_param(samos_ctx* ctx, int kind, int t1, int t2, double J, double rcut) {
+  if (t1 < 1 || t2 < 1 || t1 > kMaxTypes || t2 > kMaxTypes) return fail(ctx, "type out of range");
+  if (kind == SAMOS_ALIGN_POLAR) {
+    if (!ctx->has_pol) return fail(ctx, "polar aligner not defined");
+    set_tab(ctx->pol_tab, t1, t2, J, rcut, 0.0);
+  } else if (kind == SAMOS_ALIGN_NEMATIC) {
+    if (!ctx->has_nem) return fail(ctx, "nematic aligner not defined");
+    set_tab(ctx->nem_tab, t1, t2, J, rcut, 0.0);
+    ctx->nem_pp = true;
+  } else {
+    return fail(ctx, "unknown aligner kind");
+  }
+  return upload_tables(ctx);
+}
+
+int samos_external_self_propulsion(samos_ctx* ctx, double alpha) {
+  ctx->has_sp = true;
+  ctx->sp_alpha = alpha;
+  return 0;
+}
+
+int samos_constraint(samos_ctx* ctx, int kind, double param, int flag) {
+  if (kind < 0 || kind > 2) return fail(ctx, "unknown constraint kind");
+  ctx->con.kind = kind;
+  if (kind == 1) { ctx->con.zpos = param; ctx->con.unlimited = flag ? 1 : 0; }
+  if (kind == 2) ctx->con.r = param;
+  launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int samos_enforce_all(samos_ctx* ctx) {
+  launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+static int add_integ(samos_ctx* ctx, Integ& I, const std::string& name) {
+  if (ctx->dt <= 0.0)
+    return fail(ctx, "Integrator step size has to be larger than 0. Did you forget to use \"timestep\" command?");
+  const std::string key = name + "_all";
+  if (ctx->integs.count(key))
+    return fail(ctx, "Integrator of type " + name + " (for group all) already exists. You need to disable the old one first.");
+  if (ctx->integs.size() >= 8) return fail(ctx, "too many integrators");
+  I.name = name;
+  I.dt = ctx->dt;
+  int slot = 0;
+  for (;; ++slot) {
+    bool used = false;
+    for (auto& kv : ctx->integs) used |= kv.second.slot == slot;
+    if (!used) break;
+  }
+  I.slot = slot;
+  ctx->integs[key] = I;
+  return 0;
+}
+
+int samos_integrator_brownian(samos_ctx* ctx, int which, double v0, double nu, double mu, double mur, int seed,
+                              int nematic, double tau, int velocity_align, int tkind, double tmin, double tmax,
+                              int tsteps) {
+  if (which < 0 || which > 2) return fail(ctx, "unknown brownian integrator kind");
+  Integ I;
+  I.v0 = v0; I.nu = nu; I.mu = mu; I.mur = mur; I.seed = (uint32_t)seed;
+  I.nematic = nematic ? 1 : 0; I.velocity = velocity_align ? 1 : 0;
+  I.temp = TempD{tkind, tmin, tmax, tsteps};
+  const char* nm = which == 0 ? "brownian" : (which == 1 ? "brownian_pos" : "brownian_align");
+  if (int r = add_integ(ctx, I, nm)) return r;
+  Integ& J = ctx->integs[std::string(nm) + "_all"];
+  J.flip_p = J.dt / tau;
+  J.stoch = std::sqrt(J.nu * J.dt);
+  return 0;
+}
+
+int samos_integrator_langevin(samos_ctx* ctx, double gamma, int seed, int method, int tkind, double tmin,
+                              double tmax, int tsteps) {
+  if (method < 0 || method > 2) return fail(ctx, "Unknown integration method");
+  Integ I;
+  I.gamma = gamma; I.seed = (uint32_t)seed; I.method = method;
+  I.temp = TempD{tkind, tmin, tmax, tsteps};
+  if (int r = add_integ(ctx, I, "langevin")) return r;
+  Integ& J = ctx->integs["langevin_all"];
+  if (method == 2) {
+    HIPCHK(hipMalloc(&J.R, sizeof(double) * 3 * (size_t)std::max(ctx->n, 1)));
+    launch_bbk_init(ctx->st, fields(ctx), ctx->n, J.seed, J.R);
+    HIPCHK(hipStreamSynchronize(ctx->st));
+  }
+  return 0;
+}
+
+int samos_integrator_fire(samos_ctx* ctx, double alpha, double alpha_scale, double f_tol, double e_tol,
+                          double dt_inc, double dt_dec, double dt_max, int min_neg_steps) {
+  Integ I;
+  I.alpha = alpha; I.alpha_init = alpha; I.f_alpha = alpha_scale; I.F_tol = f_tol; I.E_tol = e_tol;
+  I.f_inc = dt_inc; I.f_dec = dt_dec; I.dt_max = dt_max; I.N_min = min_neg_steps;
+  return add_integ(ctx, I, "fire");
+}
+
+int samos_fire_state(samos_ctx* ctx, double* out) {
+  auto it = ctx->integs.find("fire_all");
+  if (it == ctx->integs.end()) return fail(ctx, "no FIRE integrator");
+  Integ& I = it->second;
+  out[0] = I.dt; out[1] = I.alpha; out[2] = I.last[0]; out[3] = I.last[1]; out[4] = I.last[2];
+  out[5] = I.last[3]; out[6] = I.converged ? 1.0 : 0.0; out[7] = I.last_neg;
+  return 0;
+}
+
+int samos_compute_forces(samos_ctx* ctx, double* energies) {
+  loop_head_forces(ctx, energies ? ctx->d_en : nullptr);
+  HIPCHK(hipGetLastError());
+  if (energies) return energies_to_host(ctx, energies);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int samos_pair_compute(samos_ctx* ctx, int kind, double* energy) {
+  int pm = kind == SAMOS_PAIR_SOFT ? POT_SOFT : POT_LJ;
+  if ((pm == POT_SOFT && !ctx->has_soft) || (pm == POT_LJ && !ctx->has_lj)) return fail(ctx, "pair potential not defined");
+  if (!ctx->has_nlist) return fail(ctx, "pair potential needs a neighbour list");
+  const bool sp = ctx->has_sp;
+  ctx->has_sp = false;
+  {
+    ProfScope ps(ctx, "interact");
+    launch_interact(ctx->st, pm, 0, 1, fields(ctx), ctx->nl, ctx->box, interact_params(ctx), ctx->n,
+                    energy ? ctx->d_en : nullptr);
+  }
+  ctx->has_sp = sp;
+  HIPCHK(hipGetLastError());
+  if (energy) {
+    double e4[4];
+    if (int r = energies_to_host(ctx, e4)) return r;
+    *energy = pm == POT_SOFT ? e4[0] : e4[1];
+    return 0;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int samos_align_compute(samos_ctx* ctx, int kind, double* energy) {
+  int am = kind == SAMOS_ALIGN_POLAR ? AL_POLAR : AL_NEMATIC;
+  if ((am == AL_POLAR && !ctx->has_pol) || (am == AL_NEMATIC && !ctx->has_nem)) return fail(ctx, "aligner not defined");
+  if (!ctx->has_nlist) return fail(ctx, "aligner needs a neighbour list");
+  const bool sp = ctx->has_sp;
+  ctx->has_sp = false;
+  {
+    ProfScope ps(ctx, "interact");
+    launch_interact(ctx->st, 0, am, 1, fields(ctx), ctx->nl, ctx->box, interact_params(ctx), ctx->n,
+                    energy ? ctx->d_en : nullptr);
+  }
+  ctx->has_sp = sp;
+  HIPCHK(hipGetLastError());
+  if (energy) {
+    double e4[4];
+    if (int r = energies_to_host(ctx, e4)) return r;
+    *energy = am == AL_POLAR ? e4[2] : e4[3];
+    return 0;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int samos_integrate(samos_ctx* ctx) {
+  if (ctx->con.kind < 0) return fail(ctx, "Constraint has not been defined. Please define it using \"constraint\" command before running simulation.");
+  for (auto& kv : ctx->integs)
+    if (int r = integrate_one(ctx, kv.second, 0)) return r;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int samos_run(samos_ctx* ctx, int steps, int flags) {
+  if (ctx->con.kind < 0)
+    return fail(ctx, "Constraint has not been defined. Please define it using \"constraint\" command before running simulation.");
+  if (ctx->integs.empty())
+    return fail(ctx, "Integrator has not been defined. Please define it using \"integrator\" command before running simulation.");
+  if (steps < 0) return fail(ctx, "negative step count");
+  const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
+  HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
+  // Fused schedule: one Langevin BAOAB integrator, pair potentials only.  The
+  // reference's loop-head force evaluation repeats the evaluation that ended
+  // the previous step on the same positions, so it is done once.
+  bool fused = !(flags & SAMOS_RUN_FAITHFUL) && ctx->integs.size() == 1 && ctx->integs.count("langevin_all") &&
+               ctx->integs["langevin_all"].method == 0 && al_mask(ctx) == 0 && pot_mask(ctx) != 0 && needs_nlist(ctx);
+  if (fused) {
+    if (int r = run_fused_langevin(ctx, ctx->integs["langevin_all"], steps, flags)) return r;
+    ctx->time_step += steps + 1;
+    return check_overflow(ctx);
+  }
+  // Pre-run force and torque evaluation (src/samos.cpp:872-875), added onto
+  // the current values as in the reference.
+  interact(ctx, pot_mask(ctx), al_mask(ctx), 1, nullptr, f32);
+  for (int t = 0; t <= steps; ++t) {
+    launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+    loop_head_forces(ctx, nullptr, f32);
+    for (auto& kv : ctx->integs)
+      if (int r = integrate_one(ctx, kv.second, t)) return r;
+    check_and_rebuild(ctx);
+    ctx->time_step++;
+  }
+  HIPCHK(hipGetLastError());
+  return check_overflow(ctx);
+}
+
+}  // extern "C"
+
+namespace {
+
+__global__ void k_observe(Fields f, int n, int what, double* col, long ld, unsigned long long* maxbits) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  switch (what) {
+    case 0: {
+      const double v = sqrt(f.fx[s] * f.fx[s] + f.fy[s] * f.fy[s] + f.fz[s] * f.fz[s]);
+      atomicMax(maxbits, (unsigned long long)__double_as_longlong(v));
+      break;
+    }
+    case 1: col[s] = f.fx[s] * f.fx[s] + f.fy[s] * f.fy[s] + f.fz[s] * f.fz[s]; break;
+    case 2: col[s] = 0.5 * f.mass[s] * (f.vx[s] * f.vx[s] + f.vy[s] * f.vy[s] + f.vz[s] * f.vz[s]); break;
+    case 3: col[s] = f.vx[s]; col[ld + s] = f.vy[s]; col[2 * ld + s] = f.vz[s]; break;
+    case 4: col[s] = sqrt(f.vx[s] * f.vx[s] + f.vy[s] * f.vy[s] + f.vz[s] * f.vz[s]); break;
+    case 5: col[s] = f.nx[s]; col[ld + s] = f.ny[s]; col[2 * ld + s] = f.nz[s]; break;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int samos_reduce(samos_ctx* ctx, int what, double* out) {
+  if (what < 0 || what > 5) return fail(ctx, "unknown observable");
+  const int n = ctx->n;
+  if (n == 0) { *out = 0.0; return 0; }
+  HIPCHK(hipMemsetAsync(ctx->d_counter, 0, sizeof(unsigned long long), ctx->st));
+  hipLaunchKernelGGL(k_observe, dim3(grid1(n)), dim3(kBlock), 0, ctx->st, fields(ctx), n, what, ctx->d_cols,
+                     (long)ctx->np, ctx->d_counter);
+  HIPCHK(hipGetLastError());
+  if (what == 0) {
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, ctx->d_counter, sizeof(bits), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    *out = v;
+    return 0;
+  }
+  const int nc = (what == 3 || what == 5) ? 3 : 1;
+  launch_reduce_sum(ctx->st, ctx->d_cols, nc, ctx->np, n, ctx->d_partial, ctx->d_out);
+  HIPCHK(hipMemcpyAsync(ctx->h_out, ctx->d_out, nc * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if (nc == 3) {
+    const double* v = ctx->h_out;
+    *out = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / n;
+  } else if (what == 4) {
+    *out = ctx->h_out[0] / n;
+  } else {
+    *out = ctx->h_out[0];
+  }
+  return 0;
+}
+
+int samos_profile(samos_ctx* ctx, int enable) {
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->prof = enable != 0;
+  for (auto& p : ctx->profs) p.second.used = 0;
+  return 0;
+}
+
+int samos_profile_read(samos_ctx* ctx, const char* cls, double* ms_total, long long* launches) {
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  *ms_total = 0.0;
+  *launches = 0;
+  auto it = ctx->profs.find(cls);
+  if (it == ctx->profs.end()) return 0;
+  double tot = 0.0;
+  for (size_t k = 0; k < it->second.used; ++k) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, it->second.ev[k].first, it->second.ev[k].second));
+    tot += ms;
+  }
+  *ms_total = tot;
+  *launches = (long long)it->second.used;
+  return 0;
+}
+
+int samos_sync(samos_ctx* ctx) {
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,564 @@
+// Integrator steps with the surface constraints fused in, FP64 throughout.
+//
+// Restates, per particle:
+//   Brownian         src/integrators/integrator_brownian.cpp:42-124
+//   Brownian pos     src/integrators/integrator_brownian_pos.cpp:37-84
+//   Brownian align   src/integrators/integrator_brownian_align.cpp:41-76
+//   Langevin BAOAB   src/integrators/integrator_langevin.cpp:54-118
+//   Langevin SPV     src/integrators/integrator_langevin.cpp:128-183
+//   Langevin BBK     src/integrators/integrator_langevin.cpp:193-253
+//   FIRE             src/integrators/integrator_fire.cpp:34-148 (per-particle parts)
+//   sphere           src/constraints/constraint_sphere.cpp:38-81
+//   plane            src/constraints/constraint_plane.cpp:36-160
+//   none             src/constraints/constraint_none.hpp:58-80
+//   rotate/project   src/constraints/constraint.cpp:96-173
+//   periodic wrap    src/system/system.cpp:1119-1155
+// The reference's per-particle virtual calls (enforce, project_torque,
+// rotate_director) become inline device functions.  The noise stream is the
+// counter-based Philox stream of samos_device.hpp, one independent draw block
+// per (particle id, integrator call).
+#include "kernels.hpp"
+
+namespace samos {
+
+namespace {
+
+struct PS {
+  double x, y, z, vx, vy, vz, nx, ny, nz;
+  int dix, diy, diz;
+};
+
+__device__ __forceinline__ void load_ps(const Fields& f, int s, PS& p) {
+  p.x = f.x[s]; p.y = f.y[s]; p.z = f.z[s];
+  p.vx = f.vx[s]; p.vy = f.vy[s]; p.vz = f.vz[s];
+  p.nx = f.nx[s]; p.ny = f.ny[s]; p.nz = f.nz[s];
+  p.dix = p.diy = p.diz = 0;
+}
+
+__device__ __forceinline__ void store_ps(const Fields& f, int s, const PS& p) {
+  f.x[s] = p.x; f.y[s] = p.y; f.z[s] = p.z;
+  f.vx[s] = p.vx; f.vy[s] = p.vy; f.vz[s] = p.vz;
+  f.nx[s] = p.nx; f.ny[s] = p.ny; f.nz[s] = p.nz;
+  if (p.dix) f.ix[s] += p.dix;
+  if (p.diy) f.iy[s] += p.diy;
+  if (p.diz) f.iz[s] += p.diz;
+}
+
+// System::enforce_periodic (src/system/system.cpp:1119-1155)
+__device__ __forceinline__ void enforce_periodic(const BoxD& b, PS& p) {
+  if (!b.periodic) return;
+  if (p.x <= b.xlo) { p.x += b.Lx; p.dix--; } else if (p.x > b.xhi) { p.x -= b.Lx; p.dix++; }
+  if (p.y <= b.ylo) { p.y += b.Ly; p.diy--; } else if (p.y > b.yhi) { p.y -= b.Ly; p.diy++; }
+  if (p.z <= b.zlo) { p.z += b.Lz; p.diz--; } else if (p.z > b.zhi) { p.z -= b.Lz; p.diz++; }
+}
+
+// Constraint::enforce of the one active constraint; returns true when the
+// plane constraint zeroed the particle's f_z.
+__device__ __forceinline__ bool enforce(const BoxD& b, const ConstraintD& c, PS& p) {
+  if (c.kind == 0) {
+    enforce_periodic(b, p);
+  } else if (c.kind == 1) {
+    const double xlo = -0.5 * b.Lx, xhi = 0.5 * b.Lx, ylo = -0.5 * b.Ly, yhi = 0.5 * b.Ly;
+    p.z = c.zpos; p.vz = 0.0;
+    if (b.periodic) enforce_periodic(b, p);
+    else if (!c.unlimited) {
+      if (p.x < xlo) { p.x = xlo; p.vx = -p.vx; } else if (p.x > xhi) { p.x = xhi; p.vx = -p.vx; }
+      if (p.y < ylo) { p.y = ylo; p.vy = -p.vy; } else if (p.y > yhi) { p.y = yhi; p.vy = -p.vy; }
+    }
+    p.nz = 0.0;
+    const double len_n = sqrt(p.nx * p.nx + p.ny * p.ny);
+    double inv_len = 1.0;
+    if (len_n != 0.0) inv_len = 1.0 / sqrt(p.nx * p.nx + p.ny * p.ny);
+    p.nx *= inv_len; p.ny *= inv_len;
+    return true;
+  } else if (c.kind == 2) {
+    const double R0 = c.r;
+    const double R = sqrt(p.x * p.x + p.y * p.y + p.z * p.z);
+    const double sc = R0 / R;
+    p.x *= sc; p.y *= sc; p.z *= sc;
+    const double Nx = p.x / R0, Ny = p.y / R0, Nz = p.z / R0;
+    const double vN = p.vx * Nx + p.vy * Ny + p.vz * Nz;
+    const double nN = p.nx * Nx + p.ny * Ny + p.nz * Nz;
+    p.vx -= vN * Nx; p.vy -= vN * Ny; p.vz -= vN * Nz;
+    p.nx -= nN * Nx; p.ny -= nN * Ny; p.nz -= nN * Nz;
+    const double inv_len = 1.0 / sqrt(p.nx * p.nx + p.ny * p.ny + p.nz * p.nz);
+    p.nx *= inv_len; p.ny *= inv_len; p.nz *= inv_len;
+  }
+  return false;
+}
+
+__device__ __forceinline__ double project_torque(const ConstraintD& c, const PS& p, double tx, double ty, double tz) {
+  if (c.kind == 1) return tz;
+  if (c.kind == 2) {
+    const double Nx = p.x / c.r, Ny = p.y / c.r, Nz = p.z / c.r;
+    return tx * Nx + ty * Ny + tz * Nz;
+  }
+  return 0.0;
+}
+
+__device__ __forceinline__ void rotate_director(const ConstraintD& c, PS& p, double phi) {
+  if (c.kind == 1) {
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    const double nx = cs * p.nx + (-sn) * p.ny;
+    const double ny = sn * p.nx + cs * p.ny;
+    const double len = sqrt(nx * nx + ny * ny);
+    p.nx = nx / len; p.ny = ny / len;
+  } else if (c.kind == 2) {
+    const double U = p.x / c.r, V = p.y / c.r, W = p.z / c.r;
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    const double un = U * p.nx + V * p.ny + W * p.nz;
+    const double nx = U * un * (1.0 - cs) + p.nx * cs + (-W * p.ny + V * p.nz) * sn;
+    const double ny = V * un * (1.0 - cs) + p.ny * cs + (W * p.nx - U * p.nz) * sn;
+    const double nz = W * un * (1.0 - cs) + p.nz * cs + (-V * p.nx + U * p.ny) * sn;
+    const double len = sqrt(nx * nx + ny * ny + nz * nz);
+    p.nx = nx / len; p.ny = ny / len; p.nz = nz / len;
+  }
+}
+
+__device__ __forceinline__ void rotate_velocity(const ConstraintD& c, PS& p, double phi) {
+  if (c.kind == 1) {
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    const double vx = cs * p.vx + (-sn) * p.vy;
+    const double vy = sn * p.vx + cs * p.vy;
+    p.vx = vx; p.vy = vy;
+  } else if (c.kind == 2) {
+    const double U = p.x / c.r, V = p.y / c.r, W = p.z / c.r;
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    const double uv = U * p.vx + V * p.vy + W * p.vz;
+    const double vx = U * uv * (1.0 - cs) + p.vx * cs + (-W * p.vy + V * p.vz) * sn;
+    const double vy = V * uv * (1.0 - cs) + p.vy * cs + (W * p.vx - U * p.vz) * sn;
+    const double vz = W * uv * (1.0 - cs) + p.vz * cs + (-V * p.vx + U * p.vy) * sn;
+    p.vx = vx; p.vy = vy; p.vz = vz;
+  }
+}
+
+__device__ __forceinline__ unsigned long long eff_call(const ClockD* clk, int slot, unsigned long long call) {
+  return clk ? clk->calls[slot] + call : call;
+}
+__device__ __forceinline__ int eff_step(const ClockD* clk, int t_off) { return clk ? clk->run_step + t_off : t_off; }
+
+__global__ void k_enforce(Fields f, BoxD b, ConstraintD c, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  PS p; load_ps(f, s, p);
+  if (enforce(b, c, p)) f.fz[s] = 0.0;
+  store_ps(f, s, p);
+}
+
+// which: 0 brownian, 1 brownian_pos, 2 brownian_align
+template <int WHICH>
+__global__ void k_brownian(Fields f, BoxD b, ConstraintD c, int n, double dt, double v0, double mu, double mur,
+                           double stoch, double flip_p, int nematic, int velocity, TempD temp, uint32_t seed,
+                           unsigned long long call0, const ClockD* clk, int slot, int t_off) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const unsigned long long call = eff_call(clk, slot, call0);
+  const uint32_t pid = (uint32_t)f.id[s];
+  const uint32_t tag = WHICH == 0 ? TAG_BROWNIAN : (WHICH == 1 ? TAG_BROWNIAN_POS : TAG_BROWNIAN_ALIGN);
+  PS p; load_ps(f, s, p);
+  bool fz0 = false;
+  double g_rot = 0.0, u_flip = 1.0;
+  if (WHICH != 1) gauss1_u1(seed, tag, pid, call, g_rot, u_flip);
+  if (WHICH != 1 && nematic && u_flip < flip_p) {
+    p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz;
+    if (WHICH == 0 && velocity) { p.vx = -p.vx; p.vy = -p.vy; p.vz = -p.vz; }
+  }
+  if (WHICH != 2) {
+    const double T = temp.at(eff_step(clk, t_off));
+    const double B = sqrt(2.0 * mu * T);
+    const double sqrt_dt = sqrt(dt);
+    const double fx = f.fx[s], fy = f.fy[s], fz = f.fz[s];
+    double fdx, fdy, fdz;
+    if (WHICH == 0) { fdx = v0 * p.nx + mu * fx; fdy = v0 * p.ny + mu * fy; fdz = v0 * p.nz + mu * fz; }
+    else { fdx = mu * fx; fdy = mu * fy; fdz = mu * fz; }
+    p.vx = fdx; p.vy = fdy; p.vz = fdz;
+    p.x += dt * fdx; p.y += dt * fdy; p.z += dt * fdz;
+    if (T > 0.0) {
+      double g[3]; gauss3(seed, tag, pid, call, g);
+      const double frx = B * g[0], fry = B * g[1], frz = B * g[2];
+      p.vx += frx; p.vy += fry; p.vz += frz;
+      p.x += sqrt_dt * frx; p.y += sqrt_dt * fry; p.z += sqrt_dt * frz;
+    }
+    fz0 = enforce(b, c, p);
+  }
+  if (WHICH != 1) {
+    const double om = mur * project_torque(c, p, f.tx[s], f.ty[s], f.tz[s]);
+    f.omega[s] = om;
+    const double dtheta = dt * om + stoch * g_rot;
+    rotate_director(c, p, dtheta);
+    if (WHICH == 0 && velocity) rotate_velocity(c, p, dtheta);
+  }
+  if (WHICH != 2) f.age[s] += dt;
+  if (fz0) f.fz[s] = 0.0;
+  store_ps(f, s, p);
+}
+
+// First per-particle pass of a Langevin step (before the internal force call).
+template <int METHOD>
+__global__ void k_langevin_a(Fields f, BoxD b, ConstraintD c, int n, double dt, double gamma, TempD temp,
+                             uint32_t seed, unsigned long long call0, const ClockD* clk, int slot, int t_off,
+                             const double* R) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  PS p; load_ps(f, s, p);
+  const double dt2 = 0.5 * dt;
+  const double m = f.mass[s];
+  bool fz0 = false;
+  if (METHOD == 0) {
+    const double T = temp.at(eff_step(clk, t_off));
+    const double B = sqrt(T * (1.0 - exp(-2.0 * gamma * dt)));
+    const double exp_dt = exp(-gamma * dt);
+    const double fact = dt2 / m;
+    p.vx += fact * f.fx[s]; p.vy += fact * f.fy[s]; p.vz += fact * f.fz[s];
+    p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+    fz0 |= enforce(b, c, p);
+    p.vx *= exp_dt; p.vy *= exp_dt; p.vz *= exp_dt;
+    if (B != 0.0) {
+      const double sf = B / sqrt(m);
+      double g[3]; gauss3(seed, TAG_LANGEVIN, (uint32_t)f.id[s], eff_call(clk, slot, call0), g);
+      p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
+    }
+    p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+    fz0 |= enforce(b, c, p);
+  } else if (METHOD == 1) {
+    p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+    fz0 |= enforce(b, c, p);
+  } else {
+    const double T = temp.at(eff_step(clk, t_off));
+    const double B = sqrt(2.0 * T * dt * gamma);
+    const double one_m_dt2 = 1.0 - gamma * dt2;
+    const double dm = dt2 / m;
+    p.vx = one_m_dt2 * p.vx + dm * f.fx[s];
+    p.vy = one_m_dt2 * p.vy + dm * f.fy[s];
+    p.vz = one_m_dt2 * p.vz + dm * f.fz[s];
+    if (B != 0.0) {
+      const double sf = 0.5 * B / sqrt(m);
+      const int id = f.id[s];
+      p.vx += sf * R[3 * id]; p.vy += sf * R[3 * id + 1]; p.vz += sf * R[3 * id + 2];
+    }
+    p.x += dt * p.vx; p.y += dt * p.vy; p.z += dt * p.vz;
+    fz0 |= enforce(b, c, p);
+  }
+  if (fz0) f.fz[s] = 0.0;
+  store_ps(f, s, p);
+}
+
+// Second per-particle pass (after the internal force call).
+template <int METHOD>
+__global__ void k_langevin_b(Fields f, int n, double dt, double gamma, TempD temp, uint32_t seed,
+                             unsigned long long call0, const ClockD* clk, int slot, int t_off, double* R, BoxD b,
+                             ConstraintD c) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const double dt2 = 0.5 * dt;
+  const double m = f.mass[s];
+  if (METHOD == 0) {
+    const double fact = dt2 / m;
+    f.vx[s] += fact * f.fx[s]; f.vy[s] += fact * f.fy[s]; f.vz[s] += fact * f.fz[s];
+    f.age[s] += dt;
+  } else if (METHOD == 1) {
+    const double T = temp.at(eff_step(clk, t_off));
+    const double zeta = sqrt(T * (1.0 - exp(-2.0 * gamma * dt)));
+    const double eta = (gamma == 0.0) ? 0.0 : (1.0 - exp(-dt * gamma)) / gamma;
+    const double exp_dt = exp(-dt * gamma);
+    PS p; load_ps(f, s, p);
+    const double fact = eta / m;
+    p.vx = exp_dt * p.vx + fact * f.fx[s];
+    p.vy = exp_dt * p.vy + fact * f.fy[s];
+    p.vz = exp_dt * p.vz + fact * f.fz[s];
+    if (zeta != 0.0) {
+      const double sf = zeta / sqrt(m);
+      double g[3]; gauss3(seed, TAG_LANGEVIN, (uint32_t)f.id[s], eff_call(clk, slot, call0), g);
+      p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
+    }
+    p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+    if (enforce(b, c, p)) f.fz[s] = 0.0;
+    store_ps(f, s, p);
+    f.age[s] += dt;
+  } else {
+    const double T = temp.at(eff_step(clk, t_off));
+    const double B = sqrt(2.0 * T * dt * gamma);
+    const double one_div = 1.0 / (1.0 + gamma * dt2);
+    const double dm = dt2 / m;
+    double vx = one_div * (f.vx[s] + dm * f.fx[s]);
+    double vy = one_div * (f.vy[s] + dm * f.fy[s]);
+    double vz = one_div * (f.vz[s] + dm * f.fz[s]);
+    if (B != 0.0) {
+      const double sf = 0.5 * B * one_div / sqrt(m);
+      const int id = f.id[s];
+      double g[3]; gauss3(seed, TAG_LANGEVIN, (uint32_t)id, eff_call(clk, slot, call0), g);
+      R[3 * id] = g[0]; R[3 * id + 1] = g[1]; R[3 * id + 2] = g[2];
+      vx += sf * g[0]; vy += sf * g[1]; vz += sf * g[2];
+      f.age[s] += dt;
+    }
+    f.vx[s] = vx; f.vy[s] = vy; f.vz[s] = vz;
+  }
+}
+
+__global__ void k_bbk_init(Fields f, int n, uint32_t seed, double* R) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int id = f.id[s];
+  double g[3]; gauss3(seed, TAG_BBK_INIT, (uint32_t)id, 0ULL, g);
+  R[3 * id] = g[0]; R[3 * id + 1] = g[1]; R[3 * id + 2] = g[2];
+}
+
+// FIRE first pass: half kick, drift, enforce, director half kick and rotation.
+__global__ void k_fire_a(Fields f, BoxD b, ConstraintD c, int n, double dt) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  PS p; load_ps(f, s, p);
+  const double dt_2 = 0.5 * dt;
+  p.vx += dt_2 * f.fx[s]; p.vy += dt_2 * f.fy[s]; p.vz += dt_2 * f.fz[s];
+  p.x += dt * p.vx; p.y += dt * p.vy; p.z += dt * p.vz;
+  const bool fz0 = enforce(b, c, p);
+  double om = f.omega[s] + dt_2 * project_torque(c, p, f.tx[s], f.ty[s], f.tz[s]);
+  f.omega[s] = om;
+  rotate_director(c, p, dt * om);
+  if (fz0) f.fz[s] = 0.0;
+  store_ps(f, s, p);
+}
+
+// FIRE second pass: half kick, per-slot P = v.F, |F|^2, |v|^2 (before the
+// enforce, as in the reference), enforce, director half kick.
+__global__ void k_fire_b(Fields f, BoxD b, ConstraintD c, int n, double dt, double* cols, long ld) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  PS p; load_ps(f, s, p);
+  const double dt_2 = 0.5 * dt;
+  const double fx = f.fx[s], fy = f.fy[s], fz = f.fz[s];
+  p.vx += dt_2 * fx; p.vy += dt_2 * fy; p.vz += dt_2 * fz;
+  cols[s] = p.vx * fx + p.vy * fy + p.vz * fz;
+  cols[ld + s] = fx * fx + fy * fy + fz * fz;
+  cols[2 * ld + s] = p.vx * p.vx + p.vy * p.vy + p.vz * p.vz;
+  const bool fz0 = enforce(b, c, p);
+  f.omega[s] += dt_2 * project_torque(c, p, f.tx[s], f.ty[s], f.tz[s]);
+  if (fz0) f.fz[s] = 0.0;
+  store_ps(f, s, p);
+}
+
+__global__ void k_fire_mix(Fields f, int n, double f1, double f2, int zero_v) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double vx = f1 * f.vx[s] + f2 * f.fx[s];
+  double vy = f1 * f.vy[s] + f2 * f.fy[s];
+  double vz = f1 * f.vz[s] + f2 * f.fz[s];
+  if (zero_v) { vx = 0.0; vy = 0.0; vz = 0.0; }
+  f.vx[s] = vx; f.vy[s] = vy; f.vz[s] = vz;
+}
+
+__global__ void k_clock_advance(ClockD* clk, int nsteps) {
+  if (threadIdx.x < 8) clk->calls[threadIdx.x] += (unsigned long long)nsteps;
+  if (threadIdx.x == 0) clk->run_step += nsteps;
+}
+
+// ---------------------------------------------------------------------------
+// Fused Langevin BAOAB step (pair forces + the whole per-particle update).
+// ---------------------------------------------------------------------------
+template <int POT, int PHASE, bool F32>
+__global__ void __launch_bounds__(kBlock) k_baoab_fused(Fields cur, Fields nxt, NListD nl, BoxD b, ConstraintD c,
+                                                        InteractD ip, int n, double dt, double gamma, TempD temp,
+                                                        uint32_t seed, const ClockD* clk, int slot, int t_off,
+                                                        int parity, int head_enforce) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && PHASE != 2) nl.flags[1 - parity] = 0;
+  bool need = false;
+  if (s < n) {
+    const double xi = cur.x[s], yi = cur.y[s], zi = cur.z[s];
+    const int ti = cur.type[s] - 1;
+    double ri = 0.0;
+    if (((POT & POT_SOFT) && ip.soft_radii) || ((POT & POT_LJ) && ip.lj_radii)) ri = cur.radius[s];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int kn = nl.nnb[s];
+    const int* nb = nl.nbr + s;
+    for (int k = 0; k < kn; ++k) {
+      const int t = nb[(long)k * nl.np];
+      double dx = cur.x[t] - xi, dy = cur.y[t] - yi, dz = cur.z[t] - zi;
+      min_image(b, dx, dy, dz);
+      const double r_sq = dx * dx + dy * dy + dz * dz;
+      const int tj = cur.type[t] - 1;
+      if (POT & POT_SOFT) {
+        const int tt = ti * ip.soft_nt + tj;
+        const double kk = ip.soft_tab[tt];
+        const double a = ip.soft_radii ? ri + cur.radius[t] : ip.soft_tab[ip.soft_nt * ip.soft_nt + tt];
+        const double r = sqrt(r_sq);
+        if (r < a) {
+          const double diff = a - r;
+          const double ff = (r > 0.0) ? kk * diff / r : kk * diff;
+          fx -= ff * dx; fy -= ff * dy; fz -= ff * dz;
+        }
+      }
+      if (POT & POT_LJ) {
+        const int tt = ti * ip.lj_nt + tj;
+        const int nt2 = ip.lj_nt * ip.lj_nt;
+        const double rc = ip.lj_tab[2 * nt2 + tt];
+        if (r_sq <= rc * rc) {
+          const double eps = ip.lj_tab[tt];
+          const double sig = ip.lj_radii ? ri + cur.radius[t] : ip.lj_tab[nt2 + tt];
+          if (F32) {
+            const float s2 = (float)(sig * sig) / (float)r_sq;
+            const float s6 = s2 * s2 * s2;
+            const float ff = 48.0f * (float)eps * s6 * (s6 - 0.5f) * s2;
+            fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy); fz -= (double)(ff * (float)dz);
+          } else {
+            const double inv_r_sq = (sig * sig) / r_sq;
+            const double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
+            const double ff = 48.0 * eps * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
+            fx -= ff * dx; fy -= ff * dy; fz -= ff * dz;
+          }
+        }
+      }
+    }
+    if (ip.has_sp) { fx += ip.sp_alpha * cur.nx[s]; fy += ip.sp_alpha * cur.ny[s]; fz += ip.sp_alpha * cur.nz[s]; }
+
+    const double dt2 = 0.5 * dt;
+    const double m = cur.mass[s];
+    const double fact = dt2 / m;
+    double vx = cur.vx[s], vy = cur.vy[s], vz = cur.vz[s];
+    if (PHASE >= 1) {  // closing B of step t-1
+      vx += fact * fx; vy += fact * fy; vz += fact * fz;
+      cur.age[s] += dt;
+    }
+    if (PHASE <= 1) {
+      const double T = temp.at(eff_step(clk, t_off));
+      const double B = sqrt(T * (1.0 - exp(-2.0 * gamma * dt)));
+      const double exp_dt = exp(-gamma * dt);
+      PS p;
+      p.x = xi; p.y = yi; p.z = zi; p.vx = vx; p.vy = vy; p.vz = vz;
+      p.nx = cur.nx[s]; p.ny = cur.ny[s]; p.nz = cur.nz[s];
+      p.dix = p.diy = p.diz = 0;
+      p.vx += fact * fx; p.vy += fact * fy; p.vz += fact * fz;
+      p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+      bool fz0 = enforce(b, c, p);
+      p.vx *= exp_dt; p.vy *= exp_dt; p.vz *= exp_dt;
+      if (B != 0.0) {
+        const double sf = B / sqrt(m);
+        double g[3]; gauss3(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], eff_call(clk, slot, 0ULL) + (unsigned long long)t_off, g);
+        p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
+      }
+      p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+      fz0 |= enforce(b, c, p);
+      if (head_enforce) fz0 |= enforce(b, c, p);
+      nxt.x[s] = p.x; nxt.y[s] = p.y; nxt.z[s] = p.z;
+      cur.vx[s] = p.vx; cur.vy[s] = p.vy; cur.vz[s] = p.vz;
+      cur.nx[s] = p.nx; cur.ny[s] = p.ny; cur.nz[s] = p.nz;
+      if (p.dix) cur.ix[s] += p.dix;
+      if (p.diy) cur.iy[s] += p.diy;
+      if (p.diz) cur.iz[s] += p.diz;
+      if (fz0) fz = 0.0;
+      double ddx = nl.ox[s] - p.x, ddy = nl.oy[s] - p.y, ddz = nl.oz[s] - p.z;
+      min_image(b, ddx, ddy, ddz);
+      need = !(ddx * ddx + ddy * ddy + ddz * ddz < nl.pad2q);
+    } else {
+      cur.vx[s] = vx; cur.vy[s] = vy; cur.vz[s] = vz;
+    }
+    cur.fx[s] = fx; cur.fy[s] = fy; cur.fz[s] = fz;
+  }
+  if (PHASE != 2) {
+    if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomicOr(&nl.flags[parity], 1);
+  }
+}
+
+template <int POT, int PHASE>
+void fused_dispatch(hipStream_t st, int fp32, Fields cur, Fields nxt, NListD nl, BoxD b, ConstraintD c, InteractD ip,
+                    int n, double dt, double gamma, TempD temp, uint32_t seed, const ClockD* clk, int slot, int t_off,
+                    int parity, int head) {
+  dim3 g(grid1(n)), blk(kBlock);
+  if (fp32)
+    hipLaunchKernelGGL((k_baoab_fused<POT, PHASE, true>), g, blk, 0, st, cur, nxt, nl, b, c, ip, n, dt, gamma, temp,
+                       seed, clk, slot, t_off, parity, head);
+  else
+    hipLaunchKernelGGL((k_baoab_fused<POT, PHASE, false>), g, blk, 0, st, cur, nxt, nl, b, c, ip, n, dt, gamma, temp,
+                       seed, clk, slot, t_off, parity, head);
+}
+
+template <int POT>
+void fused_phase(hipStream_t st, int phase, int fp32, Fields cur, Fields nxt, NListD nl, BoxD b, ConstraintD c,
+                 InteractD ip, int n, double dt, double gamma, TempD temp, uint32_t seed, const ClockD* clk, int slot,
+                 int t_off, int parity, int head) {
+  if (phase == 0) fused_dispatch<POT, 0>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head);
+  else if (phase == 1) fused_dispatch<POT, 1>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head);
+  else fused_dispatch<POT, 2>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head);
+}
+
+}  // namespace
+
+void launch_enforce(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n) {
+  if (n <= 0 || con.kind < 0) return;
+  hipLaunchKernelGGL(k_enforce, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n);
+}
+
+void launch_brownian(hipStream_t st, int which, Fields f, BoxD box, ConstraintD con, int n, double dt, double v0,
+                     double mu, double mur, double stoch, double flip_p, int nematic, int velocity, TempD temp,
+                     uint32_t seed, unsigned long long call, const ClockD* clock, int clock_slot, int t_off) {
+  dim3 g(grid1(n)), blk(kBlock);
+  if (which == 0)
+    hipLaunchKernelGGL(k_brownian<0>, g, blk, 0, st, f, box, con, n, dt, v0, mu, mur, stoch, flip_p, nematic, velocity,
+                       temp, seed, call, clock, clock_slot, t_off);
+  else if (which == 1)
+    hipLaunchKernelGGL(k_brownian<1>, g, blk, 0, st, f, box, con, n, dt, v0, mu, mur, stoch, flip_p, nematic, velocity,
+                       temp, seed, call, clock, clock_slot, t_off);
+  else
+    hipLaunchKernelGGL(k_brownian<2>, g, blk, 0, st, f, box, con, n, dt, v0, mu, mur, stoch, flip_p, nematic, velocity,
+                       temp, seed, call, clock, clock_slot, t_off);
+}
+
+void launch_langevin_a(hipStream_t st, int method, Fields f, BoxD box, ConstraintD con, int n, double dt, double gamma,
+                       TempD temp, uint32_t seed, unsigned long long call, const ClockD* clock, int clock_slot,
+                       int t_off, double* R) {
+  dim3 g(grid1(n)), blk(kBlock);
+  if (method == 0)
+    hipLaunchKernelGGL(k_langevin_a<0>, g, blk, 0, st, f, box, con, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R);
+  else if (method == 1)
+    hipLaunchKernelGGL(k_langevin_a<1>, g, blk, 0, st, f, box, con, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R);
+  else
+    hipLaunchKernelGGL(k_langevin_a<2>, g, blk, 0, st, f, box, con, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R);
+}
+
+void launch_langevin_b(hipStream_t st, int method, Fields f, int n, double dt, double gamma, TempD temp, uint32_t seed,
+                       unsigned long long call, const ClockD* clock, int clock_slot, int t_off, double* R, BoxD box,
+                       ConstraintD con) {
+  dim3 g(grid1(n)), blk(kBlock);
+  if (method == 0)
+    hipLaunchKernelGGL(k_langevin_b<0>, g, blk, 0, st, f, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R, box, con);
+  else if (method == 1)
+    hipLaunchKernelGGL(k_langevin_b<1>, g, blk, 0, st, f, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R, box, con);
+  else
+    hipLaunchKernelGGL(k_langevin_b<2>, g, blk, 0, st, f, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R, box, con);
+}
+
+void launch_bbk_init(hipStream_t st, Fields f, int n, uint32_t seed, double* R) {
+  hipLaunchKernelGGL(k_bbk_init, dim3(grid1(n)), dim3(kBlock), 0, st, f, n, seed, R);
+}
+
+void launch_fire_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt) {
+  hipLaunchKernelGGL(k_fire_a, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt);
+}
+
+void launch_fire_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double* cols, long ld) {
+  hipLaunchKernelGGL(k_fire_b, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, cols, ld);
+}
+
+void launch_fire_mix(hipStream_t st, Fields f, int n, double f1, double f2, int zero_v) {
+  hipLaunchKernelGGL(k_fire_mix, dim3(grid1(n)), dim3(kBlock), 0, st, f, n, f1, f2, zero_v);
+}
+
+void launch_clock_advance(hipStream_t st, ClockD* clock, int nsteps) {
+  hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, st, clock, nsteps);
+}
+
+void launch_baoab_fused(hipStream_t st, int pot_mask, int phase, Fields cur, Fields nxt, NListD nl, BoxD box,
+                        ConstraintD con, InteractD ip, int n, double dt, double gamma, TempD temp, uint32_t seed,
+                        const ClockD* clock, int clock_slot, int t_off, int parity, int head_enforce, int fp32) {
+  switch (pot_mask) {
+    case 1: fused_phase<1>(st, phase, fp32, cur, nxt, nl, box, con, ip, n, dt, gamma, temp, seed, clock, clock_slot, t_off, parity, head_enforce); break;
+    case 2: fused_phase<2>(st, phase, fp32, cur, nxt, nl, box, con, ip, n, dt, gamma, temp, seed, clock, clock_slot, t_off, parity, head_enforce); break;
+    default: fused_phase<3>(st, phase, fp32, cur, nxt, nl, box, con, ip, n, dt, gamma, temp, seed, clock, clock_slot, t_off, parity, head_enforce); break;
+  }
+}
+
+}  // namespace samos

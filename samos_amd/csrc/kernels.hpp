@@ -1,0 +1,79 @@
+// Host-side launchers for the device kernels.  Each launcher enqueues work on
+// the given stream and returns immediately; none of them allocates memory.
+#pragma once
+#include "samos_device.hpp"
+
+namespace samos {
+
+// ---- neighbour list (nlist.hip) -------------------------------------------
+// Rebuild stages.  `req` (may be null) points at a device flag: when it is
+// zero the stage returns at once, so a rebuild can sit in a captured step
+// sequence and cost only a dispatch when nothing moved far enough.
+void launch_bin(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req);
+void launch_scan(hipStream_t st, NListD nl, const int* req);
+void launch_scatter(hipStream_t st, NListD nl, int n, const int* req);
+void launch_cell_sort(hipStream_t st, NListD nl, const int* req);
+void launch_build(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req);
+void launch_build_nsq(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req);
+// Whole rebuild (bin, scan, scatter, sort, build) or the N^2 build.
+void launch_rebuild(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req);
+// Displacement test of every particle against the last build; sets *req_out to 1
+// when any particle moved by pad/2 or more.
+void launch_need_update(hipStream_t st, Fields f, NListD nl, BoxD box, int n, int* req_out);
+// Morton keys of each slot's cell, for the locality re-sort.
+void launch_morton_keys(hipStream_t st, Fields f, NListD nl, BoxD box, int n, unsigned long long* keys, int* slots);
+void launch_gather_d(hipStream_t st, double* dst, const double* src, const int* perm, int n);
+void launch_gather_i(hipStream_t st, int* dst, const int* src, const int* perm, int n);
+// Half pair export: for slot s, neighbours t with id[t] > id[s]; writes (id_lo, id_hi) rows.
+void launch_export_pairs(hipStream_t st, Fields f, NListD nl, int n, int* out, unsigned long long* counter, long cap);
+size_t radix_sort_temp_bytes(int n);
+void radix_sort_pairs(hipStream_t st, void* temp, size_t temp_bytes, const unsigned long long* kin,
+                      unsigned long long* kout, const int* vin, int* vout, int n);
+
+// ---- pair interactions (interact.hip) -------------------------------------
+// Forces and torques of every registered pair potential and aligner in one
+// pass over the neighbour list.  mode: 0 overwrite f and tau, 1 add to them.
+// Per-slot energies are written when e_out != nullptr: e_out[k * np + s] for
+// k = soft, lj, polar, nematic.
+void launch_interact(hipStream_t st, int pot_mask, int al_mask, int mode, Fields f, NListD nl, BoxD box,
+                     InteractD ip, int n, double* e_out);
+// Same with FP32 pair arithmetic (distances still formed in FP64, sums kept in FP64).
+void launch_interact_f32(hipStream_t st, int pot_mask, int al_mask, int mode, Fields f, NListD nl, BoxD box,
+                         InteractD ip, int n, double* e_out);
+
+// Deterministic two-pass sums of `ncols` columns of length n (column stride ld)
+// into out[ncols] (device).  partial must hold ncols * reduce_blocks(n) doubles.
+int reduce_blocks(int n);
+void launch_reduce_sum(hipStream_t st, const double* cols, int ncols, long ld, int n, double* partial, double* out);
+
+// ---- integrators and constraints (integrate.hip) --------------------------
+void launch_enforce(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n);
+void launch_brownian(hipStream_t st, int which, Fields f, BoxD box, ConstraintD con, int n, double dt,
+                     double v0, double mu, double mur, double stoch, double flip_p, int nematic, int velocity,
+                     TempD temp, uint32_t seed, unsigned long long call, const ClockD* clock, int clock_slot, int t_off);
+void launch_langevin_a(hipStream_t st, int method, Fields f, BoxD box, ConstraintD con, int n, double dt,
+                       double gamma, TempD temp, uint32_t seed, unsigned long long call, const ClockD* clock,
+                       int clock_slot, int t_off, double* R);
+void launch_langevin_b(hipStream_t st, int method, Fields f, int n, double dt, double gamma, TempD temp,
+                       uint32_t seed, unsigned long long call, const ClockD* clock, int clock_slot, int t_off,
+                       double* R, BoxD box, ConstraintD con);
+void launch_bbk_init(hipStream_t st, Fields f, int n, uint32_t seed, double* R);
+void launch_fire_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt);
+void launch_fire_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double* cols, long ld);
+void launch_fire_mix(hipStream_t st, Fields f, int n, double f1, double f2, int zero_v);
+void launch_clock_advance(hipStream_t st, ClockD* clock, int nsteps);
+
+// Fused Langevin BAOAB step over the neighbour list (one launch per step):
+//   phase 0: forces F(x_t), first half of step t (B, A, enforce, O, A, enforce)
+//   phase 1: forces F(x_t), closing B of step t-1, then phase-0 work for step t
+//   phase 2: forces F(x_t), closing B of step t-1 only
+// x_t must already carry step t's loop-head enforce; with head_enforce set the
+// kernel applies step t+1's loop-head enforce to the positions it writes.
+// Positions are read from `cur` and written to `nxt` (only x, y, z differ), so
+// neighbour reads never see a half-updated step.  Sets flags[parity] when any
+// particle passed pad/2 and clears flags[1 - parity].
+void launch_baoab_fused(hipStream_t st, int pot_mask, int phase, Fields cur, Fields nxt, NListD nl, BoxD box,
+                        ConstraintD con, InteractD ip, int n, double dt, double gamma, TempD temp, uint32_t seed,
+                        const ClockD* clock, int clock_slot, int t_off, int parity, int head_enforce, int fp32);
+
+}  // namespace samos

@@ -1,0 +1,165 @@
+// Device-side data model and shared helpers for the SAMoS particle pipeline on
+// CDNA4 (gfx950).  All particle state lives in HBM as structure-of-arrays FP64
+// columns in "slot" order: slots are a Morton (Z-order) ordering of the cell
+// grid, so particles that are close in space are close in memory.  The
+// original particle id of each slot is kept in `id`, so every export to the
+// host is in the reference's id order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace samos {
+
+constexpr int kBlock = 256;          // threads per workgroup for per-particle kernels
+constexpr int kMaxTypes = 16;        // particle types supported by the per-type tables
+
+// Box centred at the origin (src/system/box.hpp:56-62).
+struct BoxD {
+  double Lx, Ly, Lz, xlo, xhi, ylo, yhi, zlo, zhi;
+  int periodic;
+};
+
+// Single-wrap minimum image (src/system/system.cpp:1083-1095).
+__device__ __forceinline__ void min_image(const BoxD& b, double& dx, double& dy, double& dz) {
+  if (b.periodic) {
+    if (dx > b.xhi) dx -= b.Lx; else if (dx < b.xlo) dx += b.Lx;
+    if (dy > b.yhi) dy -= b.Ly; else if (dy < b.ylo) dy += b.Ly;
+    if (dz > b.zhi) dz -= b.Lz; else if (dz < b.zlo) dz += b.Lz;
+  }
+}
+
+// Particle columns (slot order).  Passed by value into every kernel.
+struct Fields {
+  double *x, *y, *z, *vx, *vy, *vz, *nx, *ny, *nz;
+  double *fx, *fy, *fz, *tx, *ty, *tz, *omega, *age, *radius, *mass;
+  int *type, *ix, *iy, *iz, *coord, *id;
+};
+constexpr int kNumDoubleFields = 19;
+constexpr int kNumIntFields = 6;
+
+// Neighbour-list state.  The list is a full (both directions) Verlet list in a
+// transposed layout: neighbour k of slot s is nbr[k * np + s], so that the
+// k-th loads of consecutive slots are coalesced.
+struct NListD {
+  int ncx, ncy, ncz, ncell;
+  double wx, wy, wz;
+  double cut2;        // (rcut + pad)^2
+  double pad2q;       // 0.25 * pad^2
+  int use_cell;
+  int np;             // slot-padded particle count (multiple of 64)
+  int stride;         // neighbour capacity per slot
+  int* cell_of;       // [n] linear cell index per slot
+  int* count;         // [ncell] members per cell; zero between builds
+  int* start;         // [ncell + 1] exclusive scan of count
+  int* members;       // [n] slots grouped by cell, ascending slot within a cell
+  int* bsum;          // [scan blocks] per-block sums for the scan
+  int* nbr;           // [stride * np]
+  int* nnb;           // [np]
+  double *ox, *oy, *oz;  // positions at the last build
+  int* flags;         // [0],[1] rebuild request by step parity, [2] overflow, [3] largest cell
+                      // count, [4] builds, [5] largest neighbour count
+};
+
+// Pair interactions evaluated in one pass over the neighbour list.
+struct InteractD {
+  // soft (src/potentials/pair/pair_soft_potential.cpp:33-119): table k, a
+  const double* soft_tab; int soft_nt; int soft_radii;
+  // lj (src/potentials/pair/pair_lj_potential.cpp:32-124): table eps, sigma, rcut
+  const double* lj_tab; int lj_nt; int lj_radii; int lj_shifted;
+  // polar (src/aligner/pair/pair_polar_aligner.cpp:32-107): table J, rcut
+  const double* pol_tab; int pol_nt;
+  // nematic (src/aligner/pair/pair_nematic_aligner.cpp:32-96): table J (already doubled
+  // when no per-type parameter was set), rcut
+  const double* nem_tab; int nem_nt;
+  // external self propulsion (src/potentials/external/external_self_propulsion.cpp:33-50)
+  int has_sp; double sp_alpha;
+};
+
+enum PotMask { POT_SOFT = 1, POT_LJ = 2 };
+enum AlMask { AL_POLAR = 1, AL_NEMATIC = 2 };
+
+// Constraint applied per particle (src/constraints/*).
+struct ConstraintD {
+  int kind;           // 0 none, 1 plane, 2 sphere, -1 unset
+  double r;           // sphere radius
+  double zpos;        // plane height
+  int unlimited;      // plane, non-periodic: no reflective walls
+};
+
+// Temperature schedule (src/system/value.hpp:115-170).
+struct TempD {
+  int kind; double tmin, tmax; int steps;
+  __device__ __host__ double at(int run_step) const {
+    if (kind == 0) return tmin;
+    if (run_step > steps) return tmax;
+    return tmin + (double)run_step * ((tmax - tmin) / (double)steps);
+  }
+};
+
+// Per-step clock in device memory, advanced after each captured chunk of steps
+// so that replayed graphs draw fresh noise.
+struct ClockD {
+  unsigned long long calls[8];   // per-integrator call counters
+  int run_step;                  // run-loop index t
+  int pad_;
+};
+
+// ---------------------------------------------------------------------------
+// Counter-based noise: Philox4x32-10 (Salmon et al., SC'11), keyed by
+// (seed, integrator tag) with counter (particle id, call lo, call hi, purpose).
+// ---------------------------------------------------------------------------
+struct U4 { uint32_t a, b, c, d; };
+
+__device__ __forceinline__ U4 philox4x32_10(U4 ctr, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) { k0 += W0; k1 += W1; }
+    uint32_t hi0 = __umulhi(M0, ctr.a), lo0 = M0 * ctr.a;
+    uint32_t hi1 = __umulhi(M1, ctr.c), lo1 = M1 * ctr.c;
+    U4 n;
+    n.a = hi1 ^ ctr.b ^ k0;
+    n.b = lo1;
+    n.c = hi0 ^ ctr.d ^ k1;
+    n.d = lo0;
+    ctr = n;
+  }
+  return ctr;
+}
+
+__device__ __forceinline__ double u01(uint32_t u) { return ((double)u + 0.5) * 2.3283064365386963e-10; }
+
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& g0, double& g1) {
+  double u1 = u01(a), u2 = u01(b);
+  double r = sqrt(-2.0 * log(u1));
+  double th = 6.283185307179586 * u2;
+  double s, c;
+  sincos(th, &s, &c);
+  g0 = r * c;
+  g1 = r * s;
+}
+
+enum { TAG_BROWNIAN = 1, TAG_LANGEVIN = 2, TAG_BROWNIAN_POS = 3, TAG_BROWNIAN_ALIGN = 4, TAG_BBK_INIT = 5 };
+
+__device__ __forceinline__ void gauss3(uint32_t seed, uint32_t tag, uint32_t pid, unsigned long long call, double g[3]) {
+  U4 c{pid, (uint32_t)call, (uint32_t)(call >> 32), 0u};
+  U4 r = philox4x32_10(c, seed, tag);
+  double g3;
+  box_muller(r.a, r.b, g[0], g[1]);
+  box_muller(r.c, r.d, g[2], g3);
+}
+
+__device__ __forceinline__ void gauss1_u1(uint32_t seed, uint32_t tag, uint32_t pid, unsigned long long call, double& g, double& u) {
+  U4 c{pid, (uint32_t)call, (uint32_t)(call >> 32), 1u};
+  U4 r = philox4x32_10(c, seed, tag);
+  double g1;
+  box_muller(r.a, r.b, g, g1);
+  u = u01(r.c) - 1.1641532182693481e-10;
+}
+
+inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
+// Grid size for one thread per slot; never zero, so empty systems launch validly.
+inline int grid1(long n) { int g = div_up(n, kBlock); return g < 1 ? 1 : g; }
+
+}  // namespace samos
