@@ -49,6 +49,7 @@ struct samos_ctx {
   double* dcol[kNumDoubleFields] = {};
   int* icol[kNumIntFields] = {};      // type, ix, iy, iz, coord, id
   double* alt[3] = {};                // second position buffer for fused steps
+  double2* xy[2] = {};                // interleaved (x, y) of the two position buffers
   std::vector<int> h_id;              // slot -> id
   double dt = 0.0;
   int ntypes = 1;
@@ -82,6 +83,9 @@ struct samos_ctx {
   unsigned long long* d_counter = nullptr;
   bool prof = false;
   std::map<std::string, ProfAcc> profs;
+  int coop = -1;                // cooperative rebuild: -1 untried, 1 works, 0 unavailable
+  bool mass_known = false;      // mass_u valid
+  double mass_u = -1.0;         // common particle mass, or -1
 };
 
 namespace {
@@ -108,6 +112,7 @@ Fields fields(samos_ctx* ctx, int parity = 0) {
   f.omega = d[15]; f.age = d[16]; f.radius = d[17]; f.mass = d[18];
   f.type = ctx->icol[0]; f.ix = ctx->icol[1]; f.iy = ctx->icol[2]; f.iz = ctx->icol[3];
   f.coord = ctx->icol[4]; f.id = ctx->icol[5];
+  f.xy = nullptr;
   return f;
 }
 
@@ -366,39 +371,77 @@ int upload_clock(samos_ctx* ctx) {
 }
 
 // One fused step t of the Langevin BAOAB schedule with positions in buffer
-// (t % 2), followed by the device-conditional rebuild on the new positions.
-void fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int t_off, int steps, bool f32) {
+// (t % 2), followed by the device-conditional rebuild on the new positions:
+// one cooperative launch when available (coop), else the five stage kernels.
+int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int t_off, int steps, bool f32, bool planar,
+               double mass_u, bool coop) {
   const int par = t & 1;
   Fields cur = fields(ctx, par), nxt = fields(ctx, par ^ 1);
+  if (planar) { cur.xy = ctx->xy[par]; nxt.xy = ctx->xy[par ^ 1]; }
   {
     ProfScope ps(ctx, "fused");
     launch_baoab_fused(ctx->st, pot_mask(ctx), phase, cur, nxt, ctx->nl, ctx->box, ctx->con, interact_params(ctx),
                        ctx->n, I.dt, I.gamma, I.temp, I.seed, ctx->d_clock, I.slot, t_off, par, t + 1 <= steps ? 1 : 0,
-                       f32 ? 1 : 0);
+                       f32 ? 1 : 0, planar ? 1 : 0, ctx->ntypes > 1 ? 1 : 0, mass_u);
   }
   if (phase != 2) {
     ProfScope ps(ctx, "rebuild");
-    launch_rebuild(ctx->st, nxt, ctx->nl, ctx->box, ctx->n, ctx->d_flags + par);
+    if (coop && ctx->coop != 0) {
+      hipError_t e = launch_rebuild_coop(ctx->st, nxt, ctx->nl, ctx->box, ctx->n, ctx->d_flags + par, planar ? 1 : 0);
+      if (e == hipSuccess) { ctx->coop = 1; return 0; }
+      if (ctx->coop == 1) { ctx->err = std::string("cooperative rebuild failed: ") + hipGetErrorString(e); return -2; }
+      (void)hipGetLastError();
+      ctx->coop = 0;  // not available here: stage kernels from now on
+    }
+    launch_rebuild(ctx->st, nxt, ctx->nl, ctx->box, ctx->n, ctx->d_flags + par, planar ? 1 : 0);
   }
+  return 0;
+}
+
+// The common mass when every particle has the same one, else -1.
+int uniform_mass(samos_ctx* ctx, double* out) {
+  if (!ctx->mass_known) {
+    std::vector<double> m(ctx->n);
+    if (ctx->n) HIPCHK(hipMemcpy(m.data(), ctx->dcol[18], sizeof(double) * ctx->n, hipMemcpyDeviceToHost));
+    ctx->mass_u = -1.0;
+    if (ctx->n && m[0] > 0.0 && std::all_of(m.begin(), m.end(), [&](double v) { return v == m[0]; })) ctx->mass_u = m[0];
+    ctx->mass_known = true;
+  }
+  *out = ctx->mass_u;
+  return 0;
 }
 
 int run_fused_langevin(samos_ctx* ctx, Integ& I, int steps, int flags) {
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
-  const bool graph = !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
+  double mass_u = -1.0;
+  if (int r = uniform_mass(ctx, &mass_u)) return r;
   if (int r = upload_clock(ctx)) return r;
   HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
+  // Planar layout: the plane constraint puts every particle on z = zpos, so
+  // neighbour gathers read the interleaved (x, y) column.
+  const bool planar = ctx->con.kind == 1;
   // loop head of t = 0
   launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
-  fused_step(ctx, I, 0, 0, 0, steps, f32);
+  if (planar) {
+    Fields f0 = fields(ctx, 0);
+    f0.xy = ctx->xy[0];
+    launch_pack_xy(ctx->st, f0, ctx->n);
+    // the binning of either buffer reads z (= zpos after the enforce)
+    HIPCHK(hipMemcpyAsync(ctx->alt[2], ctx->dcol[2], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
+  }
+  if (int r = fused_step(ctx, I, 0, 0, 0, steps, f32, planar, mass_u, true)) return r;
   launch_clock_advance(ctx->st, ctx->d_clock, 1);
   int t = 1;
+  // Without cooperative launches every skipped rebuild costs five dispatches;
+  // then chunks of steps are captured once and replayed as a graph.
   const int K = 64;  // steps per captured chunk (even, so buffer parity repeats)
+  const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
   if (graph && steps - t + 1 >= 2 * K) {
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < K; ++j) fused_step(ctx, I, 1, t + j, j, steps, f32);
+    for (int j = 0; j < K; ++j) fused_step(ctx, I, 1, t + j, j, steps, f32, planar, mass_u, false);
     launch_clock_advance(ctx->st, ctx->d_clock, K);
     HIPCHK(hipStreamEndCapture(ctx->st, &g));
     HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -407,14 +450,21 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int steps, int flags) {
       HIPCHK(hipGraphLaunch(ge, ctx->st));
       t += K;
     }
+    HIPCHK(hipStreamSynchronize(ctx->st));
     HIPCHK(hipGraphExecDestroy(ge));
     HIPCHK(hipGraphDestroy(g));
   }
   const int t0 = t;
-  for (; t <= steps; ++t) fused_step(ctx, I, 1, t, t - t0, steps, f32);
+  for (; t <= steps; ++t)
+    if (int r = fused_step(ctx, I, 1, t, t - t0, steps, f32, planar, mass_u, true)) return r;
   launch_clock_advance(ctx->st, ctx->d_clock, steps + 1 - t0);
-  fused_step(ctx, I, 2, steps + 1, 0, steps, f32);
-  if ((steps + 1) & 1) {
+  if (int r = fused_step(ctx, I, 2, steps + 1, 0, steps, f32, planar, mass_u, true)) return r;
+  const int pf = (steps + 1) & 1;  // buffer holding the final positions
+  if (planar) {
+    Fields f0 = fields(ctx, 0);
+    f0.xy = ctx->xy[pf];
+    launch_unpack_xy(ctx->st, f0, ctx->n);
+  } else if (pf) {
     for (int k = 0; k < 3; ++k)
       HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->alt[k], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
   }
@@ -462,6 +512,8 @@ int samos_create(samos_ctx** out, int device, int n, double lx, double ly, doubl
     if (hipMalloc(&ctx->icol[k], sizeof(int) * np) != hipSuccess) return bad("columns");
   for (int k = 0; k < 3; ++k)
     if (hipMalloc(&ctx->alt[k], sizeof(double) * np) != hipSuccess) return bad("columns");
+  for (int k = 0; k < 2; ++k)
+    if (hipMalloc(&ctx->xy[k], sizeof(double2) * np) != hipSuccess) return bad("columns");
   if (hipMalloc(&ctx->d_flags, 16 * sizeof(int)) != hipSuccess) return bad("flags");
   if (hipMalloc(&ctx->d_nnb, sizeof(int) * np) != hipSuccess) return bad("neighbour counts");
   hipMemsetAsync(ctx->d_nnb, 0, sizeof(int) * np, ctx->st);
@@ -514,6 +566,7 @@ int samos_destroy(samos_ctx* ctx) {
   for (int k = 0; k < kNumDoubleFields; ++k) hipFree(ctx->dcol[k]);
   for (int k = 0; k < kNumIntFields; ++k) hipFree(ctx->icol[k]);
   for (int k = 0; k < 3; ++k) hipFree(ctx->alt[k]);
+  for (int k = 0; k < 2; ++k) hipFree(ctx->xy[k]);
   hipFree(ctx->d_flags); hipFree(ctx->d_nnb); hipFree(ctx->d_tab); hipFree(ctx->d_clock); hipFree(ctx->d_en); hipFree(ctx->d_cols);
   hipFree(ctx->d_partial); hipFree(ctx->d_out); hipFree(ctx->d_counter);
   if (ctx->h_out) hipHostFree(ctx->h_out);
@@ -536,6 +589,7 @@ int samos_set_field(samos_ctx* ctx, int field, const void* host) {
     std::vector<double> tmp(n);
     for (int s = 0; s < n; ++s) tmp[s] = h[ctx->h_id[s]];
     HIPCHK(hipMemcpyAsync(ctx->dcol[field], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    if (field == SAMOS_MASS) ctx->mass_known = false;
     if (field <= 2) HIPCHK(hipMemcpyAsync(ctx->alt[field], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
   } else if (field >= SAMOS_TYPE && field < SAMOS_TYPE + 5) {
     const int* h = static_cast<const int*>(host);

@@ -18,6 +18,7 @@
 // counter-based Philox stream of samos_device.hpp, one independent draw block
 // per (particle id, integrator call).
 #include "kernels.hpp"
+#include "pair_sum.hpp"
 
 namespace samos {
 
@@ -359,66 +360,36 @@ __global__ void k_clock_advance(ClockD* clk, int nsteps) {
 // ---------------------------------------------------------------------------
 // Fused Langevin BAOAB step (pair forces + the whole per-particle update).
 // ---------------------------------------------------------------------------
-template <int POT, int PHASE, bool F32>
+template <int POT, int PHASE, bool F32, bool PLANAR, bool MT>
 __global__ void __launch_bounds__(kBlock) k_baoab_fused(Fields cur, Fields nxt, NListD nl, BoxD b, ConstraintD c,
                                                         InteractD ip, int n, double dt, double gamma, TempD temp,
                                                         uint32_t seed, const ClockD* clk, int slot, int t_off,
-                                                        int parity, int head_enforce) {
+                                                        int parity, int head_enforce, double mass_u) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0 && PHASE != 2) nl.flags[1 - parity] = 0;
   bool need = false;
   if (s < n) {
-    const double xi = cur.x[s], yi = cur.y[s], zi = cur.z[s];
-    const int ti = cur.type[s] - 1;
+    double xi, yi, zi;
+    if (PLANAR) {
+      const double2 p = cur.xy[s];
+      xi = p.x; yi = p.y; zi = c.zpos;
+    } else {
+      xi = cur.x[s]; yi = cur.y[s]; zi = cur.z[s];
+    }
+    const int ti = MT ? cur.type[s] - 1 : 0;
     double ri = 0.0;
     if (((POT & POT_SOFT) && ip.soft_radii) || ((POT & POT_LJ) && ip.lj_radii)) ri = cur.radius[s];
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    const int kn = nl.nnb[s];
-    const int* nb = nl.nbr + s;
-    for (int k = 0; k < kn; ++k) {
-      const int t = nb[(long)k * nl.np];
-      double dx = cur.x[t] - xi, dy = cur.y[t] - yi, dz = cur.z[t] - zi;
-      min_image(b, dx, dy, dz);
-      const double r_sq = dx * dx + dy * dy + dz * dz;
-      const int tj = cur.type[t] - 1;
-      if (POT & POT_SOFT) {
-        const int tt = ti * ip.soft_nt + tj;
-        const double kk = ip.soft_tab[tt];
-        const double a = ip.soft_radii ? ri + cur.radius[t] : ip.soft_tab[ip.soft_nt * ip.soft_nt + tt];
-        const double r = sqrt(r_sq);
-        if (r < a) {
-          const double diff = a - r;
-          const double ff = (r > 0.0) ? kk * diff / r : kk * diff;
-          fx -= ff * dx; fy -= ff * dy; fz -= ff * dz;
-        }
-      }
-      if (POT & POT_LJ) {
-        const int tt = ti * ip.lj_nt + tj;
-        const int nt2 = ip.lj_nt * ip.lj_nt;
-        const double rc = ip.lj_tab[2 * nt2 + tt];
-        if (r_sq <= rc * rc) {
-          const double eps = ip.lj_tab[tt];
-          const double sig = ip.lj_radii ? ri + cur.radius[t] : ip.lj_tab[nt2 + tt];
-          if (F32) {
-            const float s2 = (float)(sig * sig) / (float)r_sq;
-            const float s6 = s2 * s2 * s2;
-            const float ff = 48.0f * (float)eps * s6 * (s6 - 0.5f) * s2;
-            fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy); fz -= (double)(ff * (float)dz);
-          } else {
-            const double inv_r_sq = (sig * sig) / r_sq;
-            const double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
-            const double ff = 48.0 * eps * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
-            fx -= ff * dx; fy -= ff * dy; fz -= ff * dz;
-          }
-        }
-      }
-    }
+    pair_force_sum<POT, F32, PLANAR, MT>(cur, nl, b, ip, s, xi, yi, zi, ti, ri, fx, fy, fz);
     if (ip.has_sp) { fx += ip.sp_alpha * cur.nx[s]; fy += ip.sp_alpha * cur.ny[s]; fz += ip.sp_alpha * cur.nz[s]; }
 
+    // PLANAR keeps z = zpos, v_z = n_z = 0 in storage (every enforce resets
+    // them and planar pair forces have f_z = 0), so those columns are not
+    // streamed.
     const double dt2 = 0.5 * dt;
-    const double m = cur.mass[s];
+    const double m = mass_u > 0.0 ? mass_u : cur.mass[s];
     const double fact = dt2 / m;
-    double vx = cur.vx[s], vy = cur.vy[s], vz = cur.vz[s];
+    double vx = cur.vx[s], vy = cur.vy[s], vz = PLANAR ? 0.0 : cur.vz[s];
     if (PHASE >= 1) {  // closing B of step t-1
       vx += fact * fx; vy += fact * fy; vz += fact * fz;
       cur.age[s] += dt;
@@ -429,7 +400,7 @@ __global__ void __launch_bounds__(kBlock) k_baoab_fused(Fields cur, Fields nxt, 
       const double exp_dt = exp(-gamma * dt);
       PS p;
       p.x = xi; p.y = yi; p.z = zi; p.vx = vx; p.vy = vy; p.vz = vz;
-      p.nx = cur.nx[s]; p.ny = cur.ny[s]; p.nz = cur.nz[s];
+      p.nx = cur.nx[s]; p.ny = cur.ny[s]; p.nz = PLANAR ? 0.0 : cur.nz[s];
       p.dix = p.diy = p.diz = 0;
       p.vx += fact * fx; p.vy += fact * fy; p.vz += fact * fz;
       p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
@@ -443,46 +414,80 @@ __global__ void __launch_bounds__(kBlock) k_baoab_fused(Fields cur, Fields nxt, 
       p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
       fz0 |= enforce(b, c, p);
       if (head_enforce) fz0 |= enforce(b, c, p);
-      nxt.x[s] = p.x; nxt.y[s] = p.y; nxt.z[s] = p.z;
-      cur.vx[s] = p.vx; cur.vy[s] = p.vy; cur.vz[s] = p.vz;
-      cur.nx[s] = p.nx; cur.ny[s] = p.ny; cur.nz[s] = p.nz;
+      if (PLANAR) {
+        nxt.xy[s] = make_double2(p.x, p.y);
+      } else {
+        nxt.x[s] = p.x; nxt.y[s] = p.y; nxt.z[s] = p.z;
+      }
+      cur.vx[s] = p.vx; cur.vy[s] = p.vy;
+      if (!PLANAR) cur.vz[s] = p.vz;
+      cur.nx[s] = p.nx; cur.ny[s] = p.ny;
+      if (!PLANAR) cur.nz[s] = p.nz;
       if (p.dix) cur.ix[s] += p.dix;
       if (p.diy) cur.iy[s] += p.diy;
       if (p.diz) cur.iz[s] += p.diz;
-      if (fz0) fz = 0.0;
-      double ddx = nl.ox[s] - p.x, ddy = nl.oy[s] - p.y, ddz = nl.oz[s] - p.z;
+      double ddx = nl.ox[s] - p.x, ddy = nl.oy[s] - p.y, ddz = PLANAR ? 0.0 : nl.oz[s] - p.z;
       min_image(b, ddx, ddy, ddz);
       need = !(ddx * ddx + ddy * ddy + ddz * ddz < nl.pad2q);
     } else {
-      cur.vx[s] = vx; cur.vy[s] = vy; cur.vz[s] = vz;
+      // closing phase: forces F(x_t) are the state the reference leaves behind
+      cur.vx[s] = vx; cur.vy[s] = vy;
+      if (!PLANAR) cur.vz[s] = vz;
+      cur.fx[s] = fx; cur.fy[s] = fy; cur.fz[s] = fz;
     }
-    cur.fx[s] = fx; cur.fy[s] = fy; cur.fz[s] = fz;
   }
   if (PHASE != 2) {
     if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomicOr(&nl.flags[parity], 1);
   }
 }
 
-template <int POT, int PHASE>
+template <int POT, int PHASE, bool PLANAR, bool MT>
 void fused_dispatch(hipStream_t st, int fp32, Fields cur, Fields nxt, NListD nl, BoxD b, ConstraintD c, InteractD ip,
                     int n, double dt, double gamma, TempD temp, uint32_t seed, const ClockD* clk, int slot, int t_off,
-                    int parity, int head) {
+                    int parity, int head, double mass_u) {
   dim3 g(grid1(n)), blk(kBlock);
   if (fp32)
-    hipLaunchKernelGGL((k_baoab_fused<POT, PHASE, true>), g, blk, 0, st, cur, nxt, nl, b, c, ip, n, dt, gamma, temp,
-                       seed, clk, slot, t_off, parity, head);
+    hipLaunchKernelGGL((k_baoab_fused<POT, PHASE, true, PLANAR, MT>), g, blk, 0, st, cur, nxt, nl, b, c, ip, n, dt,
+                       gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
   else
-    hipLaunchKernelGGL((k_baoab_fused<POT, PHASE, false>), g, blk, 0, st, cur, nxt, nl, b, c, ip, n, dt, gamma, temp,
-                       seed, clk, slot, t_off, parity, head);
+    hipLaunchKernelGGL((k_baoab_fused<POT, PHASE, false, PLANAR, MT>), g, blk, 0, st, cur, nxt, nl, b, c, ip, n, dt,
+                       gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+}
+
+template <int POT, int PHASE>
+void fused_layout(hipStream_t st, int fp32, int planar, int mt, Fields cur, Fields nxt, NListD nl, BoxD b,
+                  ConstraintD c, InteractD ip, int n, double dt, double gamma, TempD temp, uint32_t seed,
+                  const ClockD* clk, int slot, int t_off, int parity, int head, double mass_u) {
+  if (planar) {
+    if (mt) fused_dispatch<POT, PHASE, true, true>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+    else fused_dispatch<POT, PHASE, true, false>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+  } else {
+    if (mt) fused_dispatch<POT, PHASE, false, true>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+    else fused_dispatch<POT, PHASE, false, false>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+  }
 }
 
 template <int POT>
-void fused_phase(hipStream_t st, int phase, int fp32, Fields cur, Fields nxt, NListD nl, BoxD b, ConstraintD c,
-                 InteractD ip, int n, double dt, double gamma, TempD temp, uint32_t seed, const ClockD* clk, int slot,
-                 int t_off, int parity, int head) {
-  if (phase == 0) fused_dispatch<POT, 0>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head);
-  else if (phase == 1) fused_dispatch<POT, 1>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head);
-  else fused_dispatch<POT, 2>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head);
+void fused_phase(hipStream_t st, int phase, int fp32, int planar, int mt, Fields cur, Fields nxt, NListD nl, BoxD b,
+                 ConstraintD c, InteractD ip, int n, double dt, double gamma, TempD temp, uint32_t seed,
+                 const ClockD* clk, int slot, int t_off, int parity, int head, double mass_u) {
+  if (phase == 0) fused_layout<POT, 0>(st, fp32, planar, mt, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+  else if (phase == 1) fused_layout<POT, 1>(st, fp32, planar, mt, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+  else fused_layout<POT, 2>(st, fp32, planar, mt, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+}
+
+__global__ void k_pack_xy(Fields f, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) f.xy[s] = make_double2(f.x[s], f.y[s]);
+}
+
+__global__ void k_unpack_xy(Fields f, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) {
+    const double2 p = f.xy[s];
+    f.x[s] = p.x;
+    f.y[s] = p.y;
+  }
 }
 
 }  // namespace
@@ -553,12 +558,21 @@ void launch_clock_advance(hipStream_t st, ClockD* clock, int nsteps) {
 
 void launch_baoab_fused(hipStream_t st, int pot_mask, int phase, Fields cur, Fields nxt, NListD nl, BoxD box,
                         ConstraintD con, InteractD ip, int n, double dt, double gamma, TempD temp, uint32_t seed,
-                        const ClockD* clock, int clock_slot, int t_off, int parity, int head_enforce, int fp32) {
+                        const ClockD* clock, int clock_slot, int t_off, int parity, int head_enforce, int fp32,
+                        int planar, int multitype, double mass_u) {
   switch (pot_mask) {
-    case 1: fused_phase<1>(st, phase, fp32, cur, nxt, nl, box, con, ip, n, dt, gamma, temp, seed, clock, clock_slot, t_off, parity, head_enforce); break;
-    case 2: fused_phase<2>(st, phase, fp32, cur, nxt, nl, box, con, ip, n, dt, gamma, temp, seed, clock, clock_slot, t_off, parity, head_enforce); break;
-    default: fused_phase<3>(st, phase, fp32, cur, nxt, nl, box, con, ip, n, dt, gamma, temp, seed, clock, clock_slot, t_off, parity, head_enforce); break;
+    case 1: fused_phase<1>(st, phase, fp32, planar, multitype, cur, nxt, nl, box, con, ip, n, dt, gamma, temp, seed, clock, clock_slot, t_off, parity, head_enforce, mass_u); break;
+    case 2: fused_phase<2>(st, phase, fp32, planar, multitype, cur, nxt, nl, box, con, ip, n, dt, gamma, temp, seed, clock, clock_slot, t_off, parity, head_enforce, mass_u); break;
+    default: fused_phase<3>(st, phase, fp32, planar, multitype, cur, nxt, nl, box, con, ip, n, dt, gamma, temp, seed, clock, clock_slot, t_off, parity, head_enforce, mass_u); break;
   }
+}
+
+void launch_pack_xy(hipStream_t st, Fields f, int n) {
+  hipLaunchKernelGGL(k_pack_xy, dim3(grid1(n)), dim3(kBlock), 0, st, f, n);
+}
+
+void launch_unpack_xy(hipStream_t st, Fields f, int n) {
+  hipLaunchKernelGGL(k_unpack_xy, dim3(grid1(n)), dim3(kBlock), 0, st, f, n);
 }
 
 }  // namespace samos

@@ -13,10 +13,11 @@ void launch_bin(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int*
 void launch_scan(hipStream_t st, NListD nl, const int* req);
 void launch_scatter(hipStream_t st, NListD nl, int n, const int* req);
 void launch_cell_sort(hipStream_t st, NListD nl, const int* req);
-void launch_build(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req);
+void launch_build(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req, int planar);
 void launch_build_nsq(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req);
-// Whole rebuild (bin, scan, scatter, sort, build) or the N^2 build.
-void launch_rebuild(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req);
+// Whole rebuild (bin, scan, scatter, sort, build) or the N^2 build.  planar:
+// read candidate positions from the interleaved (x, y) column.
+void launch_rebuild(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req, int planar = 0);
 // Displacement test of every particle against the last build; sets *req_out to 1
 // when any particle moved by pad/2 or more.
 void launch_need_update(hipStream_t st, Fields f, NListD nl, BoxD box, int n, int* req_out);
@@ -74,6 +75,15 @@ void launch_clock_advance(hipStream_t st, ClockD* clock, int nsteps);
 // particle passed pad/2 and clears flags[1 - parity].
 void launch_baoab_fused(hipStream_t st, int pot_mask, int phase, Fields cur, Fields nxt, NListD nl, BoxD box,
                         ConstraintD con, InteractD ip, int n, double dt, double gamma, TempD temp, uint32_t seed,
-                        const ClockD* clock, int clock_slot, int t_off, int parity, int head_enforce, int fp32);
+                        const ClockD* clock, int clock_slot, int t_off, int parity, int head_enforce, int fp32,
+                        int planar, int multitype, double mass_u);
+// The whole rebuild as one cooperative launch (grid barriers between the
+// phases); returns the launch status so the caller can fall back to
+// launch_rebuild where cooperative launches are unavailable.
+hipError_t launch_rebuild_coop(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req, int planar);
+// Interleaved (x, y) copy of the positions, read by the planar kernels.
+void launch_pack_xy(hipStream_t st, Fields f, int n);
+// Back from the interleaved column into the x, y columns.
+void launch_unpack_xy(hipStream_t st, Fields f, int n);
 
 }  // namespace samos

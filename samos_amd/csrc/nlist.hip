@@ -8,8 +8,20 @@
 // minimum image, the same 27-cell stencil (duplicates included when a grid side
 // has two cells) and the same truncating cell index.  This file is compiled
 // with -ffp-contract=off so d2 rounds exactly as the reference's expression.
+//
+// A rebuild has five grid-wide phases: bin (cell index + count), scan of the
+// counts, scatter into per-cell member ranges, per-cell ascending sort of the
+// members (determinism), and the list build.  They exist both as separate
+// kernels and fused into one cooperative kernel with grid barriers, which
+// returns at once when the rebuild flag is clear.
+//
+// PLANAR: positions come from the interleaved (x, y) column; all particles sit
+// on one plane, so dz = 0 and d2 is bit-identical to the 3-D expression.
 #include "kernels.hpp"
+#include <hip/hip_cooperative_groups.h>
 #include <hipcub/hipcub.hpp>
+
+namespace cg = cooperative_groups;
 
 namespace samos {
 
@@ -17,19 +29,27 @@ namespace {
 
 __device__ __forceinline__ bool req_off(const int* req) { return req != nullptr && *req == 0; }
 
+__device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ int gsize() { return gridDim.x * blockDim.x; }
+
 // CellList::populate wrap + get_cell_idx (src/system/cell_list.cpp:72-104).
 // The index is clamped to the grid: the reference indexes one past the grid for
 // a particle exactly on the high face.
-__global__ void k_bin(Fields f, NListD nl, BoxD b, int n, const int* req) {
-  if (req_off(req)) return;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    double x = f.x[s], y = f.y[s], z = f.z[s];
+template <bool PLANAR>
+__device__ void bin_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
+  for (int s = gtid(); s < n; s += gsize()) {
+    double x, y, z;
+    if (PLANAR) { const double2 p = f.xy[s]; x = p.x; y = p.y; z = f.z[s]; }
+    else { x = f.x[s]; y = f.y[s]; z = f.z[s]; }
     if (b.periodic) {
       bool w = false;
       if (x < b.xlo) { x += b.Lx; w = true; } else if (x > b.xhi) { x -= b.Lx; w = true; }
       if (y < b.ylo) { y += b.Ly; w = true; } else if (y > b.yhi) { y -= b.Ly; w = true; }
       if (z < b.zlo) { z += b.Lz; w = true; } else if (z > b.zhi) { z -= b.Lz; w = true; }
-      if (w) { f.x[s] = x; f.y[s] = y; f.z[s] = z; }
+      if (w) {
+        f.x[s] = x; f.y[s] = y; f.z[s] = z;
+        if (f.xy) f.xy[s] = make_double2(x, y);
+      }
     }
     int i = (int)((x - b.xlo) / nl.wx);
     int j = (int)((y - b.ylo) / nl.wy);
@@ -37,7 +57,7 @@ __global__ void k_bin(Fields f, NListD nl, BoxD b, int n, const int* req) {
     i = min(max(i, 0), nl.ncx - 1);
     j = min(max(j, 0), nl.ncy - 1);
     k = min(max(k, 0), nl.ncz - 1);
-    int c = nl.ncy * nl.ncz * i + nl.ncz * j + k;
+    const int c = nl.ncy * nl.ncz * i + nl.ncz * j + k;
     nl.cell_of[s] = c;
     atomicAdd(&nl.count[c], 1);
   }
@@ -47,8 +67,7 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-// Inclusive scan of one value per thread across the workgroup (LDS, Hillis-Steele
-// over wavefront sums).
+// Exclusive scan of one value per thread across the workgroup.
 __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& total) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int incl = v;
@@ -71,15 +90,14 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& total)
   return wave_off + incl - v;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(NListD nl, const int* req) {
-  if (req_off(req)) return;
-  __shared__ int lds[kScanThreads / 64];
-  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+// Tile-local exclusive scan of the counts; tile sums into bsum.  Whole-block.
+__device__ void scan_tile(NListD& nl, int tile, int* lds) {
+  const int base = tile * kScanTile + threadIdx.x * kScanItems;
   int v[kScanItems];
   int sum = 0;
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
-    int idx = base + q;
+    const int idx = base + q;
     v[q] = idx < nl.ncell ? nl.count[idx] : 0;
     sum += v[q];
   }
@@ -87,64 +105,49 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(NListD nl, const in
   int off = block_exclusive_scan(sum, lds, total);
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
-    int idx = base + q;
+    const int idx = base + q;
     if (idx < nl.ncell) nl.start[idx] = off;
     off += v[q];
   }
-  if (threadIdx.x == 0) nl.bsum[blockIdx.x] = total;
+  if (threadIdx.x == 0) nl.bsum[tile] = total;
 }
 
-__global__ void __launch_bounds__(1024) k_scan_bsum(NListD nl, int nb, const int* req) {
-  if (req_off(req)) return;
-  __shared__ int lds[16];
-  int carry = 0;
-  for (int base = 0; base < nb; base += 1024) {
-    int idx = base + threadIdx.x;
-    int v = idx < nb ? nl.bsum[idx] : 0;
-    int total;
-    int ex = block_exclusive_scan(v, lds, total);
-    if (idx < nb) nl.bsum[idx] = carry + ex;
-    carry += total;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) nl.start[nl.ncell] = carry;
-}
-
-__global__ void __launch_bounds__(kScanThreads) k_scan_add(NListD nl, const int* req) {
-  if (req_off(req)) return;
-  const int add = nl.bsum[blockIdx.x];
-  const int base = blockIdx.x * kScanTile;
-  for (int q = threadIdx.x; q < kScanTile; q += kScanThreads) {
-    int idx = base + q;
+// Adds the sum of all preceding tiles (a few hundred at most) to one tile.
+__device__ void scan_add_tile(NListD& nl, int tile, int ntiles, int* lds) {
+  int part = 0;
+  for (int i = threadIdx.x; i < tile; i += blockDim.x) part += nl.bsum[i];
+  int add;
+  block_exclusive_scan(part, lds, add);
+  const int base = tile * kScanTile;
+  for (int q = threadIdx.x; q < kScanTile; q += blockDim.x) {
+    const int idx = base + q;
     if (idx < nl.ncell) nl.start[idx] += add;
   }
+  if (tile == ntiles - 1 && threadIdx.x == 0) nl.start[nl.ncell] = add + nl.bsum[tile];
 }
 
 // Counts return to zero as members are placed, so the next build starts clean.
-__global__ void k_scatter(NListD nl, int n, const int* req) {
-  if (req_off(req)) return;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    int c = nl.cell_of[s];
-    int pos = nl.start[c] + atomicSub(&nl.count[c], 1) - 1;
+__device__ void scatter_phase(NListD& nl, int n) {
+  for (int s = gtid(); s < n; s += gsize()) {
+    const int c = nl.cell_of[s];
+    const int pos = nl.start[c] + atomicSub(&nl.count[c], 1) - 1;
     nl.members[pos] = s;
   }
 }
 
 // Ascending slot order inside each cell makes the list deterministic.
-__global__ void k_cell_sort(NListD nl, const int* req) {
-  if (req_off(req)) return;
+__device__ void sort_phase(NListD& nl) {
   int local_max = 0;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nl.ncell; c += gridDim.x * blockDim.x) {
-    int a = nl.start[c], e = nl.start[c + 1];
+  for (int c = gtid(); c < nl.ncell; c += gsize()) {
+    const int a = nl.start[c], e = nl.start[c + 1];
     local_max = max(local_max, e - a);
     for (int i = a + 1; i < e; ++i) {
-      int key = nl.members[i];
+      const int key = nl.members[i];
       int j = i - 1;
       while (j >= a && nl.members[j] > key) { nl.members[j + 1] = nl.members[j]; --j; }
       nl.members[j + 1] = key;
     }
   }
-  // wavefront max, one atomic per wavefront
   for (int o = 32; o > 0; o >>= 1) local_max = max(local_max, __shfl_xor(local_max, o, 64));
   if ((threadIdx.x & 63) == 0 && local_max > 0) atomicMax(&nl.flags[3], local_max);
 }
@@ -162,12 +165,18 @@ __device__ __forceinline__ void build_epilogue(NListD& nl, bool overflow, int km
 }
 
 // NeighbourList::build_cell (src/system/neighbour_list.cpp:216-273), full list.
-__global__ void k_build(Fields f, NListD nl, BoxD b, int n, const int* req) {
-  if (req_off(req)) return;
+// Cell members are consumed in batches so several candidate loads are in
+// flight per wavefront.
+template <bool PLANAR>
+__device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
   bool overflow = false;
   int kmax = 0;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    const double xi = f.x[s], yi = f.y[s], zi = f.z[s], ri = f.radius[s];
+  constexpr int U = 4;
+  for (int s = gtid(); s < n; s += gsize()) {
+    double xi, yi, zi;
+    if (PLANAR) { const double2 p = f.xy[s]; xi = p.x; yi = p.y; zi = f.z[s]; }
+    else { xi = f.x[s]; yi = f.y[s]; zi = f.z[s]; }
+    const double ri = f.radius[s];
     const int c = nl.cell_of[s];
     const int ci = c / (nl.ncy * nl.ncz), cj = (c / nl.ncz) % nl.ncy, ck = c % nl.ncz;
     int k = 0, coord = 0;
@@ -182,17 +191,30 @@ __global__ void k_build(Fields f, NListD nl, BoxD b, int n, const int* req) {
           if (iiz < 0) iiz = nl.ncz - 1; else if (iiz == nl.ncz) iiz = 0;
           const int cc = nl.ncy * nl.ncz * iix + nl.ncz * iiy + iiz;
           const int a = nl.start[cc], e = nl.start[cc + 1];
-          for (int m = a; m < e; ++m) {
-            const int t = nl.members[m];
-            if (t == s) continue;
-            double dx = xi - f.x[t], dy = yi - f.y[t], dz = zi - f.z[t];
-            min_image(b, dx, dy, dz);
-            const double d2 = dx * dx + dy * dy + dz * dz;
-            if (d2 < nl.cut2) {
-              if (k < nl.stride) nl.nbr[(long)k * nl.np + s] = t;
-              ++k;
-              const double r = ri + f.radius[t];
-              if (d2 < r * r) ++coord;
+          for (int m0 = a; m0 < e; m0 += U) {
+            int t[U];
+            double xj[U], yj[U], zj[U], rj[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) t[u] = (m0 + u < e) ? nl.members[m0 + u] : s;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              if (PLANAR) { const double2 p = f.xy[t[u]]; xj[u] = p.x; yj[u] = p.y; zj[u] = zi; }
+              else { xj[u] = f.x[t[u]]; yj[u] = f.y[t[u]]; zj[u] = f.z[t[u]]; }
+              rj[u] = f.radius[t[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              if (m0 + u >= e) break;
+              if (t[u] == s) continue;
+              double dx = xi - xj[u], dy = yi - yj[u], dz = zi - zj[u];
+              min_image(b, dx, dy, dz);
+              const double d2 = dx * dx + dy * dy + dz * dz;
+              if (d2 < nl.cut2) {
+                if (k < nl.stride) nl.nbr[(long)k * nl.np + s] = t[u];
+                ++k;
+                const double r = ri + rj[u];
+                if (d2 < r * r) ++coord;
+              }
             }
           }
         }
@@ -207,12 +229,69 @@ __global__ void k_build(Fields f, NListD nl, BoxD b, int n, const int* req) {
   build_epilogue(nl, overflow, kmax);
 }
 
+// ---- standalone stage kernels ---------------------------------------------
+template <bool PLANAR>
+__global__ void k_bin(Fields f, NListD nl, BoxD b, int n, const int* req) {
+  if (req_off(req)) return;
+  bin_phase<PLANAR>(f, nl, b, n);
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(NListD nl, const int* req) {
+  if (req_off(req)) return;
+  __shared__ int lds[kScanThreads / 64];
+  scan_tile(nl, blockIdx.x, lds);
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_add(NListD nl, int nb, const int* req) {
+  if (req_off(req)) return;
+  __shared__ int lds[kScanThreads / 64];
+  scan_add_tile(nl, blockIdx.x, nb, lds);
+}
+
+__global__ void k_scatter(NListD nl, int n, const int* req) {
+  if (req_off(req)) return;
+  scatter_phase(nl, n);
+}
+
+__global__ void k_cell_sort(NListD nl, const int* req) {
+  if (req_off(req)) return;
+  sort_phase(nl);
+}
+
+template <bool PLANAR>
+__global__ void k_build(Fields f, NListD nl, BoxD b, int n, const int* req) {
+  if (req_off(req)) return;
+  build_phase<PLANAR>(f, nl, b, n);
+}
+
+// ---- the whole rebuild in one cooperative launch ----------------------------
+// Every thread reads the same flag, so either the whole grid returns at once or
+// the whole grid reaches every barrier.
+template <bool PLANAR>
+__global__ void __launch_bounds__(kScanThreads) k_rebuild_coop(Fields f, NListD nl, BoxD b, int n, const int* req) {
+  if (req_off(req)) return;
+  cg::grid_group grid = cg::this_grid();
+  __shared__ int lds[kScanThreads / 64];
+  const int ntiles = (nl.ncell + kScanTile - 1) / kScanTile;
+  bin_phase<PLANAR>(f, nl, b, n);
+  grid.sync();
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) scan_tile(nl, tile, lds);
+  grid.sync();
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) scan_add_tile(nl, tile, ntiles, lds);
+  grid.sync();
+  scatter_phase(nl, n);
+  grid.sync();
+  sort_phase(nl);
+  grid.sync();
+  build_phase<PLANAR>(f, nl, b, n);
+}
+
 // NeighbourList::build_nsq (src/system/neighbour_list.cpp:178-213), full list.
 __global__ void k_build_nsq(Fields f, NListD nl, BoxD b, int n, const int* req) {
   if (req_off(req)) return;
   bool overflow = false;
   int kmax = 0;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+  for (int s = gtid(); s < n; s += gsize()) {
     const double xi = f.x[s], yi = f.y[s], zi = f.z[s], ri = f.radius[s];
     int k = 0, coord = 0;
     for (int t = 0; t < n; ++t) {
@@ -241,7 +320,7 @@ __global__ void k_build_nsq(Fields f, NListD nl, BoxD b, int n, const int* req) 
 // a particle move by pad/2 or more.
 __global__ void k_need_update(Fields f, NListD nl, BoxD b, int n, int* req_out) {
   bool need = false;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+  for (int s = gtid(); s < n; s += gsize()) {
     double dx = nl.ox[s] - f.x[s], dy = nl.oy[s] - f.y[s], dz = nl.oz[s] - f.z[s];
     min_image(b, dx, dy, dz);
     need |= !(dx * dx + dy * dy + dz * dz < nl.pad2q);
@@ -260,7 +339,7 @@ __device__ __forceinline__ unsigned long long spread3(unsigned long long v) {
 }
 
 __global__ void k_morton(Fields f, NListD nl, BoxD b, int n, unsigned long long* keys, int* slots) {
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+  for (int s = gtid(); s < n; s += gsize()) {
     int i = (int)((f.x[s] - b.xlo) / nl.wx);
     int j = (int)((f.y[s] - b.ylo) / nl.wy);
     int k = (int)((f.z[s] - b.zlo) / nl.wz);
@@ -273,14 +352,14 @@ __global__ void k_morton(Fields f, NListD nl, BoxD b, int n, unsigned long long*
 }
 
 __global__ void k_gather_d(double* dst, const double* src, const int* perm, int n) {
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) dst[s] = src[perm[s]];
+  for (int s = gtid(); s < n; s += gsize()) dst[s] = src[perm[s]];
 }
 __global__ void k_gather_i(int* dst, const int* src, const int* perm, int n) {
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) dst[s] = src[perm[s]];
+  for (int s = gtid(); s < n; s += gsize()) dst[s] = src[perm[s]];
 }
 
 __global__ void k_export_pairs(Fields f, NListD nl, int n, int* out, unsigned long long* counter, long cap) {
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+  for (int s = gtid(); s < n; s += gsize()) {
     const int is = f.id[s];
     const int kn = nl.nnb[s];
     for (int k = 0; k < kn; ++k) {
@@ -294,22 +373,25 @@ __global__ void k_export_pairs(Fields f, NListD nl, int n, int* out, unsigned lo
   }
 }
 
+// Grid-stride launches are capped at 1024 workgroups (4 per CU): the stages of
+// a device-conditional rebuild then cost one small dispatch when skipped.
 int grid_for(int n) {
   int g = div_up(n, kBlock);
-  return g < 1 ? 1 : (g > 8192 ? 8192 : g);
+  return g < 1 ? 1 : (g > 1024 ? 1024 : g);
 }
+
+int coop_grid_cache[2] = {-1, -1};
 
 }  // namespace
 
 void launch_bin(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req) {
-  hipLaunchKernelGGL(k_bin, dim3(grid_for(n)), dim3(kBlock), 0, st, f, nl, box, n, req);
+  hipLaunchKernelGGL(k_bin<false>, dim3(grid_for(n)), dim3(kBlock), 0, st, f, nl, box, n, req);
 }
 
 void launch_scan(hipStream_t st, NListD nl, const int* req) {
-  int nb = div_up(nl.ncell, kScanTile);
+  const int nb = div_up(nl.ncell, kScanTile);
   hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(kScanThreads), 0, st, nl, req);
-  hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(1024), 0, st, nl, nb, req);
-  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanThreads), 0, st, nl, req);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanThreads), 0, st, nl, nb, req);
 }
 
 void launch_scatter(hipStream_t st, NListD nl, int n, const int* req) {
@@ -320,21 +402,44 @@ void launch_cell_sort(hipStream_t st, NListD nl, const int* req) {
   hipLaunchKernelGGL(k_cell_sort, dim3(grid_for(nl.ncell)), dim3(kBlock), 0, st, nl, req);
 }
 
-void launch_build(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req) {
-  hipLaunchKernelGGL(k_build, dim3(grid1(n)), dim3(kBlock), 0, st, f, nl, box, n, req);
+void launch_build(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req, int planar) {
+  if (planar) hipLaunchKernelGGL(k_build<true>, dim3(grid_for(n)), dim3(kBlock), 0, st, f, nl, box, n, req);
+  else hipLaunchKernelGGL(k_build<false>, dim3(grid_for(n)), dim3(kBlock), 0, st, f, nl, box, n, req);
 }
 
 void launch_build_nsq(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req) {
-  hipLaunchKernelGGL(k_build_nsq, dim3(grid1(n)), dim3(kBlock), 0, st, f, nl, box, n, req);
+  hipLaunchKernelGGL(k_build_nsq, dim3(grid_for(n)), dim3(kBlock), 0, st, f, nl, box, n, req);
 }
 
-void launch_rebuild(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req) {
+void launch_rebuild(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req, int planar) {
   if (!nl.use_cell) { launch_build_nsq(st, f, nl, box, n, req); return; }
-  launch_bin(st, f, nl, box, n, req);
+  if (planar) hipLaunchKernelGGL(k_bin<true>, dim3(grid_for(n)), dim3(kBlock), 0, st, f, nl, box, n, req);
+  else launch_bin(st, f, nl, box, n, req);
   launch_scan(st, nl, req);
   launch_scatter(st, nl, n, req);
   launch_cell_sort(st, nl, req);
-  launch_build(st, f, nl, box, n, req);
+  launch_build(st, f, nl, box, n, req, planar);
+}
+
+hipError_t launch_rebuild_coop(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req, int planar) {
+  if (!nl.use_cell) { launch_build_nsq(st, f, nl, box, n, req); return hipGetLastError(); }
+  int& grid = coop_grid_cache[planar ? 1 : 0];
+  if (grid < 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+      e = planar ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rebuild_coop<true>, kScanThreads, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rebuild_coop<false>, kScanThreads, 0);
+    if (e != hipSuccess) return e;
+    grid = cus * std::min(per_cu, 4);
+    if (grid < 1) return hipErrorInvalidConfiguration;
+  }
+  void* args[] = {&f, &nl, &box, &n, &req};
+  return planar ? hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_rebuild_coop<true>), dim3(grid),
+                                             dim3(kScanThreads), args, 0, st)
+                : hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_rebuild_coop<false>), dim3(grid),
+                                             dim3(kScanThreads), args, 0, st);
 }
 
 void launch_need_update(hipStream_t st, Fields f, NListD nl, BoxD box, int n, int* req_out) {

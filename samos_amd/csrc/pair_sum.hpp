@@ -1,0 +1,97 @@
+// Pair-force sum of one slot over its neighbour list, shared by the force and
+// fused step kernels.  Neighbours are consumed in batches of kBatch: all
+// indices of a batch are loaded first, then all neighbour positions, then the
+// arithmetic, so each wavefront keeps kBatch independent gathers in flight
+// instead of one dependent chain per neighbour.
+//
+// Per-pair arithmetic restates src/potentials/pair/pair_soft_potential.cpp:66-109
+// and src/potentials/pair/pair_lj_potential.cpp:48-121 (LJ with sigma^2/r^2 in
+// the force factor, inclusive cutoff; soft with strict a_ij).
+//
+// PLANAR: every particle lies on the same plane z = const (plane constraint),
+// so neighbour positions come from the interleaved (x, y) column and dz = 0
+// exactly, which leaves r^2 bit-identical to the 3-D expression.
+// MT: more than one particle type is present (per-type table lookups).
+#pragma once
+#include "samos_device.hpp"
+
+namespace samos {
+
+constexpr int kBatch = 4;
+
+template <int POT, bool F32, bool PLANAR, bool MT>
+__device__ __forceinline__ void pair_force_sum(const Fields& f, const NListD& nl, const BoxD& b, const InteractD& ip,
+                                               int s, double xi, double yi, double zi, int ti, double ri,
+                                               double& fx, double& fy, double& fz) {
+  const bool radii = ((POT & POT_SOFT) && ip.soft_radii) || ((POT & POT_LJ) && ip.lj_radii);
+  const int kn = nl.nnb[s];
+  const int* nb = nl.nbr + s;
+  for (int k0 = 0; k0 < kn; k0 += kBatch) {
+    int t[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) t[u] = (k0 + u < kn) ? nb[(long)(k0 + u) * nl.np] : s;
+    double xj[kBatch], yj[kBatch], zj[kBatch], rj[kBatch];
+    int tj[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (PLANAR) {
+        const double2 p = f.xy[t[u]];
+        xj[u] = p.x; yj[u] = p.y; zj[u] = zi;
+      } else {
+        xj[u] = f.x[t[u]]; yj[u] = f.y[t[u]]; zj[u] = f.z[t[u]];
+      }
+      tj[u] = MT ? f.type[t[u]] - 1 : 0;
+      rj[u] = radii ? f.radius[t[u]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (k0 + u >= kn) break;
+      // d = x_j - x_i; LJ's reference convention x_i - x_j is -d
+      double dx = xj[u] - xi, dy = yj[u] - yi, dz = zj[u] - zi;
+      min_image(b, dx, dy, dz);
+      const double r_sq = dx * dx + dy * dy + dz * dz;
+      if (POT & POT_SOFT) {
+        const int tt = MT ? ti * ip.soft_nt + tj[u] : 0;
+        const double kk = ip.soft_tab[tt];
+        const double a = ip.soft_radii ? ri + rj[u] : ip.soft_tab[ip.soft_nt * ip.soft_nt + tt];
+        if (F32) {
+          const float r = sqrtf((float)r_sq);
+          if (r < (float)a) {
+            const float diff = (float)a - r;
+            const float ff = (r > 0.0f) ? (float)kk * diff / r : (float)kk * diff;
+            fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy); fz -= (double)(ff * (float)dz);
+          }
+        } else {
+          const double r = sqrt(r_sq);
+          if (r < a) {
+            const double diff = a - r;
+            const double ff = (r > 0.0) ? kk * diff / r : kk * diff;
+            fx -= ff * dx; fy -= ff * dy; fz -= ff * dz;
+          }
+        }
+      }
+      if (POT & POT_LJ) {
+        const int tt = MT ? ti * ip.lj_nt + tj[u] : 0;
+        const int nt2 = ip.lj_nt * ip.lj_nt;
+        const double rc = ip.lj_tab[2 * nt2 + tt];
+        if (r_sq <= rc * rc) {
+          const double eps = ip.lj_tab[tt];
+          const double sig = ip.lj_radii ? ri + rj[u] : ip.lj_tab[nt2 + tt];
+          if (F32) {
+            const float s2 = (float)(sig * sig) / (float)r_sq;
+            const float s6 = s2 * s2 * s2;
+            const float ff = 48.0f * (float)eps * s6 * (s6 - 0.5f) * s2;
+            fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy); fz -= (double)(ff * (float)dz);
+          } else {
+            const double inv_r_sq = (sig * sig) / r_sq;
+            const double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
+            const double ff = 48.0 * eps * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
+            fx -= ff * dx; fy -= ff * dy; fz -= ff * dz;
+          }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace samos

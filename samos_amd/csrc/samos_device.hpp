@@ -33,6 +33,7 @@ struct Fields {
   double *x, *y, *z, *vx, *vy, *vz, *nx, *ny, *nz;
   double *fx, *fy, *fz, *tx, *ty, *tz, *omega, *age, *radius, *mass;
   int *type, *ix, *iy, *iz, *coord, *id;
+  double2* xy;        // interleaved (x, y) for neighbour gathers on planar systems; may be null
 };
 constexpr int kNumDoubleFields = 19;
 constexpr int kNumIntFields = 6;
