@@ -129,6 +129,9 @@ int samos_integrator_langevin(samos_ctx* ctx, double gamma, int seed, int method
 /* IntegratorFIRE (integrator_fire.hpp:58-154). */
 int samos_integrator_fire(samos_ctx* ctx, double alpha, double alpha_scale, double f_tol, double e_tol,
                           double dt_inc, double dt_dec, double dt_max, int min_neg_steps);
+/* `disable <type> { group = all }` (src/samos.cpp, disable command): removes
+ * the integrator registered under type. */
+int samos_integrator_remove(samos_ctx* ctx, const char* type);
 /* FIRE state out[8]: dt, alpha, P, |F|, |v|, E, converged, last_neg. */
 int samos_fire_state(samos_ctx* ctx, double* out);
 
@@ -147,8 +150,19 @@ int samos_integrate(samos_ctx* ctx);
 /* The `run N` loop (src/samos.cpp:872-970): N+1 iterations of enforce, reset,
  * forces, torques, integrators, neighbour-list check.  flags: SAMOS_RUN_*. */
 int samos_run(samos_ctx* ctx, int steps, int flags);
-/* Scalar observables: what = 0 max|F|, 1 sum |F|^2, 2 kinetic energy
- * (sum m v^2 / 2), 3 |sum v| / N, 4 <|v|>, 5 |sum n| / N. */
+/* Iterations t_begin .. t_end-1 of a `run steps` (t_end <= steps + 1), so a
+ * driver can stop where the reference calls dumps and logs: on return the
+ * state is the loop-head state of iteration t_end (after enforce, reset and
+ * the force/torque evaluation, src/samos.cpp:887-898).  Ranges of one run
+ * must be called in order; samos_run(steps) is samos_run_range(0, steps+1).
+ * samos_run_range(0, 0, steps) performs only the loop head of iteration 0. */
+int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags);
+/* Energies of the current configuration without touching forces/torques
+ * (Potential/Aligner energy getters used by the pair-energy logs,
+ * src/log/log_pair_eng.hpp:56, src/log/log_pair_align.hpp:56). */
+int samos_energies(samos_ctx* ctx, double* energies);
+/* Scalar observables: what = 0 max|F|, 1 sum |F|^2, 2 kinetic energy as the
+ * reference logs it (sum v^2 / (2 m), src/log/log_kinetic_eng.hpp:58-64), 3 |sum v| / N, 4 <|v|>, 5 |sum n| / N. */
 int samos_reduce(samos_ctx* ctx, int what, double* out);
 
 /* ---- timing -------------------------------------------------------------- */

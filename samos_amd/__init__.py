@@ -79,6 +79,9 @@ def load_library(path: str = LIB_PATH):
         "samos_align_compute": (I, [P, I, dp]),
         "samos_integrate": (I, [P]),
         "samos_run": (I, [P, I, I]),
+        "samos_run_range": (I, [P, I, I, I, I]),
+        "samos_energies": (I, [P, dp]),
+        "samos_integrator_remove": (I, [P, ctypes.c_char_p]),
         "samos_reduce": (I, [P, I, dp]),
         "samos_profile": (I, [P, I]),
         "samos_profile_read": (I, [P, ctypes.c_char_p, dp, llp]),
@@ -254,6 +257,14 @@ class System:
 
     def run(self, steps, flags=0):
         self._chk(self.L.samos_run(self.h, int(steps), int(flags)))
+
+    def run_range(self, t_begin, t_end, steps, flags=0):
+        self._chk(self.L.samos_run_range(self.h, int(t_begin), int(t_end), int(steps), int(flags)))
+
+    def energies(self):
+        e = (ctypes.c_double * 4)()
+        self._chk(self.L.samos_energies(self.h, e))
+        return dict(zip(["soft", "lj", "polar", "nematic"], list(e)))
 
     def reduce(self, what):
         v = ctypes.c_double()

@@ -84,6 +84,7 @@ struct samos_ctx {
   bool prof = false;
   std::map<std::string, ProfAcc> profs;
   int coop = -1;                // cooperative rebuild: -1 untried, 1 works, 0 unavailable
+  bool run_begun = false;       // loop head of t = 0 of the current run done
   bool mass_known = false;      // mass_u valid
   double mass_u = -1.0;         // common particle mass, or -1
 };
@@ -362,10 +363,10 @@ int integrate_one(samos_ctx* ctx, Integ& I, int run_step) {
 }
 
 // ---- fused Langevin run ---------------------------------------------------
-int upload_clock(samos_ctx* ctx) {
+int upload_clock(samos_ctx* ctx, int run_step) {
   ClockD c{};
   for (auto& kv : ctx->integs) c.calls[kv.second.slot] = kv.second.calls;
-  c.run_step = 0;
+  c.run_step = run_step;
   HIPCHK(hipMemcpyAsync(ctx->d_clock, &c, sizeof(ClockD), hipMemcpyHostToDevice, ctx->st));
   return 0;
 }
@@ -373,9 +374,9 @@ int upload_clock(samos_ctx* ctx) {
 // One fused step t of the Langevin BAOAB schedule with positions in buffer
 // (t % 2), followed by the device-conditional rebuild on the new positions:
 // one cooperative launch when available (coop), else the five stage kernels.
-int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int t_off, int steps, bool f32, bool planar,
+int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int rel, int t_off, int steps, bool f32, bool planar,
                double mass_u, bool coop) {
-  const int par = t & 1;
+  const int par = rel & 1;
   Fields cur = fields(ctx, par), nxt = fields(ctx, par ^ 1);
   if (planar) { cur.xy = ctx->xy[par]; nxt.xy = ctx->xy[par ^ 1]; }
   {
@@ -411,18 +412,22 @@ int uniform_mass(samos_ctx* ctx, double* out) {
   return 0;
 }
 
-int run_fused_langevin(samos_ctx* ctx, Integ& I, int steps, int flags) {
+// Iterations a..b-1 of a run whose last index is `steps`, fused: each launch
+// computes F(x_t) once and applies the closing half-kick of step t-1 and the
+// first part of step t.  Positions of iteration t live in buffer (t - a) % 2;
+// x_a is in the primary columns on entry and x_b is back there on return,
+// with f = F(x_b): the state the reference has at the loop head of b.
+int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int flags) {
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   double mass_u = -1.0;
   if (int r = uniform_mass(ctx, &mass_u)) return r;
-  if (int r = upload_clock(ctx)) return r;
+  if (int r = upload_clock(ctx, a)) return r;
   HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
-  HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
   // Planar layout: the plane constraint puts every particle on z = zpos, so
   // neighbour gathers read the interleaved (x, y) column.
   const bool planar = ctx->con.kind == 1;
-  // loop head of t = 0
-  launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+  // the loop-head enforce of iteration a was applied by the run's begin or by
+  // the last kernel of the previous range
   if (planar) {
     Fields f0 = fields(ctx, 0);
     f0.xy = ctx->xy[0];
@@ -430,23 +435,23 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int steps, int flags) {
     // the binning of either buffer reads z (= zpos after the enforce)
     HIPCHK(hipMemcpyAsync(ctx->alt[2], ctx->dcol[2], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
   }
-  if (int r = fused_step(ctx, I, 0, 0, 0, steps, f32, planar, mass_u, true)) return r;
+  if (int r = fused_step(ctx, I, 0, a, 0, 0, steps, f32, planar, mass_u, true)) return r;
   launch_clock_advance(ctx->st, ctx->d_clock, 1);
-  int t = 1;
+  int t = a + 1;
   // Without cooperative launches every skipped rebuild costs five dispatches;
   // then chunks of steps are captured once and replayed as a graph.
   const int K = 64;  // steps per captured chunk (even, so buffer parity repeats)
   const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
-  if (graph && steps - t + 1 >= 2 * K) {
+  if (graph && b - t >= 2 * K) {
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < K; ++j) fused_step(ctx, I, 1, t + j, j, steps, f32, planar, mass_u, false);
+    for (int j = 0; j < K; ++j) fused_step(ctx, I, 1, t + j, t + j - a, j, steps, f32, planar, mass_u, false);
     launch_clock_advance(ctx->st, ctx->d_clock, K);
     HIPCHK(hipStreamEndCapture(ctx->st, &g));
     HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-    // every chunk must end before the last step, which has no following loop head
-    while (steps - t + 1 > K) {
+    // chunks stop before the range's last iteration, whose loop-head flag differs
+    while (b - t > K) {
       HIPCHK(hipGraphLaunch(ge, ctx->st));
       t += K;
     }
@@ -455,11 +460,11 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int steps, int flags) {
     HIPCHK(hipGraphDestroy(g));
   }
   const int t0 = t;
-  for (; t <= steps; ++t)
-    if (int r = fused_step(ctx, I, 1, t, t - t0, steps, f32, planar, mass_u, true)) return r;
-  launch_clock_advance(ctx->st, ctx->d_clock, steps + 1 - t0);
-  if (int r = fused_step(ctx, I, 2, steps + 1, 0, steps, f32, planar, mass_u, true)) return r;
-  const int pf = (steps + 1) & 1;  // buffer holding the final positions
+  for (; t < b; ++t)
+    if (int r = fused_step(ctx, I, 1, t, t - a, t - t0, steps, f32, planar, mass_u, true)) return r;
+  launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
+  if (int r = fused_step(ctx, I, 2, b, b - a, 0, steps, f32, planar, mass_u, true)) return r;
+  const int pf = (b - a) & 1;  // buffer holding x_b
   if (planar) {
     Fields f0 = fields(ctx, 0);
     f0.xy = ctx->xy[pf];
@@ -469,7 +474,7 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int steps, int flags) {
       HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->alt[k], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
   }
   HIPCHK(hipGetLastError());
-  I.calls += (unsigned long long)(steps + 1);
+  I.calls += (unsigned long long)(b - a);
   return 0;
 }
 
@@ -966,37 +971,68 @@ int samos_integrate(samos_ctx* ctx) {
   return 0;
 }
 
-int samos_run(samos_ctx* ctx, int steps, int flags) {
+int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags) {
   if (ctx->con.kind < 0)
     return fail(ctx, "Constraint has not been defined. Please define it using \"constraint\" command before running simulation.");
   if (ctx->integs.empty())
     return fail(ctx, "Integrator has not been defined. Please define it using \"integrator\" command before running simulation.");
-  if (steps < 0) return fail(ctx, "negative step count");
+  if (steps < 0 || t_begin < 0 || t_end > steps + 1 || t_begin > t_end || (t_begin == t_end && t_begin != 0))
+    return fail(ctx, "invalid run range");
+  if (t_begin > 0 && !ctx->run_begun) return fail(ctx, "run range does not continue a started run");
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
+  if (t_begin == 0 && !ctx->run_begun) {
+    // Loop head of t = 0 (src/samos.cpp:887-892): enforce, reset, forces and
+    // torques.  The pre-run evaluation before it (:872-875) is overwritten by
+    // this reset and leaves no trace, so it is not repeated here.
+    launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+    loop_head_forces(ctx, nullptr, f32);
+    ctx->run_begun = true;
+  }
+  if (t_end == t_begin) {
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+  }
+  if (t_end == steps + 1) ctx->run_begun = false;
   // Fused schedule: one Langevin BAOAB integrator, pair potentials only.  The
   // reference's loop-head force evaluation repeats the evaluation that ended
   // the previous step on the same positions, so it is done once.
-  bool fused = !(flags & SAMOS_RUN_FAITHFUL) && ctx->integs.size() == 1 && ctx->integs.count("langevin_all") &&
-               ctx->integs["langevin_all"].method == 0 && al_mask(ctx) == 0 && pot_mask(ctx) != 0 && needs_nlist(ctx);
+  const bool fused = !(flags & SAMOS_RUN_FAITHFUL) && ctx->integs.size() == 1 && ctx->integs.count("langevin_all") &&
+                     ctx->integs["langevin_all"].method == 0 && al_mask(ctx) == 0 && pot_mask(ctx) != 0 &&
+                     needs_nlist(ctx);
   if (fused) {
-    if (int r = run_fused_langevin(ctx, ctx->integs["langevin_all"], steps, flags)) return r;
-    ctx->time_step += steps + 1;
+    if (int r = run_fused_langevin(ctx, ctx->integs["langevin_all"], t_begin, t_end, steps, flags)) return r;
+    ctx->time_step += t_end - t_begin;
     return check_overflow(ctx);
   }
-  // Pre-run force and torque evaluation (src/samos.cpp:872-875), added onto
-  // the current values as in the reference.
-  interact(ctx, pot_mask(ctx), al_mask(ctx), 1, nullptr, f32);
-  for (int t = 0; t <= steps; ++t) {
-    launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
-    loop_head_forces(ctx, nullptr, f32);
+  for (int t = t_begin; t < t_end; ++t) {
     for (auto& kv : ctx->integs)
       if (int r = integrate_one(ctx, kv.second, t)) return r;
     check_and_rebuild(ctx);
     ctx->time_step++;
+    if (t + 1 <= steps) {  // loop head of t + 1: enforce, reset, forces, torques
+      launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+      loop_head_forces(ctx, nullptr, f32);
+    }
   }
   HIPCHK(hipGetLastError());
   return check_overflow(ctx);
+}
+
+int samos_integrator_remove(samos_ctx* ctx, const char* type) {
+  auto it = ctx->integs.find(std::string(type) + "_all");
+  if (it == ctx->integs.end()) return fail(ctx, std::string("Integrator ") + type + " (for group all) does not exist.");
+  hipFree(it->second.R);
+  ctx->integs.erase(it);
+  return 0;
+}
+
+int samos_run(samos_ctx* ctx, int steps, int flags) { return samos_run_range(ctx, 0, steps + 1, steps, flags); }
+
+int samos_energies(samos_ctx* ctx, double* energies) {
+  interact(ctx, pot_mask(ctx), al_mask(ctx), 2, ctx->d_en);
+  HIPCHK(hipGetLastError());
+  return energies_to_host(ctx, energies);
 }
 
 }  // extern "C"
@@ -1013,7 +1049,8 @@ __global__ void k_observe(Fields f, int n, int what, double* col, long ld, unsig
       break;
     }
     case 1: col[s] = f.fx[s] * f.fx[s] + f.fy[s] * f.fy[s] + f.fz[s] * f.fz[s]; break;
-    case 2: col[s] = 0.5 * f.mass[s] * (f.vx[s] * f.vx[s] + f.vy[s] * f.vy[s] + f.vz[s] * f.vz[s]); break;
+    // LogKineticEng sums 0.5 v^2 / m (src/log/log_kinetic_eng.hpp:58-64)
+    case 2: col[s] = 0.5 * (f.vx[s] * f.vx[s] + f.vy[s] * f.vy[s] + f.vz[s] * f.vz[s]) / f.mass[s]; break;
     case 3: col[s] = f.vx[s]; col[ld + s] = f.vy[s]; col[2 * ld + s] = f.vz[s]; break;
     case 4: col[s] = sqrt(f.vx[s] * f.vx[s] + f.vy[s] * f.vy[s] + f.vz[s] * f.vz[s]); break;
     case 5: col[s] = f.nx[s]; col[ld + s] = f.ny[s]; col[2 * ld + s] = f.nz[s]; break;

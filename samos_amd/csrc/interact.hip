@@ -131,11 +131,11 @@ __global__ void __launch_bounds__(kBlock) k_interact(int mode, Fields f, NListD 
   if (ip.has_sp) {
     fx += ip.sp_alpha * f.nx[s]; fy += ip.sp_alpha * f.ny[s]; fz += ip.sp_alpha * f.nz[s];
   }
-  if (POT || ip.has_sp) {
+  if ((POT || ip.has_sp) && mode != 2) {
     if (mode == 0) { f.fx[s] = fx; f.fy[s] = fy; f.fz[s] = fz; }
     else { f.fx[s] += fx; f.fy[s] += fy; f.fz[s] += fz; }
   }
-  if (AL) {
+  if (AL && mode != 2) {
     if (mode == 0) { f.tx[s] = tx; f.ty[s] = ty; f.tz[s] = tz; }
     else { f.tx[s] += tx; f.ty[s] += ty; f.tz[s] += tz; }
   }

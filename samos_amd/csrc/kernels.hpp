@@ -33,7 +33,8 @@ void radix_sort_pairs(hipStream_t st, void* temp, size_t temp_bytes, const unsig
 
 // ---- pair interactions (interact.hip) -------------------------------------
 // Forces and torques of every registered pair potential and aligner in one
-// pass over the neighbour list.  mode: 0 overwrite f and tau, 1 add to them.
+// pass over the neighbour list.  mode: 0 overwrite f and tau, 1 add to them,
+// 2 leave them untouched (energies only).
 // Per-slot energies are written when e_out != nullptr: e_out[k * np + s] for
 // k = soft, lj, polar, nematic.
 void launch_interact(hipStream_t st, int pot_mask, int al_mask, int mode, Fields f, NListD nl, BoxD box,

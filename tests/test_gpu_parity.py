@@ -370,6 +370,31 @@ def test_fp32_force_variant_error(gpu):
     assert sc.rel_force_err(forces(d32), forces(d64)) <= 1e-6
 
 
+@pytest.mark.parametrize("kind", ["langevin", "brownian"])
+def test_run_in_ranges_equals_one_run(gpu, kind):
+    """Stopping at observation steps (run_range) leaves the trajectory bit-identical."""
+    st = sc.lj_lattice(40, rho=0.7, seed=26) if kind == "langevin" else sc.plane_disks(1500, seed=26)
+    a, b = sc.load(gpu.System, st), sc.load(gpu.System, st)
+    for s in (a, b):
+        s.constraint("plane")
+        s.set_timestep(0.002 if kind == "langevin" else 0.01)
+        if kind == "langevin":
+            s.nlist(2.5, 0.3)
+            s.pair_lj(1.0, 1.0, 2.5, shifted=True)
+            s.integrator_langevin(gamma=1.0, seed=9, temp=(0, 1.0, 1.0, 1))
+        else:
+            s.nlist(2.4, 0.3)
+            s.pair_soft(1.0, 2.0)
+            s.align_polar(0.1, 2.4)
+            s.integrator_brownian(v0=0.1, nu=0.1, seed=9)
+    steps = 150
+    a.run(steps)
+    for lo, hi in [(0, 1), (1, 37), (37, 100), (100, 151)]:
+        b.run_range(lo, hi, steps)
+    for k in ["x", "y", "vx", "vy", "nx", "ny", "fx", "fy"]:
+        assert np.array_equal(a.get(k), b.get(k)), k
+
+
 def test_observables(gpu):
     st = sc.lj_lattice(32, seed=25)
     d = sc.load(gpu.System, st)
