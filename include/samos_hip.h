@@ -172,6 +172,11 @@ int samos_reduce(samos_ctx* ctx, int what, double* out);
 int samos_profile(samos_ctx* ctx, int enable);
 int samos_profile_read(samos_ctx* ctx, const char* kernel_class, double* ms_total, long long* launches);
 int samos_sync(samos_ctx* ctx);
+/* Tuning switches (no counterpart in the reference; results do not depend on
+ * them): "rebuild_coop" 1 = one cooperative launch per neighbour-list rebuild
+ * check, 0 = five stage kernels replayed in graphs (default); "gather_batch" 4 or 8 = neighbour
+ * gathers in flight per thread in the fused step. */
+int samos_set_option(samos_ctx* ctx, const char* name, double value);
 
 #ifdef __cplusplus
 }

@@ -86,6 +86,7 @@ def load_library(path: str = LIB_PATH):
         "samos_profile": (I, [P, I]),
         "samos_profile_read": (I, [P, ctypes.c_char_p, dp, llp]),
         "samos_sync": (I, [P]),
+        "samos_set_option": (I, [P, ctypes.c_char_p, D]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -273,6 +274,9 @@ class System:
 
     def sync(self):
         self._chk(self.L.samos_sync(self.h))
+
+    def set_option(self, name, value):
+        self._chk(self.L.samos_set_option(self.h, name.encode(), float(value)))
 
     def profile(self, enable=True):
         self._chk(self.L.samos_profile(self.h, int(bool(enable))))

@@ -83,10 +83,17 @@ struct samos_ctx {
   unsigned long long* d_counter = nullptr;
   bool prof = false;
   std::map<std::string, ProfAcc> profs;
-  int coop = -1;                // cooperative rebuild: -1 untried, 1 works, 0 unavailable
+  // Cooperative rebuild: -1 untried, 1 works, 0 off.  Off by default: the five
+  // stage kernels replayed from a captured graph measured faster on MI355X
+  // (skipped stages cost little inside a graph; the cooperative kernel runs
+  // its build phase at lower occupancy).
+  int coop = 0;
+  int gather_batch = 4;         // neighbour gathers in flight per thread in the fused step
   bool run_begun = false;       // loop head of t = 0 of the current run done
   bool mass_known = false;      // mass_u valid
   double mass_u = -1.0;         // common particle mass, or -1
+  bool age_uniform = false;     // fused range in progress with one common age
+  double age_u = 0.0;           // that age
 };
 
 namespace {
@@ -381,9 +388,18 @@ int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int rel, int t_off, i
   if (planar) { cur.xy = ctx->xy[par]; nxt.xy = ctx->xy[par ^ 1]; }
   {
     ProfScope ps(ctx, "fused");
+    FusedArgs fa;
+    fa.dt = I.dt; fa.gamma = I.gamma; fa.temp = I.temp; fa.seed = I.seed;
+    fa.clock = ctx->d_clock; fa.clock_slot = I.slot; fa.t_off = t_off; fa.parity = par;
+    fa.head_enforce = t + 1 <= steps ? 1 : 0;
+    fa.mass_u = mass_u;
+    // BAOAB O-step constants (src/integrators/integrator_langevin.cpp:57-58)
+    fa.exp_dt = std::exp(-I.gamma * I.dt);
+    fa.Bc = I.temp.kind == SAMOS_VALUE_CONSTANT ? std::sqrt(I.temp.tmin * (1.0 - std::exp(-2.0 * I.gamma * I.dt))) : -1.0;
+    fa.batch = ctx->gather_batch;
+    fa.age_uniform = ctx->age_uniform ? 1 : 0;
     launch_baoab_fused(ctx->st, pot_mask(ctx), phase, cur, nxt, ctx->nl, ctx->box, ctx->con, interact_params(ctx),
-                       ctx->n, I.dt, I.gamma, I.temp, I.seed, ctx->d_clock, I.slot, t_off, par, t + 1 <= steps ? 1 : 0,
-                       f32 ? 1 : 0, planar ? 1 : 0, ctx->ntypes > 1 ? 1 : 0, mass_u);
+                       ctx->n, fa, f32 ? 1 : 0, planar ? 1 : 0, ctx->ntypes > 1 ? 1 : 0);
   }
   if (phase != 2) {
     ProfScope ps(ctx, "rebuild");
@@ -421,6 +437,15 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   double mass_u = -1.0;
   if (int r = uniform_mass(ctx, &mass_u)) return r;
+  // Particle ages only grow by dt per step (src/integrators/integrator_langevin.cpp:115).
+  // When all ages are equal they stay equal, so the sum is kept once here and
+  // written back at the end of the range instead of streaming the column.
+  {
+    std::vector<double> age(ctx->n);
+    if (ctx->n) HIPCHK(hipMemcpy(age.data(), ctx->dcol[16], sizeof(double) * ctx->n, hipMemcpyDeviceToHost));
+    ctx->age_uniform = ctx->n > 0 && std::all_of(age.begin(), age.end(), [&](double v) { return v == age[0]; });
+    ctx->age_u = ctx->n ? age[0] : 0.0;
+  }
   if (int r = upload_clock(ctx, a)) return r;
   HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   // Planar layout: the plane constraint puts every particle on z = zpos, so
@@ -475,6 +500,14 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   }
   HIPCHK(hipGetLastError());
   I.calls += (unsigned long long)(b - a);
+  if (ctx->age_uniform) {
+    double age = ctx->age_u;
+    for (int k = a; k < b; ++k) age += I.dt;  // one closing half-kick per iteration
+    std::vector<double> col(ctx->n, age);
+    HIPCHK(hipMemcpyAsync(ctx->dcol[16], col.data(), sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    ctx->age_uniform = false;
+  }
   return 0;
 }
 
@@ -1120,6 +1153,19 @@ int samos_profile_read(samos_ctx* ctx, const char* cls, double* ms_total, long l
 int samos_sync(samos_ctx* ctx) {
   HIPCHK(hipStreamSynchronize(ctx->st));
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int samos_set_option(samos_ctx* ctx, const char* name, double value) {
+  const std::string k = name ? name : "";
+  if (k == "rebuild_coop") {
+    ctx->coop = value != 0.0 ? -1 : 0;
+  } else if (k == "gather_batch") {
+    if (value != 4.0 && value != 8.0) return fail(ctx, "gather_batch must be 4 or 8");
+    ctx->gather_batch = (int)value;
+  } else {
+    return fail(ctx, "unknown option " + k);
+  }
   return 0;
 }
 

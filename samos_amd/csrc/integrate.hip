@@ -360,11 +360,17 @@ __global__ void k_clock_advance(ClockD* clk, int nsteps) {
 // ---------------------------------------------------------------------------
 // Fused Langevin BAOAB step (pair forces + the whole per-particle update).
 // ---------------------------------------------------------------------------
-template <int POT, int PHASE, bool F32, bool PLANAR, bool MT>
+// exp_dt = exp(-gamma dt) and, for a constant temperature, Bc = sqrt(T (1 -
+// exp(-2 gamma dt))) come from the host (the same libm the reference uses);
+// Bc < 0 selects the per-step schedule value instead.
+template <int POT, int PHASE, bool F32, bool PLANAR, bool MT, int BATCH>
 __global__ void __launch_bounds__(kBlock) k_baoab_fused(Fields cur, Fields nxt, NListD nl, BoxD b, ConstraintD c,
-                                                        InteractD ip, int n, double dt, double gamma, TempD temp,
-                                                        uint32_t seed, const ClockD* clk, int slot, int t_off,
-                                                        int parity, int head_enforce, double mass_u) {
+                                                        InteractD ip, int n, FusedArgs fa) {
+  const double dt = fa.dt, gamma = fa.gamma, mass_u = fa.mass_u, exp_dt = fa.exp_dt, Bc = fa.Bc;
+  const TempD temp = fa.temp;
+  const uint32_t seed = fa.seed;
+  const ClockD* clk = fa.clock;
+  const int slot = fa.clock_slot, t_off = fa.t_off, parity = fa.parity, head_enforce = fa.head_enforce;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0 && PHASE != 2) nl.flags[1 - parity] = 0;
   bool need = false;
@@ -380,7 +386,7 @@ __global__ void __launch_bounds__(kBlock) k_baoab_fused(Fields cur, Fields nxt, 
     double ri = 0.0;
     if (((POT & POT_SOFT) && ip.soft_radii) || ((POT & POT_LJ) && ip.lj_radii)) ri = cur.radius[s];
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    pair_force_sum<POT, F32, PLANAR, MT>(cur, nl, b, ip, s, xi, yi, zi, ti, ri, fx, fy, fz);
+    pair_force_sum<POT, F32, PLANAR, MT, BATCH>(cur, nl, b, ip, s, xi, yi, zi, ti, ri, fx, fy, fz);
     if (ip.has_sp) { fx += ip.sp_alpha * cur.nx[s]; fy += ip.sp_alpha * cur.ny[s]; fz += ip.sp_alpha * cur.nz[s]; }
 
     // PLANAR keeps z = zpos, v_z = n_z = 0 in storage (every enforce resets
@@ -392,12 +398,14 @@ __global__ void __launch_bounds__(kBlock) k_baoab_fused(Fields cur, Fields nxt, 
     double vx = cur.vx[s], vy = cur.vy[s], vz = PLANAR ? 0.0 : cur.vz[s];
     if (PHASE >= 1) {  // closing B of step t-1
       vx += fact * fx; vy += fact * fy; vz += fact * fz;
-      cur.age[s] += dt;
+      if (!fa.age_uniform) cur.age[s] += dt;
     }
     if (PHASE <= 1) {
-      const double T = temp.at(eff_step(clk, t_off));
-      const double B = sqrt(T * (1.0 - exp(-2.0 * gamma * dt)));
-      const double exp_dt = exp(-gamma * dt);
+      double B = Bc;
+      if (B < 0.0) {
+        const double T = temp.at(eff_step(clk, t_off));
+        B = sqrt(T * (1.0 - exp(-2.0 * gamma * dt)));
+      }
       PS p;
       p.x = xi; p.y = yi; p.z = zi; p.vx = vx; p.vy = vy; p.vz = vz;
       p.nx = cur.nx[s]; p.ny = cur.ny[s]; p.nz = PLANAR ? 0.0 : cur.nz[s];
@@ -441,39 +449,37 @@ __global__ void __launch_bounds__(kBlock) k_baoab_fused(Fields cur, Fields nxt, 
   }
 }
 
-template <int POT, int PHASE, bool PLANAR, bool MT>
+template <int POT, int PHASE, bool PLANAR, bool MT, int BATCH>
 void fused_dispatch(hipStream_t st, int fp32, Fields cur, Fields nxt, NListD nl, BoxD b, ConstraintD c, InteractD ip,
-                    int n, double dt, double gamma, TempD temp, uint32_t seed, const ClockD* clk, int slot, int t_off,
-                    int parity, int head, double mass_u) {
+                    int n, const FusedArgs& fa) {
   dim3 g(grid1(n)), blk(kBlock);
   if (fp32)
-    hipLaunchKernelGGL((k_baoab_fused<POT, PHASE, true, PLANAR, MT>), g, blk, 0, st, cur, nxt, nl, b, c, ip, n, dt,
-                       gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+    hipLaunchKernelGGL((k_baoab_fused<POT, PHASE, true, PLANAR, MT, BATCH>), g, blk, 0, st, cur, nxt, nl, b, c, ip, n, fa);
   else
-    hipLaunchKernelGGL((k_baoab_fused<POT, PHASE, false, PLANAR, MT>), g, blk, 0, st, cur, nxt, nl, b, c, ip, n, dt,
-                       gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+    hipLaunchKernelGGL((k_baoab_fused<POT, PHASE, false, PLANAR, MT, BATCH>), g, blk, 0, st, cur, nxt, nl, b, c, ip, n, fa);
 }
 
+// The gather batch is a tuning choice for the planar single-type layout (the
+// benchmarked one); the other layouts use batches of 4.
 template <int POT, int PHASE>
 void fused_layout(hipStream_t st, int fp32, int planar, int mt, Fields cur, Fields nxt, NListD nl, BoxD b,
-                  ConstraintD c, InteractD ip, int n, double dt, double gamma, TempD temp, uint32_t seed,
-                  const ClockD* clk, int slot, int t_off, int parity, int head, double mass_u) {
+                  ConstraintD c, InteractD ip, int n, const FusedArgs& fa) {
   if (planar) {
-    if (mt) fused_dispatch<POT, PHASE, true, true>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
-    else fused_dispatch<POT, PHASE, true, false>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+    if (mt) fused_dispatch<POT, PHASE, true, true, 4>(st, fp32, cur, nxt, nl, b, c, ip, n, fa);
+    else if (fa.batch == 8) fused_dispatch<POT, PHASE, true, false, 8>(st, fp32, cur, nxt, nl, b, c, ip, n, fa);
+    else fused_dispatch<POT, PHASE, true, false, 4>(st, fp32, cur, nxt, nl, b, c, ip, n, fa);
   } else {
-    if (mt) fused_dispatch<POT, PHASE, false, true>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
-    else fused_dispatch<POT, PHASE, false, false>(st, fp32, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+    if (mt) fused_dispatch<POT, PHASE, false, true, 4>(st, fp32, cur, nxt, nl, b, c, ip, n, fa);
+    else fused_dispatch<POT, PHASE, false, false, 4>(st, fp32, cur, nxt, nl, b, c, ip, n, fa);
   }
 }
 
 template <int POT>
 void fused_phase(hipStream_t st, int phase, int fp32, int planar, int mt, Fields cur, Fields nxt, NListD nl, BoxD b,
-                 ConstraintD c, InteractD ip, int n, double dt, double gamma, TempD temp, uint32_t seed,
-                 const ClockD* clk, int slot, int t_off, int parity, int head, double mass_u) {
-  if (phase == 0) fused_layout<POT, 0>(st, fp32, planar, mt, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
-  else if (phase == 1) fused_layout<POT, 1>(st, fp32, planar, mt, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
-  else fused_layout<POT, 2>(st, fp32, planar, mt, cur, nxt, nl, b, c, ip, n, dt, gamma, temp, seed, clk, slot, t_off, parity, head, mass_u);
+                 ConstraintD c, InteractD ip, int n, const FusedArgs& fa) {
+  if (phase == 0) fused_layout<POT, 0>(st, fp32, planar, mt, cur, nxt, nl, b, c, ip, n, fa);
+  else if (phase == 1) fused_layout<POT, 1>(st, fp32, planar, mt, cur, nxt, nl, b, c, ip, n, fa);
+  else fused_layout<POT, 2>(st, fp32, planar, mt, cur, nxt, nl, b, c, ip, n, fa);
 }
 
 __global__ void k_pack_xy(Fields f, int n) {
@@ -557,13 +563,12 @@ void launch_clock_advance(hipStream_t st, ClockD* clock, int nsteps) {
 }
 
 void launch_baoab_fused(hipStream_t st, int pot_mask, int phase, Fields cur, Fields nxt, NListD nl, BoxD box,
-                        ConstraintD con, InteractD ip, int n, double dt, double gamma, TempD temp, uint32_t seed,
-                        const ClockD* clock, int clock_slot, int t_off, int parity, int head_enforce, int fp32,
-                        int planar, int multitype, double mass_u) {
+                        ConstraintD con, InteractD ip, int n, const FusedArgs& fa, int fp32, int planar,
+                        int multitype) {
   switch (pot_mask) {
-    case 1: fused_phase<1>(st, phase, fp32, planar, multitype, cur, nxt, nl, box, con, ip, n, dt, gamma, temp, seed, clock, clock_slot, t_off, parity, head_enforce, mass_u); break;
-    case 2: fused_phase<2>(st, phase, fp32, planar, multitype, cur, nxt, nl, box, con, ip, n, dt, gamma, temp, seed, clock, clock_slot, t_off, parity, head_enforce, mass_u); break;
-    default: fused_phase<3>(st, phase, fp32, planar, multitype, cur, nxt, nl, box, con, ip, n, dt, gamma, temp, seed, clock, clock_slot, t_off, parity, head_enforce, mass_u); break;
+    case 1: fused_phase<1>(st, phase, fp32, planar, multitype, cur, nxt, nl, box, con, ip, n, fa); break;
+    case 2: fused_phase<2>(st, phase, fp32, planar, multitype, cur, nxt, nl, box, con, ip, n, fa); break;
+    default: fused_phase<3>(st, phase, fp32, planar, multitype, cur, nxt, nl, box, con, ip, n, fa); break;
   }
 }
 

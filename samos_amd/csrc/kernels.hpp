@@ -74,10 +74,21 @@ void launch_clock_advance(hipStream_t st, ClockD* clock, int nsteps);
 // Positions are read from `cur` and written to `nxt` (only x, y, z differ), so
 // neighbour reads never see a half-updated step.  Sets flags[parity] when any
 // particle passed pad/2 and clears flags[1 - parity].
+struct FusedArgs {
+  double dt, gamma;
+  TempD temp;
+  uint32_t seed;
+  const ClockD* clock;
+  int clock_slot, t_off, parity, head_enforce;
+  double mass_u;   // common mass, or -1 to read each particle's
+  double exp_dt;   // exp(-gamma dt)
+  double Bc;       // O-step noise amplitude at constant temperature, or -1
+  int batch;       // neighbour gathers in flight per thread (4 or 8)
+  int age_uniform; // every particle has the same age: the host advances it once
+};
 void launch_baoab_fused(hipStream_t st, int pot_mask, int phase, Fields cur, Fields nxt, NListD nl, BoxD box,
-                        ConstraintD con, InteractD ip, int n, double dt, double gamma, TempD temp, uint32_t seed,
-                        const ClockD* clock, int clock_slot, int t_off, int parity, int head_enforce, int fp32,
-                        int planar, int multitype, double mass_u);
+                        ConstraintD con, InteractD ip, int n, const FusedArgs& fa, int fp32, int planar,
+                        int multitype);
 // The whole rebuild as one cooperative launch (grid barriers between the
 // phases); returns the launch status so the caller can fall back to
 // launch_rebuild where cooperative launches are unavailable.

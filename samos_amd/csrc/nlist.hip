@@ -180,13 +180,17 @@ __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
     const int c = nl.cell_of[s];
     const int ci = c / (nl.ncy * nl.ncz), cj = (c / nl.ncz) % nl.ncy, ck = c % nl.ncz;
     int k = 0, coord = 0;
+    // Planar: every particle is in the same z-layer ck, and a used cell grid
+    // has ncz >= 2, so the stencil's other z-layers hold nobody and the own
+    // layer appears exactly once; only it is visited.
+    const int iz_lo = PLANAR ? 0 : -1, iz_hi = PLANAR ? 0 : 1;
     for (int ix = -1; ix <= 1; ix++) {
       int iix = ci + ix;
       if (iix < 0) iix = nl.ncx - 1; else if (iix == nl.ncx) iix = 0;
       for (int iy = -1; iy <= 1; iy++) {
         int iiy = cj + iy;
         if (iiy < 0) iiy = nl.ncy - 1; else if (iiy == nl.ncy) iiy = 0;
-        for (int iz = -1; iz <= 1; iz++) {
+        for (int iz = iz_lo; iz <= iz_hi; iz++) {
           int iiz = ck + iz;
           if (iiz < 0) iiz = nl.ncz - 1; else if (iiz == nl.ncz) iiz = 0;
           const int cc = nl.ncy * nl.ncz * iix + nl.ncz * iiy + iiz;
@@ -432,7 +436,7 @@ hipError_t launch_rebuild_coop(hipStream_t st, Fields f, NListD nl, BoxD box, in
       e = planar ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rebuild_coop<true>, kScanThreads, 0)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rebuild_coop<false>, kScanThreads, 0);
     if (e != hipSuccess) return e;
-    grid = cus * std::min(per_cu, 4);
+    grid = cus * per_cu;
     if (grid < 1) return hipErrorInvalidConfiguration;
   }
   void* args[] = {&f, &nl, &box, &n, &req};

@@ -17,9 +17,7 @@
 
 namespace samos {
 
-constexpr int kBatch = 4;
-
-template <int POT, bool F32, bool PLANAR, bool MT>
+template <int POT, bool F32, bool PLANAR, bool MT, int kBatch = 4>
 __device__ __forceinline__ void pair_force_sum(const Fields& f, const NListD& nl, const BoxD& b, const InteractD& ip,
                                                int s, double xi, double yi, double zi, int ti, double ri,
                                                double& fx, double& fy, double& fz) {

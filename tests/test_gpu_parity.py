@@ -289,6 +289,7 @@ def test_langevin_noise_on_same_stream(gpu, flags):
     o.run(49)
     assert pos_err(positions(d), positions(o), st["box"], True) <= 1e-9
     assert np.abs(d.get("vx") - o.get("vx")).max() <= 1e-9
+    assert np.array_equal(d.get("age"), o.get("age"))
 
 
 @pytest.mark.parametrize("method", ["spv", "bbk"])
