@@ -178,6 +178,64 @@ int samos_sync(samos_ctx* ctx);
  * gathers in flight per thread in the fused step. */
 int samos_set_option(samos_ctx* ctx, const char* name, double value);
 
+/* ---- replica ensembles ---------------------------------------------------
+ * Many independent systems (replicas) of uneven size and parameters resident
+ * together and advanced by one fused launch per step.  Each replica behaves
+ * as the reference run alone on its own conf (System, NeighbourList,
+ * PairSoftPotential, PairPolarAlign, IntegratorBrownian / IntegratorFIRE,
+ * one ConstraintSphere / ConstraintPlane / none): the reference has one
+ * system per process, so a study of R parameter sets is R invocations of
+ * `samos run.conf`; an ensemble replaces those invocations.
+ * Columns cross the boundary replica-major: replica 0's particles in its
+ * input order, then replica 1's, ... */
+typedef struct samos_ens samos_ens;
+const char* samos_ens_last_error(const samos_ens* ens); /* NULL: error of the last create */
+/* counts[nrep]; boxes[3 nrep] (Lx, Ly, Lz per replica, centred at the origin);
+ * periodic[nrep]; constraint = SAMOS_CONSTRAINT_*, the same for every replica;
+ * con_param[nrep]: sphere radius R (ConstraintSphere, constraint_sphere.hpp:58-62)
+ * or plane height (may be NULL for the plane at z = 0). */
+int samos_ens_create(samos_ens** out, int device, int nrep, const int* counts, const double* boxes,
+                     const int* periodic, int constraint, const double* con_param);
+int samos_ens_destroy(samos_ens* ens);
+int samos_ens_num_particles(const samos_ens* ens);
+/* Columns SAMOS_X .. SAMOS_RADIUS (not MASS or the int columns). */
+int samos_ens_set_field(samos_ens* ens, int field, const double* host);
+int samos_ens_get_field(samos_ens* ens, int field, double* host);
+/* Per-replica parameters, 12 doubles each:
+ *   0 k, 1 a, 2 use_particle_radii   pair_potential soft (pair_soft_potential.hpp:63-110)
+ *   3 J, 4 rcut (<= 0: no aligner)    pair_align polar (pair_polar_aligner.hpp:65-110)
+ *   5 v0, 6 mu, 7 mur, 8 nu, 9 seed   integrator brownian at T = 0 (integrator_brownian.hpp:60-120)
+ *   10 dt                             timestep
+ *   11 unused */
+int samos_ens_set_params(samos_ens* ens, const double* params);
+/* NeighbourList(rcut, pad) of every replica (neighbour_list.hpp:56-95): hashed
+ * per-replica cell grids, each box side at least three cells of rcut + pad. */
+int samos_ens_nlist(samos_ens* ens, double rcut, double pad);
+/* out[7]: builds, capacity per particle, buckets, largest bucket, largest
+ * neighbour count, particles, replicas. */
+int samos_ens_nlist_info(samos_ens* ens, long long* out);
+/* Sorted (i, j), i < j, pairs of one replica in its input order, as samos_nlist_pairs. */
+int samos_ens_pairs(samos_ens* ens, int rep, int* out, long long cap, long long* count);
+int samos_ens_enforce(samos_ens* ens);
+/* Forces and torques of every replica; energies[nrep] (may be NULL): soft energy. */
+int samos_ens_forces(samos_ens* ens, double* energies);
+/* `run steps` with the Brownian integrator in every replica (steps + 1
+ * iterations, src/samos.cpp:878-970).  flags: SAMOS_RUN_NO_GRAPH. */
+int samos_ens_run_brownian(samos_ens* ens, int steps, int flags);
+/* IntegratorFIRE parameters for every replica (integrator_fire.hpp:58-154) and
+ * the stop rule max|F| < stop_fmax that freezes a replica (no counterpart:
+ * the reference's FIRE never stops the run loop). */
+int samos_ens_fire_setup(samos_ens* ens, double dt, double alpha, double alpha_scale, double f_tol, double e_tol,
+                         double dt_inc, double dt_dec, double dt_max, int min_neg_steps, double stop_fmax);
+/* Up to max_iter FIRE run-loop iterations, stopping once every replica has met
+ * the stop rule (tested every `check` iterations). */
+int samos_ens_fire_run(samos_ens* ens, int max_iter, int check, int* remaining, int* iters);
+/* out[8 nrep]: iterations, stopped (1/0), energy, max|F|, |F|, dt, alpha, P per replica. */
+int samos_ens_fire_state(samos_ens* ens, double* out);
+/* "gather_batch" 4 or 8. */
+int samos_ens_set_option(samos_ens* ens, const char* name, double value);
+int samos_ens_sync(samos_ens* ens);
+
 #ifdef __cplusplus
 }
 #endif

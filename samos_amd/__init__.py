@@ -87,6 +87,24 @@ def load_library(path: str = LIB_PATH):
         "samos_profile_read": (I, [P, ctypes.c_char_p, dp, llp]),
         "samos_sync": (I, [P]),
         "samos_set_option": (I, [P, ctypes.c_char_p, D]),
+        "samos_ens_last_error": (ctypes.c_char_p, [P]),
+        "samos_ens_create": (I, [ctypes.POINTER(P), I, I, ip, dp, ip, I, dp]),
+        "samos_ens_destroy": (I, [P]),
+        "samos_ens_num_particles": (I, [P]),
+        "samos_ens_set_field": (I, [P, I, P]),
+        "samos_ens_get_field": (I, [P, I, P]),
+        "samos_ens_set_params": (I, [P, dp]),
+        "samos_ens_nlist": (I, [P, D, D]),
+        "samos_ens_nlist_info": (I, [P, llp]),
+        "samos_ens_pairs": (I, [P, I, ip, LL, llp]),
+        "samos_ens_enforce": (I, [P]),
+        "samos_ens_forces": (I, [P, dp]),
+        "samos_ens_run_brownian": (I, [P, I, I]),
+        "samos_ens_fire_setup": (I, [P, D, D, D, D, D, D, D, D, I, D]),
+        "samos_ens_fire_run": (I, [P, I, I, ip, ip]),
+        "samos_ens_fire_state": (I, [P, dp]),
+        "samos_ens_set_option": (I, [P, ctypes.c_char_p, D]),
+        "samos_ens_sync": (I, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -285,3 +303,138 @@ class System:
         ms, cnt = ctypes.c_double(), ctypes.c_longlong()
         self._chk(self.L.samos_profile_read(self.h, cls.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
+
+
+ENS_FIELDS = DOUBLE_FIELDS[:18]  # x .. radius
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Ensemble:
+    """Replicas of independent particle systems resident together on one GPU
+    (a samos_ens).  Columns are replica-major: replica 0's particles in input
+    order, then replica 1's, and so on."""
+
+    PARAMS = ["k", "a", "use_radii", "J", "rcut_align", "v0", "mu", "mur", "nu", "seed", "dt"]
+
+    def __init__(self, counts, boxes, periodic, constraint="none", con_param=None, device=0):
+        self.L = load_library()
+        self.counts = np.ascontiguousarray(counts, dtype=np.int32)
+        self.nrep = len(self.counts)
+        self.n = int(self.counts.sum())
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
+        b = np.ascontiguousarray(boxes, dtype=np.float64).reshape(self.nrep, 3)
+        per = np.ascontiguousarray(np.broadcast_to(np.asarray(periodic, dtype=np.int32), (self.nrep,)))
+        cp = None if con_param is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(con_param, dtype=np.float64), (self.nrep,)))
+        h = ctypes.c_void_p()
+        rc = self.L.samos_ens_create(ctypes.byref(h), int(device), self.nrep, _iptr(self.counts), _dptr(b),
+                                     _iptr(per), CONSTRAINT[constraint], None if cp is None else _dptr(cp))
+        if rc != 0:
+            raise SamosError(self.L.samos_ens_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SamosError(self.L.samos_ens_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.samos_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        if a.shape != (self.n,):
+            raise ValueError(f"{name}: expected shape ({self.n},), got {a.shape}")
+        self._chk(self.L.samos_ens_set_field(self.h, FIELD_CODE[name], a.ctypes.data))
+
+    def get(self, name):
+        a = np.empty(self.n, dtype=np.float64)
+        self._chk(self.L.samos_ens_get_field(self.h, FIELD_CODE[name], a.ctypes.data))
+        return a
+
+    def replica(self, r, col):
+        """Slice of a replica-major column belonging to replica r."""
+        return col[self.offsets[r]:self.offsets[r + 1]]
+
+    def set_params(self, **cols):
+        """Per-replica parameters by name (scalars broadcast): k, a, use_radii,
+        J, rcut_align, v0, mu, mur, nu, seed, dt."""
+        defaults = {"k": 1.0, "a": 2.0, "use_radii": 0.0, "J": 0.0, "rcut_align": 0.0, "v0": 0.0, "mu": 1.0,
+                    "mur": 1.0, "nu": 0.0, "seed": 0.0, "dt": 0.01}
+        unknown = set(cols) - set(defaults)
+        if unknown:
+            raise ValueError(f"unknown parameters {sorted(unknown)}")
+        p = np.zeros((self.nrep, 12))
+        for i, name in enumerate(self.PARAMS):
+            p[:, i] = np.broadcast_to(np.asarray(cols.get(name, defaults[name]), dtype=np.float64), (self.nrep,))
+        p = np.ascontiguousarray(p)
+        self._chk(self.L.samos_ens_set_params(self.h, _dptr(p)))
+
+    def nlist(self, rcut, pad):
+        self._chk(self.L.samos_ens_nlist(self.h, float(rcut), float(pad)))
+
+    def nlist_info(self):
+        out = (ctypes.c_longlong * 7)()
+        self._chk(self.L.samos_ens_nlist_info(self.h, out))
+        keys = ["builds", "stride", "buckets", "max_bucket", "max_neighbours", "particles", "replicas"]
+        return dict(zip(keys, list(out)))
+
+    def pairs(self, r):
+        cnt = ctypes.c_longlong()
+        self._chk(self.L.samos_ens_pairs(self.h, int(r), None, 0, ctypes.byref(cnt)))
+        out = np.empty((max(cnt.value, 1), 2), dtype=np.int32)
+        self._chk(self.L.samos_ens_pairs(self.h, int(r), _iptr(out), cnt.value, ctypes.byref(cnt)))
+        return out[:cnt.value]
+
+    def enforce(self):
+        self._chk(self.L.samos_ens_enforce(self.h))
+
+    def forces(self):
+        e = np.zeros(self.nrep)
+        self._chk(self.L.samos_ens_forces(self.h, _dptr(e)))
+        return e
+
+    def run_brownian(self, steps, flags=0):
+        self._chk(self.L.samos_ens_run_brownian(self.h, int(steps), int(flags)))
+
+    def fire_setup(self, dt=0.01, alpha=0.1, alpha_scale=0.99, f_tol=1e-4, e_tol=1e-6, dt_inc=1.1, dt_dec=0.5,
+                   dt_max=1.0, min_neg=5, stop_fmax=1e-10):
+        self._chk(self.L.samos_ens_fire_setup(self.h, dt, alpha, alpha_scale, f_tol, e_tol, dt_inc, dt_dec, dt_max,
+                                              int(min_neg), stop_fmax))
+
+    def fire_run(self, max_iter, check=50):
+        rem, it = ctypes.c_int(), ctypes.c_int()
+        self._chk(self.L.samos_ens_fire_run(self.h, int(max_iter), int(check), ctypes.byref(rem), ctypes.byref(it)))
+        return rem.value, it.value
+
+    def fire_state(self):
+        out = np.zeros((self.nrep, 8))
+        self._chk(self.L.samos_ens_fire_state(self.h, _dptr(out)))
+        keys = ["iters", "stopped", "energy", "max_force", "force_norm", "dt", "alpha", "power"]
+        return {k: out[:, i].copy() for i, k in enumerate(keys)}
+
+    def set_option(self, name, value):
+        self._chk(self.L.samos_ens_set_option(self.h, name.encode(), float(value)))
+
+    def sync(self):
+        self._chk(self.L.samos_ens_sync(self.h))

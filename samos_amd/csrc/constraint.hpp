@@ -1,0 +1,114 @@
+// Surface constraints as inline device functions, FP64, shared by the
+// single-system integrators and the replica ensembles.
+//
+//   sphere           src/constraints/constraint_sphere.cpp:38-81
+//   plane            src/constraints/constraint_plane.cpp:36-160
+//   none             src/constraints/constraint_none.hpp:58-80
+//   rotate/project   src/constraints/constraint.cpp:96-173
+//   periodic wrap    src/system/system.cpp:1119-1155
+#pragma once
+#include "samos_device.hpp"
+
+namespace samos {
+
+// One particle's kinematic state in registers; dix/diy/diz collect image-flag
+// changes from periodic wraps.
+struct PS {
+  double x, y, z, vx, vy, vz, nx, ny, nz;
+  int dix, diy, diz;
+};
+
+
+// System::enforce_periodic (src/system/system.cpp:1119-1155)
+__device__ __forceinline__ void enforce_periodic(const BoxD& b, PS& p) {
+  if (!b.periodic) return;
+  if (p.x <= b.xlo) { p.x += b.Lx; p.dix--; } else if (p.x > b.xhi) { p.x -= b.Lx; p.dix++; }
+  if (p.y <= b.ylo) { p.y += b.Ly; p.diy--; } else if (p.y > b.yhi) { p.y -= b.Ly; p.diy++; }
+  if (p.z <= b.zlo) { p.z += b.Lz; p.diz--; } else if (p.z > b.zhi) { p.z -= b.Lz; p.diz++; }
+}
+
+// Constraint::enforce of the one active constraint; returns true when the
+// plane constraint zeroed the particle's f_z.
+__device__ __forceinline__ bool enforce(const BoxD& b, const ConstraintD& c, PS& p) {
+  if (c.kind == 0) {
+    enforce_periodic(b, p);
+  } else if (c.kind == 1) {
+    const double xlo = -0.5 * b.Lx, xhi = 0.5 * b.Lx, ylo = -0.5 * b.Ly, yhi = 0.5 * b.Ly;
+    p.z = c.zpos; p.vz = 0.0;
+    if (b.periodic) enforce_periodic(b, p);
+    else if (!c.unlimited) {
+      if (p.x < xlo) { p.x = xlo; p.vx = -p.vx; } else if (p.x > xhi) { p.x = xhi; p.vx = -p.vx; }
+      if (p.y < ylo) { p.y = ylo; p.vy = -p.vy; } else if (p.y > yhi) { p.y = yhi; p.vy = -p.vy; }
+    }
+    p.nz = 0.0;
+    const double len_n = sqrt(p.nx * p.nx + p.ny * p.ny);
+    double inv_len = 1.0;
+    if (len_n != 0.0) inv_len = 1.0 / sqrt(p.nx * p.nx + p.ny * p.ny);
+    p.nx *= inv_len; p.ny *= inv_len;
+    return true;
+  } else if (c.kind == 2) {
+    const double R0 = c.r;
+    const double R = sqrt(p.x * p.x + p.y * p.y + p.z * p.z);
+    const double sc = R0 / R;
+    p.x *= sc; p.y *= sc; p.z *= sc;
+    const double Nx = p.x / R0, Ny = p.y / R0, Nz = p.z / R0;
+    const double vN = p.vx * Nx + p.vy * Ny + p.vz * Nz;
+    const double nN = p.nx * Nx + p.ny * Ny + p.nz * Nz;
+    p.vx -= vN * Nx; p.vy -= vN * Ny; p.vz -= vN * Nz;
+    p.nx -= nN * Nx; p.ny -= nN * Ny; p.nz -= nN * Nz;
+    const double inv_len = 1.0 / sqrt(p.nx * p.nx + p.ny * p.ny + p.nz * p.nz);
+    p.nx *= inv_len; p.ny *= inv_len; p.nz *= inv_len;
+  }
+  return false;
+}
+
+__device__ __forceinline__ double project_torque(const ConstraintD& c, const PS& p, double tx, double ty, double tz) {
+  if (c.kind == 1) return tz;
+  if (c.kind == 2) {
+    const double Nx = p.x / c.r, Ny = p.y / c.r, Nz = p.z / c.r;
+    return tx * Nx + ty * Ny + tz * Nz;
+  }
+  return 0.0;
+}
+
+__device__ __forceinline__ void rotate_director(const ConstraintD& c, PS& p, double phi) {
+  if (c.kind == 1) {
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    const double nx = cs * p.nx + (-sn) * p.ny;
+    const double ny = sn * p.nx + cs * p.ny;
+    const double len = sqrt(nx * nx + ny * ny);
+    p.nx = nx / len; p.ny = ny / len;
+  } else if (c.kind == 2) {
+    const double U = p.x / c.r, V = p.y / c.r, W = p.z / c.r;
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    const double un = U * p.nx + V * p.ny + W * p.nz;
+    const double nx = U * un * (1.0 - cs) + p.nx * cs + (-W * p.ny + V * p.nz) * sn;
+    const double ny = V * un * (1.0 - cs) + p.ny * cs + (W * p.nx - U * p.nz) * sn;
+    const double nz = W * un * (1.0 - cs) + p.nz * cs + (-V * p.nx + U * p.ny) * sn;
+    const double len = sqrt(nx * nx + ny * ny + nz * nz);
+    p.nx = nx / len; p.ny = ny / len; p.nz = nz / len;
+  }
+}
+
+__device__ __forceinline__ void rotate_velocity(const ConstraintD& c, PS& p, double phi) {
+  if (c.kind == 1) {
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    const double vx = cs * p.vx + (-sn) * p.vy;
+    const double vy = sn * p.vx + cs * p.vy;
+    p.vx = vx; p.vy = vy;
+  } else if (c.kind == 2) {
+    const double U = p.x / c.r, V = p.y / c.r, W = p.z / c.r;
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    const double uv = U * p.vx + V * p.vy + W * p.vz;
+    const double vx = U * uv * (1.0 - cs) + p.vx * cs + (-W * p.vy + V * p.vz) * sn;
+    const double vy = V * uv * (1.0 - cs) + p.vy * cs + (W * p.vx - U * p.vz) * sn;
+    const double vz = W * uv * (1.0 - cs) + p.vz * cs + (-V * p.vx + U * p.vy) * sn;
+    p.vx = vx; p.vy = vy; p.vz = vz;
+  }
+}
+
+}  // namespace samos

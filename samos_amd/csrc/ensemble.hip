@@ -1,0 +1,1141 @@
+// Replica ensembles: hundreds of independent particle systems of different
+// sizes and parameters advanced together, one fused step launch for all of
+// them per step.
+//
+// Layout.  The replicas' particles are concatenated in HBM: replica r owns the
+// slots [n0_r, n0_r + n_r), Morton-ordered on its own cell grid.  Each
+// replica's cell grid is hashed into its own bucket table of 2^h >= n_r
+// buckets, so a sphere of radius R costs O(n) buckets instead of (2R / r_l)^3
+// grid cells.  The build visits the distinct buckets of the 27 stencil cells
+// and keeps only candidates whose own cell lies in the stencil, which gives
+// exactly the pair set of the reference's cell list when every side has three
+// or more cells (src/system/neighbour_list.cpp:216-273,
+// src/system/cell_list.cpp:37-104).
+//
+// Each replica decides its own rebuild from its own particles' displacements
+// (src/system/neighbour_list.cpp:278-302), so a replica advances exactly as it
+// would alone.  Its noise is the single-system Philox stream keyed by the
+// replica's seed and the particle's index inside the replica.
+//
+// Physics per replica:
+//   Brownian step (T = 0, rotational noise)  src/integrators/integrator_brownian.cpp:42-124
+//   soft repulsion                            src/potentials/pair/pair_soft_potential.cpp:33-119
+//   polar alignment                           src/aligner/pair/pair_polar_aligner.cpp:32-107
+//   FIRE                                      src/integrators/integrator_fire.cpp:34-148
+//   run loop order                            src/samos.cpp:878-970
+#include "constraint.hpp"
+#include "kernels.hpp"
+#include "samos_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace samos {
+
+struct RepD {
+  BoxD box;
+  ConstraintD con;
+  int n0, n;                  // slot range
+  int b0, hbits;              // bucket base and log2 of the bucket count
+  int ncx, ncy, ncz, pad0_;
+  double wx, wy, wz, cut2, pad2q;
+  // soft repulsion and polar alignment
+  double k, a, J, rc2_al;
+  int radii, has_al;
+  // Brownian
+  double v0, mu, mur, stoch, dt;
+  uint32_t seed;
+  int just_frozen;             // FIRE: stopped in the current iteration
+  // FIRE
+  double fdt, alpha, alpha0, f_alpha, f_inc, f_dec, dt_max, f_tol, e_tol, E_old, f1, f2, stop_fmax;
+  int n_min, last_neg, frozen, zero_v;
+  long long iters;
+  double P, F2, V2, E, Fmax;
+};
+
+// Every column of an ensemble (slot order).  Positions and directors are
+// double-buffered for the fused step.
+struct EState {
+  double *x[2], *y[2], *z[2], *nx[2], *ny[2], *nz[2];
+  double *vx, *vy, *vz, *fx, *fy, *fz, *tx, *ty, *tz, *omega, *age, *radius;
+  int *rep, *lid, *cellc;
+};
+
+namespace {
+
+__device__ __forceinline__ int bucket_of(const RepD& R, int c) {
+  const unsigned long long h = (unsigned long long)(unsigned)c * 0x9E3779B97F4A7C15ull;
+  return R.b0 + (int)(h >> (64 - R.hbits));
+}
+
+__device__ __forceinline__ void cell_ijk(const RepD& R, double x, double y, double z, int& i, int& j, int& k) {
+  i = (int)((x - R.box.xlo) / R.wx);
+  j = (int)((y - R.box.ylo) / R.wy);
+  k = (int)((z - R.box.zlo) / R.wz);
+  i = min(max(i, 0), R.ncx - 1);
+  j = min(max(j, 0), R.ncy - 1);
+  k = min(max(k, 0), R.ncz - 1);
+}
+
+// ---- neighbour-list rebuild of the flagged replicas ------------------------
+// CellList::populate (src/system/cell_list.cpp:86-104): periodic wrap, then the
+// clamped cell index; the bucket count is raised.
+__global__ void k_ens_bin(EState e, int b, const RepD* reps, const int* rep_need, NListD nl, int n, const int* need) {
+  if (*need == 0) return;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const int r = e.rep[s];
+    if (!rep_need[r]) continue;
+    const RepD& R = reps[r];
+    double x = e.x[b][s], y = e.y[b][s], z = e.z[b][s];
+    if (R.box.periodic) {
+      const BoxD& B = R.box;
+      bool w = false;
+      if (x < B.xlo) { x += B.Lx; w = true; } else if (x > B.xhi) { x -= B.Lx; w = true; }
+      if (y < B.ylo) { y += B.Ly; w = true; } else if (y > B.yhi) { y -= B.Ly; w = true; }
+      if (z < B.zlo) { z += B.Lz; w = true; } else if (z > B.zhi) { z -= B.Lz; w = true; }
+      if (w) { e.x[b][s] = x; e.y[b][s] = y; e.z[b][s] = z; }
+    }
+    int i, j, k;
+    cell_ijk(R, x, y, z, i, j, k);
+    e.cellc[s] = (i << 20) | (j << 10) | k;
+    const int bk = bucket_of(R, R.ncy * R.ncz * i + R.ncz * j + k);
+    nl.cell_of[s] = bk;
+    atomicAdd(&nl.count[bk], 1);
+  }
+}
+
+__global__ void k_ens_scatter(EState e, const int* rep_need, NListD nl, int n, const int* need) {
+  if (*need == 0) return;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    if (!rep_need[e.rep[s]]) continue;
+    const int c = nl.cell_of[s];
+    nl.members[nl.start[c] + atomicSub(&nl.count[c], 1) - 1] = s;
+  }
+}
+
+// Cell a is in the periodic stencil of cell b along one axis of nc >= 3 cells.
+__device__ __forceinline__ bool adjacent(int a, int b, int nc) {
+  const int d = a - b;
+  return (d >= -1 && d <= 1) || (a == 0 && b == nc - 1) || (a == nc - 1 && b == 0);
+}
+
+// Full list of every slot of a flagged replica: the distinct buckets of its 27
+// stencil cells, candidates restricted to stencil cells, d^2 < (rcut + pad)^2.
+__global__ void __launch_bounds__(kBlock) k_ens_build(EState e, int b, const RepD* reps, const int* rep_need,
+                                                      NListD nl, int n, const int* need) {
+  if (*need == 0) return;
+  bool overflow = false;
+  int kmax = 0;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const int r = e.rep[s];
+    if (!rep_need[r]) continue;
+    const RepD& R = reps[r];
+    const double xi = e.x[b][s], yi = e.y[b][s], zi = e.z[b][s];
+    const int cc = e.cellc[s];
+    const int ci = cc >> 20, cj = (cc >> 10) & 1023, ck = cc & 1023;
+    int bk[27];
+    int nbk = 0;
+    for (int ix = -1; ix <= 1; ix++) {
+      int iix = ci + ix;
+      if (iix < 0) iix = R.ncx - 1; else if (iix == R.ncx) iix = 0;
+      for (int iy = -1; iy <= 1; iy++) {
+        int iiy = cj + iy;
+        if (iiy < 0) iiy = R.ncy - 1; else if (iiy == R.ncy) iiy = 0;
+        for (int iz = -1; iz <= 1; iz++) {
+          int iiz = ck + iz;
+          if (iiz < 0) iiz = R.ncz - 1; else if (iiz == R.ncz) iiz = 0;
+          const int q = bucket_of(R, R.ncy * R.ncz * iix + R.ncz * iiy + iiz);
+          bool dup = false;
+          for (int u = 0; u < nbk; ++u) dup |= bk[u] == q;
+          if (!dup) bk[nbk++] = q;
+        }
+      }
+    }
+    int k = 0;
+    for (int u = 0; u < nbk; ++u) {
+      const int a0 = nl.start[bk[u]], a1 = nl.start[bk[u] + 1];
+      for (int m = a0; m < a1; ++m) {
+        const int t = nl.members[m];
+        if (t == s) continue;
+        const int tc = e.cellc[t];
+        if (!adjacent(tc >> 20, ci, R.ncx) || !adjacent((tc >> 10) & 1023, cj, R.ncy) ||
+            !adjacent(tc & 1023, ck, R.ncz))
+          continue;
+        double dx = xi - e.x[b][t], dy = yi - e.y[b][t], dz = zi - e.z[b][t];
+        min_image(R.box, dx, dy, dz);
+        if (dx * dx + dy * dy + dz * dz < R.cut2) {
+          if (k < nl.stride) nl.nbr[(long)k * nl.np + s] = t;
+          ++k;
+        }
+      }
+    }
+    overflow |= k > nl.stride;
+    kmax = max(kmax, k);
+    nl.nnb[s] = min(k, nl.stride);
+    nl.ox[s] = xi; nl.oy[s] = yi; nl.oz[s] = zi;
+  }
+  for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o, 64));
+  const bool any_over = __ballot(overflow) != 0;
+  if ((threadIdx.x & 63) == 0) {
+    if (any_over) atomicOr(&nl.flags[2], 1);
+    if (kmax > 0) atomicMax(&nl.flags[5], kmax);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&nl.flags[4], 1);
+}
+
+// Start of a step: no replica needs a rebuild yet.
+__global__ void k_ens_clear_need(int* rep_need, int nrep, int* need) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrep; r += gridDim.x * blockDim.x) rep_need[r] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *need = 0;
+}
+
+__global__ void k_ens_set_need(int* rep_need, int nrep, int* need) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrep; r += gridDim.x * blockDim.x) rep_need[r] = 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *need = 1;
+}
+
+// Advances the noise call counter; optionally clears one counter.
+__global__ void k_ens_advance(unsigned long long* call, int by, int* zero_me) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *call += (unsigned long long)by;
+    if (zero_me) *zero_me = 0;
+  }
+}
+
+__device__ __forceinline__ void flag_rebuild(const RepD& R, const NListD& nl, int s, int r, double x, double y,
+                                             double z, int* rep_need, int* need) {
+  double dx = nl.ox[s] - x, dy = nl.oy[s] - y, dz = nl.oz[s] - z;
+  min_image(R.box, dx, dy, dz);
+  if (!(dx * dx + dy * dy + dz * dz < R.pad2q)) {
+    rep_need[r] = 1;
+    *need = 1;
+  }
+}
+
+// Soft force, its energy and (AL) the polar torque on slot s over its list.
+// The soft sum follows pair order d = x_j - x_i, F_i -= ff d; the torque on i
+// is J n_i x n_j per neighbour.
+template <int BATCH, bool AL, bool EN>
+__device__ __forceinline__ void ens_pair_sum(const EState& e, int b, const NListD& nl, const RepD& R, int s,
+                                             const PS& p, double ri, double& fx, double& fy, double& fz,
+                                             double& tx, double& ty, double& tz, double& en) {
+  const int kn = nl.nnb[s];
+  const int* nb = nl.nbr + s;
+  const double* X = e.x[b];
+  const double* Y = e.y[b];
+  const double* Z = e.z[b];
+  for (int k0 = 0; k0 < kn; k0 += BATCH) {
+    int t[BATCH];
+#pragma unroll
+    for (int u = 0; u < BATCH; ++u) t[u] = (k0 + u < kn) ? nb[(long)(k0 + u) * nl.np] : s;
+    double xj[BATCH], yj[BATCH], zj[BATCH];
+#pragma unroll
+    for (int u = 0; u < BATCH; ++u) { xj[u] = X[t[u]]; yj[u] = Y[t[u]]; zj[u] = Z[t[u]]; }
+#pragma unroll
+    for (int u = 0; u < BATCH; ++u) {
+      if (k0 + u >= kn) break;
+      double dx = xj[u] - p.x, dy = yj[u] - p.y, dz = zj[u] - p.z;
+      min_image(R.box, dx, dy, dz);
+      const double r_sq = dx * dx + dy * dy + dz * dz;
+      const double r = sqrt(r_sq);
+      const double a = R.radii ? ri + e.radius[t[u]] : R.a;
+      if (r < a) {
+        const double diff = a - r;
+        if (EN) en += 0.5 * R.k * diff * diff;
+        const double ff = (r > 0.0) ? R.k * diff / r : R.k * diff;
+        fx -= ff * dx; fy -= ff * dy; fz -= ff * dz;
+      }
+      if (AL && r_sq <= R.rc2_al) {
+        const double nxj = e.nx[b][t[u]], nyj = e.ny[b][t[u]], nzj = e.nz[b][t[u]];
+        tx += R.J * (p.ny * nzj - p.nz * nyj);
+        ty += R.J * (p.nz * nxj - p.nx * nzj);
+        tz += R.J * (p.nx * nyj - p.ny * nxj);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void load_pos(const EState& e, int b, int s, PS& p) {
+  p.x = e.x[b][s]; p.y = e.y[b][s]; p.z = e.z[b][s];
+  p.nx = e.nx[b][s]; p.ny = e.ny[b][s]; p.nz = e.nz[b][s];
+  p.dix = p.diy = p.diz = 0;
+}
+
+// One run-loop iteration of every replica (Brownian, T = 0): forces and
+// torques at (x_t, n_t) from buffer b, the integrator update, the rebuild test
+// on the integrated positions, then the next iteration's loop-head enforce
+// (head != 0) and the store into buffer b ^ 1.  Velocities, forces, torques
+// and omega are stored on the last iteration of a run only (last != 0): no
+// step reads them back.
+template <int BATCH>
+__global__ void __launch_bounds__(kBlock) k_ens_brownian(EState e, int b, const RepD* reps, NListD nl, int* rep_need,
+                                                         int* need, int n, const unsigned long long* callp, int j,
+                                                         int head, int last) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int r = e.rep[s];
+  const RepD& R = reps[r];
+  PS p;
+  load_pos(e, b, s, p);
+  double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, en = 0;
+  const double ri = R.radii ? e.radius[s] : 0.0;
+  if (R.has_al) ens_pair_sum<BATCH, true, false>(e, b, nl, R, s, p, ri, fx, fy, fz, tx, ty, tz, en);
+  else ens_pair_sum<BATCH, false, false>(e, b, nl, R, s, p, ri, fx, fy, fz, tx, ty, tz, en);
+  const double dt = R.dt;
+  const double fdx = R.v0 * p.nx + R.mu * fx, fdy = R.v0 * p.ny + R.mu * fy, fdz = R.v0 * p.nz + R.mu * fz;
+  p.vx = fdx; p.vy = fdy; p.vz = fdz;
+  p.x += dt * fdx; p.y += dt * fdy; p.z += dt * fdz;
+  const bool fz0 = enforce(R.box, R.con, p);
+  const double om = R.mur * project_torque(R.con, p, tx, ty, tz);
+  double g, u;
+  gauss1_u1(R.seed, TAG_BROWNIAN, (uint32_t)e.lid[s], *callp + (unsigned long long)j, g, u);
+  rotate_director(R.con, p, dt * om + R.stoch * g);
+  flag_rebuild(R, nl, s, r, p.x, p.y, p.z, rep_need, need);
+  if (last) {
+    e.vx[s] = p.vx; e.vy[s] = p.vy; e.vz[s] = p.vz;
+    e.fx[s] = fx; e.fy[s] = fy; e.fz[s] = fz0 ? 0.0 : fz;
+    e.tx[s] = tx; e.ty[s] = ty; e.tz[s] = tz;
+    e.omega[s] = om;
+  }
+  if (head) enforce(R.box, R.con, p);
+  const int c = b ^ 1;
+  e.x[c][s] = p.x; e.y[c][s] = p.y; e.z[c][s] = p.z;
+  e.nx[c][s] = p.nx; e.ny[c][s] = p.ny; e.nz[c][s] = p.nz;
+}
+
+// age += dt once per iteration, repeated exactly as often as the run advanced.
+__global__ void k_ens_age(EState e, const RepD* reps, int n, int iters) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const double dt = reps[e.rep[s]].dt;
+  double a = e.age[s];
+  for (int i = 0; i < iters; ++i) a += dt;
+  e.age[s] = a;
+}
+
+__global__ void k_ens_enforce(EState e, int b, const RepD* reps, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const RepD& R = reps[e.rep[s]];
+  PS p;
+  load_pos(e, b, s, p);
+  p.vx = e.vx[s]; p.vy = e.vy[s]; p.vz = e.vz[s];
+  if (enforce(R.box, R.con, p)) e.fz[s] = 0.0;
+  e.x[b][s] = p.x; e.y[b][s] = p.y; e.z[b][s] = p.z;
+  e.nx[b][s] = p.nx; e.ny[b][s] = p.ny; e.nz[b][s] = p.nz;
+  e.vx[s] = p.vx; e.vy[s] = p.vy; e.vz[s] = p.vz;
+}
+
+// Forces, torques and per-slot energies (0.5 k (a - r)^2 per listed neighbour).
+__global__ void k_ens_forces(EState e, int b, const RepD* reps, NListD nl, double* e_out, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const RepD& R = reps[e.rep[s]];
+  PS p;
+  load_pos(e, b, s, p);
+  double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, en = 0;
+  const double ri = R.radii ? e.radius[s] : 0.0;
+  if (R.has_al) ens_pair_sum<4, true, true>(e, b, nl, R, s, p, ri, fx, fy, fz, tx, ty, tz, en);
+  else ens_pair_sum<4, false, true>(e, b, nl, R, s, p, ri, fx, fy, fz, tx, ty, tz, en);
+  e.fx[s] = fx; e.fy[s] = fy; e.fz[s] = fz;
+  e.tx[s] = tx; e.ty[s] = ty; e.tz[s] = tz;
+  if (e_out) e_out[s] = en;
+}
+
+// ---- FIRE ----------------------------------------------------------------
+// IntegratorFIRE::integrate, first loop: half kick with the current forces,
+// drift, enforce (src/integrators/integrator_fire.cpp:50-70).  Directors do
+// not enter the soft force and are left as they are.
+__global__ void k_ens_fire_a(EState e, int b, const RepD* reps, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const RepD& R = reps[e.rep[s]];
+  if (R.frozen) return;
+  const double dt = R.fdt, dt_2 = 0.5 * R.fdt;
+  PS p;
+  load_pos(e, b, s, p);
+  p.vx = e.vx[s] + dt_2 * e.fx[s];
+  p.vy = e.vy[s] + dt_2 * e.fy[s];
+  p.vz = e.vz[s] + dt_2 * e.fz[s];
+  p.x += dt * p.vx; p.y += dt * p.vy; p.z += dt * p.vz;
+  enforce(R.box, R.con, p);
+  e.x[b][s] = p.x; e.y[b][s] = p.y; e.z[b][s] = p.z;
+  e.vx[s] = p.vx; e.vy[s] = p.vy; e.vz[s] = p.vz;
+}
+
+// Forces at the drifted positions, second half kick, the per-slot terms of
+// P = v.F, |F|^2, |v|^2, the energy and |F|, enforce, and the rebuild test
+// (src/integrators/integrator_fire.cpp:72-96).  Columns: 0 P, 1 F2, 2 V2, 3 E, 4 |F|.
+// Other threads read this slot's position as a neighbour, so the enforced
+// position is not stored here: k_ens_fire_mix applies the same (position-only)
+// projection after every force has been formed.
+template <int BATCH>
+__global__ void __launch_bounds__(kBlock) k_ens_fire_b(EState e, int b, const RepD* reps, NListD nl, double* cols,
+                                                       long ld, int n, int* rep_need, int* need) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int r = e.rep[s];
+  const RepD& R = reps[r];
+  if (R.frozen) return;
+  PS p;
+  load_pos(e, b, s, p);
+  double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, en = 0;
+  ens_pair_sum<BATCH, false, true>(e, b, nl, R, s, p, R.radii ? e.radius[s] : 0.0, fx, fy, fz, tx, ty, tz, en);
+  const double dt_2 = 0.5 * R.fdt;
+  p.vx = e.vx[s] + dt_2 * fx;
+  p.vy = e.vy[s] + dt_2 * fy;
+  p.vz = e.vz[s] + dt_2 * fz;
+  const double f2 = fx * fx + fy * fy + fz * fz;
+  cols[s] = p.vx * fx + p.vy * fy + p.vz * fz;
+  cols[ld + s] = f2;
+  cols[2 * ld + s] = p.vx * p.vx + p.vy * p.vy + p.vz * p.vz;
+  cols[3 * ld + s] = en;
+  cols[4 * ld + s] = sqrt(f2);
+  if (enforce(R.box, R.con, p)) fz = 0.0;
+  e.fx[s] = fx; e.fy[s] = fy; e.fz[s] = fz;
+  e.nx[b][s] = p.nx; e.ny[b][s] = p.ny; e.nz[b][s] = p.nz;
+  e.vx[s] = p.vx; e.vy[s] = p.vy; e.vz[s] = p.vz;
+  flag_rebuild(R, nl, s, r, p.x, p.y, p.z, rep_need, need);
+}
+
+// Per-replica sums of columns 0-3 and the max of column 4: one workgroup per
+// replica, a fixed summation tree, so results do not depend on scheduling.
+__global__ void __launch_bounds__(kBlock) k_ens_fire_reduce(const double* cols, long ld, RepD* reps) {
+  RepD& R = reps[blockIdx.x];
+  if (R.frozen) return;
+  __shared__ double lds[5][kBlock / 64];
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int s = R.n0 + threadIdx.x; s < R.n0 + R.n; s += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] += cols[q * ld + s];
+    acc[4] = fmax(acc[4], cols[4 * ld + s]);
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    double v = acc[q];
+    for (int o = 32; o > 0; o >>= 1) {
+      const double w = __shfl_xor(v, o, 64);
+      v = q < 4 ? v + w : fmax(v, w);
+    }
+    if ((threadIdx.x & 63) == 0) lds[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[5] = {0, 0, 0, 0, 0};
+    for (int w = 0; w < kBlock / 64; ++w)
+      for (int q = 0; q < 5; ++q) t[q] = q < 4 ? t[q] + lds[q][w] : fmax(t[q], lds[q][w]);
+    R.P = t[0]; R.F2 = t[1]; R.V2 = t[2]; R.E = 0.5 * t[3]; R.Fmax = t[4];
+  }
+}
+
+// The scalar part of IntegratorFIRE::integrate per replica
+// (src/integrators/integrator_fire.cpp:97-147), preceded by the stop rule
+// max |F| < stop_fmax, which freezes a converged replica.  *remaining counts
+// the replicas still running.
+__global__ void k_ens_fire_update(RepD* reps, int nrep, int* remaining) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  RepD& R = reps[r];
+  if (R.frozen) {
+    R.just_frozen = 0;
+    return;
+  }
+  R.iters++;
+  if (R.Fmax < R.stop_fmax) {
+    R.frozen = 1;
+    R.just_frozen = 1;
+    R.f1 = 1.0; R.f2 = 0.0; R.zero_v = 0;
+    return;
+  }
+  atomicAdd(remaining, 1);
+  const double Fn = sqrt(R.F2), Vn = sqrt(R.V2);
+  const double sqrt_ndof = sqrt((double)(3 * R.n));
+  if (Fn / sqrt_ndof < R.f_tol && fabs(R.E - R.E_old) < R.e_tol) {
+    R.f1 = 1.0; R.f2 = 0.0; R.zero_v = 0;
+    return;
+  }
+  const double inv_F = 1.0 / Fn;
+  R.f1 = 1.0 - R.alpha;
+  R.f2 = R.alpha * inv_F * Vn;
+  R.zero_v = 0;
+  if (R.P > 0.0) {
+    R.last_neg++;
+    if (R.last_neg > R.n_min) { R.fdt = fmin(R.fdt * R.f_inc, R.dt_max); R.alpha *= R.f_alpha; }
+  } else {
+    R.fdt *= R.f_dec;
+    R.alpha = R.alpha0;
+    R.last_neg = 0;
+    R.zero_v = 1;
+  }
+  R.E_old = R.E;
+}
+
+// The position part of the second enforce (it depends on the position only,
+// so it equals what k_ens_fire_b computed), then
+// v = (1 - alpha) v + alpha |v| F / |F| with the coefficients from before the
+// adaptation of dt and alpha; v = 0 after uphill motion.
+__global__ void k_ens_fire_mix(EState e, int b, const RepD* reps, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const RepD& R = reps[e.rep[s]];
+  if (R.frozen && !R.just_frozen) return;
+  PS q;
+  q.x = e.x[b][s]; q.y = e.y[b][s]; q.z = e.z[b][s];
+  q.vx = q.vy = q.vz = 0.0;
+  q.nx = 1.0; q.ny = 0.0; q.nz = 0.0;
+  q.dix = q.diy = q.diz = 0;
+  enforce(R.box, R.con, q);
+  e.x[b][s] = q.x; e.y[b][s] = q.y; e.z[b][s] = q.z;
+  if (R.frozen) return;
+  double ux = R.f1 * e.vx[s] + R.f2 * e.fx[s];
+  double uy = R.f1 * e.vy[s] + R.f2 * e.fy[s];
+  double uz = R.f1 * e.vz[s] + R.f2 * e.fz[s];
+  if (R.zero_v) { ux = 0.0; uy = 0.0; uz = 0.0; }
+  e.vx[s] = ux; e.vy[s] = uy; e.vz[s] = uz;
+}
+
+__global__ void k_ens_morton(EState e, int b, const RepD* reps, unsigned long long* keys, int* slots, int n) {
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const int r = e.rep[s];
+    int i, j, k;
+    cell_ijk(reps[r], e.x[b][s], e.y[b][s], e.z[b][s], i, j, k);
+    unsigned long long m = 0;
+    for (int q = 0; q < 10; ++q)
+      m |= ((unsigned long long)((i >> q) & 1) << (3 * q)) | ((unsigned long long)((j >> q) & 1) << (3 * q + 1)) |
+           ((unsigned long long)((k >> q) & 1) << (3 * q + 2));
+    keys[s] = ((unsigned long long)r << 32) | m;
+    slots[s] = s;
+  }
+}
+
+int grid_cap(long n) {
+  const int g = div_up(n, kBlock);
+  return g < 1 ? 1 : (g > 2048 ? 2048 : g);
+}
+
+}  // namespace
+}  // namespace samos
+
+using namespace samos;
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+struct samos_ens {
+  hipStream_t st = nullptr;
+  std::string err;
+  int nrep = 0, n = 0, np = 0;
+  std::vector<RepD> h_rep;
+  std::vector<int> h_gid;        // slot -> replica-major input index
+  RepD* d_rep = nullptr;
+  EState e{};
+  int cur = 0;                   // buffer holding the current positions and directors
+  NListD nl{};
+  bool has_nlist = false;
+  int* d_flags = nullptr;        // [0] rebuild needed, [2] overflow, [3] largest bucket, [4] builds,
+                                 // [5] largest neighbour count, [8] replicas still relaxing
+  int* rep_need = nullptr;
+  unsigned long long* d_call = nullptr;
+  double* cols = nullptr;        // 5 reduction columns of np
+  int batch = 4;
+  int kind = -1;                 // 0 Brownian, 1 FIRE
+  double rcut = 2.4, pad = 0.3;
+  // graphs: Brownian chunk (K steps) and FIRE chunk (check iterations)
+  hipGraphExec_t g_brown = nullptr;
+  int g_brown_cur = -1, g_brown_batch = 0;
+  hipGraphExec_t g_fire = nullptr;
+  int g_fire_k = 0, g_fire_cur = -1, g_fire_batch = 0;
+  std::vector<void*> owned;      // every device allocation
+};
+
+namespace {
+
+thread_local std::string g_ens_err;
+
+#define ECHK(expr)                                                                \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      E->err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr; \
+      return -2;                                                                  \
+    }                                                                             \
+  } while (0)
+
+template <class T>
+hipError_t ens_alloc(samos_ens* E, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t r = hipMalloc(&q, sizeof(T) * std::max<size_t>(count, 1));
+  if (r == hipSuccess) {
+    E->owned.push_back(q);
+    *p = static_cast<T*>(q);
+  }
+  return r;
+}
+
+void ens_free(samos_ens* E, void* p) {
+  if (!p) return;
+  auto it = std::find(E->owned.begin(), E->owned.end(), p);
+  if (it != E->owned.end()) E->owned.erase(it);
+  hipFree(p);
+}
+
+void drop_graphs(samos_ens* E) {
+  if (E->g_brown) { hipGraphExecDestroy(E->g_brown); E->g_brown = nullptr; }
+  if (E->g_fire) { hipGraphExecDestroy(E->g_fire); E->g_fire = nullptr; }
+}
+
+int ens_upload(samos_ens* E) {
+  ECHK(hipMemcpyAsync(E->d_rep, E->h_rep.data(), sizeof(RepD) * E->nrep, hipMemcpyHostToDevice, E->st));
+  ECHK(hipStreamSynchronize(E->st));
+  return 0;
+}
+
+int ens_download(samos_ens* E) {
+  ECHK(hipMemcpyAsync(E->h_rep.data(), E->d_rep, sizeof(RepD) * E->nrep, hipMemcpyDeviceToHost, E->st));
+  ECHK(hipStreamSynchronize(E->st));
+  return 0;
+}
+
+int rep_grid(const samos_ens* E) { return div_up(std::max(E->nrep, 1), kBlock); }
+
+// Rebuild stages for the replicas flagged in rep_need, gated by *need.
+void ens_rebuild(samos_ens* E, const int* need) {
+  const int b = E->cur;
+  const int g = grid_cap(E->n);
+  hipLaunchKernelGGL(k_ens_bin, dim3(g), dim3(kBlock), 0, E->st, E->e, b, E->d_rep, E->rep_need, E->nl, E->n, need);
+  launch_scan(E->st, E->nl, need);
+  hipLaunchKernelGGL(k_ens_scatter, dim3(g), dim3(kBlock), 0, E->st, E->e, E->rep_need, E->nl, E->n, need);
+  launch_cell_sort(E->st, E->nl, need);
+  hipLaunchKernelGGL(k_ens_build, dim3(g), dim3(kBlock), 0, E->st, E->e, b, E->d_rep, E->rep_need, E->nl, E->n, need);
+}
+
+// Builds every replica's list, growing the capacity until nothing overflows.
+int ens_build_all(samos_ens* E) {
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    hipLaunchKernelGGL(k_ens_set_need, dim3(rep_grid(E)), dim3(kBlock), 0, E->st, E->rep_need, E->nrep, E->d_flags);
+    ECHK(hipMemsetAsync(E->d_flags + 2, 0, 2 * sizeof(int), E->st));
+    ECHK(hipMemsetAsync(E->d_flags + 5, 0, sizeof(int), E->st));
+    ens_rebuild(E, E->d_flags);
+    ECHK(hipGetLastError());
+    int h[8];
+    ECHK(hipMemcpyAsync(h, E->d_flags, 8 * sizeof(int), hipMemcpyDeviceToHost, E->st));
+    ECHK(hipStreamSynchronize(E->st));
+    if (h[2] == 0 && E->nl.stride >= (int)(h[5] * 1.25) + 8) {
+      hipLaunchKernelGGL(k_ens_clear_need, dim3(rep_grid(E)), dim3(kBlock), 0, E->st, E->rep_need, E->nrep,
+                         E->d_flags);
+      ECHK(hipStreamSynchronize(E->st));
+      return 0;
+    }
+    drop_graphs(E);
+    ens_free(E, E->nl.nbr);
+    E->nl.nbr = nullptr;
+    E->nl.stride = (int)std::ceil(h[5] * 1.5) + 16;
+    ECHK(ens_alloc(E, &E->nl.nbr, (size_t)E->nl.stride * E->np));
+  }
+  E->err = "neighbour list capacity could not be established";
+  return -1;
+}
+
+int check_run_flags(samos_ens* E) {
+  int h[8];
+  ECHK(hipMemcpyAsync(h, E->d_flags, 8 * sizeof(int), hipMemcpyDeviceToHost, E->st));
+  ECHK(hipStreamSynchronize(E->st));
+  if (h[2]) {
+    // a list overflowed during the run: grow it for the next run and report
+    ens_build_all(E);
+    E->err = "neighbour list overflow during the run (capacity grown; repeat the run from a saved state)";
+    return -1;
+  }
+  return 0;
+}
+
+double* ens_column(samos_ens* E, int field) {
+  const int b = E->cur;
+  EState& e = E->e;
+  switch (field) {
+    case SAMOS_X: return e.x[b];
+    case SAMOS_Y: return e.y[b];
+    case SAMOS_Z: return e.z[b];
+    case SAMOS_NX: return e.nx[b];
+    case SAMOS_NY: return e.ny[b];
+    case SAMOS_NZ: return e.nz[b];
+    case SAMOS_VX: return e.vx;
+    case SAMOS_VY: return e.vy;
+    case SAMOS_VZ: return e.vz;
+    case SAMOS_FX: return e.fx;
+    case SAMOS_FY: return e.fy;
+    case SAMOS_FZ: return e.fz;
+    case SAMOS_TAUX: return e.tx;
+    case SAMOS_TAUY: return e.ty;
+    case SAMOS_TAUZ: return e.tz;
+    case SAMOS_OMEGA: return e.omega;
+    case SAMOS_AGE: return e.age;
+    case SAMOS_RADIUS: return e.radius;
+  }
+  return nullptr;
+}
+
+// One Brownian iteration: j is the iteration's offset from the device call
+// counter, which advances at the end of each chunk.
+void brownian_iter(samos_ens* E, int j, int head, int last) {
+  hipLaunchKernelGGL(k_ens_clear_need, dim3(rep_grid(E)), dim3(kBlock), 0, E->st, E->rep_need, E->nrep, E->d_flags);
+  const dim3 g(grid1(E->n)), bl(kBlock);
+  if (E->batch == 8)
+    hipLaunchKernelGGL(k_ens_brownian<8>, g, bl, 0, E->st, E->e, E->cur, E->d_rep, E->nl, E->rep_need, E->d_flags,
+                       E->n, E->d_call, j, head, last);
+  else
+    hipLaunchKernelGGL(k_ens_brownian<4>, g, bl, 0, E->st, E->e, E->cur, E->d_rep, E->nl, E->rep_need, E->d_flags,
+                       E->n, E->d_call, j, head, last);
+  E->cur ^= 1;
+  ens_rebuild(E, E->d_flags);
+}
+
+// One FIRE iteration; count_remaining clears the running-replica counter first
+// so that it holds this iteration's count afterwards.
+void fire_iter(samos_ens* E, bool count_remaining) {
+  const dim3 g(grid1(E->n)), bl(kBlock);
+  if (count_remaining) hipLaunchKernelGGL(k_ens_advance, dim3(1), dim3(64), 0, E->st, E->d_call, 0, E->d_flags + 8);
+  hipLaunchKernelGGL(k_ens_fire_a, g, bl, 0, E->st, E->e, E->cur, E->d_rep, E->n);
+  hipLaunchKernelGGL(k_ens_clear_need, dim3(rep_grid(E)), bl, 0, E->st, E->rep_need, E->nrep, E->d_flags);
+  if (E->batch == 8)
+    hipLaunchKernelGGL(k_ens_fire_b<8>, g, bl, 0, E->st, E->e, E->cur, E->d_rep, E->nl, E->cols, (long)E->np, E->n,
+                       E->rep_need, E->d_flags);
+  else
+    hipLaunchKernelGGL(k_ens_fire_b<4>, g, bl, 0, E->st, E->e, E->cur, E->d_rep, E->nl, E->cols, (long)E->np, E->n,
+                       E->rep_need, E->d_flags);
+  hipLaunchKernelGGL(k_ens_fire_reduce, dim3(E->nrep), bl, 0, E->st, E->cols, (long)E->np, E->d_rep);
+  hipLaunchKernelGGL(k_ens_fire_update, dim3(rep_grid(E)), bl, 0, E->st, E->d_rep, E->nrep, E->d_flags + 8);
+  hipLaunchKernelGGL(k_ens_fire_mix, g, bl, 0, E->st, E->e, E->cur, E->d_rep, E->n);
+  ens_rebuild(E, E->d_flags);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* samos_ens_last_error(const samos_ens* E) { return E ? E->err.c_str() : g_ens_err.c_str(); }
+
+int samos_ens_create(samos_ens** out, int device, int nrep, const int* counts, const double* boxes,
+                     const int* periodic, int constraint, const double* con_param) {
+  if (!out) { g_ens_err = "null output handle"; return -1; }
+  *out = nullptr;
+  if (nrep <= 0 || !counts || !boxes || !periodic) { g_ens_err = "no replicas"; return -1; }
+  if (constraint < 0 || constraint > 2) { g_ens_err = "unknown constraint"; return -1; }
+  if (constraint == 2 && !con_param) { g_ens_err = "sphere radii missing"; return -1; }
+  long long total = 0;
+  for (int r = 0; r < nrep; ++r) {
+    if (counts[r] <= 0) { g_ens_err = "replica " + std::to_string(r) + " has no particles"; return -1; }
+    total += counts[r];
+  }
+  if (total >= (1LL << 30)) { g_ens_err = "too many particles"; return -1; }
+  if (hipSetDevice(device) != hipSuccess) { g_ens_err = "hipSetDevice failed"; return -2; }
+  samos_ens* E = new samos_ens();
+  E->nrep = nrep;
+  E->n = (int)total;
+  E->np = ((E->n + 63) / 64) * 64;
+  E->h_rep.assign(nrep, RepD{});
+  long long n0 = 0;
+  for (int r = 0; r < nrep; ++r) {
+    RepD& R = E->h_rep[r];
+    R.n0 = (int)n0; R.n = counts[r];
+    n0 += counts[r];
+    BoxD& B = R.box;
+    B.Lx = boxes[3 * r]; B.Ly = boxes[3 * r + 1]; B.Lz = boxes[3 * r + 2];
+    B.xlo = -0.5 * B.Lx; B.xhi = 0.5 * B.Lx; B.ylo = -0.5 * B.Ly; B.yhi = 0.5 * B.Ly;
+    B.zlo = -0.5 * B.Lz; B.zhi = 0.5 * B.Lz;
+    B.periodic = periodic[r] ? 1 : 0;
+    R.con.kind = constraint;
+    R.con.r = constraint == 2 ? con_param[r] : 0.0;
+    R.con.zpos = (constraint == 1 && con_param) ? con_param[r] : 0.0;
+    R.con.unlimited = 0;
+    R.k = 1.0; R.a = 2.0; R.mu = 1.0; R.mur = 1.0; R.dt = 0.01;
+  }
+  const size_t np = E->np;
+  auto fail = [&](const char* what) {
+    g_ens_err = std::string("device allocation failed: ") + what;
+    samos_ens_destroy(E);
+    return -2;
+  };
+  if (hipStreamCreateWithFlags(&E->st, hipStreamNonBlocking) != hipSuccess) return fail("stream");
+  EState& e = E->e;
+  double** dcols[] = {&e.x[0], &e.y[0], &e.z[0], &e.nx[0], &e.ny[0], &e.nz[0], &e.x[1], &e.y[1], &e.z[1],
+                      &e.nx[1], &e.ny[1], &e.nz[1], &e.vx, &e.vy, &e.vz, &e.fx, &e.fy, &e.fz, &e.tx,
+                      &e.ty, &e.tz, &e.omega, &e.age, &e.radius};
+  for (double** c : dcols) {
+    if (ens_alloc(E, c, np) != hipSuccess) return fail("columns");
+    hipMemsetAsync(*c, 0, sizeof(double) * np, E->st);
+  }
+  if (ens_alloc(E, &e.rep, np) != hipSuccess || ens_alloc(E, &e.lid, np) != hipSuccess ||
+      ens_alloc(E, &e.cellc, np) != hipSuccess)
+    return fail("index columns");
+  if (ens_alloc(E, &E->d_rep, nrep) != hipSuccess || ens_alloc(E, &E->rep_need, nrep) != hipSuccess ||
+      ens_alloc(E, &E->d_flags, 16) != hipSuccess || ens_alloc(E, &E->d_call, 1) != hipSuccess ||
+      ens_alloc(E, &E->cols, 5 * np) != hipSuccess)
+    return fail("replica state");
+  hipMemsetAsync(E->d_flags, 0, 16 * sizeof(int), E->st);
+  hipMemsetAsync(E->rep_need, 0, sizeof(int) * nrep, E->st);
+  hipMemsetAsync(E->d_call, 0, sizeof(unsigned long long), E->st);
+  std::vector<int> repof(E->n), lid(E->n);
+  E->h_gid.resize(E->n);
+  for (int r = 0; r < nrep; ++r)
+    for (int i = 0; i < counts[r]; ++i) {
+      const int s = E->h_rep[r].n0 + i;
+      repof[s] = r; lid[s] = i; E->h_gid[s] = s;
+    }
+  std::vector<double> ones(E->n, 1.0);
+  hipMemcpyAsync(e.rep, repof.data(), sizeof(int) * E->n, hipMemcpyHostToDevice, E->st);
+  hipMemcpyAsync(e.lid, lid.data(), sizeof(int) * E->n, hipMemcpyHostToDevice, E->st);
+  hipMemcpyAsync(e.radius, ones.data(), sizeof(double) * E->n, hipMemcpyHostToDevice, E->st);
+  if (hipStreamSynchronize(E->st) != hipSuccess) return fail("initialisation");
+  if (ens_upload(E)) return fail("replica table");
+  *out = E;
+  return 0;
+}
+
+int samos_ens_destroy(samos_ens* E) {
+  if (!E) return 0;
+  if (E->st) hipStreamSynchronize(E->st);
+  drop_graphs(E);
+  for (void* p : E->owned) hipFree(p);
+  if (E->st) hipStreamDestroy(E->st);
+  delete E;
+  return 0;
+}
+
+int samos_ens_num_particles(const samos_ens* E) { return E ? E->n : -1; }
+
+int samos_ens_set_field(samos_ens* E, int field, const double* host) {
+  double* d = ens_column(E, field);
+  if (!d) { E->err = "field " + std::to_string(field) + " is not held by an ensemble"; return -1; }
+  std::vector<double> tmp(E->n);
+  for (int s = 0; s < E->n; ++s) tmp[s] = host[E->h_gid[s]];
+  ECHK(hipMemcpyAsync(d, tmp.data(), sizeof(double) * E->n, hipMemcpyHostToDevice, E->st));
+  ECHK(hipStreamSynchronize(E->st));
+  return 0;
+}
+
+int samos_ens_get_field(samos_ens* E, int field, double* host) {
+  double* d = ens_column(E, field);
+  if (!d) { E->err = "field " + std::to_string(field) + " is not held by an ensemble"; return -1; }
+  std::vector<double> tmp(E->n);
+  ECHK(hipMemcpyAsync(tmp.data(), d, sizeof(double) * E->n, hipMemcpyDeviceToHost, E->st));
+  ECHK(hipStreamSynchronize(E->st));
+  for (int s = 0; s < E->n; ++s) host[E->h_gid[s]] = tmp[s];
+  return 0;
+}
+
+int samos_ens_set_params(samos_ens* E, const double* p) {
+  for (int r = 0; r < E->nrep; ++r) {
+    const double* q = p + 12 * r;
+    if (!(q[10] > 0.0)) { E->err = "replica " + std::to_string(r) + ": timestep must be positive"; return -1; }
+    if (q[8] < 0.0) { E->err = "replica " + std::to_string(r) + ": nu must be non-negative"; return -1; }
+    if (q[0] < 0.0 || (q[2] == 0.0 && q[1] <= 0.0)) {
+      E->err = "replica " + std::to_string(r) + ": soft potential needs k >= 0 and a > 0";
+      return -1;
+    }
+  }
+  for (int r = 0; r < E->nrep; ++r) {
+    const double* q = p + 12 * r;
+    RepD& R = E->h_rep[r];
+    R.k = q[0]; R.a = q[1]; R.radii = q[2] != 0.0; R.J = q[3];
+    R.has_al = (q[4] > 0.0 && q[3] != 0.0) ? 1 : 0;
+    R.rc2_al = q[4] * q[4];
+    R.v0 = q[5]; R.mu = q[6]; R.mur = q[7];
+    R.seed = (uint32_t)(unsigned long long)q[9];
+    R.dt = q[10];
+    R.stoch = std::sqrt(q[8] * q[10]);
+  }
+  drop_graphs(E);
+  return ens_upload(E);
+}
+
+int samos_ens_nlist(samos_ens* E, double rcut, double pad) {
+  if (!(rcut > 0.0) || pad < 0.0) { E->err = "nlist needs rcut > 0 and pad >= 0"; return -1; }
+  if (E->has_nlist) { E->err = "the ensemble already has a neighbour list"; return -1; }
+  const double cut = rcut + pad;
+  long long nb_total = 0;
+  for (int r = 0; r < E->nrep; ++r) {
+    const RepD& R = E->h_rep[r];
+    const int ncx = (int)(R.box.Lx / cut), ncy = (int)(R.box.Ly / cut), ncz = (int)(R.box.Lz / cut);
+    if (ncx < 3 || ncy < 3 || ncz < 3) {
+      E->err = "replica " + std::to_string(r) + ": every box side must hold at least three cells of rcut + pad";
+      return -1;
+    }
+    if (ncx > 1024 || ncy > 1024 || ncz > 1024) {
+      E->err = "replica " + std::to_string(r) + ": more than 1024 cells per side";
+      return -1;
+    }
+  }
+  for (int r = 0; r < E->nrep; ++r) {
+    RepD& R = E->h_rep[r];
+    R.ncx = (int)(R.box.Lx / cut); R.ncy = (int)(R.box.Ly / cut); R.ncz = (int)(R.box.Lz / cut);
+    R.wx = R.box.Lx / R.ncx; R.wy = R.box.Ly / R.ncy; R.wz = R.box.Lz / R.ncz;
+    R.cut2 = cut * cut;
+    R.pad2q = 0.25 * pad * pad;
+    int h = 1;
+    while ((1LL << h) < R.n) ++h;
+    R.hbits = h;
+    R.b0 = (int)nb_total;
+    nb_total += 1LL << h;
+  }
+  if (nb_total >= (1LL << 30)) { E->err = "bucket table too large"; return -1; }
+  E->rcut = rcut; E->pad = pad;
+  NListD& nl = E->nl;
+  nl = NListD{};
+  nl.ncell = (int)nb_total;
+  nl.use_cell = 1;
+  nl.np = E->np;
+  nl.flags = E->d_flags;
+  nl.cut2 = cut * cut;
+  nl.pad2q = 0.25 * pad * pad;
+  const size_t np = E->np;
+  ECHK(ens_alloc(E, &nl.count, nb_total));
+  ECHK(ens_alloc(E, &nl.start, nb_total + 1));
+  ECHK(ens_alloc(E, &nl.bsum, nb_total / 4096 + 2));
+  ECHK(ens_alloc(E, &nl.cell_of, np));
+  ECHK(ens_alloc(E, &nl.members, np));
+  ECHK(ens_alloc(E, &nl.nnb, np));
+  ECHK(ens_alloc(E, &nl.ox, np));
+  ECHK(ens_alloc(E, &nl.oy, np));
+  ECHK(ens_alloc(E, &nl.oz, np));
+  ECHK(hipMemsetAsync(nl.count, 0, sizeof(int) * nb_total, E->st));
+  ECHK(hipMemsetAsync(nl.nnb, 0, sizeof(int) * np, E->st));
+  nl.stride = 32;
+  ECHK(ens_alloc(E, &nl.nbr, (size_t)nl.stride * np));
+  if (int rc = ens_upload(E)) return rc;
+  // Morton re-sort inside each replica: key = (replica, Morton code of the cell)
+  {
+    unsigned long long *k1 = nullptr, *k2 = nullptr;
+    int *p1 = nullptr, *p2 = nullptr, *itmp = nullptr;
+    double* dtmp = nullptr;
+    char* tmp = nullptr;
+    const size_t tb = radix_sort_temp_bytes(E->n);
+    ECHK(ens_alloc(E, &k1, np));
+    ECHK(ens_alloc(E, &k2, np));
+    ECHK(ens_alloc(E, &p1, np));
+    ECHK(ens_alloc(E, &p2, np));
+    ECHK(ens_alloc(E, &itmp, np));
+    ECHK(ens_alloc(E, &dtmp, np));
+    ECHK(ens_alloc(E, &tmp, tb + 256));
+    hipLaunchKernelGGL(k_ens_morton, dim3(grid_cap(E->n)), dim3(kBlock), 0, E->st, E->e, E->cur, E->d_rep, k1, p1,
+                       E->n);
+    radix_sort_pairs(E->st, tmp, tb, k1, k2, p1, p2, E->n);
+    EState& e = E->e;
+    const int b = E->cur;
+    double* cols[] = {e.x[b], e.y[b], e.z[b], e.nx[b], e.ny[b], e.nz[b], e.vx, e.vy, e.vz, e.fx, e.fy,
+                      e.fz, e.tx, e.ty, e.tz, e.omega, e.age, e.radius};
+    for (double* c : cols) {
+      launch_gather_d(E->st, dtmp, c, p2, E->n);
+      ECHK(hipMemcpyAsync(c, dtmp, sizeof(double) * E->n, hipMemcpyDeviceToDevice, E->st));
+    }
+    launch_gather_i(E->st, itmp, e.lid, p2, E->n);
+    ECHK(hipMemcpyAsync(e.lid, itmp, sizeof(int) * E->n, hipMemcpyDeviceToDevice, E->st));
+    std::vector<int> perm(E->n);
+    ECHK(hipMemcpyAsync(perm.data(), p2, sizeof(int) * E->n, hipMemcpyDeviceToHost, E->st));
+    ECHK(hipStreamSynchronize(E->st));
+    const std::vector<int> old = E->h_gid;
+    for (int s = 0; s < E->n; ++s) E->h_gid[s] = old[perm[s]];
+    for (void* q : {(void*)k1, (void*)k2, (void*)p1, (void*)p2, (void*)itmp, (void*)dtmp, (void*)tmp}) ens_free(E, q);
+  }
+  E->has_nlist = true;
+  return ens_build_all(E);
+}
+
+int samos_ens_nlist_info(samos_ens* E, long long* out) {
+  int h[8];
+  ECHK(hipMemcpyAsync(h, E->d_flags, 8 * sizeof(int), hipMemcpyDeviceToHost, E->st));
+  ECHK(hipStreamSynchronize(E->st));
+  out[0] = h[4]; out[1] = E->nl.stride; out[2] = E->nl.ncell; out[3] = h[3]; out[4] = h[5];
+  out[5] = E->n; out[6] = E->nrep;
+  return 0;
+}
+
+int samos_ens_pairs(samos_ens* E, int rep, int* out, long long cap, long long* count) {
+  if (!E->has_nlist) { E->err = "no neighbour list"; return -1; }
+  if (rep < 0 || rep >= E->nrep) { E->err = "replica out of range"; return -1; }
+  const RepD& R = E->h_rep[rep];
+  std::vector<int> nnb(R.n);
+  ECHK(hipMemcpy(nnb.data(), E->nl.nnb + R.n0, sizeof(int) * R.n, hipMemcpyDeviceToHost));
+  std::vector<int> nbr((size_t)E->nl.stride * R.n);
+  for (int k = 0; k < E->nl.stride; ++k)
+    ECHK(hipMemcpy(nbr.data() + (size_t)k * R.n, E->nl.nbr + (size_t)k * E->np + R.n0, sizeof(int) * R.n,
+                   hipMemcpyDeviceToHost));
+  std::vector<std::pair<int, int>> pr;
+  for (int i = 0; i < R.n; ++i) {
+    const int gi = E->h_gid[R.n0 + i] - R.n0;
+    for (int k = 0; k < nnb[i]; ++k) {
+      const int t = nbr[(size_t)k * R.n + i];
+      const int gt = E->h_gid[t] - R.n0;
+      if (gt > gi) pr.push_back({gi, gt});
+    }
+  }
+  std::sort(pr.begin(), pr.end());
+  *count = (long long)pr.size();
+  if (out) {
+    if ((long long)pr.size() > cap) { E->err = "output capacity too small"; return -1; }
+    for (size_t k = 0; k < pr.size(); ++k) { out[2 * k] = pr[k].first; out[2 * k + 1] = pr[k].second; }
+  }
+  return 0;
+}
+
+int samos_ens_set_option(samos_ens* E, const char* name, double value) {
+  const std::string k = name ? name : "";
+  if (k == "gather_batch") {
+    if (value != 4.0 && value != 8.0) { E->err = "gather_batch must be 4 or 8"; return -1; }
+    E->batch = (int)value;
+    drop_graphs(E);
+    return 0;
+  }
+  E->err = "unknown option '" + k + "'";
+  return -1;
+}
+
+int samos_ens_enforce(samos_ens* E) {
+  hipLaunchKernelGGL(k_ens_enforce, dim3(grid1(E->n)), dim3(kBlock), 0, E->st, E->e, E->cur, E->d_rep, E->n);
+  ECHK(hipGetLastError());
+  ECHK(hipStreamSynchronize(E->st));
+  return 0;
+}
+
+int samos_ens_forces(samos_ens* E, double* energies) {
+  if (!E->has_nlist) { E->err = "no neighbour list"; return -1; }
+  hipLaunchKernelGGL(k_ens_forces, dim3(grid1(E->n)), dim3(kBlock), 0, E->st, E->e, E->cur, E->d_rep, E->nl, E->cols,
+                     E->n);
+  ECHK(hipGetLastError());
+  std::vector<double> en(E->n);
+  ECHK(hipMemcpyAsync(en.data(), E->cols, sizeof(double) * E->n, hipMemcpyDeviceToHost, E->st));
+  ECHK(hipStreamSynchronize(E->st));
+  if (energies)
+    for (int r = 0; r < E->nrep; ++r) {
+      double t = 0.0;
+      for (int s = E->h_rep[r].n0; s < E->h_rep[r].n0 + E->h_rep[r].n; ++s) t += en[s];
+      energies[r] = 0.5 * t;
+    }
+  return 0;
+}
+
+int samos_ens_run_brownian(samos_ens* E, int steps, int flags) {
+  if (!E->has_nlist) { E->err = "no neighbour list"; return -1; }
+  if (steps < 0) { E->err = "negative step count"; return -1; }
+  E->kind = 0;
+  ECHK(hipMemsetAsync(E->d_flags + 2, 0, sizeof(int), E->st));
+  hipLaunchKernelGGL(k_ens_enforce, dim3(grid1(E->n)), dim3(kBlock), 0, E->st, E->e, E->cur, E->d_rep, E->n);
+  const int iters = steps + 1;
+  const int K = 32;  // even: a chunk returns the buffers to their parity
+  int t = 0;
+  if (!(flags & SAMOS_RUN_NO_GRAPH) && iters > K) {
+    if (E->g_brown && (E->g_brown_cur != E->cur || E->g_brown_batch != E->batch)) {
+      hipGraphExecDestroy(E->g_brown);
+      E->g_brown = nullptr;
+    }
+    if (!E->g_brown) {
+      const int cur0 = E->cur;
+      hipGraph_t g = nullptr;
+      ECHK(hipStreamBeginCapture(E->st, hipStreamCaptureModeThreadLocal));
+      for (int j = 0; j < K; ++j) brownian_iter(E, j, 1, 0);
+      hipLaunchKernelGGL(k_ens_advance, dim3(1), dim3(64), 0, E->st, E->d_call, K, (int*)nullptr);
+      ECHK(hipStreamEndCapture(E->st, &g));
+      const hipError_t ir = hipGraphInstantiate(&E->g_brown, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      E->cur = cur0;
+      ECHK(ir);
+      E->g_brown_cur = cur0;
+      E->g_brown_batch = E->batch;
+    }
+    while (iters - t > K) {
+      ECHK(hipGraphLaunch(E->g_brown, E->st));
+      t += K;
+    }
+  }
+  for (; t < iters; ++t) {
+    brownian_iter(E, 0, t + 1 < iters, t + 1 == iters);
+    hipLaunchKernelGGL(k_ens_advance, dim3(1), dim3(64), 0, E->st, E->d_call, 1, (int*)nullptr);
+  }
+  hipLaunchKernelGGL(k_ens_age, dim3(grid1(E->n)), dim3(kBlock), 0, E->st, E->e, E->d_rep, E->n, iters);
+  ECHK(hipGetLastError());
+  ECHK(hipStreamSynchronize(E->st));
+  return check_run_flags(E);
+}
+
+int samos_ens_fire_setup(samos_ens* E, double dt0, double alpha, double alpha_scale, double f_tol, double e_tol,
+                         double dt_inc, double dt_dec, double dt_max, int min_neg, double stop_fmax) {
+  if (!E->has_nlist) { E->err = "no neighbour list"; return -1; }
+  if (!(dt0 > 0.0)) { E->err = "FIRE timestep must be positive"; return -1; }
+  if (int rc = ens_download(E)) return rc;
+  for (const auto& R : E->h_rep)
+    if (R.has_al) { E->err = "FIRE ensembles take the soft potential only (no aligner)"; return -1; }
+  for (auto& R : E->h_rep) {
+    R.fdt = dt0; R.alpha = alpha; R.alpha0 = alpha; R.f_alpha = alpha_scale; R.f_tol = f_tol; R.e_tol = e_tol;
+    R.f_inc = dt_inc; R.f_dec = dt_dec; R.dt_max = dt_max; R.n_min = min_neg; R.last_neg = 0; R.frozen = 0;
+    R.E_old = 1e15; R.iters = 0; R.just_frozen = 0; R.stop_fmax = stop_fmax; R.f1 = 1.0; R.f2 = 0.0; R.zero_v = 0;
+    R.P = R.F2 = R.V2 = R.E = 0.0; R.Fmax = INFINITY;
+  }
+  E->kind = 1;
+  if (int rc = ens_upload(E)) return rc;
+  // the run loop's first head: enforce, then forces at the enforced positions
+  hipLaunchKernelGGL(k_ens_enforce, dim3(grid1(E->n)), dim3(kBlock), 0, E->st, E->e, E->cur, E->d_rep, E->n);
+  hipLaunchKernelGGL(k_ens_forces, dim3(grid1(E->n)), dim3(kBlock), 0, E->st, E->e, E->cur, E->d_rep, E->nl,
+                     (double*)nullptr, E->n);
+  ECHK(hipGetLastError());
+  ECHK(hipStreamSynchronize(E->st));
+  return 0;
+}
+
+int samos_ens_fire_run(samos_ens* E, int max_iter, int check, int* remaining, int* iters) {
+  if (E->kind != 1) { E->err = "call samos_ens_fire_setup first"; return -1; }
+  if (check < 1) check = 1;
+  ECHK(hipMemsetAsync(E->d_flags + 2, 0, sizeof(int), E->st));
+  int done = 0, rem = E->nrep;
+  const bool graph = check >= 4;
+  if (graph && (!E->g_fire || E->g_fire_k != check || E->g_fire_cur != E->cur || E->g_fire_batch != E->batch)) {
+    if (E->g_fire) { hipGraphExecDestroy(E->g_fire); E->g_fire = nullptr; }
+    hipGraph_t g = nullptr;
+    ECHK(hipStreamBeginCapture(E->st, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < check; ++j) fire_iter(E, j == check - 1);
+    ECHK(hipStreamEndCapture(E->st, &g));
+    const hipError_t ir = hipGraphInstantiate(&E->g_fire, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    ECHK(ir);
+    E->g_fire_k = check;
+    E->g_fire_cur = E->cur;
+    E->g_fire_batch = E->batch;
+  }
+  while (done < max_iter) {
+    const int chunk = std::min(check, max_iter - done);
+    if (graph && chunk == check) {
+      ECHK(hipGraphLaunch(E->g_fire, E->st));
+    } else {
+      for (int j = 0; j < chunk; ++j) fire_iter(E, j == chunk - 1);
+    }
+    done += chunk;
+    ECHK(hipMemcpyAsync(&rem, E->d_flags + 8, sizeof(int), hipMemcpyDeviceToHost, E->st));
+    ECHK(hipStreamSynchronize(E->st));
+    if (rem == 0) break;
+  }
+  ECHK(hipGetLastError());
+  if (remaining) *remaining = rem;
+  if (iters) *iters = done;
+  return check_run_flags(E);
+}
+
+int samos_ens_fire_state(samos_ens* E, double* out) {
+  if (int rc = ens_download(E)) return rc;
+  for (int r = 0; r < E->nrep; ++r) {
+    const RepD& R = E->h_rep[r];
+    double* o = out + 8 * r;
+    o[0] = (double)R.iters; o[1] = R.frozen; o[2] = R.E; o[3] = R.Fmax;
+    o[4] = std::sqrt(R.F2); o[5] = R.fdt; o[6] = R.alpha; o[7] = R.P;
+  }
+  return 0;
+}
+
+int samos_ens_sync(samos_ens* E) {
+  ECHK(hipStreamSynchronize(E->st));
+  ECHK(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

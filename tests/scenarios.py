@@ -71,3 +71,57 @@ def rel_force_err(a, b):
     """max_i |a_i - b_i| / max_i |b_i| over stacked (n, 3) arrays."""
     scale = max(float(np.max(np.linalg.norm(b, axis=1))), 1e-300)
     return float(np.max(np.linalg.norm(a - b, axis=1))) / scale
+
+
+def sphere_replicas(counts, seed=40, phi_range=(0.5, 1.2)):
+    """One sphere_disks state per replica, packing fraction uniform in phi_range."""
+    rng = np.random.default_rng(seed)
+    phis = rng.uniform(phi_range[0], phi_range[1], len(counts))
+    return [sphere_disks(int(n), phi=float(p), seed=seed * 1000 + i) for i, (n, p) in enumerate(zip(counts, phis))]
+
+
+def plane_replicas(counts, seed=50, phi=0.88, polydisperse=(0.8, 1.2)):
+    """Periodic polydisperse plane_disks states, one per replica."""
+    return [plane_disks(int(n), phi=phi, seed=seed * 1000 + i, polydisperse=polydisperse)
+            for i, n in enumerate(counts)]
+
+
+def active_replica_params(nrep, seed=60):
+    """Per-replica active-matter parameters: v0 log-uniform in [0.01, 1], J
+    uniform in [0, 1], rotational diffusion nu log-uniform in [0.01, 1]."""
+    rng = np.random.default_rng(seed)
+    return {
+        "v0": np.exp(rng.uniform(math.log(0.01), math.log(1.0), nrep)),
+        "J": rng.uniform(0.0, 1.0, nrep),
+        "nu": np.exp(rng.uniform(math.log(0.01), math.log(1.0), nrep)),
+        "seed": rng.integers(1, 2**31 - 1, nrep).astype(np.float64),
+    }
+
+
+ENSEMBLE_COLUMNS = ["x", "y", "z", "vx", "vy", "vz", "nx", "ny", "nz", "radius"]
+
+
+def load_ensemble(cls, states, constraint):
+    """Concatenate per-replica states into an Ensemble (replica-major columns)."""
+    counts = [st["n"] for st in states]
+    boxes = [st["box"] for st in states]
+    periodic = [st["periodic"] for st in states]
+    con = [st["R"] for st in states] if constraint == "sphere" else None
+    e = cls(counts, boxes, periodic, constraint, con)
+    for k in ENSEMBLE_COLUMNS:
+        parts = [st["cols"][k] if k in st["cols"] else (np.ones(st["n"]) if k == "radius" else np.zeros(st["n"]))
+                 for st in states]
+        e.set(k, np.concatenate(parts))
+    return e
+
+
+def c4_ensemble_workload(nrep=256, seed=2025, nmin=1024, nmax=65536):
+    """Batched active-sphere workload: per-replica N log-uniform in [nmin, nmax],
+    packing fraction uniform in [0.5, 1.2], R = sqrt(N / (4 phi)); v0, J and nu
+    as active_replica_params; k = 10, a = 2, polar rcut 2.4, dt = 0.005."""
+    rng = np.random.default_rng(seed)
+    counts = np.round(np.exp(rng.uniform(math.log(nmin), math.log(nmax), nrep))).astype(int)
+    states = sphere_replicas(counts, seed=seed)
+    p = active_replica_params(nrep, seed=seed + 1)
+    params = dict(k=10.0, a=2.0, rcut_align=2.4, mu=1.0, mur=1.0, dt=0.005, **p)
+    return states, params
