@@ -21,6 +21,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -114,6 +116,160 @@ def copy_probe(mib, reps=10):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
 
 
+ENS_BYTES_PER_PARTICLE_STEP = 120   # read x, y, z, nx, ny, nz; write those and vx, vy, vz (FP64)
+FIRE_BYTES_PER_PARTICLE_ITER = 72   # read x, y, vx, vy, r; write x, y, vx, vy (FP64)
+
+
+def plain_replica_system(samos_amd, sc, st, p, r):
+    """Replica r as its own single-system run (the drop-in path, one conf per replica)."""
+    s = sc.load(samos_amd.System, st)
+    s.nlist(2.4, 0.3)
+    s.pair_soft(p["k"], p["a"])
+    if p["J"][r] != 0.0:
+        s.align_polar(float(p["J"][r]), p["rcut_align"])
+    s.constraint("sphere", st["R"])
+    s.set_timestep(p["dt"])
+    s.integrator_brownian(v0=float(p["v0"][r]), nu=float(p["nu"][r]), mu=p["mu"], mur=p["mur"],
+                          seed=int(p["seed"][r]))
+    return s
+
+
+def bench_ensemble(samos_amd, sc, steps, plain_steps, nrep=256, pad=None):
+    """Batched active-sphere ensemble: every replica advanced by one fused launch
+    per step, against the same replicas run one at a time."""
+    states, p = sc.c4_ensemble_workload(nrep=nrep)
+    counts = np.array([st["n"] for st in states])
+    n_tot = int(counts.sum())
+    e = sc.load_ensemble(samos_amd.Ensemble, states, "sphere")
+    e.set_params(**p)
+    if pad is None:
+        e.nlist(2.4, 0.3)
+        tune = e.autotune(trial_steps=128, pads=(0.3, 0.45, 0.6, 0.8, 1.0))
+    else:
+        e.nlist(2.4, pad)
+        tune = None
+    print("bench: ensemble %d replicas, %d particles, autotune %s" % (nrep, n_tot, tune), file=sys.stderr, flush=True)
+    e.run_brownian(63)
+    e.sync()
+    t0 = time.perf_counter()
+    e.run_brownian(steps - 1)
+    e.sync()
+    secs = time.perf_counter() - t0
+    value = n_tot * steps / secs
+    print("bench: ensemble %d steps in %.3f s -> %.4e particle-steps/s" % (steps, secs, value), file=sys.stderr,
+          flush=True)
+    e.profile(True)
+    e.run_brownian(199)
+    ms, launches = e.profile_read("step")
+    rb_ms, rb_launches = e.profile_read("rebuild")
+    e.profile(False)
+    info = e.nlist_info()
+    e.close()
+    avg_s = ms / 1e3 / max(launches, 1)
+    achieved = ENS_BYTES_PER_PARTICLE_STEP * n_tot / avg_s / 1e9
+    plain_value = one_value = float("nan")
+    if plain_steps > 0:
+        # the same replicas as plain single-system runs, one after another
+        plain_t, plain_ps = 0.0, 0
+        for r, st in enumerate(states):
+            s = plain_replica_system(samos_amd, sc, st, p, r)
+            s.run(9)
+            s.sync()
+            t0 = time.perf_counter()
+            s.run(plain_steps - 1)
+            s.sync()
+            plain_t += time.perf_counter() - t0
+            plain_ps += st["n"] * plain_steps
+            s.close()
+        plain_value = plain_ps / plain_t
+        # the ensemble engine on one replica at a time (batching gain alone)
+        one_t, one_ps = 0.0, 0
+        for r, st in enumerate(states):
+            q = {k: (v[r:r + 1] if isinstance(v, np.ndarray) else v) for k, v in p.items()}
+            e1 = sc.load_ensemble(samos_amd.Ensemble, [st], "sphere")
+            e1.set_params(**q)
+            e1.nlist(2.4, tune["pad"] if tune else pad)
+            if tune:
+                e1.set_option("gather_batch", tune["gather_batch"])
+            e1.run_brownian(63)
+            e1.sync()
+            t0 = time.perf_counter()
+            e1.run_brownian(plain_steps - 1)
+            e1.sync()
+            one_t += time.perf_counter() - t0
+            one_ps += st["n"] * plain_steps
+            e1.close()
+        one_value = one_ps / one_t
+        print("bench: plain per-replica %.4e, ensemble engine one replica at a time %.4e particle-steps/s"
+              % (plain_value, one_value), file=sys.stderr, flush=True)
+    return {
+        "workload": "active_sphere_ensemble: %d replicas, N_r log-uniform in [1024, 65536] (total %d), "
+                    "phi_r U[0.5, 1.2], R_r = sqrt(N_r / (4 phi_r)); soft k=10 a=2, polar J U[0, 1] rcut 2.4, "
+                    "Brownian v0 log-U[0.01, 1], nu log-U[0.01, 1], dt 0.005; nlist rcut 2.4" % (nrep, n_tot),
+        "value": value,
+        "unit": "particle-steps/s",
+        "steps": steps,
+        "ms_per_step": secs * 1e3 / steps,
+        "autotune": tune,
+        "plain_per_replica_value": plain_value,
+        "plain_per_replica_steps": plain_steps,
+        "vs_plain_per_replica": value / plain_value,
+        "ensemble_engine_one_replica_at_a_time_value": one_value,
+        "vs_one_replica_at_a_time": value / one_value,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "kernel": "k_ens_brownian",
+                     "avg_launch_us": avg_s * 1e6, "launches_timed": launches,
+                     "alg_bytes_per_launch": ENS_BYTES_PER_PARTICLE_STEP * n_tot},
+        "nlist": {"builds": info["builds"], "stride": info["stride"], "buckets": info["buckets"],
+                  "rebuild_ms_total": rb_ms, "rebuild_sequences": rb_launches},
+    }
+
+
+def bench_fire(samos_amd, sc, nrep=512, n=4096, max_iter=400000):
+    """Plain batched FIRE to max|F| < 1e-10 on every replica."""
+    states = sc.plane_replicas([n] * nrep, seed=5)
+    e = sc.load_ensemble(samos_amd.Ensemble, states, "plane")
+    e.set_params(k=1.0, a=2.0, use_radii=1.0, dt=0.01)
+    e.nlist(2.4, 0.3)
+    e.fire_setup(dt=0.01, alpha=0.1, dt_max=0.1, f_tol=0.0, stop_fmax=1e-10)
+    e.sync()
+    t0 = time.perf_counter()
+    rem, iters = e.fire_run(max_iter, check=100)
+    e.sync()
+    secs = time.perf_counter() - t0
+    fs = e.fire_state()
+    particle_iters = float(np.sum(fs["iters"]) * n)
+    print("bench: FIRE %d x %d: %d iterations, %.3f s, %d not converged" % (nrep, n, iters, secs, rem),
+          file=sys.stderr, flush=True)
+    # force-pass kernel timing on a fresh copy of the same start
+    e.close()
+    e = sc.load_ensemble(samos_amd.Ensemble, states, "plane")
+    e.set_params(k=1.0, a=2.0, use_radii=1.0, dt=0.01)
+    e.nlist(2.4, 0.3)
+    e.fire_setup(dt=0.01, alpha=0.1, dt_max=0.1, f_tol=0.0, stop_fmax=1e-10)
+    e.profile(True)
+    e.fire_run(200, check=200)
+    ms, launches = e.profile_read("step")
+    e.profile(False)
+    e.close()
+    avg_s = ms / 1e3 / max(launches, 1)
+    achieved = FIRE_BYTES_PER_PARTICLE_ITER * nrep * n / avg_s / 1e9
+    return {
+        "workload": "fire_polydisperse: %d replicas x N=%d, radii U[0.8, 1.2], periodic, packing 0.88; soft k=1 "
+                    "use_particle_radii; FIRE dt0 0.01 dt_max 0.1 alpha 0.1, stop at max|F| < 1e-10" % (nrep, n),
+        "time_to_converge_s": secs,
+        "iterations": iters,
+        "not_converged": rem,
+        "iterations_per_replica": {"min": float(fs["iters"].min()), "median": float(np.median(fs["iters"])),
+                                   "max": float(fs["iters"].max())},
+        "particle_iterations_per_s": particle_iters / secs,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "kernel": "k_ens_fire_b",
+                     "avg_launch_us": avg_s * 1e6, "launches_timed": launches,
+                     "alg_bytes_per_launch": FIRE_BYTES_PER_PARTICLE_ITER * nrep * n},
+    }
+
+
 def traffic_from_profiles():
     """HBM bytes per launch of the fused step kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
@@ -140,7 +296,17 @@ def main():
     ap.add_argument("--batch", type=int, choices=[4, 8], default=4)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this size (HBM counter calibration) and exit")
+    ap.add_argument("--parts", default="main,ensemble,fire",
+                    help="comma list: main (the N=1M metric), ensemble (batched active spheres), "
+                         "fire (batched FIRE relaxation)")
+    ap.add_argument("--ensemble-steps", type=int, default=5000)
+    ap.add_argument("--ensemble-replicas", type=int, default=256)
+    ap.add_argument("--ensemble-pad", type=float, default=None, help="fixed skin instead of the autotuner")
+    ap.add_argument("--plain-steps", type=int, default=200,
+                    help="steps per replica when timing the replicas as separate runs")
+    ap.add_argument("--fire-replicas", type=int, default=512)
     args = ap.parse_args()
+    parts = set(args.parts.split(","))
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -151,6 +317,16 @@ def main():
 
     import samos_amd
     from tests import scenarios as sc
+
+    extras = {}
+    if "ensemble" in parts:
+        extras["ensemble"] = bench_ensemble(samos_amd, sc, args.ensemble_steps, args.plain_steps,
+                                            nrep=args.ensemble_replicas, pad=args.ensemble_pad)
+    if "fire" in parts:
+        extras["fire"] = bench_fire(samos_amd, sc, nrep=args.fire_replicas)
+    if "main" not in parts:
+        print(json.dumps(extras))
+        return
 
     s, st = build_system(samos_amd, sc, args.nside, args.pad, args.rebuild, args.batch)
     n = st["n"]
@@ -224,6 +400,7 @@ def main():
                   "rebuild_stage_launches": rb_launches},
     }
     out.update(extra)
+    out.update(extras)
     s.close()
     if not args.no_cpu:
         cpu_side = min(args.nside, NSIDE)

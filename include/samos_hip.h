@@ -222,6 +222,12 @@ int samos_ens_forces(samos_ens* ens, double* energies);
 /* `run steps` with the Brownian integrator in every replica (steps + 1
  * iterations, src/samos.cpp:878-970).  flags: SAMOS_RUN_NO_GRAPH. */
 int samos_ens_run_brownian(samos_ens* ens, int steps, int flags);
+/* Autotuner (no counterpart in the reference): times trial_steps Brownian
+ * steps for every candidate skin pads[i] (cell size rcut + pad) and neighbour
+ * gather batch 4 / 8, each from a saved copy of the state, then restores the
+ * state and keeps the fastest.  out[3] (may be NULL): pad, batch, ms per step.
+ * Trajectories differ between candidates only by summation order. */
+int samos_ens_autotune(samos_ens* ens, int trial_steps, const double* pads, int npads, double* out);
 /* IntegratorFIRE parameters for every replica (integrator_fire.hpp:58-154) and
  * the stop rule max|F| < stop_fmax that freezes a replica (no counterpart:
  * the reference's FIRE never stops the run loop). */
@@ -234,6 +240,11 @@ int samos_ens_fire_run(samos_ens* ens, int max_iter, int check, int* remaining, 
 int samos_ens_fire_state(samos_ens* ens, double* out);
 /* "gather_batch" 4 or 8. */
 int samos_ens_set_option(samos_ens* ens, const char* name, double value);
+/* HIP events around every launch of the per-particle step kernel ("step":
+ * the Brownian step or the FIRE force pass) and every rebuild sequence
+ * ("rebuild"); launches go step by step while enabled. */
+int samos_ens_profile(samos_ens* ens, int enable);
+int samos_ens_profile_read(samos_ens* ens, const char* kernel_class, double* ms_total, long long* launches);
 int samos_ens_sync(samos_ens* ens);
 
 #ifdef __cplusplus

@@ -100,11 +100,14 @@ def load_library(path: str = LIB_PATH):
         "samos_ens_enforce": (I, [P]),
         "samos_ens_forces": (I, [P, dp]),
         "samos_ens_run_brownian": (I, [P, I, I]),
+        "samos_ens_autotune": (I, [P, I, dp, I, dp]),
         "samos_ens_fire_setup": (I, [P, D, D, D, D, D, D, D, D, I, D]),
         "samos_ens_fire_run": (I, [P, I, I, ip, ip]),
         "samos_ens_fire_state": (I, [P, dp]),
         "samos_ens_set_option": (I, [P, ctypes.c_char_p, D]),
         "samos_ens_sync": (I, [P]),
+        "samos_ens_profile": (I, [P, I]),
+        "samos_ens_profile_read": (I, [P, ctypes.c_char_p, dp, llp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -417,6 +420,13 @@ class Ensemble:
     def run_brownian(self, steps, flags=0):
         self._chk(self.L.samos_ens_run_brownian(self.h, int(steps), int(flags)))
 
+    def autotune(self, trial_steps=64, pads=(0.2, 0.3, 0.45, 0.6)):
+        """Pick the skin and gather batch that run fastest; the state is left unchanged."""
+        p = np.ascontiguousarray(pads, dtype=np.float64)
+        out = np.zeros(3)
+        self._chk(self.L.samos_ens_autotune(self.h, int(trial_steps), _dptr(p), len(p), _dptr(out)))
+        return {"pad": float(out[0]), "gather_batch": int(out[1]), "ms_per_step": float(out[2])}
+
     def fire_setup(self, dt=0.01, alpha=0.1, alpha_scale=0.99, f_tol=1e-4, e_tol=1e-6, dt_inc=1.1, dt_dec=0.5,
                    dt_max=1.0, min_neg=5, stop_fmax=1e-10):
         self._chk(self.L.samos_ens_fire_setup(self.h, dt, alpha, alpha_scale, f_tol, e_tol, dt_inc, dt_dec, dt_max,
@@ -438,3 +448,11 @@ class Ensemble:
 
     def sync(self):
         self._chk(self.L.samos_ens_sync(self.h))
+
+    def profile(self, enable=True):
+        self._chk(self.L.samos_ens_profile(self.h, int(bool(enable))))
+
+    def profile_read(self, cls):
+        ms, cnt = ctypes.c_double(), ctypes.c_longlong()
+        self._chk(self.L.samos_ens_profile_read(self.h, cls.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value

@@ -161,6 +161,22 @@ def test_brownian_split_runs_match_oracle(gpu):
 # f_tolerance = 0 switches off the reference's RMS-force / energy-change test,
 # which would otherwise stop the velocity mixing (IntegratorFIRE::integrate
 # returns before it) long before max|F| reaches 1e-10.
+def test_autotune_leaves_state_and_physics(gpu):
+    e, states, p = active_ensemble(gpu, [900, 400, 1600], seed=47)
+    before = {k: e.get(k) for k in ["x", "y", "z", "nx", "ny", "nz", "age"]}
+    best = e.autotune(trial_steps=40, pads=(0.2, 0.45))
+    assert best["pad"] in (0.2, 0.45) and best["gather_batch"] in (4, 8) and best["ms_per_step"] > 0
+    for k, v in before.items():
+        assert np.array_equal(e.get(k), v), k
+    e.run_brownian(99)
+    for r, st in enumerate(states):
+        o = oracle_active(st, p, r)
+        o.run(99)
+        sl = slice(e.offsets[r], e.offsets[r + 1])
+        assert np.abs(stack(e, ["x", "y", "z"])[sl] - stack(o, ["x", "y", "z"])).max() <= 1e-9
+        assert np.abs(stack(e, ["nx", "ny", "nz"])[sl] - stack(o, ["nx", "ny", "nz"])).max() <= 1e-9
+
+
 def fire_ensemble(gpu, counts, seed):
     states = sc.plane_replicas(counts, seed=seed)
     e = sc.load_ensemble(gpu.Ensemble, states, "plane")
