@@ -245,6 +245,44 @@ int samos_ens_set_option(samos_ens* ens, const char* name, double value);
  * ("rebuild"); launches go step by step while enabled. */
 int samos_ens_profile(samos_ens* ens, int enable);
 int samos_ens_profile_read(samos_ens* ens, const char* kernel_class, double* ms_total, long long* launches);
+
+/* ---- warm-start network for FIRE relaxation --------------------------------
+ * No counterpart in the reference (which relaxes with IntegratorFIRE only,
+ * src/integrators/integrator_fire.cpp:34-148): a message-passing network on
+ * an ensemble's neighbour list that predicts each particle's displacement to
+ * its FIRE minimum, trained on (start, relaxed) pairs from batched FIRE.  The
+ * ensemble's replicas are 2-D configurations (plane constraint); the network
+ * sees radii and pair geometry only.  Bfloat16 MFMA products with FP32
+ * accumulation; FP32 parameters (flat order in samos_amd/csrc/mpnn.hip). */
+typedef struct samos_mpnn samos_mpnn;
+const char* samos_mpnn_last_error(const samos_mpnn* net); /* NULL: error of the last create */
+int samos_mpnn_num_params(void);
+int samos_mpnn_create(samos_mpnn** out, int device, unsigned long long seed);
+int samos_mpnn_destroy(samos_mpnn* net);
+int samos_mpnn_get_params(samos_mpnn* net, float* out);
+int samos_mpnn_set_params(samos_mpnn* net, const float* in);
+/* Displacements of the ensemble's current configuration; dxdy[2 n] (x, y per
+ * particle, replica-major input order) may be NULL. */
+int samos_mpnn_predict(samos_mpnn* net, samos_ens* ens, double* dxdy);
+/* Loss (1/2N) sum |delta - target|^2 and its gradient (flat order) for the
+ * current configuration; no update. */
+int samos_mpnn_gradients(samos_mpnn* net, samos_ens* ens, const double* target_dxdy, float* grad, double* loss);
+/* x <- x + scale * delta for every particle, then enforce and rebuild lists. */
+int samos_mpnn_warm_start(samos_mpnn* net, samos_ens* ens, double scale);
+/* Training pairs held on the device: reserve, then for each batch call
+ * dataset_begin before relaxing the ensemble and dataset_end after. */
+int samos_mpnn_dataset_reserve(samos_mpnn* net, int configs, int per_config);
+int samos_mpnn_dataset_begin(samos_mpnn* net, samos_ens* ens);
+int samos_mpnn_dataset_end(samos_mpnn* net, samos_ens* ens);
+int samos_mpnn_dataset_size(const samos_mpnn* net);
+/* Adam steps on minibatches of `work`'s replica count drawn from dataset
+ * configurations [first, first + count); losses[steps] may be NULL. */
+int samos_mpnn_train(samos_mpnn* net, samos_ens* work, int steps, double lr, unsigned long long seed,
+                     int first_config, int num_configs, double* losses);
+int samos_mpnn_evaluate(samos_mpnn* net, samos_ens* work, int first_config, int num_configs, double* mean_loss);
+/* HIP events around every MFMA product launch: total ms, flops and launches. */
+int samos_mpnn_profile(samos_mpnn* net, int enable);
+int samos_mpnn_profile_read(samos_mpnn* net, double* ms_total, double* flops, long long* launches);
 int samos_ens_sync(samos_ens* ens);
 
 #ifdef __cplusplus

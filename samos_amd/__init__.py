@@ -108,6 +108,23 @@ def load_library(path: str = LIB_PATH):
         "samos_ens_sync": (I, [P]),
         "samos_ens_profile": (I, [P, I]),
         "samos_ens_profile_read": (I, [P, ctypes.c_char_p, dp, llp]),
+        "samos_mpnn_last_error": (ctypes.c_char_p, [P]),
+        "samos_mpnn_num_params": (I, []),
+        "samos_mpnn_create": (I, [ctypes.POINTER(P), I, ctypes.c_ulonglong]),
+        "samos_mpnn_destroy": (I, [P]),
+        "samos_mpnn_get_params": (I, [P, P]),
+        "samos_mpnn_set_params": (I, [P, P]),
+        "samos_mpnn_predict": (I, [P, P, dp]),
+        "samos_mpnn_gradients": (I, [P, P, dp, P, dp]),
+        "samos_mpnn_warm_start": (I, [P, P, D]),
+        "samos_mpnn_dataset_reserve": (I, [P, I, I]),
+        "samos_mpnn_dataset_begin": (I, [P, P]),
+        "samos_mpnn_dataset_end": (I, [P, P]),
+        "samos_mpnn_dataset_size": (I, [P]),
+        "samos_mpnn_train": (I, [P, P, I, D, ctypes.c_ulonglong, I, I, dp]),
+        "samos_mpnn_evaluate": (I, [P, P, I, I, dp]),
+        "samos_mpnn_profile": (I, [P, I]),
+        "samos_mpnn_profile_read": (I, [P, dp, dp, llp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -456,3 +473,89 @@ class Ensemble:
         ms, cnt = ctypes.c_double(), ctypes.c_longlong()
         self._chk(self.L.samos_ens_profile_read(self.h, cls.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
+
+
+class Mpnn:
+    """Warm-start network for FIRE relaxation (a samos_mpnn) working on the
+    configurations held by an Ensemble."""
+
+    def __init__(self, seed=0, device=0):
+        self.L = load_library()
+        self.nparam = int(self.L.samos_mpnn_num_params())
+        h = ctypes.c_void_p()
+        rc = self.L.samos_mpnn_create(ctypes.byref(h), int(device), int(seed))
+        if rc != 0:
+            raise SamosError(self.L.samos_mpnn_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SamosError(self.L.samos_mpnn_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.samos_mpnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        a = np.empty(self.nparam, dtype=np.float32)
+        self._chk(self.L.samos_mpnn_get_params(self.h, a.ctypes.data))
+        return a
+
+    def set_params(self, values):
+        a = np.ascontiguousarray(values, dtype=np.float32)
+        if a.shape != (self.nparam,):
+            raise ValueError(f"expected {self.nparam} parameters, got {a.shape}")
+        self._chk(self.L.samos_mpnn_set_params(self.h, a.ctypes.data))
+
+    def predict(self, ens):
+        out = np.empty((ens.n, 2))
+        self._chk(self.L.samos_mpnn_predict(self.h, ens.h, _dptr(out)))
+        return out
+
+    def gradients(self, ens, target):
+        t = np.ascontiguousarray(target, dtype=np.float64).reshape(ens.n, 2)
+        g = np.empty(self.nparam, dtype=np.float32)
+        loss = ctypes.c_double()
+        self._chk(self.L.samos_mpnn_gradients(self.h, ens.h, _dptr(t), g.ctypes.data, ctypes.byref(loss)))
+        return loss.value, g
+
+    def warm_start(self, ens, scale=1.0):
+        self._chk(self.L.samos_mpnn_warm_start(self.h, ens.h, float(scale)))
+
+    def dataset_reserve(self, configs, per_config):
+        self._chk(self.L.samos_mpnn_dataset_reserve(self.h, int(configs), int(per_config)))
+
+    def dataset_begin(self, ens):
+        self._chk(self.L.samos_mpnn_dataset_begin(self.h, ens.h))
+
+    def dataset_end(self, ens):
+        self._chk(self.L.samos_mpnn_dataset_end(self.h, ens.h))
+
+    def dataset_size(self):
+        return int(self.L.samos_mpnn_dataset_size(self.h))
+
+    def train(self, work, steps, lr=1e-3, seed=0, first=0, count=0):
+        losses = np.zeros(max(int(steps), 1))
+        self._chk(self.L.samos_mpnn_train(self.h, work.h, int(steps), float(lr), int(seed), int(first), int(count),
+                                          _dptr(losses)))
+        return losses[:int(steps)]
+
+    def evaluate(self, work, first, count):
+        out = ctypes.c_double()
+        self._chk(self.L.samos_mpnn_evaluate(self.h, work.h, int(first), int(count), ctypes.byref(out)))
+        return out.value
+
+    def profile(self, enable=True):
+        self._chk(self.L.samos_mpnn_profile(self.h, int(bool(enable))))
+
+    def profile_read(self):
+        ms, fl, cnt = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        self._chk(self.L.samos_mpnn_profile_read(self.h, ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(cnt)))
+        return ms.value, fl.value, cnt.value

@@ -30,6 +30,7 @@
 //   FIRE                                      src/integrators/integrator_fire.cpp:34-148
 //   run loop order                            src/samos.cpp:878-970
 #include "constraint.hpp"
+#include "ensemble_impl.hpp"
 #include "kernels.hpp"
 #include "samos_hip.h"
 
@@ -40,48 +41,6 @@
 #include <vector>
 
 namespace samos {
-
-struct alignas(32) D4 {
-  double x, y, z, w;
-};
-
-struct RepD {
-  BoxD box;
-  ConstraintD con;
-  int n0, n;                  // slot range
-  int b0, nb;                 // bucket base and bucket count (one end entry follows)
-  int hbits, ncx, ncy, ncz;
-  double wx, wy, wz, cut2, pad2q;
-  // soft repulsion and polar alignment
-  double k, a, J, rc2_al;
-  int radii, has_al;
-  // Brownian
-  double v0, mu, mur, stoch, dt;
-  uint32_t seed;
-  int pad0_;
-  // FIRE
-  double fdt, alpha, alpha0, f_alpha, f_inc, f_dec, dt_max, f_tol, e_tol, E_old, f1, f2, stop_fmax;
-  int n_min, last_neg, frozen, zero_v;
-  long long iters;
-  double P, F2, V2, E, Fmax;
-};
-
-// Every column of an ensemble (slot order).
-struct EState {
-  D4* p[2];                   // x, y, z, radius
-  D4* d[2];                   // nx, ny, nz, 0
-  double *vx, *vy, *vz, *fx, *fy, *fz, *tx, *ty, *tz, *omega, *age;
-  int *rep, *lid, *cellc;
-};
-
-// Replicas selected for a pass: ids and exclusive prefix sums of their
-// particle and bucket-entry counts; cnt = {selected, particles, bucket entries}.
-struct Work {
-  int* rep;
-  int* poff;
-  int* boff;
-  int* cnt;
-};
 
 namespace {
 
@@ -769,38 +728,6 @@ using namespace samos;
 // ===========================================================================
 // C-ABI
 // ===========================================================================
-struct samos_ens {
-  hipStream_t st = nullptr;
-  std::string err;
-  int nrep = 0, n = 0, np = 0;
-  std::vector<RepD> h_rep;
-  std::vector<int> h_gid;        // slot -> replica-major input index
-  RepD* d_rep = nullptr;
-  EState e{};
-  int cur = 0;                   // buffer holding the current positions and directors
-  NListD nl{};
-  bool has_nlist = false;
-  long long ntiles = 1;          // scan tiles over every bucket entry
-  Work wr{}, wa{};               // rebuild list, FIRE running list
-  int* d_flags = nullptr;        // [0] rebuild needed, [2] overflow, [3] largest bucket, [4] builds,
-                                 // [5] largest neighbour count, [8] replicas still relaxing
-  int* rep_need = nullptr;
-  unsigned long long* d_call = nullptr;
-  double* cols = nullptr;        // 5 reduction columns of np
-  int batch = 4;
-  int kind = -1;                 // 0 Brownian, 1 FIRE
-  double rcut = 2.4, pad = 0.3;
-  // graphs: Brownian chunk (K steps) and FIRE chunk (check iterations)
-  hipGraphExec_t g_brown = nullptr;
-  int g_brown_cur = -1, g_brown_batch = 0;
-  hipGraphExec_t g_fire = nullptr;
-  int g_fire_k = 0, g_fire_cur = -1, g_fire_batch = 0;
-  std::vector<void*> owned;      // every device allocation
-  // timing: HIP events around each step kernel and each rebuild sequence
-  // (launches go step by step while enabled)
-  bool prof = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_step, ev_rebuild;
-};
 
 namespace {
 
@@ -882,8 +809,13 @@ void ens_rebuild(samos_ens* E, const int* need) {
   hipLaunchKernelGGL(k_ens_build, dim3(g), dim3(kBlock), 0, E->st, E->e, b, E->d_rep, E->wr, E->nl, need);
 }
 
-// Builds every replica's list, growing the capacity until nothing overflows.
-int ens_build_all(samos_ens* E) {
+}  // namespace
+
+void samos::ens_enforce_async(samos_ens* E) {
+  hipLaunchKernelGGL(k_ens_enforce, dim3(grid1(E->n)), dim3(kBlock), 0, E->st, E->e, E->cur, E->d_rep, E->n);
+}
+
+int samos::ens_build_all(samos_ens* E) {
   for (int attempt = 0; attempt < 4; ++attempt) {
     hipLaunchKernelGGL(k_ens_set_need, dim3(rep_grid(E)), dim3(kBlock), 0, E->st, E->rep_need, E->nrep, E->d_flags);
     ECHK(hipMemsetAsync(E->d_flags + 2, 0, 2 * sizeof(int), E->st));
@@ -908,6 +840,8 @@ int ens_build_all(samos_ens* E) {
   E->err = "neighbour list capacity could not be established";
   return -1;
 }
+
+namespace {
 
 int check_run_flags(samos_ens* E) {
   int h[8];
