@@ -225,12 +225,141 @@ def bench_ensemble(samos_amd, sc, steps, plain_steps, nrep=256, pad=None):
     }
 
 
-def bench_fire(samos_amd, sc, nrep=512, n=4096, max_iter=400000):
-    """Plain batched FIRE to max|F| < 1e-10 on every replica."""
-    states = sc.plane_replicas([n] * nrep, seed=5)
+def c1_system(samos_amd, sc):
+    """Active polar soft disks in a periodic 2-D box (N = 4096, packing 0.8)."""
+    st = sc.plane_disks(4096, phi=0.8, seed=101)
+    s = sc.load(samos_amd.System, st)
+    s.nlist(2.4, 0.3)
+    s.pair_soft(1.0, 2.0)
+    s.align_polar(0.1, 2.4)
+    s.constraint("plane")
+    s.set_timestep(0.01)
+    s.integrator_brownian(v0=0.1, nu=0.1, mu=1.0, seed=11)
+    return s, st
+
+
+def c2_system(samos_amd, sc):
+    """Active nematic soft disks on a sphere (N = 100000, R = sqrt(N / 4))."""
+    st = sc.sphere_disks(100000, phi=1.0, seed=102)
+    s = sc.load(samos_amd.System, st)
+    s.nlist(2.4, 0.3)
+    s.pair_soft(10.0, 2.0)
+    s.align_nematic(0.5, 2.4)
+    s.constraint("sphere", st["R"])
+    s.set_timestep(0.005)
+    s.integrator_brownian(v0=0.5, nu=0.05, mu=1.0, seed=12)
+    return s, st
+
+
+def bench_single(samos_amd, sc, which, steps, cpu_steps):
+    """One system through the drop-in single-system path (the `run` loop)."""
+    build = c1_system if which == "c1" else c2_system
+    out = {}
+    variants = [("f64", 0)] + ([("f32_force", samos_amd.RUN_FP32_FORCE)] if which == "c2" else [])
+    for label, flags in variants:
+        s, st = build(samos_amd, sc)
+        s.run(99, flags)
+        secs = timed_run(s, steps, flags)
+        out[label] = {"value": st["n"] * steps / secs, "ms_per_step": secs * 1e3 / steps}
+        s.close()
+    print("bench: %s %s" % (which, out), file=sys.stderr, flush=True)
+    res = {"workload": build.__doc__.strip(), "particles": st["n"], "steps": steps, "unit": "particle-steps/s",
+           **out}
+    if cpu_steps > 0:
+        code = r"""
+import os, sys, time, json
+sys.path.insert(0, %r)
+os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+import oracle, bench
+from tests import scenarios as sc
+class O:
+    System = staticmethod(lambda n, b, p: oracle.OracleSystem(n, b, p, fast=True))
+    RUN_FP32_FORCE = 4
+s, st = bench.%s_system(O, sc)
+t0 = time.perf_counter()
+s.run(%d - 1)
+print(json.dumps({"seconds": time.perf_counter() - t0, "n": st["n"], "steps": %d}))
+""" % (ROOT, which, cpu_steps, cpu_steps)
+        r = json.loads(subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                                      check=True).stdout.strip().splitlines()[-1])
+        res["cpu_baseline"] = {"value": r["n"] * r["steps"] / r["seconds"], "unit": "particle-steps/s", "cores": 1,
+                               "kind": "port", "sample": "same system, %d run-loop iterations (%.1f s), one thread"
+                                                         % (r["steps"], r["seconds"])}
+    return res
+
+
+WARM_WEIGHTS = os.path.join(ROOT, "samos_amd", "data", "warm_start_mpnn_n4096_phi088.npz")
+
+
+def fire_batch(samos_amd, sc, states):
     e = sc.load_ensemble(samos_amd.Ensemble, states, "plane")
     e.set_params(k=1.0, a=2.0, use_radii=1.0, dt=0.01)
     e.nlist(2.4, 0.3)
+    return e
+
+
+def warm_start_ab(samos_amd, sc, states, fs_plain, secs_plain, max_iter, weights):
+    """The same configurations relaxed from the network's predicted positions:
+    inference, displacement, list rebuild and FIRE are all timed.  An answer
+    counts as the plain one when max|F| < 1e-10 and its energy equals plain
+    FIRE's to 1e-12 (relative); configurations whose warm run does not stop
+    fall back to plain FIRE from the raw start (timed too)."""
+    z = np.load(weights, allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    mp = samos_amd.Mpnn(seed=0)
+    mp.set_params(z["params"])
+    out = {"weights": os.path.relpath(weights, ROOT), "trained_on_configs": meta.get("train_configs"),
+           "val_loss": meta["history"][-1]["val_loss"] if meta.get("history") else None,
+           "val_loss_zero_prediction": meta.get("val_loss_zero_prediction")}
+    for scale in (1.0, 0.5):
+        e = fire_batch(samos_amd, sc, states)
+        e.sync()
+        t0 = time.perf_counter()
+        mp.warm_start(e, scale)
+        e.sync()
+        t_inf = time.perf_counter() - t0
+        e.fire_setup(dt=0.01, alpha=0.1, dt_max=0.1, f_tol=0.0, stop_fmax=1e-10)
+        rem, iters = e.fire_run(max_iter, check=100)
+        e.sync()
+        secs = time.perf_counter() - t0
+        fs = e.fire_state()
+        e.close()
+        fallback_s = 0.0
+        stopped = fs["stopped"] == 1
+        if not np.all(stopped):
+            idx = np.nonzero(~stopped)[0]
+            f = fire_batch(samos_amd, sc, [states[i] for i in idx])
+            f.fire_setup(dt=0.01, alpha=0.1, dt_max=0.1, f_tol=0.0, stop_fmax=1e-10)
+            t1 = time.perf_counter()
+            f.fire_run(max_iter, check=100)
+            f.sync()
+            fallback_s = time.perf_counter() - t1
+            f.close()
+        same = np.abs(fs["energy"] - fs_plain["energy"]) <= 1e-12 * np.abs(fs_plain["energy"])
+        rel = (fs["energy"] - fs_plain["energy"]) / fs_plain["energy"]
+        total = secs + fallback_s
+        print("bench: warm start x%.2f: %.3f s (inference + rebuild %.1f ms), %d iterations, same minimum %d/%d"
+              % (scale, total, t_inf * 1e3, iters, int(same.sum()), len(same)), file=sys.stderr, flush=True)
+        out["scale_%g" % scale] = {
+            "time_to_converge_s": total,
+            "inference_and_rebuild_ms": t_inf * 1e3,
+            "fallback_s": fallback_s,
+            "iterations": iters,
+            "iterations_per_replica": {"median": float(np.median(fs["iters"])), "max": float(fs["iters"].max())},
+            "speedup_vs_plain": secs_plain / total,
+            "same_minimum_fraction": float(np.mean(same & stopped)),
+            "energy_rel_diff_vs_plain": {"median": float(np.median(rel)), "mean": float(np.mean(rel)),
+                                         "lower_fraction": float(np.mean(rel < -1e-12))},
+        }
+    mp.close()
+    return out
+
+
+def bench_fire(samos_amd, sc, nrep=512, n=4096, max_iter=400000, weights=WARM_WEIGHTS):
+    """Plain batched FIRE to max|F| < 1e-10 on every replica, then (when trained
+    weights exist) the same configurations warm-started by the network."""
+    states = sc.plane_replicas([n] * nrep, seed=5)
+    e = fire_batch(samos_amd, sc, states)
     e.fire_setup(dt=0.01, alpha=0.1, dt_max=0.1, f_tol=0.0, stop_fmax=1e-10)
     e.sync()
     t0 = time.perf_counter()
@@ -243,9 +372,7 @@ def bench_fire(samos_amd, sc, nrep=512, n=4096, max_iter=400000):
           file=sys.stderr, flush=True)
     # force-pass kernel timing on a fresh copy of the same start
     e.close()
-    e = sc.load_ensemble(samos_amd.Ensemble, states, "plane")
-    e.set_params(k=1.0, a=2.0, use_radii=1.0, dt=0.01)
-    e.nlist(2.4, 0.3)
+    e = fire_batch(samos_amd, sc, states)
     e.fire_setup(dt=0.01, alpha=0.1, dt_max=0.1, f_tol=0.0, stop_fmax=1e-10)
     e.profile(True)
     e.fire_run(200, check=200)
@@ -254,7 +381,11 @@ def bench_fire(samos_amd, sc, nrep=512, n=4096, max_iter=400000):
     e.close()
     avg_s = ms / 1e3 / max(launches, 1)
     achieved = FIRE_BYTES_PER_PARTICLE_ITER * nrep * n / avg_s / 1e9
+    warm = None
+    if weights and os.path.exists(weights):
+        warm = warm_start_ab(samos_amd, sc, states, fs, secs, max_iter, weights)
     return {
+        "warm_start": warm,
         "workload": "fire_polydisperse: %d replicas x N=%d, radii U[0.8, 1.2], periodic, packing 0.88; soft k=1 "
                     "use_particle_radii; FIRE dt0 0.01 dt_max 0.1 alpha 0.1, stop at max|F| < 1e-10" % (nrep, n),
         "time_to_converge_s": secs,
@@ -296,15 +427,17 @@ def main():
     ap.add_argument("--batch", type=int, choices=[4, 8], default=4)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this size (HBM counter calibration) and exit")
-    ap.add_argument("--parts", default="main,ensemble,fire",
+    ap.add_argument("--parts", default="main,ensemble,fire,c1,c2",
                     help="comma list: main (the N=1M metric), ensemble (batched active spheres), "
-                         "fire (batched FIRE relaxation)")
+                         "fire (batched FIRE relaxation and the warm start), c1 (polar disks, N=4096), "
+                         "c2 (nematic sphere, N=100000)")
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--ensemble-replicas", type=int, default=256)
     ap.add_argument("--ensemble-pad", type=float, default=None, help="fixed skin instead of the autotuner")
     ap.add_argument("--plain-steps", type=int, default=200,
                     help="steps per replica when timing the replicas as separate runs")
     ap.add_argument("--fire-replicas", type=int, default=512)
+    ap.add_argument("--warm-weights", default=WARM_WEIGHTS, help="trained warm-start network (.npz)")
     args = ap.parse_args()
     parts = set(args.parts.split(","))
     if args.copy_probe:
@@ -323,7 +456,11 @@ def main():
         extras["ensemble"] = bench_ensemble(samos_amd, sc, args.ensemble_steps, args.plain_steps,
                                             nrep=args.ensemble_replicas, pad=args.ensemble_pad)
     if "fire" in parts:
-        extras["fire"] = bench_fire(samos_amd, sc, nrep=args.fire_replicas)
+        extras["fire"] = bench_fire(samos_amd, sc, nrep=args.fire_replicas, weights=args.warm_weights)
+    if "c1" in parts:
+        extras["polar_disks_plane_n4096"] = bench_single(samos_amd, sc, "c1", 10000, 0 if args.no_cpu else 1000)
+    if "c2" in parts:
+        extras["nematic_sphere_n100000"] = bench_single(samos_amd, sc, "c2", 5000, 0 if args.no_cpu else 40)
     if "main" not in parts:
         print(json.dumps(extras))
         return

@@ -15,7 +15,7 @@ relaxation.  Definition (H = 64):
   edge features   f_e = [rho_e / r_l, ov_e, r_i - 1, r_j - 1]
   node features   u_i = [r_i - 1, sum_e max(ov_e, 0), #{e : ov_e > 0} / 6]
   embedding       h_i = silu(u_i W0 + b0)
-  3 layers        Z_e = [h_i, h_j, f_e] We + be,  A_i = sum_{e in row i} silu(Z_e)
+  3 layers        Z_e = [h_i, h_j, f_e] We + be,  A_i = (1/8) sum_{e in row i} silu(Z_e)
                   h_i <- h_i + silu([h_i, A_i] Wn + bn)
   head            U_e = silu([h_i, h_j, f_e] Wo + bo),  w_e = U_e . v + c
                   delta_i = sum_{e in row i} w_e dhat_e
@@ -36,6 +36,7 @@ K_NODE = 32       # node-feature operand width (3 used, zero padded)
 K_EDGE = 160      # [h_i, h_j, f_e] operand width (132 used, zero padded)
 K_UPD = 128       # [h_i, A_i]
 N_LAYERS = 3
+AGG = 0.125       # A_i = AGG * sum over row i of the messages
 
 # parameter blocks in the flat order shared with the device (name, rows, cols)
 LAYOUT = [("W0", K_NODE, H), ("b0", 1, H)]
@@ -132,6 +133,7 @@ class Net:
             Me = silu(Ze)
             A = np.zeros((n, H))
             np.add.at(A, src, Me)
+            A *= AGG
             Xn = np.concatenate([q(h), q(A)], 1)
             Zn = self._acc(Xn @ q(p[f"Wn{l}"])) + p[f"bn{l}"]
             h = h + silu(Zn)
@@ -173,7 +175,7 @@ class Net:
             gXn = self._acc(gZn @ q(p[f"Wn{l}"]).T)
             gA = gXn[:, H:]
             gh = gh + gXn[:, :H]
-            gZe = q(gA[src] * dsilu(c["Ze"][l]))
+            gZe = q(AGG * gA[src] * dsilu(c["Ze"][l]))
             g[f"We{l}"] = self._acc(c["Xe"][l].T @ gZe)
             g[f"be{l}"] = gZe.sum(0)
             gXe = self._acc(gZe @ q(p[f"We{l}"]).T)

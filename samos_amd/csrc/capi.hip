@@ -100,6 +100,25 @@ namespace {
 
 constexpr int kTab = 3 * kMaxTypes * kMaxTypes;
 
+// Fills used inside captured step sequences (where a memset node is avoided).
+__global__ void k_fill_d(double* p, double v, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_fill_i(int* p, int v, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+void fill_d(hipStream_t st, double* p, double v, long n) {
+  const int g = (int)std::max<long>(1, std::min<long>((n + kBlock - 1) / kBlock, 1024));
+  hipLaunchKernelGGL(k_fill_d, dim3(g), dim3(kBlock), 0, st, p, v, n);
+}
+
+void fill_i(hipStream_t st, int* p, int v, long n) {
+  const int g = (int)std::max<long>(1, std::min<long>((n + kBlock - 1) / kBlock, 1024));
+  hipLaunchKernelGGL(k_fill_i, dim3(g), dim3(kBlock), 0, st, p, v, n);
+}
+
 #define HIPCHK(expr)                                                            \
   do {                                                                          \
     hipError_t e_ = (expr);                                                     \
@@ -282,9 +301,9 @@ void loop_head_forces(samos_ctx* ctx, double* e_out, bool f32 = false) {
   const int pm = ctx->has_nlist ? pot_mask(ctx) : 0;
   const int am = ctx->has_nlist ? al_mask(ctx) : 0;
   if (pm == 0 && !ctx->has_sp)
-    for (int k = 9; k <= 11; ++k) hipMemsetAsync(ctx->dcol[k], 0, sizeof(double) * ctx->n, ctx->st);
+    for (int k = 9; k <= 11; ++k) fill_d(ctx->st, ctx->dcol[k], 0.0, ctx->n);
   if (am == 0)
-    for (int k = 12; k <= 14; ++k) hipMemsetAsync(ctx->dcol[k], 0, sizeof(double) * ctx->n, ctx->st);
+    for (int k = 12; k <= 14; ++k) fill_d(ctx->st, ctx->dcol[k], 0.0, ctx->n);
   interact(ctx, pm, am, 0, e_out, f32);
 }
 
@@ -295,7 +314,7 @@ bool needs_nlist(const samos_ctx* c) {
 // Device-side conditional rebuild after a step: flags[0] = any(need_update).
 void check_and_rebuild(samos_ctx* ctx) {
   if (!needs_nlist(ctx)) return;
-  hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->st);
+  fill_i(ctx->st, ctx->d_flags, 0, 1);
   Fields f = fields(ctx);
   launch_need_update(ctx->st, f, ctx->nl, ctx->box, ctx->n, ctx->d_flags);
   ProfScope ps(ctx, "rebuild");
@@ -303,24 +322,29 @@ void check_and_rebuild(samos_ctx* ctx) {
 }
 
 // ---- integrators ----------------------------------------------------------
-int integrate_one(samos_ctx* ctx, Integ& I, int run_step) {
+// j >= 0: the step is the j-th of a captured chunk; its noise call and run
+// step are read from the device clock plus j instead of being baked in.
+int integrate_one(samos_ctx* ctx, Integ& I, int run_step, int j = -1) {
   Fields f = fields(ctx);
   const int n = ctx->n;
-  const unsigned long long call = I.calls++;
+  const bool clocked = j >= 0;
+  const unsigned long long call = clocked ? (unsigned long long)j : I.calls++;
+  const ClockD* clk = clocked ? ctx->d_clock : nullptr;
+  const int t_off = clocked ? j : run_step;
   if (I.name == "brownian" || I.name == "brownian_pos" || I.name == "brownian_align") {
     int which = I.name == "brownian" ? 0 : (I.name == "brownian_pos" ? 1 : 2);
     ProfScope ps(ctx, "integrate");
     launch_brownian(ctx->st, which, f, ctx->box, ctx->con, n, I.dt, I.v0, I.mu, I.mur, I.stoch, I.flip_p, I.nematic,
-                    I.velocity, I.temp, I.seed, call, nullptr, I.slot, run_step);
+                    I.velocity, I.temp, I.seed, call, clk, I.slot, t_off);
   } else if (I.name == "langevin") {
     {
       ProfScope ps(ctx, "integrate");
-      launch_langevin_a(ctx->st, I.method, f, ctx->box, ctx->con, n, I.dt, I.gamma, I.temp, I.seed, call, nullptr,
-                        I.slot, run_step, I.R);
+      launch_langevin_a(ctx->st, I.method, f, ctx->box, ctx->con, n, I.dt, I.gamma, I.temp, I.seed, call, clk,
+                        I.slot, t_off, I.R);
     }
     interact(ctx, pot_mask(ctx), 0, 0, nullptr);
     ProfScope ps(ctx, "integrate");
-    launch_langevin_b(ctx->st, I.method, f, n, I.dt, I.gamma, I.temp, I.seed, call, nullptr, I.slot, run_step, I.R,
+    launch_langevin_b(ctx->st, I.method, f, n, I.dt, I.gamma, I.temp, I.seed, call, clk, I.slot, t_off, I.R,
                       ctx->box, ctx->con);
   } else if (I.name == "fire") {
     {
@@ -1038,7 +1062,40 @@ int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags
     ctx->time_step += t_end - t_begin;
     return check_overflow(ctx);
   }
-  for (int t = t_begin; t < t_end; ++t) {
+  int t = t_begin;
+  // Without FIRE (whose scalar update runs on the host) the step sequence is
+  // launch-only: chunks of K iterations are captured once, with the noise call
+  // and run step read from the device clock, and replayed.
+  bool graphable = !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof && ctx->coop == 0;
+  for (auto& kv : ctx->integs) graphable = graphable && kv.second.name != "fire";
+  const int K = 32;
+  if (graphable && t_end - t > K && t + K <= steps) {
+    if (int r = upload_clock(ctx, t)) return r;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < K; ++j) {
+      for (auto& kv : ctx->integs) integrate_one(ctx, kv.second, 0, j);
+      check_and_rebuild(ctx);
+      launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+      loop_head_forces(ctx, nullptr, f32);
+    }
+    launch_clock_advance(ctx->st, ctx->d_clock, K);
+    HIPCHK(hipStreamEndCapture(ctx->st, &g));
+    const hipError_t ir = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HIPCHK(ir);
+    // every captured iteration ends with the next loop head: t + K <= steps
+    while (t_end - t > K && t + K <= steps) {
+      HIPCHK(hipGraphLaunch(ge, ctx->st));
+      t += K;
+      ctx->time_step += K;
+      for (auto& kv : ctx->integs) kv.second.calls += K;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    HIPCHK(hipGraphExecDestroy(ge));
+  }
+  for (; t < t_end; ++t) {
     for (auto& kv : ctx->integs)
       if (int r = integrate_one(ctx, kv.second, t)) return r;
     check_and_rebuild(ctx);

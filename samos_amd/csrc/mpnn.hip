@@ -12,7 +12,7 @@
 //   ov_e = r_i + r_j - rho_e, f_e = [rho_e / r_l, ov_e, r_i - 1, r_j - 1]
 //   u_i = [r_i - 1, sum_e max(ov_e, 0), #{ov_e > 0} / 6]
 //   h_i = silu(u_i W0 + b0)
-//   3 x: Z_e = [h_i, h_j, f_e] We + be;  A_i = sum_{e in row i} silu(Z_e);
+//   3 x: Z_e = [h_i, h_j, f_e] We + be;  A_i = (1/8) sum_{e in row i} silu(Z_e);
 //        h_i += silu([h_i, A_i] Wn + bn)
 //   U_e = silu([h_i, h_j, f_e] Wo + bo);  w_e = U_e . v + c;  delta_i = sum_e w_e dhat_e
 //   loss = (1 / 2N) sum_i |delta_i - delta*_i|^2
@@ -45,6 +45,7 @@ constexpr int KN = 32;    // node-feature operand width
 constexpr int KE = 160;   // [h_i, h_j, f_e] operand width
 constexpr int KU = 128;   // [h_i, A_i] operand width
 constexpr int NLAYER = 3;
+constexpr float kAgg = 0.125f;  // A_i = kAgg * sum over row i of the messages
 constexpr int kThreads = 256;
 constexpr int kTnRows = 2048;  // rows of one weight-gradient workgroup
 
@@ -295,7 +296,7 @@ __global__ void k_aggregate(const float* M, const int* row, float* A, int n) {
   const int i = (int)(t / H), c = (int)(t % H);
   float s = 0.f;
   for (int e = row[i]; e < row[i + 1]; ++e) s += M[(size_t)e * H + c];
-  A[t] = s;
+  A[t] = kAgg * s;
 }
 
 // w_e = U_e . v + c
@@ -399,7 +400,7 @@ __global__ void k_edge_back(const float* gA, const int* src, const float* Ze, bf
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)E * H) return;
   const int e = (int)(t / H), c = (int)(t % H);
-  gZ[t] = f2bf(gA[(size_t)src[e] * H + c] * dsilu(Ze[t]));
+  gZ[t] = f2bf(kAgg * gA[(size_t)src[e] * H + c] * dsilu(Ze[t]));
 }
 
 // ---- parameters ---------------------------------------------------------------

@@ -322,6 +322,29 @@ def test_brownian_noise_on_same_stream(gpu):
     assert np.abs(directors(d) - directors(o)).max() <= 1e-9
 
 
+@pytest.mark.parametrize("kind", ["brownian", "spv"])
+def test_captured_chunks_equal_direct_launches(gpu, kind):
+    st = sc.plane_disks(2500, phi=0.7, seed=21)
+    runs = []
+    for flags in (0, gpu.RUN_NO_GRAPH):
+        d = sc.load(gpu.System, st)
+        d.nlist(2.4, 0.3)
+        d.pair_soft(1.0, 2.0)
+        d.constraint("plane")
+        d.set_timestep(0.01)
+        if kind == "brownian":
+            d.align_polar(0.1, 2.4)
+            d.integrator_brownian(v0=0.1, nu=0.1, seed=7, temp=(1, 0.0, 0.1, 60))
+        else:
+            d.integrator_langevin(gamma=1.0, seed=4, method="spv", temp=(1, 0.2, 0.5, 60))
+        d.run_range(0, 70, 150, flags)
+        d.run_range(70, 151, 150, flags)
+        runs.append({k: d.get(k) for k in ("x", "y", "vx", "nx", "fx", "age")})
+        d.close()
+    for k in runs[0]:
+        assert np.array_equal(runs[0][k], runs[1][k]), k
+
+
 def test_brownian_pos_align_self_propulsion(gpu):
     st = sc.sphere_disks(1000, R=12.0, seed=22)
     d, o = both(gpu, st)
