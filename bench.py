@@ -401,6 +401,51 @@ def bench_fire(samos_amd, sc, nrep=512, n=4096, max_iter=400000, weights=WARM_WE
     }
 
 
+def profile_run(samos_amd, sc, name):
+    """Workloads for rocprofv3, launched step by step (no graph replay, so every
+    kernel appears in a kernel trace):
+      ensemble  300 Brownian steps of the active-sphere ensemble (skin 1.0), then
+                300 FIRE iterations of 128 x 4096 polydisperse disks
+      mpnn      64 training pairs (FIRE, 200 iterations), 5 + 40 Adam steps on
+                32-configuration minibatches, one prediction on 64 configurations"""
+    if name == "ensemble":
+        states, p = sc.c4_ensemble_workload()
+        e = sc.load_ensemble(samos_amd.Ensemble, states, "sphere")
+        e.set_params(**p)
+        e.nlist(2.4, 1.0)
+        e.run_brownian(299, samos_amd.RUN_NO_GRAPH)
+        e.sync()
+        print(json.dumps({"ensemble": e.nlist_info()}), flush=True)
+        e.close()
+        f = fire_batch(samos_amd, sc, sc.plane_replicas([4096] * 128, seed=5))
+        f.fire_setup(dt=0.01, alpha=0.1, dt_max=0.1, f_tol=0.0, stop_fmax=1e-10)
+        print(json.dumps({"fire": f.fire_run(300, check=2)}), flush=True)
+        f.close()
+    elif name == "mpnn":
+        mp = samos_amd.Mpnn(seed=3)
+        e = fire_batch(samos_amd, sc, sc.plane_replicas([4096] * 64, seed=4242))
+        e.fire_setup(dt=0.01, alpha=0.1, dt_max=0.1, f_tol=0.0, stop_fmax=1e-10)
+        mp.dataset_reserve(64, 4096)
+        mp.dataset_begin(e)
+        e.fire_run(200, check=2)
+        mp.dataset_end(e)
+        work = fire_batch(samos_amd, sc, sc.plane_replicas([4096] * 32, seed=4243))
+        mp.train(work, 5, lr=1e-3)
+        mp.profile(True)
+        mp.train(work, 40, lr=1e-3)
+        ms, fl, cnt = mp.profile_read()
+        mp.profile(False)
+        mp.profile(True)
+        mp.predict(e)
+        ims, ifl, icnt = mp.profile_read()
+        mp.profile(False)
+        print(json.dumps({"training_products": {"launches": cnt, "ms": ms, "flop": fl, "tflops": fl / ms / 1e9},
+                          "inference_products_64_configs": {"launches": icnt, "ms": ims, "flop": ifl,
+                                                            "tflops": ifl / ims / 1e9}}), flush=True)
+    else:
+        raise SystemExit("unknown profile run " + name)
+
+
 def traffic_from_profiles():
     """HBM bytes per launch of the fused step kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
@@ -431,6 +476,8 @@ def main():
                     help="comma list: main (the N=1M metric), ensemble (batched active spheres), "
                          "fire (batched FIRE relaxation and the warm start), c1 (polar disks, N=4096), "
                          "c2 (nematic sphere, N=100000)")
+    ap.add_argument("--profile-run", choices=["ensemble", "mpnn"], default=None,
+                    help="only run a fixed workload for rocprofv3 and exit")
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--ensemble-replicas", type=int, default=256)
     ap.add_argument("--ensemble-pad", type=float, default=None, help="fixed skin instead of the autotuner")
@@ -450,6 +497,10 @@ def main():
 
     import samos_amd
     from tests import scenarios as sc
+
+    if args.profile_run:
+        profile_run(samos_amd, sc, args.profile_run)
+        return
 
     extras = {}
     if "ensemble" in parts:

@@ -202,6 +202,19 @@ __global__ void k_reduce_partials(const float* partial, int nblk, int len, float
   out[j] = s;
 }
 
+// First level of a two-level reduction: group g sums partial rows
+// [g * kGroup, (g + 1) * kGroup) in order.
+constexpr int kGroup = 32;
+__global__ void k_reduce_groups(const float* partial, int nblk, int len, float* tmp) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y;
+  if (j >= len) return;
+  const int b1 = min(nblk, (g + 1) * kGroup);
+  float s = 0.f;
+  for (int b = g * kGroup; b < b1; ++b) s += partial[(size_t)b * len + j];
+  tmp[(size_t)g * len + j] = s;
+}
+
 // ---- graph construction ----------------------------------------------------
 __global__ void k_csr_fill(NListD nl, const int* row, int* col, int* src, int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -354,27 +367,37 @@ __global__ void k_head_back(const float* gdelta, const float* dh, const int* src
   gZ[t] = f2bf(g * v[c] * dsilu(Zo[t]));
 }
 
-// Partial sums over edge chunks of g_w U (64 columns) and g_w (column 64).
-__global__ void k_head_vgrad(const float* gw, const float* U, int E, float* partial) {
-  const int c = threadIdx.x;  // 0..64
-  const int e0 = blockIdx.x * kTnRows, e1 = min(E, e0 + kTnRows);
-  float s = 0.f;
-  if (c < H)
-    for (int e = e0; e < e1; ++e) s += gw[e] * U[(size_t)e * H + c];
-  else if (c == H)
-    for (int e = e0; e < e1; ++e) s += gw[e];
-  if (c <= H) partial[(size_t)blockIdx.x * (H + 1) + c] = s;
+// Partial sums over chunks of kVRows edges of g_w U (64 columns) and g_w
+// (column 64): four 64-lane groups take every fourth edge, then combine in a
+// fixed order.
+constexpr int kVRows = 256;
+__global__ void __launch_bounds__(kThreads) k_head_vgrad(const float* gw, const float* U, int E, float* partial) {
+  __shared__ float red[4][H + 1];
+  const int c = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int e0 = blockIdx.x * kVRows, e1 = min(E, e0 + kVRows);
+  float s = 0.f, sw = 0.f;
+  for (int e = e0 + grp; e < e1; e += 4) {
+    const float g = gw[e];
+    s += g * U[(size_t)e * H + c];
+    sw += g;
+  }
+  red[grp][c] = s;
+  if (c == 0) red[grp][H] = sw;
+  __syncthreads();
+  if (threadIdx.x <= H)
+    partial[(size_t)blockIdx.x * (H + 1) + threadIdx.x] =
+        ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
 }
 
-// gh[i] (+)= sum over row i of gX[e][0:64] + gX[rev e][64:128].
+// gh[i] (+)= sum over row i of gX[e][0:64] + gX[rev e][64:128]  (gX rows of KU).
 __global__ void k_edge_to_node(const float* gX, const int* row, const int* rev, float* gh, int n, int accumulate) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)n * H) return;
   const int i = (int)(t / H), c = (int)(t % H);
   float s = accumulate ? gh[t] : 0.f;
   for (int e = row[i]; e < row[i + 1]; ++e) {
-    s += gX[(size_t)e * KE + c];
-    if (rev[e] >= 0) s += gX[(size_t)rev[e] * KE + H + c];
+    s += gX[(size_t)e * KU + c];
+    if (rev[e] >= 0) s += gX[(size_t)rev[e] * KU + H + c];
   }
   gh[t] = s;
 }
@@ -517,6 +540,8 @@ struct samos_mpnn {
   bf16_t *gZ = nullptr, *gZn = nullptr;
   float* partial = nullptr;
   long cap_partial = 0;
+  float* partial2 = nullptr;     // first level of two-level reductions
+  long cap_partial2 = 0;
   double* loss_part = nullptr;
   void* scan_tmp = nullptr;
   size_t scan_bytes = 0;
@@ -650,7 +675,7 @@ int ensure(samos_mpnn* M, int n, int E, bool train) {
   if (train) {
     MCHK(m_alloc(M, &M->gdelta, 2 * N));
     MCHK(m_alloc(M, &M->gw, EE));
-    MCHK(m_alloc(M, &M->gX, EE * KE));
+    MCHK(m_alloc(M, &M->gX, EE * KU));
     MCHK(m_alloc(M, &M->gh, N * H));
     MCHK(m_alloc(M, &M->gXn, N * KU));
     MCHK(m_alloc(M, &M->gA, N * H));
@@ -662,6 +687,9 @@ int ensure(samos_mpnn* M, int n, int E, bool train) {
     const long rows = std::max(EE, N);
     M->cap_partial = (rows / kTnRows + 2) * ((long)KE * H + H);
     MCHK(m_alloc(M, &M->partial, (size_t)M->cap_partial));
+    m_free(M, M->partial2);
+    M->cap_partial2 = (rows / kVRows / kGroup + 2) * ((long)KE * H + H);
+    MCHK(m_alloc(M, &M->partial2, (size_t)M->cap_partial2));
   }
   M->capN = N;
   M->capE = EE;
@@ -695,6 +723,24 @@ void gemm(samos_mpnn* M, hipStream_t st, const bf16_t* A, int rows, const bf16_t
   gprof_end(M, st, ev, 2.0 * rows * KI * NO);
 }
 
+// out[0:len] = sum over the nblk rows of M->partial, in row order (two levels
+// when there are many rows).
+int reduce_rows(samos_mpnn* M, hipStream_t st, int nblk, int len, float* out) {
+  if (nblk <= 2 * kGroup) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(blocks(len)), dim3(kThreads), 0, st, M->partial, nblk, len, out);
+    return 0;
+  }
+  const int groups = (nblk + kGroup - 1) / kGroup;
+  if ((long)groups * len > M->cap_partial2) {
+    M->err = "internal: reduction buffer too small";
+    return -1;
+  }
+  hipLaunchKernelGGL(k_reduce_groups, dim3(blocks(len), groups), dim3(kThreads), 0, st, M->partial, nblk, len,
+                     M->partial2);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(blocks(len)), dim3(kThreads), 0, st, M->partial2, groups, len, out);
+  return 0;
+}
+
 // Weight gradient of one layer into grad + off (matrix, then bias).
 template <int KI, int NO>
 int gemm_tn(samos_mpnn* M, hipStream_t st, const bf16_t* A, const bf16_t* G, int rows, long off) {
@@ -707,9 +753,7 @@ int gemm_tn(samos_mpnn* M, hipStream_t st, const bf16_t* A, const bf16_t* G, int
   auto ev = gprof_begin(M, st);
   hipLaunchKernelGGL((k_gemm_tn<KI, NO>), dim3(nblk), dim3(kThreads), 0, st, A, G, rows, M->partial);
   gprof_end(M, st, ev, 2.0 * rows * KI * NO);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(blocks(len)), dim3(kThreads), 0, st, M->partial, nblk, (int)len,
-                     M->g + off);
-  return 0;
+  return reduce_rows(M, st, nblk, (int)len, M->g + off);
 }
 
 // Graph (CSR of the neighbour list), features and the forward pass on the
@@ -787,17 +831,18 @@ int backward(samos_mpnn* M, samos_ens* E, double* loss) {
   hipLaunchKernelGGL(k_head_back, dim3(blocks((long)Et * H)), dim3(kThreads), 0, st, M->gdelta, M->dh, M->src,
                      M->Ze[NLAYER], P + M->vv.off, M->gw, M->gZ, Et);
   {
-    const int nblk = std::max(1, (Et + kTnRows - 1) / kTnRows);
+    const int nblk = std::max(1, (Et + kVRows - 1) / kVRows);
     if ((long)nblk * (H + 1) > M->cap_partial) {
       M->err = "internal: head partial buffer too small";
       return -1;
     }
-    hipLaunchKernelGGL(k_head_vgrad, dim3(nblk), dim3(128), 0, st, M->gw, M->U, Et, M->partial);
+    hipLaunchKernelGGL(k_head_vgrad, dim3(nblk), dim3(kThreads), 0, st, M->gw, M->U, Et, M->partial);
     // v then c are adjacent in the flat order
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(128), 0, st, M->partial, nblk, H + 1, M->g + M->vv.off);
+    if (int rc = reduce_rows(M, st, nblk, H + 1, M->g + M->vv.off)) return rc;
   }
   if (int rc = gemm_tn<KE, H>(M, st, M->Xe[NLAYER], M->gZ, Et, M->Wo.off)) return rc;
-  gemm<H, KE, 0>(M, st, M->gZ, Et, M->Wok, nullptr, nullptr, M->gX);
+  // input gradients of the [h_i, h_j] columns only (the edge features need none)
+  gemm<H, KU, 0>(M, st, M->gZ, Et, M->Wok, nullptr, nullptr, M->gX);
   hipLaunchKernelGGL(k_edge_to_node, dim3(blocks((long)n * H)), dim3(kThreads), 0, st, M->gX, M->row, M->rev, M->gh,
                      n, 0);
   for (int l = NLAYER - 1; l >= 0; --l) {
@@ -809,7 +854,7 @@ int backward(samos_mpnn* M, samos_ens* E, double* loss) {
     hipLaunchKernelGGL(k_edge_back, dim3(blocks((long)Et * H)), dim3(kThreads), 0, st, M->gA, M->src, M->Ze[l], M->gZ,
                        Et);
     if (int rc = gemm_tn<KE, H>(M, st, M->Xe[l], M->gZ, Et, M->We[l].off)) return rc;
-    gemm<H, KE, 0>(M, st, M->gZ, Et, M->Wek[l], nullptr, nullptr, M->gX);
+    gemm<H, KU, 0>(M, st, M->gZ, Et, M->Wek[l], nullptr, nullptr, M->gX);
     hipLaunchKernelGGL(k_edge_to_node, dim3(blocks((long)n * H)), dim3(kThreads), 0, st, M->gX, M->row, M->rev,
                        M->gh, n, 1);
   }
