@@ -25,6 +25,7 @@ FIELD_CODE = {name: i for i, name in enumerate(DOUBLE_FIELDS)}
 FIELD_CODE.update({name: 32 + i for i, name in enumerate(INT_FIELDS)})
 CONSTRAINT = {"none": 0, "plane": 1, "sphere": 2}
 LANGEVIN_METHOD = {"baoab": 0, "spv": 1, "bbk": 2}
+RNG_STREAM = {"philox": 0, "mt19937": 1}
 
 _libs = {}
 
@@ -78,6 +79,8 @@ def load(fast=False):
         "orc_run": (I, [P, I]),
         "orc_philox": (None, [ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "orc_gauss3": (None, [u32, u32, u32, u64, dp]),
+        "orc_set_rng_stream": (I, [P, I]),
+        "orc_gsl_stream_sample": (I, [u32, I, dp, I, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -202,6 +205,12 @@ class OracleSystem:
         self._chk(self.L.orc_project_torques(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return a
 
+    def rng_stream(self, kind):
+        """Noise stream of integrators added afterwards: "philox" (the device's
+        counter-based stream, default) or "mt19937" (the reference's GSL
+        MT19937 + polar Gaussian, drawn in the reference's per-particle order)."""
+        self._chk(self.L.orc_set_rng_stream(self.h, RNG_STREAM[kind]))
+
     def integrator_brownian(self, which="brownian", v0=1.0, nu=1.0, mu=1.0, mur=1.0, seed=0, nematic=False,
                             tau=1.0, velocity_align=False, temp=(0, 0.0, 1.0, 1000)):
         w = {"brownian": 0, "brownian_pos": 1, "brownian_align": 2}[which]
@@ -249,3 +258,14 @@ def gauss3(seed, tag, pid, call):
     g = (ctypes.c_double * 3)()
     L.orc_gauss3(seed, tag, pid, call, g)
     return list(g)
+
+
+def gsl_stream_sample(seed, n_uniform, n_gauss=0):
+    """The reference-order stream seeded with `seed`: its first n_uniform
+    uniforms, then the next n_gauss Gaussians."""
+    L = load()
+    u = np.empty(max(n_uniform, 1))
+    g = np.empty(max(n_gauss, 1))
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.orc_gsl_stream_sample(seed, n_uniform, u.ctypes.data_as(dp), n_gauss, g.ctypes.data_as(dp))
+    return u[:n_uniform], g[:n_gauss]

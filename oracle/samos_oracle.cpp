@@ -25,11 +25,15 @@
 //   * constraints: invariants (|x| = R, n.N = 0, |n| = 1, rotation inverses);
 //   * integrators: single-particle closed forms (free ABP drift, O-step decay,
 //     FIRE two-particle relaxation), equipartition with noise on;
-//   * the noise stream is the project's counter-based Philox4x32-10 (checked
-//     against the Random123 known-answer vectors), NOT the reference's GSL
-//     MT19937 stream: bit-level parity with GSL draws is unpinned.
+//   * the default noise stream is the project's counter-based Philox4x32-10
+//     (checked against the Random123 known-answer vectors), the one the device
+//     library draws.  A second stream restates the reference's own noise
+//     source (GSL MT19937 + gsl_ran_gaussian, drawn sequentially in the
+//     reference's per-particle order); it is used to check noisy runs
+//     statistically.  Bit-level parity with GSL draws is unpinned (no GSL here).
 
 #include <cmath>
+#include <random>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -158,6 +162,35 @@ struct PairAl {         // polar (kind 0) or nematic (kind 1)
   double energy = 0.0;
 };
 
+// Reference noise source (src/utils/rng.cpp:34-61): one sequential stream per
+// integrator.  GSL's default generator is MT19937; gsl_rng_set(r, 0) seeds it
+// with 4357 and otherwise with the seed, using the Matsumoto-Nishimura (2002)
+// initialisation, which is std::mt19937's seeding.  gsl_rng_uniform returns
+// word / 2^32; gsl_rng_uniform_pos rejects 0; gsl_ran_gaussian is the
+// Marsaglia polar method (two uniform_pos draws mapped to the square
+// (-1,1)^2, rejected outside the unit disc or at the origin) returning
+// sigma * y * sqrt(-2 ln r2 / r2), one value per accepted pair.  Written from
+// those published algorithms; GSL itself is absent, so bit parity is unpinned.
+struct GslStream {
+  std::mt19937 mt;
+  void set(uint32_t seed) { mt.seed(seed == 0 ? 4357u : seed); }
+  double uniform() { return (double)mt() / 4294967296.0; }
+  double uniform_pos() {
+    double u;
+    do { u = uniform(); } while (u == 0.0);
+    return u;
+  }
+  double gauss(double sigma) {
+    double x, y, r2;
+    do {
+      x = -1.0 + 2.0 * uniform_pos();
+      y = -1.0 + 2.0 * uniform_pos();
+      r2 = x * x + y * y;
+    } while (r2 > 1.0 || r2 == 0.0);
+    return sigma * y * std::sqrt(-2.0 * std::log(r2) / r2);
+  }
+};
+
 struct Constraint {
   int kind = -1;        // 0 none, 1 plane, 2 sphere
   double r = 1.0, zpos = 0.0;
@@ -173,6 +206,8 @@ struct Integ {
   double v0 = 1.0, nu = 1.0, mu = 1.0, mur = 1.0, tau = 0.0, stoch = 0.0;
   bool nematic = false, velocity = false;
   uint32_t seed = 0; uint64_t calls = 0;
+  int stream = 0;       // 0: counter-based Philox (device stream); 1: GslStream (reference order)
+  GslStream gsl;
   // langevin (integrator_langevin.hpp:64-121)
   double gamma = 1.0; int method = 0;  // 0 baoab, 1 spv, 2 bbk
   std::vector<double> Rx, Ry, Rz;
@@ -211,6 +246,7 @@ struct Sim {
   Constraint con;
   std::map<std::string, Integ> integs;   // key: type + "_" + group (src/samos.cpp:666-690)
   int ntypes = 1;
+  int rng_stream = 0;   // noise stream of integrators added from now on
 };
 
 // System::apply_periodic (src/system/system.cpp:1083-1095)
@@ -651,6 +687,28 @@ void rotate_velocity(Sim& s, Particle& p, double phi) {
 // ---------------------------------------------------------------------------
 // Integrators
 // ---------------------------------------------------------------------------
+// Noise draws of one integrator.  Philox: keyed by (seed, tag, particle id,
+// call), so the order of evaluation does not matter.  GslStream: drawn in call
+// order, which the integrators below make the reference's per-particle order
+// (flip uniform, then [three translational Gaussians], then the rotational one).
+inline void noise3(Integ& I, uint32_t tag, uint32_t pid, uint64_t call, double g[3]) {
+  if (I.stream) {
+    g[0] = I.gsl.gauss(1.0); g[1] = I.gsl.gauss(1.0); g[2] = I.gsl.gauss(1.0);
+  } else {
+    gauss3(I.seed, tag, pid, call, g);
+  }
+}
+inline double noise_rot(Integ& I, uint32_t tag, uint32_t pid, uint64_t call) {
+  if (I.stream) return I.gsl.gauss(1.0);
+  double g, u; gauss1_u1(I.seed, tag, pid, call, g, u);
+  return g;
+}
+inline double noise_flip(Integ& I, uint32_t tag, uint32_t pid, uint64_t call) {
+  if (I.stream) return I.gsl.uniform();
+  double g, u; gauss1_u1(I.seed, tag, pid, call, g, u);
+  return u;
+}
+
 // IntegratorBrownian::integrate (src/integrators/integrator_brownian.cpp:42-124)
 void brownian(Sim& s, Integ& I) {
   double T = I.Tval(s.run_step);
@@ -661,8 +719,7 @@ void brownian(Sim& s, Integ& I) {
   if (I.nematic) {
     for (int i = 0; i < N; i++) {
       Particle& p = s.p[i];
-      double g, u; gauss1_u1(I.seed, TAG_BROWNIAN, (uint32_t)p.id, call, g, u);
-      if (u < I.tau) {
+      if (noise_flip(I, TAG_BROWNIAN, (uint32_t)p.id, call) < I.tau) {
         p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz;
         if (I.velocity) { p.vx = -p.vx; p.vy = -p.vy; p.vz = -p.vz; }
       }
@@ -676,14 +733,14 @@ void brownian(Sim& s, Integ& I) {
     p.vx = fdx; p.vy = fdy; p.vz = fdz;
     p.x += I.dt * fdx; p.y += I.dt * fdy; p.z += I.dt * fdz;
     if (T > 0.0) {
-      double g[3]; gauss3(I.seed, TAG_BROWNIAN, (uint32_t)p.id, call, g);
+      double g[3]; noise3(I, TAG_BROWNIAN, (uint32_t)p.id, call, g);
       double frx = B * g[0], fry = B * g[1], frz = B * g[2];
       p.vx += frx; p.vy += fry; p.vz += frz;
       p.x += sqrt_dt * frx; p.y += sqrt_dt * fry; p.z += sqrt_dt * frz;
     }
     enforce(s, p);
     p.omega = I.mur * project_torque(s, p);
-    double g, u; gauss1_u1(I.seed, TAG_BROWNIAN, (uint32_t)p.id, call, g, u);
+    double g = noise_rot(I, TAG_BROWNIAN, (uint32_t)p.id, call);
     double dtheta = I.dt * p.omega + I.stoch * g;
     rotate_director(s, p, dtheta);
     if (I.velocity) rotate_velocity(s, p, dtheta);
@@ -702,7 +759,7 @@ void brownian_pos(Sim& s, Integ& I) {
     p.vx = fdx; p.vy = fdy; p.vz = fdz;
     p.x += I.dt * p.vx; p.y += I.dt * p.vy; p.z += I.dt * p.vz;
     if (T > 0.0) {
-      double g[3]; gauss3(I.seed, TAG_BROWNIAN_POS, (uint32_t)p.id, call, g);
+      double g[3]; noise3(I, TAG_BROWNIAN_POS, (uint32_t)p.id, call, g);
       double frx = B * g[0], fry = B * g[1], frz = B * g[2];
       p.vx += frx; p.vy += fry; p.vz += frz;
       p.x += sqrt_dt * frx; p.y += sqrt_dt * fry; p.z += sqrt_dt * frz;
@@ -717,13 +774,12 @@ void brownian_align(Sim& s, Integ& I) {
   uint64_t call = I.calls++;
   if (I.nematic) {
     for (auto& p : s.p) {
-      double g, u; gauss1_u1(I.seed, TAG_BROWNIAN_ALIGN, (uint32_t)p.id, call, g, u);
-      if (u < I.tau) { p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz; }
+      if (noise_flip(I, TAG_BROWNIAN_ALIGN, (uint32_t)p.id, call) < I.tau) { p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz; }
     }
   }
   for (auto& p : s.p) {
     p.omega = I.mur * project_torque(s, p);
-    double g, u; gauss1_u1(I.seed, TAG_BROWNIAN_ALIGN, (uint32_t)p.id, call, g, u);
+    double g = noise_rot(I, TAG_BROWNIAN_ALIGN, (uint32_t)p.id, call);
     double dtheta = I.dt * p.omega + I.stoch * g;
     rotate_director(s, p, dtheta);
   }
@@ -744,7 +800,7 @@ void langevin_baoab(Sim& s, Integ& I) {
     p.vx *= exp_dt; p.vy *= exp_dt; p.vz *= exp_dt;
     if (B != 0.0) {
       double sf = B / std::sqrt(p.mass);
-      double g[3]; gauss3(I.seed, TAG_LANGEVIN, (uint32_t)p.id, call, g);
+      double g[3]; noise3(I, TAG_LANGEVIN, (uint32_t)p.id, call, g);
       p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
     }
     p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
@@ -780,7 +836,7 @@ void langevin_spv(Sim& s, Integ& I) {
     p.vz = exp_dt * p.vz + fact * p.fz;
     if (zeta != 0.0) {
       double sf = zeta / std::sqrt(p.mass);
-      double g[3]; gauss3(I.seed, TAG_LANGEVIN, (uint32_t)p.id, call, g);
+      double g[3]; noise3(I, TAG_LANGEVIN, (uint32_t)p.id, call, g);
       p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
     }
     p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
@@ -820,7 +876,7 @@ void langevin_bbk(Sim& s, Integ& I) {
     p.vz = one_div * (p.vz + dm * p.fz);
     if (B != 0.0) {
       double sf = 0.5 * B * one_div / std::sqrt(p.mass);
-      double g[3]; gauss3(I.seed, TAG_LANGEVIN, (uint32_t)p.id, call, g);
+      double g[3]; noise3(I, TAG_LANGEVIN, (uint32_t)p.id, call, g);
       I.Rx[p.id] = g[0]; I.Ry[p.id] = g[1]; I.Rz[p.id] = g[2];
       p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
       p.age += I.dt;
@@ -1142,7 +1198,24 @@ int orc_project_torques(void* h, double* out) {
 
 static Integ make_integ(Sim* s, const char* name, double dt) {
   Integ I; I.name = name; I.dt = dt > 0.0 ? dt : s->dt;
+  I.stream = s->rng_stream;
   return I;
+}
+
+// Noise stream for integrators added after this call: 0 counter-based Philox
+// (the device stream, the default), 1 the reference's sequential MT19937 order.
+int orc_set_rng_stream(void* h, int stream) {
+  if (stream != 0 && stream != 1) { g_err = "unknown noise stream"; return -1; }
+  static_cast<Sim*>(h)->rng_stream = stream;
+  return 0;
+}
+
+// First draws of the reference-order stream: nu uniforms, then ng Gaussians.
+int orc_gsl_stream_sample(uint32_t seed, int nu, double* uni, int ng, double* gau) {
+  GslStream a; a.set(seed);
+  for (int i = 0; i < nu; i++) uni[i] = a.uniform();
+  for (int i = 0; i < ng; i++) gau[i] = a.gauss(1.0);
+  return 0;
 }
 
 // Brownian family.  which: 0 brownian, 1 brownian_pos, 2 brownian_align.
@@ -1154,6 +1227,7 @@ int orc_add_brownian(void* h, int which, double v0, double nu, double mu, double
   Integ I = make_integ(s, nm, 0.0);
   if (I.dt <= 0.0) { g_err = "Integrator step has not been set."; return -1; }
   I.v0 = v0; I.nu = nu; I.mu = mu; I.mur = mur; I.seed = (uint32_t)seed;
+  I.gsl.set(I.seed);
   I.nematic = nematic != 0; I.tau = I.dt / tau; I.velocity = velocity_align != 0;
   I.tkind = tkind; I.tmin = tmin; I.tmax = tmax; I.tsteps = tsteps;
   I.stoch = std::sqrt(I.nu * I.dt);
@@ -1166,12 +1240,13 @@ int orc_add_langevin(void* h, double gamma, int seed, int method, int tkind, dou
   Integ I = make_integ(s, "langevin", 0.0);
   if (I.dt <= 0.0) { g_err = "Integrator step has not been set."; return -1; }
   I.gamma = gamma; I.seed = (uint32_t)seed; I.method = method;
+  I.gsl.set(I.seed);
   I.tkind = tkind; I.tmin = tmin; I.tmax = tmax; I.tsteps = tsteps;
   if (method == 2) {
     int n = (int)s->p.size();
     I.Rx.resize(n); I.Ry.resize(n); I.Rz.resize(n);
     for (int i = 0; i < n; i++) {
-      double g[3]; gauss3(I.seed, TAG_BBK_INIT, (uint32_t)i, 0, g);
+      double g[3]; noise3(I, TAG_BBK_INIT, (uint32_t)i, 0, g);
       I.Rx[i] = g[0]; I.Ry[i] = g[1]; I.Rz[i] = g[2];
     }
   }
