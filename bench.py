@@ -470,6 +470,9 @@ def main():
     ap.add_argument("--pad", type=float, default=0.5, help="neighbour-list skin")
     ap.add_argument("--rebuild", choices=["coop", "stages"], default="stages")
     ap.add_argument("--batch", type=int, choices=[4, 8], default=4)
+    ap.add_argument("--no-tiles", action="store_true",
+                    help="time the neighbour-list kernels instead of the cell-tiled engine")
+    ap.add_argument("--tile-cells", type=int, default=0, help="cells per tile (0: automatic)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this size (HBM counter calibration) and exit")
     ap.add_argument("--parts", default="main,ensemble,fire,c1,c2",
@@ -517,12 +520,14 @@ def main():
         return
 
     s, st = build_system(samos_amd, sc, args.nside, args.pad, args.rebuild, args.batch)
+    s.set_option("tile_cells", args.tile_cells)
     n = st["n"]
     print("bench: N=%d box %.3f, warmup %d, steps %d" % (n, st["box"][0], args.warmup, args.steps),
           file=sys.stderr, flush=True)
+    run_flags = samos_amd.RUN_NO_TILES if args.no_tiles else 0
     if args.warmup > 0:
-        timed_run(s, args.warmup, 0)
-    secs = timed_run(s, args.steps, 0)
+        timed_run(s, args.warmup, run_flags)
+    secs = timed_run(s, args.steps, run_flags)
     value = n * args.steps / secs
     print("bench: timed %.3f s -> %.4e particle-steps/s" % (secs, value), file=sys.stderr, flush=True)
 
@@ -530,7 +535,7 @@ def main():
     # launch on the context's stream (step-by-step launches, same kernels).
     s.profile(True)
     prof_steps = min(args.steps, 500)
-    timed_run(s, prof_steps, 0)
+    timed_run(s, prof_steps, run_flags)
     ms, launches = s.profile_read("fused")
     rb_ms, rb_launches = s.profile_read("rebuild")
     s.profile(False)
@@ -538,6 +543,7 @@ def main():
     alg_bytes = ALG_BYTES_PER_PARTICLE_STEP * n
     achieved = alg_bytes / avg_s / 1e9
     info = s.nlist_info()
+    tiles = s.tile_info()
 
     extra = {}
     if args.variants:
@@ -579,13 +585,16 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(),
-            "kernel": "k_baoab_fused<LJ>",
+            "kernel": "k_baoab_fused<LJ>" if args.no_tiles else "k_tile_step<LJ>",
             "avg_launch_us": avg_s * 1e6,
             "launches_timed": launches,
             "alg_bytes_per_launch": alg_bytes,
         },
         "nlist": {"builds": info["builds"], "stride": info["stride"], "rebuild_ms_total": rb_ms,
                   "rebuild_stage_launches": rb_launches},
+        "tiles": None if args.no_tiles else {"cells_per_tile": tiles["tile_cells"], "mask_words": tiles["mask_words"],
+                                             "max_candidates": tiles["max_candidates"],
+                                             "max_halo": tiles["max_halo"], "rebuilds": tiles["rebuilds"]},
     }
     out.update(extra)
     out.update(extras)

@@ -49,6 +49,7 @@ enum samos_value_kind { SAMOS_VALUE_CONSTANT = 0, SAMOS_VALUE_LINEAR = 1 };
 #define SAMOS_RUN_FAITHFUL   1  /* evaluate forces exactly where the reference does (no fused step) */
 #define SAMOS_RUN_NO_GRAPH   2  /* launch step by step instead of replaying captured chunks */
 #define SAMOS_RUN_FP32_FORCE 4  /* pair arithmetic in FP32, sums and state in FP64 */
+#define SAMOS_RUN_NO_TILES   8  /* planar fused runs: neighbour-list kernels instead of the cell-tiled engine */
 
 /* ---- context and state ---------------------------------------------------
  * System(input, msg, box) (src/system/system.cpp:64-452) and Box(lx, ly, lz)
@@ -173,10 +174,18 @@ int samos_profile(samos_ctx* ctx, int enable);
 int samos_profile_read(samos_ctx* ctx, const char* kernel_class, double* ms_total, long long* launches);
 int samos_sync(samos_ctx* ctx);
 /* Tuning switches (no counterpart in the reference; results do not depend on
- * them): "rebuild_coop" 1 = one cooperative launch per neighbour-list rebuild
- * check, 0 = five stage kernels replayed in graphs (default); "gather_batch" 4 or 8 = neighbour
- * gathers in flight per thread in the fused step. */
+ * them beyond summation order): "rebuild_coop" 1 = one cooperative launch per
+ * neighbour-list rebuild check, 0 = five stage kernels replayed in graphs
+ * (default); "gather_batch" 4 or 8 = neighbour gathers in flight per thread in
+ * the fused step; "tile_cells" 0 = automatic, 1..80 = cells per workgroup tile
+ * of the cell-tiled planar engine. */
 int samos_set_option(samos_ctx* ctx, const char* name, double value);
+/* Cell-tiled planar engine (no counterpart in the reference): planar fused
+ * Langevin runs keep the state in cell order and the Verlet list as stencil
+ * bitmasks (see SAMOS_RUN_NO_TILES).  info[6]: tiled runs, rebuilds inside
+ * them, largest stencil candidate count, largest staged halo, cells per tile
+ * and mask words of the last run. */
+int samos_tile_info(samos_ctx* ctx, long long* info);
 
 /* ---- replica ensembles ---------------------------------------------------
  * Many independent systems (replicas) of uneven size and parameters resident

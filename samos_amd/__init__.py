@@ -26,7 +26,7 @@ FIELD_CODE.update({name: 32 + i for i, name in enumerate(INT_FIELDS)})
 
 CONSTRAINT = {"none": 0, "plane": 1, "sphere": 2}
 LANGEVIN_METHOD = {"baoab": 0, "spv": 1, "bbk": 2}
-RUN_FAITHFUL, RUN_NO_GRAPH, RUN_FP32_FORCE = 1, 2, 4
+RUN_FAITHFUL, RUN_NO_GRAPH, RUN_FP32_FORCE, RUN_NO_TILES = 1, 2, 4, 8
 OBSERVABLE = {"max_force": 0, "force_sq": 1, "kinetic_energy": 2, "vec_velocity": 3,
               "velocity": 4, "polar_order": 5}
 
@@ -59,6 +59,7 @@ def load_library(path: str = LIB_PATH):
         "samos_nlist_build": (I, [P]),
         "samos_nlist_need_update": (I, [P, ip]),
         "samos_nlist_info": (I, [P, llp]),
+        "samos_tile_info": (I, [P, llp]),
         "samos_nlist_pairs": (I, [P, ip, LL, llp]),
         "samos_resort": (I, [P]),
         "samos_pair_soft": (I, [P, D, D, I]),
@@ -213,6 +214,12 @@ class System:
         a = (ctypes.c_longlong * 8)()
         self._chk(self.L.samos_nlist_info(self.h, a))
         keys = ["uses_cells", "ncx", "ncy", "ncz", "stride", "builds", "max_cell_count", "overflow"]
+        return dict(zip(keys, list(a)))
+
+    def tile_info(self):
+        a = (ctypes.c_longlong * 6)()
+        self._chk(self.L.samos_tile_info(self.h, a))
+        keys = ["runs", "rebuilds", "max_candidates", "max_halo", "tile_cells", "mask_words"]
         return dict(zip(keys, list(a)))
 
     def pairs(self):

@@ -3,6 +3,7 @@
 // reference's parameter semantics, the run loop and captured step chunks.
 #include "samos_hip.h"
 #include "kernels.hpp"
+#include "tiles.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -94,6 +95,13 @@ struct samos_ctx {
   double mass_u = -1.0;         // common particle mass, or -1
   bool age_uniform = false;     // fused range in progress with one common age
   double age_u = 0.0;           // that age
+  // cell-tiled engine of planar fused runs (allocated on first use)
+  TileD td{};
+  void* tile_mem = nullptr;
+  size_t tile_bytes = 0;
+  int tile_key[4] = {-1, -1, -1, -1};  // n, ncx, ncy, T of the allocation
+  long long tile_stats[4] = {0, 0, 0, 0};  // tiled runs, rebuilds, largest candidate count, largest halo
+  int tile_cells = 0;                      // cells per tile along x; 0: from the density
 };
 
 namespace {
@@ -236,13 +244,14 @@ int read_flags(samos_ctx* ctx, int* h) {
 
 // Synchronous build with neighbour capacity growth: rebuilds until no slot
 // overflowed, then keeps 1.5x headroom for the device-side rebuilds of a run.
-int build_sync(samos_ctx* ctx) {
+// `at` (optional) replaces the slot columns the build reads positions from.
+int build_sync(samos_ctx* ctx, const Fields* at = nullptr) {
   for (int attempt = 0; attempt < 4; ++attempt) {
     HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
     HIPCHK(hipMemsetAsync(ctx->d_flags + 5, 0, sizeof(int), ctx->st));
     {
       ProfScope ps(ctx, "rebuild");
-      launch_rebuild(ctx->st, fields(ctx), ctx->nl, ctx->box, ctx->n, nullptr);
+      launch_rebuild(ctx->st, at ? *at : fields(ctx), ctx->nl, ctx->box, ctx->n, nullptr);
     }
     HIPCHK(hipGetLastError());
     int h[8];
@@ -457,6 +466,9 @@ int uniform_mass(samos_ctx* ctx, double* out) {
 // first part of step t.  Positions of iteration t live in buffer (t - a) % 2;
 // x_a is in the primary columns on entry and x_b is back there on return,
 // with f = F(x_b): the state the reference has at the loop head of b.
+bool tile_eligible(samos_ctx* ctx, int flags, double mass_u);
+int run_tiled_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int flags, double mass_u);
+
 int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int flags) {
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   double mass_u = -1.0;
@@ -469,6 +481,10 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     if (ctx->n) HIPCHK(hipMemcpy(age.data(), ctx->dcol[16], sizeof(double) * ctx->n, hipMemcpyDeviceToHost));
     ctx->age_uniform = ctx->n > 0 && std::all_of(age.begin(), age.end(), [&](double v) { return v == age[0]; });
     ctx->age_u = ctx->n ? age[0] : 0.0;
+  }
+  if (tile_eligible(ctx, flags, mass_u)) {
+    const int r = run_tiled_langevin(ctx, I, a, b, steps, flags, mass_u);
+    if (r <= 0) return r;
   }
   if (int r = upload_clock(ctx, a)) return r;
   HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
@@ -532,6 +548,188 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     HIPCHK(hipStreamSynchronize(ctx->st));
     ctx->age_uniform = false;
   }
+  return 0;
+}
+
+
+// ---- cell-tiled fused Langevin run (planar systems) ------------------------
+// Eligible: plane constraint, cell grid with at least three cells per side,
+// one particle type, no per-particle radii, one common mass and one common age
+// (the fused run's uniform-age path).
+bool tile_eligible(samos_ctx* ctx, int flags, double mass_u) {
+  const NListD& nl = ctx->nl;
+  return !(flags & SAMOS_RUN_NO_TILES) && ctx->con.kind == 1 && nl.use_cell && nl.ncx >= 3 && nl.ncy >= 3 &&
+         ctx->ntypes == 1 && !ctx->soft_radii && !ctx->lj_radii && mass_u > 0.0 && ctx->age_uniform && ctx->n > 0;
+}
+
+// Carves the tiled engine's arrays out of one allocation, reused while the
+// particle count, grid and tile width stay the same.
+int tile_prepare(samos_ctx* ctx) {
+  const NListD& nl = ctx->nl;
+  const int n = ctx->n, ncx = nl.ncx, ncy = nl.ncy;
+  const double mean = (double)n / ((double)ncx * ncy);
+  // about 224 particles per tile, so most tiles fill one 256-thread workgroup
+  int T = ctx->tile_cells > 0 ? ctx->tile_cells : (int)std::lround(224.0 / std::max(mean, 1e-9));
+  T = std::max(1, std::min(T, std::min(kTileMaxCells, ncx)));
+  // mask words: room for twice the mean stencil population
+  const int W = std::max(2, std::min(4, (int)std::ceil(2.0 * 9.0 * mean / 64.0)));
+  TileD& td = ctx->td;
+  const int key[4] = {n, ncx, ncy, T};
+  if (!(ctx->tile_mem && std::equal(key, key + 4, ctx->tile_key))) {
+    hipFree(ctx->tile_mem);
+    ctx->tile_mem = nullptr;
+    const int ntx = (ncx + T - 1) / T;
+    const long ncell = (long)ncx * ncy, ntiles = (long)ntx * ncy;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t d2 = al(sizeof(double2) * n), i4 = al(sizeof(int) * n);
+    const size_t bytes = 2 * (2 * d2 + 3 * d2 + 2 * i4) + 2 * al(sizeof(int) * (ncell + 1)) +
+                         al(sizeof(unsigned long long) * 4 * (size_t)n) + al(sizeof(int) * ncell) +
+                         al(sizeof(int) * ntiles) + al(sizeof(int) * (ntiles + 1)) + 2 * i4 + al(sizeof(TileCtl)) +
+                         al(2 * sizeof(TileSet));
+    HIPCHK(hipMalloc(&ctx->tile_mem, bytes));
+    ctx->tile_bytes = bytes;
+    char* p = static_cast<char*>(ctx->tile_mem);
+    auto take = [&](size_t b) { char* q = p; p += b; return q; };
+    td = TileD{};
+    TileSet sets[2];
+    for (int s = 0; s < 2; ++s) {
+      sets[s].xy[0] = (double2*)take(d2); sets[s].xy[1] = (double2*)take(d2);
+      sets[s].v = (double2*)take(d2); sets[s].nd = (double2*)take(d2); sets[s].o = (double2*)take(d2);
+      sets[s].id = (int*)take(i4); sets[s].slot = (int*)take(i4);
+      sets[s].cstart = (int*)take(al(sizeof(int) * (ncell + 1)));
+    }
+    TileSet* d_sets = (TileSet*)take(al(2 * sizeof(TileSet)));
+    HIPCHK(hipMemcpy(d_sets, sets, 2 * sizeof(TileSet), hipMemcpyHostToDevice));
+    td.sets = d_sets;
+    td.mask = (unsigned long long*)take(al(sizeof(unsigned long long) * 4 * (size_t)n));
+    td.count = (int*)take(al(sizeof(int) * ncell));
+    td.tcount = (int*)take(al(sizeof(int) * ntiles));
+    td.tstart = (int*)take(al(sizeof(int) * (ntiles + 1)));
+    td.cell_of = (int*)take(i4);
+    td.members = (int*)take(i4);
+    td.ctl = (TileCtl*)take(al(sizeof(TileCtl)));
+    td.n = n; td.ncx = ncx; td.ncy = ncy; td.ncell = (int)ncell; td.T = T; td.ntx = ntx; td.ntiles = (int)ntiles;
+    std::copy(key, key + 4, ctx->tile_key);
+  }
+  td.W = W;
+  HIPCHK(hipMemsetAsync(td.count, 0, sizeof(int) * (size_t)td.ncell, ctx->st));
+  HIPCHK(hipMemsetAsync(td.tcount, 0, sizeof(int) * (size_t)td.ntiles, ctx->st));
+  td.wx = nl.wx; td.wy = nl.wy; td.xlo = ctx->box.xlo; td.ylo = ctx->box.ylo;
+  td.cut2 = nl.cut2; td.pad2q = nl.pad2q;
+  return 0;
+}
+
+int read_tile_ctl(samos_ctx* ctx, TileCtl* h) {
+  HIPCHK(hipMemcpyAsync(h, ctx->td.ctl, sizeof(TileCtl), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+// par: the position buffer step t reads (its offset from the range start, mod 2).
+void tile_fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int t_off, int steps, bool f32, double mass_u,
+                     int par) {
+  FusedArgs fa;
+  fa.dt = I.dt; fa.gamma = I.gamma; fa.temp = I.temp; fa.seed = I.seed;
+  fa.clock = ctx->d_clock; fa.clock_slot = I.slot; fa.t_off = t_off; fa.parity = 0;
+  fa.head_enforce = t + 1 <= steps ? 1 : 0;
+  fa.mass_u = mass_u;
+  // BAOAB O-step constants (src/integrators/integrator_langevin.cpp:57-58)
+  fa.exp_dt = std::exp(-I.gamma * I.dt);
+  fa.Bc = I.temp.kind == SAMOS_VALUE_CONSTANT ? std::sqrt(I.temp.tmin * (1.0 - std::exp(-2.0 * I.gamma * I.dt))) : -1.0;
+  fa.batch = 4;
+  fa.age_uniform = 1;
+  const Fields f = fields(ctx);
+  const SlotOut so{f.fx, f.fy, f.fz, f.ix, f.iy, f.iz};
+  PairConst pc{};
+  const int nn = kMaxTypes * kMaxTypes;
+  if (ctx->has_soft) { pc.soft_k = ctx->soft_tab[0]; pc.soft_a = ctx->soft_tab[nn]; }
+  if (ctx->has_lj) {
+    pc.lj_eps = ctx->lj_tab[0];
+    pc.lj_sig2 = ctx->lj_tab[nn] * ctx->lj_tab[nn];
+    pc.lj_rc2 = ctx->lj_tab[2 * nn] * ctx->lj_tab[2 * nn];
+  }
+  pc.has_sp = ctx->has_sp ? 1 : 0;
+  pc.sp_alpha = ctx->sp_alpha;
+  {
+    ProfScope ps(ctx, "fused");
+    launch_tile_step(ctx->st, pot_mask(ctx), phase, f32 ? 1 : 0, ctx->td, so, ctx->box, ctx->con, pc, fa, par);
+  }
+  if (phase != 2) {
+    ProfScope ps(ctx, "rebuild");
+    launch_tile_build(ctx->st, ctx->td, ctx->box, 0, par);
+  }
+}
+
+// Iterations a..b-1 of a run on the tiled engine; same contract as
+// run_fused_langevin.  Returns 1 (nothing changed) when the system does not fit
+// the engine, so the caller runs the list-based path instead.
+int run_tiled_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int flags, double mass_u) {
+  const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
+  if (int r = tile_prepare(ctx)) return r;
+  Fields f = fields(ctx);
+  f.xy = ctx->xy[0];
+  launch_tile_load(ctx->st, ctx->td, f, ctx->nl);
+  launch_tile_build(ctx->st, ctx->td, ctx->box, 1, 1);
+  TileCtl h;
+  if (int r = read_tile_ctl(ctx, &h)) return r;
+  if (h.overflow || !(h.word & 2) || h.maxcand > 48 * ctx->td.W) return 1;  // too dense for the masks: list path
+  if (int r = upload_clock(ctx, a)) return r;
+  tile_fused_step(ctx, I, 0, a, 0, steps, f32, mass_u, 0);
+  launch_clock_advance(ctx->st, ctx->d_clock, 1);
+  int t = a + 1;
+  const int K = 64;  // steps per captured chunk
+  const bool graph = !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
+  if (graph && b - t >= 2 * K) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < K; ++j) tile_fused_step(ctx, I, 1, t + j, j, steps, f32, mass_u, (t + j - a) & 1);
+    launch_clock_advance(ctx->st, ctx->d_clock, K);
+    HIPCHK(hipStreamEndCapture(ctx->st, &g));
+    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    while (b - t > K) {
+      HIPCHK(hipGraphLaunch(ge, ctx->st));
+      t += K;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    HIPCHK(hipGraphExecDestroy(ge));
+    HIPCHK(hipGraphDestroy(g));
+  }
+  const int t0 = t;
+  for (; t < b; ++t) tile_fused_step(ctx, I, 1, t, t - t0, steps, f32, mass_u, (t - a) & 1);
+  launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
+  tile_fused_step(ctx, I, 2, b, 0, steps, f32, mass_u, (b - a) & 1);
+  launch_tile_store(ctx->st, ctx->td, f, ctx->nl, (b - a) & 1);
+  HIPCHK(hipGetLastError());
+  if (int r = read_tile_ctl(ctx, &h)) return r;
+  ctx->tile_stats[0] += 1;
+  ctx->tile_stats[1] += h.rebuilds;
+  ctx->tile_stats[2] = std::max<long long>(ctx->tile_stats[2], h.maxcand);
+  ctx->tile_stats[3] = std::max<long long>(ctx->tile_stats[3], h.maxhalo);
+  if (h.far) return fail(ctx, "tiled run: a particle moved more than one cell width between neighbour-list builds");
+  if (h.overflow)
+    return fail(ctx, "tiled run: a particle had more than " + std::to_string(64 * ctx->td.W) +
+                         " stencil candidates (largest " + std::to_string(h.maxcand) + ")");
+  if (h.rebuilds > 0) {
+    // The slot-ordered list is rebuilt at the last build's snapshot, so it
+    // holds exactly the pair set (and coordination) of that build.
+    int before[8], after[8];
+    if (int r = read_flags(ctx, before)) return r;
+    Fields at = fields(ctx);
+    at.x = ctx->nl.ox; at.y = ctx->nl.oy;
+    if (int r = build_sync(ctx, &at)) return r;
+    if (int r = read_flags(ctx, after)) return r;
+    after[4] = before[4] + h.rebuilds;
+    HIPCHK(hipMemcpy(ctx->d_flags + 4, &after[4], sizeof(int), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
+  I.calls += (unsigned long long)(b - a);
+  double age = ctx->age_u;
+  for (int k = a; k < b; ++k) age += I.dt;  // one closing half-kick per iteration
+  std::vector<double> col(ctx->n, age);
+  HIPCHK(hipMemcpyAsync(ctx->dcol[16], col.data(), sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->age_uniform = false;
   return 0;
 }
 
@@ -634,6 +832,7 @@ int samos_destroy(samos_ctx* ctx) {
   if (ctx->h_out) hipHostFree(ctx->h_out);
   hipFree(ctx->keys); hipFree(ctx->keys2); hipFree(ctx->perm); hipFree(ctx->perm2);
   hipFree(ctx->dtmp); hipFree(ctx->itmp); hipFree(ctx->sort_tmp);
+  hipFree(ctx->tile_mem);
   if (ctx->st) hipStreamDestroy(ctx->st);
   delete ctx;
   return 0;
@@ -1213,6 +1412,13 @@ int samos_sync(samos_ctx* ctx) {
   return 0;
 }
 
+int samos_tile_info(samos_ctx* ctx, long long* info) {
+  for (int k = 0; k < 4; ++k) info[k] = ctx->tile_stats[k];
+  info[4] = ctx->tile_mem ? ctx->td.T : 0;
+  info[5] = ctx->tile_mem ? ctx->td.W : 0;
+  return 0;
+}
+
 int samos_set_option(samos_ctx* ctx, const char* name, double value) {
   const std::string k = name ? name : "";
   if (k == "rebuild_coop") {
@@ -1220,6 +1426,10 @@ int samos_set_option(samos_ctx* ctx, const char* name, double value) {
   } else if (k == "gather_batch") {
     if (value != 4.0 && value != 8.0) return fail(ctx, "gather_batch must be 4 or 8");
     ctx->gather_batch = (int)value;
+  } else if (k == "tile_cells") {
+    if (value < 0.0 || value > kTileMaxCells || value != std::floor(value))
+      return fail(ctx, "tile_cells must be 0 (automatic) or an integer in 1.." + std::to_string(kTileMaxCells));
+    ctx->tile_cells = (int)value;
   } else {
     return fail(ctx, "unknown option " + k);
   }

@@ -320,7 +320,14 @@ __global__ void __launch_bounds__(kBlock) k_baoab_fused(Fields cur, Fields nxt, 
       p.vx *= exp_dt; p.vy *= exp_dt; p.vz *= exp_dt;
       if (B != 0.0) {
         const double sf = B / sqrt(m);
-        double g[3]; gauss3(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], eff_call(clk, slot, 0ULL) + (unsigned long long)t_off, g);
+        const unsigned long long call = eff_call(clk, slot, 0ULL) + (unsigned long long)t_off;
+        double g[3];
+        if (PLANAR) {  // the z draw only reaches v_z, which the next enforce resets
+          gauss2(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], call, g);
+          g[2] = 0.0;
+        } else {
+          gauss3(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], call, g);
+        }
         p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
       }
       p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;

@@ -151,6 +151,14 @@ __device__ __forceinline__ void gauss3(uint32_t seed, uint32_t tag, uint32_t pid
   box_muller(r.c, r.d, g[2], g3);
 }
 
+// The first two Gaussians of gauss3 (the same draw block and pair): enough for
+// a planar system, whose z velocity every plane enforce resets.
+__device__ __forceinline__ void gauss2(uint32_t seed, uint32_t tag, uint32_t pid, unsigned long long call, double g[2]) {
+  U4 c{pid, (uint32_t)call, (uint32_t)(call >> 32), 0u};
+  U4 r = philox4x32_10(c, seed, tag);
+  box_muller(r.a, r.b, g[0], g[1]);
+}
+
 __device__ __forceinline__ void gauss1_u1(uint32_t seed, uint32_t tag, uint32_t pid, unsigned long long call, double& g, double& u) {
   U4 c{pid, (uint32_t)call, (uint32_t)(call >> 32), 1u};
   U4 r = philox4x32_10(c, seed, tag);

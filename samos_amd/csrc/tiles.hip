@@ -1,0 +1,696 @@
+// Cell-tiled fused Langevin step for planar single systems.
+//
+// Same physics and per-pair arithmetic as k_baoab_fused (integrate.hip) and
+// the same pair set as the neighbour-list build (nlist.hip), restating
+//   cell list / build     src/system/cell_list.cpp:37-104, src/system/neighbour_list.cpp:216-302
+//   soft, LJ pair terms   src/potentials/pair/pair_soft_potential.cpp:66-109,
+//                         src/potentials/pair/pair_lj_potential.cpp:48-121
+//   Langevin BAOAB        src/integrators/integrator_langevin.cpp:54-118
+//   plane constraint      src/constraints/constraint_plane.cpp:36-88
+// with a different data layout:
+//   * the state of a run lives in cell order (row-major cells of the 2-D grid,
+//     members of a cell contiguous), in two sets that a build swaps between;
+//   * a workgroup owns T consecutive cells of one row and stages the positions
+//     of the 3 x (T + 2) cells around them in LDS with coalesced loads;
+//   * a particle's Verlet list is a bitmask over its 3 x 3 stencil candidates
+//     in that staging order (bit k set: candidate k was within rcut + pad at the
+//     last build), so a step reads 8 * W bytes of list instead of 4 bytes per
+//     neighbour.
+// Launch order and control: the position buffer a step reads alternates with
+// the step and is a launch argument; which state set is current and whether a
+// build is requested live in one control word changed only by atomics (see
+// TileCtl), so a captured sequence of step and build kernels needs no host
+// decisions and no device-wide fences.
+#include "tiles.hpp"
+#include "constraint.hpp"
+
+namespace samos {
+
+namespace {
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+__device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ int gsize() { return gridDim.x * blockDim.x; }
+
+__device__ __forceinline__ unsigned long long eff_call(const ClockD* clk, int slot, unsigned long long call) {
+  return clk ? clk->calls[slot] + call : call;
+}
+__device__ __forceinline__ int eff_step(const ClockD* clk, int t_off) { return clk ? clk->run_step + t_off : t_off; }
+
+__device__ __forceinline__ int load_agent(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive scan of one value per thread over a 256-thread workgroup.
+__device__ __forceinline__ int block_exscan(int v, int* lds4, int& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) lds4[wave] = incl;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kTileThreads / 64; ++w) {
+    const int ws = lds4[w];
+    if (w < wave) off += ws;
+    tot += ws;
+  }
+  __syncthreads();
+  total = tot;
+  return off + incl - v;
+}
+
+__device__ __forceinline__ void wave_max_to(int v, int* dst) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(dst, v);
+}
+
+// counter[key] += 1 for every active lane with one atomic per distinct key
+// in the wavefront (lanes of a cell-ordered set share keys in runs).  Every
+// lane of the wavefront must call it.
+__device__ __forceinline__ void wave_add1(int* counter, int key, bool active) {
+  unsigned long long todo = __ballot(active);
+  const int lane = threadIdx.x & 63;
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const int k0 = __shfl(key, leader, 64);
+    const unsigned long long same = todo & __ballot(key == k0);
+    if (lane == leader) atomicAdd(&counter[k0], __popcll(same));
+    todo &= ~same;
+  }
+}
+
+__device__ __forceinline__ int ctl_word(const TileCtl* ctl) {
+  return __hip_atomic_load(&ctl->word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Tile of the workgroup: XCD-aware, so the eight XCDs (workgroup i goes to
+// XCD i mod 8) each sweep one contiguous band of rows and keep the halo rows
+// they share in their own L2.
+struct TileGeom {
+  bool active;
+  int ty, i0, nc, nh;
+};
+
+__device__ __forceinline__ int xcd_tile(const TileD& td) {
+  const int per = (td.ntiles + 7) >> 3;
+  return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+
+__device__ __forceinline__ TileGeom tile_geom(const TileD& td, int t) {
+  TileGeom g;
+  g.active = t < td.ntiles;
+  const int tt = g.active ? t : 0;
+  g.ty = tt / td.ntx;
+  const int tx = tt - g.ty * td.ntx;
+  g.i0 = tx * td.T;
+  g.nc = min(td.T, td.ncx - g.i0);
+  g.nh = 3 * (g.nc + 2);
+  return g;
+}
+
+// Halo cell h (row r = h / (nc + 2), column k = h % (nc + 2)) is cell
+// (i0 - 1 + k, ty - 1 + r), both wrapped.  s_hcs[h] is its first staged
+// index, s_hgs[h] its first index in the set's cell order; s_hcs[nh] = total.
+__device__ __forceinline__ int stage_index(const TileD& td, const TileGeom& g, const int* cs, int* s_hcs, int* s_hgs,
+                                           int* s_scan) {
+  int cnt = 0;
+  const int h = threadIdx.x;
+  if (g.active && h < g.nh) {
+    const int r = h / (g.nc + 2), k = h - r * (g.nc + 2);
+    const int ci = wrapi(g.i0 - 1 + k, td.ncx), cj = wrapi(g.ty - 1 + r, td.ncy);
+    const int c = cj * td.ncx + ci;
+    const int a = cs[c];
+    cnt = cs[c + 1] - a;
+    s_hgs[h] = a;
+  }
+  int total;
+  const int ex = block_exscan(cnt, s_scan, total);
+  if (h < g.nh) s_hcs[h] = ex;
+  if (h == 0) s_hcs[g.nh] = total;
+  __syncthreads();
+  return total;
+}
+
+// Global (set-order) index of staged index k: each row is three contiguous
+// pieces (left halo cell, the tile's own cells, right halo cell).
+__device__ __forceinline__ int staged_global(const TileGeom& g, const int* s_hcs, const int* s_hgs, int k) {
+  int h = 0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int base = r * (g.nc + 2);
+    if (k >= s_hcs[base]) h = base;
+    if (k >= s_hcs[base + 1]) h = base + 1;
+    if (k >= s_hcs[base + g.nc + 1]) h = base + g.nc + 1;
+  }
+  return s_hgs[h] + (k - s_hcs[h]);
+}
+
+// Own column (1 .. nc) of staged index k of the middle row: the last column
+// whose first staged index is <= k.
+__device__ __forceinline__ int own_column(const TileGeom& g, const int* s_hcs, int k) {
+  const int base = g.nc + 2;
+  int lo = 1, hi = g.nc;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s_hcs[base + mid] <= k) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Build, kernel 1: bin every particle by its snapshot position
+// (CellList::get_cell_idx, src/system/cell_list.cpp:72-79, clamped) and count
+// per cell and per tile.  par: the position buffer holding the current
+// positions.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kTileThreads) k_tile_bin(TileD td, int par) {
+  const TileCtl* ctl = td.ctl;
+  const int w = ctl_word(ctl);
+  if (!(w & (4 << par)) || ctl->far) return;
+  const int S = w & 1, init = ctl->init;
+  const TileSet& A = td.sets[S];
+  bool far = false;
+  for (int base = blockIdx.x * blockDim.x; base < td.n; base += gsize()) {
+    const int p = base + threadIdx.x;
+    const bool active = p < td.n;
+    double2 q = make_double2(0.0, 0.0);
+    if (active) q = init ? A.o[p] : A.xy[par ^ 1][p];
+    int i = (int)((q.x - td.xlo) / td.wx);
+    int j = (int)((q.y - td.ylo) / td.wy);
+    i = min(max(i, 0), td.ncx - 1);
+    j = min(max(j, 0), td.ncy - 1);
+    const int c = j * td.ncx + i;
+    if (active) td.cell_of[p] = c;
+    wave_add1(td.count, c, active);
+    wave_add1(td.tcount, j * td.ntx + i / td.T, active);
+    if (active && !init) {
+      const double2 o = A.o[p];
+      int oi = (int)((o.x - td.xlo) / td.wx);
+      int oj = (int)((o.y - td.ylo) / td.wy);
+      oi = min(max(oi, 0), td.ncx - 1);
+      oj = min(max(oj, 0), td.ncy - 1);
+      int di = abs(i - oi), dj = abs(j - oj);
+      di = min(di, td.ncx - di);
+      dj = min(dj, td.ncy - dj);
+      far |= (di > 1 || dj > 1);
+    }
+  }
+  if (__ballot(far) != 0 && (threadIdx.x & 63) == 0) atomicOr(&td.ctl->far, 1);
+}
+
+// Build, kernel 2 (a range of kTilesPerBlock tiles per workgroup): the tile
+// starts (the sum of all earlier tiles' counts, then a scan over the range),
+// each cell's start (tile start + counts of the tile's earlier cells) and,
+// after the first build, each cell's member list gathered from the 3 x 3 old
+// cells around it (a particle moves by at most one cell between builds, which
+// kernel 1 checks), sorted by slot.
+constexpr int kTilesPerBlock = 8;
+
+__global__ void __launch_bounds__(kTileThreads) k_tile_cells(TileD td, int par) {
+  const TileCtl* ctl = td.ctl;
+  const int w = ctl_word(ctl);
+  if (!(w & (4 << par)) || ctl->far) return;
+  const int S = w & 1, init = ctl->init, N = S ^ 1;
+  const int* cso = td.sets[S].cstart;
+  int* csn = td.sets[N].cstart;
+  __shared__ int s_scan[kTileThreads / 64];
+  __shared__ int s_tstart[kTilesPerBlock + 1];
+  const int tb = blockIdx.x * kTilesPerBlock, te = min(tb + kTilesPerBlock, td.ntiles);
+  int part = 0;
+  for (int t = threadIdx.x; t < tb; t += kTileThreads) part += td.tcount[t];
+  int before;
+  block_exscan(part, s_scan, before);
+  const int own = (int)threadIdx.x < te - tb ? td.tcount[tb + threadIdx.x] : 0;
+  int total;
+  const int ex = block_exscan(own, s_scan, total);
+  if ((int)threadIdx.x < te - tb) s_tstart[threadIdx.x] = before + ex;
+  if (threadIdx.x == 0 && te == td.ntiles) csn[td.ncell] = before + total;
+  __syncthreads();
+  // cells of the range: tile t covers cells [j*ncx + tx*T, j*ncx + min(tx*T + T, ncx))
+  bool far = false;
+  for (int u = threadIdx.x; u < (te - tb) * td.T; u += kTileThreads) {
+    const int t = tb + u / td.T, l = u % td.T;
+    const int j = t / td.ntx, i0 = (t - j * td.ntx) * td.T;
+    const int i = i0 + l;
+    if (i >= td.ncx) continue;
+    const int cb = j * td.ncx + i0, c = cb + l;
+    int start = s_tstart[t - tb];
+    for (int k = cb; k < c; ++k) start += td.count[k];
+    csn[c] = start;
+    if (!init) {
+      const int cnt = td.count[c];
+      int m = 0;
+      for (int dj = -1; dj <= 1; ++dj) {
+        const int jj = wrapi(j + dj, td.ncy);
+        for (int di = -1; di <= 1; ++di) {
+          const int oc = jj * td.ncx + wrapi(i + di, td.ncx);
+          const int e = cso[oc + 1];
+          for (int p = cso[oc]; p < e; ++p) {
+            if (td.cell_of[p] == c) {
+              if (m < cnt) td.members[start + m] = p;
+              ++m;
+            }
+          }
+        }
+      }
+      far |= (m != cnt);
+      // canonical order inside a cell: ascending slot, as the first build of a
+      // run sorts it, so the order depends only on the snapshot
+      const int* key = td.sets[S].slot;
+      const int e = start + min(m, cnt);
+      for (int x = start + 1; x < e; ++x) {
+        const int mx = td.members[x];
+        const int kx = key[mx];
+        int y = x - 1;
+        while (y >= start && key[td.members[y]] > kx) {
+          td.members[y + 1] = td.members[y];
+          --y;
+        }
+        td.members[y + 1] = mx;
+      }
+    }
+  }
+  if (__ballot(far) != 0 && (threadIdx.x & 63) == 0) atomicOr(&td.ctl->far, 1);
+}
+
+// First build of a run only: members by atomic scatter (counts return to
+// zero), then each cell sorted by slot, which makes the order deterministic.
+__global__ void __launch_bounds__(kTileThreads) k_tile_scatter(TileD td, int par) {
+  const TileCtl* ctl = td.ctl;
+  const int w = ctl_word(ctl);
+  if (!(w & (4 << par)) || !ctl->init) return;
+  const int* csn = td.sets[(w & 1) ^ 1].cstart;
+  for (int p = gtid(); p < td.n; p += gsize()) {
+    const int c = td.cell_of[p];
+    const int r = atomicSub(&td.count[c], 1) - 1;
+    td.members[csn[c] + r] = p;
+  }
+}
+
+__global__ void __launch_bounds__(kTileThreads) k_tile_sort(TileD td, int par) {
+  const TileCtl* ctl = td.ctl;
+  const int w = ctl_word(ctl);
+  if (!(w & (4 << par)) || !ctl->init) return;
+  const int S = w & 1;
+  const int* csn = td.sets[S ^ 1].cstart;
+  const int* key = td.sets[S].slot;
+  for (int c = gtid(); c < td.ncell; c += gsize()) {
+    const int a = csn[c], e = csn[c + 1];
+    for (int i = a + 1; i < e; ++i) {
+      const int m = td.members[i];
+      const int km = key[m];
+      int j = i - 1;
+      while (j >= a && key[td.members[j]] > km) {
+        td.members[j + 1] = td.members[j];
+        --j;
+      }
+      td.members[j + 1] = m;
+    }
+  }
+}
+
+// Build, kernel 3 (per tile): stage the snapshot positions of the halo in new
+// order, write the new set for the tile's own particles and their stencil
+// masks (NeighbourList::build_cell's test, src/system/neighbour_list.cpp:
+// 240-262: dx = x_i - x_j, single-wrap image, d2 = dx*dx + dy*dy + dz*dz,
+// kept when d2 < (rcut + pad)^2).  The current positions are in buffer
+// par ^ 1, and the new set's go there too (the next step reads it).  Tiles
+// are visited grid-stride.  Marks the build finished (bit 1 of the word).
+__global__ void __launch_bounds__(kTileThreads) k_tile_build(TileD td, BoxD b, int par_req) {
+  TileCtl* ctl = td.ctl;
+  const int w = ctl_word(ctl);
+  if (!(w & (4 << par_req)) || ctl->far) return;
+  const int S = w & 1, init = ctl->init, N = S ^ 1, par = par_req ^ 1;
+  __shared__ double2 s_snap[kTileHaloMax];
+  __shared__ int s_hcs[3 * (kTileMaxCells + 2) + 1];
+  __shared__ int s_hgs[3 * (kTileMaxCells + 2)];
+  __shared__ int s_scan[kTileThreads / 64];
+  bool over = false;
+  int cmax = 0, hmax = 0;
+  for (int t = blockIdx.x; t < td.ntiles; t += gridDim.x) {
+  const TileGeom g = tile_geom(td, t);
+  const int* csn = td.sets[N].cstart;
+  const int total = stage_index(td, g, csn, s_hcs, s_hgs, s_scan);
+  const bool staged = total <= kTileHaloMax;
+  const double2* snap_src = init ? td.sets[S].o : td.sets[S].xy[par];
+  if (g.active && staged) {
+    for (int k = threadIdx.x; k < total; k += kTileThreads)
+      s_snap[k] = snap_src[td.members[staged_global(g, s_hcs, s_hgs, k)]];
+  }
+  __syncthreads();
+  hmax = max(hmax, total);
+  if (g.active) {
+    const int row1 = g.nc + 2;
+    const int kb = s_hcs[row1 + 1], ke = s_hcs[row1 + g.nc + 1];
+    const int qa = s_hgs[row1 + 1];
+    const double zero = 0.0;
+    for (int k = kb + (int)threadIdx.x; k < ke; k += kTileThreads) {
+      const int q = qa + (k - kb);
+      const int mem = td.members[q];
+      const double2 si = staged ? s_snap[k] : snap_src[mem];
+      td.sets[N].xy[par][q] = td.sets[S].xy[par][mem];
+      td.sets[N].o[q] = si;
+      td.sets[N].v[q] = td.sets[S].v[mem];
+      td.sets[N].nd[q] = td.sets[S].nd[mem];
+      td.sets[N].id[q] = td.sets[S].id[mem];
+      td.sets[N].slot[q] = td.sets[S].slot[mem];
+      const int lc = (td.cell_of[mem] % td.ncx) - g.i0 + 1;
+      unsigned long long m[4] = {0ULL, 0ULL, 0ULL, 0ULL};
+      int base = 0;
+      for (int r = 0; r < 3; ++r) {
+        const int a = s_hcs[r * row1 + lc - 1], e = s_hcs[r * row1 + lc + 2];
+        for (int kk = a; kk < e; ++kk) {
+          if (kk == k) continue;
+          const double2 sj = staged ? s_snap[kk] : snap_src[td.members[staged_global(g, s_hcs, s_hgs, kk)]];
+          double dx = si.x - sj.x, dy = si.y - sj.y, dz = zero;
+          min_image(b, dx, dy, dz);
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          const int bit = base + (kk - a);
+          if (d2 < td.cut2 && bit < 64 * td.W) m[bit >> 6] |= 1ULL << (bit & 63);
+        }
+        base += e - a;
+      }
+      over |= base > 64 * td.W;
+      cmax = max(cmax, base);
+      for (int w = 0; w < td.W; ++w) td.mask[(long)w * td.n + q] = m[w];
+    }
+    if ((int)threadIdx.x < g.nc) td.count[g.ty * td.ncx + g.i0 + threadIdx.x] = 0;
+    if (threadIdx.x == 0) td.tcount[t] = 0;
+  }
+  __syncthreads();  // the stage is reused by the next tile
+  }
+  if (__ballot(over) != 0 && (threadIdx.x & 63) == 0) atomicOr(&ctl->overflow, 1);
+  wave_max_to(cmax, &ctl->maxcand);
+  if (threadIdx.x == 0 && hmax > 0) atomicMax(&ctl->maxhalo, hmax);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicOr(&ctl->word, 2);
+    if (!init) atomicAdd(&ctl->rebuilds, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-pair force terms of a planar system (dz = 0), as pair_force_sum in
+// pair_sum.hpp with one particle type: d = x_j - x_i.
+// ---------------------------------------------------------------------------
+template <int POT, bool F32>
+__device__ __forceinline__ void pair_term(const PairConst& pc, double dx, double dy, double& fx, double& fy) {
+  const double dz = 0.0;
+  const double r_sq = dx * dx + dy * dy + dz * dz;
+  if (POT & POT_SOFT) {
+    if (F32) {
+      const float r = sqrtf((float)r_sq);
+      if (r < (float)pc.soft_a) {
+        const float diff = (float)pc.soft_a - r;
+        const float ff = (r > 0.0f) ? (float)pc.soft_k * diff / r : (float)pc.soft_k * diff;
+        fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy);
+      }
+    } else {
+      const double r = sqrt(r_sq);
+      if (r < pc.soft_a) {
+        const double diff = pc.soft_a - r;
+        const double ff = (r > 0.0) ? pc.soft_k * diff / r : pc.soft_k * diff;
+        fx -= ff * dx; fy -= ff * dy;
+      }
+    }
+  }
+  if (POT & POT_LJ) {
+    if (r_sq <= pc.lj_rc2) {
+      if (F32) {
+        const float s2 = (float)pc.lj_sig2 / (float)r_sq;
+        const float s6 = s2 * s2 * s2;
+        const float ff = 48.0f * (float)pc.lj_eps * s6 * (s6 - 0.5f) * s2;
+        fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy);
+      } else {
+        const double inv_r_sq = pc.lj_sig2 / r_sq;
+        const double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
+        const double ff = 48.0 * pc.lj_eps * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
+        fx -= ff * dx; fy -= ff * dy;
+      }
+    }
+  }
+}
+
+// Candidate ranges of a particle in staged order: row r's three stencil
+// cells are staged indices [a_r, e_r); mask bit b is candidate b of the
+// concatenation (rows 0, 1, 2).
+struct Stencil {
+  int a0, a1, a2, c0, c1;
+  __device__ __forceinline__ int staged(int bit) const {
+    int off = a0;
+    off = bit >= c0 ? a1 - c0 : off;
+    off = bit >= c1 ? a2 - c1 : off;
+    return off + bit;
+  }
+};
+
+// Forces on one particle from its masked candidates, kB candidates per batch
+// so that kB staged reads are in flight together.  A batch slot without a
+// candidate reads the particle's own row start and is moved out of range.
+// WRAP: the tile's halo crosses the periodic boundary, so the single-wrap image
+// is applied (inside the box it never changes a difference and is skipped).
+// STAGED: candidates come from LDS, else from the set's positions (L2).
+template <int POT, bool F32, bool WRAP, bool STAGED>
+__device__ __forceinline__ void masked_forces(const BoxD& b, const PairConst& pc, const Stencil& sc,
+                                              const unsigned long long* mw, int W, const double2* src,
+                                              const TileGeom& g, const int* s_hcs, const int* s_hgs, double xi,
+                                              double yi, double& fx, double& fy) {
+  constexpr int kB = 4;
+  for (int w = 0; w < W; ++w) {
+    unsigned long long m = mw[w];
+    while (m) {
+      int k[kB];
+      bool ok[kB];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        ok[u] = m != 0ULL;
+        const int bit = 64 * w + __builtin_ctzll(m | (1ULL << 63));
+        m &= m - 1;
+        k[u] = ok[u] ? sc.staged(bit) : sc.a1;
+      }
+      double2 pj[kB];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) pj[u] = STAGED ? src[k[u]] : src[staged_global(g, s_hcs, s_hgs, k[u])];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        double dx = ok[u] ? pj[u].x - xi : 1.0e30, dy = pj[u].y - yi;
+        if (WRAP) {
+          double dz = 0.0;
+          min_image(b, dx, dy, dz);
+        }
+        pair_term<POT, F32>(pc, dx, dy, fx, fy);
+      }
+    }
+  }
+}
+
+// One fused BAOAB step on the current set: phases as k_baoab_fused.
+template <int POT, int PHASE, bool F32>
+__global__ void __launch_bounds__(kTileThreads) k_tile_step(TileD td, SlotOut so, BoxD b, ConstraintD c,
+                                                            PairConst pc, FusedArgs fa, int par) {
+  TileCtl* ctl = td.ctl;
+  const int w = ctl_word(ctl);
+  const int S = (w ^ (w >> 1)) & 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // commit a finished build, clear the request it answered
+    if (w & 2) atomicXor(&ctl->word, 3);
+    atomicAnd(&ctl->word, ~(4 << (par ^ 1)));
+    if (ctl->init) ctl->init = 0;
+  }
+  const TileSet& A = td.sets[S];
+  __shared__ double2 s_pos[kTileHaloMax];
+  __shared__ int s_hcs[3 * (kTileMaxCells + 2) + 1];
+  __shared__ int s_hgs[3 * (kTileMaxCells + 2)];
+  __shared__ int s_scan[kTileThreads / 64];
+  const TileGeom g = tile_geom(td, xcd_tile(td));
+  const double2* cur = A.xy[par];
+  const int total = stage_index(td, g, A.cstart, s_hcs, s_hgs, s_scan);
+  const bool staged = total <= kTileHaloMax;
+  if (g.active && staged) {
+    for (int k = threadIdx.x; k < total; k += kTileThreads) s_pos[k] = cur[staged_global(g, s_hcs, s_hgs, k)];
+  }
+  __syncthreads();
+  const bool wrap = b.periodic && (g.i0 == 0 || g.i0 + g.nc >= td.ncx || g.ty == 0 || g.ty == td.ncy - 1);
+  bool need = false;
+  if (g.active) {
+    const int row1 = g.nc + 2;
+    const int kb = s_hcs[row1 + 1], ke = s_hcs[row1 + g.nc + 1];
+    const int qa = s_hgs[row1 + 1];
+    const double dt = fa.dt, dt2 = 0.5 * dt, m = fa.mass_u, fact = dt2 / m;
+    for (int k = kb + (int)threadIdx.x; k < ke; k += kTileThreads) {
+      const int q = qa + (k - kb);
+      unsigned long long mw[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) mw[w] = w < td.W ? td.mask[(long)w * td.n + q] : 0ULL;
+      const double2 pi = staged ? s_pos[k] : cur[q];
+      const int lc = own_column(g, s_hcs, k);
+      Stencil sc;
+      sc.a0 = s_hcs[lc - 1];
+      sc.a1 = s_hcs[row1 + lc - 1];
+      sc.a2 = s_hcs[2 * row1 + lc - 1];
+      sc.c0 = s_hcs[lc + 2] - sc.a0;
+      sc.c1 = sc.c0 + (s_hcs[row1 + lc + 2] - sc.a1);
+      // noise first, so its temporaries are dead before the force loop; the z
+      // draw only reaches v_z, which the plane enforce resets
+      double gn[2] = {0.0, 0.0};
+      double B = fa.Bc;
+      if (PHASE <= 1) {
+        if (B < 0.0) {
+          const double T = fa.temp.at(eff_step(fa.clock, fa.t_off));
+          B = sqrt(T * (1.0 - exp(-2.0 * fa.gamma * dt)));
+        }
+        if (B != 0.0)
+          gauss2(fa.seed, TAG_LANGEVIN, (uint32_t)A.id[q],
+                 eff_call(fa.clock, fa.clock_slot, 0ULL) + (unsigned long long)fa.t_off, gn);
+      }
+      double fx = 0.0, fy = 0.0, fz = 0.0;
+      if (staged) {
+        if (wrap) masked_forces<POT, F32, true, true>(b, pc, sc, mw, td.W, s_pos, g, s_hcs, s_hgs, pi.x, pi.y, fx, fy);
+        else masked_forces<POT, F32, false, true>(b, pc, sc, mw, td.W, s_pos, g, s_hcs, s_hgs, pi.x, pi.y, fx, fy);
+      } else {
+        masked_forces<POT, F32, true, false>(b, pc, sc, mw, td.W, cur, g, s_hcs, s_hgs, pi.x, pi.y, fx, fy);
+      }
+      const double2 n0 = A.nd[q];
+      if (pc.has_sp) { fx += pc.sp_alpha * n0.x; fy += pc.sp_alpha * n0.y; fz += pc.sp_alpha * 0.0; }
+      const double2 v0 = A.v[q];
+      double vx = v0.x, vy = v0.y, vz = 0.0;
+      if (PHASE >= 1) {  // closing B of step t-1
+        vx += fact * fx; vy += fact * fy; vz += fact * fz;
+      }
+      if (PHASE <= 1) {
+        PS p;
+        p.x = pi.x; p.y = pi.y; p.z = c.zpos; p.vx = vx; p.vy = vy; p.vz = vz;
+        p.nx = n0.x; p.ny = n0.y; p.nz = 0.0;
+        p.dix = p.diy = p.diz = 0;
+        p.vx += fact * fx; p.vy += fact * fy; p.vz += fact * fz;
+        p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+        enforce(b, c, p);
+        p.vx *= fa.exp_dt; p.vy *= fa.exp_dt; p.vz *= fa.exp_dt;
+        if (B != 0.0) {
+          const double sfac = B / sqrt(m);
+          p.vx += sfac * gn[0]; p.vy += sfac * gn[1]; p.vz += sfac * 0.0;
+        }
+        p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+        enforce(b, c, p);
+        if (fa.head_enforce) enforce(b, c, p);
+        A.xy[par ^ 1][q] = make_double2(p.x, p.y);
+        A.v[q] = make_double2(p.vx, p.vy);
+        if (p.nx != n0.x || p.ny != n0.y) A.nd[q] = make_double2(p.nx, p.ny);
+        if (p.dix | p.diy | p.diz) {
+          const int s = A.slot[q];
+          if (p.dix) so.ix[s] += p.dix;
+          if (p.diy) so.iy[s] += p.diy;
+          if (p.diz) so.iz[s] += p.diz;
+        }
+        const double2 o = A.o[q];
+        double ddx = o.x - p.x, ddy = o.y - p.y, ddz = 0.0;
+        min_image(b, ddx, ddy, ddz);
+        need |= !(ddx * ddx + ddy * ddy + ddz * ddz < td.pad2q);
+      } else {
+        // closing phase: F(x_t) is the state the reference leaves behind
+        A.v[q] = make_double2(vx, vy);
+        const int s = A.slot[q];
+        so.fx[s] = fx; so.fy[s] = fy; so.fz[s] = fz;
+      }
+    }
+  }
+  if (PHASE != 2) {
+    if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomicOr(&ctl->word, 4 << par);
+  }
+}
+
+__global__ void k_tile_load(TileD td, Fields f, NListD nl) {
+  const int p = gtid();
+  if (p == 0) {
+    TileCtl z{};
+    z.word = 1 | (4 << 1);  // set 1 holds the loaded state; the first build answers parity 1
+    z.init = 1;
+    *td.ctl = z;
+  }
+  if (p < td.n) {
+    td.sets[1].xy[0][p] = make_double2(f.x[p], f.y[p]);
+    td.sets[1].o[p] = make_double2(nl.ox[p], nl.oy[p]);
+    td.sets[1].v[p] = make_double2(f.vx[p], f.vy[p]);
+    td.sets[1].nd[p] = make_double2(f.nx[p], f.ny[p]);
+    td.sets[1].id[p] = f.id[p];
+    td.sets[1].slot[p] = p;
+  }
+}
+
+__global__ void k_tile_store(TileD td, Fields f, NListD nl, int par) {
+  const int w = ctl_word(td.ctl);
+  const int S = (w ^ (w >> 1)) & 1;
+  const int q = gtid();
+  if (q < td.n) {
+    const int s = td.sets[S].slot[q];
+    const double2 p = td.sets[S].xy[par][q];
+    f.x[s] = p.x; f.y[s] = p.y;
+    if (f.xy) f.xy[s] = p;
+    const double2 v = td.sets[S].v[q];
+    f.vx[s] = v.x; f.vy[s] = v.y;
+    const double2 d = td.sets[S].nd[q];
+    f.nx[s] = d.x; f.ny[s] = d.y;
+    const double2 o = td.sets[S].o[q];
+    nl.ox[s] = o.x; nl.oy[s] = o.y;
+  }
+}
+
+int capped_grid(long n) {
+  long g = (n + kTileThreads - 1) / kTileThreads;
+  return (int)std::max(1L, std::min(g, 2048L));
+}
+
+int tile_grid(const TileD& td) { return 8 * ((td.ntiles + 7) / 8); }
+
+template <int POT, int PHASE>
+void step_dispatch(hipStream_t st, int fp32, TileD td, SlotOut so, BoxD b, ConstraintD c, PairConst pc,
+                   const FusedArgs& fa, int par) {
+  const dim3 g(tile_grid(td)), blk(kTileThreads);
+  if (fp32) hipLaunchKernelGGL((k_tile_step<POT, PHASE, true>), g, blk, 0, st, td, so, b, c, pc, fa, par);
+  else hipLaunchKernelGGL((k_tile_step<POT, PHASE, false>), g, blk, 0, st, td, so, b, c, pc, fa, par);
+}
+
+template <int POT>
+void step_phase(hipStream_t st, int phase, int fp32, TileD td, SlotOut so, BoxD b, ConstraintD c, PairConst pc,
+                const FusedArgs& fa, int par) {
+  if (phase == 0) step_dispatch<POT, 0>(st, fp32, td, so, b, c, pc, fa, par);
+  else if (phase == 1) step_dispatch<POT, 1>(st, fp32, td, so, b, c, pc, fa, par);
+  else step_dispatch<POT, 2>(st, fp32, td, so, b, c, pc, fa, par);
+}
+
+}  // namespace
+
+void launch_tile_load(hipStream_t st, TileD td, Fields f, NListD nl) {
+  hipLaunchKernelGGL(k_tile_load, dim3(grid1(td.n)), dim3(kBlock), 0, st, td, f, nl);
+}
+
+void launch_tile_build(hipStream_t st, TileD td, BoxD box, int init, int par) {
+  const int g = std::max(1, std::min(td.ntiles, 2048));
+  hipLaunchKernelGGL(k_tile_bin, dim3(capped_grid(td.n)), dim3(kTileThreads), 0, st, td, par);
+  hipLaunchKernelGGL(k_tile_cells, dim3((td.ntiles + kTilesPerBlock - 1) / kTilesPerBlock), dim3(kTileThreads), 0,
+                     st, td, par);
+  if (init) {
+    hipLaunchKernelGGL(k_tile_scatter, dim3(capped_grid(td.n)), dim3(kTileThreads), 0, st, td, par);
+    hipLaunchKernelGGL(k_tile_sort, dim3(capped_grid(td.ncell)), dim3(kTileThreads), 0, st, td, par);
+  }
+  hipLaunchKernelGGL(k_tile_build, dim3(g), dim3(kTileThreads), 0, st, td, box, par);
+}
+
+void launch_tile_step(hipStream_t st, int pot_mask, int phase, int fp32, TileD td, SlotOut out, BoxD box,
+                      ConstraintD con, PairConst pc, const FusedArgs& fa, int par) {
+  switch (pot_mask) {
+    case 1: step_phase<1>(st, phase, fp32, td, out, box, con, pc, fa, par); break;
+    case 2: step_phase<2>(st, phase, fp32, td, out, box, con, pc, fa, par); break;
+    default: step_phase<3>(st, phase, fp32, td, out, box, con, pc, fa, par); break;
+  }
+}
+
+void launch_tile_store(hipStream_t st, TileD td, Fields f, NListD nl, int par) {
+  hipLaunchKernelGGL(k_tile_store, dim3(grid1(td.n)), dim3(kBlock), 0, st, td, f, nl, par);
+}
+
+}  // namespace samos

@@ -1,0 +1,99 @@
+// Cell-tiled engine for planar single-system runs: the state lives in
+// cell order (row-major over a 2-D cell grid), each workgroup owns a run of
+// cells in one row and stages the positions of the 3 x (run + 2) surrounding
+// cells in LDS, and the Verlet list is a per-particle bitmask over that
+// stencil's candidates instead of a list of global indices.
+#pragma once
+#include "kernels.hpp"
+
+namespace samos {
+
+constexpr int kTileThreads = 256;
+constexpr int kTileMaxCells = 80;     // cells per tile along x (3 * (80 + 2) halo cells <= 256)
+constexpr int kTileHaloMax = 2048;    // halo positions staged in LDS; larger halos read L2
+
+// Control block in device memory.  `word` holds the current set and the
+// build handshake, changed only by atomics:
+//   bit 0  set of the last committed build
+//   bit 1  a build finished and is not yet committed (the current set is
+//          bit 0 ^ bit 1 for a step kernel; build kernels run only after a
+//          step kernel committed, when bit 1 is clear)
+//   bit 2 + p  a particle moved pad/2 in a step reading position buffer p:
+//          build before the next step
+// The first workgroup of every step kernel commits a finished build (flips
+// bit 0, clears bit 1) and clears the other parity's request; both keep
+// bit 0 ^ bit 1 unchanged for the workgroups reading it concurrently.
+struct TileCtl {
+  int word;
+  int init;      // the pending build is a run's first (snapshot from the slot list)
+  int rebuilds;  // builds during the run (the first one not counted)
+  int overflow;  // some particle had more stencil candidates than the mask holds
+  int far;       // some particle changed cell by more than one between builds
+  int maxcand;   // largest stencil candidate count of the builds
+  int maxhalo;   // largest staged halo of the builds
+  int pad_;
+};
+
+// One state set: the arrays of the cell-ordered state.
+struct TileSet {
+  double2* xy[2];        // positions, double-buffered by step
+  double2* v;            // velocities
+  double2* nd;           // directors
+  double2* o;            // positions at the last build
+  int* id;               // particle ids (noise keys)
+  int* slot;             // slot of each particle in the slot-ordered columns
+  int* cstart;           // ncell + 1 cell starts of this set's order
+};
+
+struct TileD {
+  int n;                 // particles
+  int ncx, ncy, ncell;   // 2-D cell grid (the reference's n_x, n_y; one z layer)
+  int T;                 // cells per tile along x
+  int ntx, ntiles;       // tiles per row, tiles
+  int W;                 // 64-bit mask words per particle
+  double wx, wy, xlo, ylo;
+  double cut2;           // (rcut + pad)^2
+  double pad2q;          // pad^2 / 4
+  const TileSet* sets;   // [2], in device memory
+  unsigned long long* mask;  // [W][n] candidate bitmasks of the current set
+  int* count;            // [ncell] particles per cell of the build in progress
+  int* tcount;           // [ntiles] particles per tile of the build in progress
+  int* tstart;           // [ntiles + 1] exclusive scan of tcount
+  int* cell_of;          // [n] new cell of each particle of the current set
+  int* members;          // [n] current-set index of each new position
+  TileCtl* ctl;
+};
+
+// Slot-ordered columns a step writes: closing forces, image-flag changes.
+struct SlotOut {
+  double *fx, *fy, *fz;
+  int *ix, *iy, *iz;
+};
+
+// Pair and self-propulsion constants of a one-type planar system.
+struct PairConst {
+  double soft_k, soft_a;          // pair_soft_potential: k, a
+  double lj_eps, lj_sig2, lj_rc2; // pair_lj_potential: epsilon, sigma^2, rcut^2
+  double sp_alpha;                // external self_propulsion alpha
+  int has_sp;
+};
+
+// Copies the slot-ordered columns into set 1 (identity order) and arms the
+// initial build: its snapshot is the slot list's (nl.ox, nl.oy), so the tiled
+// run continues the reference's current neighbour list exactly.
+void launch_tile_load(hipStream_t st, TileD td, Fields f, NListD nl);
+// One build: bin by the snapshot, cell starts, member lists, then the new
+// cell-ordered set with stencil masks.  Each kernel returns at once unless the
+// request bit of parity par is set (a step that read buffer par asked for it);
+// the current positions are in buffer par ^ 1, and so are the new set's.
+// init: the first build of a run (scatter + sort).
+void launch_tile_build(hipStream_t st, TileD td, BoxD box, int init, int par);
+// One fused BAOAB step (phases as launch_baoab_fused) on the current set,
+// reading positions from buffer par and writing buffer par ^ 1.
+void launch_tile_step(hipStream_t st, int pot_mask, int phase, int fp32, TileD td, SlotOut out, BoxD box,
+                      ConstraintD con, PairConst pc, const FusedArgs& fa, int par);
+// Writes the current set (positions in buffer par) back into the slot-ordered
+// columns (x, y, xy, v, n) and the last build's snapshot into nl.ox / nl.oy.
+void launch_tile_store(hipStream_t st, TileD td, Fields f, NListD nl, int par);
+
+}  // namespace samos

@@ -1,0 +1,168 @@
+"""Cell-tiled planar engine (tiles.hip) against the CPU oracle.
+
+Planar single-type Langevin BAOAB runs with one common mass and age go
+through the tiled engine by default: state in cell order, positions of each
+tile's 3 x (T + 2) surrounding cells staged in LDS, Verlet lists as stencil
+bitmasks.  Bars as the list-based path (tests/test_gpu_parity.py):
+
+  noise-off trajectories   |dx|, |dv| <= 1e-9 after the run (FP64)
+  forces left by the run   max_i |dF_i| / max_i |F_i| <= 1e-12 on identical
+                           positions (the final forces are F(x_end))
+  pair set after the run   bit-exact: the slot list is rebuilt at the last
+                           tiled build's snapshot
+  build count              equal to the oracle's
+  FP32 pair arithmetic     forces <= 1e-6 relative to FP64
+and, specific to the engine: the halo read from L2 instead of LDS (halo
+larger than the LDS stage), runs split into ranges bit-identical to one run,
+and agreement with the list-based kernels (SAMOS_RUN_NO_TILES) to 1e-9.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import scenarios as sc
+
+pytestmark = pytest.mark.gpu
+
+
+def setup_lj(s, gamma=1.0, T=0.0, seed=3, pad=0.3):
+    s.nlist(2.5, pad)
+    s.pair_lj(1.0, 1.0, 2.5, shifted=True)
+    s.constraint("plane")
+    s.set_timestep(0.002)
+    s.integrator_langevin(gamma=gamma, seed=seed, temp=(0, T, 1.0, 1))
+
+
+def state(s):
+    return {k: s.get(k) for k in ["x", "y", "vx", "vy", "fx", "fy", "nx", "ny", "ix", "iy"]}
+
+
+def pos_err(a, b, box):
+    return max(float(np.abs(sc.min_image(a["x"] - b["x"], box[0])).max()),
+               float(np.abs(sc.min_image(a["y"] - b["y"], box[1])).max()))
+
+
+def test_tiled_lj_langevin_matches_oracle(gpu):
+    st = sc.lj_lattice(48, rho=0.7, seed=41)
+    d, o = sc.load(gpu.System, st), sc.load(oracle.OracleSystem, st)
+    d.set_option("tile_cells", 4)
+    for s in (d, o):
+        setup_lj(s)
+    print("tiled LJ Langevin, N=%d, 800 steps" % st["n"], flush=True)
+    d0, o0 = d.nlist_info()["builds"], o.builds()
+    d.run(799)
+    o.run(799)
+    info = d.tile_info()
+    assert info["runs"] == 1 and info["rebuilds"] >= 1, info
+    a, b = state(d), state(o)
+    assert pos_err(a, b, st["box"]) <= 1e-9
+    assert max(np.abs(a[k] - b[k]).max() for k in ("vx", "vy")) <= 1e-9
+    F = np.stack([a["fx"], a["fy"]], 1)
+    G = np.stack([b["fx"], b["fy"]], 1)
+    assert np.abs(F - G).max() / np.abs(G).max() <= 1e-9
+    assert np.array_equal(a["ix"], b["ix"]) and np.array_equal(a["iy"], b["iy"])
+    assert np.array_equal(d.pairs(), o.pairs())
+    assert d.nlist_info()["builds"] - d0 == o.builds() - o0
+    # forces at the identical final positions
+    d.compute_forces()
+    o.set("x", d.get("x"))
+    o.set("y", d.get("y"))
+    o.compute_forces()
+    F = np.stack([d.get("fx"), d.get("fy")], 1)
+    G = np.stack([o.get("fx"), o.get("fy")], 1)
+    assert np.abs(F - G).max() / np.abs(G).max() <= 1e-12
+
+
+def test_tiled_soft_dense_halo_beyond_lds(gpu):
+    # rho ~ 1.2 and 80 cells per tile: staged halos exceed the LDS stage, so
+    # candidate positions are read through L2 (the fallback path)
+    st = sc.plane_disks(60000, phi=3.8, seed=42)
+    d, o = sc.load(gpu.System, st), sc.load(oracle.OracleSystem, st)
+    d.set_option("tile_cells", 80)
+    for s in (d, o):
+        s.nlist(2.4, 0.3)
+        s.pair_soft(1.0, 2.0)
+        s.constraint("plane")
+        s.set_timestep(0.002)
+        s.integrator_langevin(gamma=1.0, seed=5, temp=(0, 0.0, 1.0, 1))
+    print("tiled dense soft, N=%d, 40 steps" % st["n"], flush=True)
+    d.run(39)
+    o.run(39)
+    info = d.tile_info()
+    assert info["runs"] == 1 and info["max_halo"] > 2048 and info["tile_cells"] == 80, info
+    a, b = state(d), state(o)
+    assert pos_err(a, b, st["box"]) <= 1e-9
+    assert max(np.abs(a[k] - b[k]).max() for k in ("vx", "vy")) <= 1e-9
+    assert np.array_equal(d.pairs(), o.pairs())
+
+
+@pytest.mark.parametrize("cells", [1, 3, 0])
+def test_tiled_noise_on_same_stream(gpu, cells):
+    st = sc.lj_lattice(40, rho=0.7, seed=43)
+    d, o = sc.load(gpu.System, st), sc.load(oracle.OracleSystem, st)
+    d.set_option("tile_cells", cells)
+    for s in (d, o):
+        setup_lj(s, T=1.0, seed=7)
+    print("tiled noise-on, tile_cells=%d" % cells, flush=True)
+    d.run(199)
+    o.run(199)
+    assert d.tile_info()["runs"] == 1
+    a, b = state(d), state(o)
+    assert pos_err(a, b, st["box"]) <= 1e-9
+    assert max(np.abs(a[k] - b[k]).max() for k in ("vx", "vy")) <= 1e-9
+    assert np.array_equal(d.pairs(), o.pairs())
+
+
+def test_tiled_equals_list_kernels(gpu):
+    st = sc.lj_lattice(64, rho=0.7, seed=44)
+    runs = []
+    for flags in (0, gpu.RUN_NO_TILES):
+        d = sc.load(gpu.System, st)
+        setup_lj(d, T=1.0, seed=11)
+        d.run(499, flags)
+        runs.append((state(d), d.tile_info()["runs"], d.pairs()))
+    (a, ta, pa), (b, tb, pb) = runs
+    assert ta == 1 and tb == 0
+    assert pos_err(a, b, st["box"]) <= 1e-9
+    assert max(np.abs(a[k] - b[k]).max() for k in ("vx", "vy")) <= 1e-9
+    assert np.array_equal(pa, pb)
+
+
+def test_tiled_ranges_bit_identical(gpu):
+    st = sc.lj_lattice(40, rho=0.7, seed=45)
+    a, b = sc.load(gpu.System, st), sc.load(gpu.System, st)
+    for s in (a, b):
+        setup_lj(s, T=1.0, seed=9)
+    steps = 300
+    a.run(steps)
+    for lo, hi in [(0, 1), (1, 77), (77, 200), (200, 301)]:
+        b.run_range(lo, hi, steps)
+    assert b.tile_info()["runs"] == 4
+    sa, sb = state(a), state(b)
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_tiled_fp32_forces(gpu):
+    st = sc.lj_lattice(40, rho=0.7, seed=46)
+    d, o = sc.load(gpu.System, st), sc.load(oracle.OracleSystem, st)
+    for s in (d, o):
+        setup_lj(s)
+    d.run(20, gpu.RUN_FP32_FORCE)
+    assert d.tile_info()["runs"] == 1
+    o.set("x", d.get("x"))
+    o.set("y", d.get("y"))
+    o.compute_forces()
+    F = np.stack([d.get("fx"), d.get("fy")], 1)
+    G = np.stack([o.get("fx"), o.get("fy")], 1)
+    assert np.abs(F - G).max() / np.abs(G).max() <= 1e-6
+
+
+def test_tiled_not_used_where_it_does_not_apply(gpu):
+    # two particle types: the list-based kernels run
+    st = sc.lj_lattice(24, rho=0.7, seed=47)
+    d = sc.load(gpu.System, st)
+    d.set("type", (np.arange(st["n"]) % 2) + 1)
+    setup_lj(d)
+    d.run(10)
+    assert d.tile_info()["runs"] == 0
