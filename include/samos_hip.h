@@ -49,7 +49,7 @@ enum samos_value_kind { SAMOS_VALUE_CONSTANT = 0, SAMOS_VALUE_LINEAR = 1 };
 #define SAMOS_RUN_FAITHFUL   1  /* evaluate forces exactly where the reference does (no fused step) */
 #define SAMOS_RUN_NO_GRAPH   2  /* launch step by step instead of replaying captured chunks */
 #define SAMOS_RUN_FP32_FORCE 4  /* pair arithmetic in FP32, sums and state in FP64 */
-#define SAMOS_RUN_NO_TILES   8  /* planar fused runs: neighbour-list kernels instead of the cell-tiled engine */
+#define SAMOS_RUN_TILES      8  /* planar fused runs on the cell-tiled engine instead of the neighbour-list kernels */
 
 /* ---- context and state ---------------------------------------------------
  * System(input, msg, box) (src/system/system.cpp:64-452) and Box(lx, ly, lz)
@@ -182,7 +182,7 @@ int samos_sync(samos_ctx* ctx);
 int samos_set_option(samos_ctx* ctx, const char* name, double value);
 /* Cell-tiled planar engine (no counterpart in the reference): planar fused
  * Langevin runs keep the state in cell order and the Verlet list as stencil
- * bitmasks (see SAMOS_RUN_NO_TILES).  info[6]: tiled runs, rebuilds inside
+ * bitmasks (see SAMOS_RUN_TILES).  info[6]: tiled runs, rebuilds inside
  * them, largest stencil candidate count, largest staged halo, cells per tile
  * and mask words of the last run. */
 int samos_tile_info(samos_ctx* ctx, long long* info);

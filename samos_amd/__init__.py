@@ -26,7 +26,7 @@ FIELD_CODE.update({name: 32 + i for i, name in enumerate(INT_FIELDS)})
 
 CONSTRAINT = {"none": 0, "plane": 1, "sphere": 2}
 LANGEVIN_METHOD = {"baoab": 0, "spv": 1, "bbk": 2}
-RUN_FAITHFUL, RUN_NO_GRAPH, RUN_FP32_FORCE, RUN_NO_TILES = 1, 2, 4, 8
+RUN_FAITHFUL, RUN_NO_GRAPH, RUN_FP32_FORCE, RUN_TILES = 1, 2, 4, 8
 OBSERVABLE = {"max_force": 0, "force_sq": 1, "kinetic_energy": 2, "vec_velocity": 3,
               "velocity": 4, "polar_order": 5}
 

@@ -553,12 +553,13 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
 
 
 // ---- cell-tiled fused Langevin run (planar systems) ------------------------
-// Eligible: plane constraint, cell grid with at least three cells per side,
-// one particle type, no per-particle radii, one common mass and one common age
-// (the fused run's uniform-age path).
+// Used when the run asks for it (SAMOS_RUN_TILES) and the system fits: plane
+// constraint, cell grid with at least three cells per side, one particle type,
+// no per-particle radii, one common mass and one common age (the fused run's
+// uniform-age path).
 bool tile_eligible(samos_ctx* ctx, int flags, double mass_u) {
   const NListD& nl = ctx->nl;
-  return !(flags & SAMOS_RUN_NO_TILES) && ctx->con.kind == 1 && nl.use_cell && nl.ncx >= 3 && nl.ncy >= 3 &&
+  return (flags & SAMOS_RUN_TILES) && ctx->con.kind == 1 && nl.use_cell && nl.ncx >= 3 && nl.ncy >= 3 &&
          ctx->ntypes == 1 && !ctx->soft_radii && !ctx->lj_radii && mass_u > 0.0 && ctx->age_uniform && ctx->n > 0;
 }
 

@@ -22,6 +22,18 @@ __device__ __forceinline__ void pair_force_sum(const Fields& f, const NListD& nl
                                                int s, double xi, double yi, double zi, int ti, double ri,
                                                double& fx, double& fy, double& fz) {
   const bool radii = ((POT & POT_SOFT) && ip.soft_radii) || ((POT & POT_LJ) && ip.lj_radii);
+  // One particle type: the table entries of the pair (1, 1) are loop constants,
+  // and so are the first products of each expression (sigma^2, rcut^2,
+  // 48 epsilon), which round exactly as when formed per pair.
+  const int nt2s = ip.soft_nt * ip.soft_nt, nt2l = ip.lj_nt * ip.lj_nt;
+  double c_kk = 0.0, c_a = 0.0, c_rc2 = 0.0, c_eps = 0.0, c_e48 = 0.0, c_sig2 = 0.0;
+  if (!MT) {
+    if (POT & POT_SOFT) { c_kk = ip.soft_tab[0]; c_a = ip.soft_tab[nt2s]; }
+    if (POT & POT_LJ) {
+      const double rc = ip.lj_tab[2 * nt2l], sig = ip.lj_tab[nt2l];
+      c_eps = ip.lj_tab[0]; c_rc2 = rc * rc; c_e48 = 48.0 * c_eps; c_sig2 = sig * sig;
+    }
+  }
   const int kn = nl.nnb[s];
   const int* nb = nl.nbr + s;
   for (int k0 = 0; k0 < kn; k0 += kBatch) {
@@ -47,44 +59,52 @@ __device__ __forceinline__ void pair_force_sum(const Fields& f, const NListD& nl
       // d = x_j - x_i; LJ's reference convention x_i - x_j is -d
       double dx = xj[u] - xi, dy = yj[u] - yi, dz = zj[u] - zi;
       min_image(b, dx, dy, dz);
-      const double r_sq = dx * dx + dy * dy + dz * dz;
+      // planar: dz = 0, and adding +0 to a sum of squares changes no bit
+      const double r_sq = PLANAR ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
       if (POT & POT_SOFT) {
         const int tt = MT ? ti * ip.soft_nt + tj[u] : 0;
-        const double kk = ip.soft_tab[tt];
-        const double a = ip.soft_radii ? ri + rj[u] : ip.soft_tab[ip.soft_nt * ip.soft_nt + tt];
+        const double kk = MT ? ip.soft_tab[tt] : c_kk;
+        const double a = ip.soft_radii ? ri + rj[u] : (MT ? ip.soft_tab[nt2s + tt] : c_a);
         if (F32) {
           const float r = sqrtf((float)r_sq);
           if (r < (float)a) {
             const float diff = (float)a - r;
             const float ff = (r > 0.0f) ? (float)kk * diff / r : (float)kk * diff;
-            fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy); fz -= (double)(ff * (float)dz);
+            fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy);
+            if (!PLANAR) fz -= (double)(ff * (float)dz);
           }
         } else {
           const double r = sqrt(r_sq);
           if (r < a) {
             const double diff = a - r;
             const double ff = (r > 0.0) ? kk * diff / r : kk * diff;
-            fx -= ff * dx; fy -= ff * dy; fz -= ff * dz;
+            fx -= ff * dx; fy -= ff * dy;
+            if (!PLANAR) fz -= ff * dz;
           }
         }
       }
       if (POT & POT_LJ) {
         const int tt = MT ? ti * ip.lj_nt + tj[u] : 0;
-        const int nt2 = ip.lj_nt * ip.lj_nt;
-        const double rc = ip.lj_tab[2 * nt2 + tt];
-        if (r_sq <= rc * rc) {
-          const double eps = ip.lj_tab[tt];
-          const double sig = ip.lj_radii ? ri + rj[u] : ip.lj_tab[nt2 + tt];
+        double rc2 = c_rc2;
+        if (MT) { const double rc = ip.lj_tab[2 * nt2l + tt]; rc2 = rc * rc; }
+        if (r_sq <= rc2) {
+          const double eps = MT ? ip.lj_tab[tt] : c_eps;
+          const double e48 = MT ? 48.0 * eps : c_e48;
+          double sig2 = c_sig2;
+          if (ip.lj_radii) { const double sg = ri + rj[u]; sig2 = sg * sg; }
+          else if (MT) { const double sg = ip.lj_tab[nt2l + tt]; sig2 = sg * sg; }
           if (F32) {
-            const float s2 = (float)(sig * sig) / (float)r_sq;
+            const float s2 = (float)sig2 / (float)r_sq;
             const float s6 = s2 * s2 * s2;
             const float ff = 48.0f * (float)eps * s6 * (s6 - 0.5f) * s2;
-            fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy); fz -= (double)(ff * (float)dz);
+            fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy);
+            if (!PLANAR) fz -= (double)(ff * (float)dz);
           } else {
-            const double inv_r_sq = (sig * sig) / r_sq;
+            const double inv_r_sq = sig2 / r_sq;
             const double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
-            const double ff = 48.0 * eps * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
-            fx -= ff * dx; fy -= ff * dy; fz -= ff * dz;
+            const double ff = e48 * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
+            fx -= ff * dx; fy -= ff * dy;
+            if (!PLANAR) fz -= ff * dz;
           }
         }
       }

@@ -399,8 +399,8 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_build(TileD td, BoxD b, i
 // ---------------------------------------------------------------------------
 template <int POT, bool F32>
 __device__ __forceinline__ void pair_term(const PairConst& pc, double dx, double dy, double& fx, double& fy) {
-  const double dz = 0.0;
-  const double r_sq = dx * dx + dy * dy + dz * dz;
+  // dz = 0: adding +0 to a sum of squares changes no bit
+  const double r_sq = dx * dx + dy * dy;
   if (POT & POT_SOFT) {
     if (F32) {
       const float r = sqrtf((float)r_sq);

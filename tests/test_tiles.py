@@ -1,9 +1,9 @@
 """Cell-tiled planar engine (tiles.hip) against the CPU oracle.
 
 Planar single-type Langevin BAOAB runs with one common mass and age go
-through the tiled engine by default: state in cell order, positions of each
-tile's 3 x (T + 2) surrounding cells staged in LDS, Verlet lists as stencil
-bitmasks.  Bars as the list-based path (tests/test_gpu_parity.py):
+through the tiled engine when the run asks for it (SAMOS_RUN_TILES): state in
+cell order, positions of each tile's 3 x (T + 2) surrounding cells staged in
+LDS, Verlet lists as stencil bitmasks.  Bars as the list-based path (tests/test_gpu_parity.py):
 
   noise-off trajectories   |dx|, |dv| <= 1e-9 after the run (FP64)
   forces left by the run   max_i |dF_i| / max_i |F_i| <= 1e-12 on identical
@@ -14,7 +14,7 @@ bitmasks.  Bars as the list-based path (tests/test_gpu_parity.py):
   FP32 pair arithmetic     forces <= 1e-6 relative to FP64
 and, specific to the engine: the halo read from L2 instead of LDS (halo
 larger than the LDS stage), runs split into ranges bit-identical to one run,
-and agreement with the list-based kernels (SAMOS_RUN_NO_TILES) to 1e-9.
+and agreement with the list-based kernels (the default path) to 1e-9.
 """
 import numpy as np
 import pytest
@@ -23,6 +23,8 @@ import oracle
 from tests import scenarios as sc
 
 pytestmark = pytest.mark.gpu
+
+TILES = 8  # SAMOS_RUN_TILES
 
 
 def setup_lj(s, gamma=1.0, T=0.0, seed=3, pad=0.3):
@@ -50,7 +52,7 @@ def test_tiled_lj_langevin_matches_oracle(gpu):
         setup_lj(s)
     print("tiled LJ Langevin, N=%d, 800 steps" % st["n"], flush=True)
     d0, o0 = d.nlist_info()["builds"], o.builds()
-    d.run(799)
+    d.run(799, TILES)
     o.run(799)
     info = d.tile_info()
     assert info["runs"] == 1 and info["rebuilds"] >= 1, info
@@ -86,7 +88,7 @@ def test_tiled_soft_dense_halo_beyond_lds(gpu):
         s.set_timestep(0.002)
         s.integrator_langevin(gamma=1.0, seed=5, temp=(0, 0.0, 1.0, 1))
     print("tiled dense soft, N=%d, 40 steps" % st["n"], flush=True)
-    d.run(39)
+    d.run(39, TILES)
     o.run(39)
     info = d.tile_info()
     assert info["runs"] == 1 and info["max_halo"] > 2048 and info["tile_cells"] == 80, info
@@ -104,7 +106,7 @@ def test_tiled_noise_on_same_stream(gpu, cells):
     for s in (d, o):
         setup_lj(s, T=1.0, seed=7)
     print("tiled noise-on, tile_cells=%d" % cells, flush=True)
-    d.run(199)
+    d.run(199, TILES)
     o.run(199)
     assert d.tile_info()["runs"] == 1
     a, b = state(d), state(o)
@@ -116,7 +118,7 @@ def test_tiled_noise_on_same_stream(gpu, cells):
 def test_tiled_equals_list_kernels(gpu):
     st = sc.lj_lattice(64, rho=0.7, seed=44)
     runs = []
-    for flags in (0, gpu.RUN_NO_TILES):
+    for flags in (TILES, 0):
         d = sc.load(gpu.System, st)
         setup_lj(d, T=1.0, seed=11)
         d.run(499, flags)
@@ -134,9 +136,9 @@ def test_tiled_ranges_bit_identical(gpu):
     for s in (a, b):
         setup_lj(s, T=1.0, seed=9)
     steps = 300
-    a.run(steps)
+    a.run(steps, TILES)
     for lo, hi in [(0, 1), (1, 77), (77, 200), (200, 301)]:
-        b.run_range(lo, hi, steps)
+        b.run_range(lo, hi, steps, TILES)
     assert b.tile_info()["runs"] == 4
     sa, sb = state(a), state(b)
     for k in sa:
@@ -148,7 +150,7 @@ def test_tiled_fp32_forces(gpu):
     d, o = sc.load(gpu.System, st), sc.load(oracle.OracleSystem, st)
     for s in (d, o):
         setup_lj(s)
-    d.run(20, gpu.RUN_FP32_FORCE)
+    d.run(20, gpu.RUN_FP32_FORCE | TILES)
     assert d.tile_info()["runs"] == 1
     o.set("x", d.get("x"))
     o.set("y", d.get("y"))
@@ -163,6 +165,11 @@ def test_tiled_not_used_where_it_does_not_apply(gpu):
     st = sc.lj_lattice(24, rho=0.7, seed=47)
     d = sc.load(gpu.System, st)
     d.set("type", (np.arange(st["n"]) % 2) + 1)
+    setup_lj(d)
+    d.run(10, TILES)
+    assert d.tile_info()["runs"] == 0
+    # not asked for: the list-based kernels run
+    d = sc.load(gpu.System, sc.lj_lattice(24, rho=0.7, seed=48))
     setup_lj(d)
     d.run(10)
     assert d.tile_info()["runs"] == 0
