@@ -130,6 +130,11 @@ int samos_integrator_langevin(samos_ctx* ctx, double gamma, int seed, int method
 /* IntegratorFIRE (integrator_fire.hpp:58-154). */
 int samos_integrator_fire(samos_ctx* ctx, double alpha, double alpha_scale, double f_tol, double e_tol,
                           double dt_inc, double dt_dec, double dt_max, int min_neg_steps);
+/* IntegratorNVE {limit; angle_limit} (integrator_nve.hpp:59-86,
+ * integrator_nve.cpp:37-141): velocity Verlet with an optional displacement
+ * limit per step and director rotation limit; a negative value means the
+ * parameter is not given. */
+int samos_integrator_nve(samos_ctx* ctx, double limit, double angle_limit);
 /* `disable <type> { group = all }` (src/samos.cpp, disable command): removes
  * the integrator registered under type. */
 int samos_integrator_remove(samos_ctx* ctx, const char* type);

@@ -80,6 +80,7 @@ def load(fast=False):
         "orc_philox": (None, [ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "orc_gauss3": (None, [u32, u32, u32, u64, dp]),
         "orc_set_rng_stream": (I, [P, I]),
+        "orc_add_nve": (I, [P, D, D]),
         "orc_gsl_stream_sample": (I, [u32, I, dp, I, dp]),
     }
     for name, (res, args) in sig.items():
@@ -226,6 +227,10 @@ class OracleSystem:
                         dt_max=1.0, min_neg_steps=5):
         self._chk(self.L.orc_add_fire(self.h, alpha, alpha_scale, f_tol, e_tol, dt_inc, dt_dec, dt_max,
                                       int(min_neg_steps)))
+
+    def integrator_nve(self, limit=None, angle_limit=None):
+        self._chk(self.L.orc_add_nve(self.h, -1.0 if limit is None else float(limit),
+                                     -1.0 if angle_limit is None else float(angle_limit)))
 
     def fire_state(self):
         a = (ctypes.c_double * 8)()

@@ -215,6 +215,8 @@ struct Integ {
   double alpha = 0.1, alpha_init = 0.1, f_alpha = 0.99, F_tol = 1e-4, E_tol = 1e-6,
          f_inc = 1.1, f_dec = 0.5, dt_max = 1.0;
   int N_min = 5, last_neg = 0; bool converged = false; double old_energy = 1e15;
+  // nve (integrator_nve.hpp:59-86)
+  bool has_limit = false, has_theta_limit = false; double limit = 0.0, theta_limit = 0.0;
   double last_P = 0, last_Fnorm = 0, last_Vnorm = 0, last_E = 0;
   double Tval(int step) const {
     if (tkind == 0) return tmin;
@@ -940,6 +942,59 @@ void fire(Sim& s, Integ& I) {
   I.old_energy = E;
 }
 
+// IntegratorNVE::integrate (src/integrators/integrator_nve.cpp:37-141): velocity
+// Verlet with optional displacement / rotation limits.  The reference clamps
+// omega with its angle limit whenever a displacement limit is set, reading an
+// unset member when no angle_limit was given; here that clamp needs both.
+inline double sign_of(double x) { return x >= 0.0 ? 1.0 : -1.0; }   // integrator_nve.hpp:38
+
+void nve(Sim& s, Integ& I) {
+  double dt_2 = 0.5 * I.dt;
+  for (auto& p : s.p) {
+    p.vx += dt_2 * p.fx; p.vy += dt_2 * p.fy; p.vz += dt_2 * p.fz;
+    enforce(s, p);
+    p.omega += dt_2 * project_torque(s, p);
+  }
+  for (auto& p : s.p) {
+    double dx = I.dt * p.vx, dy = I.dt * p.vy, dz = I.dt * p.vz;
+    if (I.has_limit) {
+      double dr = std::sqrt(dx * dx + dy * dy + dz * dz);
+      if (dr > I.limit) {
+        dx = I.limit * dx / dr;
+        dy = I.limit * dy / dr;
+        dz = I.limit * dz / dr;
+      }
+    }
+    p.x += dx; p.y += dy; p.z += dz;
+    enforce(s, p);
+  }
+  for (auto& p : s.p) {
+    double dtheta = I.dt * p.omega;
+    if (I.has_theta_limit && std::fabs(dtheta) > I.theta_limit) dtheta = sign_of(dtheta) * I.theta_limit;
+    rotate_director(s, p, dtheta);
+  }
+  reset_forces(s);
+  reset_torques(s);
+  pot_compute(s);
+  align_compute(s);
+  for (auto& p : s.p) {
+    p.vx += dt_2 * p.fx; p.vy += dt_2 * p.fy; p.vz += dt_2 * p.fz;
+    if (I.has_limit) {
+      double v = std::sqrt(p.vx * p.vx + p.vy * p.vy + p.vz * p.vz);
+      if (v * I.dt > I.limit) {
+        p.vx = p.vx / v * I.limit / I.dt;
+        p.vy = p.vy / v * I.limit / I.dt;
+        p.vz = p.vz / v * I.limit / I.dt;
+      }
+    }
+    enforce(s, p);
+    p.omega += dt_2 * project_torque(s, p);
+    if (I.has_limit && I.has_theta_limit && std::fabs(p.omega) * I.dt > I.theta_limit)
+      p.omega = sign_of(p.omega) * I.theta_limit / I.dt;
+    p.age += I.dt;
+  }
+}
+
 void integrate(Sim& s, Integ& I) {
   if (I.name == "brownian") brownian(s, I);
   else if (I.name == "brownian_pos") brownian_pos(s, I);
@@ -947,6 +1002,7 @@ void integrate(Sim& s, Integ& I) {
   else if (I.name == "langevin") {
     if (I.method == 0) langevin_baoab(s, I); else if (I.method == 1) langevin_spv(s, I); else langevin_bbk(s, I);
   } else if (I.name == "fire") fire(s, I);
+  else if (I.name == "nve") nve(s, I);
 }
 
 bool needs_nlist(const Sim& s) { return s.has_nlist && (!s.pots.empty() || !s.als.empty()); }
@@ -1262,6 +1318,17 @@ int orc_add_fire(void* h, double alpha, double alpha_scale, double f_tol, double
   I.alpha = alpha; I.alpha_init = alpha; I.f_alpha = alpha_scale; I.F_tol = f_tol; I.E_tol = e_tol;
   I.f_inc = dt_inc; I.f_dec = dt_dec; I.dt_max = dt_max; I.N_min = min_neg;
   s->integs["fire_all"] = I;
+  return 0;
+}
+
+// NVE; a negative limit or angle_limit means the parameter is not given.
+int orc_add_nve(void* h, double limit, double angle_limit) {
+  Sim* s = static_cast<Sim*>(h);
+  Integ I = make_integ(s, "nve", 0.0);
+  if (I.dt <= 0.0) { g_err = "Integrator step has not been set."; return -1; }
+  I.has_limit = limit >= 0.0; I.limit = limit;
+  I.has_theta_limit = angle_limit >= 0.0; I.theta_limit = angle_limit;
+  s->integs["nve_all"] = I;
   return 0;
 }
 

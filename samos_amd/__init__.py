@@ -74,6 +74,7 @@ def load_library(path: str = LIB_PATH):
         "samos_integrator_brownian": (I, [P, I, D, D, D, D, I, I, D, I, I, D, D, I]),
         "samos_integrator_langevin": (I, [P, D, I, I, I, D, D, I]),
         "samos_integrator_fire": (I, [P, D, D, D, D, D, D, D, I]),
+        "samos_integrator_nve": (I, [P, D, D]),
         "samos_fire_state": (I, [P, dp]),
         "samos_compute_forces": (I, [P, dp]),
         "samos_pair_compute": (I, [P, I, dp]),
@@ -276,6 +277,11 @@ class System:
                         dt_max=1.0, min_neg_steps=5):
         self._chk(self.L.samos_integrator_fire(self.h, alpha, alpha_scale, f_tol, e_tol, dt_inc, dt_dec, dt_max,
                                                int(min_neg_steps)))
+
+    def integrator_nve(self, limit=None, angle_limit=None):
+        """IntegratorNVE {limit; angle_limit} (src/integrators/integrator_nve.hpp:59-86)."""
+        self._chk(self.L.samos_integrator_nve(self.h, -1.0 if limit is None else float(limit),
+                                              -1.0 if angle_limit is None else float(angle_limit)))
 
     def fire_state(self):
         a = (ctypes.c_double * 8)()

@@ -32,6 +32,7 @@ struct Integ {
          dt_max = 1.0;
   int N_min = 5, last_neg = 0; bool converged = false; double old_energy = 1e15;
   double last[4] = {0, 0, 0, 0};
+  double limit = -1.0, theta_limit = -1.0;  // nve; negative: not given
 };
 
 struct ProfAcc {
@@ -355,6 +356,15 @@ int integrate_one(samos_ctx* ctx, Integ& I, int run_step, int j = -1) {
     ProfScope ps(ctx, "integrate");
     launch_langevin_b(ctx->st, I.method, f, n, I.dt, I.gamma, I.temp, I.seed, call, clk, I.slot, t_off, I.R,
                       ctx->box, ctx->con);
+  } else if (I.name == "nve") {
+    {
+      ProfScope ps(ctx, "integrate");
+      launch_nve_a(ctx->st, f, ctx->box, ctx->con, n, I.dt, I.limit, I.theta_limit);
+    }
+    // reset forces and torques, Potential::compute, Aligner::compute
+    loop_head_forces(ctx, nullptr);
+    ProfScope ps(ctx, "integrate");
+    launch_nve_b(ctx->st, f, ctx->box, ctx->con, n, I.dt, I.limit, I.theta_limit);
   } else if (I.name == "fire") {
     {
       ProfScope ps(ctx, "integrate");
@@ -1155,6 +1165,13 @@ int samos_integrator_fire(samos_ctx* ctx, double alpha, double alpha_scale, doub
   I.alpha = alpha; I.alpha_init = alpha; I.f_alpha = alpha_scale; I.F_tol = f_tol; I.E_tol = e_tol;
   I.f_inc = dt_inc; I.f_dec = dt_dec; I.dt_max = dt_max; I.N_min = min_neg_steps;
   return add_integ(ctx, I, "fire");
+}
+
+int samos_integrator_nve(samos_ctx* ctx, double limit, double angle_limit) {
+  Integ I;
+  I.limit = limit < 0.0 ? -1.0 : limit;
+  I.theta_limit = angle_limit < 0.0 ? -1.0 : angle_limit;
+  return add_integ(ctx, I, "nve");
 }
 
 int samos_fire_state(samos_ctx* ctx, double* out) {

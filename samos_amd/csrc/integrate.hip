@@ -228,6 +228,66 @@ __global__ void k_fire_a(Fields f, BoxD b, ConstraintD c, int n, double dt) {
   store_ps(f, s, p);
 }
 
+// IntegratorNVE::integrate (src/integrators/integrator_nve.cpp:37-141), first
+// part: the reference's first three loops touch one particle each, so they run
+// fused per particle: half kick, enforce, omega half kick; the displacement
+// (limited to `limit` when lim_on); enforce; the director rotation by dt omega
+// (limited to `theta` when theta_on).  limit < 0 / theta < 0: not given.
+__device__ __forceinline__ double sign_of(double x) { return x >= 0.0 ? 1.0 : -1.0; }
+
+__global__ void k_nve_a(Fields f, BoxD b, ConstraintD c, int n, double dt, double limit, double theta) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  PS p; load_ps(f, s, p);
+  const double dt_2 = 0.5 * dt;
+  p.vx += dt_2 * f.fx[s]; p.vy += dt_2 * f.fy[s]; p.vz += dt_2 * f.fz[s];
+  bool fz0 = enforce(b, c, p);
+  double om = f.omega[s] + dt_2 * project_torque(c, p, f.tx[s], f.ty[s], f.tz[s]);
+  double dx = dt * p.vx, dy = dt * p.vy, dz = dt * p.vz;
+  if (limit >= 0.0) {
+    const double dr = sqrt(dx * dx + dy * dy + dz * dz);
+    if (dr > limit) {
+      dx = limit * dx / dr;
+      dy = limit * dy / dr;
+      dz = limit * dz / dr;
+    }
+  }
+  p.x += dx; p.y += dy; p.z += dz;
+  fz0 |= enforce(b, c, p);
+  double dtheta = dt * om;
+  if (theta >= 0.0 && fabs(dtheta) > theta) dtheta = sign_of(dtheta) * theta;
+  rotate_director(c, p, dtheta);
+  f.omega[s] = om;
+  if (fz0) f.fz[s] = 0.0;
+  store_ps(f, s, p);
+}
+
+// IntegratorNVE::integrate, last loop (after the force and torque evaluation):
+// half kick, speed limit, enforce, omega half kick and its limit (applied when
+// both limits are given; the reference reads an unset member otherwise), age.
+__global__ void k_nve_b(Fields f, BoxD b, ConstraintD c, int n, double dt, double limit, double theta) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  PS p; load_ps(f, s, p);
+  const double dt_2 = 0.5 * dt;
+  p.vx += dt_2 * f.fx[s]; p.vy += dt_2 * f.fy[s]; p.vz += dt_2 * f.fz[s];
+  if (limit >= 0.0) {
+    const double v = sqrt(p.vx * p.vx + p.vy * p.vy + p.vz * p.vz);
+    if (v * dt > limit) {
+      p.vx = p.vx / v * limit / dt;
+      p.vy = p.vy / v * limit / dt;
+      p.vz = p.vz / v * limit / dt;
+    }
+  }
+  const bool fz0 = enforce(b, c, p);
+  double om = f.omega[s] + dt_2 * project_torque(c, p, f.tx[s], f.ty[s], f.tz[s]);
+  if (limit >= 0.0 && theta >= 0.0 && fabs(om) * dt > theta) om = sign_of(om) * theta / dt;
+  f.omega[s] = om;
+  f.age[s] += dt;
+  if (fz0) f.fz[s] = 0.0;
+  store_ps(f, s, p);
+}
+
 // FIRE second pass: half kick, per-slot P = v.F, |F|^2, |v|^2 (before the
 // enforce, as in the reference), enforce, director half kick.
 __global__ void k_fire_b(Fields f, BoxD b, ConstraintD c, int n, double dt, double* cols, long ld) {
@@ -463,6 +523,14 @@ void launch_fire_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, d
 
 void launch_fire_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double* cols, long ld) {
   hipLaunchKernelGGL(k_fire_b, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, cols, ld);
+}
+
+void launch_nve_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta) {
+  hipLaunchKernelGGL(k_nve_a, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, limit, theta);
+}
+
+void launch_nve_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta) {
+  hipLaunchKernelGGL(k_nve_b, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, limit, theta);
 }
 
 void launch_fire_mix(hipStream_t st, Fields f, int n, double f1, double f2, int zero_v) {

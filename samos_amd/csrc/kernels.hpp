@@ -63,6 +63,10 @@ void launch_bbk_init(hipStream_t st, Fields f, int n, uint32_t seed, double* R);
 void launch_fire_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt);
 void launch_fire_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double* cols, long ld);
 void launch_fire_mix(hipStream_t st, Fields f, int n, double f1, double f2, int zero_v);
+// IntegratorNVE (src/integrators/integrator_nve.cpp:37-141): the part before the
+// force evaluation and the part after it; limit / theta < 0: not given.
+void launch_nve_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta);
+void launch_nve_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta);
 void launch_clock_advance(hipStream_t st, ClockD* clock, int nsteps);
 
 // Fused Langevin BAOAB step over the neighbour list (one launch per step):

@@ -569,6 +569,25 @@ IntegratorFIRE::IntegratorFIRE(SystemPtr sys, MessengerPtr msg, ValuePtr temp, p
                                     has(p, "min_neg_steps") ? to_int(p["min_neg_steps"], "min_neg_steps") : 5));
 }
 
+// IntegratorNVE (src/integrators/integrator_nve.hpp:59-77): `limit` and
+// `angle_limit` are optional; no unknown-parameter check.
+IntegratorNVE::IntegratorNVE(SystemPtr sys, MessengerPtr msg, ValuePtr temp, pairs_type& p)
+    : Integrator(sys, msg, temp, p) {
+  double limit = -1.0, angle = -1.0;
+  if (has(p, "limit")) {
+    msg_->msg(Messenger::WARNING, "NVE integrator has limit on the maximum particle displacement. It is set to " +
+                                      p["limit"] + ".");
+    limit = to_double(p["limit"], "limit");
+  }
+  if (has(p, "angle_limit")) {
+    msg_->msg(Messenger::WARNING, "NVE integrator has limit on the maximum rotation of the director. It is set to " +
+                                      p["angle_limit"] + ".");
+    angle = to_double(p["angle_limit"], "angle_limit");
+  }
+  StepScope sc(sys_, dt_);
+  sys_->check(samos_integrator_nve(sys_->ctx(), limit, angle));
+}
+
 // ---------------------------------------------------------------------------
 // Registries (src/utils/register_pair_potentials.cpp:35,39,
 // register_pair_aligners.cpp:35,37, register_integrators.cpp:35-53,
@@ -610,7 +629,8 @@ std::map<std::string, IntegratorFactory>& integrators() {
       {"brownian_pos", [](SystemPtr s, MessengerPtr m_, ValuePtr t, pairs_type& p) -> IntegratorPtr { return std::make_shared<IntegratorBrownian>(s, m_, t, p, 1); }},
       {"brownian_align", [](SystemPtr s, MessengerPtr m_, ValuePtr t, pairs_type& p) -> IntegratorPtr { return std::make_shared<IntegratorBrownian>(s, m_, t, p, 2); }},
       {"langevin", [](SystemPtr s, MessengerPtr m_, ValuePtr t, pairs_type& p) -> IntegratorPtr { return std::make_shared<IntegratorLangevin>(s, m_, t, p); }},
-      {"fire", [](SystemPtr s, MessengerPtr m_, ValuePtr t, pairs_type& p) -> IntegratorPtr { return std::make_shared<IntegratorFIRE>(s, m_, t, p); }}};
+      {"fire", [](SystemPtr s, MessengerPtr m_, ValuePtr t, pairs_type& p) -> IntegratorPtr { return std::make_shared<IntegratorFIRE>(s, m_, t, p); }},
+      {"nve", [](SystemPtr s, MessengerPtr m_, ValuePtr t, pairs_type& p) -> IntegratorPtr { return std::make_shared<IntegratorNVE>(s, m_, t, p); }}};
   return m;
 }
 

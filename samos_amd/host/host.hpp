@@ -275,6 +275,10 @@ class IntegratorFIRE : public Integrator {
  public:
   IntegratorFIRE(SystemPtr sys, MessengerPtr msg, ValuePtr temp, pairs_type& param);
 };
+class IntegratorNVE : public Integrator {
+ public:
+  IntegratorNVE(SystemPtr sys, MessengerPtr msg, ValuePtr temp, pairs_type& param);
+};
 
 // ---------------------------------------------------------------------------
 // Registries (src/utils/factory_types.hpp:42-63, src/utils/register_*.cpp).

@@ -156,6 +156,50 @@ run 400
 
 
 @pytest.mark.gpu
+def test_conf_nve_limit_then_brownian(gpu, driver, tmp_path):
+    # the example confs' pattern: an NVE overlap relaxation with a displacement
+    # limit, `disable nve`, then the production integrator
+    st = sc.plane_disks(1500, phi=1.0, seed=34)
+    L = st["box"][0]
+    write_input(tmp_path / "p.dat", st)
+    conf = f"""box periodic {{ lx = {L!r}; ly = {L!r}; lz = 10.0 }}
+input p.dat
+nlist {{ rcut = 2.4; pad = 0.3 }}
+constraint plane {{ }}
+pair_potential soft {{ k = 1.0; a = 2.0 }}
+timestep 0.01
+integrator nve {{ limit = 0.0001 }}
+run 100
+disable nve {{ group = all }}
+integrator brownian {{ v0 = 0.1; nu = 0.0; mu = 1.0; seed = 1 }}
+dump fin {{ type = full; start = 150; freq = 150; multi; id; coordinate; keys }}
+run 200
+"""
+    r = run_conf(tmp_path, conf)
+    assert r.returncode == 0, r.stderr
+    o = sc.load(oracle.OracleSystem, st)
+    o.nlist(2.4, 0.3)
+    o.pair_soft(1.0, 2.0)
+    o.constraint("plane")
+    o.set_timestep(0.01)
+    o.integrator_nve(limit=0.0001)
+    o.run(100)
+    o2_state = {k: o.get(k) for k in ("x", "y", "vx", "vy", "nx", "ny")}
+    b = sc.load(oracle.OracleSystem, {"n": st["n"], "box": st["box"], "periodic": True,
+                                       "cols": dict(st["cols"], **o2_state)})
+    b.nlist(2.4, 0.3)
+    b.pair_soft(1.0, 2.0)
+    b.constraint("plane")
+    b.set_timestep(0.01)
+    b.integrator_brownian(v0=0.1, nu=0.0, mu=1.0, seed=1)
+    b.run(48)  # time steps 101..149; the dump at 150 is the loop-head state of step 150
+    d = read_full(tmp_path / "fin_0000000150.dat")
+    assert d.shape[0] == st["n"]
+    dx = sc.min_image(d[:, 1] - b.get("x"), L)
+    assert np.abs(dx).max() < 1e-6
+
+
+@pytest.mark.gpu
 def test_conf_unknown_parameter_rejected(gpu, driver, tmp_path):
     st = sc.plane_disks(50, seed=33)
     L = st["box"][0]

@@ -362,6 +362,43 @@ def test_brownian_pos_align_self_propulsion(gpu):
     assert np.abs(directors(d) - directors(o)).max() <= 1e-9
 
 
+@pytest.mark.parametrize("limits", [(0.01, 0.02), (None, None)])
+def test_nve_soft_polar_overlaps(gpu, limits):
+    # the overlap-relaxation pre-stage of the reference's example confs:
+    # NVE with a displacement limit on overlapping soft polar disks
+    st = sc.plane_disks(3000, phi=1.0, seed=27)
+    d, o = both(gpu, st)
+    for s in (d, o):
+        s.nlist(2.4, 0.3)
+        s.pair_soft(1.0, 2.0)
+        s.align_polar(1.0, 2.4)
+        s.constraint("plane")
+        s.set_timestep(0.005)
+        s.integrator_nve(limit=limits[0], angle_limit=limits[1])
+        s.run(199)
+    assert pos_err(positions(d), positions(o), st["box"], True) <= 1e-9
+    assert np.abs(directors(d) - directors(o)).max() <= 1e-9
+    assert np.abs(d.get("omega") - o.get("omega")).max() <= 1e-9
+    assert np.abs(d.get("vx") - o.get("vx")).max() <= 1e-9
+    assert np.array_equal(d.get("age"), o.get("age"))
+
+
+def test_nve_soft_nematic_sphere(gpu):
+    st = sc.sphere_disks(1500, phi=0.9, seed=28)
+    d, o = both(gpu, st)
+    for s in (d, o):
+        s.nlist(2.4, 0.3)
+        s.pair_soft(10.0, 2.0)
+        s.align_nematic(0.5, 2.4)
+        s.constraint("sphere", st["R"])
+        s.set_timestep(0.005)
+        s.integrator_nve(limit=0.01, angle_limit=0.05)
+        s.run(299)
+    assert pos_err(positions(d), positions(o), st["box"], False) <= 1e-9
+    assert np.abs(d.get("vy") - o.get("vy")).max() <= 1e-9
+    assert np.abs(directors(d) - directors(o)).max() <= 1e-9
+
+
 def test_fire_parity_polydisperse(gpu):
     st = sc.plane_disks(1024, phi=0.88, seed=23, polydisperse=(0.8, 1.2))
     d, o = both(gpu, st)

@@ -353,6 +353,77 @@ def test_brownian_rotational_variance():
     assert abs(var - expect) < 5 * expect * math.sqrt(2.0 / n)
 
 
+def test_nve_free_flight():
+    # no forces: x(t) = x0 + t v exactly step by step (integrator_nve.cpp:47-83)
+    n = 5
+    s = oracle.OracleSystem(n, (1000.0, 1000.0, 10.0), True)
+    v = np.linspace(-1.0, 1.0, n)
+    s.set("vx", v)
+    s.set("vy", 0.5 * v)
+    s.constraint("plane")
+    s.set_timestep(0.01)
+    s.integrator_nve()
+    x = np.zeros(n)
+    for _ in range(20):
+        x = x + 0.01 * v
+    s.run(19)
+    assert np.array_equal(s.get("x"), x)
+    assert np.array_equal(s.get("vx"), v)
+
+
+def test_nve_limit_caps_displacement_and_speed():
+    # |dt v| > limit: each step moves exactly `limit` along v, and the speed is
+    # then cut to limit / dt (integrator_nve.cpp:66-76, 115-124)
+    s = oracle.OracleSystem(1, (1000.0, 1000.0, 10.0), True)
+    s.set("vx", [3.0])
+    s.set("vy", [4.0])
+    s.constraint("plane")
+    s.set_timestep(0.01)
+    s.integrator_nve(limit=0.002)
+    s.integrate()
+    assert s.get("x")[0] == pytest.approx(0.002 * 0.6, rel=1e-14)
+    assert s.get("y")[0] == pytest.approx(0.002 * 0.8, rel=1e-14)
+    assert math.hypot(s.get("vx")[0], s.get("vy")[0]) == pytest.approx(0.2, rel=1e-14)
+    s.integrate()
+    assert s.get("x")[0] == pytest.approx(2 * 0.002 * 0.6, rel=1e-13)
+
+
+def test_nve_lj_pair_conserves_energy():
+    # velocity Verlet: total energy of an LJ pair oscillating about r = 2^(1/6)
+    # drifts only by O(dt^2); a dropped or doubled half kick breaks this at O(dt)
+    s = two(1.2)
+    s.nlist(2.5, 0.3)
+    s.pair_lj(1.0, 1.0, 2.5, shifted=False)
+    s.constraint("plane")
+    s.set_timestep(0.001)
+    s.integrator_nve()
+    e0 = s.compute_forces()["pair"]
+    s.run(1999)
+    e = s.compute_forces()["pair"]
+    ke = 0.5 * float(np.sum(s.get("vx") ** 2 + s.get("vy") ** 2))
+    assert abs(e + ke - e0) < 1e-5 * abs(e0)
+    assert abs(s.get("x")[1] - s.get("x")[0] - 1.2) > 1e-3  # it moved
+
+
+def test_nve_angle_limit_caps_rotation():
+    # two aligned-against-each-other polar disks: J n_i x n_j torques; with
+    # angle_limit each step's director rotation is at most the limit
+    s = two(1.5)
+    s.set("nx", [1.0, 0.0])
+    s.set("ny", [0.0, 1.0])
+    s.nlist(2.4, 0.3)
+    s.align_polar(5.0, 2.4)
+    s.constraint("plane")
+    s.set_timestep(0.1)
+    s.integrator_nve(angle_limit=0.01)
+    th0 = np.arctan2(s.get("ny"), s.get("nx"))
+    s.compute_forces()
+    s.integrate()
+    th1 = np.arctan2(s.get("ny"), s.get("nx"))
+    assert np.all(np.abs(th1 - th0) <= 0.01 + 1e-15)
+    assert np.all(np.abs(th1 - th0) > 0.0)
+
+
 # ---------------------------------------------------------- golden pins ---
 @pytest.mark.parametrize("name", ["plane", "sphere", "lj", "relax"])
 def test_golden_example_inputs(name):
