@@ -39,7 +39,10 @@ enum samos_field {
 };
 
 enum samos_constraint_kind { SAMOS_CONSTRAINT_NONE = 0, SAMOS_CONSTRAINT_PLANE = 1, SAMOS_CONSTRAINT_SPHERE = 2 };
-enum samos_pair_kind { SAMOS_PAIR_SOFT = 0, SAMOS_PAIR_LJ = 1 };
+enum samos_pair_kind {
+  SAMOS_PAIR_SOFT = 0, SAMOS_PAIR_LJ = 1,
+  SAMOS_PAIR_MORSE = 2, SAMOS_PAIR_GAUSSIAN = 3, SAMOS_PAIR_YUKAWA = 4, SAMOS_PAIR_SOFT_ATTRACTIVE = 5
+};
 enum samos_align_kind { SAMOS_ALIGN_POLAR = 0, SAMOS_ALIGN_NEMATIC = 1 };
 enum samos_brownian_kind { SAMOS_BROWNIAN = 0, SAMOS_BROWNIAN_POS = 1, SAMOS_BROWNIAN_ALIGN = 2 };
 enum samos_langevin_method { SAMOS_LANGEVIN_BAOAB = 0, SAMOS_LANGEVIN_SPV = 1, SAMOS_LANGEVIN_BBK = 2 };
@@ -107,6 +110,18 @@ int samos_align_nematic(samos_ctx* ctx, double J, double rcut);
 /* set_pair_parameters (pair_polar_aligner.hpp:120-163, pair_nematic_aligner.hpp:110-155). */
 int samos_align_type_param(samos_ctx* ctx, int kind, int type1, int type2, double J, double rcut);
 
+/* Further pair potentials with global parameters (phase_in not supported),
+ * params in this order:
+ *   MORSE            {D, a, re, rcut}                 pair_morse_potential.hpp:62-160, .cpp:32-107
+ *   GAUSSIAN         {A, B, alpha, beta, rA, rB, rcut} pair_gaussian_potential.hpp:60-200, .cpp:32-111
+ *   YUKAWA           {g, kappa, rcut}                 pair_yukawa_potential.hpp:60-120, .cpp:32-80
+ *   SOFT_ATTRACTIVE  {k, a, re_fact}                  pair_soft_attractive_potential.hpp:62-140, .cpp:32-118
+ * use_radii: use_particle_radii (morse: re, gaussian: rB, soft_attractive: a
+ * become a_i + a_j); shifted: subtract the value at rcut (morse, gaussian).
+ * soft_attractive recounts each particle's coordination on every evaluation. */
+int samos_pair_extra(samos_ctx* ctx, int kind, const double* params, int nparams, int use_radii, int shifted);
+/* Sum of the further pair potentials' energies (PairPotential::get_potential_energy). */
+int samos_pair_extra_energy(samos_ctx* ctx, double* energy);
 /* ExternalSelfPropulsion {alpha} (src/potentials/external/external_self_propulsion.cpp:33-50). */
 int samos_external_self_propulsion(samos_ctx* ctx, double alpha);
 

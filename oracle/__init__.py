@@ -26,6 +26,7 @@ FIELD_CODE.update({name: 32 + i for i, name in enumerate(INT_FIELDS)})
 CONSTRAINT = {"none": 0, "plane": 1, "sphere": 2}
 LANGEVIN_METHOD = {"baoab": 0, "spv": 1, "bbk": 2}
 RNG_STREAM = {"philox": 0, "mt19937": 1}
+EXTRA_KIND = {"morse": 2, "gaussian": 3, "yukawa": 4, "soft_attractive": 5}
 
 _libs = {}
 
@@ -81,6 +82,8 @@ def load(fast=False):
         "orc_gauss3": (None, [u32, u32, u32, u64, dp]),
         "orc_set_rng_stream": (I, [P, I]),
         "orc_add_nve": (I, [P, D, D]),
+        "orc_add_pair_extra": (I, [P, I, dp, I, I]),
+        "orc_extra_energy": (D, [P]),
         "orc_gsl_stream_sample": (I, [u32, I, dp, I, dp]),
     }
     for name, (res, args) in sig.items():
@@ -174,6 +177,18 @@ class OracleSystem:
     def pair_lj(self, eps=1.0, sigma=1.0, rcut=3.0, shifted=False, use_radii=False):
         self._chk(self.L.orc_add_pair_lj(self.h, float(eps), float(sigma), float(rcut), int(bool(shifted)),
                                          int(bool(use_radii))))
+
+    def pair_extra(self, kind, params, use_radii=False, shifted=False):
+        """morse {D, a, re, rcut}, gaussian {A, B, alpha, beta, rA, rB, rcut},
+        yukawa {g, kappa, rcut} or soft_attractive {k, a, re_fact}, as a list in
+        that order."""
+        q = np.zeros(8)
+        q[:len(params)] = params
+        self._chk(self.L.orc_add_pair_extra(self.h, EXTRA_KIND[kind], q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                            int(bool(use_radii)), int(bool(shifted))))
+
+    def extra_energy(self):
+        return float(self.L.orc_extra_energy(self.h))
 
     def pair_type_param(self, kind, t1, t2, p0, p1, p2=0.0, shifted=False):
         self._chk(self.L.orc_pair_type_param(self.h, kind.encode(), t1, t2, float(p0), float(p1), float(p2), 0))

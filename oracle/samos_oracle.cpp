@@ -145,8 +145,9 @@ struct TypeTable {
   }
 };
 
-struct PairPot {        // soft (kind 0) or lj (kind 1)
+struct PairPot {        // soft (kind 0), lj (1), morse (2), gaussian (3), yukawa (4), soft_attractive (5)
   int kind;
+  double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // kinds 2-5: the global parameters (see extra_compute)
   double k, a;                         // soft (pair_soft_potential.hpp:63-180)
   double eps, sigma, rcut;             // lj (pair_lj_potential.hpp:61-248)
   bool shifted = false, use_radii = false, has_pair_params = false;
@@ -565,9 +566,108 @@ void self_prop_compute(Sim& s) {
 }
 
 // Potential::compute (src/potentials/potential.cpp:36-52): pair (map order) then external.
+// Global-parameter forms of four further pair potentials; phase_in is not
+// supported (alpha = 1) and the per-type tables hold the globals.
+//   morse            q = {D, a, re, rcut}        pair_morse_potential.cpp:32-107
+//   gaussian         q = {A, B, alpha, beta, rA, rB, rcut}  pair_gaussian_potential.cpp:32-111
+//   yukawa           q = {g, kappa, rcut}        pair_yukawa_potential.cpp:32-80
+//   soft_attractive  q = {k, a, re_fact}         pair_soft_attractive_potential.cpp:32-118
+// (morse and gaussian subtract the value at rcut when shifted; morse and
+// gaussian with use_particle_radii take re / rB = a_i + a_j, soft_attractive
+// takes a = a_i + a_j; soft_attractive also recounts the coordination.)
+void extra_compute(Sim& s, PairPot& P) {
+  int N = (int)s.p.size();
+  const double* q = P.q;
+  P.energy = 0.0;
+  if (P.kind == 5)
+    for (auto& p : s.p) p.coordination = 0;
+  for (int i = 0; i < N; i++) {
+    Particle& pi = s.p[i];
+    double ai = pi.radius;
+    for (int j : s.list[i]) {
+      Particle& pj = s.p[j];
+      if (P.kind == 5) {
+        // PairSoftAttractivePotential: d = x_j - x_i, forces with the minus sign on i
+        double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        apply_periodic(s, dx, dy, dz);
+        double r_sq = dx * dx + dy * dy + dz * dz;
+        double r = std::sqrt(r_sq);
+        double k = q[0], alpha = 1.0;
+        double ai_p_aj = P.use_radii ? ai + pj.radius : q[1];
+        double r_turn = q[2] * ai_p_aj;
+        double r_f = 2.0 * r_turn - ai_p_aj;
+        double ff = 0.0, e = 0.0;
+        if (r < r_f) {
+          pi.coordination += 1;
+          pj.coordination += 1;
+          if (r <= r_turn) {
+            double diff = ai_p_aj - r;
+            e = 0.5 * k * alpha * diff * diff;
+            if (r > 0.0) ff = k * alpha * diff / r; else ff = k * diff;
+          } else if (r > r_turn && r < r_f) {
+            double diff = r_f - r;
+            e = -0.5 * k * alpha * diff * diff;
+            ff = -k * alpha * diff / r;
+          }
+          P.energy += e;
+          pi.fx -= ff * dx; pi.fy -= ff * dy; pi.fz -= ff * dz;
+          pj.fx += ff * dx; pj.fy += ff * dy; pj.fz += ff * dz;
+        }
+        continue;
+      }
+      double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      apply_periodic(s, dx, dy, dz);
+      double r_sq = dx * dx + dy * dy + dz * dz;
+      double ff = 0.0, e = 0.0;
+      if (P.kind == 2) {                       // morse
+        double rcut = q[3];
+        if (!(r_sq <= rcut * rcut)) continue;
+        double D = q[0], a = q[1], re = q[2], alpha = 1.0;
+        if (P.use_radii) re = ai + pj.radius;
+        double r = std::sqrt(r_sq);
+        double exp_fact = std::exp(-a * (r - re));
+        double pot_fact = exp_fact - 1.0;
+        e = D * alpha * (pot_fact * pot_fact - 1.0);
+        if (P.shifted) {
+          double shift_fact = std::exp(-a * (rcut - re)) - 1.0;
+          e -= D * alpha * (shift_fact * shift_fact - 1.0);
+        }
+        ff = 2.0 * D * a * alpha * exp_fact * pot_fact / r;
+      } else if (P.kind == 3) {                // gaussian
+        double rcut = q[6];
+        if (!(r_sq <= rcut * rcut)) continue;
+        double A = q[0], B = q[1], al = q[2], be = q[3], rA = q[4], rB = q[5], pf = 1.0;
+        if (P.use_radii) rB = ai + pj.radius;
+        double r = std::sqrt(r_sq);
+        double r_m_rA = r - rA, r_m_rB = r - rB;
+        e = A * pf * std::exp(-al * r_m_rA * r_m_rA) + B * std::exp(-be * r_m_rB * r_m_rB);
+        if (P.shifted) {
+          double rcut_m_rA = rcut - rA, rcut_m_rB = rcut - rB;
+          e -= A * pf * std::exp(-al * rcut_m_rA * rcut_m_rA) + B * std::exp(-be * rcut_m_rB * rcut_m_rB);
+        }
+        ff = (2.0 / r) * pf * (A * al * r_m_rA * std::exp(-al * r_m_rA * r_m_rA) +
+                               B * be * r_m_rB * std::exp(-be * r_m_rB * r_m_rB));
+      } else if (P.kind == 4) {                // yukawa
+        double rcut = q[2];
+        if (!(r_sq <= rcut * rcut)) continue;
+        double g = q[0], kappa = q[1];
+        double r = std::sqrt(r_sq);
+        double exp_fact = g * std::exp(-kappa * r);
+        e = exp_fact / r;
+        ff = exp_fact * (1 + kappa * r) / (r_sq * r);
+      }
+      P.energy += e;
+      pi.fx += ff * dx; pi.fy += ff * dy; pi.fz += ff * dz;
+      pj.fx -= ff * dx; pj.fy -= ff * dy; pj.fz -= ff * dz;
+    }
+  }
+}
+
 void pot_compute(Sim& s) {
   for (auto& kv : s.pots) {
-    if (kv.second.kind == 0) soft_compute(s, kv.second); else lj_compute(s, kv.second);
+    if (kv.second.kind == 0) soft_compute(s, kv.second);
+    else if (kv.second.kind == 1) lj_compute(s, kv.second);
+    else extra_compute(s, kv.second);
   }
   if (s.has_self_prop) self_prop_compute(s);
 }
@@ -1178,6 +1278,18 @@ int orc_add_pair_lj(void* h, double eps, double sigma, double rcut, int shifted,
   return 0;
 }
 
+// kind: 2 morse, 3 gaussian, 4 yukawa, 5 soft_attractive; q as in extra_compute.
+int orc_add_pair_extra(void* h, int kind, const double* q, int use_radii, int shifted) {
+  static const char* names[] = {"", "", "morse", "gaussian", "yukawa", "soft_attractive"};
+  if (kind < 2 || kind > 5) { g_err = "unknown pair potential kind"; return -1; }
+  Sim* s = static_cast<Sim*>(h);
+  PairPot P; P.kind = kind;
+  for (int i = 0; i < 8; i++) P.q[i] = q[i];
+  P.use_radii = use_radii != 0; P.shifted = shifted != 0;
+  s->pots[names[kind]] = P;
+  return 0;
+}
+
 // Per-type pair parameters (set_pair_parameters, symmetric fill).
 // soft: (k, a); lj: (eps, sigma, rcut, wca flag -> rcut = 2^(1/6) sigma, shifted).
 int orc_pair_type_param(void* h, const char* name, int t1, int t2, double p0, double p1, double p2, int wca) {
@@ -1341,6 +1453,14 @@ int orc_fire_state(void* h, double* out) {
   out[0] = I.dt; out[1] = I.alpha; out[2] = I.last_P; out[3] = I.last_Fnorm; out[4] = I.last_Vnorm;
   out[5] = I.last_E; out[6] = I.converged ? 1.0 : 0.0; out[7] = I.last_neg;
   return 0;
+}
+
+// Energy of the further pair potentials (kinds 2-5) from the last evaluation.
+double orc_extra_energy(void* h) {
+  Sim* s = static_cast<Sim*>(h);
+  double e = 0.0;
+  for (auto& kv : s->pots) if (kv.second.kind >= 2) e += kv.second.energy;
+  return e;
 }
 
 // Loop-head force/torque evaluation: reset, Potential::compute, Aligner::compute.

@@ -27,6 +27,7 @@ FIELD_CODE.update({name: 32 + i for i, name in enumerate(INT_FIELDS)})
 CONSTRAINT = {"none": 0, "plane": 1, "sphere": 2}
 LANGEVIN_METHOD = {"baoab": 0, "spv": 1, "bbk": 2}
 RUN_FAITHFUL, RUN_NO_GRAPH, RUN_FP32_FORCE, RUN_TILES = 1, 2, 4, 8
+EXTRA_KIND = {"morse": 2, "gaussian": 3, "yukawa": 4, "soft_attractive": 5}
 OBSERVABLE = {"max_force": 0, "force_sq": 1, "kinetic_energy": 2, "vec_velocity": 3,
               "velocity": 4, "polar_order": 5}
 
@@ -75,6 +76,8 @@ def load_library(path: str = LIB_PATH):
         "samos_integrator_langevin": (I, [P, D, I, I, I, D, D, I]),
         "samos_integrator_fire": (I, [P, D, D, D, D, D, D, D, I]),
         "samos_integrator_nve": (I, [P, D, D]),
+        "samos_pair_extra": (I, [P, I, dp, I, I, I]),
+        "samos_pair_extra_energy": (I, [P, dp]),
         "samos_fire_state": (I, [P, dp]),
         "samos_compute_forces": (I, [P, dp]),
         "samos_pair_compute": (I, [P, I, dp]),
@@ -277,6 +280,19 @@ class System:
                         dt_max=1.0, min_neg_steps=5):
         self._chk(self.L.samos_integrator_fire(self.h, alpha, alpha_scale, f_tol, e_tol, dt_inc, dt_dec, dt_max,
                                                int(min_neg_steps)))
+
+    def pair_extra(self, kind, params, use_radii=False, shifted=False):
+        """Further pair potentials: morse {D, a, re, rcut}, gaussian {A, B, alpha,
+        beta, rA, rB, rcut}, yukawa {g, kappa, rcut}, soft_attractive {k, a,
+        re_fact}; params as a list in that order."""
+        q = np.ascontiguousarray(params, dtype=np.float64)
+        self._chk(self.L.samos_pair_extra(self.h, EXTRA_KIND[kind], q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                          len(q), int(bool(use_radii)), int(bool(shifted))))
+
+    def extra_energy(self):
+        e = ctypes.c_double()
+        self._chk(self.L.samos_pair_extra_energy(self.h, ctypes.byref(e)))
+        return e.value
 
     def integrator_nve(self, limit=None, angle_limit=None):
         """IntegratorNVE {limit; angle_limit} (src/integrators/integrator_nve.hpp:59-86)."""

@@ -77,7 +77,7 @@ struct samos_ctx {
   std::map<std::string, Integ> integs;
   ClockD* d_clock = nullptr;
   int time_step = 0;
-  double* d_en = nullptr;       // 4 * np per-slot energies
+  double* d_en = nullptr;       // 5 * np per-slot energies: soft, lj, polar, nematic, further pair potentials
   double* d_cols = nullptr;     // 3 * np FIRE columns
   double* d_partial = nullptr;
   double* d_out = nullptr;      // 8 reduced values
@@ -103,6 +103,10 @@ struct samos_ctx {
   int tile_key[4] = {-1, -1, -1, -1};  // n, ncx, ncy, T of the allocation
   long long tile_stats[4] = {0, 0, 0, 0};  // tiled runs, rebuilds, largest candidate count, largest halo
   int tile_cells = 0;                      // cells per tile along x; 0: from the density
+  // further pair potentials (morse, gaussian, yukawa, soft_attractive), by name
+  std::map<std::string, std::pair<int, std::vector<double>>> extras;
+  std::map<std::string, std::pair<int, int>> extra_flags;  // use_particle_radii, shifted
+  ExtraD extra{};
 };
 
 namespace {
@@ -293,15 +297,25 @@ int energies_to_host(samos_ctx* ctx, double* e4) {
   return 0;
 }
 
+// Soft / LJ / aligners in one pass, then the further pair potentials, which
+// add to the forces (mode 0 overwrites first, mode 2 leaves forces alone).
 void interact(samos_ctx* ctx, int pm, int am, int mode, double* e_out, bool f32 = false) {
+  const bool ext = ctx->has_nlist && ctx->extra.count > 0;
   if (!ctx->has_nlist) { pm = 0; am = 0; }
   if (pm == 0 && am == 0 && !ctx->has_sp) {
-    if (e_out) hipMemsetAsync(e_out, 0, sizeof(double) * 4 * ctx->np, ctx->st);
-    return;
+    if (e_out) fill_d(ctx->st, e_out, 0.0, 4 * ctx->np);
+    if (ext && mode == 0)
+      for (int k = 9; k <= 11; ++k) fill_d(ctx->st, ctx->dcol[k], 0.0, ctx->n);
+  } else {
+    ProfScope ps(ctx, "interact");
+    if (f32) launch_interact_f32(ctx->st, pm, am, mode, fields(ctx), ctx->nl, ctx->box, interact_params(ctx), ctx->n, e_out);
+    else launch_interact(ctx->st, pm, am, mode, fields(ctx), ctx->nl, ctx->box, interact_params(ctx), ctx->n, e_out);
   }
-  ProfScope ps(ctx, "interact");
-  if (f32) launch_interact_f32(ctx->st, pm, am, mode, fields(ctx), ctx->nl, ctx->box, interact_params(ctx), ctx->n, e_out);
-  else launch_interact(ctx->st, pm, am, mode, fields(ctx), ctx->nl, ctx->box, interact_params(ctx), ctx->n, e_out);
+  if (ext) {
+    ProfScope ps(ctx, "interact");
+    launch_interact_extra(ctx->st, fields(ctx), ctx->nl, ctx->box, ctx->extra, ctx->n,
+                          e_out ? e_out + 4L * ctx->np : nullptr, mode != 2 ? 1 : 0);
+  }
 }
 
 // Loop head: reset f and tau (System::reset_forces / reset_torques), then the
@@ -318,7 +332,7 @@ void loop_head_forces(samos_ctx* ctx, double* e_out, bool f32 = false) {
 }
 
 bool needs_nlist(const samos_ctx* c) {
-  return c->has_nlist && (c->has_soft || c->has_lj || c->has_pol || c->has_nem);
+  return c->has_nlist && (c->has_soft || c->has_lj || c->has_pol || c->has_nem || c->extra.count > 0);
 }
 
 // Device-side conditional rebuild after a step: flags[0] = any(need_update).
@@ -378,13 +392,16 @@ int integrate_one(samos_ctx* ctx, Integ& I, int run_step, int j = -1) {
     }
     launch_reduce_sum(ctx->st, ctx->d_cols, 3, ctx->np, n, ctx->d_partial, ctx->d_out);
     launch_reduce_sum(ctx->st, ctx->d_en, 2, ctx->np, n, ctx->d_partial, ctx->d_out + 3);
-    HIPCHK(hipMemcpyAsync(ctx->h_out, ctx->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    if (ctx->extra.count > 0)
+      launch_reduce_sum(ctx->st, ctx->d_en + 4L * ctx->np, 1, ctx->np, n, ctx->d_partial, ctx->d_out + 5);
+    HIPCHK(hipMemcpyAsync(ctx->h_out, ctx->d_out, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     // IntegratorFIRE::integrate, scalar part (src/integrators/integrator_fire.cpp:97-147)
     const double P = ctx->h_out[0];
     const double Fn = std::sqrt(ctx->h_out[1]);
     const double Vn = std::sqrt(ctx->h_out[2]);
-    const double E = 0.5 * ctx->h_out[3] + 0.5 * ctx->h_out[4];
+    double E = 0.5 * ctx->h_out[3] + 0.5 * ctx->h_out[4];
+    if (ctx->extra.count > 0) E += 0.5 * ctx->h_out[5];
     const double sqrt_ndof = std::sqrt((double)(3 * n));
     I.last[0] = P; I.last[1] = Fn; I.last[2] = Vn; I.last[3] = E;
     if (Fn / sqrt_ndof < I.F_tol && std::fabs(E - I.old_energy) < I.E_tol) {
@@ -793,7 +810,7 @@ int samos_create(samos_ctx** out, int device, int n, double lx, double ly, doubl
   ctx->nl.flags = ctx->d_flags;
   if (hipMalloc(&ctx->d_tab, 4 * kTab * sizeof(double)) != hipSuccess) return bad("tables");
   if (hipMalloc(&ctx->d_clock, sizeof(ClockD)) != hipSuccess) return bad("clock");
-  if (hipMalloc(&ctx->d_en, 4 * np * sizeof(double)) != hipSuccess) return bad("energies");
+  if (hipMalloc(&ctx->d_en, 5 * np * sizeof(double)) != hipSuccess) return bad("energies");
   if (hipMalloc(&ctx->d_cols, 3 * np * sizeof(double)) != hipSuccess) return bad("fire columns");
   if (hipMalloc(&ctx->d_partial, 8 * (size_t)reduce_blocks(ctx->np) * sizeof(double)) != hipSuccess) return bad("partials");
   if (hipMalloc(&ctx->d_out, 16 * sizeof(double)) != hipSuccess) return bad("out");
@@ -1174,6 +1191,49 @@ int samos_integrator_nve(samos_ctx* ctx, double limit, double angle_limit) {
   return add_integ(ctx, I, "nve");
 }
 
+// Rebuilds the device table of further pair potentials from the by-name map
+// (std::map order = the reference's registry order).
+void upload_extras(samos_ctx* ctx) {
+  ExtraD& ex = ctx->extra;
+  ex = ExtraD{};
+  for (auto& kv : ctx->extras) {
+    const int e = ex.count++;
+    ex.kind[e] = kv.second.first;
+    for (int i = 0; i < 8; ++i) ex.q[e][i] = kv.second.second[i];
+    ex.radii[e] = ctx->extra_flags[kv.first].first;
+    ex.shifted[e] = ctx->extra_flags[kv.first].second;
+  }
+}
+
+int samos_pair_extra(samos_ctx* ctx, int kind, const double* params, int nparams, int use_radii, int shifted) {
+  static const char* names[] = {"", "", "morse", "gaussian", "yukawa", "soft_attractive"};
+  static const int need[] = {0, 0, 4, 7, 3, 3};
+  if (kind < SAMOS_PAIR_MORSE || kind > SAMOS_PAIR_SOFT_ATTRACTIVE) return fail(ctx, "unknown pair potential kind");
+  if (nparams != need[kind]) return fail(ctx, std::string("wrong parameter count for ") + names[kind]);
+  std::vector<double> q(8, 0.0);
+  for (int i = 0; i < nparams; ++i) q[i] = params[i];
+  if (!ctx->extras.count(names[kind]) && ctx->extras.size() >= (size_t)kMaxExtra)
+    return fail(ctx, "too many further pair potentials");
+  ctx->extras[names[kind]] = {kind, q};
+  ctx->extra_flags[names[kind]] = {use_radii ? 1 : 0, shifted ? 1 : 0};
+  upload_extras(ctx);
+  return 0;
+}
+
+int samos_pair_extra_energy(samos_ctx* ctx, double* energy) {
+  *energy = 0.0;
+  if (ctx->extra.count == 0 || !ctx->has_nlist) return 0;
+  {
+    ProfScope ps(ctx, "interact");
+    launch_interact_extra(ctx->st, fields(ctx), ctx->nl, ctx->box, ctx->extra, ctx->n, ctx->d_en + 4L * ctx->np, 0);
+  }
+  launch_reduce_sum(ctx->st, ctx->d_en + 4L * ctx->np, 1, ctx->np, ctx->n, ctx->d_partial, ctx->d_out);
+  HIPCHK(hipMemcpyAsync(ctx->h_out, ctx->d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  *energy = 0.5 * ctx->h_out[0];
+  return 0;
+}
+
 int samos_fire_state(samos_ctx* ctx, double* out) {
   auto it = ctx->integs.find("fire_all");
   if (it == ctx->integs.end()) return fail(ctx, "no FIRE integrator");
@@ -1271,7 +1331,8 @@ int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags
   // Fused schedule: one Langevin BAOAB integrator, pair potentials only.  The
   // reference's loop-head force evaluation repeats the evaluation that ended
   // the previous step on the same positions, so it is done once.
-  const bool fused = !(flags & SAMOS_RUN_FAITHFUL) && ctx->integs.size() == 1 && ctx->integs.count("langevin_all") &&
+  const bool fused = !(flags & SAMOS_RUN_FAITHFUL) && ctx->extra.count == 0 && ctx->integs.size() == 1 &&
+                     ctx->integs.count("langevin_all") &&
                      ctx->integs["langevin_all"].method == 0 && al_mask(ctx) == 0 && pot_mask(ctx) != 0 &&
                      needs_nlist(ctx);
   if (fused) {

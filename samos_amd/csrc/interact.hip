@@ -147,6 +147,103 @@ __global__ void __launch_bounds__(kBlock) k_interact(int mode, Fields f, NListD 
   }
 }
 
+// Further pair potentials over the full list, added to the forces.  Per-pair
+// terms restate
+//   morse            src/potentials/pair/pair_morse_potential.cpp:71-97
+//   gaussian         src/potentials/pair/pair_gaussian_potential.cpp:71-101
+//   yukawa           src/potentials/pair/pair_yukawa_potential.cpp:56-72
+//   soft_attractive  src/potentials/pair/pair_soft_attractive_potential.cpp:72-110
+// (phase_in not supported).  morse, gaussian and yukawa use d = x_i - x_j, the
+// single-wrap image of which is exactly minus that of x_j - x_i.
+// soft_attractive recounts the coordination (neighbours closer than r_f).
+__global__ void __launch_bounds__(kBlock) k_interact_extra(Fields f, NListD nl, BoxD b, ExtraD ex, int n,
+                                                           double* e_col, int recount, int add_forces) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const double xi = f.x[s], yi = f.y[s], zi = f.z[s], ai = f.radius[s];
+  double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
+  int coord = 0;
+  const int kn = nl.nnb[s];
+  const int* nb = nl.nbr + s;
+  for (int k = 0; k < kn; ++k) {
+    const int t = nb[(long)k * nl.np];
+    double dx = f.x[t] - xi, dy = f.y[t] - yi, dz = f.z[t] - zi;   // x_j - x_i
+    min_image(b, dx, dy, dz);
+    const double r_sq = dx * dx + dy * dy + dz * dz;
+    const double aj = f.radius[t];
+    for (int e = 0; e < ex.count; ++e) {
+      const double* q = ex.q[e];
+      const int kind = ex.kind[e];
+      if (kind == EXTRA_SOFT_ATTRACTIVE) {
+        const double r = sqrt(r_sq);
+        const double kk = q[0], alpha = 1.0;
+        const double ai_p_aj = ex.radii[e] ? ai + aj : q[1];
+        const double r_turn = q[2] * ai_p_aj;
+        const double r_f = 2.0 * r_turn - ai_p_aj;
+        if (r < r_f) {
+          ++coord;
+          double ff = 0.0, pe = 0.0;
+          if (r <= r_turn) {
+            const double diff = ai_p_aj - r;
+            pe = 0.5 * kk * alpha * diff * diff;
+            ff = (r > 0.0) ? kk * alpha * diff / r : kk * diff;
+          } else if (r > r_turn && r < r_f) {
+            const double diff = r_f - r;
+            pe = -0.5 * kk * alpha * diff * diff;
+            ff = -kk * alpha * diff / r;
+          }
+          en += pe;
+          fx -= ff * dx; fy -= ff * dy; fz -= ff * dz;
+        }
+        continue;
+      }
+      double ff = 0.0, pe = 0.0;
+      if (kind == EXTRA_MORSE) {
+        const double rcut = q[3];
+        if (!(r_sq <= rcut * rcut)) continue;
+        const double D = q[0], a = q[1], alpha = 1.0;
+        const double re = ex.radii[e] ? ai + aj : q[2];
+        const double r = sqrt(r_sq);
+        const double exp_fact = exp(-a * (r - re));
+        const double pot_fact = exp_fact - 1.0;
+        pe = D * alpha * (pot_fact * pot_fact - 1.0);
+        if (ex.shifted[e]) {
+          const double shift_fact = exp(-a * (rcut - re)) - 1.0;
+          pe -= D * alpha * (shift_fact * shift_fact - 1.0);
+        }
+        ff = 2.0 * D * a * alpha * exp_fact * pot_fact / r;
+      } else if (kind == EXTRA_GAUSSIAN) {
+        const double rcut = q[6];
+        if (!(r_sq <= rcut * rcut)) continue;
+        const double A = q[0], B = q[1], al = q[2], be = q[3], rA = q[4], pf = 1.0;
+        const double rB = ex.radii[e] ? ai + aj : q[5];
+        const double r = sqrt(r_sq);
+        const double r_m_rA = r - rA, r_m_rB = r - rB;
+        pe = A * pf * exp(-al * r_m_rA * r_m_rA) + B * exp(-be * r_m_rB * r_m_rB);
+        if (ex.shifted[e]) {
+          const double rcut_m_rA = rcut - rA, rcut_m_rB = rcut - rB;
+          pe -= A * pf * exp(-al * rcut_m_rA * rcut_m_rA) + B * exp(-be * rcut_m_rB * rcut_m_rB);
+        }
+        ff = (2.0 / r) * pf * (A * al * r_m_rA * exp(-al * r_m_rA * r_m_rA) + B * be * r_m_rB * exp(-be * r_m_rB * r_m_rB));
+      } else {  // yukawa
+        const double rcut = q[2];
+        if (!(r_sq <= rcut * rcut)) continue;
+        const double g = q[0], kappa = q[1];
+        const double r = sqrt(r_sq);
+        const double exp_fact = g * exp(-kappa * r);
+        pe = exp_fact / r;
+        ff = exp_fact * (1 + kappa * r) / (r_sq * r);
+      }
+      en += pe;
+      // force on i along x_i - x_j = -(x_j - x_i)
+      fx += ff * -dx; fy += ff * -dy; fz += ff * -dz;
+    }
+  }
+  if (add_forces) { f.fx[s] += fx; f.fy[s] += fy; f.fz[s] += fz; }
+  if (recount) f.coord[s] = coord;
+  if (e_col) e_col[s] = en;
+}
+
 template <int POT, int AL>
 void dispatch_en(hipStream_t st, int mode, Fields f, NListD nl, BoxD b, InteractD ip, int n, double* e_out, bool f32) {
   dim3 g(grid1(n)), blk(kBlock);
@@ -229,6 +326,15 @@ void launch_interact_f32(hipStream_t st, int pot_mask, int al_mask, int mode, Fi
     case 2: dispatch_al<2>(st, al_mask, mode, f, nl, box, ip, n, e_out, true); break;
     default: dispatch_al<3>(st, al_mask, mode, f, nl, box, ip, n, e_out, true); break;
   }
+}
+
+void launch_interact_extra(hipStream_t st, Fields f, NListD nl, BoxD box, const ExtraD& ex, int n, double* e_col,
+                           int add_forces) {
+  if (ex.count == 0) return;
+  int recount = 0;
+  for (int e = 0; e < ex.count; ++e) recount |= ex.kind[e] == EXTRA_SOFT_ATTRACTIVE;
+  hipLaunchKernelGGL(k_interact_extra, dim3(grid1(n)), dim3(kBlock), 0, st, f, nl, box, ex, n, e_col, recount,
+                     add_forces);
 }
 
 int reduce_blocks(int n) {

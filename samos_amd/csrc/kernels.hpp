@@ -43,6 +43,12 @@ void launch_interact(hipStream_t st, int pot_mask, int al_mask, int mode, Fields
 void launch_interact_f32(hipStream_t st, int pot_mask, int al_mask, int mode, Fields f, NListD nl, BoxD box,
                          InteractD ip, int n, double* e_out);
 
+// Further pair potentials (morse, gaussian, yukawa, soft_attractive) in one pass,
+// added to the forces when add_forces; per-slot energies (the reference's
+// per-pair energy summed over listed neighbours) into e_col when not null.
+void launch_interact_extra(hipStream_t st, Fields f, NListD nl, BoxD box, const ExtraD& ex, int n, double* e_col,
+                           int add_forces);
+
 // Deterministic two-pass sums of `ncols` columns of length n (column stride ld)
 // into out[ncols] (device).  partial must hold ncols * reduce_blocks(n) doubles.
 int reduce_blocks(int n);

@@ -77,6 +77,18 @@ struct InteractD {
 };
 
 enum PotMask { POT_SOFT = 1, POT_LJ = 2 };
+
+// Further pair potentials with global parameters, evaluated in their own pass
+// after soft / LJ, in the reference's registry-name order.
+enum ExtraKind { EXTRA_MORSE = 2, EXTRA_GAUSSIAN = 3, EXTRA_YUKAWA = 4, EXTRA_SOFT_ATTRACTIVE = 5 };
+constexpr int kMaxExtra = 4;
+struct ExtraD {
+  int count;
+  int kind[kMaxExtra];
+  int radii[kMaxExtra];
+  int shifted[kMaxExtra];
+  double q[kMaxExtra][8];
+};
 enum AlMask { AL_POLAR = 1, AL_NEMATIC = 2 };
 
 // Constraint applied per particle (src/constraints/*).

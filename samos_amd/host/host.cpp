@@ -601,6 +601,16 @@ std::map<std::string, PairPotentialFactory>& pair_potentials() {
       {"lj", [](SystemPtr s, MessengerPtr m_, NeighbourListPtr n, ValuePtr v, pairs_type& p) -> PairPotentialPtr {
          return std::make_shared<PairLJPotential>(s, m_, n, v, p);
        }}};
+  static bool extras_added = false;
+  if (!extras_added) {
+    extras_added = true;
+    for (const char* nm : {"morse", "gaussian", "yukawa", "soft_attractive"}) {
+      const std::string name = nm;
+      m[name] = [name](SystemPtr s, MessengerPtr m_, NeighbourListPtr n, ValuePtr v, pairs_type& p) -> PairPotentialPtr {
+        return std::make_shared<PairExtraPotential>(s, m_, n, v, p, name);
+      };
+    }
+  }
   return m;
 }
 
@@ -777,5 +787,61 @@ void Dump::dump(int step) {
     single_.flush();
   }
 }
+
+// Further pair potentials with their reference parameter names and defaults:
+//   morse            D 1, a 1, re 1, rcut 2.5 re     (pair_morse_potential.hpp:69-160)
+//   gaussian         A 2, B -1, alpha 1, beta 3, rA 0, rB 1.5, rcut 3
+//                    (pair_gaussian_potential.hpp:72-200; no unknown-parameter check)
+//   yukawa           g 1, kappa 1, rcut 3            (pair_yukawa_potential.hpp:65-120)
+//   soft_attractive  k 1, a 2, re_fact 1.25          (pair_soft_attractive_potential.hpp:70-140)
+PairExtraPotential::PairExtraPotential(SystemPtr sys, MessengerPtr msg, NeighbourListPtr nlist, ValuePtr val,
+                                       pairs_type& p, const std::string& name)
+    : PairPotential(sys, msg, nlist, val, p), name_(name) {
+  struct Spec { int kind; std::vector<std::pair<const char*, double>> params; bool check, shift; const char* label; };
+  static const std::map<std::string, Spec> specs = {
+      {"morse", {SAMOS_PAIR_MORSE, {{"D", 1.0}, {"a", 1.0}, {"re", 1.0}, {"rcut", -1.0}}, true, true, "Morse"}},
+      {"gaussian", {SAMOS_PAIR_GAUSSIAN, {{"A", 2.0}, {"B", -1.0}, {"alpha", 1.0}, {"beta", 3.0}, {"rA", 0.0},
+                                          {"rB", 1.5}, {"rcut", 3.0}}, false, true, "Gaussian"}},
+      {"yukawa", {SAMOS_PAIR_YUKAWA, {{"g", 1.0}, {"kappa", 1.0}, {"rcut", 3.0}}, true, false, "Yukawa"}},
+      {"soft_attractive", {SAMOS_PAIR_SOFT_ATTRACTIVE, {{"k", 1.0}, {"a", 2.0}, {"re_fact", 1.25}}, true, false,
+                           "soft attractive"}}};
+  if (!nlist_)
+    fail(msg_, std::string(specs.at(name).label) + " pair potential requires neighbour list. None given.",
+         "Neighbour list required by " + std::string(specs.at(name).label) + " potential, but non specified.");
+  const Spec& sp = specs.at(name);
+  for (auto& kv : sp.params) known_.push_back(kv.first);
+  if (name != "yukawa") known_.push_back("use_particle_radii");
+  if (name != "yukawa") known_.push_back("phase_in");
+  if (sp.shift) known_.push_back("shifted");
+  if (sp.check) {
+    std::string bad = params_ok(p);
+    if (!bad.empty())
+      fail(msg_, "Parameter \"" + bad + "\" is not a valid parameter for " + sp.label + " pair potential.",
+           "Unknown parameter \"" + bad + "\" in " + sp.label + " potential.");
+  }
+  if (has(p, "phase_in"))
+    fail(msg_, std::string(sp.label) + " pair potential: phase_in (population growth) is not supported by this build.",
+         "Unsupported parameter phase_in.");
+  std::vector<double> q;
+  for (auto& kv : sp.params) {
+    double v = kv.second;
+    if (has(p, kv.first)) v = to_double(p[kv.first], kv.first);
+    else msg_->msg(Messenger::WARNING, std::string("No ") + kv.first + " specified for " + sp.label +
+                                          " pair potential. Using the default.");
+    q.push_back(v);
+  }
+  if (name == "morse" && !has(p, "rcut")) q[3] = 2.5 * q[2];
+  const bool radii = has(p, "use_particle_radii") && name != "yukawa";
+  const bool shifted = has(p, "shifted") && sp.shift;
+  sys_->check(samos_pair_extra(sys_->ctx(), sp.kind, q.data(), (int)q.size(), radii ? 1 : 0, shifted ? 1 : 0));
+}
+
+void PairExtraPotential::set_pair_parameters(pairs_type&) {
+  fail(msg_, "Per-type parameters of the " + name_ + " pair potential are not supported by this build.",
+       "Unsupported pair_param for " + name_ + ".");
+}
+
+void PairExtraPotential::compute(double) { sys_->check(samos_pair_extra_energy(sys_->ctx(), &energy_)); }
+
 
 }  // namespace samos_host

@@ -162,6 +162,19 @@ class PairLJPotential : public PairPotential {
   double eps_ = 1.0, sigma_ = 1.0, rcut_ = 3.0;
 };
 
+// morse, gaussian, yukawa, soft_attractive: global parameters only (the
+// per-type pair_param form is rejected).
+class PairExtraPotential : public PairPotential {
+ public:
+  PairExtraPotential(SystemPtr sys, MessengerPtr msg, NeighbourListPtr nlist, ValuePtr val, pairs_type& param,
+                     const std::string& name);
+  void set_pair_parameters(pairs_type& pair_param) override;
+  void compute(double dt) override;
+
+ private:
+  std::string name_;
+};
+
 // ---------------------------------------------------------------------------
 // Pair aligners (src/aligner/pair/pair_aligner.hpp:53-89).
 // ---------------------------------------------------------------------------

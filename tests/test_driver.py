@@ -210,3 +210,51 @@ pair_potential soft {{ k = 1.0; kappa = 2 }}
 """)
     assert r.returncode != 0
     assert 'Unknown parameter "kappa" in soft potential.' in r.stderr
+
+
+@pytest.mark.gpu
+def test_conf_morse_and_yukawa_defaults(gpu, driver, tmp_path):
+    # reference parameter names; morse rcut defaults to 2.5 re, yukawa takes
+    # its defaults g = kappa = 1, rcut = 3
+    st = sc.plane_disks(1200, phi=0.7, seed=36)
+    L = st["box"][0]
+    write_input(tmp_path / "p.dat", st)
+    conf = f"""box periodic {{ lx = {L!r}; ly = {L!r}; lz = 10.0 }}
+input p.dat
+nlist {{ rcut = 3.2; pad = 0.3 }}
+constraint plane {{ }}
+pair_potential morse {{ D = 0.5; a = 2.0; re = 1.1; shifted }}
+pair_potential yukawa {{ }}
+timestep 0.005
+integrator brownian {{ v0 = 0.1; nu = 0.0; mu = 1.0; seed = 2 }}
+dump fin {{ type = full; start = 60; freq = 60; multi; id; coordinate; keys }}
+run 61
+"""
+    r = run_conf(tmp_path, conf)
+    assert r.returncode == 0, r.stderr
+    o = sc.load(oracle.OracleSystem, st)
+    o.nlist(3.2, 0.3)
+    o.pair_extra("morse", [0.5, 2.0, 1.1, 2.75], shifted=True)
+    o.pair_extra("yukawa", [1.0, 1.0, 3.0])
+    o.constraint("plane")
+    o.set_timestep(0.005)
+    o.integrator_brownian(v0=0.1, nu=0.0, mu=1.0, seed=2)
+    o.run(59)
+    d = read_full(tmp_path / "fin_0000000060.dat")
+    dx = sc.min_image(d[:, 1] - o.get("x"), L)
+    assert np.abs(dx).max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_conf_gaussian_per_type_rejected(gpu, driver, tmp_path):
+    st = sc.plane_disks(50, seed=37)
+    L = st["box"][0]
+    write_input(tmp_path / "p.dat", st)
+    r = run_conf(tmp_path, f"""box periodic {{ lx = {L!r}; ly = {L!r}; lz = 10.0 }}
+input p.dat
+nlist {{ rcut = 3.2; pad = 0.3 }}
+pair_potential gaussian {{ A = 1.0 }}
+pair_param gaussian {{ type_1 = 1; type_2 = 1; A = 2.0 }}
+""")
+    assert r.returncode != 0
+    assert "not supported" in r.stderr
