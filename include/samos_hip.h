@@ -38,7 +38,10 @@ enum samos_field {
   SAMOS_TYPE = 32, SAMOS_IX, SAMOS_IY, SAMOS_IZ, SAMOS_COORD
 };
 
-enum samos_constraint_kind { SAMOS_CONSTRAINT_NONE = 0, SAMOS_CONSTRAINT_PLANE = 1, SAMOS_CONSTRAINT_SPHERE = 2 };
+enum samos_constraint_kind {
+  SAMOS_CONSTRAINT_NONE = 0, SAMOS_CONSTRAINT_PLANE = 1, SAMOS_CONSTRAINT_SPHERE = 2,
+  SAMOS_CONSTRAINT_CYLINDER = 3, SAMOS_CONSTRAINT_TORUS = 4, SAMOS_CONSTRAINT_ELLIPSOID = 5
+};
 enum samos_pair_kind {
   SAMOS_PAIR_SOFT = 0, SAMOS_PAIR_LJ = 1,
   SAMOS_PAIR_MORSE = 2, SAMOS_PAIR_GAUSSIAN = 3, SAMOS_PAIR_YUKAWA = 4, SAMOS_PAIR_SOFT_ATTRACTIVE = 5
@@ -129,6 +132,16 @@ int samos_external_self_propulsion(samos_ctx* ctx, double alpha);
 /* kind NONE; PLANE with param = zpos, flag = unlimited; SPHERE with param = r.
  * Like the `constraint` command (src/samos.cpp:510-522) it enforces at once. */
 int samos_constraint(samos_ctx* ctx, int kind, double param, int flag);
+/* Further surfaces, enforced at once like samos_constraint:
+ *   CYLINDER  params {r}        (src/constraints/constraint_cylinder.hpp:42-67, .cpp:30-66)
+ *   TORUS     params {c, a}     (src/constraints/constraint_torus.hpp:42-77, .cpp:26-52)
+ *   ELLIPSOID params {a, b, c}  (src/constraints/constraint_ellipsoid.hpp:42-88, .cpp:26-48)
+ * The torus and the ellipsoid are projected by the generic SHAKE iteration of
+ * Constraint::enforce (src/constraints/constraint.cpp:40-87): at most max_iter
+ * steps along the gradient taken at the entry point, until |g(x)| <= tol
+ * (the reference's defaults: maxiter 100, tol 1e-6). Fails when the tube
+ * radius a is not smaller than the ring radius c. */
+int samos_constraint_surface(samos_ctx* ctx, int kind, const double* params, int nparams, int max_iter, double tol);
 /* Constrainer::enforce on every particle (src/samos.cpp:887-888). */
 int samos_enforce_all(samos_ctx* ctx);
 

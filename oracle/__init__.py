@@ -23,7 +23,7 @@ DOUBLE_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "nx", "ny", "nz", "fx", "fy", 
 INT_FIELDS = ["type", "ix", "iy", "iz", "coordination"]
 FIELD_CODE = {name: i for i, name in enumerate(DOUBLE_FIELDS)}
 FIELD_CODE.update({name: 32 + i for i, name in enumerate(INT_FIELDS)})
-CONSTRAINT = {"none": 0, "plane": 1, "sphere": 2}
+CONSTRAINT = {"none": 0, "plane": 1, "sphere": 2, "cylinder": 3, "torus": 4, "ellipsoid": 5}
 LANGEVIN_METHOD = {"baoab": 0, "spv": 1, "bbk": 2}
 RNG_STREAM = {"philox": 0, "mt19937": 1}
 EXTRA_KIND = {"morse": 2, "gaussian": 3, "yukawa": 4, "soft_attractive": 5}
@@ -68,6 +68,7 @@ def load(fast=False):
         "orc_align_type_param": (I, [P, I, I, I, D, D]),
         "orc_set_self_propulsion": (I, [P, D]),
         "orc_set_constraint": (I, [P, I, D, I]),
+        "orc_set_constraint_surface": (I, [P, I, dp, I, D]),
         "orc_enforce_all": (I, [P]),
         "orc_rotate_directors": (I, [P, dp]),
         "orc_project_torques": (I, [P, dp]),
@@ -208,6 +209,13 @@ class OracleSystem:
 
     def constraint(self, kind, param=0.0, flag=0):
         self._chk(self.L.orc_set_constraint(self.h, CONSTRAINT[kind], float(param), int(flag)))
+
+    def constraint_surface(self, kind, params, max_iter=100, tol=1e-6):
+        """cylinder {r}, torus {c, a}, ellipsoid {a, b, c}; SHAKE max_iter and tol."""
+        q = np.zeros(3)
+        q[:len(params)] = params
+        self._chk(self.L.orc_set_constraint_surface(self.h, CONSTRAINT[kind], q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                    int(max_iter), float(tol)))
 
     def enforce_all(self):
         self._chk(self.L.orc_enforce_all(self.h))

@@ -193,9 +193,12 @@ struct GslStream {
 };
 
 struct Constraint {
-  int kind = -1;        // 0 none, 1 plane, 2 sphere
+  int kind = -1;        // 0 none, 1 plane, 2 sphere, 3 cylinder, 4 torus, 5 ellipsoid
   double r = 1.0, zpos = 0.0;
   bool unlimited = false;
+  double tc = 1.0, ta = 0.5;                 // torus: ring radius c, tube radius a
+  double ea = 1.0, eb = 1.0, ec = 1.0;       // ellipsoid semi-axes
+  int max_iter = 100; double tol = 1e-6;     // SHAKE iteration (constraint.hpp:62-84)
 };
 
 struct Integ {
@@ -722,6 +725,94 @@ void plane_enforce(Sim& s, Particle& p) {
   p.nx *= inv_len; p.ny *= inv_len;
 }
 
+// Implicit surfaces projected by the generic SHAKE iteration.
+// ConstraintTorus (src/constraints/constraint_torus.cpp:26-52):
+//   g = (c - sqrt(x^2 + y^2))^2 + z^2 - a^2
+// ConstraintEllipsoid (src/constraints/constraint_ellipsoid.cpp:26-48):
+//   g = x^2/a^2 + y^2/b^2 + z^2/c^2 - 1
+double surface_value(const Sim& s, double x, double y, double z) {
+  const Constraint& c = s.con;
+  if (c.kind == 4) {
+    double ft = c.tc - std::sqrt(x * x + y * y);
+    return ft * ft + z * z - c.ta * c.ta;
+  }
+  return x * x / c.ea / c.ea + y * y / c.eb / c.eb + z * z / c.ec / c.ec - 1.0;
+}
+
+void surface_gradient(const Sim& s, double x, double y, double z, double g[3]) {
+  const Constraint& c = s.con;
+  if (c.kind == 4) {
+    double rho = std::sqrt(x * x + y * y);
+    double f = -2.0 * (c.tc - rho) / rho;
+    g[0] = x * f; g[1] = y * f; g[2] = 2.0 * z;
+  } else {
+    g[0] = 2.0 * x / c.ea / c.ea; g[1] = 2.0 * y / c.eb / c.eb; g[2] = 2.0 * z / c.ec / c.ec;
+  }
+}
+
+// compute_normal of the sphere, cylinder (x/r, y/r, 0), torus and ellipsoid
+// (normalised gradient); records it in the particle's N like the reference.
+void surface_normal(const Sim& s, Particle& p, double N[3]) {
+  const Constraint& c = s.con;
+  if (c.kind == 2) { N[0] = p.x / c.r; N[1] = p.y / c.r; N[2] = p.z / c.r; }
+  else if (c.kind == 3) { N[0] = p.x / c.r; N[1] = p.y / c.r; N[2] = 0.0; }
+  else {
+    surface_gradient(s, p.x, p.y, p.z, N);
+    double len = std::sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+    N[0] /= len; N[1] /= len; N[2] /= len;
+  }
+  p.Nx = N[0]; p.Ny = N[1]; p.Nz = N[2];
+}
+
+// Constraint::enforce (src/constraints/constraint.cpp:40-87): move along the
+// gradient taken at the entry point by lambda = g / (grad g . grad g_0) until
+// |g| <= tol or max_iter moves; then tangent projections of v and n, n
+// renormalised, periodic wrap.
+void shake_enforce(Sim& s, Particle& p) {
+  double g0[3];
+  surface_gradient(s, p.x, p.y, p.z, g0);
+  int iter = 0;
+  while (iter < s.con.max_iter) {
+    ++iter;
+    double gr[3];
+    surface_gradient(s, p.x, p.y, p.z, gr);
+    double dot = gr[0] * g0[0] + gr[1] * g0[1] + gr[2] * g0[2];
+    double g = surface_value(s, p.x, p.y, p.z);
+    if (!(std::fabs(g) > s.con.tol)) break;
+    double lambda = g / dot;
+    p.x -= lambda * g0[0]; p.y -= lambda * g0[1]; p.z -= lambda * g0[2];
+  }
+  double N[3];
+  surface_normal(s, p, N);
+  double vN = p.vx * N[0] + p.vy * N[1] + p.vz * N[2];
+  double nN = p.nx * N[0] + p.ny * N[1] + p.nz * N[2];
+  p.vx -= vN * N[0]; p.vy -= vN * N[1]; p.vz -= vN * N[2];
+  p.nx -= nN * N[0]; p.ny -= nN * N[1]; p.nz -= nN * N[2];
+  double inv_len = 1.0 / std::sqrt(p.nx * p.nx + p.ny * p.ny + p.nz * p.nz);
+  p.nx *= inv_len; p.ny *= inv_len; p.nz *= inv_len;
+  enforce_periodic(s, p);
+}
+
+// ConstraintCylinder::enforce (src/constraints/constraint_cylinder.cpp:30-66):
+// radial scaling in xy, xy projections of v and n, n renormalised over all
+// three components, z wrapped (strict bounds, image flag untouched).
+void cylinder_enforce(Sim& s, Particle& p) {
+  double R = std::sqrt(p.x * p.x + p.y * p.y);
+  double sc = s.con.r / R;
+  p.x *= sc; p.y *= sc;
+  double Nx = p.x / s.con.r, Ny = p.y / s.con.r;
+  double vN = p.vx * Nx + p.vy * Ny;
+  double nN = p.nx * Nx + p.ny * Ny;
+  p.vx -= vN * Nx; p.vy -= vN * Ny;
+  p.nx -= nN * Nx; p.ny -= nN * Ny;
+  double inv_len = 1.0 / std::sqrt(p.nx * p.nx + p.ny * p.ny + p.nz * p.nz);
+  p.nx *= inv_len; p.ny *= inv_len; p.nz *= inv_len;
+  if (s.periodic) {
+    if (p.z > s.box.zhi) p.z -= s.box.Lz;
+    else if (p.z < s.box.zlo) p.z += s.box.Lz;
+  }
+}
+
 // Constrainer::enforce (src/constraints/constrainer.hpp:73-78) with one constraint;
 // ConstraintNone::enforce (src/constraints/constraint_none.hpp:58-62)
 void enforce(Sim& s, Particle& p) {
@@ -729,6 +820,8 @@ void enforce(Sim& s, Particle& p) {
     case 0: enforce_periodic(s, p); break;
     case 1: plane_enforce(s, p); break;
     case 2: sphere_enforce(s, p); break;
+    case 3: cylinder_enforce(s, p); break;
+    case 4: case 5: shake_enforce(s, p); break;
     default: break;
   }
 }
@@ -738,10 +831,10 @@ void enforce(Sim& s, Particle& p) {
 double project_torque(Sim& s, Particle& p) {
   switch (s.con.kind) {
     case 1: return p.tau_z;
-    case 2: {
-      double Nx = p.x / s.con.r, Ny = p.y / s.con.r, Nz = p.z / s.con.r;
-      p.Nx = Nx; p.Ny = Ny; p.Nz = Nz;
-      return p.tau_x * Nx + p.tau_y * Ny + p.tau_z * Nz;
+    case 2: case 3: case 4: case 5: {
+      double N[3];
+      surface_normal(s, p, N);
+      return p.tau_x * N[0] + p.tau_y * N[1] + p.tau_z * N[2];
     }
     default: return 0.0;
   }
@@ -757,9 +850,10 @@ void rotate_director(Sim& s, Particle& p, double phi) {
     double ny = Ryx * p.nx + Ryy * p.ny;
     double len = std::sqrt(nx * nx + ny * ny);
     p.nx = nx / len; p.ny = ny / len;
-  } else if (s.con.kind == 2) {
-    double U = p.x / s.con.r, V = p.y / s.con.r, W = p.z / s.con.r;
-    p.Nx = U; p.Ny = V; p.Nz = W;
+  } else if (s.con.kind >= 2) {
+    double N[3];
+    surface_normal(s, p, N);
+    double U = N[0], V = N[1], W = N[2];
     double c = std::cos(phi), sn = std::sin(phi);
     double nx = U * (U * p.nx + V * p.ny + W * p.nz) * (1.0 - c) + p.nx * c + (-W * p.ny + V * p.nz) * sn;
     double ny = V * (U * p.nx + V * p.ny + W * p.nz) * (1.0 - c) + p.ny * c + (W * p.nx - U * p.nz) * sn;
@@ -776,8 +870,10 @@ void rotate_velocity(Sim& s, Particle& p, double phi) {
     double vx = c * p.vx + (-sn) * p.vy;
     double vy = sn * p.vx + c * p.vy;
     p.vx = vx; p.vy = vy;
-  } else if (s.con.kind == 2) {
-    double U = p.x / s.con.r, V = p.y / s.con.r, W = p.z / s.con.r;
+  } else if (s.con.kind >= 2) {
+    double N[3];
+    surface_normal(s, p, N);
+    double U = N[0], V = N[1], W = N[2];
     double c = std::cos(phi), sn = std::sin(phi);
     double vx = U * (U * p.vx + V * p.vy + W * p.vz) * (1.0 - c) + p.vx * c + (-W * p.vy + V * p.vz) * sn;
     double vy = V * (U * p.vx + V * p.vy + W * p.vz) * (1.0 - c) + p.vy * c + (W * p.vx - U * p.vz) * sn;
@@ -1342,6 +1438,20 @@ int orc_set_constraint(void* h, int kind, double param, int flag) {
   s->con.kind = kind;
   if (kind == 1) { s->con.zpos = param; s->con.unlimited = flag != 0; }
   if (kind == 2) s->con.r = param;
+  for (auto& p : s->p) enforce(*s, p);
+  return 0;
+}
+
+// Cylinder {r}, torus {c, a}, ellipsoid {a, b, c}; enforced at once.
+int orc_set_constraint_surface(void* h, int kind, const double* q, int max_iter, double tol) {
+  Sim* s = static_cast<Sim*>(h);
+  if (kind < 3 || kind > 5) { g_err = "unknown constraint kind"; return -1; }
+  if (kind == 4 && q[1] >= q[0]) { g_err = "Toroidal constraint. Tube radius has to be smaller than torus radius."; return -1; }
+  s->con.kind = kind;
+  s->con.max_iter = max_iter; s->con.tol = tol;
+  if (kind == 3) s->con.r = q[0];
+  if (kind == 4) { s->con.tc = q[0]; s->con.ta = q[1]; }
+  if (kind == 5) { s->con.ea = q[0]; s->con.eb = q[1]; s->con.ec = q[2]; }
   for (auto& p : s->p) enforce(*s, p);
   return 0;
 }

@@ -24,7 +24,7 @@ INT_FIELDS = ["type", "ix", "iy", "iz", "coordination"]
 FIELD_CODE = {name: i for i, name in enumerate(DOUBLE_FIELDS)}
 FIELD_CODE.update({name: 32 + i for i, name in enumerate(INT_FIELDS)})
 
-CONSTRAINT = {"none": 0, "plane": 1, "sphere": 2}
+CONSTRAINT = {"none": 0, "plane": 1, "sphere": 2, "cylinder": 3, "torus": 4, "ellipsoid": 5}
 LANGEVIN_METHOD = {"baoab": 0, "spv": 1, "bbk": 2}
 RUN_FAITHFUL, RUN_NO_GRAPH, RUN_FP32_FORCE, RUN_TILES = 1, 2, 4, 8
 EXTRA_KIND = {"morse": 2, "gaussian": 3, "yukawa": 4, "soft_attractive": 5}
@@ -71,6 +71,7 @@ def load_library(path: str = LIB_PATH):
         "samos_align_type_param": (I, [P, I, I, I, D, D]),
         "samos_external_self_propulsion": (I, [P, D]),
         "samos_constraint": (I, [P, I, D, I]),
+        "samos_constraint_surface": (I, [P, I, dp, I, I, D]),
         "samos_enforce_all": (I, [P]),
         "samos_integrator_brownian": (I, [P, I, D, D, D, D, I, I, D, I, I, D, D, I]),
         "samos_integrator_langevin": (I, [P, D, I, I, I, D, D, I]),
@@ -261,6 +262,12 @@ class System:
 
     def constraint(self, kind, param=0.0, flag=0):
         self._chk(self.L.samos_constraint(self.h, CONSTRAINT[kind], float(param), int(flag)))
+
+    def constraint_surface(self, kind, params, max_iter=100, tol=1e-6):
+        """cylinder {r}, torus {c, a}, ellipsoid {a, b, c}; SHAKE max_iter and tol."""
+        q = np.ascontiguousarray(params, dtype=np.float64)
+        self._chk(self.L.samos_constraint_surface(self.h, CONSTRAINT[kind], q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                  len(q), int(max_iter), float(tol)))
 
     def enforce_all(self):
         self._chk(self.L.samos_enforce_all(self.h))

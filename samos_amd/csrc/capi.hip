@@ -73,7 +73,7 @@ struct samos_ctx {
   double sp_alpha = 1.0;
   std::vector<double> soft_tab, lj_tab, pol_tab, nem_tab;  // host copies [3][16][16]
   double* d_tab = nullptr;                                  // 4 tables x 3 x 256
-  ConstraintD con{-1, 1.0, 0.0, 0};
+  ConstraintD con{-1, 1.0, 0.0, 0, 0.0, 0.0, 0.0, 100, 1e-6};
   std::map<std::string, Integ> integs;
   ClockD* d_clock = nullptr;
   int time_step = 0;
@@ -1112,6 +1112,28 @@ int samos_constraint(samos_ctx* ctx, int kind, double param, int flag) {
   ctx->con.kind = kind;
   if (kind == 1) { ctx->con.zpos = param; ctx->con.unlimited = flag ? 1 : 0; }
   if (kind == 2) ctx->con.r = param;
+  launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int samos_constraint_surface(samos_ctx* ctx, int kind, const double* params, int nparams, int max_iter, double tol) {
+  static const int need[] = {0, 0, 0, 1, 2, 3};
+  if (kind < SAMOS_CONSTRAINT_CYLINDER || kind > SAMOS_CONSTRAINT_ELLIPSOID) return fail(ctx, "unknown constraint kind");
+  if (nparams != need[kind]) return fail(ctx, "wrong parameter count for the constraint");
+  if (max_iter < 0) return fail(ctx, "Constraint. Maximum number of iterations has to be non-negative.");
+  ConstraintD c = ctx->con;
+  c.kind = kind;
+  c.max_iter = max_iter;
+  c.tol = tol;
+  if (kind == SAMOS_CONSTRAINT_CYLINDER) c.r = params[0];
+  if (kind == SAMOS_CONSTRAINT_TORUS) {
+    c.q0 = params[0]; c.q1 = params[1];
+    if (c.q1 >= c.q0) return fail(ctx, "Toroidal constraint. Tube radius has to be smaller than torus radius.");
+  }
+  if (kind == SAMOS_CONSTRAINT_ELLIPSOID) { c.q0 = params[0]; c.q1 = params[1]; c.q2 = params[2]; }
+  ctx->con = c;
   launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
   HIPCHK(hipStreamSynchronize(ctx->st));
   HIPCHK(hipGetLastError());

@@ -4,8 +4,16 @@
 //   sphere           src/constraints/constraint_sphere.cpp:38-81
 //   plane            src/constraints/constraint_plane.cpp:36-160
 //   none             src/constraints/constraint_none.hpp:58-80
+//   cylinder         src/constraints/constraint_cylinder.cpp:30-66, constraint_cylinder.hpp:55-67
+//   torus            src/constraints/constraint_torus.cpp:26-52 (generic SHAKE projection)
+//   ellipsoid        src/constraints/constraint_ellipsoid.cpp:26-48 (generic SHAKE projection)
+//   SHAKE projection src/constraints/constraint.cpp:40-87
 //   rotate/project   src/constraints/constraint.cpp:96-173
 //   periodic wrap    src/system/system.cpp:1119-1155
+//
+// Each function takes a template bitmask SET of the constraint kinds it is
+// compiled for (bit k = kind k), so kernels that only ever see some kinds
+// (the planar fused step, the replica ensembles) carry no code for the others.
 #pragma once
 #include "samos_device.hpp"
 
@@ -27,12 +35,51 @@ __device__ __forceinline__ void enforce_periodic(const BoxD& b, PS& p) {
   if (p.z <= b.zlo) { p.z += b.Lz; p.diz--; } else if (p.z > b.zhi) { p.z -= b.Lz; p.diz++; }
 }
 
+constexpr int kSurfBasic = 0x07;   // none, plane, sphere
+constexpr int kSurfAll = 0x3f;     // and cylinder, torus, ellipsoid
+constexpr int kSurfPlane = 0x02;
+
+// Implicit function g(x) = 0 of the torus (kind 4) or the ellipsoid (kind 5)
+// and its gradient, in the reference's operation order.
+__device__ __forceinline__ double surf_value(const ConstraintD& c, double x, double y, double z) {
+  if (c.kind == 4) {
+    const double ft = c.q0 - sqrt(x * x + y * y);
+    return ft * ft + z * z - c.q1 * c.q1;
+  }
+  return x * x / c.q0 / c.q0 + y * y / c.q1 / c.q1 + z * z / c.q2 / c.q2 - 1.0;
+}
+
+__device__ __forceinline__ void surf_gradient(const ConstraintD& c, double x, double y, double z, double& gx,
+                                              double& gy, double& gz) {
+  if (c.kind == 4) {
+    const double sxy = sqrt(x * x + y * y);
+    const double fact = -2.0 * (c.q0 - sxy) / sxy;
+    gx = x * fact; gy = y * fact; gz = 2.0 * z;
+  } else {
+    gx = 2.0 * x / c.q0 / c.q0; gy = 2.0 * y / c.q1 / c.q1; gz = 2.0 * z / c.q2 / c.q2;
+  }
+}
+
+// Unit normal of kinds 2-5 at the particle (compute_normal of each surface).
+__device__ __forceinline__ void surf_normal(const ConstraintD& c, const PS& p, double& Nx, double& Ny, double& Nz) {
+  if (c.kind == 2) {
+    Nx = p.x / c.r; Ny = p.y / c.r; Nz = p.z / c.r;
+  } else if (c.kind == 3) {
+    Nx = p.x / c.r; Ny = p.y / c.r; Nz = 0.0;
+  } else {
+    surf_gradient(c, p.x, p.y, p.z, Nx, Ny, Nz);
+    const double len = sqrt(Nx * Nx + Ny * Ny + Nz * Nz);
+    Nx /= len; Ny /= len; Nz /= len;
+  }
+}
+
 // Constraint::enforce of the one active constraint; returns true when the
 // plane constraint zeroed the particle's f_z.
+template <int SET = kSurfAll>
 __device__ __forceinline__ bool enforce(const BoxD& b, const ConstraintD& c, PS& p) {
-  if (c.kind == 0) {
+  if ((SET & 0x01) && c.kind == 0) {
     enforce_periodic(b, p);
-  } else if (c.kind == 1) {
+  } else if ((SET & 0x02) && c.kind == 1) {
     const double xlo = -0.5 * b.Lx, xhi = 0.5 * b.Lx, ylo = -0.5 * b.Ly, yhi = 0.5 * b.Ly;
     p.z = c.zpos; p.vz = 0.0;
     if (b.periodic) enforce_periodic(b, p);
@@ -46,7 +93,7 @@ __device__ __forceinline__ bool enforce(const BoxD& b, const ConstraintD& c, PS&
     if (len_n != 0.0) inv_len = 1.0 / sqrt(p.nx * p.nx + p.ny * p.ny);
     p.nx *= inv_len; p.ny *= inv_len;
     return true;
-  } else if (c.kind == 2) {
+  } else if ((SET & 0x04) && c.kind == 2) {
     const double R0 = c.r;
     const double R = sqrt(p.x * p.x + p.y * p.y + p.z * p.z);
     const double sc = R0 / R;
@@ -58,29 +105,71 @@ __device__ __forceinline__ bool enforce(const BoxD& b, const ConstraintD& c, PS&
     p.nx -= nN * Nx; p.ny -= nN * Ny; p.nz -= nN * Nz;
     const double inv_len = 1.0 / sqrt(p.nx * p.nx + p.ny * p.ny + p.nz * p.nz);
     p.nx *= inv_len; p.ny *= inv_len; p.nz *= inv_len;
+  } else if ((SET & 0x08) && c.kind == 3) {
+    // radial scaling in the xy plane; z is wrapped without an image-flag change
+    const double R = sqrt(p.x * p.x + p.y * p.y);
+    const double sc = c.r / R;
+    p.x *= sc; p.y *= sc;
+    const double Nx = p.x / c.r, Ny = p.y / c.r;
+    const double vN = p.vx * Nx + p.vy * Ny;
+    const double nN = p.nx * Nx + p.ny * Ny;
+    p.vx -= vN * Nx; p.vy -= vN * Ny;
+    p.nx -= nN * Nx; p.ny -= nN * Ny;
+    const double inv_len = 1.0 / sqrt(p.nx * p.nx + p.ny * p.ny + p.nz * p.nz);
+    p.nx *= inv_len; p.ny *= inv_len; p.nz *= inv_len;
+    if (b.periodic) {
+      if (p.z > b.zhi) p.z -= b.Lz;
+      else if (p.z < b.zlo) p.z += b.Lz;
+    }
+  } else if ((SET & 0x30) && (c.kind == 4 || c.kind == 5)) {
+    // SHAKE: step along the gradient taken at the entry point until |g| <= tol
+    double rgx, rgy, rgz;
+    surf_gradient(c, p.x, p.y, p.z, rgx, rgy, rgz);
+    for (int iter = 0; iter < c.max_iter; ++iter) {
+      double gx, gy, gz;
+      surf_gradient(c, p.x, p.y, p.z, gx, gy, gz);
+      const double sdot = gx * rgx + gy * rgy + gz * rgz;
+      const double g = surf_value(c, p.x, p.y, p.z);
+      if (!(fabs(g) > c.tol)) break;
+      const double lambda = g / sdot;
+      p.x -= lambda * rgx; p.y -= lambda * rgy; p.z -= lambda * rgz;
+    }
+    double Nx, Ny, Nz;
+    surf_normal(c, p, Nx, Ny, Nz);
+    const double vN = p.vx * Nx + p.vy * Ny + p.vz * Nz;
+    const double nN = p.nx * Nx + p.ny * Ny + p.nz * Nz;
+    p.vx -= vN * Nx; p.vy -= vN * Ny; p.vz -= vN * Nz;
+    p.nx -= nN * Nx; p.ny -= nN * Ny; p.nz -= nN * Nz;
+    const double inv_len = 1.0 / sqrt(p.nx * p.nx + p.ny * p.ny + p.nz * p.nz);
+    p.nx *= inv_len; p.ny *= inv_len; p.nz *= inv_len;
+    enforce_periodic(b, p);
   }
   return false;
 }
 
+template <int SET = kSurfAll>
 __device__ __forceinline__ double project_torque(const ConstraintD& c, const PS& p, double tx, double ty, double tz) {
-  if (c.kind == 1) return tz;
-  if (c.kind == 2) {
-    const double Nx = p.x / c.r, Ny = p.y / c.r, Nz = p.z / c.r;
+  if ((SET & 0x02) && c.kind == 1) return tz;
+  if ((SET & 0x3c) && c.kind >= 2) {
+    double Nx, Ny, Nz;
+    surf_normal(c, p, Nx, Ny, Nz);
     return tx * Nx + ty * Ny + tz * Nz;
   }
   return 0.0;
 }
 
+template <int SET = kSurfAll>
 __device__ __forceinline__ void rotate_director(const ConstraintD& c, PS& p, double phi) {
-  if (c.kind == 1) {
+  if ((SET & 0x02) && c.kind == 1) {
     double sn, cs;
     sincos(phi, &sn, &cs);
     const double nx = cs * p.nx + (-sn) * p.ny;
     const double ny = sn * p.nx + cs * p.ny;
     const double len = sqrt(nx * nx + ny * ny);
     p.nx = nx / len; p.ny = ny / len;
-  } else if (c.kind == 2) {
-    const double U = p.x / c.r, V = p.y / c.r, W = p.z / c.r;
+  } else if ((SET & 0x3c) && c.kind >= 2) {
+    double U, V, W;
+    surf_normal(c, p, U, V, W);
     double sn, cs;
     sincos(phi, &sn, &cs);
     const double un = U * p.nx + V * p.ny + W * p.nz;
@@ -92,15 +181,17 @@ __device__ __forceinline__ void rotate_director(const ConstraintD& c, PS& p, dou
   }
 }
 
+template <int SET = kSurfAll>
 __device__ __forceinline__ void rotate_velocity(const ConstraintD& c, PS& p, double phi) {
-  if (c.kind == 1) {
+  if ((SET & 0x02) && c.kind == 1) {
     double sn, cs;
     sincos(phi, &sn, &cs);
     const double vx = cs * p.vx + (-sn) * p.vy;
     const double vy = sn * p.vx + cs * p.vy;
     p.vx = vx; p.vy = vy;
-  } else if (c.kind == 2) {
-    const double U = p.x / c.r, V = p.y / c.r, W = p.z / c.r;
+  } else if ((SET & 0x3c) && c.kind >= 2) {
+    double U, V, W;
+    surf_normal(c, p, U, V, W);
     double sn, cs;
     sincos(phi, &sn, &cs);
     const double uv = U * p.vx + V * p.vy + W * p.vz;

@@ -458,11 +458,11 @@ __global__ void __launch_bounds__(kBlock) k_ens_brownian(EState e, int b, const 
     const double fdx = R.v0 * p.nx + R.mu * fx, fdy = R.v0 * p.ny + R.mu * fy, fdz = R.v0 * p.nz + R.mu * fz;
     p.vx = fdx; p.vy = fdy; p.vz = fdz;
     p.x += dt * fdx; p.y += dt * fdy; p.z += dt * fdz;
-    const bool fz0 = enforce(R.box, R.con, p);
-    const double om = R.mur * project_torque(R.con, p, tx, ty, tz);
+    const bool fz0 = enforce<kSurfBasic>(R.box, R.con, p);
+    const double om = R.mur * project_torque<kSurfBasic>(R.con, p, tx, ty, tz);
     double g, u;
     gauss1_u1(R.seed, TAG_BROWNIAN, (uint32_t)e.lid[s], call, g, u);
-    rotate_director(R.con, p, dt * om + R.stoch * g);
+    rotate_director<kSurfBasic>(R.con, p, dt * om + R.stoch * g);
     flag_rebuild(R, nl, s, r, p.x, p.y, p.z, rep_need, need);
     if (last) {
       e.vx[s] = p.vx; e.vy[s] = p.vy; e.vz[s] = p.vz;
@@ -470,7 +470,7 @@ __global__ void __launch_bounds__(kBlock) k_ens_brownian(EState e, int b, const 
       e.tx[s] = tx; e.ty[s] = ty; e.tz[s] = tz;
       e.omega[s] = om;
     }
-    if (head) enforce(R.box, R.con, p);
+    if (head) enforce<kSurfBasic>(R.box, R.con, p);
     store_pd(e, b ^ 1, s, p, rad);
   });
 }
@@ -493,7 +493,7 @@ __global__ void k_ens_enforce(EState e, int b, const RepD* __restrict__ reps, in
   double rad;
   load_ps(e, b, s, p, rad);
   p.vx = e.vx[s]; p.vy = e.vy[s]; p.vz = e.vz[s];
-  if (enforce(R.box, R.con, p)) e.fz[s] = 0.0;
+  if (enforce<kSurfBasic>(R.box, R.con, p)) e.fz[s] = 0.0;
   store_pd(e, b, s, p, rad);
   e.vx[s] = p.vx; e.vy[s] = p.vy; e.vz[s] = p.vz;
 }
@@ -539,12 +539,12 @@ __global__ void k_ens_fire_a(EState e, int b, const RepD* __restrict__ reps, Wor
     load_ps(e, b, s, p, rad);
     p.vx = e.vx[s]; p.vy = e.vy[s]; p.vz = e.vz[s];
     double fz = e.fz[s];
-    if (enforce(R.box, R.con, p)) fz = 0.0;
+    if (enforce<kSurfBasic>(R.box, R.con, p)) fz = 0.0;
     p.vx += dt_2 * e.fx[s];
     p.vy += dt_2 * e.fy[s];
     p.vz += dt_2 * fz;
     p.x += dt * p.vx; p.y += dt * p.vy; p.z += dt * p.vz;
-    enforce(R.box, R.con, p);
+    enforce<kSurfBasic>(R.box, R.con, p);
     store_pd(e, b, s, p, rad);
     e.vx[s] = p.vx; e.vy[s] = p.vy; e.vz[s] = p.vz;
   }
@@ -579,7 +579,7 @@ __global__ void __launch_bounds__(kBlock) k_ens_fire_b(EState e, int b, const Re
       cols[2 * ld + s] = p.vx * p.vx + p.vy * p.vy + p.vz * p.vz;
       cols[3 * ld + s] = en;
       cols[4 * ld + s] = sqrt(f2);
-      if (enforce(R.box, R.con, p)) fz = 0.0;
+      if (enforce<kSurfBasic>(R.box, R.con, p)) fz = 0.0;
       e.fx[s] = fx; e.fy[s] = fy; e.fz[s] = fz;
       e.vx[s] = p.vx; e.vy[s] = p.vy; e.vz[s] = p.vz;
       flag_rebuild(R, nl, s, r, p.x, p.y, p.z, rep_need, need);
@@ -669,7 +669,7 @@ __global__ void k_ens_fire_mix(EState e, int b, const RepD* __restrict__ reps, W
     PS q;
     double rad;
     load_ps(e, b, s, q, rad);
-    enforce(R.box, R.con, q);
+    enforce<kSurfBasic>(R.box, R.con, q);
     store_pd(e, b, s, q, rad);
     if (R.frozen) continue;
     double ux = R.f1 * e.vx[s] + R.f2 * e.fx[s];

@@ -374,9 +374,19 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 6 : 1) k_baoab_fused
       p.x = xi; p.y = yi; p.z = zi; p.vx = vx; p.vy = vy; p.vz = vz;
       p.nx = cur.nx[s]; p.ny = cur.ny[s]; p.nz = PLANAR ? 0.0 : cur.nz[s];
       p.dix = p.diy = p.diz = 0;
+      constexpr int kSet = PLANAR ? kSurfPlane : kSurfAll;
+      if (!PLANAR && PHASE == 1) {
+        // The loop-head enforce of step t acts on the velocity after step t-1's
+        // closing kick.  The positions already carry it (applied when written);
+        // the velocity projection is repeated here on the kicked velocity.  On a
+        // plane the kick has no normal component, so this is skipped there.
+        PS q = p;
+        enforce<kSet>(b, c, q);
+        p.vx = q.vx; p.vy = q.vy; p.vz = q.vz;
+      }
       p.vx += fact * fx; p.vy += fact * fy; p.vz += fact * fz;
       p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
-      bool fz0 = enforce(b, c, p);
+      bool fz0 = enforce<kSet>(b, c, p);
       p.vx *= exp_dt; p.vy *= exp_dt; p.vz *= exp_dt;
       if (B != 0.0) {
         const double sf = B / sqrt(m);
@@ -391,8 +401,8 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 6 : 1) k_baoab_fused
         p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
       }
       p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
-      fz0 |= enforce(b, c, p);
-      if (head_enforce) fz0 |= enforce(b, c, p);
+      fz0 |= enforce<kSet>(b, c, p);
+      if (head_enforce) fz0 |= enforce<kSet>(b, c, p);
       if (PLANAR) {
         nxt.xy[s] = make_double2(p.x, p.y);
       } else {

@@ -93,10 +93,13 @@ enum AlMask { AL_POLAR = 1, AL_NEMATIC = 2 };
 
 // Constraint applied per particle (src/constraints/*).
 struct ConstraintD {
-  int kind;           // 0 none, 1 plane, 2 sphere, -1 unset
-  double r;           // sphere radius
+  int kind;           // 0 none, 1 plane, 2 sphere, 3 cylinder, 4 torus, 5 ellipsoid, -1 unset
+  double r;           // sphere and cylinder radius
   double zpos;        // plane height
   int unlimited;      // plane, non-periodic: no reflective walls
+  double q0, q1, q2;  // torus: c (ring radius), a (tube radius); ellipsoid: a, b, c
+  int max_iter;       // SHAKE projection (torus, ellipsoid): iteration cap and
+  double tol;         //   tolerance on |g(x)|
 };
 
 // Temperature schedule (src/system/value.hpp:115-170).

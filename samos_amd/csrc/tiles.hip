@@ -568,15 +568,15 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_step(TileD td, SlotOut so
         p.dix = p.diy = p.diz = 0;
         p.vx += fact * fx; p.vy += fact * fy; p.vz += fact * fz;
         p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
-        enforce(b, c, p);
+        enforce<kSurfPlane>(b, c, p);
         p.vx *= fa.exp_dt; p.vy *= fa.exp_dt; p.vz *= fa.exp_dt;
         if (B != 0.0) {
           const double sfac = B / sqrt(m);
           p.vx += sfac * gn[0]; p.vy += sfac * gn[1]; p.vz += sfac * 0.0;
         }
         p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
-        enforce(b, c, p);
-        if (fa.head_enforce) enforce(b, c, p);
+        enforce<kSurfPlane>(b, c, p);
+        if (fa.head_enforce) enforce<kSurfPlane>(b, c, p);
         A.xy[par ^ 1][q] = make_double2(p.x, p.y);
         A.v[q] = make_double2(p.vx, p.vy);
         if (p.nx != n0.x || p.ny != n0.y) A.nd[q] = make_double2(p.nx, p.ny);

@@ -463,6 +463,21 @@ Constraint::Constraint(SystemPtr sys, MessengerPtr msg, pairs_type& p) : sys_(sy
     fail(msg_, "Constraint rescaling is not supported by this build.", "Unsupported parameter rescale.");
   if (has(p, "group") && p["group"] != "all")
     fail(msg_, "Constraints on groups other than \"all\" are not supported by this build.", "Unsupported group.");
+  // The reference reads maxiter and tol for every constraint; both go through an
+  // integer conversion (constraint.hpp:71,82), so a fractional tolerance such as
+  // 1e-8 is a conversion error there as here.
+  if (has(p, "maxiter")) {
+    max_iter_ = to_int(p["maxiter"], "maxiter");
+    msg_->msg(Messenger::INFO, "Constraint. Maximum number of iterations set to " + p["maxiter"] + ".");
+  } else {
+    msg_->msg(Messenger::WARNING, "Constraint. Maximum number of iterations has not been set. Assuming 100");
+  }
+  if (has(p, "tol")) {
+    tol_ = (double)to_int(p["tol"], "tol");
+    msg_->msg(Messenger::INFO, "Constraint. Tolerance set to " + p["tol"] + ".");
+  } else {
+    msg_->msg(Messenger::WARNING, "Constraint. Tolerance has not been set. Assuming 1e-6.");
+  }
 }
 
 ConstraintNone::ConstraintNone(SystemPtr sys, MessengerPtr msg, pairs_type& p) : Constraint(sys, msg, p) {
@@ -478,6 +493,32 @@ ConstraintPlane::ConstraintPlane(SystemPtr sys, MessengerPtr msg, pairs_type& p)
 ConstraintSphere::ConstraintSphere(SystemPtr sys, MessengerPtr msg, pairs_type& p) : Constraint(sys, msg, p) {
   if (has(p, "r")) r_ = to_double(p["r"], "r");
   else msg_->msg(Messenger::WARNING, "Spherical constraint. No radius set. Assuming 1.");
+}
+
+ConstraintSurface::ConstraintSurface(SystemPtr sys, MessengerPtr msg, pairs_type& p, const std::string& name)
+    : Constraint(sys, msg, p) {
+  struct Spec { int kind; std::vector<std::pair<const char*, double>> params; std::vector<std::string> warn; };
+  static const std::map<std::string, Spec> specs = {
+      {"cylinder", {SAMOS_CONSTRAINT_CYLINDER, {{"r", 1.0}}, {"Cylindrical constraint. No radius set. Assuming 1."}}},
+      {"torus", {SAMOS_CONSTRAINT_TORUS, {{"c", 1.0}, {"a", 0.5}},
+                 {"Toroidal constraint. No torus radius set. Assuming 1.",
+                  "Toroidal constraint. No tube radius set. Assuming 0.5."}}},
+      {"ellipsoid", {SAMOS_CONSTRAINT_ELLIPSOID, {{"a", 1.0}, {"b", 1.0}, {"c", 1.0}},
+                     {"Ellipsoid constraint. No parameter a. Assuming 1.", "Ellipsoid constraint. No parameter b. Assuming 1.",
+                      "Ellipsoid constraint. No parameter c. Assuming 1."}}}};
+  const Spec& sp = specs.at(name);
+  kind_ = sp.kind;
+  for (size_t i = 0; i < sp.params.size(); ++i) {
+    const char* key = sp.params[i].first;
+    if (has(p, key)) q_.push_back(to_double(p[key], key));
+    else {
+      msg_->msg(Messenger::WARNING, sp.warn[i]);
+      q_.push_back(sp.params[i].second);
+    }
+  }
+  if (kind_ == SAMOS_CONSTRAINT_TORUS && q_[1] >= q_[0])
+    fail(msg_, "Toroidal constraint. Tube radius has to be smaller than torus radius.",
+         "Toroidal constraint. Incompatible parameters.");
 }
 
 // ---------------------------------------------------------------------------
@@ -629,7 +670,10 @@ std::map<std::string, ConstraintFactory>& constraints() {
   static std::map<std::string, ConstraintFactory> m = {
       {"none", [](SystemPtr s, MessengerPtr m_, pairs_type& p) -> ConstraintPtr { return std::make_shared<ConstraintNone>(s, m_, p); }},
       {"plane", [](SystemPtr s, MessengerPtr m_, pairs_type& p) -> ConstraintPtr { return std::make_shared<ConstraintPlane>(s, m_, p); }},
-      {"sphere", [](SystemPtr s, MessengerPtr m_, pairs_type& p) -> ConstraintPtr { return std::make_shared<ConstraintSphere>(s, m_, p); }}};
+      {"sphere", [](SystemPtr s, MessengerPtr m_, pairs_type& p) -> ConstraintPtr { return std::make_shared<ConstraintSphere>(s, m_, p); }},
+      {"cylinder", [](SystemPtr s, MessengerPtr m_, pairs_type& p) -> ConstraintPtr { return std::make_shared<ConstraintSurface>(s, m_, p, "cylinder"); }},
+      {"torus", [](SystemPtr s, MessengerPtr m_, pairs_type& p) -> ConstraintPtr { return std::make_shared<ConstraintSurface>(s, m_, p, "torus"); }},
+      {"ellipsoid", [](SystemPtr s, MessengerPtr m_, pairs_type& p) -> ConstraintPtr { return std::make_shared<ConstraintSurface>(s, m_, p, "ellipsoid"); }}};
   return m;
 }
 

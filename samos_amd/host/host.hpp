@@ -221,10 +221,14 @@ class Constraint {
   virtual int kind() const = 0;
   virtual double param() const { return 0.0; }
   virtual int flag() const { return 0; }
+  // Registers the constraint with the device context and enforces it once.
+  virtual int install(samos_ctx* ctx) { return samos_constraint(ctx, kind(), param(), flag()); }
 
  protected:
   SystemPtr sys_;
   MessengerPtr msg_;
+  int max_iter_ = 100;   // maxiter (constraint.hpp:62-73)
+  double tol_ = 1e-6;    // tol (constraint.hpp:74-84)
 };
 using ConstraintPtr = std::shared_ptr<Constraint>;
 
@@ -254,6 +258,21 @@ class ConstraintSphere : public Constraint {
 
  private:
   double r_ = 1.0;
+};
+
+// Cylinder {r}, torus {c, a} and ellipsoid {a, b, c}
+// (src/constraints/constraint_cylinder.hpp, constraint_torus.hpp, constraint_ellipsoid.hpp).
+class ConstraintSurface : public Constraint {
+ public:
+  ConstraintSurface(SystemPtr sys, MessengerPtr msg, pairs_type& param, const std::string& name);
+  int kind() const override { return kind_; }
+  int install(samos_ctx* ctx) override {
+    return samos_constraint_surface(ctx, kind_, q_.data(), (int)q_.size(), max_iter_, tol_);
+  }
+
+ private:
+  int kind_;
+  std::vector<double> q_;
 };
 
 // ---------------------------------------------------------------------------

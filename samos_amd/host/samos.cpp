@@ -91,16 +91,17 @@ struct Driver {
         fail(msg, "More than one constraint is not supported by this build.", "Unsupported constraint combination.");
       auto it = constraints().find(attrib);
       if (it == constraints().end())
-        fail(msg, "Constraint " + attrib + " is not available (sphere, plane, none)" + where, "Unknown constraint.");
+        fail(msg, "Constraint " + attrib + " is not available (sphere, plane, none, cylinder, torus, ellipsoid)" + where,
+             "Unknown constraint.");
       constraint = it->second(sys, msg, p);
-      sys->check(samos_constraint(sys->ctx(), constraint->kind(), constraint->param(), constraint->flag()));
+      sys->check(constraint->install(sys->ctx()));
       msg->msg(Messenger::INFO, "Adding constraint of type " + attrib + ".");
     } else if (cmd == "pair_potential") {
       need_system("adding any pair potentials");
       parse();
       auto it = pair_potentials().find(attrib);
       if (it == pair_potentials().end())
-        fail(msg, "Pair potential " + attrib + " is not available (soft, lj)" + where, "Unknown pair potential.");
+        fail(msg, "Pair potential " + attrib + " is not available (soft, lj, morse, gaussian, yukawa, soft_attractive)" + where, "Unknown pair potential.");
       default_nlist();
       const std::string phase_in = p.count("phase_in") ? p["phase_in"] : "constant";
       ValuePtr v = make_value(phase_in.empty() ? "constant" : phase_in, msg, p);

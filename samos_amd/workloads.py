@@ -41,6 +41,41 @@ def sphere_disks(n, R=None, phi=1.0, seed=2, box=None):
     return {"n": n, "box": (L, L, L), "periodic": False, "cols": cols, "R": R}
 
 
+def surface_disks(kind, n, params, seed=5, jitter=0.05, lz=None):
+    """Unit disks near a cylinder {r}, torus {c, a} or ellipsoid {a, b, c},
+    displaced off the surface by up to `jitter` along x, y, z, with random
+    directors and small random velocities (neither tangent: the first enforce
+    projects them).  The cylinder box is periodic in z (length lz, default
+    n / (2 r) so the surface density is about 1/pi); the others are open boxes."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(0.0, 2.0 * math.pi, n)
+    if kind == "cylinder":
+        r = params[0]
+        Lz = lz if lz else max(4.0, n / (2.0 * r))
+        p = np.stack([r * np.cos(u), r * np.sin(u), rng.uniform(-Lz / 2, Lz / 2, n)], 1)
+        box, periodic = (2.0 * r + 8.0, 2.0 * r + 8.0, Lz), True
+    elif kind == "torus":
+        c, a = params
+        w = rng.uniform(0.0, 2.0 * math.pi, n)
+        rho = c + a * np.cos(w)
+        p = np.stack([rho * np.cos(u), rho * np.sin(u), a * np.sin(w)], 1)
+        L = 2.0 * (c + a) + 8.0
+        box, periodic = (L, L, 2.0 * a + 8.0), False
+    else:
+        v = rng.normal(size=(n, 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        p = v * np.asarray(params, dtype=float)[None, :]
+        L = 2.0 * max(params) + 8.0
+        box, periodic = (L, L, L), False
+    p = p + rng.uniform(-jitter, jitter, size=p.shape)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    vel = rng.normal(0.0, 0.1, size=(n, 3))
+    cols = {"x": p[:, 0], "y": p[:, 1], "z": p[:, 2], "nx": d[:, 0], "ny": d[:, 1], "nz": d[:, 2],
+            "vx": vel[:, 0], "vy": vel[:, 1], "vz": vel[:, 2]}
+    return {"n": n, "box": box, "periodic": periodic, "cols": cols}
+
+
 def lj_lattice(nside, rho=0.7, jitter=0.1, T=1.0, seed=3, lz=10.0):
     """Jittered square lattice at number density rho with Maxwell velocities at T."""
     rng = np.random.default_rng(seed)

@@ -258,3 +258,48 @@ pair_param gaussian {{ type_1 = 1; type_2 = 1; A = 2.0 }}
 """)
     assert r.returncode != 0
     assert "not supported" in r.stderr
+
+
+@pytest.mark.gpu
+def test_conf_torus_brownian(gpu, driver, tmp_path):
+    st = sc.surface_disks("torus", 1500, [12.0, 4.0], seed=38, jitter=0.05)
+    Lx, Ly, Lz = st["box"]
+    write_input(tmp_path / "p.dat", st)
+    conf = f"""box fixed {{ lx = {Lx!r}; ly = {Ly!r}; lz = {Lz!r} }}
+input p.dat
+nlist {{ rcut = 2.4; pad = 0.3 }}
+constraint torus {{ c = 12.0; a = 4.0; maxiter = 50 }}
+pair_potential soft {{ k = 1.0; a = 2.0 }}
+pair_align polar {{ J = 0.5; rcut = 2.4 }}
+timestep 0.01
+integrator brownian {{ v0 = 0.2; nu = 0.0; mu = 1.0; seed = 2 }}
+dump fin {{ type = full; start = 80; freq = 80; multi; id; coordinate; keys }}
+run 81
+"""
+    r = run_conf(tmp_path, conf)
+    assert r.returncode == 0, r.stderr
+    o = sc.load(oracle.OracleSystem, st)
+    o.nlist(2.4, 0.3)
+    o.constraint_surface("torus", [12.0, 4.0], max_iter=50)
+    o.pair_soft(1.0, 2.0)
+    o.align_polar(0.5, 2.4)
+    o.set_timestep(0.01)
+    o.integrator_brownian(v0=0.2, nu=0.0, mu=1.0, seed=2)
+    o.run(79)
+    d = read_full(tmp_path / "fin_0000000080.dat")
+    assert np.abs(d[:, 1] - o.get("x")).max() < 1e-6
+    assert np.abs(d[:, 3] - o.get("z")).max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_conf_constraint_tolerance_is_an_integer(gpu, driver, tmp_path):
+    # the reference converts tol with an integer cast: 1e-8 is a conversion error
+    st = sc.surface_disks("ellipsoid", 50, [6.0, 5.0, 4.0], seed=39)
+    write_input(tmp_path / "p.dat", st)
+    L = st["box"][0]
+    r = run_conf(tmp_path, f"""box fixed {{ lx = {L!r}; ly = {L!r}; lz = {L!r} }}
+input p.dat
+constraint ellipsoid {{ a = 6.0; b = 5.0; c = 4.0; tol = 1e-8 }}
+""")
+    assert r.returncode != 0
+    assert 'bad integer value "1e-8" for tol' in r.stderr

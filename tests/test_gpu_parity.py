@@ -292,6 +292,41 @@ def test_langevin_noise_on_same_stream(gpu, flags):
     assert np.array_equal(d.get("age"), o.get("age"))
 
 
+@pytest.mark.parametrize("flags", [0, 1])
+def test_langevin_sphere_noise_on_same_stream(gpu, flags):
+    # off the plane the closing kick has a normal component, which the next
+    # loop-head enforce removes: the fused step must do the same
+    st = sc.sphere_disks(1500, phi=0.8, seed=29)
+    d, o = both(gpu, st)
+    for s in (d, o):
+        s.nlist(2.4, 0.3)
+        s.pair_soft(1.0, 2.0)
+        s.constraint("sphere", st["R"])
+        s.set_timestep(0.005)
+        s.integrator_langevin(gamma=1.0, seed=4, temp=(0, 0.5, 0.5, 1))
+    d.run(99, flags)
+    o.run(99)
+    assert pos_err(positions(d), positions(o), st["box"], False) <= 1e-9
+    assert np.abs(d.get("vz") - o.get("vz")).max() <= 1e-9
+
+
+@pytest.mark.parametrize("flags", [0, 1])
+def test_langevin_linear_temperature_same_stream(gpu, flags):
+    # ValueLinear temperature: the fused step reads T at each step's run step
+    st = sc.lj_lattice(40, rho=0.7, seed=30)
+    d, o = both(gpu, st)
+    for s in (d, o):
+        s.nlist(2.5, 0.3)
+        s.pair_lj(1.0, 1.0, 2.5, shifted=True)
+        s.constraint("plane")
+        s.set_timestep(0.002)
+        s.integrator_langevin(gamma=1.0, seed=6, temp=(1, 0.2, 1.5, 30))
+    d.run(59, flags)
+    o.run(59)
+    assert pos_err(positions(d), positions(o), st["box"], True) <= 1e-9
+    assert np.abs(d.get("vy") - o.get("vy")).max() <= 1e-9
+
+
 @pytest.mark.parametrize("method", ["spv", "bbk"])
 def test_langevin_methods_noise_on(gpu, method):
     st = sc.lj_lattice(30, rho=0.6, seed=19)

@@ -353,6 +353,22 @@ def test_brownian_rotational_variance():
     assert abs(var - expect) < 5 * expect * math.sqrt(2.0 / n)
 
 
+def test_linear_temperature_ramp_variance():
+    # ValueLinear (src/system/value.hpp:140-172): T(step) = min + step (max - min) / steps,
+    # max beyond; Brownian noise sqrt(2 mu T dt) per step, so after run steps
+    # 0..99 of a 0 -> 1 ramp over 100 steps the variance per axis is
+    # 2 mu dt sum_s s/100 = 0.99 (an off-by-one step gives 1.01, 6 sigma away)
+    n = 100000
+    s = oracle.OracleSystem(n, (1e6, 1e6, 10.0), True)
+    s.set("nx", np.ones(n))
+    s.constraint("plane")
+    s.set_timestep(0.01)
+    s.integrator_brownian(v0=0.0, nu=0.0, mu=1.0, seed=9, temp=(1, 0.0, 1.0, 100))
+    s.run(99)
+    var = 0.5 * float(np.mean(s.get("x") ** 2 + s.get("y") ** 2))
+    assert var == pytest.approx(0.99, abs=3.5 * 0.99 * math.sqrt(1.0 / n))
+
+
 def test_nve_free_flight():
     # no forces: x(t) = x0 + t v exactly step by step (integrator_nve.cpp:47-83)
     n = 5
