@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(kBlock) k_interact(int mode, Fields f, NListD 
           const float ff = 48.0f * (float)eps * s6 * (s6 - 0.5f) * s2;
           fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy); fz -= (double)(ff * (float)dz);
         } else {
-          const double inv_r_sq = sig_sq / r_sq;
+          const double inv_r_sq = lj_inv_r_sq(sig_sq, r_sq);
           const double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
           if (EN) {
             double e = 4.0 * eps * inv_r_6 * (inv_r_6 - 1.0);

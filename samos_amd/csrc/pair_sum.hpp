@@ -17,10 +17,17 @@
 
 namespace samos {
 
-template <int POT, bool F32, bool PLANAR, bool MT, int kBatch = 4>
-__device__ __forceinline__ void pair_force_sum(const Fields& f, const NListD& nl, const BoxD& b, const InteractD& ip,
-                                               int s, double xi, double yi, double zi, int ti, double ri,
-                                               double& fx, double& fy, double& fz) {
+// wrap_i: apply the minimum image to each pair.  A slot farther than
+// 2 (rcut + pad) from every periodic face has all its listed neighbours
+// (within rcut + pad at the build, each moved by less than pad / 2 since)
+// inside the box at their unwrapped offsets, so for it the image step is an
+// exact no-op and is skipped (a per-lane test inside one loop: whole
+// wavefronts away from the faces branch over it).  PLANAR pairs have dz = 0
+// and no z image.
+template <int POT, bool F32, bool PLANAR, bool MT, int kBatch>
+__device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& nl, const BoxD& b, const InteractD& ip,
+                                                int s, double xi, double yi, double zi, int ti, double ri,
+                                                double& fx, double& fy, double& fz, bool wrap_i) {
   const bool radii = ((POT & POT_SOFT) && ip.soft_radii) || ((POT & POT_LJ) && ip.lj_radii);
   // One particle type: the table entries of the pair (1, 1) are loop constants,
   // and so are the first products of each expression (sigma^2, rcut^2,
@@ -57,8 +64,11 @@ __device__ __forceinline__ void pair_force_sum(const Fields& f, const NListD& nl
     for (int u = 0; u < kBatch; ++u) {
       if (k0 + u >= kn) break;
       // d = x_j - x_i; LJ's reference convention x_i - x_j is -d
-      double dx = xj[u] - xi, dy = yj[u] - yi, dz = zj[u] - zi;
-      min_image(b, dx, dy, dz);
+      double dx = xj[u] - xi, dy = yj[u] - yi, dz = PLANAR ? 0.0 : zj[u] - zi;
+      if (wrap_i) {
+        if (PLANAR) min_image_xy(b, dx, dy);
+        else min_image(b, dx, dy, dz);
+      }
       // planar: dz = 0, and adding +0 to a sum of squares changes no bit
       const double r_sq = PLANAR ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
       if (POT & POT_SOFT) {
@@ -100,7 +110,7 @@ __device__ __forceinline__ void pair_force_sum(const Fields& f, const NListD& nl
             fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy);
             if (!PLANAR) fz -= (double)(ff * (float)dz);
           } else {
-            const double inv_r_sq = sig2 / r_sq;
+            const double inv_r_sq = lj_inv_r_sq(sig2, r_sq);
             const double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
             const double ff = e48 * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
             fx -= ff * dx; fy -= ff * dy;
@@ -110,6 +120,20 @@ __device__ __forceinline__ void pair_force_sum(const Fields& f, const NListD& nl
       }
     }
   }
+}
+
+template <int POT, bool F32, bool PLANAR, bool MT, int kBatch = 4>
+__device__ __forceinline__ void pair_force_sum(const Fields& f, const NListD& nl, const BoxD& b, const InteractD& ip,
+                                               int s, double xi, double yi, double zi, int ti, double ri,
+                                               double& fx, double& fy, double& fz) {
+  bool wrap = b.periodic != 0;
+  if (wrap) {
+    const double R = 2.0 * sqrt(nl.cut2);
+    bool inner = xi - b.xlo > R && b.xhi - xi > R && yi - b.ylo > R && b.yhi - yi > R;
+    if (!PLANAR) inner = inner && zi - b.zlo > R && b.zhi - zi > R;
+    wrap = !inner;
+  }
+  pair_force_loop<POT, F32, PLANAR, MT, kBatch>(f, nl, b, ip, s, xi, yi, zi, ti, ri, fx, fy, fz, wrap);
 }
 
 }  // namespace samos

@@ -28,6 +28,26 @@ __device__ __forceinline__ void min_image(const BoxD& b, double& dx, double& dy,
   }
 }
 
+// sig2 / r_sq for the LJ force factor without the FP64 division sequence:
+// the hardware reciprocal refined by two Newton-Raphson steps (each doubles
+// the correct bits; the result is within one rounding of 1 / r_sq), then one
+// multiply.  Differs from the correctly rounded quotient by at most about one
+// unit in the last place.
+__device__ __forceinline__ double lj_inv_r_sq(double sig2, double r_sq) {
+  double y = __builtin_amdgcn_rcp(r_sq);
+  double e = fma(-r_sq, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-r_sq, y, 1.0);
+  y = fma(y, e, y);
+  return sig2 * y;
+}
+
+// The x and y steps of min_image, for pairs with dz = 0.
+__device__ __forceinline__ void min_image_xy(const BoxD& b, double& dx, double& dy) {
+  if (dx > b.xhi) dx -= b.Lx; else if (dx < b.xlo) dx += b.Lx;
+  if (dy > b.yhi) dy -= b.Ly; else if (dy < b.ylo) dy += b.Ly;
+}
+
 // Particle columns (slot order).  Passed by value into every kernel.
 struct Fields {
   double *x, *y, *z, *vx, *vy, *vz, *nx, *ny, *nz;

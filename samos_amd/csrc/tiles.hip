@@ -426,7 +426,7 @@ __device__ __forceinline__ void pair_term(const PairConst& pc, double dx, double
         const float ff = 48.0f * (float)pc.lj_eps * s6 * (s6 - 0.5f) * s2;
         fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy);
       } else {
-        const double inv_r_sq = pc.lj_sig2 / r_sq;
+        const double inv_r_sq = lj_inv_r_sq(pc.lj_sig2, r_sq);
         const double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
         const double ff = 48.0 * pc.lj_eps * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
         fx -= ff * dx; fy -= ff * dy;
