@@ -461,6 +461,7 @@ def bench_tiled_line(samos_amd, sc, args):
     halos, stencil-bitmask lists): whole-step rate and its step kernel."""
     s, st = build_system(samos_amd, sc, args.nside, args.pad, args.rebuild, args.batch)
     s.set_option("tile_cells", args.tile_cells)
+    s.set_option("chunk_slots", args.chunk_slots)
     n = st["n"]
     flags = samos_amd.RUN_TILES
     timed_run(s, max(args.warmup, 20), flags)
@@ -498,6 +499,8 @@ def main():
     ap.add_argument("--pad", type=float, default=0.5, help="neighbour-list skin")
     ap.add_argument("--rebuild", choices=["coop", "stages"], default="stages")
     ap.add_argument("--batch", type=int, choices=[4, 8], default=4)
+    ap.add_argument("--chunk-slots", type=int, default=32,
+                    help="fused steps per captured chunk (each chunk ends in one conditional rebuild)")
     ap.add_argument("--tiles", action="store_true",
                     help="headline line on the cell-tiled engine instead of the neighbour-list kernels")
     ap.add_argument("--no-tiled-line", action="store_true", help="skip the cell-tiled engine's extra line")
@@ -550,6 +553,7 @@ def main():
 
     s, st = build_system(samos_amd, sc, args.nside, args.pad, args.rebuild, args.batch)
     s.set_option("tile_cells", args.tile_cells)
+    s.set_option("chunk_slots", args.chunk_slots)
     n = st["n"]
     print("bench: N=%d box %.3f, warmup %d, steps %d" % (n, st["box"][0], args.warmup, args.steps),
           file=sys.stderr, flush=True)
@@ -604,8 +608,10 @@ def main():
             "nlist_pad": args.pad,
             "rebuild": args.rebuild,
             "gather_batch": args.batch,
+            "chunk_slots": args.chunk_slots,
             "schedule": "fused: one force evaluation per step (the loop-head re-evaluation of identical "
-                        "positions is not repeated)",
+                        "positions is not repeated); captured chunks of fused steps, each ending in one "
+                        "device-conditional rebuild",
         },
         "roofline": {
             "bound": "hbm",

@@ -208,10 +208,12 @@ int samos_profile_read(samos_ctx* ctx, const char* kernel_class, double* ms_tota
 int samos_sync(samos_ctx* ctx);
 /* Tuning switches (no counterpart in the reference; results do not depend on
  * them beyond summation order): "rebuild_coop" 1 = one cooperative launch per
- * neighbour-list rebuild check, 0 = five stage kernels replayed in graphs
+ * neighbour-list rebuild check, 0 = stage kernels replayed in graphs
  * (default); "gather_batch" 4 or 8 = neighbour gathers in flight per thread in
- * the fused step; "tile_cells" 0 = automatic, 1..80 = cells per workgroup tile
- * of the cell-tiled planar engine. */
+ * the fused step; "chunk_slots" (even, 2..256, default 32) = fused Langevin
+ * steps per captured chunk, each chunk ending in one device-conditional
+ * rebuild; "tile_cells" 0 = automatic, 1..80 = cells per workgroup tile of the
+ * cell-tiled planar engine. */
 int samos_set_option(samos_ctx* ctx, const char* name, double value);
 /* Cell-tiled planar engine (no counterpart in the reference): planar fused
  * Langevin runs keep the state in cell order and the Verlet list as stencil

@@ -76,6 +76,8 @@ struct samos_ctx {
   ConstraintD con{-1, 1.0, 0.0, 0, 0.0, 0.0, 0.0, 100, 1e-6};
   std::map<std::string, Integ> integs;
   ClockD* d_clock = nullptr;
+  ChunkCtl* d_chunk = nullptr;  // control words of the captured step chunks
+  int chunk_slots = 32;         // fused steps per captured chunk (even)
   int time_step = 0;
   double* d_en = nullptr;       // 5 * np per-slot energies: soft, lj, polar, nematic, further pair potentials
   double* d_cols = nullptr;     // 3 * np FIRE columns
@@ -442,7 +444,7 @@ int upload_clock(samos_ctx* ctx, int run_step) {
 // (t % 2), followed by the device-conditional rebuild on the new positions:
 // one cooperative launch when available (coop), else the five stage kernels.
 int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int rel, int t_off, int steps, bool f32, bool planar,
-               double mass_u, bool coop) {
+               double mass_u, bool coop, ChunkCtl* chunk = nullptr, int slot = 0) {
   const int par = rel & 1;
   Fields cur = fields(ctx, par), nxt = fields(ctx, par ^ 1);
   if (planar) { cur.xy = ctx->xy[par]; nxt.xy = ctx->xy[par ^ 1]; }
@@ -458,10 +460,12 @@ int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int rel, int t_off, i
     fa.Bc = I.temp.kind == SAMOS_VALUE_CONSTANT ? std::sqrt(I.temp.tmin * (1.0 - std::exp(-2.0 * I.gamma * I.dt))) : -1.0;
     fa.batch = ctx->gather_batch;
     fa.age_uniform = ctx->age_uniform ? 1 : 0;
+    fa.chunk = chunk;
+    fa.slot = slot;
     launch_baoab_fused(ctx->st, pot_mask(ctx), phase, cur, nxt, ctx->nl, ctx->box, ctx->con, interact_params(ctx),
                        ctx->n, fa, f32 ? 1 : 0, planar ? 1 : 0, ctx->ntypes > 1 ? 1 : 0);
   }
-  if (phase != 2) {
+  if (phase != 2 && !chunk) {
     ProfScope ps(ctx, "rebuild");
     if (coop && ctx->coop != 0) {
       hipError_t e = launch_rebuild_coop(ctx->st, nxt, ctx->nl, ctx->box, ctx->n, ctx->d_flags + par, planar ? 1 : 0);
@@ -530,33 +534,68 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   if (int r = fused_step(ctx, I, 0, a, 0, 0, steps, f32, planar, mass_u, true)) return r;
   launch_clock_advance(ctx->st, ctx->d_clock, 1);
   int t = a + 1;
-  // Without cooperative launches every skipped rebuild costs five dispatches;
-  // then chunks of steps are captured once and replayed as a graph.
-  const int K = 64;  // steps per captured chunk (even, so buffer parity repeats)
+  int buf = 1;  // buffer holding x_t
+  // Captured chunks: M fused steps ("slots") followed by one device-conditional
+  // rebuild.  A slot whose new positions need a rebuild stops the chunk: the
+  // later slots return at once, the positions settle into buffer 0, the
+  // rebuild runs, and the next chunk resumes at the following step.  The
+  // rebuild decisions and the steps are exactly those of one launch per step;
+  // only the skipped-rebuild dispatches between steps are gone.  Slots cover
+  // iterations up to b - 2 (the range's last iteration has its own loop-head
+  // flag and is launched directly).
+  const int M = ctx->chunk_slots;
   const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
-  if (graph && b - t >= 2 * K) {
+  if (graph && b - 1 - t >= 2 * M) {
+    Fields f0 = fields(ctx, 0), f1 = fields(ctx, 1);
+    if (planar) { f0.xy = ctx->xy[0]; f1.xy = ctx->xy[1]; }
+    // slots start at buffer parity 0
+    if (planar) {
+      HIPCHK(hipMemcpyAsync(ctx->xy[0], ctx->xy[1], sizeof(double2) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
+    } else {
+      for (int k = 0; k < 3; ++k)
+        HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->alt[k], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
+    }
+    buf = 0;
+    launch_chunk_init(ctx->st, ctx->d_chunk, t, b - 1);
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < K; ++j) fused_step(ctx, I, 1, t + j, t + j - a, j, steps, f32, planar, mass_u, false);
-    launch_clock_advance(ctx->st, ctx->d_clock, K);
+    // head_enforce of every slot: its iteration t' <= b - 2 < steps
+    for (int j = 0; j < M; ++j) fused_step(ctx, I, 1, b - 2, j, j, steps, f32, planar, mass_u, false, ctx->d_chunk, j);
+    launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, ctx->n, planar ? 1 : 0);
+    launch_rebuild(ctx->st, f0, ctx->nl, ctx->box, ctx->n, &ctx->d_chunk->rebuild, planar ? 1 : 0);
+    launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
     HIPCHK(hipStreamEndCapture(ctx->st, &g));
     HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-    // chunks stop before the range's last iteration, whose loop-head flag differs
-    while (b - t > K) {
-      HIPCHK(hipGraphLaunch(ge, ctx->st));
-      t += K;
+    // Every chunk runs at least its first slot, so t grows with each launch;
+    // launches are issued in batches sized by the slots per chunk seen so far.
+    double per = 0.5 * M;
+    int* h_base = reinterpret_cast<int*>(ctx->h_out);
+    while (t < b - 1) {
+      const int k = std::max(1, std::min(256, (int)((b - 1 - t) / per)));
+      for (int i = 0; i < k; ++i) HIPCHK(hipGraphLaunch(ge, ctx->st));
+      HIPCHK(hipMemcpyAsync(h_base, &ctx->d_chunk->base, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+      HIPCHK(hipStreamSynchronize(ctx->st));
+      const int tn = *h_base;
+      if (tn <= t || tn > b - 1) {
+        hipGraphExecDestroy(ge);
+        hipGraphDestroy(g);
+        return fail(ctx, "captured step chunks made no progress");
+      }
+      per = std::max(1.0, (double)(tn - t) / k);
+      t = tn;
     }
-    HIPCHK(hipStreamSynchronize(ctx->st));
     HIPCHK(hipGraphExecDestroy(ge));
     HIPCHK(hipGraphDestroy(g));
+    // the last chunk served any request; the direct steps below use the parity flags
+    HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   }
   const int t0 = t;
   for (; t < b; ++t)
-    if (int r = fused_step(ctx, I, 1, t, t - a, t - t0, steps, f32, planar, mass_u, true)) return r;
+    if (int r = fused_step(ctx, I, 1, t, buf + (t - t0), t - t0, steps, f32, planar, mass_u, true)) return r;
   launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
-  if (int r = fused_step(ctx, I, 2, b, b - a, 0, steps, f32, planar, mass_u, true)) return r;
-  const int pf = (b - a) & 1;  // buffer holding x_b
+  if (int r = fused_step(ctx, I, 2, b, buf + (b - t0), 0, steps, f32, planar, mass_u, true)) return r;
+  const int pf = (buf + (b - t0)) & 1;  // buffer holding x_b
   if (planar) {
     Fields f0 = fields(ctx, 0);
     f0.xy = ctx->xy[pf];
@@ -810,6 +849,7 @@ int samos_create(samos_ctx** out, int device, int n, double lx, double ly, doubl
   ctx->nl.flags = ctx->d_flags;
   if (hipMalloc(&ctx->d_tab, 4 * kTab * sizeof(double)) != hipSuccess) return bad("tables");
   if (hipMalloc(&ctx->d_clock, sizeof(ClockD)) != hipSuccess) return bad("clock");
+  if (hipMalloc(&ctx->d_chunk, sizeof(ChunkCtl)) != hipSuccess) return bad("chunk control");
   if (hipMalloc(&ctx->d_en, 5 * np * sizeof(double)) != hipSuccess) return bad("energies");
   if (hipMalloc(&ctx->d_cols, 3 * np * sizeof(double)) != hipSuccess) return bad("fire columns");
   if (hipMalloc(&ctx->d_partial, 8 * (size_t)reduce_blocks(ctx->np) * sizeof(double)) != hipSuccess) return bad("partials");
@@ -855,7 +895,7 @@ int samos_destroy(samos_ctx* ctx) {
   for (int k = 0; k < kNumIntFields; ++k) hipFree(ctx->icol[k]);
   for (int k = 0; k < 3; ++k) hipFree(ctx->alt[k]);
   for (int k = 0; k < 2; ++k) hipFree(ctx->xy[k]);
-  hipFree(ctx->d_flags); hipFree(ctx->d_nnb); hipFree(ctx->d_tab); hipFree(ctx->d_clock); hipFree(ctx->d_en); hipFree(ctx->d_cols);
+  hipFree(ctx->d_flags); hipFree(ctx->d_nnb); hipFree(ctx->d_tab); hipFree(ctx->d_clock); hipFree(ctx->d_chunk); hipFree(ctx->d_en); hipFree(ctx->d_cols);
   hipFree(ctx->d_partial); hipFree(ctx->d_out); hipFree(ctx->d_counter);
   if (ctx->h_out) hipHostFree(ctx->h_out);
   hipFree(ctx->keys); hipFree(ctx->keys2); hipFree(ctx->perm); hipFree(ctx->perm2);
@@ -1527,6 +1567,10 @@ int samos_set_option(samos_ctx* ctx, const char* name, double value) {
   } else if (k == "gather_batch") {
     if (value != 4.0 && value != 8.0) return fail(ctx, "gather_batch must be 4 or 8");
     ctx->gather_batch = (int)value;
+  } else if (k == "chunk_slots") {
+    if (value < 2.0 || value > 256.0 || value != std::floor(value) || ((int)value & 1))
+      return fail(ctx, "chunk_slots must be an even integer in 2..256");
+    ctx->chunk_slots = (int)value;
   } else if (k == "tile_cells") {
     if (value < 0.0 || value > kTileMaxCells || value != std::floor(value))
       return fail(ctx, "tile_cells must be 0 (automatic) or an integer in 1.." + std::to_string(kTileMaxCells));

@@ -17,6 +17,8 @@
 // rotate_director) become inline device functions.  The noise stream is the
 // counter-based Philox stream of samos_device.hpp, one independent draw block
 // per (particle id, integrator call).
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "pair_sum.hpp"
 #include "constraint.hpp"
@@ -335,8 +337,21 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 6 : 1) k_baoab_fused
   const uint32_t seed = fa.seed;
   const ClockD* clk = fa.clock;
   const int slot = fa.clock_slot, t_off = fa.t_off, parity = fa.parity, head_enforce = fa.head_enforce;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && PHASE != 2) nl.flags[1 - parity] = 0;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
+  ChunkCtl* const ck = fa.chunk;
+  if (ck) {
+    // Every wavefront reads the same words, written by earlier kernels only:
+    // the whole grid either returns here or runs the step.
+    const int j = fa.slot;
+    if (ck->halt | ck->req[(j + 1) & 1]) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) ck->halt = 1;
+      return;
+    }
+    if (ck->base + j >= ck->limit) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ck->last = j + 1;
+  } else if (blockIdx.x == 0 && threadIdx.x == 0 && PHASE != 2) {
+    nl.flags[1 - parity] = 0;
+  }
   bool need = false;
   if (s < n) {
     double xi, yi, zi;
@@ -426,7 +441,38 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 6 : 1) k_baoab_fused
     }
   }
   if (PHASE != 2) {
-    if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomicOr(&nl.flags[parity], 1);
+    if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomicOr(ck ? &ck->req[fa.slot & 1] : &nl.flags[parity], 1);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_chunk_settle(ChunkCtl* ck, Fields f0, Fields f1, int n, int planar) {
+  const int last = ck->last;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ck->rebuild = ck->halt | ck->req[0] | ck->req[1];
+  if (!(last & 1)) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (planar) {
+      f0.xy[i] = f1.xy[i];
+    } else {
+      f0.x[i] = f1.x[i]; f0.y[i] = f1.y[i]; f0.z[i] = f1.z[i];
+    }
+  }
+}
+
+__global__ void k_chunk_finish(ChunkCtl* ck, ClockD* clk) {
+  const int last = ck->last;
+  __syncthreads();
+  if (threadIdx.x < 8) clk->calls[threadIdx.x] += (unsigned long long)last;
+  if (threadIdx.x == 0) {
+    clk->run_step += last;
+    ck->base += last;
+    ck->halt = 0; ck->req[0] = 0; ck->req[1] = 0; ck->last = 0; ck->rebuild = 0;
+  }
+}
+
+__global__ void k_chunk_init(ChunkCtl* ck, int base, int limit) {
+  if (threadIdx.x == 0) {
+    ck->halt = 0; ck->req[0] = 0; ck->req[1] = 0; ck->last = 0; ck->rebuild = 0;
+    ck->base = base; ck->limit = limit;
   }
 }
 
@@ -545,6 +591,19 @@ void launch_nve_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, do
 
 void launch_fire_mix(hipStream_t st, Fields f, int n, double f1, double f2, int zero_v) {
   hipLaunchKernelGGL(k_fire_mix, dim3(grid1(n)), dim3(kBlock), 0, st, f, n, f1, f2, zero_v);
+}
+
+void launch_chunk_settle(hipStream_t st, ChunkCtl* ck, Fields f0, Fields f1, int n, int planar) {
+  const int g = std::max(1, std::min(grid1(n), 1024));
+  hipLaunchKernelGGL(k_chunk_settle, dim3(g), dim3(kBlock), 0, st, ck, f0, f1, n, planar);
+}
+
+void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock) {
+  hipLaunchKernelGGL(k_chunk_finish, dim3(1), dim3(64), 0, st, ck, clock);
+}
+
+void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit) {
+  hipLaunchKernelGGL(k_chunk_init, dim3(1), dim3(64), 0, st, ck, base, limit);
 }
 
 void launch_clock_advance(hipStream_t st, ClockD* clock, int nsteps) {

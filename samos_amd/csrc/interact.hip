@@ -20,7 +20,7 @@ namespace {
 template <int POT, int AL, bool EN, bool F32>
 __global__ void __launch_bounds__(kBlock) k_interact(int mode, Fields f, NListD nl, BoxD b, InteractD ip, int n,
                                                      double* e_out) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const double xi = f.x[s], yi = f.y[s], zi = f.z[s];
   const int ti = f.type[s] - 1;
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(kBlock) k_interact(int mode, Fields f, NListD 
 // soft_attractive recounts the coordination (neighbours closer than r_f).
 __global__ void __launch_bounds__(kBlock) k_interact_extra(Fields f, NListD nl, BoxD b, ExtraD ex, int n,
                                                            double* e_col, int recount, int add_forces) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const double xi = f.x[s], yi = f.y[s], zi = f.z[s], ai = f.radius[s];
   double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;

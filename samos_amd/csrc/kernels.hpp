@@ -95,6 +95,8 @@ struct FusedArgs {
   double Bc;       // O-step noise amplitude at constant temperature, or -1
   int batch;       // neighbour gathers in flight per thread (4 or 8)
   int age_uniform; // every particle has the same age: the host advances it once
+  ChunkCtl* chunk = nullptr;  // captured-chunk mode (null: direct launch with the parity flags)
+  int slot = 0;               // slot index inside the chunk
 };
 void launch_baoab_fused(hipStream_t st, int pot_mask, int phase, Fields cur, Fields nxt, NListD nl, BoxD box,
                         ConstraintD con, InteractD ip, int n, const FusedArgs& fa, int fp32, int planar,
@@ -103,6 +105,13 @@ void launch_baoab_fused(hipStream_t st, int pot_mask, int phase, Fields cur, Fie
 // phases); returns the launch status so the caller can fall back to
 // launch_rebuild where cooperative launches are unavailable.
 hipError_t launch_rebuild_coop(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req, int planar);
+// Captured-chunk bookkeeping (see ChunkCtl): the settle step after the slots
+// copies positions left in buffer 1 back to buffer 0 and forms the rebuild
+// request; the finish step advances the clock and the chunk base by the slots
+// that ran and clears the control words; init sets base and limit.
+void launch_chunk_settle(hipStream_t st, ChunkCtl* ck, Fields f0, Fields f1, int n, int planar);
+void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock);
+void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit);
 // Interleaved (x, y) copy of the positions, read by the planar kernels.
 void launch_pack_xy(hipStream_t st, Fields f, int n);
 // Back from the interleaved column into the x, y columns.

@@ -172,7 +172,7 @@ __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
   bool overflow = false;
   int kmax = 0;
   constexpr int U = 4;
-  for (int s = gtid(); s < n; s += gsize()) {
+  for (int s = xcd_block() * blockDim.x + threadIdx.x; s < n; s += gsize()) {
     double xi, yi, zi;
     if (PLANAR) { const double2 p = f.xy[s]; xi = p.x; yi = p.y; zi = f.z[s]; }
     else { xi = f.x[s]; yi = f.y[s]; zi = f.z[s]; }
@@ -180,6 +180,17 @@ __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
     const int c = nl.cell_of[s];
     const int ci = c / (nl.ncy * nl.ncz), cj = (c / nl.ncz) % nl.ncy, ck = c % nl.ncz;
     int k = 0, coord = 0;
+    // Candidates come from the 3 x 3 (x 3) cells around the particle's own,
+    // all within two cell widths of it per axis.  Farther than that from every
+    // periodic face, none of them is across a face, the raw difference is the
+    // minimum image and the image step is skipped (an exact no-op there).
+    bool wrap = b.periodic != 0;
+    if (wrap) {
+      const double Rx = 2.0 * nl.wx, Ry = 2.0 * nl.wy, Rz = 2.0 * nl.wz;
+      bool inner = xi - b.xlo > Rx && b.xhi - xi > Rx && yi - b.ylo > Ry && b.yhi - yi > Ry;
+      if (!PLANAR) inner = inner && zi - b.zlo > Rz && b.zhi - zi > Rz;
+      wrap = !inner;
+    }
     // Planar: every particle is in the same z-layer ck, and a used cell grid
     // has ncz >= 2, so the stencil's other z-layers hold nobody and the own
     // layer appears exactly once; only it is visited.
@@ -210,9 +221,13 @@ __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
             for (int u = 0; u < U; ++u) {
               if (m0 + u >= e) break;
               if (t[u] == s) continue;
-              double dx = xi - xj[u], dy = yi - yj[u], dz = zi - zj[u];
-              min_image(b, dx, dy, dz);
-              const double d2 = dx * dx + dy * dy + dz * dz;
+              double dx = xi - xj[u], dy = yi - yj[u], dz = PLANAR ? 0.0 : zi - zj[u];
+              if (wrap) {
+                if (PLANAR) min_image_xy(b, dx, dy);
+                else min_image(b, dx, dy, dz);
+              }
+              // planar: dz = 0, and adding +0 to a sum of squares changes no bit
+              const double d2 = PLANAR ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
               if (d2 < nl.cut2) {
                 if (k < nl.stride) nl.nbr[(long)k * nl.np + s] = t[u];
                 ++k;

@@ -42,6 +42,15 @@ __device__ __forceinline__ double lj_inv_r_sq(double sig2, double r_sq) {
   return sig2 * y;
 }
 
+// Workgroup b of a launch runs on XCD b % 8, and each XCD has its own L2.
+// This maps the launch's workgroups so that each XCD takes one contiguous
+// range of logical blocks (a bijection on [0, gridDim.x)): slots that are
+// neighbours in memory, and mostly in space, then share one L2.
+__device__ __forceinline__ int xcd_block() {
+  const int G = gridDim.x, q = G >> 3, r = G & 7, x = blockIdx.x & 7;
+  return x * q + min(x, r) + (blockIdx.x >> 3);
+}
+
 // The x and y steps of min_image, for pairs with dz = 0.
 __device__ __forceinline__ void min_image_xy(const BoxD& b, double& dx, double& dy) {
   if (dx > b.xhi) dx -= b.Lx; else if (dx < b.xlo) dx += b.Lx;
@@ -134,6 +143,21 @@ struct TempD {
 
 // Per-step clock in device memory, advanced after each captured chunk of steps
 // so that replayed graphs draw fresh noise.
+// Control words of a captured chunk of fused Langevin steps ("slots").  A slot
+// whose new positions need a rebuild sets req[slot & 1]; the next slot sees it
+// and every later slot of the chunk then does nothing.  After the slots, the
+// chunk settles the positions into buffer 0, rebuilds if requested and
+// advances the clock by the number of slots that ran.
+struct ChunkCtl {
+  int halt;      // set once a slot has seen its predecessor's request
+  int req[2];    // rebuild request of the slot with this parity
+  int last;      // slots executed in this chunk
+  int base;      // run step of slot 0
+  int limit;     // slot j runs only while base + j < limit
+  int rebuild;   // rebuild request read by the chunk's rebuild stages
+  int pad_;
+};
+
 struct ClockD {
   unsigned long long calls[8];   // per-integrator call counters
   int run_step;                  // run-loop index t

@@ -327,6 +327,29 @@ def test_langevin_linear_temperature_same_stream(gpu, flags):
     assert np.abs(d.get("vy") - o.get("vy")).max() <= 1e-9
 
 
+@pytest.mark.parametrize("slots", [2, 6, 24])
+def test_chunked_fused_steps_equal_direct_launches(gpu, slots):
+    # captured chunks stop at the step whose positions need a rebuild and
+    # resume after it: same steps, same rebuilds, bit for bit
+    st = sc.lj_lattice(48, rho=0.7, seed=31)
+    out = []
+    for flags in (0, gpu.RUN_NO_GRAPH):
+        d = sc.load(gpu.System, st)
+        d.nlist(2.5, 0.3)
+        d.pair_lj(1.0, 1.0, 2.5, shifted=True)
+        d.constraint("plane")
+        d.set_timestep(0.004)
+        d.integrator_langevin(gamma=1.0, seed=8, temp=(0, 1.5, 1.5, 1))
+        d.set_option("chunk_slots", slots)
+        b0 = d.nlist_info()["builds"]
+        d.run(399, flags)
+        out.append(({k: d.get(k) for k in ("x", "y", "vx", "vy", "ix", "iy")}, d.pairs(), d.nlist_info()["builds"] - b0))
+    (a, pa, ba), (c, pc, bc) = out
+    assert all(np.array_equal(a[k], c[k]) for k in a)
+    assert np.array_equal(pa, pc)
+    assert ba == bc and ba > 10
+
+
 @pytest.mark.parametrize("method", ["spv", "bbk"])
 def test_langevin_methods_noise_on(gpu, method):
     st = sc.lj_lattice(30, rho=0.6, seed=19)
