@@ -579,12 +579,16 @@ def main():
     tiles = s.tile_info()
 
     extra = {}
-    if args.variants:
+    if not args.tiles:
+        # the FP32-force variant of the same workload (pair arithmetic in FP32,
+        # sums, positions and the integrator in FP64)
         s32, _ = build_system(samos_amd, sc, args.nside, args.pad, args.rebuild, args.batch)
-        timed_run(s32, 50, samos_amd.RUN_FP32_FORCE)
+        s32.set_option("chunk_slots", args.chunk_slots)
+        timed_run(s32, args.warmup, samos_amd.RUN_FP32_FORCE)
         t32 = timed_run(s32, args.steps, samos_amd.RUN_FP32_FORCE)
         extra["fp32_force_value"] = n * args.steps / t32
         s32.close()
+    if args.variants:
         tf = timed_run(s, min(args.steps, 500), samos_amd.RUN_FAITHFUL)
         extra["faithful_two_force_evals_value"] = n * min(args.steps, 500) / tf
 

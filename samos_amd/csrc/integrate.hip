@@ -330,7 +330,7 @@ __global__ void k_clock_advance(ClockD* clk, int nsteps) {
 // exp(-2 gamma dt))) come from the host (the same libm the reference uses);
 // Bc < 0 selects the per-step schedule value instead.
 template <int POT, int PHASE, bool F32, bool PLANAR, bool MT, int BATCH>
-__global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 6 : 1) k_baoab_fused(Fields cur, Fields nxt, NListD nl, BoxD b, ConstraintD c,
+__global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused(Fields cur, Fields nxt, NListD nl, BoxD b, ConstraintD c,
                                                         InteractD ip, int n, FusedArgs fa) {
   const double dt = fa.dt, gamma = fa.gamma, mass_u = fa.mass_u, exp_dt = fa.exp_dt, Bc = fa.Bc;
   const TempD temp = fa.temp;
