@@ -51,6 +51,7 @@ struct samos_ctx {
   double* dcol[kNumDoubleFields] = {};
   int* icol[kNumIntFields] = {};      // type, ix, iy, iz, coord, id
   double* alt[3] = {};                // second position buffer for fused steps
+  double* altn[3] = {};               // second director buffer for fused Brownian steps
   double2* xy[2] = {};                // interleaved (x, y) of the two position buffers
   std::vector<int> h_id;              // slot -> id
   double dt = 0.0;
@@ -520,8 +521,9 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   if (int r = upload_clock(ctx, a)) return r;
   HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   // Planar layout: the plane constraint puts every particle on z = zpos, so
-  // neighbour gathers read the interleaved (x, y) column.
-  const bool planar = ctx->con.kind == 1;
+  // neighbour gathers read the interleaved (x, y) column.  The N^2 build reads
+  // the x, y, z columns, so lists without a cell grid keep the 3-D layout.
+  const bool planar = ctx->con.kind == 1 && ctx->nl.use_cell;
   // the loop-head enforce of iteration a was applied by the run's begin or by
   // the last kernel of the previous range
   if (planar) {
@@ -562,7 +564,7 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
     // head_enforce of every slot: its iteration t' <= b - 2 < steps
     for (int j = 0; j < M; ++j) fused_step(ctx, I, 1, b - 2, j, j, steps, f32, planar, mass_u, false, ctx->d_chunk, j);
-    launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, ctx->n, planar ? 1 : 0);
+    launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, ctx->n, planar ? 1 : 0, 0);
     launch_rebuild(ctx->st, f0, ctx->nl, ctx->box, ctx->n, &ctx->d_chunk->rebuild, planar ? 1 : 0);
     launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
     HIPCHK(hipStreamEndCapture(ctx->st, &g));
@@ -617,6 +619,112 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   return 0;
 }
 
+
+// ---- fused Brownian run ------------------------------------------------------
+// One IntegratorBrownian ("brownian") with pair potentials and at most one
+// aligner: each launch evaluates the loop-head forces and torques of step t
+// and integrates it (k_brownian_fused), followed by the device-conditional
+// rebuild; chunks are captured as for the fused Langevin run.  On entry the
+// primary columns hold the loop-head state of a; on return that of b.
+bool fused_brownian_eligible(samos_ctx* ctx, int flags) {
+  if ((flags & SAMOS_RUN_FAITHFUL) || ctx->extra.count != 0 || ctx->integs.size() != 1) return false;
+  const Integ& I = ctx->integs.begin()->second;
+  return I.name == "brownian" && needs_nlist(ctx) && brownian_fused_supported(pot_mask(ctx), al_mask(ctx));
+}
+
+int run_fused_brownian(samos_ctx* ctx, Integ& I, int a, int b, int steps, int flags) {
+  const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
+  const bool planar = ctx->con.kind == 1 && ctx->nl.use_cell;
+  const int n = ctx->n;
+  auto buffer = [&](int par) {
+    Fields f = fields(ctx, par);
+    if (par) { f.nx = ctx->altn[0]; f.ny = ctx->altn[1]; f.nz = ctx->altn[2]; }
+    if (planar) f.xy = ctx->xy[par];
+    return f;
+  };
+  if (int r = upload_clock(ctx, a)) return r;
+  HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
+  if (planar) {
+    Fields f0 = buffer(0);
+    launch_pack_xy(ctx->st, f0, n);
+    // the binning of either buffer reads z (= zpos after the enforce)
+    HIPCHK(hipMemcpyAsync(ctx->alt[2], ctx->dcol[2], sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
+  }
+  auto step = [&](int t, int par, int t_off, ChunkCtl* chunk, int slot) {
+    Fields cur = buffer(par), nxt = buffer(par ^ 1);
+    BrownArgs ba;
+    ba.dt = I.dt; ba.v0 = I.v0; ba.mu = I.mu; ba.mur = I.mur; ba.stoch = I.stoch; ba.flip_p = I.flip_p;
+    ba.nematic = I.nematic ? 1 : 0; ba.velocity = I.velocity ? 1 : 0;
+    ba.temp = I.temp; ba.seed = I.seed; ba.clock = ctx->d_clock; ba.clock_slot = I.slot; ba.t_off = t_off;
+    ba.parity = par; ba.head_enforce = t + 1 <= steps ? 1 : 0;
+    ba.chunk = chunk; ba.slot = slot;
+    {
+      ProfScope ps(ctx, "fused");
+      launch_brownian_fused(ctx->st, pot_mask(ctx), al_mask(ctx), cur, nxt, ctx->nl, ctx->box, ctx->con,
+                            interact_params(ctx), n, ba, f32 ? 1 : 0, planar ? 1 : 0, ctx->ntypes > 1 ? 1 : 0);
+    }
+    if (!chunk) {
+      ProfScope ps(ctx, "rebuild");
+      launch_rebuild(ctx->st, nxt, ctx->nl, ctx->box, n, ctx->d_flags + par, planar ? 1 : 0);
+    }
+  };
+  int t = a, buf = 0;
+  const int M = ctx->chunk_slots;
+  const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
+  if (graph && b - 1 - t >= 2 * M) {
+    Fields f0 = buffer(0), f1 = buffer(1);
+    launch_chunk_init(ctx->st, ctx->d_chunk, t, b - 1);
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    // head_enforce of every slot: its iteration t' <= b - 2 < steps
+    for (int j = 0; j < M; ++j) step(b - 2, j & 1, j, ctx->d_chunk, j);
+    launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, n, planar ? 1 : 0, 1);
+    launch_rebuild(ctx->st, f0, ctx->nl, ctx->box, n, &ctx->d_chunk->rebuild, planar ? 1 : 0);
+    launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
+    HIPCHK(hipStreamEndCapture(ctx->st, &g));
+    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    double per = 0.5 * M;
+    int* h_base = reinterpret_cast<int*>(ctx->h_out);
+    while (t < b - 1) {
+      const int k = std::max(1, std::min(256, (int)((b - 1 - t) / per)));
+      for (int i = 0; i < k; ++i) HIPCHK(hipGraphLaunch(ge, ctx->st));
+      HIPCHK(hipMemcpyAsync(h_base, &ctx->d_chunk->base, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+      HIPCHK(hipStreamSynchronize(ctx->st));
+      const int tn = *h_base;
+      if (tn <= t || tn > b - 1) {
+        hipGraphExecDestroy(ge);
+        hipGraphDestroy(g);
+        return fail(ctx, "captured step chunks made no progress");
+      }
+      per = std::max(1.0, (double)(tn - t) / k);
+      t = tn;
+    }
+    HIPCHK(hipGraphExecDestroy(ge));
+    HIPCHK(hipGraphDestroy(g));
+    HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
+  }
+  const int t0 = t;
+  for (; t < b; ++t) step(t, (buf + t - t0) & 1, t - t0, nullptr, 0);
+  launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
+  const int pf = (buf + b - t0) & 1;  // buffer holding x_b and n_b
+  if (planar) {
+    Fields f0 = fields(ctx, 0);
+    f0.xy = ctx->xy[pf];
+    launch_unpack_xy(ctx->st, f0, n);
+  } else if (pf) {
+    for (int k = 0; k < 3; ++k)
+      HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->alt[k], sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
+  }
+  if (pf)
+    for (int k = 0; k < (planar ? 2 : 3); ++k)
+      HIPCHK(hipMemcpyAsync(ctx->dcol[6 + k], ctx->altn[k], sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
+  I.calls += (unsigned long long)(b - a);
+  // loop head of b: its enforce was applied with the positions; forces and torques
+  if (b <= steps) loop_head_forces(ctx, nullptr, f32);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
 
 // ---- cell-tiled fused Langevin run (planar systems) ------------------------
 // Used when the run asks for it (SAMOS_RUN_TILES) and the system fits: plane
@@ -838,7 +946,8 @@ int samos_create(samos_ctx** out, int device, int n, double lx, double ly, doubl
   for (int k = 0; k < kNumIntFields; ++k)
     if (hipMalloc(&ctx->icol[k], sizeof(int) * np) != hipSuccess) return bad("columns");
   for (int k = 0; k < 3; ++k)
-    if (hipMalloc(&ctx->alt[k], sizeof(double) * np) != hipSuccess) return bad("columns");
+    if (hipMalloc(&ctx->alt[k], sizeof(double) * np) != hipSuccess || hipMalloc(&ctx->altn[k], sizeof(double) * np) != hipSuccess)
+      return bad("columns");
   for (int k = 0; k < 2; ++k)
     if (hipMalloc(&ctx->xy[k], sizeof(double2) * np) != hipSuccess) return bad("columns");
   if (hipMalloc(&ctx->d_flags, 16 * sizeof(int)) != hipSuccess) return bad("flags");
@@ -893,7 +1002,7 @@ int samos_destroy(samos_ctx* ctx) {
   for (auto& kv : ctx->integs) hipFree(kv.second.R);
   for (int k = 0; k < kNumDoubleFields; ++k) hipFree(ctx->dcol[k]);
   for (int k = 0; k < kNumIntFields; ++k) hipFree(ctx->icol[k]);
-  for (int k = 0; k < 3; ++k) hipFree(ctx->alt[k]);
+  for (int k = 0; k < 3; ++k) { hipFree(ctx->alt[k]); hipFree(ctx->altn[k]); }
   for (int k = 0; k < 2; ++k) hipFree(ctx->xy[k]);
   hipFree(ctx->d_flags); hipFree(ctx->d_nnb); hipFree(ctx->d_tab); hipFree(ctx->d_clock); hipFree(ctx->d_chunk); hipFree(ctx->d_en); hipFree(ctx->d_cols);
   hipFree(ctx->d_partial); hipFree(ctx->d_out); hipFree(ctx->d_counter);
@@ -1399,6 +1508,11 @@ int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags
                      needs_nlist(ctx);
   if (fused) {
     if (int r = run_fused_langevin(ctx, ctx->integs["langevin_all"], t_begin, t_end, steps, flags)) return r;
+    ctx->time_step += t_end - t_begin;
+    return check_overflow(ctx);
+  }
+  if (fused_brownian_eligible(ctx, flags)) {
+    if (int r = run_fused_brownian(ctx, ctx->integs.begin()->second, t_begin, t_end, steps, flags)) return r;
     ctx->time_step += t_end - t_begin;
     return check_overflow(ctx);
   }

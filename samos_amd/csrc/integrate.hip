@@ -43,10 +43,6 @@ __device__ __forceinline__ void store_ps(const Fields& f, int s, const PS& p) {
   if (p.diz) f.iz[s] += p.diz;
 }
 
-__device__ __forceinline__ unsigned long long eff_call(const ClockD* clk, int slot, unsigned long long call) {
-  return clk ? clk->calls[slot] + call : call;
-}
-__device__ __forceinline__ int eff_step(const ClockD* clk, int t_off) { return clk ? clk->run_step + t_off : t_off; }
 
 __global__ void k_enforce(Fields f, BoxD b, ConstraintD c, int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -445,7 +441,8 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_chunk_settle(ChunkCtl* ck, Fields f0, Fields f1, int n, int planar) {
+__global__ void __launch_bounds__(kBlock) k_chunk_settle(ChunkCtl* ck, Fields f0, Fields f1, int n, int planar,
+                                                         int directors) {
   const int last = ck->last;
   if (blockIdx.x == 0 && threadIdx.x == 0) ck->rebuild = ck->halt | ck->req[0] | ck->req[1];
   if (!(last & 1)) return;
@@ -454,6 +451,10 @@ __global__ void __launch_bounds__(kBlock) k_chunk_settle(ChunkCtl* ck, Fields f0
       f0.xy[i] = f1.xy[i];
     } else {
       f0.x[i] = f1.x[i]; f0.y[i] = f1.y[i]; f0.z[i] = f1.z[i];
+    }
+    if (directors) {
+      f0.nx[i] = f1.nx[i]; f0.ny[i] = f1.ny[i];
+      if (!planar) f0.nz[i] = f1.nz[i];
     }
   }
 }
@@ -593,9 +594,9 @@ void launch_fire_mix(hipStream_t st, Fields f, int n, double f1, double f2, int 
   hipLaunchKernelGGL(k_fire_mix, dim3(grid1(n)), dim3(kBlock), 0, st, f, n, f1, f2, zero_v);
 }
 
-void launch_chunk_settle(hipStream_t st, ChunkCtl* ck, Fields f0, Fields f1, int n, int planar) {
+void launch_chunk_settle(hipStream_t st, ChunkCtl* ck, Fields f0, Fields f1, int n, int planar, int directors) {
   const int g = std::max(1, std::min(grid1(n), 1024));
-  hipLaunchKernelGGL(k_chunk_settle, dim3(g), dim3(kBlock), 0, st, ck, f0, f1, n, planar);
+  hipLaunchKernelGGL(k_chunk_settle, dim3(g), dim3(kBlock), 0, st, ck, f0, f1, n, planar, directors);
 }
 
 void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock) {

@@ -105,11 +105,30 @@ void launch_baoab_fused(hipStream_t st, int pot_mask, int phase, Fields cur, Fie
 // phases); returns the launch status so the caller can fall back to
 // launch_rebuild where cooperative launches are unavailable.
 hipError_t launch_rebuild_coop(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req, int planar);
+// Fused Brownian step (IntegratorBrownian, src/integrators/integrator_brownian.cpp:42-117)
+// with the loop-head forces and torques of the same step; positions and
+// directors read from `cur` and written to `nxt`.
+struct BrownArgs {
+  double dt, v0, mu, mur, stoch, flip_p;
+  int nematic, velocity;
+  TempD temp;
+  uint32_t seed;
+  const ClockD* clock;
+  int clock_slot, t_off, parity, head_enforce;
+  ChunkCtl* chunk = nullptr;  // captured-chunk mode (null: direct launch with the parity flags)
+  int slot = 0;
+};
+bool brownian_fused_supported(int pot_mask, int al_mask);
+void launch_brownian_fused(hipStream_t st, int pot_mask, int al_mask, Fields cur, Fields nxt, NListD nl, BoxD box,
+                           ConstraintD con, InteractD ip, int n, const BrownArgs& ba, int fp32, int planar,
+                           int multitype);
+
 // Captured-chunk bookkeeping (see ChunkCtl): the settle step after the slots
-// copies positions left in buffer 1 back to buffer 0 and forms the rebuild
+// copies positions (and, with `directors`, directors) left in buffer 1 back
+// to buffer 0 and forms the rebuild
 // request; the finish step advances the clock and the chunk base by the slots
 // that ran and clears the control words; init sets base and limit.
-void launch_chunk_settle(hipStream_t st, ChunkCtl* ck, Fields f0, Fields f1, int n, int planar);
+void launch_chunk_settle(hipStream_t st, ChunkCtl* ck, Fields f0, Fields f1, int n, int planar, int directors);
 void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock);
 void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit);
 // Interleaved (x, y) copy of the positions, read by the planar kernels.

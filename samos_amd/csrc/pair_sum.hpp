@@ -24,10 +24,17 @@ namespace samos {
 // exact no-op and is skipped (a per-lane test inside one loop: whole
 // wavefronts away from the faces branch over it).  PLANAR pairs have dz = 0
 // and no z image.
-template <int POT, bool F32, bool PLANAR, bool MT, int kBatch>
+//
+// AL (AL_POLAR, AL_NEMATIC or 0) adds the aligner torques on the slot's
+// director n_i (src/aligner/pair/pair_polar_aligner.cpp:58-103,
+// src/aligner/pair/pair_nematic_aligner.cpp:53-92): neighbour directors are
+// gathered with the positions.
+template <int POT, bool F32, bool PLANAR, bool MT, int kBatch, int AL = 0>
 __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& nl, const BoxD& b, const InteractD& ip,
                                                 int s, double xi, double yi, double zi, int ti, double ri,
-                                                double& fx, double& fy, double& fz, bool wrap_i) {
+                                                double& fx, double& fy, double& fz, bool wrap_i,
+                                                double nxi = 0.0, double nyi = 0.0, double nzi = 0.0,
+                                                double* tq = nullptr) {
   const bool radii = ((POT & POT_SOFT) && ip.soft_radii) || ((POT & POT_LJ) && ip.lj_radii);
   // One particle type: the table entries of the pair (1, 1) are loop constants,
   // and so are the first products of each expression (sigma^2, rcut^2,
@@ -41,6 +48,13 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
       c_eps = ip.lj_tab[0]; c_rc2 = rc * rc; c_e48 = 48.0 * c_eps; c_sig2 = sig * sig;
     }
   }
+  const int nt2p = ip.pol_nt * ip.pol_nt, nt2n = ip.nem_nt * ip.nem_nt;
+  double c_pj = 0.0, c_prc = 0.0, c_nj = 0.0, c_nrc = 0.0;
+  if (!MT) {
+    if (AL & AL_POLAR) { c_pj = ip.pol_tab[0]; c_prc = ip.pol_tab[nt2p]; }
+    if (AL & AL_NEMATIC) { c_nj = ip.nem_tab[0]; c_nrc = ip.nem_tab[nt2n]; }
+  }
+  double tx = 0.0, ty = 0.0, tz = 0.0;
   const int kn = nl.nnb[s];
   const int* nb = nl.nbr + s;
   for (int k0 = 0; k0 < kn; k0 += kBatch) {
@@ -48,6 +62,7 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) t[u] = (k0 + u < kn) ? nb[(long)(k0 + u) * nl.np] : s;
     double xj[kBatch], yj[kBatch], zj[kBatch], rj[kBatch];
+    double nxj[kBatch], nyj[kBatch], nzj[kBatch];
     int tj[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
@@ -58,6 +73,10 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
         xj[u] = f.x[t[u]]; yj[u] = f.y[t[u]]; zj[u] = f.z[t[u]];
       }
       tj[u] = MT ? f.type[t[u]] - 1 : 0;
+      if (AL) {
+        nxj[u] = f.nx[t[u]]; nyj[u] = f.ny[t[u]];
+        nzj[u] = PLANAR ? 0.0 : f.nz[t[u]];
+      }
       rj[u] = radii ? f.radius[t[u]] : 0.0;
     }
 #pragma unroll
@@ -118,8 +137,35 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
           }
         }
       }
+      if (AL) {
+        // n_i x n_j; on a plane both directors have n_z = 0 and only the z
+        // component is non-zero (the x and y terms would add exact zeros)
+        const double cz = nxi * nyj[u] - nyi * nxj[u];
+        const double cx = PLANAR ? 0.0 : nyi * nzj[u] - nzi * nyj[u];
+        const double cy = PLANAR ? 0.0 : nzi * nxj[u] - nxi * nzj[u];
+        if (AL & AL_POLAR) {
+          const int tt = MT ? ti * ip.pol_nt + tj[u] : 0;
+          const double J = MT ? ip.pol_tab[tt] : c_pj;
+          const double rc = MT ? ip.pol_tab[nt2p + tt] : c_prc;
+          if (r_sq <= rc * rc) {
+            if (!PLANAR) { tx += J * cx; ty += J * cy; }
+            tz += J * cz;
+          }
+        }
+        if (AL & AL_NEMATIC) {
+          const int tt = MT ? ti * ip.nem_nt + tj[u] : 0;
+          const double J = MT ? ip.nem_tab[tt] : c_nj;
+          const double rc = MT ? ip.nem_tab[nt2n + tt] : c_nrc;
+          if (r_sq <= rc * rc) {
+            const double nn = PLANAR ? nxi * nxj[u] + nyi * nyj[u] : nxi * nxj[u] + nyi * nyj[u] + nzi * nzj[u];
+            if (!PLANAR) { tx += J * nn * cx; ty += J * nn * cy; }
+            tz += J * nn * cz;
+          }
+        }
+      }
     }
   }
+  if (AL) { tq[0] = tx; tq[1] = ty; tq[2] = tz; }
 }
 
 template <int POT, bool F32, bool PLANAR, bool MT, int kBatch = 4>

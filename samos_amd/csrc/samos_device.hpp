@@ -164,6 +164,13 @@ struct ClockD {
   int pad_;
 };
 
+// Noise call and run step of a launch: read from the device clock plus an
+// offset inside captured step sequences, or given directly (clk null).
+__device__ __forceinline__ unsigned long long eff_call(const ClockD* clk, int slot, unsigned long long call) {
+  return clk ? clk->calls[slot] + call : call;
+}
+__device__ __forceinline__ int eff_step(const ClockD* clk, int t_off) { return clk ? clk->run_step + t_off : t_off; }
+
 // ---------------------------------------------------------------------------
 // Counter-based noise: Philox4x32-10 (Salmon et al., SC'11), keyed by
 // (seed, integrator tag) with counter (particle id, call lo, call hi, purpose).

@@ -32,10 +32,6 @@ __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >
 __device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ int gsize() { return gridDim.x * blockDim.x; }
 
-__device__ __forceinline__ unsigned long long eff_call(const ClockD* clk, int slot, unsigned long long call) {
-  return clk ? clk->calls[slot] + call : call;
-}
-__device__ __forceinline__ int eff_step(const ClockD* clk, int t_off) { return clk ? clk->run_step + t_off : t_off; }
 
 __device__ __forceinline__ int load_agent(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

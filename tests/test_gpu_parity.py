@@ -327,6 +327,27 @@ def test_langevin_linear_temperature_same_stream(gpu, flags):
     assert np.abs(d.get("vy") - o.get("vy")).max() <= 1e-9
 
 
+def test_langevin_plane_n_squared_list(gpu):
+    # a box too thin for a cell grid: the fused steps keep the 3-D position
+    # layout that the N^2 build reads
+    rng = np.random.default_rng(6)
+    n = 900
+    st = {"n": n, "box": (30.0, 30.0, 4.0), "periodic": True,
+          "cols": {"x": rng.uniform(-15, 15, n), "y": rng.uniform(-15, 15, n), "nx": np.ones(n)}}
+    d, o = both(gpu, st)
+    for s in (d, o):
+        s.nlist(2.4, 0.3)
+        s.pair_soft(1.0, 2.0)
+        s.constraint("plane")
+        s.set_timestep(0.01)
+        s.integrator_langevin(gamma=1.0, seed=9, temp=(0, 0.3, 0.3, 1))
+    assert d.nlist_info()["uses_cells"] == 0
+    d.run(299)
+    o.run(299)
+    assert pos_err(positions(d), positions(o), st["box"], True) <= 1e-9
+    assert np.array_equal(d.pairs(), o.pairs())
+
+
 @pytest.mark.parametrize("slots", [2, 6, 24])
 def test_chunked_fused_steps_equal_direct_launches(gpu, slots):
     # captured chunks stop at the step whose positions need a rebuild and
@@ -401,6 +422,88 @@ def test_captured_chunks_equal_direct_launches(gpu, kind):
         d.close()
     for k in runs[0]:
         assert np.array_equal(runs[0][k], runs[1][k]), k
+
+
+@pytest.mark.parametrize("case", ["flips_velocity_sp", "types_radii_nematic", "lj_polar_noise", "sphere_f32"])
+def test_fused_brownian_cases(gpu, case):
+    # the fused Brownian step (loop-head forces and torques + integrator in one
+    # launch) against the oracle's two-pass loop
+    if case == "sphere_f32":
+        st = sc.sphere_disks(2000, phi=0.9, seed=32)
+    elif case == "lj_polar_noise":
+        st = sc.lj_lattice(48, rho=0.7, seed=32)
+        th = np.random.default_rng(4).uniform(0.0, 2.0 * np.pi, st["n"])
+        st["cols"]["nx"], st["cols"]["ny"] = np.cos(th), np.sin(th)
+    else:
+        st = sc.plane_disks(2500, phi=0.75, seed=32, polydisperse=(0.8, 1.2) if case == "types_radii_nematic" else None)
+    if case == "types_radii_nematic":
+        st["cols"]["type"] = np.random.default_rng(3).integers(1, 3, st["n"]).astype(np.int32)
+    d, o = both(gpu, st)
+    for s in (d, o):
+        s.nlist(2.6, 0.3)
+        if case == "flips_velocity_sp":
+            s.pair_soft(1.0, 2.0)
+            s.align_polar(0.3, 2.4)
+            s.self_propulsion(0.5)
+            s.constraint("plane")
+            s.set_timestep(0.01)
+            s.integrator_brownian(v0=0.2, nu=0.2, seed=5, nematic=True, tau=0.3, velocity_align=True,
+                                  temp=(0, 0.02, 0.02, 1))
+        elif case == "types_radii_nematic":
+            s.pair_soft(1.0, 2.0, use_radii=True)
+            s.pair_type_param("soft", 1, 2, 2.0, 2.0)
+            s.align_nematic(0.4, 2.4)
+            s.align_type_param("nematic", 1, 2, 0.7, 2.2)
+            s.constraint("plane")
+            s.set_timestep(0.01)
+            s.integrator_brownian(v0=0.1, nu=0.1, seed=6, temp=(1, 0.0, 0.05, 100))
+        elif case == "lj_polar_noise":
+            s.pair_lj(1.0, 1.0, 2.5, shifted=True)
+            s.pair_soft(1.0, 1.2)
+            s.align_polar(0.2, 2.4)
+            s.constraint("plane")
+            s.set_timestep(0.002)
+            s.integrator_brownian(v0=0.5, nu=0.05, seed=7, temp=(0, 0.1, 0.1, 1))
+        else:
+            s.pair_soft(10.0, 2.0)
+            s.align_nematic(0.5, 2.4)
+            s.constraint("sphere", st["R"])
+            s.set_timestep(0.005)
+            s.integrator_brownian(v0=0.5, nu=0.05, seed=8, temp=(0, 0.0, 0.0, 1))
+    flags = gpu.RUN_FP32_FORCE if case == "sphere_f32" else 0
+    d.run(199, flags)
+    o.run(199)
+    # FP32 pair terms: positions and directors after 200 steps within 1e-6, the
+    # forces of the final positions within 1e-5 (FP32 rounding on top of the drift)
+    tol = 1e-6 if case == "sphere_f32" else 1e-9
+    assert pos_err(positions(d), positions(o), st["box"], st["periodic"]) <= tol
+    assert np.abs(directors(d) - directors(o)).max() <= tol
+    assert np.abs(d.get("vx") - o.get("vx")).max() <= tol * 100
+    assert sc.rel_force_err(forces(d), forces(o)) <= tol * (10 if case == "sphere_f32" else 1)
+    assert np.array_equal(d.get("age"), o.get("age"))
+
+
+@pytest.mark.parametrize("slots", [2, 8])
+def test_chunked_brownian_equals_direct_launches(gpu, slots):
+    st = sc.plane_disks(3000, phi=0.8, seed=33)
+    out = []
+    for flags in (0, gpu.RUN_NO_GRAPH):
+        d = sc.load(gpu.System, st)
+        d.nlist(2.4, 0.2)
+        d.pair_soft(1.0, 2.0)
+        d.align_polar(0.3, 2.4)
+        d.constraint("plane")
+        d.set_timestep(0.02)
+        d.integrator_brownian(v0=0.5, nu=0.1, seed=9, temp=(0, 0.1, 0.1, 1))
+        d.set_option("chunk_slots", slots)
+        b0 = d.nlist_info()["builds"]
+        d.run(299, flags)
+        out.append(({k: d.get(k) for k in ("x", "y", "nx", "ny", "fx", "tau_z", "omega", "ix")}, d.pairs(),
+                    d.nlist_info()["builds"] - b0))
+    (a, pa, ba), (c, pc, bc) = out
+    assert all(np.array_equal(a[k], c[k]) for k in a)
+    assert np.array_equal(pa, pc)
+    assert ba == bc and ba > 10
 
 
 def test_brownian_pos_align_self_propulsion(gpu):
