@@ -251,12 +251,14 @@ def c2_system(samos_amd, sc):
     return s, st
 
 
-def bench_single(samos_amd, sc, which, steps, cpu_steps):
-    """One system through the drop-in single-system path (the `run` loop)."""
+def bench_single(samos_amd, sc, which, steps, cpu_steps, extra_flags=0):
+    """One system through the drop-in single-system path (the `run` loop).
+    extra_flags: e.g. RUN_NO_GRAPH for a profiler run (step-by-step launches)."""
     build = c1_system if which == "c1" else c2_system
     out = {}
     variants = [("f64", 0)] + ([("f32_force", samos_amd.RUN_FP32_FORCE)] if which == "c2" else [])
     for label, flags in variants:
+        flags |= extra_flags
         s, st = build(samos_amd, sc)
         s.run(99, flags)
         secs = timed_run(s, steps, flags)
@@ -499,6 +501,8 @@ def main():
     ap.add_argument("--pad", type=float, default=0.5, help="neighbour-list skin")
     ap.add_argument("--rebuild", choices=["coop", "stages"], default="stages")
     ap.add_argument("--batch", type=int, choices=[4, 8], default=4)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the c1 and c2 runs step by step (for kernel-trace profiling)")
     ap.add_argument("--chunk-slots", type=int, default=32,
                     help="fused steps per captured chunk (each chunk ends in one conditional rebuild)")
     ap.add_argument("--tiles", action="store_true",
@@ -543,10 +547,11 @@ def main():
                                             nrep=args.ensemble_replicas, pad=args.ensemble_pad)
     if "fire" in parts:
         extras["fire"] = bench_fire(samos_amd, sc, nrep=args.fire_replicas, weights=args.warm_weights)
+    ng = samos_amd.RUN_NO_GRAPH if args.no_graph else 0
     if "c1" in parts:
-        extras["polar_disks_plane_n4096"] = bench_single(samos_amd, sc, "c1", 10000, 0 if args.no_cpu else 1000)
+        extras["polar_disks_plane_n4096"] = bench_single(samos_amd, sc, "c1", 10000, 0 if args.no_cpu else 1000, ng)
     if "c2" in parts:
-        extras["nematic_sphere_n100000"] = bench_single(samos_amd, sc, "c2", 5000, 0 if args.no_cpu else 40)
+        extras["nematic_sphere_n100000"] = bench_single(samos_amd, sc, "c2", 5000, 0 if args.no_cpu else 40, ng)
     if "main" not in parts:
         print(json.dumps(extras))
         return

@@ -41,7 +41,11 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
   } else if (blockIdx.x == 0 && threadIdx.x == 0) {
     nl.flags[1 - ba.parity] = 0;
   }
-  const int s = xcd_block() * blockDim.x + threadIdx.x;
+  // ba.lanes (a power of two dividing 64) consecutive threads share one slot:
+  // each sums every lanes-th batch of the list, the partial sums are added
+  // in a fixed butterfly order, and lane 0 integrates and writes.
+  const int gt = xcd_block() * blockDim.x + threadIdx.x;
+  const int s = gt >> ba.lane_shift, lane = gt & (ba.lanes - 1);
   bool need = false;
   if (s < n) {
     double xi, yi, zi;
@@ -64,7 +68,15 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
     }
     double fx = 0.0, fy = 0.0, fz = 0.0, tq[3] = {0.0, 0.0, 0.0};
     pair_force_loop<POT, F32, PLANAR, MT, 4, AL>(cur, nl, b, ip, s, xi, yi, zi, ti, ri, fx, fy, fz, wrap, nxi, nyi,
-                                                 nzi, tq);
+                                                 nzi, tq, lane, ba.lanes);
+    for (int off = 1; off < ba.lanes; off <<= 1) {
+      fx += __shfl_xor(fx, off); fy += __shfl_xor(fy, off);
+      if (!PLANAR) fz += __shfl_xor(fz, off);
+      if (AL) {
+        if (!PLANAR) { tq[0] += __shfl_xor(tq[0], off); tq[1] += __shfl_xor(tq[1], off); }
+        tq[2] += __shfl_xor(tq[2], off);
+      }
+    }
     if (ip.has_sp) { fx += ip.sp_alpha * nxi; fy += ip.sp_alpha * nyi; fz += ip.sp_alpha * nzi; }
 
     // IntegratorBrownian::integrate for this particle, with the forces and
@@ -97,31 +109,33 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
       p.x += sqrt_dt * frx; p.y += sqrt_dt * fry; p.z += sqrt_dt * frz;
     }
     const bool fz0 = enforce<kSet>(b, c, p);
+    const bool writer = lane == 0;
     const double om = ba.mur * project_torque<kSet>(c, p, tq[0], tq[1], tq[2]);
     const double dtheta = ba.dt * om + ba.stoch * g_rot;
     rotate_director<kSet>(c, p, dtheta);
     if (ba.velocity) rotate_velocity<kSet>(c, p, dtheta);
     if (ba.head_enforce) enforce<kSet>(b, c, p);
-
-    cur.fx[s] = fx; cur.fy[s] = fy; cur.fz[s] = fz0 ? 0.0 : fz;
-    if (AL) { cur.tx[s] = tq[0]; cur.ty[s] = tq[1]; cur.tz[s] = tq[2]; }
-    cur.omega[s] = om;
-    cur.age[s] += ba.dt;
-    cur.vx[s] = p.vx; cur.vy[s] = p.vy;
-    if (!PLANAR) cur.vz[s] = p.vz;
-    if (PLANAR) {
-      nxt.xy[s] = make_double2(p.x, p.y);
-    } else {
-      nxt.x[s] = p.x; nxt.y[s] = p.y; nxt.z[s] = p.z;
+    if (writer) {
+      cur.fx[s] = fx; cur.fy[s] = fy; cur.fz[s] = fz0 ? 0.0 : fz;
+      if (AL) { cur.tx[s] = tq[0]; cur.ty[s] = tq[1]; cur.tz[s] = tq[2]; }
+      cur.omega[s] = om;
+      cur.age[s] += ba.dt;
+      cur.vx[s] = p.vx; cur.vy[s] = p.vy;
+      if (!PLANAR) cur.vz[s] = p.vz;
+      if (PLANAR) {
+        nxt.xy[s] = make_double2(p.x, p.y);
+      } else {
+        nxt.x[s] = p.x; nxt.y[s] = p.y; nxt.z[s] = p.z;
+      }
+      nxt.nx[s] = p.nx; nxt.ny[s] = p.ny;
+      if (!PLANAR) nxt.nz[s] = p.nz;
+      if (p.dix) cur.ix[s] += p.dix;
+      if (p.diy) cur.iy[s] += p.diy;
+      if (p.diz) cur.iz[s] += p.diz;
+      double ddx = nl.ox[s] - p.x, ddy = nl.oy[s] - p.y, ddz = PLANAR ? 0.0 : nl.oz[s] - p.z;
+      min_image(b, ddx, ddy, ddz);
+      need = !(ddx * ddx + ddy * ddy + ddz * ddz < nl.pad2q);
     }
-    nxt.nx[s] = p.nx; nxt.ny[s] = p.ny;
-    if (!PLANAR) nxt.nz[s] = p.nz;
-    if (p.dix) cur.ix[s] += p.dix;
-    if (p.diy) cur.iy[s] += p.diy;
-    if (p.diz) cur.iz[s] += p.diz;
-    double ddx = nl.ox[s] - p.x, ddy = nl.oy[s] - p.y, ddz = PLANAR ? 0.0 : nl.oz[s] - p.z;
-    min_image(b, ddx, ddy, ddz);
-    need = !(ddx * ddx + ddy * ddy + ddz * ddz < nl.pad2q);
   }
   if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomicOr(ck ? &ck->req[ba.slot & 1] : &nl.flags[ba.parity], 1);
 }
@@ -129,7 +143,7 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
 template <int POT, int AL, bool F32, bool PLANAR>
 void brownian_mt(hipStream_t st, int multitype, Fields cur, Fields nxt, NListD nl, BoxD b, ConstraintD c, InteractD ip,
                  int n, const BrownArgs& ba) {
-  dim3 g(grid1(n)), blk(kBlock);
+  dim3 g(grid1((long)n * ba.lanes)), blk(kBlock);
   if (multitype)
     hipLaunchKernelGGL((k_brownian_fused<POT, AL, F32, PLANAR, true>), g, blk, 0, st, cur, nxt, nl, b, c, ip, n, ba);
   else

@@ -658,6 +658,11 @@ int run_fused_brownian(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     ba.temp = I.temp; ba.seed = I.seed; ba.clock = ctx->d_clock; ba.clock_slot = I.slot; ba.t_off = t_off;
     ba.parity = par; ba.head_enforce = t + 1 <= steps ? 1 : 0;
     ba.chunk = chunk; ba.slot = slot;
+    // small systems (the step is then latency-bound): several threads per
+    // slot, up to 16, while the launch stays under 64 Ki threads; larger
+    // systems measured faster with one thread per slot
+    ba.lanes = 1; ba.lane_shift = 0;
+    while (ba.lanes < 16 && (long)n * ba.lanes * 2 <= 65536L) { ba.lanes <<= 1; ++ba.lane_shift; }
     {
       ProfScope ps(ctx, "fused");
       launch_brownian_fused(ctx->st, pot_mask(ctx), al_mask(ctx), cur, nxt, ctx->nl, ctx->box, ctx->con,

@@ -117,6 +117,7 @@ struct BrownArgs {
   int clock_slot, t_off, parity, head_enforce;
   ChunkCtl* chunk = nullptr;  // captured-chunk mode (null: direct launch with the parity flags)
   int slot = 0;
+  int lanes = 1, lane_shift = 0;  // threads per slot (power of two up to 64) and its log2
 };
 bool brownian_fused_supported(int pot_mask, int al_mask);
 void launch_brownian_fused(hipStream_t st, int pot_mask, int al_mask, Fields cur, Fields nxt, NListD nl, BoxD box,

@@ -29,12 +29,16 @@ namespace samos {
 // director n_i (src/aligner/pair/pair_polar_aligner.cpp:58-103,
 // src/aligner/pair/pair_nematic_aligner.cpp:53-92): neighbour directors are
 // gathered with the positions.
+//
+// lane, lanes: this thread takes batches lane, lane + lanes, ... of the
+// slot's list (several threads per slot for small systems; the caller adds
+// the partial sums).
 template <int POT, bool F32, bool PLANAR, bool MT, int kBatch, int AL = 0>
 __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& nl, const BoxD& b, const InteractD& ip,
                                                 int s, double xi, double yi, double zi, int ti, double ri,
                                                 double& fx, double& fy, double& fz, bool wrap_i,
                                                 double nxi = 0.0, double nyi = 0.0, double nzi = 0.0,
-                                                double* tq = nullptr) {
+                                                double* tq = nullptr, int lane = 0, int lanes = 1) {
   const bool radii = ((POT & POT_SOFT) && ip.soft_radii) || ((POT & POT_LJ) && ip.lj_radii);
   // One particle type: the table entries of the pair (1, 1) are loop constants,
   // and so are the first products of each expression (sigma^2, rcut^2,
@@ -57,7 +61,7 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
   double tx = 0.0, ty = 0.0, tz = 0.0;
   const int kn = nl.nnb[s];
   const int* nb = nl.nbr + s;
-  for (int k0 = 0; k0 < kn; k0 += kBatch) {
+  for (int k0 = lane * kBatch; k0 < kn; k0 += lanes * kBatch) {
     int t[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) t[u] = (k0 + u < kn) ? nb[(long)(k0 + u) * nl.np] : s;

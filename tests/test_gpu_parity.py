@@ -483,6 +483,23 @@ def test_fused_brownian_cases(gpu, case):
     assert np.array_equal(d.get("age"), o.get("age"))
 
 
+def test_fused_brownian_one_thread_per_slot(gpu):
+    # above 32 Ki particles the fused Brownian step uses one thread per slot
+    st = sc.plane_disks(40000, phi=0.8, seed=34)
+    d, o = both(gpu, st)
+    for s in (d, o):
+        s.nlist(2.4, 0.3)
+        s.pair_soft(1.0, 2.0)
+        s.align_polar(0.2, 2.4)
+        s.constraint("plane")
+        s.set_timestep(0.01)
+        s.integrator_brownian(v0=0.2, nu=0.1, seed=10, temp=(0, 0.05, 0.05, 1))
+        s.run(99)
+    assert pos_err(positions(d), positions(o), st["box"], True) <= 1e-9
+    assert np.abs(directors(d) - directors(o)).max() <= 1e-9
+    assert sc.rel_force_err(torques(d), torques(o)) <= 1e-9
+
+
 @pytest.mark.parametrize("slots", [2, 8])
 def test_chunked_brownian_equals_direct_launches(gpu, slots):
     st = sc.plane_disks(3000, phi=0.8, seed=33)
