@@ -383,7 +383,8 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
       }
       PS p;
       p.x = xi; p.y = yi; p.z = zi; p.vx = vx; p.vy = vy; p.vz = vz;
-      p.nx = cur.nx[s]; p.ny = cur.ny[s]; p.nz = PLANAR ? 0.0 : cur.nz[s];
+      const double nx0 = cur.nx[s], ny0 = cur.ny[s], nz0 = PLANAR ? 0.0 : cur.nz[s];
+      p.nx = nx0; p.ny = ny0; p.nz = nz0;
       p.dix = p.diy = p.diz = 0;
       constexpr int kSet = PLANAR ? kSurfPlane : kSurfAll;
       if (!PLANAR && PHASE == 1) {
@@ -421,8 +422,9 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
       }
       cur.vx[s] = p.vx; cur.vy[s] = p.vy;
       if (!PLANAR) cur.vz[s] = p.vz;
-      cur.nx[s] = p.nx; cur.ny[s] = p.ny;
-      if (!PLANAR) cur.nz[s] = p.nz;
+      // the enforce renormalises the director; store only what changed
+      if (p.nx != nx0 || p.ny != ny0) { cur.nx[s] = p.nx; cur.ny[s] = p.ny; }
+      if (!PLANAR && p.nz != nz0) cur.nz[s] = p.nz;
       if (p.dix) cur.ix[s] += p.dix;
       if (p.diy) cur.iy[s] += p.diy;
       if (p.diz) cur.iz[s] += p.diz;

@@ -17,6 +17,8 @@
 //
 // PLANAR: positions come from the interleaved (x, y) column; all particles sit
 // on one plane, so dz = 0 and d2 is bit-identical to the 3-D expression.
+#include <climits>
+
 #include "kernels.hpp"
 #include <hip/hip_cooperative_groups.h>
 #include <hipcub/hipcub.hpp>
@@ -135,12 +137,51 @@ __device__ void scatter_phase(NListD& nl, int n) {
   }
 }
 
+// Bitonic sorting network over N values held in registers (N a power of
+// two; every index is a compile-time constant, so nothing spills).
+template <int N>
+__device__ __forceinline__ void bitonic_sort(int (&v)[N]) {
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool up = (i & k) == 0;
+          const int lo = min(v[i], v[l]), hi = max(v[i], v[l]);
+          v[i] = up ? lo : hi;
+          v[l] = up ? hi : lo;
+        }
+      }
+    }
+  }
+}
+
+// One cell's members (m <= N) sorted in registers: all loads are issued
+// together instead of one dependent global access per insertion step.
+template <int N>
+__device__ __forceinline__ void sort_cell_registers(int* mem, int m) {
+  int v[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = q < m ? mem[q] : INT_MAX;
+  bitonic_sort<N>(v);
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+    if (q < m) mem[q] = v[q];
+}
+
 // Ascending slot order inside each cell makes the list deterministic.
 __device__ void sort_phase(NListD& nl) {
   int local_max = 0;
   for (int c = gtid(); c < nl.ncell; c += gsize()) {
     const int a = nl.start[c], e = nl.start[c + 1];
-    local_max = max(local_max, e - a);
+    const int m = e - a;
+    local_max = max(local_max, m);
+    if (m <= 1) continue;
+    if (m <= 8) { sort_cell_registers<8>(nl.members + a, m); continue; }
+    if (m <= 16) { sort_cell_registers<16>(nl.members + a, m); continue; }
     for (int i = a + 1; i < e; ++i) {
       const int key = nl.members[i];
       int j = i - 1;
