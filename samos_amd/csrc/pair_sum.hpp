@@ -111,8 +111,8 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
           if (r < a) {
             const double diff = a - r;
             const double ff = (r > 0.0) ? kk * diff / r : kk * diff;
-            fx -= ff * dx; fy -= ff * dy;
-            if (!PLANAR) fz -= ff * dz;
+            fx = fma(-ff, dx, fx); fy = fma(-ff, dy, fy);
+            if (!PLANAR) fz = fma(-ff, dz, fz);
           }
         }
       }
@@ -136,8 +136,9 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
             const double inv_r_sq = lj_inv_r_sq(sig2, r_sq);
             const double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
             const double ff = e48 * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
-            fx -= ff * dx; fy -= ff * dy;
-            if (!PLANAR) fz -= ff * dz;
+            // one rounding per accumulation (fused multiply-add)
+            fx = fma(-ff, dx, fx); fy = fma(-ff, dy, fy);
+            if (!PLANAR) fz = fma(-ff, dz, fz);
           }
         }
       }
