@@ -257,12 +257,25 @@ def bench_single(samos_amd, sc, which, steps, cpu_steps, extra_flags=0):
     build = c1_system if which == "c1" else c2_system
     out = {}
     variants = [("f64", 0)] + ([("f32_force", samos_amd.RUN_FP32_FORCE)] if which == "c2" else [])
+    alg = 80 if which == "c1" else 120   # algorithmic bytes per particle-step
     for label, flags in variants:
         flags |= extra_flags
         s, st = build(samos_amd, sc)
         s.run(99, flags)
         secs = timed_run(s, steps, flags)
         out[label] = {"value": st["n"] * steps / secs, "ms_per_step": secs * 1e3 / steps}
+        # step kernel (fused loop-head forces + Brownian step): HIP events on the
+        # context's stream around each launch of a step-by-step pass
+        s.profile(True)
+        timed_run(s, 200, flags)
+        ms, launches = s.profile_read("fused")
+        s.profile(False)
+        if launches:
+            avg = ms / 1e3 / launches
+            out[label]["roofline"] = {"bound": "hbm", "achieved": alg * st["n"] / avg / 1e9, "peak": HBM_PEAK_GBS,
+                                      "unit": "GB/s", "frac": alg * st["n"] / avg / 1e9 / HBM_PEAK_GBS,
+                                      "kernel": "k_brownian_fused", "avg_launch_us": avg * 1e6,
+                                      "launches_timed": launches, "alg_bytes_per_launch": alg * st["n"]}
         s.close()
     print("bench: %s %s" % (which, out), file=sys.stderr, flush=True)
     res = {"workload": build.__doc__.strip(), "particles": st["n"], "steps": steps, "unit": "particle-steps/s",
