@@ -442,8 +442,11 @@ int upload_clock(samos_ctx* ctx, int run_step) {
 }
 
 // One fused step t of the Langevin BAOAB schedule with positions in buffer
-// (t % 2), followed by the device-conditional rebuild on the new positions:
-// one cooperative launch when available (coop), else the five stage kernels.
+// (rel % 2), followed (outside captured chunks) by the device-conditional
+// rebuild on the new positions: one cooperative launch when available
+// (coop), else the stage kernels (bin, scan, scatter, sort, build).  In a
+// chunk (chunk != null) the step is slot `slot` and the chunk's own rebuild
+// follows the slots.
 int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int rel, int t_off, int steps, bool f32, bool planar,
                double mass_u, bool coop, ChunkCtl* chunk = nullptr, int slot = 0) {
   const int par = rel & 1;
