@@ -79,47 +79,77 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
     }
     if (ip.has_sp) { fx += ip.sp_alpha * nxi; fy += ip.sp_alpha * nyi; fz += ip.sp_alpha * nzi; }
 
-    // IntegratorBrownian::integrate for this particle, with the forces and
-    // torques just computed
-    const unsigned long long call = eff_call(ba.clock, ba.clock_slot, 0ULL) + (unsigned long long)ba.t_off;
     const uint32_t pid = (uint32_t)cur.id[s];
     PS p;
     p.x = xi; p.y = yi; p.z = zi;
     p.vx = cur.vx[s]; p.vy = cur.vy[s]; p.vz = PLANAR ? 0.0 : cur.vz[s];
     p.nx = nxi; p.ny = nyi; p.nz = nzi;
     p.dix = p.diy = p.diz = 0;
-    double g_rot = 0.0, u_flip = 1.0;
-    gauss1_u1(ba.seed, TAG_BROWNIAN, pid, call, g_rot, u_flip);
-    if (ba.nematic && u_flip < ba.flip_p) {
-      p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz;
-      if (ba.velocity) { p.vx = -p.vx; p.vy = -p.vy; p.vz = -p.vz; }
-    }
     constexpr int kSet = PLANAR ? kSurfPlane : kSurfAll;
-    const double T = ba.temp.at(eff_step(ba.clock, ba.t_off));
-    const double B = sqrt(2.0 * ba.mu * T);
-    const double sqrt_dt = sqrt(ba.dt);
-    const double fdx = ba.v0 * p.nx + ba.mu * fx, fdy = ba.v0 * p.ny + ba.mu * fy, fdz = ba.v0 * p.nz + ba.mu * fz;
-    p.vx = fdx; p.vy = fdy; p.vz = fdz;
-    p.x += ba.dt * fdx; p.y += ba.dt * fdy; p.z += ba.dt * fdz;
-    if (T > 0.0) {
-      double g[3];
-      gauss3(ba.seed, TAG_BROWNIAN, pid, call, g);
-      const double frx = B * g[0], fry = B * g[1], frz = B * g[2];
-      p.vx += frx; p.vy += fry; p.vz += frz;
-      p.x += sqrt_dt * frx; p.y += sqrt_dt * fry; p.z += sqrt_dt * frz;
+    bool fz0 = false;
+    double om, age_dt;
+    if (ba.mode == 0) {
+      // IntegratorBrownian::integrate for this particle, with the forces and
+      // torques just computed
+      const unsigned long long call = eff_call(ba.clock, ba.clock_slot, 0ULL) + (unsigned long long)ba.t_off;
+      double g_rot = 0.0, u_flip = 1.0;
+      gauss1_u1(ba.seed, TAG_BROWNIAN, pid, call, g_rot, u_flip);
+      if (ba.nematic && u_flip < ba.flip_p) {
+        p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz;
+        if (ba.velocity) { p.vx = -p.vx; p.vy = -p.vy; p.vz = -p.vz; }
+      }
+      const double T = ba.temp.at(eff_step(ba.clock, ba.t_off));
+      const double B = sqrt(2.0 * ba.mu * T);
+      const double sqrt_dt = sqrt(ba.dt);
+      const double fdx = ba.v0 * p.nx + ba.mu * fx, fdy = ba.v0 * p.ny + ba.mu * fy, fdz = ba.v0 * p.nz + ba.mu * fz;
+      p.vx = fdx; p.vy = fdy; p.vz = fdz;
+      p.x += ba.dt * fdx; p.y += ba.dt * fdy; p.z += ba.dt * fdz;
+      if (T > 0.0) {
+        double g[3];
+        gauss3(ba.seed, TAG_BROWNIAN, pid, call, g);
+        const double frx = B * g[0], fry = B * g[1], frz = B * g[2];
+        p.vx += frx; p.vy += fry; p.vz += frz;
+        p.x += sqrt_dt * frx; p.y += sqrt_dt * fry; p.z += sqrt_dt * frz;
+      }
+      fz0 = enforce<kSet>(b, c, p);
+      om = ba.mur * project_torque<kSet>(c, p, tq[0], tq[1], tq[2]);
+      const double dtheta = ba.dt * om + ba.stoch * g_rot;
+      rotate_director<kSet>(c, p, dtheta);
+      if (ba.velocity) rotate_velocity<kSet>(c, p, dtheta);
+      age_dt = ba.dt;
+    } else {
+      // IntegratorBrownianAlign (directors, at the loop-head position), then
+      // IntegratorBrownianPos (positions), each with its own noise stream
+      const unsigned long long call_a = eff_call(ba.clock, ba.clock_slot, 0ULL) + (unsigned long long)ba.t_off;
+      double g_rot = 0.0, u_flip = 1.0;
+      gauss1_u1(ba.seed, TAG_BROWNIAN_ALIGN, pid, call_a, g_rot, u_flip);
+      if (ba.nematic && u_flip < ba.flip_p) { p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz; }
+      om = ba.mur * project_torque<kSet>(c, p, tq[0], tq[1], tq[2]);
+      rotate_director<kSet>(c, p, ba.dt * om + ba.stoch * g_rot);
+      const unsigned long long call_p = eff_call(ba.clock, ba.pos_slot, 0ULL) + (unsigned long long)ba.t_off;
+      const double T = ba.pos_temp.at(eff_step(ba.clock, ba.t_off));
+      const double B = sqrt(2.0 * ba.pos_mu * T);
+      const double sqrt_dt = sqrt(ba.pos_dt);
+      const double fdx = ba.pos_mu * fx, fdy = ba.pos_mu * fy, fdz = ba.pos_mu * fz;
+      p.vx = fdx; p.vy = fdy; p.vz = fdz;
+      p.x += ba.pos_dt * fdx; p.y += ba.pos_dt * fdy; p.z += ba.pos_dt * fdz;
+      if (T > 0.0) {
+        double g[3];
+        gauss3(ba.pos_seed, TAG_BROWNIAN_POS, pid, call_p, g);
+        const double frx = B * g[0], fry = B * g[1], frz = B * g[2];
+        p.vx += frx; p.vy += fry; p.vz += frz;
+        p.x += sqrt_dt * frx; p.y += sqrt_dt * fry; p.z += sqrt_dt * frz;
+      }
+      fz0 = enforce<kSet>(b, c, p);
+      age_dt = ba.pos_dt;
     }
-    const bool fz0 = enforce<kSet>(b, c, p);
     const bool writer = lane == 0;
-    const double om = ba.mur * project_torque<kSet>(c, p, tq[0], tq[1], tq[2]);
-    const double dtheta = ba.dt * om + ba.stoch * g_rot;
-    rotate_director<kSet>(c, p, dtheta);
-    if (ba.velocity) rotate_velocity<kSet>(c, p, dtheta);
     if (ba.head_enforce) enforce<kSet>(b, c, p);
     if (writer) {
       cur.fx[s] = fx; cur.fy[s] = fy; cur.fz[s] = fz0 ? 0.0 : fz;
       if (AL) { cur.tx[s] = tq[0]; cur.ty[s] = tq[1]; cur.tz[s] = tq[2]; }
       cur.omega[s] = om;
-      cur.age[s] += ba.dt;
+      cur.age[s] += age_dt;
       cur.vx[s] = p.vx; cur.vy[s] = p.vy;
       if (!PLANAR) cur.vz[s] = p.vz;
       if (PLANAR) {

@@ -629,13 +629,19 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
 // and integrates it (k_brownian_fused), followed by the device-conditional
 // rebuild; chunks are captured as for the fused Langevin run.  On entry the
 // primary columns hold the loop-head state of a; on return that of b.
+// Either one "brownian" integrator, or exactly "brownian_align" and
+// "brownian_pos" (both over all particles, the reference's example confs).
 bool fused_brownian_eligible(samos_ctx* ctx, int flags) {
-  if ((flags & SAMOS_RUN_FAITHFUL) || ctx->extra.count != 0 || ctx->integs.size() != 1) return false;
-  const Integ& I = ctx->integs.begin()->second;
-  return I.name == "brownian" && needs_nlist(ctx) && brownian_fused_supported(pot_mask(ctx), al_mask(ctx));
+  if ((flags & SAMOS_RUN_FAITHFUL) || ctx->extra.count != 0) return false;
+  if (!needs_nlist(ctx) || !brownian_fused_supported(pot_mask(ctx), al_mask(ctx))) return false;
+  if (ctx->integs.size() == 1) return ctx->integs.begin()->second.name == "brownian";
+  return ctx->integs.size() == 2 && ctx->integs.count("brownian_align_all") && ctx->integs.count("brownian_pos_all");
 }
 
-int run_fused_brownian(samos_ctx* ctx, Integ& I, int a, int b, int steps, int flags) {
+int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
+  const bool pair = ctx->integs.size() == 2;
+  Integ& I = pair ? ctx->integs["brownian_align_all"] : ctx->integs.begin()->second;
+  Integ* P = pair ? &ctx->integs["brownian_pos_all"] : nullptr;
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   const bool planar = ctx->con.kind == 1 && ctx->nl.use_cell;
   const int n = ctx->n;
@@ -661,6 +667,10 @@ int run_fused_brownian(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     ba.temp = I.temp; ba.seed = I.seed; ba.clock = ctx->d_clock; ba.clock_slot = I.slot; ba.t_off = t_off;
     ba.parity = par; ba.head_enforce = t + 1 <= steps ? 1 : 0;
     ba.chunk = chunk; ba.slot = slot;
+    if (pair) {
+      ba.mode = 1;
+      ba.pos_dt = P->dt; ba.pos_mu = P->mu; ba.pos_temp = P->temp; ba.pos_seed = P->seed; ba.pos_slot = P->slot;
+    }
     // small systems (the step is then latency-bound): several threads per
     // slot, up to 16, while the launch stays under 64 Ki threads; larger
     // systems measured faster with one thread per slot
@@ -728,6 +738,7 @@ int run_fused_brownian(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     for (int k = 0; k < (planar ? 2 : 3); ++k)
       HIPCHK(hipMemcpyAsync(ctx->dcol[6 + k], ctx->altn[k], sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
   I.calls += (unsigned long long)(b - a);
+  if (P) P->calls += (unsigned long long)(b - a);
   // loop head of b: its enforce was applied with the positions; forces and torques
   if (b <= steps) loop_head_forces(ctx, nullptr, f32);
   HIPCHK(hipGetLastError());
@@ -1520,7 +1531,7 @@ int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags
     return check_overflow(ctx);
   }
   if (fused_brownian_eligible(ctx, flags)) {
-    if (int r = run_fused_brownian(ctx, ctx->integs.begin()->second, t_begin, t_end, steps, flags)) return r;
+    if (int r = run_fused_brownian(ctx, t_begin, t_end, steps, flags)) return r;
     ctx->time_step += t_end - t_begin;
     return check_overflow(ctx);
   }

@@ -108,6 +108,11 @@ hipError_t launch_rebuild_coop(hipStream_t st, Fields f, NListD nl, BoxD box, in
 // Fused Brownian step (IntegratorBrownian, src/integrators/integrator_brownian.cpp:42-117)
 // with the loop-head forces and torques of the same step; positions and
 // directors read from `cur` and written to `nxt`.
+// mode 0: IntegratorBrownian ("brownian"); mode 1: the pair of
+// IntegratorBrownianAlign then IntegratorBrownianPos over the same particles
+// (the map order of "brownian_align" and "brownian_pos"), the align half with
+// dt, mur, stoch, flip_p, nematic, seed and clock_slot, the position half with
+// the pos_* fields.
 struct BrownArgs {
   double dt, v0, mu, mur, stoch, flip_p;
   int nematic, velocity;
@@ -115,6 +120,11 @@ struct BrownArgs {
   uint32_t seed;
   const ClockD* clock;
   int clock_slot, t_off, parity, head_enforce;
+  int mode = 0;
+  double pos_dt = 0.0, pos_mu = 0.0;
+  TempD pos_temp{0, 0.0, 0.0, 1};
+  uint32_t pos_seed = 0;
+  int pos_slot = 0;
   ChunkCtl* chunk = nullptr;  // captured-chunk mode (null: direct launch with the parity flags)
   int slot = 0;
   int lanes = 1, lane_shift = 0;  // threads per slot (power of two up to 64) and its log2

@@ -540,6 +540,36 @@ def test_brownian_pos_align_self_propulsion(gpu):
     assert np.abs(directors(d) - directors(o)).max() <= 1e-9
 
 
+@pytest.mark.parametrize("case", ["sphere_polar", "plane_nematic_flips"])
+def test_fused_brownian_align_pos_pair(gpu, case):
+    # the production stage of the reference's example confs (brownian_align +
+    # brownian_pos with self propulsion), run as one fused launch per step in
+    # captured chunks
+    if case == "sphere_polar":
+        st = sc.sphere_disks(1500, R=14.0, seed=23)
+    else:
+        st = sc.plane_disks(2500, phi=0.8, seed=23)
+    d, o = both(gpu, st)
+    for s in (d, o):
+        s.nlist(2.4, 0.5)
+        s.pair_soft(10.0, 1.0 if case == "sphere_polar" else 2.0)
+        if case == "sphere_polar":
+            s.align_polar(1.0)
+            s.constraint("sphere", 14.0)
+        else:
+            s.align_nematic(0.5, 2.4)
+            s.constraint("plane")
+        s.self_propulsion(1.0)
+        s.set_timestep(0.01)
+        s.integrator_brownian(which="brownian_pos", mu=1.0, seed=3, temp=(0, 0.1, 0.1, 1))
+        s.integrator_brownian(which="brownian_align", nu=0.01, seed=4, nematic=case != "sphere_polar", tau=0.5)
+        s.run(299)
+    assert pos_err(positions(d), positions(o), st["box"], st["periodic"]) <= 1e-9
+    assert np.abs(directors(d) - directors(o)).max() <= 1e-9
+    assert np.abs(d.get("omega") - o.get("omega")).max() <= 1e-9
+    assert np.array_equal(d.get("age"), o.get("age"))
+
+
 @pytest.mark.parametrize("limits", [(0.01, 0.02), (None, None)])
 def test_nve_soft_polar_overlaps(gpu, limits):
     # the overlap-relaxation pre-stage of the reference's example confs:
