@@ -23,8 +23,15 @@
 //   * soft / LJ / polar / nematic: closed forms, Newton's third law,
 //     finite-difference gradients;
 //   * constraints: invariants (|x| = R, n.N = 0, |n| = 1, rotation inverses);
+//     cylinder, torus and ellipsoid (generic SHAKE) against closed-form
+//     projections with tol = 0 (tests/test_surfaces.py);
+//   * morse / gaussian / yukawa / soft_attractive: two-particle closed forms,
+//     the rcut shift, Newton's third law, finite-difference gradients, the
+//     soft_attractive coordination against a brute-force count
+//     (tests/test_pair_extra.py);
 //   * integrators: single-particle closed forms (free ABP drift, O-step decay,
-//     FIRE two-particle relaxation), equipartition with noise on;
+//     FIRE two-particle relaxation, NVE free flight, limits and LJ-pair energy
+//     conservation), equipartition with noise on, the linear temperature ramp;
 //   * the default noise stream is the project's counter-based Philox4x32-10
 //     (checked against the Random123 known-answer vectors), the one the device
 //     library draws.  A second stream restates the reference's own noise

@@ -13,7 +13,11 @@ function is pinned by checks chosen so that a plausible mistake fails one:
                             rotate by phi then -phi
   integrators               free ABP drift, BAOAB O-step decay at T = 0,
                             SPV/BBK free flight, FIRE two-particle relaxation,
-                            Langevin equipartition with noise on
+                            Langevin equipartition with noise on, NVE free
+                            flight / limits / LJ-pair energy conservation /
+                            angle limit, the linear temperature ramp
+  (further pair potentials: tests/test_pair_extra.py; cylinder, torus and
+   ellipsoid constraints: tests/test_surfaces.py)
   RNG                       Random123 philox4x32-10 known-answer vectors,
                             Box-Muller moments
   golden files              regression pins of the example-input outputs
