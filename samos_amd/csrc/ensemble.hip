@@ -109,11 +109,21 @@ __device__ __forceinline__ int block_scan(int v, int* lds, int& total) {
 
 // Work list of the replicas with rep_need set (MODE 0) or not yet stopped
 // (MODE 1).  One workgroup.
+// clear_rep_need / clear_need / zero_me (may be null): also clear the
+// per-replica rebuild requests, the global request and a counter, which the
+// FIRE iteration needs at this point (it saves separate launches).
 template <int MODE>
 __global__ void __launch_bounds__(kSelThreads) k_ens_select(const RepD* __restrict__ reps, int nrep,
-                                                            const int* rep_need, Work w, const int* need) {
+                                                            const int* rep_need, Work w, const int* need,
+                                                            int* clear_rep_need, int* clear_need, int* zero_me) {
   if (need && *need == 0) return;
   __shared__ int lds[kSelThreads / 64];
+  if (clear_rep_need)
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x) clear_rep_need[r] = 0;
+  if (threadIdx.x == 0) {
+    if (clear_need) *clear_need = 0;
+    if (zero_me) *zero_me = 0;
+  }
   int cs = 0, cp = 0, cb = 0;
   for (int base = 0; base < nrep; base += blockDim.x) {
     const int r = base + threadIdx.x;
@@ -589,7 +599,10 @@ __global__ void __launch_bounds__(kBlock) k_ens_fire_b(EState e, int b, const Re
 
 // Per-replica sums of columns 0-3 and the max of column 4 for each listed
 // replica: one workgroup per replica, a fixed summation tree.
-__global__ void __launch_bounds__(kBlock) k_ens_fire_reduce(const double* cols, long ld, RepD* reps, Work w) {
+__device__ void fire_update_one(RepD& R, int* remaining);
+
+__global__ void __launch_bounds__(kBlock) k_ens_fire_reduce(const double* cols, long ld, RepD* reps, Work w,
+                                                            int* remaining) {
   if ((int)blockIdx.x >= w.cnt[0]) return;
   RepD& R = reps[w.rep[blockIdx.x]];
   __shared__ double lds[5][kBlock / 64];
@@ -614,6 +627,8 @@ __global__ void __launch_bounds__(kBlock) k_ens_fire_reduce(const double* cols, 
     for (int wv = 0; wv < kBlock / 64; ++wv)
       for (int q = 0; q < 5; ++q) t[q] = q < 4 ? t[q] + lds[q][wv] : fmax(t[q], lds[q][wv]);
     R.P = t[0]; R.F2 = t[1]; R.V2 = t[2]; R.E = 0.5 * t[3]; R.Fmax = t[4];
+    // every listed replica is running (not frozen): its FIRE update follows
+    fire_update_one(R, remaining);
   }
 }
 
@@ -621,10 +636,7 @@ __global__ void __launch_bounds__(kBlock) k_ens_fire_reduce(const double* cols, 
 // (src/integrators/integrator_fire.cpp:97-147), preceded by the stop rule
 // max |F| < stop_fmax, which stops a converged replica.  *remaining counts the
 // replicas still running.
-__global__ void k_ens_fire_update(RepD* reps, int nrep, int* remaining) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrep) return;
-  RepD& R = reps[r];
+__device__ void fire_update_one(RepD& R, int* remaining) {
   if (R.frozen) return;
   R.iters++;
   if (R.Fmax < R.stop_fmax) {
@@ -800,7 +812,7 @@ void ens_rebuild(samos_ens* E, const int* need) {
   const int g = grid_cap(E->n);
   const int gt = (int)std::min<long long>(E->ntiles, 1024);
   hipLaunchKernelGGL(k_ens_select<0>, dim3(1), dim3(kSelThreads), 0, E->st, E->d_rep, E->nrep, E->rep_need, E->wr,
-                     need);
+                     need, (int*)nullptr, (int*)nullptr, (int*)nullptr);
   hipLaunchKernelGGL(k_ens_bin, dim3(g), dim3(kBlock), 0, E->st, E->e, b, E->d_rep, E->wr, E->nl, need);
   hipLaunchKernelGGL(k_ens_scan_tiles, dim3(gt), dim3(kTileThreads), 0, E->st, E->d_rep, E->wr, E->nl, need);
   hipLaunchKernelGGL(k_ens_scan_add, dim3(gt), dim3(kTileThreads), 0, E->st, E->d_rep, E->wr, E->nl, need);
@@ -912,11 +924,11 @@ void brownian_iter(samos_ens* E, int j, int head, int last) {
 // the running-replica counter first so that it holds this iteration's count.
 void fire_iter(samos_ens* E, bool count_remaining) {
   const dim3 g(grid_cap(E->n)), bl(kBlock);
-  if (count_remaining) hipLaunchKernelGGL(k_ens_advance, dim3(1), dim3(64), 0, E->st, E->d_call, 0, E->d_flags + 8);
+  // the running-replica work list; clears the rebuild requests (and the
+  // running-replica counter when counting)
   hipLaunchKernelGGL(k_ens_select<1>, dim3(1), dim3(kSelThreads), 0, E->st, E->d_rep, E->nrep, E->rep_need, E->wa,
-                     (const int*)nullptr);
+                     (const int*)nullptr, E->rep_need, E->d_flags, count_remaining ? E->d_flags + 8 : (int*)nullptr);
   hipLaunchKernelGGL(k_ens_fire_a, g, bl, 0, E->st, E->e, E->cur, E->d_rep, E->wa);
-  hipLaunchKernelGGL(k_ens_clear_need, dim3(rep_grid(E)), bl, 0, E->st, E->rep_need, E->nrep, E->d_flags);
   auto ev = prof_begin(E);
   if (E->batch == 8)
     hipLaunchKernelGGL(k_ens_fire_b<8>, g, bl, 0, E->st, E->e, E->cur, E->d_rep, E->wa, E->nl, E->cols, (long)E->np,
@@ -925,8 +937,9 @@ void fire_iter(samos_ens* E, bool count_remaining) {
     hipLaunchKernelGGL(k_ens_fire_b<4>, g, bl, 0, E->st, E->e, E->cur, E->d_rep, E->wa, E->nl, E->cols, (long)E->np,
                        E->rep_need, E->d_flags);
   prof_end(E, ev, E->ev_step);
-  hipLaunchKernelGGL(k_ens_fire_reduce, dim3(E->nrep), bl, 0, E->st, E->cols, (long)E->np, E->d_rep, E->wa);
-  hipLaunchKernelGGL(k_ens_fire_update, dim3(rep_grid(E)), bl, 0, E->st, E->d_rep, E->nrep, E->d_flags + 8);
+  // per-replica sums, then IntegratorFIRE's scalar update of that replica
+  hipLaunchKernelGGL(k_ens_fire_reduce, dim3(E->nrep), bl, 0, E->st, E->cols, (long)E->np, E->d_rep, E->wa,
+                     E->d_flags + 8);
   hipLaunchKernelGGL(k_ens_fire_mix, g, bl, 0, E->st, E->e, E->cur, E->d_rep, E->wa);
   ev = prof_begin(E);
   ens_rebuild(E, E->d_flags);
