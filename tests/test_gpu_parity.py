@@ -348,6 +348,40 @@ def test_langevin_plane_n_squared_list(gpu):
     assert np.array_equal(d.pairs(), o.pairs())
 
 
+@pytest.mark.parametrize("integrator", ["langevin", "brownian"])
+def test_fused_3d_periodic_box_without_constraint(gpu, integrator):
+    # constraint none in a periodic 3-D box: the fused steps with the 3-D
+    # position layout, periodic wraps in every direction and image flags
+    rng = np.random.default_rng(12)
+    n, L = 3000, 24.0
+    g = np.arange(-L / 2, L / 2, L / 15)[:15]
+    x, y, z = np.meshgrid(g, g, g[:14], indexing="ij")
+    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], 1)[:n] + rng.uniform(-0.1, 0.1, (n, 3))
+    d3 = rng.normal(size=(n, 3))
+    d3 /= np.linalg.norm(d3, axis=1)[:, None]
+    st = {"n": n, "box": (L, L, L), "periodic": True,
+          "cols": {"x": pts[:, 0], "y": pts[:, 1], "z": pts[:, 2], "nx": d3[:, 0], "ny": d3[:, 1], "nz": d3[:, 2],
+                   "vx": rng.normal(0, 0.5, n), "vy": rng.normal(0, 0.5, n), "vz": rng.normal(0, 0.5, n)}}
+    d, o = both(gpu, st)
+    for s in (d, o):
+        s.nlist(2.5, 0.4)
+        s.constraint("none")
+        if integrator == "langevin":
+            s.pair_lj(1.0, 1.0, 2.5, shifted=True)
+            s.set_timestep(0.002)
+            s.integrator_langevin(gamma=1.0, seed=13, temp=(0, 1.0, 1.0, 1))
+        else:
+            s.pair_soft(1.0, 1.2)
+            s.align_polar(0.5, 2.0)
+            s.set_timestep(0.01)
+            s.integrator_brownian(v0=1.0, nu=0.1, seed=14, temp=(0, 0.1, 0.1, 1))
+        s.run(299)
+    assert pos_err(positions(d), positions(o), st["box"], True) <= 1e-9
+    for k in ("ix", "iy", "iz"):
+        assert np.array_equal(d.get(k), o.get(k)), k
+    assert np.array_equal(d.pairs(), o.pairs())
+
+
 @pytest.mark.parametrize("slots", [2, 6, 24])
 def test_chunked_fused_steps_equal_direct_launches(gpu, slots):
     # captured chunks stop at the step whose positions need a rebuild and
