@@ -348,6 +348,29 @@ def test_langevin_plane_n_squared_list(gpu):
     assert np.array_equal(d.pairs(), o.pairs())
 
 
+def test_fused_langevin_types_and_radii(gpu):
+    # the multi-type path of the fused Langevin step (per-type LJ and soft
+    # tables, LJ sigma from the radii) in captured chunks, noise on
+    st = sc.lj_lattice(48, rho=0.6, seed=35)
+    rng = np.random.default_rng(35)
+    st["cols"]["type"] = rng.integers(1, 3, st["n"]).astype(np.int32)
+    st["cols"]["radius"] = rng.uniform(0.45, 0.55, st["n"])
+    d, o = both(gpu, st)
+    for s in (d, o):
+        s.nlist(2.6, 0.3)
+        s.pair_lj(1.0, 1.0, 2.5, shifted=True, use_radii=True)
+        s.pair_type_param("lj", 1, 2, 0.6, 1.0, 2.2)
+        s.pair_soft(2.0, 0.9)
+        s.pair_type_param("soft", 2, 2, 1.0, 0.8)
+        s.constraint("plane")
+        s.set_timestep(0.002)
+        s.integrator_langevin(gamma=1.0, seed=15, temp=(0, 0.8, 0.8, 1))
+        s.run(299)
+    assert pos_err(positions(d), positions(o), st["box"], True) <= 1e-9
+    assert np.abs(d.get("vx") - o.get("vx")).max() <= 1e-9
+    assert np.array_equal(d.pairs(), o.pairs())
+
+
 @pytest.mark.parametrize("integrator", ["langevin", "brownian"])
 def test_fused_3d_periodic_box_without_constraint(gpu, integrator):
     # constraint none in a periodic 3-D box: the fused steps with the 3-D
