@@ -1003,6 +1003,7 @@ int samos_create(samos_ctx** out, int device, int n, double lx, double ly, doubl
   std::vector<int> ione(n, 1);
   if (n > 0) {
     hipMemcpyAsync(ctx->dcol[17], ones.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st);
+    ctx->nl.r_uni = 1.0;  // every radius is the default 1
     hipMemcpyAsync(ctx->dcol[18], ones.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st);
     hipMemcpyAsync(ctx->icol[0], ione.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->st);
     hipMemcpyAsync(ctx->icol[5], ctx->h_id.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->st);
@@ -1047,6 +1048,11 @@ int samos_set_field(samos_ctx* ctx, int field, const void* host) {
     for (int s = 0; s < n; ++s) tmp[s] = h[ctx->h_id[s]];
     HIPCHK(hipMemcpyAsync(ctx->dcol[field], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
     if (field == SAMOS_MASS) ctx->mass_known = false;
+    if (field == SAMOS_RADIUS) {
+      bool uni = true;
+      for (int s = 1; s < n && uni; ++s) uni = tmp[s] == tmp[0];
+      ctx->nl.r_uni = uni ? tmp[0] : -1.0;
+    }
     if (field <= 2) HIPCHK(hipMemcpyAsync(ctx->alt[field], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
   } else if (field >= SAMOS_TYPE && field < SAMOS_TYPE + 5) {
     const int* h = static_cast<const int*>(host);
@@ -1101,7 +1107,9 @@ int samos_nlist_create(samos_ctx* ctx, double rcut, double pad) {
   ctx->rcut = rcut;
   ctx->pad = pad;
   NListD& nl = ctx->nl;
+  const double r_uni = nl.r_uni;  // a property of the particles, kept across lists
   nl = NListD{};
+  nl.r_uni = r_uni;
   const double cut = rcut + pad;
   nl.cut2 = cut * cut;
   nl.pad2q = 0.25 * pad * pad;

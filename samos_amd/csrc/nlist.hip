@@ -217,7 +217,10 @@ __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
     double xi, yi, zi;
     if (PLANAR) { const double2 p = f.xy[s]; xi = p.x; yi = p.y; zi = f.z[s]; }
     else { xi = f.x[s]; yi = f.y[s]; zi = f.z[s]; }
-    const double ri = f.radius[s];
+    // one common radius (the usual case): the coordination test needs no
+    // per-candidate radius gather
+    const bool runi = nl.r_uni >= 0.0;
+    const double ri = runi ? nl.r_uni : f.radius[s];
     const int c = nl.cell_of[s];
     const int ci = c / (nl.ncy * nl.ncz), cj = (c / nl.ncz) % nl.ncy, ck = c % nl.ncz;
     int k = 0, coord = 0;
@@ -256,7 +259,7 @@ __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
             for (int u = 0; u < U; ++u) {
               if (PLANAR) { const double2 p = f.xy[t[u]]; xj[u] = p.x; yj[u] = p.y; zj[u] = zi; }
               else { xj[u] = f.x[t[u]]; yj[u] = f.y[t[u]]; zj[u] = f.z[t[u]]; }
-              rj[u] = f.radius[t[u]];
+              rj[u] = runi ? nl.r_uni : f.radius[t[u]];
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {

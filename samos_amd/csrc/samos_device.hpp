@@ -88,6 +88,7 @@ struct NListD {
   double *ox, *oy, *oz;  // positions at the last build
   int* flags;         // [0],[1] rebuild request by step parity, [2] overflow, [3] largest cell
                       // count, [4] builds, [5] largest neighbour count
+  double r_uni = -1.0;  // the common particle radius, or -1: the build then reads each radius
 };
 
 // Pair interactions evaluated in one pass over the neighbour list.
