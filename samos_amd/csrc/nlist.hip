@@ -37,31 +37,73 @@ __device__ __forceinline__ int gsize() { return gridDim.x * blockDim.x; }
 // CellList::populate wrap + get_cell_idx (src/system/cell_list.cpp:72-104).
 // The index is clamped to the grid: the reference indexes one past the grid for
 // a particle exactly on the high face.
+// Adds 1 to counter[key] for every active lane, one atomic per distinct key
+// in the wavefront (slots in Morton order share cells in runs).  Every lane
+// of the wavefront must call it.
+__device__ __forceinline__ void wave_count(int* counter, int key, bool active) {
+  unsigned long long todo = __ballot(active);
+  const int lane = threadIdx.x & 63;
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const int k0 = __shfl(key, leader, 64);
+    const unsigned long long same = todo & __ballot(key == k0);
+    if (lane == leader) atomicAdd(&counter[k0], __popcll(same));
+    todo &= ~same;
+  }
+}
+
+// Takes one from counter[key] per active lane and returns each lane a
+// distinct index in [new value, old value): one atomic per distinct key.
+__device__ __forceinline__ int wave_take(int* counter, int key, bool active) {
+  unsigned long long todo = __ballot(active);
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int mine = 0;
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const int k0 = __shfl(key, leader, 64);
+    const unsigned long long same = todo & __ballot(key == k0);
+    int old = 0;
+    if (lane == leader) old = atomicSub(&counter[k0], __popcll(same));
+    old = __shfl(old, leader, 64);
+    if (active && key == k0) mine = old - 1 - __popcll(same & below);
+    todo &= ~same;
+  }
+  return mine;
+}
+
 template <bool PLANAR>
 __device__ void bin_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
-  for (int s = gtid(); s < n; s += gsize()) {
-    double x, y, z;
-    if (PLANAR) { const double2 p = f.xy[s]; x = p.x; y = p.y; z = f.z[s]; }
-    else { x = f.x[s]; y = f.y[s]; z = f.z[s]; }
-    if (b.periodic) {
-      bool w = false;
-      if (x < b.xlo) { x += b.Lx; w = true; } else if (x > b.xhi) { x -= b.Lx; w = true; }
-      if (y < b.ylo) { y += b.Ly; w = true; } else if (y > b.yhi) { y -= b.Ly; w = true; }
-      if (z < b.zlo) { z += b.Lz; w = true; } else if (z > b.zhi) { z -= b.Lz; w = true; }
-      if (w) {
-        f.x[s] = x; f.y[s] = y; f.z[s] = z;
-        if (f.xy) f.xy[s] = make_double2(x, y);
+  // whole wavefronts step together so the per-cell counts can be aggregated
+  const int lane = threadIdx.x & 63;
+  for (int s0 = gtid() - lane; s0 < n; s0 += gsize()) {
+    const int s = s0 + lane;
+    const bool active = s < n;
+    int c = 0;
+    if (active) {
+      double x, y, z;
+      if (PLANAR) { const double2 p = f.xy[s]; x = p.x; y = p.y; z = f.z[s]; }
+      else { x = f.x[s]; y = f.y[s]; z = f.z[s]; }
+      if (b.periodic) {
+        bool w = false;
+        if (x < b.xlo) { x += b.Lx; w = true; } else if (x > b.xhi) { x -= b.Lx; w = true; }
+        if (y < b.ylo) { y += b.Ly; w = true; } else if (y > b.yhi) { y -= b.Ly; w = true; }
+        if (z < b.zlo) { z += b.Lz; w = true; } else if (z > b.zhi) { z -= b.Lz; w = true; }
+        if (w) {
+          f.x[s] = x; f.y[s] = y; f.z[s] = z;
+          if (f.xy) f.xy[s] = make_double2(x, y);
+        }
       }
+      int i = (int)((x - b.xlo) / nl.wx);
+      int j = (int)((y - b.ylo) / nl.wy);
+      int k = (int)((z - b.zlo) / nl.wz);
+      i = min(max(i, 0), nl.ncx - 1);
+      j = min(max(j, 0), nl.ncy - 1);
+      k = min(max(k, 0), nl.ncz - 1);
+      c = nl.ncy * nl.ncz * i + nl.ncz * j + k;
+      nl.cell_of[s] = c;
     }
-    int i = (int)((x - b.xlo) / nl.wx);
-    int j = (int)((y - b.ylo) / nl.wy);
-    int k = (int)((z - b.zlo) / nl.wz);
-    i = min(max(i, 0), nl.ncx - 1);
-    j = min(max(j, 0), nl.ncy - 1);
-    k = min(max(k, 0), nl.ncz - 1);
-    const int c = nl.ncy * nl.ncz * i + nl.ncz * j + k;
-    nl.cell_of[s] = c;
-    atomicAdd(&nl.count[c], 1);
+    wave_count(nl.count, c, active);
   }
 }
 
@@ -130,10 +172,13 @@ __device__ void scan_add_tile(NListD& nl, int tile, int ntiles, int* lds) {
 
 // Counts return to zero as members are placed, so the next build starts clean.
 __device__ void scatter_phase(NListD& nl, int n) {
-  for (int s = gtid(); s < n; s += gsize()) {
-    const int c = nl.cell_of[s];
-    const int pos = nl.start[c] + atomicSub(&nl.count[c], 1) - 1;
-    nl.members[pos] = s;
+  const int lane = threadIdx.x & 63;
+  for (int s0 = gtid() - lane; s0 < n; s0 += gsize()) {
+    const int s = s0 + lane;
+    const bool active = s < n;
+    const int c = active ? nl.cell_of[s] : 0;
+    const int r = wave_take(nl.count, c, active);
+    if (active) nl.members[nl.start[c] + r] = s;
   }
 }
 
