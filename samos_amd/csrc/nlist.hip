@@ -257,7 +257,7 @@ template <bool PLANAR>
 __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
   bool overflow = false;
   int kmax = 0;
-  constexpr int U = 4;
+  constexpr int U = 8;
   for (int s = xcd_block() * blockDim.x + threadIdx.x; s < n; s += gsize()) {
     double xi, yi, zi;
     if (PLANAR) { const double2 p = f.xy[s]; xi = p.x; yi = p.y; zi = f.z[s]; }
