@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
       need = !(ddx * ddx + ddy * ddy + ddz * ddz < nl.pad2q);
     }
   }
-  if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomicOr(ck ? &ck->req[ba.slot & 1] : &nl.flags[ba.parity], 1);
+  if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomic_set_once(ck ? &ck->req[ba.slot & 1] : &nl.flags[ba.parity]);
 }
 
 template <int POT, int AL, bool F32, bool PLANAR>

@@ -280,7 +280,7 @@ __global__ void k_ens_sort(const RepD* __restrict__ reps, Work w, NListD nl, con
     }
   }
   for (int o = 32; o > 0; o >>= 1) local_max = max(local_max, __shfl_xor(local_max, o, 64));
-  if ((threadIdx.x & 63) == 0 && local_max > 0) atomicMax(&nl.flags[3], local_max);
+  if ((threadIdx.x & 63) == 0 && local_max > 0) atomic_max_if_above(&nl.flags[3], local_max);
 }
 
 // Cell a is in the periodic stencil of cell b along one axis of nc >= 3 cells.
@@ -351,7 +351,7 @@ __global__ void __launch_bounds__(kBlock) k_ens_build(EState e, int b, const Rep
   const bool any_over = __ballot(overflow) != 0;
   if ((threadIdx.x & 63) == 0) {
     if (any_over) atomicOr(&nl.flags[2], 1);
-    if (kmax > 0) atomicMax(&nl.flags[5], kmax);
+    if (kmax > 0) atomic_max_if_above(&nl.flags[5], kmax);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&nl.flags[4], 1);
 }

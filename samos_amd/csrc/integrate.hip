@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
     }
   }
   if (PHASE != 2) {
-    if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomicOr(ck ? &ck->req[fa.slot & 1] : &nl.flags[parity], 1);
+    if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomic_set_once(ck ? &ck->req[fa.slot & 1] : &nl.flags[parity]);
   }
 }
 

@@ -235,7 +235,7 @@ __device__ void sort_phase(NListD& nl) {
     }
   }
   for (int o = 32; o > 0; o >>= 1) local_max = max(local_max, __shfl_xor(local_max, o, 64));
-  if ((threadIdx.x & 63) == 0 && local_max > 0) atomicMax(&nl.flags[3], local_max);
+  if ((threadIdx.x & 63) == 0 && local_max > 0) atomic_max_if_above(&nl.flags[3], local_max);
 }
 
 // Flags: [2] overflow, [4] builds, [5] largest neighbour count.  One atomic per
@@ -245,7 +245,7 @@ __device__ __forceinline__ void build_epilogue(NListD& nl, bool overflow, int km
   const bool any_over = __ballot(overflow) != 0;
   if ((threadIdx.x & 63) == 0) {
     if (any_over) atomicOr(&nl.flags[2], 1);
-    if (kmax > 0) atomicMax(&nl.flags[5], kmax);
+    if (kmax > 0) atomic_max_if_above(&nl.flags[5], kmax);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&nl.flags[4], 1);
 }
@@ -433,7 +433,7 @@ __global__ void k_need_update(Fields f, NListD nl, BoxD b, int n, int* req_out) 
     min_image(b, dx, dy, dz);
     need |= !(dx * dx + dy * dy + dz * dz < nl.pad2q);
   }
-  if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomicOr(req_out, 1);
+  if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomic_set_once(req_out);
 }
 
 __device__ __forceinline__ unsigned long long spread3(unsigned long long v) {

@@ -51,6 +51,18 @@ __device__ __forceinline__ int xcd_block() {
   return x * q + min(x, r) + (blockIdx.x >> 3);
 }
 
+// Device-scope atomics on one address from thousands of wavefronts are
+// serialised where the address lives.  These flags only grow within a launch
+// (a running maximum; a 0 -> 1 request), so a plain read that already shows
+// the value the atomic would leave makes the atomic a no-op and it is skipped.
+// A stale read can only be smaller, which costs the atomic, never a result.
+__device__ __forceinline__ void atomic_max_if_above(int* p, int v) {
+  if (v > *(volatile int*)p) atomicMax(p, v);
+}
+__device__ __forceinline__ void atomic_set_once(int* p) {
+  if (*(volatile int*)p == 0) atomicOr(p, 1);
+}
+
 // The x and y steps of min_image, for pairs with dz = 0.
 __device__ __forceinline__ void min_image_xy(const BoxD& b, double& dx, double& dy) {
   if (dx > b.xhi) dx -= b.Lx; else if (dx < b.xlo) dx += b.Lx;

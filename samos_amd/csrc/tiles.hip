@@ -62,7 +62,7 @@ __device__ __forceinline__ int block_exscan(int v, int* lds4, int& total) {
 
 __device__ __forceinline__ void wave_max_to(int v, int* dst) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(dst, v);
+  if ((threadIdx.x & 63) == 0 && v > 0) atomic_max_if_above(dst, v);
 }
 
 // counter[key] += 1 for every active lane with one atomic per distinct key
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_bin(TileD td, int par) {
       far |= (di > 1 || dj > 1);
     }
   }
-  if (__ballot(far) != 0 && (threadIdx.x & 63) == 0) atomicOr(&td.ctl->far, 1);
+  if (__ballot(far) != 0 && (threadIdx.x & 63) == 0) atomic_set_once(&td.ctl->far);
 }
 
 // Build, kernel 2 (a range of kTilesPerBlock tiles per workgroup): the tile
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_cells(TileD td, int par) 
       }
     }
   }
-  if (__ballot(far) != 0 && (threadIdx.x & 63) == 0) atomicOr(&td.ctl->far, 1);
+  if (__ballot(far) != 0 && (threadIdx.x & 63) == 0) atomic_set_once(&td.ctl->far);
 }
 
 // First build of a run only: members by atomic scatter (counts return to
