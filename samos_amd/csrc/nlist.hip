@@ -53,23 +53,27 @@ __device__ __forceinline__ void wave_count(int* counter, int key, bool active) {
 }
 
 // Takes one from counter[key] per active lane and returns each lane a
-// distinct index in [new value, old value): one atomic per distinct key.
+// distinct index in [new value, old value): one atomic per distinct key.  The
+// groups of equal keys are found first (register work only), then every
+// group's first lane issues its atomic at once, so the wavefront waits for one
+// round trip instead of one per distinct key.
 __device__ __forceinline__ int wave_take(int* counter, int key, bool active) {
   unsigned long long todo = __ballot(active);
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  int mine = 0;
+  int my_leader = lane;
+  unsigned long long my_same = 0;
   while (todo) {
     const int leader = __builtin_ctzll(todo);
     const int k0 = __shfl(key, leader, 64);
     const unsigned long long same = todo & __ballot(key == k0);
-    int old = 0;
-    if (lane == leader) old = atomicSub(&counter[k0], __popcll(same));
-    old = __shfl(old, leader, 64);
-    if (active && key == k0) mine = old - 1 - __popcll(same & below);
+    if (active && key == k0) { my_leader = leader; my_same = same; }
     todo &= ~same;
   }
-  return mine;
+  int old = 0;
+  if (active && lane == my_leader) old = atomicSub(&counter[key], __popcll(my_same));
+  old = __shfl(old, my_leader, 64);
+  return active ? old - 1 - __popcll(my_same & below) : 0;
 }
 
 template <bool PLANAR>
