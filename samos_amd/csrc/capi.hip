@@ -1612,13 +1612,20 @@ namespace {
 
 __global__ void k_observe(Fields f, int n, int what, double* col, long ld, unsigned long long* maxbits) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (what == 0) {
+    // |F| >= 0 (or NaN), so the largest bit pattern is the largest value and a
+    // NaN wins as it would in a scalar max; one atomic per wavefront
+    unsigned long long u = 0;
+    if (s < n) u = (unsigned long long)__double_as_longlong(sqrt(f.fx[s] * f.fx[s] + f.fy[s] * f.fy[s] + f.fz[s] * f.fz[s]));
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long t = __shfl_xor(u, o, 64);
+      u = t > u ? t : u;
+    }
+    if ((threadIdx.x & 63) == 0 && u != 0) atomicMax(maxbits, u);
+    return;
+  }
   if (s >= n) return;
   switch (what) {
-    case 0: {
-      const double v = sqrt(f.fx[s] * f.fx[s] + f.fy[s] * f.fy[s] + f.fz[s] * f.fz[s]);
-      atomicMax(maxbits, (unsigned long long)__double_as_longlong(v));
-      break;
-    }
     case 1: col[s] = f.fx[s] * f.fx[s] + f.fy[s] * f.fy[s] + f.fz[s] * f.fz[s]; break;
     // LogKineticEng sums 0.5 v^2 / m (src/log/log_kinetic_eng.hpp:58-64)
     case 2: col[s] = 0.5 * (f.vx[s] * f.vx[s] + f.vy[s] * f.vy[s] + f.vz[s] * f.vz[s]) / f.mass[s]; break;
