@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
       cur.fx[s] = fx; cur.fy[s] = fy; cur.fz[s] = fz0 ? 0.0 : fz;
       if (AL) { cur.tx[s] = tq[0]; cur.ty[s] = tq[1]; cur.tz[s] = tq[2]; }
       cur.omega[s] = om;
-      cur.age[s] += age_dt;
+      if (!(ba.age && ba.age->uniform)) cur.age[s] += age_dt;
       cur.vx[s] = p.vx; cur.vy[s] = p.vy;
       if (!PLANAR) cur.vz[s] = p.vz;
       if (PLANAR) {

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -97,8 +98,15 @@ struct samos_ctx {
   bool run_begun = false;       // loop head of t = 0 of the current run done
   bool mass_known = false;      // mass_u valid
   double mass_u = -1.0;         // common particle mass, or -1
-  bool age_uniform = false;     // fused range in progress with one common age
-  double age_u = 0.0;           // that age
+  AgeCtl* d_age = nullptr;      // common-age state of the fused ranges
+  bool age_checked = false;     // d_age describes the age column as it is now
+  // Instantiated step-chunk graphs, reused across run ranges.  The key holds
+  // every value the captured nodes bake in (pointers, sizes, parameters), so
+  // an equal key means an identical graph.
+  struct GraphEntry { std::string key; hipGraphExec_t exec; unsigned long long used; };
+  std::vector<GraphEntry> graphs;
+  unsigned long long graph_tick = 0;
+  long long graph_stats[2] = {0, 0};  // instantiations, reuses
   // cell-tiled engine of planar fused runs (allocated on first use)
   TileD td{};
   void* tile_mem = nullptr;
@@ -200,6 +208,133 @@ void set_tab(std::vector<double>& t, int t1, int t2, double a, double b, double 
   const int nn = kMaxTypes * kMaxTypes;
   int i = (t1 - 1) * kMaxTypes + (t2 - 1), j = (t2 - 1) * kMaxTypes + (t1 - 1);
   t[i] = t[j] = a; t[nn + i] = t[nn + j] = b; t[2 * nn + i] = t[2 * nn + j] = c;
+}
+
+// ---- cached step-chunk graphs ----------------------------------------------
+// Byte key of scalars and pointers, added one member at a time (no struct
+// padding enters it).
+struct KeyBuf {
+  std::string s;
+  template <class T> KeyBuf& operator<<(const T& v) {
+    s.append(reinterpret_cast<const char*>(&v), sizeof(T));
+    return *this;
+  }
+};
+
+KeyBuf& operator<<(KeyBuf& k, const TempD& t) { return k << t.kind << t.tmin << t.tmax << t.steps; }
+
+// Everything of the context a captured step sequence reads on the host side:
+// column and scratch pointers, sizes, list geometry, box, constraint and the
+// interaction switches (the parameter tables are read by the kernels from
+// device memory).
+void key_context(KeyBuf& k, const samos_ctx* c) {
+  k << c->n << c->np << c->ntypes;
+  for (int i = 0; i < kNumDoubleFields; ++i) k << c->dcol[i];
+  for (int i = 0; i < kNumIntFields; ++i) k << c->icol[i];
+  for (int i = 0; i < 3; ++i) k << c->alt[i] << c->altn[i];
+  k << c->xy[0] << c->xy[1];
+  const NListD& l = c->nl;
+  k << c->has_nlist << l.ncx << l.ncy << l.ncz << l.ncell << l.wx << l.wy << l.wz << l.cut2 << l.pad2q << l.use_cell
+    << l.np << l.stride << l.cell_of << l.count << l.start << l.members << l.bsum << l.nbr << l.nnb << l.ox << l.oy
+    << l.oz << l.flags << l.r_uni;
+  const BoxD& b = c->box;
+  k << b.Lx << b.Ly << b.Lz << b.xlo << b.xhi << b.ylo << b.yhi << b.zlo << b.zhi << b.periodic;
+  const ConstraintD& q = c->con;
+  k << q.kind << q.r << q.zpos << q.unlimited << q.q0 << q.q1 << q.q2 << q.max_iter << q.tol;
+  k << c->has_soft << c->has_lj << c->has_pol << c->has_nem << c->has_sp << c->soft_radii << c->lj_radii
+    << c->lj_shifted << c->sp_alpha << c->d_tab;
+  const ExtraD& e = c->extra;
+  k << e.count;
+  for (int i = 0; i < e.count; ++i) {
+    k << e.kind[i] << e.radii[i] << e.shifted[i];
+    for (int j = 0; j < 8; ++j) k << e.q[i][j];
+  }
+  k << c->d_flags << c->d_clock << c->d_chunk << c->d_age << c->d_en << c->gather_batch << c->coop;
+}
+
+void key_integ(KeyBuf& k, const Integ& I) {
+  k << I.name.size();
+  k.s += I.name;
+  k << I.slot << I.dt << I.temp << I.v0 << I.nu << I.mu << I.mur << I.stoch << I.flip_p << I.nematic << I.velocity
+    << I.seed << I.gamma << I.method << I.R << I.limit << I.theta_limit;
+}
+
+// The instantiated graph of `key`, recorded by `record` on first use.  The
+// least recently used entry is dropped beyond 24.
+int cached_graph(samos_ctx* ctx, const std::string& key, const std::function<int()>& record, hipGraphExec_t* out) {
+  for (auto& g : ctx->graphs)
+    if (g.key == key) {
+      g.used = ++ctx->graph_tick;
+      ctx->graph_stats[1]++;
+      *out = g.exec;
+      return 0;
+    }
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+  const int rc = record();
+  const hipError_t ec = hipStreamEndCapture(ctx->st, &g);
+  if (rc) { if (g) hipGraphDestroy(g); return rc; }
+  HIPCHK(ec);
+  hipGraphExec_t ge = nullptr;
+  const hipError_t ir = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  HIPCHK(ir);
+  if (ctx->graphs.size() >= 24) {
+    auto lru = std::min_element(ctx->graphs.begin(), ctx->graphs.end(),
+                                [](const samos_ctx::GraphEntry& a, const samos_ctx::GraphEntry& b) { return a.used < b.used; });
+    hipGraphExecDestroy(lru->exec);
+    ctx->graphs.erase(lru);
+  }
+  ctx->graphs.push_back({key, ge, ++ctx->graph_tick});
+  ctx->graph_stats[0]++;
+  *out = ge;
+  return 0;
+}
+
+// Slots per captured chunk for a range needing `need` slots: the configured
+// count, or the smallest power of two (at least 2) that covers a shorter range.
+int chunk_size(int configured, int need) {
+  int m = 2;
+  while (m < need && m < configured) m <<= 1;
+  return std::min(m, configured);
+}
+
+// The chunk graph of M slots for the sequence keyed by `base`.  When the
+// sequence has no instantiated chunk yet, every power-of-two size up to the
+// configured one is instantiated together, so a later range of another length
+// finds its chunk ready instead of paying for an instantiation.
+int chunk_graphs(samos_ctx* ctx, const std::string& base, int M, const std::function<int(int)>& record,
+                 hipGraphExec_t* out) {
+  auto key = [&](int m) { KeyBuf k; k << m; return base + k.s; };
+  const std::string want = key(M);
+  bool have = false;
+  for (auto& g : ctx->graphs) have |= g.key == want;
+  if (!have)
+    for (int m = 2; m <= ctx->chunk_slots; m <<= 1)
+      if (m != M) {
+        hipGraphExec_t ge = nullptr;
+        if (int r = cached_graph(ctx, key(m), [&]() { return record(m); }, &ge)) return r;
+      }
+  return cached_graph(ctx, want, [&]() { return record(M); }, out);
+}
+
+// Launches chunks of the graph until the chunk base reaches `end`: each chunk
+// runs at least its first slot, and launches go in batches sized by the slots
+// per chunk seen so far.  Returns the base reached in *t.
+int replay_chunks(samos_ctx* ctx, hipGraphExec_t ge, int M, int end, int* t) {
+  double per = 0.5 * M;
+  int* h_base = reinterpret_cast<int*>(ctx->h_out);
+  while (*t < end) {
+    const int k = std::max(1, std::min(256, (int)((end - *t) / per)));
+    for (int i = 0; i < k; ++i) HIPCHK(hipGraphLaunch(ge, ctx->st));
+    HIPCHK(hipMemcpyAsync(h_base, &ctx->d_chunk->base, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    const int tn = *h_base;
+    if (tn <= *t || tn > end) return fail(ctx, "captured step chunks made no progress");
+    per = std::max(1.0, (double)(tn - *t) / k);
+    *t = tn;
+  }
+  return 0;
 }
 
 // ---- profiling ----------------------------------------------------------
@@ -352,6 +487,7 @@ void check_and_rebuild(samos_ctx* ctx) {
 // j >= 0: the step is the j-th of a captured chunk; its noise call and run
 // step are read from the device clock plus j instead of being baked in.
 int integrate_one(samos_ctx* ctx, Integ& I, int run_step, int j = -1) {
+  ctx->age_checked = false;  // these kernels advance each slot's age
   Fields f = fields(ctx);
   const int n = ctx->n;
   const bool clocked = j >= 0;
@@ -463,7 +599,7 @@ int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int rel, int t_off, i
     fa.exp_dt = std::exp(-I.gamma * I.dt);
     fa.Bc = I.temp.kind == SAMOS_VALUE_CONSTANT ? std::sqrt(I.temp.tmin * (1.0 - std::exp(-2.0 * I.gamma * I.dt))) : -1.0;
     fa.batch = ctx->gather_batch;
-    fa.age_uniform = ctx->age_uniform ? 1 : 0;
+    fa.age = ctx->d_age;
     fa.chunk = chunk;
     fa.slot = slot;
     launch_baoab_fused(ctx->st, pot_mask(ctx), phase, cur, nxt, ctx->nl, ctx->box, ctx->con, interact_params(ctx),
@@ -508,14 +644,9 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   double mass_u = -1.0;
   if (int r = uniform_mass(ctx, &mass_u)) return r;
-  // Particle ages only grow by dt per step (src/integrators/integrator_langevin.cpp:115).
-  // When all ages are equal they stay equal, so the sum is kept once here and
-  // written back at the end of the range instead of streaming the column.
-  {
-    std::vector<double> age(ctx->n);
-    if (ctx->n) HIPCHK(hipMemcpy(age.data(), ctx->dcol[16], sizeof(double) * ctx->n, hipMemcpyDeviceToHost));
-    ctx->age_uniform = ctx->n > 0 && std::all_of(age.begin(), age.end(), [&](double v) { return v == age[0]; });
-    ctx->age_u = ctx->n ? age[0] : 0.0;
+  if (!ctx->age_checked) {
+    launch_age_check(ctx->st, ctx->d_age, ctx->dcol[16], ctx->n);
+    ctx->age_checked = true;
   }
   if (tile_eligible(ctx, flags, mass_u)) {
     const int r = run_tiled_langevin(ctx, I, a, b, steps, flags, mass_u);
@@ -543,16 +674,33 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   // Captured chunks: M fused steps ("slots") followed by one device-conditional
   // rebuild.  A slot whose new positions need a rebuild stops the chunk: the
   // later slots return at once, the positions settle into buffer 0, the
-  // rebuild runs, and the next chunk resumes at the following step.  The
-  // rebuild decisions and the steps are exactly those of one launch per step;
-  // only the skipped-rebuild dispatches between steps are gone.  Slots cover
+  // rebuild runs, and the next chunk resumes at the following step.  Slots
+  // past the chunk's limit return at once as well, so one chunk serves any
+  // range; short ranges use a smaller power-of-two chunk.  The rebuild
+  // decisions and the steps are exactly those of one launch per step; only
+  // the skipped-rebuild dispatches between steps are gone.  Slots cover
   // iterations up to b - 2 (the range's last iteration has its own loop-head
-  // flag and is launched directly).
-  const int M = ctx->chunk_slots;
+  // flag and is launched directly).  Instantiated chunks are kept per context.
   const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
-  if (graph && b - 1 - t >= 2 * M) {
+  if (graph && b - 1 - t >= 2) {
+    const int M = chunk_size(ctx->chunk_slots, b - 1 - t);
     Fields f0 = fields(ctx, 0), f1 = fields(ctx, 1);
     if (planar) { f0.xy = ctx->xy[0]; f1.xy = ctx->xy[1]; }
+    KeyBuf base;
+    base << 'L' << f32 << planar << mass_u << pot_mask(ctx);
+    key_context(base, ctx);
+    key_integ(base, I);
+    auto record = [&](int m) -> int {
+      // head_enforce of every slot: its iteration t' <= b - 2 < steps
+      for (int j = 0; j < m; ++j)
+        if (int r = fused_step(ctx, I, 1, b - 2, j, j, steps, f32, planar, mass_u, false, ctx->d_chunk, j)) return r;
+      launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, ctx->n, planar ? 1 : 0, 0);
+      launch_rebuild(ctx->st, f0, ctx->nl, ctx->box, ctx->n, &ctx->d_chunk->rebuild, planar ? 1 : 0);
+      launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
+      return 0;
+    };
+    hipGraphExec_t ge = nullptr;
+    if (int r = chunk_graphs(ctx, base.s, M, record, &ge)) return r;
     // slots start at buffer parity 0
     if (planar) {
       HIPCHK(hipMemcpyAsync(ctx->xy[0], ctx->xy[1], sizeof(double2) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
@@ -562,36 +710,7 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     }
     buf = 0;
     launch_chunk_init(ctx->st, ctx->d_chunk, t, b - 1);
-    hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-    HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-    // head_enforce of every slot: its iteration t' <= b - 2 < steps
-    for (int j = 0; j < M; ++j) fused_step(ctx, I, 1, b - 2, j, j, steps, f32, planar, mass_u, false, ctx->d_chunk, j);
-    launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, ctx->n, planar ? 1 : 0, 0);
-    launch_rebuild(ctx->st, f0, ctx->nl, ctx->box, ctx->n, &ctx->d_chunk->rebuild, planar ? 1 : 0);
-    launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
-    HIPCHK(hipStreamEndCapture(ctx->st, &g));
-    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-    // Every chunk runs at least its first slot, so t grows with each launch;
-    // launches are issued in batches sized by the slots per chunk seen so far.
-    double per = 0.5 * M;
-    int* h_base = reinterpret_cast<int*>(ctx->h_out);
-    while (t < b - 1) {
-      const int k = std::max(1, std::min(256, (int)((b - 1 - t) / per)));
-      for (int i = 0; i < k; ++i) HIPCHK(hipGraphLaunch(ge, ctx->st));
-      HIPCHK(hipMemcpyAsync(h_base, &ctx->d_chunk->base, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
-      HIPCHK(hipStreamSynchronize(ctx->st));
-      const int tn = *h_base;
-      if (tn <= t || tn > b - 1) {
-        hipGraphExecDestroy(ge);
-        hipGraphDestroy(g);
-        return fail(ctx, "captured step chunks made no progress");
-      }
-      per = std::max(1.0, (double)(tn - t) / k);
-      t = tn;
-    }
-    HIPCHK(hipGraphExecDestroy(ge));
-    HIPCHK(hipGraphDestroy(g));
+    if (int r = replay_chunks(ctx, ge, M, b - 1, &t)) return r;
     // the last chunk served any request; the direct steps below use the parity flags
     HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   }
@@ -609,16 +728,10 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     for (int k = 0; k < 3; ++k)
       HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->alt[k], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
   }
+  // one closing half-kick, and one age step, per iteration
+  launch_age_finish(ctx->st, ctx->d_age, ctx->dcol[16], ctx->n, I.dt, b - a);
   HIPCHK(hipGetLastError());
   I.calls += (unsigned long long)(b - a);
-  if (ctx->age_uniform) {
-    double age = ctx->age_u;
-    for (int k = a; k < b; ++k) age += I.dt;  // one closing half-kick per iteration
-    std::vector<double> col(ctx->n, age);
-    HIPCHK(hipMemcpyAsync(ctx->dcol[16], col.data(), sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->st));
-    HIPCHK(hipStreamSynchronize(ctx->st));
-    ctx->age_uniform = false;
-  }
   return 0;
 }
 
@@ -651,6 +764,10 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
     if (planar) f.xy = ctx->xy[par];
     return f;
   };
+  if (!ctx->age_checked) {
+    launch_age_check(ctx->st, ctx->d_age, ctx->dcol[16], n);
+    ctx->age_checked = true;
+  }
   if (int r = upload_clock(ctx, a)) return r;
   HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   if (planar) {
@@ -667,6 +784,7 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
     ba.temp = I.temp; ba.seed = I.seed; ba.clock = ctx->d_clock; ba.clock_slot = I.slot; ba.t_off = t_off;
     ba.parity = par; ba.head_enforce = t + 1 <= steps ? 1 : 0;
     ba.chunk = chunk; ba.slot = slot;
+    ba.age = ctx->d_age;
     if (pair) {
       ba.mode = 1;
       ba.pos_dt = P->dt; ba.pos_mu = P->mu; ba.pos_temp = P->temp; ba.pos_seed = P->seed; ba.pos_slot = P->slot;
@@ -687,39 +805,27 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
     }
   };
   int t = a, buf = 0;
-  const int M = ctx->chunk_slots;
   const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
-  if (graph && b - 1 - t >= 2 * M) {
+  if (graph && b - 1 - t >= 2) {
+    const int M = chunk_size(ctx->chunk_slots, b - 1 - t);
     Fields f0 = buffer(0), f1 = buffer(1);
-    launch_chunk_init(ctx->st, ctx->d_chunk, t, b - 1);
-    hipGraph_t g = nullptr;
+    KeyBuf base;
+    base << 'B' << f32 << planar << pot_mask(ctx) << al_mask(ctx);
+    key_context(base, ctx);
+    key_integ(base, I);
+    if (P) key_integ(base, *P);
+    auto record = [&](int m) -> int {
+      // head_enforce of every slot: its iteration t' <= b - 2 < steps
+      for (int j = 0; j < m; ++j) step(b - 2, j & 1, j, ctx->d_chunk, j);
+      launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, n, planar ? 1 : 0, 1);
+      launch_rebuild(ctx->st, f0, ctx->nl, ctx->box, n, &ctx->d_chunk->rebuild, planar ? 1 : 0);
+      launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
+      return 0;
+    };
     hipGraphExec_t ge = nullptr;
-    HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-    // head_enforce of every slot: its iteration t' <= b - 2 < steps
-    for (int j = 0; j < M; ++j) step(b - 2, j & 1, j, ctx->d_chunk, j);
-    launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, n, planar ? 1 : 0, 1);
-    launch_rebuild(ctx->st, f0, ctx->nl, ctx->box, n, &ctx->d_chunk->rebuild, planar ? 1 : 0);
-    launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
-    HIPCHK(hipStreamEndCapture(ctx->st, &g));
-    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-    double per = 0.5 * M;
-    int* h_base = reinterpret_cast<int*>(ctx->h_out);
-    while (t < b - 1) {
-      const int k = std::max(1, std::min(256, (int)((b - 1 - t) / per)));
-      for (int i = 0; i < k; ++i) HIPCHK(hipGraphLaunch(ge, ctx->st));
-      HIPCHK(hipMemcpyAsync(h_base, &ctx->d_chunk->base, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
-      HIPCHK(hipStreamSynchronize(ctx->st));
-      const int tn = *h_base;
-      if (tn <= t || tn > b - 1) {
-        hipGraphExecDestroy(ge);
-        hipGraphDestroy(g);
-        return fail(ctx, "captured step chunks made no progress");
-      }
-      per = std::max(1.0, (double)(tn - t) / k);
-      t = tn;
-    }
-    HIPCHK(hipGraphExecDestroy(ge));
-    HIPCHK(hipGraphDestroy(g));
+    if (int r = chunk_graphs(ctx, base.s, M, record, &ge)) return r;
+    launch_chunk_init(ctx->st, ctx->d_chunk, t, b - 1);
+    if (int r = replay_chunks(ctx, ge, M, b - 1, &t)) return r;
     HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   }
   const int t0 = t;
@@ -737,6 +843,7 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
   if (pf)
     for (int k = 0; k < (planar ? 2 : 3); ++k)
       HIPCHK(hipMemcpyAsync(ctx->dcol[6 + k], ctx->altn[k], sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
+  launch_age_finish(ctx->st, ctx->d_age, ctx->dcol[16], n, P ? P->dt : I.dt, b - a);
   I.calls += (unsigned long long)(b - a);
   if (P) P->calls += (unsigned long long)(b - a);
   // loop head of b: its enforce was applied with the positions; forces and torques
@@ -752,8 +859,16 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
 // uniform-age path).
 bool tile_eligible(samos_ctx* ctx, int flags, double mass_u) {
   const NListD& nl = ctx->nl;
-  return (flags & SAMOS_RUN_TILES) && ctx->con.kind == 1 && nl.use_cell && nl.ncx >= 3 && nl.ncy >= 3 &&
-         ctx->ntypes == 1 && !ctx->soft_radii && !ctx->lj_radii && mass_u > 0.0 && ctx->age_uniform && ctx->n > 0;
+  if (!((flags & SAMOS_RUN_TILES) && ctx->con.kind == 1 && nl.use_cell && nl.ncx >= 3 && nl.ncy >= 3 &&
+        ctx->ntypes == 1 && !ctx->soft_radii && !ctx->lj_radii && mass_u > 0.0 && ctx->n > 0))
+    return false;
+  // the engine keeps no age column: it needs the common age decided at the
+  // start of this range
+  AgeCtl h{};
+  if (hipMemcpyAsync(&h, ctx->d_age, sizeof(AgeCtl), hipMemcpyDeviceToHost, ctx->st) != hipSuccess ||
+      hipStreamSynchronize(ctx->st) != hipSuccess)
+    return false;
+  return h.uniform != 0;
 }
 
 // Carves the tiled engine's arrays out of one allocation, reused while the
@@ -831,7 +946,7 @@ void tile_fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int t_off, int 
   fa.exp_dt = std::exp(-I.gamma * I.dt);
   fa.Bc = I.temp.kind == SAMOS_VALUE_CONSTANT ? std::sqrt(I.temp.tmin * (1.0 - std::exp(-2.0 * I.gamma * I.dt))) : -1.0;
   fa.batch = 4;
-  fa.age_uniform = 1;
+  fa.age = ctx->d_age;
   const Fields f = fields(ctx);
   const SlotOut so{f.fx, f.fy, f.fz, f.ix, f.iy, f.iz};
   PairConst pc{};
@@ -918,12 +1033,9 @@ int run_tiled_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   }
   HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   I.calls += (unsigned long long)(b - a);
-  double age = ctx->age_u;
-  for (int k = a; k < b; ++k) age += I.dt;  // one closing half-kick per iteration
-  std::vector<double> col(ctx->n, age);
-  HIPCHK(hipMemcpyAsync(ctx->dcol[16], col.data(), sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->st));
-  HIPCHK(hipStreamSynchronize(ctx->st));
-  ctx->age_uniform = false;
+  // one closing half-kick, and one age step, per iteration
+  launch_age_finish(ctx->st, ctx->d_age, ctx->dcol[16], ctx->n, I.dt, b - a);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 
@@ -978,6 +1090,7 @@ int samos_create(samos_ctx** out, int device, int n, double lx, double ly, doubl
   if (hipMalloc(&ctx->d_tab, 4 * kTab * sizeof(double)) != hipSuccess) return bad("tables");
   if (hipMalloc(&ctx->d_clock, sizeof(ClockD)) != hipSuccess) return bad("clock");
   if (hipMalloc(&ctx->d_chunk, sizeof(ChunkCtl)) != hipSuccess) return bad("chunk control");
+  if (hipMalloc(&ctx->d_age, sizeof(AgeCtl)) != hipSuccess) return bad("age control");
   if (hipMalloc(&ctx->d_en, 5 * np * sizeof(double)) != hipSuccess) return bad("energies");
   if (hipMalloc(&ctx->d_cols, 3 * np * sizeof(double)) != hipSuccess) return bad("fire columns");
   if (hipMalloc(&ctx->d_partial, 8 * (size_t)reduce_blocks(ctx->np) * sizeof(double)) != hipSuccess) return bad("partials");
@@ -1018,13 +1131,15 @@ int samos_destroy(samos_ctx* ctx) {
   if (ctx->st) hipStreamSynchronize(ctx->st);
   for (auto& p : ctx->profs)
     for (auto& e : p.second.ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+  for (auto& g : ctx->graphs) hipGraphExecDestroy(g.exec);
+  ctx->graphs.clear();
   free_nlist(ctx);
   for (auto& kv : ctx->integs) hipFree(kv.second.R);
   for (int k = 0; k < kNumDoubleFields; ++k) hipFree(ctx->dcol[k]);
   for (int k = 0; k < kNumIntFields; ++k) hipFree(ctx->icol[k]);
   for (int k = 0; k < 3; ++k) { hipFree(ctx->alt[k]); hipFree(ctx->altn[k]); }
   for (int k = 0; k < 2; ++k) hipFree(ctx->xy[k]);
-  hipFree(ctx->d_flags); hipFree(ctx->d_nnb); hipFree(ctx->d_tab); hipFree(ctx->d_clock); hipFree(ctx->d_chunk); hipFree(ctx->d_en); hipFree(ctx->d_cols);
+  hipFree(ctx->d_flags); hipFree(ctx->d_nnb); hipFree(ctx->d_tab); hipFree(ctx->d_clock); hipFree(ctx->d_chunk); hipFree(ctx->d_age); hipFree(ctx->d_en); hipFree(ctx->d_cols);
   hipFree(ctx->d_partial); hipFree(ctx->d_out); hipFree(ctx->d_counter);
   if (ctx->h_out) hipHostFree(ctx->h_out);
   hipFree(ctx->keys); hipFree(ctx->keys2); hipFree(ctx->perm); hipFree(ctx->perm2);
@@ -1048,6 +1163,7 @@ int samos_set_field(samos_ctx* ctx, int field, const void* host) {
     for (int s = 0; s < n; ++s) tmp[s] = h[ctx->h_id[s]];
     HIPCHK(hipMemcpyAsync(ctx->dcol[field], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
     if (field == SAMOS_MASS) ctx->mass_known = false;
+    if (field == SAMOS_AGE) ctx->age_checked = false;
     if (field == SAMOS_RADIUS) {
       bool uni = true;
       for (int s = 1; s < n && uni; ++s) uni = tmp[s] == tmp[0];
@@ -1513,19 +1629,6 @@ int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags
   if (t_begin > 0 && !ctx->run_begun) return fail(ctx, "run range does not continue a started run");
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
-  if (t_begin == 0 && !ctx->run_begun) {
-    // Loop head of t = 0 (src/samos.cpp:887-892): enforce, reset, forces and
-    // torques.  The pre-run evaluation before it (:872-875) is overwritten by
-    // this reset and leaves no trace, so it is not repeated here.
-    launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
-    loop_head_forces(ctx, nullptr, f32);
-    ctx->run_begun = true;
-  }
-  if (t_end == t_begin) {
-    HIPCHK(hipStreamSynchronize(ctx->st));
-    return 0;
-  }
-  if (t_end == steps + 1) ctx->run_begun = false;
   // Fused schedule: one Langevin BAOAB integrator, pair potentials only.  The
   // reference's loop-head force evaluation repeats the evaluation that ended
   // the previous step on the same positions, so it is done once.
@@ -1533,12 +1636,28 @@ int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags
                      ctx->integs.count("langevin_all") &&
                      ctx->integs["langevin_all"].method == 0 && al_mask(ctx) == 0 && pot_mask(ctx) != 0 &&
                      needs_nlist(ctx);
+  const bool fused_b = !fused && fused_brownian_eligible(ctx, flags);
+  if (t_begin == 0 && !ctx->run_begun) {
+    // Loop head of t = 0 (src/samos.cpp:887-892): enforce, reset, forces and
+    // torques.  The pre-run evaluation before it (:872-875) is overwritten by
+    // this reset and leaves no trace, so it is not repeated here.  The fused
+    // schedules evaluate the loop-head forces of every step they integrate,
+    // so they need the evaluation only when the range stops at t = 0.
+    launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+    if (!(fused || fused_b) || t_end == t_begin) loop_head_forces(ctx, nullptr, f32);
+    ctx->run_begun = true;
+  }
+  if (t_end == t_begin) {
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+  }
+  if (t_end == steps + 1) ctx->run_begun = false;
   if (fused) {
     if (int r = run_fused_langevin(ctx, ctx->integs["langevin_all"], t_begin, t_end, steps, flags)) return r;
     ctx->time_step += t_end - t_begin;
     return check_overflow(ctx);
   }
-  if (fused_brownian_eligible(ctx, flags)) {
+  if (fused_b) {
     if (int r = run_fused_brownian(ctx, t_begin, t_end, steps, flags)) return r;
     ctx->time_step += t_end - t_begin;
     return check_overflow(ctx);
@@ -1546,35 +1665,40 @@ int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags
   int t = t_begin;
   // Without FIRE (whose scalar update runs on the host) the step sequence is
   // launch-only: chunks of K iterations are captured once, with the noise call
-  // and run step read from the device clock, and replayed.
+  // and run step read from the device clock plus the iteration's offset, and
+  // replayed; K is the largest power of two up to 32 that fits what is left.
   bool graphable = !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof && ctx->coop == 0;
   for (auto& kv : ctx->integs) graphable = graphable && kv.second.name != "fire";
-  const int K = 32;
-  if (graphable && t_end - t > K && t + K <= steps) {
-    if (int r = upload_clock(ctx, t)) return r;
-    hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-    HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < K; ++j) {
-      for (auto& kv : ctx->integs) integrate_one(ctx, kv.second, 0, j);
-      check_and_rebuild(ctx);
-      launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
-      loop_head_forces(ctx, nullptr, f32);
-    }
-    launch_clock_advance(ctx->st, ctx->d_clock, K);
-    HIPCHK(hipStreamEndCapture(ctx->st, &g));
-    const hipError_t ir = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    HIPCHK(ir);
+  if (graphable) {
+    ctx->age_checked = false;
     // every captured iteration ends with the next loop head: t + K <= steps
-    while (t_end - t > K && t + K <= steps) {
-      HIPCHK(hipGraphLaunch(ge, ctx->st));
-      t += K;
-      ctx->time_step += K;
-      for (auto& kv : ctx->integs) kv.second.calls += K;
+    for (int K = 32; K >= 4; K >>= 1) {
+      if (!(t_end - t >= K && t + K <= steps)) continue;
+      if (int r = upload_clock(ctx, t)) return r;
+      KeyBuf key;
+      key << 'G' << K << f32 << pot_mask(ctx) << al_mask(ctx);
+      key_context(key, ctx);
+      for (auto& kv : ctx->integs) key_integ(key, kv.second);
+      hipGraphExec_t ge = nullptr;
+      auto record = [&]() -> int {
+        for (int j = 0; j < K; ++j) {
+          for (auto& kv : ctx->integs)
+            if (int r = integrate_one(ctx, kv.second, 0, j)) return r;
+          check_and_rebuild(ctx);
+          launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+          loop_head_forces(ctx, nullptr, f32);
+        }
+        launch_clock_advance(ctx->st, ctx->d_clock, K);
+        return 0;
+      };
+      if (int r = cached_graph(ctx, key.s, record, &ge)) return r;
+      while (t_end - t >= K && t + K <= steps) {
+        HIPCHK(hipGraphLaunch(ge, ctx->st));
+        t += K;
+        ctx->time_step += K;
+        for (auto& kv : ctx->integs) kv.second.calls += K;
+      }
     }
-    HIPCHK(hipStreamSynchronize(ctx->st));
-    HIPCHK(hipGraphExecDestroy(ge));
   }
   for (; t < t_end; ++t) {
     for (auto& kv : ctx->integs)

@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
     double vx = cur.vx[s], vy = cur.vy[s], vz = PLANAR ? 0.0 : cur.vz[s];
     if (PHASE >= 1) {  // closing B of step t-1
       vx += fact * fx; vy += fact * fy; vz += fact * fz;
-      if (!fa.age_uniform) cur.age[s] += dt;
+      if (!(fa.age && fa.age->uniform)) cur.age[s] += dt;
     }
     if (PHASE <= 1) {
       double B = Bc;
@@ -512,6 +512,36 @@ void fused_phase(hipStream_t st, int phase, int fp32, int planar, int mt, Fields
   else fused_layout<POT, 2>(st, fp32, planar, mt, cur, nxt, nl, b, c, ip, n, fa);
 }
 
+__global__ void k_age_init(AgeCtl* ctl, const double* age, int n) {
+  if (threadIdx.x == 0) {
+    ctl->uniform = 1;
+    ctl->age = n > 0 ? age[0] : 0.0;
+  }
+}
+
+// Any age that differs from age[0] (a NaN included) clears the flag; every
+// writer stores the same 0.
+__global__ void k_age_check(AgeCtl* ctl, const double* age, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool differs = s < n && !(age[s] == ctl->age);
+  if (__ballot(differs) != 0 && (threadIdx.x & 63) == 0) ctl->uniform = 0;
+}
+
+// The reference adds dt to each age once per step: the same additions, in
+// order, on the common value.
+__global__ void k_age_advance(AgeCtl* ctl, double dt, int count) {
+  if (threadIdx.x == 0 && ctl->uniform) {
+    double a = ctl->age;
+    for (int k = 0; k < count; ++k) a += dt;
+    ctl->age = a;
+  }
+}
+
+__global__ void k_age_fill(const AgeCtl* ctl, double* age, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n && ctl->uniform) age[s] = ctl->age;
+}
+
 __global__ void k_pack_xy(Fields f, int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < n) f.xy[s] = make_double2(f.x[s], f.y[s]);
@@ -621,6 +651,16 @@ void launch_baoab_fused(hipStream_t st, int pot_mask, int phase, Fields cur, Fie
     case 2: fused_phase<2>(st, phase, fp32, planar, multitype, cur, nxt, nl, box, con, ip, n, fa); break;
     default: fused_phase<3>(st, phase, fp32, planar, multitype, cur, nxt, nl, box, con, ip, n, fa); break;
   }
+}
+
+void launch_age_check(hipStream_t st, AgeCtl* ctl, const double* age, int n) {
+  hipLaunchKernelGGL(k_age_init, dim3(1), dim3(64), 0, st, ctl, age, n);
+  if (n > 0) hipLaunchKernelGGL(k_age_check, dim3(grid1(n)), dim3(kBlock), 0, st, ctl, age, n);
+}
+
+void launch_age_finish(hipStream_t st, AgeCtl* ctl, double* age, int n, double dt, int count) {
+  hipLaunchKernelGGL(k_age_advance, dim3(1), dim3(64), 0, st, ctl, dt, count);
+  if (n > 0) hipLaunchKernelGGL(k_age_fill, dim3(grid1(n)), dim3(kBlock), 0, st, ctl, age, n);
 }
 
 void launch_pack_xy(hipStream_t st, Fields f, int n) {

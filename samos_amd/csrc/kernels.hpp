@@ -94,7 +94,7 @@ struct FusedArgs {
   double exp_dt;   // exp(-gamma dt)
   double Bc;       // O-step noise amplitude at constant temperature, or -1
   int batch;       // neighbour gathers in flight per thread (4 or 8)
-  int age_uniform; // every particle has the same age: the host advances it once
+  const AgeCtl* age = nullptr;  // common-age state (null or not uniform: each slot adds dt)
   ChunkCtl* chunk = nullptr;  // captured-chunk mode (null: direct launch with the parity flags)
   int slot = 0;               // slot index inside the chunk
 };
@@ -128,6 +128,7 @@ struct BrownArgs {
   ChunkCtl* chunk = nullptr;  // captured-chunk mode (null: direct launch with the parity flags)
   int slot = 0;
   int lanes = 1, lane_shift = 0;  // threads per slot (power of two up to 64) and its log2
+  const AgeCtl* age = nullptr;    // common-age state (null or not uniform: each slot adds dt)
 };
 bool brownian_fused_supported(int pot_mask, int al_mask);
 void launch_brownian_fused(hipStream_t st, int pot_mask, int al_mask, Fields cur, Fields nxt, NListD nl, BoxD box,
@@ -142,6 +143,11 @@ void launch_brownian_fused(hipStream_t st, int pot_mask, int al_mask, Fields cur
 void launch_chunk_settle(hipStream_t st, ChunkCtl* ck, Fields f0, Fields f1, int n, int planar, int directors);
 void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock);
 void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit);
+// Common-age bookkeeping of fused ranges (AgeCtl): `check` records whether
+// every age equals age[0]; `finish` advances the common value by `count`
+// additions of dt and writes it to every slot (nothing when not uniform).
+void launch_age_check(hipStream_t st, AgeCtl* ctl, const double* age, int n);
+void launch_age_finish(hipStream_t st, AgeCtl* ctl, double* age, int n, double dt, int count);
 // Interleaved (x, y) copy of the positions, read by the planar kernels.
 void launch_pack_xy(hipStream_t st, Fields f, int n);
 // Back from the interleaved column into the x, y columns.

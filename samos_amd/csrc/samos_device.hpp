@@ -177,6 +177,18 @@ struct ClockD {
   int pad_;
 };
 
+// Particle ages only grow by the same dt per step for every integrated
+// particle (src/integrators/integrator_langevin.cpp:115,
+// src/integrators/integrator_brownian.cpp:118).  While every age is equal the
+// fused steps leave the column alone and the common value is advanced once
+// per range by the exact sequence of additions; `uniform` is decided on the
+// device at the start of a range.
+struct AgeCtl {
+  int uniform;
+  int pad_;
+  double age;
+};
+
 // Noise call and run step of a launch: read from the device clock plus an
 // offset inside captured step sequences, or given directly (clk null).
 __device__ __forceinline__ unsigned long long eff_call(const ClockD* clk, int slot, unsigned long long call) {
