@@ -116,6 +116,80 @@ def copy_probe(mib, reps=10):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
 
 
+def host_cpu():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def oracle_timed(body):
+    """Runs `body` (Python source using `oracle`, `sc`, `time`, `np`) in a child
+    process pinned to one core with the reference-flag oracle build; the body
+    prints one JSON line, which is returned."""
+    code = ("import os, sys, time, json\nimport numpy as np\nsys.path.insert(0, %r)\n"
+            "os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})\n"
+            "import oracle\nfrom tests import scenarios as sc\n" % ROOT) + body
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True)
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline_ensemble(sample_every=32, steps=100):
+    """The ensemble's replicas as the reference runs them, one `samos` per
+    replica: the oracle's reference-flag build on one core, over every
+    `sample_every`-th replica of the same workload for `steps` run-loop
+    iterations each."""
+    r = oracle_timed(r"""
+states, p = sc.c4_ensemble_workload()
+tot_ps, tot_t, picked = 0, 0.0, []
+for r in range(0, len(states), %d):
+    st = states[r]
+    o = sc.load(lambda n, b, q: oracle.OracleSystem(n, b, q, fast=True), st)
+    o.nlist(2.4, 0.3)
+    o.pair_soft(p["k"], p["a"])
+    if p["J"][r] != 0.0:
+        o.align_polar(float(p["J"][r]), p["rcut_align"])
+    o.constraint("sphere", st["R"])
+    o.set_timestep(p["dt"])
+    o.integrator_brownian(v0=float(p["v0"][r]), nu=float(p["nu"][r]), mu=p["mu"], mur=p["mur"], seed=int(p["seed"][r]))
+    t0 = time.perf_counter()
+    o.run(%d - 1)
+    tot_t += time.perf_counter() - t0
+    tot_ps += st["n"] * %d
+    picked.append(st["n"])
+    o.close()
+print(json.dumps({"ps": tot_ps, "seconds": tot_t, "counts": picked}))
+""" % (sample_every, steps, steps))
+    return {"value": r["ps"] / r["seconds"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "every %dth replica of the same ensemble (%d replicas, N = %s), each run alone for %d run-loop "
+                      "iterations (%.1f s), one thread; host: %s" % (sample_every, len(r["counts"]),
+                                                                     r["counts"], steps, r["seconds"], host_cpu())}
+
+
+def cpu_baseline_fire(iterations=6000, n=4096):
+    """One configuration of the FIRE workload relaxed by the oracle's
+    reference-flag build on one core (IntegratorFIRE, which never stops by
+    itself): `iterations` run-loop iterations."""
+    r = oracle_timed(r"""
+st = sc.plane_replicas([%d], seed=5)[0]
+o = sc.load(lambda n, b, q: oracle.OracleSystem(n, b, q, fast=True), st)
+o.nlist(2.4, 0.3)
+o.pair_soft(1.0, 2.0, use_radii=True)
+o.constraint("plane")
+o.set_timestep(0.01)
+o.integrator_fire(alpha=0.1, dt_max=0.1, f_tol=0.0, e_tol=0.0)
+t0 = time.perf_counter()
+o.run(%d - 1)
+print(json.dumps({"seconds": time.perf_counter() - t0, "n": st["n"], "iters": %d}))
+""" % (n, iterations, iterations))
+    return {"value": r["n"] * r["iters"] / r["seconds"], "unit": "particle-iterations/s", "cores": 1, "kind": "port",
+            "sample": "the first configuration of the same batch (N = %d), %d FIRE run-loop iterations (%.1f s), one "
+                      "thread; host: %s" % (r["n"], r["iters"], r["seconds"], host_cpu())}
+
+
 ENS_BYTES_PER_PARTICLE_STEP = 120   # read x, y, z, nx, ny, nz; write those and vx, vy, vz (FP64)
 FIRE_BYTES_PER_PARTICLE_ITER = 72   # read x, y, vx, vy, r; write x, y, vx, vy (FP64)
 
@@ -134,7 +208,7 @@ def plain_replica_system(samos_amd, sc, st, p, r):
     return s
 
 
-def bench_ensemble(samos_amd, sc, steps, plain_steps, nrep=256, pad=None):
+def bench_ensemble(samos_amd, sc, steps, plain_steps, nrep=256, pad=None, cpu=True):
     """Batched active-sphere ensemble: every replica advanced by one fused launch
     per step, against the same replicas run one at a time."""
     states, p = sc.c4_ensemble_workload(nrep=nrep)
@@ -222,6 +296,7 @@ def bench_ensemble(samos_amd, sc, steps, plain_steps, nrep=256, pad=None):
                      "alg_bytes_per_launch": ENS_BYTES_PER_PARTICLE_STEP * n_tot},
         "nlist": {"builds": info["builds"], "stride": info["stride"], "buckets": info["buckets"],
                   "rebuild_ms_total": rb_ms, "rebuild_sequences": rb_launches},
+        "cpu_baseline": cpu_baseline_ensemble() if cpu and nrep == 256 else None,
     }
 
 
@@ -313,12 +388,12 @@ def fire_batch(samos_amd, sc, states):
     return e
 
 
-def warm_start_ab(samos_amd, sc, states, fs_plain, secs_plain, max_iter, weights):
+def warm_start_ab(samos_amd, sc, states, fs_plain, secs_plain, max_iter, weights, scales=(1.0,)):
     """The same configurations relaxed from the network's predicted positions:
     inference, displacement, list rebuild and FIRE are all timed.  An answer
-    counts as the plain one when max|F| < 1e-10 and its energy equals plain
-    FIRE's to 1e-12 (relative); configurations whose warm run does not stop
-    fall back to plain FIRE from the raw start (timed too)."""
+    counts only when its replica stopped at max|F| < 1e-10 with an energy equal
+    to plain FIRE's to 1e-12 (relative); every other configuration is charged
+    the fallback, plain FIRE from its raw start, run and timed as one batch."""
     z = np.load(weights, allow_pickle=False)
     meta = json.loads(str(z["meta"]))
     mp = samos_amd.Mpnn(seed=0)
@@ -326,7 +401,7 @@ def warm_start_ab(samos_amd, sc, states, fs_plain, secs_plain, max_iter, weights
     out = {"weights": os.path.relpath(weights, ROOT), "trained_on_configs": meta.get("train_configs"),
            "val_loss": meta["history"][-1]["val_loss"] if meta.get("history") else None,
            "val_loss_zero_prediction": meta.get("val_loss_zero_prediction")}
-    for scale in (1.0, 0.5):
+    for scale in scales:
         e = fire_batch(samos_amd, sc, states)
         e.sync()
         t0 = time.perf_counter()
@@ -339,30 +414,35 @@ def warm_start_ab(samos_amd, sc, states, fs_plain, secs_plain, max_iter, weights
         secs = time.perf_counter() - t0
         fs = e.fire_state()
         e.close()
-        fallback_s = 0.0
         stopped = fs["stopped"] == 1
-        if not np.all(stopped):
-            idx = np.nonzero(~stopped)[0]
-            f = fire_batch(samos_amd, sc, [states[i] for i in idx])
+        same = stopped & (np.abs(fs["energy"] - fs_plain["energy"]) <= 1e-12 * np.abs(fs_plain["energy"]))
+        fallback_s = 0.0
+        failed = np.nonzero(~same)[0]
+        if len(failed):
+            f = fire_batch(samos_amd, sc, [states[i] for i in failed])
             f.fire_setup(dt=0.01, alpha=0.1, dt_max=0.1, f_tol=0.0, stop_fmax=1e-10)
+            f.sync()
             t1 = time.perf_counter()
             f.fire_run(max_iter, check=100)
             f.sync()
             fallback_s = time.perf_counter() - t1
             f.close()
-        same = np.abs(fs["energy"] - fs_plain["energy"]) <= 1e-12 * np.abs(fs_plain["energy"])
         rel = (fs["energy"] - fs_plain["energy"]) / fs_plain["energy"]
         total = secs + fallback_s
-        print("bench: warm start x%.2f: %.3f s (inference + rebuild %.1f ms), %d iterations, same minimum %d/%d"
-              % (scale, total, t_inf * 1e3, iters, int(same.sum()), len(same)), file=sys.stderr, flush=True)
+        print("bench: warm start x%.2f: %.3f s warm (inference + rebuild %.1f ms) + %.3f s fallback for %d/%d, "
+              "same minimum %d/%d" % (scale, secs, t_inf * 1e3, fallback_s, len(failed), len(same), int(same.sum()),
+                                     len(same)), file=sys.stderr, flush=True)
         out["scale_%g" % scale] = {
             "time_to_converge_s": total,
+            "warm_run_s": secs,
             "inference_and_rebuild_ms": t_inf * 1e3,
             "fallback_s": fallback_s,
+            "fallback_configs": int(len(failed)),
             "iterations": iters,
             "iterations_per_replica": {"median": float(np.median(fs["iters"])), "max": float(fs["iters"].max())},
             "speedup_vs_plain": secs_plain / total,
-            "same_minimum_fraction": float(np.mean(same & stopped)),
+            "speedup_vs_plain_without_fallback": secs_plain / secs,
+            "same_minimum_fraction": float(np.mean(same)),
             "energy_rel_diff_vs_plain": {"median": float(np.median(rel)), "mean": float(np.mean(rel)),
                                          "lower_fraction": float(np.mean(rel < -1e-12))},
         }
@@ -370,7 +450,7 @@ def warm_start_ab(samos_amd, sc, states, fs_plain, secs_plain, max_iter, weights
     return out
 
 
-def bench_fire(samos_amd, sc, nrep=512, n=4096, max_iter=400000, weights=WARM_WEIGHTS):
+def bench_fire(samos_amd, sc, nrep=512, n=4096, max_iter=400000, weights=WARM_WEIGHTS, cpu=True):
     """Plain batched FIRE to max|F| < 1e-10 on every replica, then (when trained
     weights exist) the same configurations warm-started by the network."""
     states = sc.plane_replicas([n] * nrep, seed=5)
@@ -413,6 +493,7 @@ def bench_fire(samos_amd, sc, nrep=512, n=4096, max_iter=400000, weights=WARM_WE
                      "frac": achieved / HBM_PEAK_GBS, "kernel": "k_ens_fire_b",
                      "avg_launch_us": avg_s * 1e6, "launches_timed": launches,
                      "alg_bytes_per_launch": FIRE_BYTES_PER_PARTICLE_ITER * nrep * n},
+        "cpu_baseline": cpu_baseline_fire(n=n) if cpu else None,
     }
 
 
@@ -557,9 +638,10 @@ def main():
     extras = {}
     if "ensemble" in parts:
         extras["ensemble"] = bench_ensemble(samos_amd, sc, args.ensemble_steps, args.plain_steps,
-                                            nrep=args.ensemble_replicas, pad=args.ensemble_pad)
+                                            nrep=args.ensemble_replicas, pad=args.ensemble_pad, cpu=not args.no_cpu)
     if "fire" in parts:
-        extras["fire"] = bench_fire(samos_amd, sc, nrep=args.fire_replicas, weights=args.warm_weights)
+        extras["fire"] = bench_fire(samos_amd, sc, nrep=args.fire_replicas, weights=args.warm_weights,
+                                    cpu=not args.no_cpu)
     ng = samos_amd.RUN_NO_GRAPH if args.no_graph else 0
     if "c1" in parts:
         extras["polar_disks_plane_n4096"] = bench_single(samos_amd, sc, "c1", 10000, 0 if args.no_cpu else 1000, ng)

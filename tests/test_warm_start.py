@@ -5,8 +5,9 @@ against the method without it, on configurations it was not trained on
 What it gains: its predicted displacement explains part of the relaxation
 (loss below that of predicting no displacement).  What every answer must
 satisfy: warm-started FIRE stops at max|F| < 1e-10 on every configuration.
-The same-minimum rate (energy equal to plain FIRE's to 1e-12) and the
-iteration counts are measured and checked for consistency, not assumed.
+The same-minimum rate (energy equal to plain FIRE's to 1e-12) is asserted
+at its stated expected value: the shipped network does not keep plain FIRE's
+minimum, and bench.py charges every such configuration the plain-FIRE fallback.
 """
 import json
 import os
@@ -87,5 +88,10 @@ def test_warm_started_fire_meets_the_residual_bar(gpu, trained):
     en = warm.forces()
     np.testing.assert_allclose(en, fw["energy"], rtol=1e-12)
     same = np.abs(fw["energy"] - fp["energy"]) <= 1e-12 * np.abs(fp["energy"])
-    assert 0 <= same.sum() <= len(same)
+    # Stated expectation for the shipped network (0 of 512 on the bench
+    # workload): its starting points lead FIRE into other minima of this
+    # jammed packing, so the bench charges those configurations the plain-FIRE
+    # fallback.  A network that kept the basin would fail this line and the
+    # accounting text would have to change with it.
+    assert same.mean() <= 0.25
     assert np.all(fw["iters"] > 0)
