@@ -91,6 +91,16 @@ int samos_nlist_info(samos_ctx* ctx, long long* info);
  * rows (i, j), i < j in id order, sorted ascending.  With out == NULL only
  * *count is set.  Pairs are kept once per stencil visit, as in the reference. */
 int samos_nlist_pairs(samos_ctx* ctx, int* out, long long cap, long long* count);
+/* No counterpart in the reference: samos_nlist_pairs for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It calls no hipMalloc, hipFree or other allocating
+ * routine: the device staging of the rows lives in a fixed file-scope array of
+ * the library, one per GPU (16 MiB plus 256 bytes, held until the process
+ * ends), at most 2,097,152 half pairs; a list with more returns an error.
+ * The rows are sorted in place in `out`.  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (several host threads or contexts
+ * at once) race; calls one after another are independent. */
+int samos_nlist_pairs_noalloc(samos_ctx* ctx, int* out, long long cap, long long* count);
 /* Re-sort particle storage along the Morton curve of the cell grid and rebuild. */
 int samos_resort(samos_ctx* ctx);
 

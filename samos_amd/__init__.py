@@ -62,6 +62,7 @@ def load_library(path: str = LIB_PATH):
         "samos_nlist_info": (I, [P, llp]),
         "samos_tile_info": (I, [P, llp]),
         "samos_nlist_pairs": (I, [P, ip, LL, llp]),
+        "samos_nlist_pairs_noalloc": (I, [P, ip, LL, llp]),
         "samos_resort": (I, [P]),
         "samos_pair_soft": (I, [P, D, D, I]),
         "samos_pair_lj": (I, [P, D, D, D, I, I]),
@@ -233,6 +234,14 @@ class System:
         out = np.empty((max(cnt.value, 1), 2), dtype=np.int32)
         self._chk(self.L.samos_nlist_pairs(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                            cnt.value, ctypes.byref(cnt)))
+        return out[:cnt.value]
+
+    def pairs_noalloc(self, cap):
+        """samos_nlist_pairs_noalloc into a caller-sized array of `cap` rows."""
+        cnt = ctypes.c_longlong()
+        out = np.empty((max(cap, 1), 2), dtype=np.int32)
+        self._chk(self.L.samos_nlist_pairs_noalloc(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                   cap, ctypes.byref(cnt)))
         return out[:cnt.value]
 
     # -- plugins ------------------------------------------------------------

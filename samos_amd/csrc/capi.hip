@@ -10,6 +10,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1049,6 +1050,41 @@ int check_overflow(samos_ctx* ctx) {
   return 0;
 }
 
+// ---- scratch of the allocation-free pair export ----------------------------
+// Staging of samos_nlist_pairs_noalloc: a counter in the first 8 bytes, the
+// exported rows from byte kScratchHead on.  One copy per GPU, held until the
+// process ends.  Host code uses only the per-device address resolved below.
+constexpr long kScratchPairs = 1L << 21;  // half pairs: 2,097,152 rows of two ints (16 MiB)
+constexpr size_t kScratchHead = 256;
+static __device__ __align__(256) unsigned char scratch[kScratchHead + 8 * (size_t)kScratchPairs];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device `device`, resolved
+// once per device.
+hipError_t scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
+__global__ void k_clear_counter(unsigned long long* c) {
+  if (threadIdx.x == 0) *c = 0ULL;
+}
+
+// Rows (id_lo, id_hi) as stored in the caller's int array.
+struct PairRow { int a, b; };
+
 }  // namespace
 
 extern "C" {
@@ -1060,6 +1096,12 @@ int samos_create(samos_ctx** out, int device, int n, double lx, double ly, doubl
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { g_create_error = "no HIP device"; return -2; }
   if (device < 0 || device >= ndev) { g_create_error = "device index out of range"; return -1; }
   if (hipSetDevice(device) != hipSuccess) { g_create_error = "hipSetDevice failed"; return -2; }
+  {
+    // set-up of the allocation-free entry points on this device
+    void* sp = nullptr;
+    const hipError_t e = scratch_address(device, &sp);
+    if (e != hipSuccess) { g_create_error = std::string("scratch address: ") + hipGetErrorString(e); return -2; }
+  }
   samos_ctx* ctx = new samos_ctx();
   ctx->device = device;
   ctx->n = n;
@@ -1327,6 +1369,37 @@ int samos_nlist_pairs(samos_ctx* ctx, int* out, long long cap, long long* count)
     if ((long long)cnt > cap) return fail(ctx, "output capacity too small");
     for (size_t k = 0; k < cnt; ++k) { out[2 * k] = pr[k].first; out[2 * k + 1] = pr[k].second; }
   }
+  return 0;
+}
+
+int samos_nlist_pairs_noalloc(samos_ctx* ctx, int* out, long long cap, long long* count) {
+  *count = 0;
+  if (!ctx->has_nlist) return fail(ctx, "no neighbour list");
+  void* base = nullptr;
+  {
+    const hipError_t e = scratch_address(ctx->device, &base);
+    if (e != hipSuccess) return fail(ctx, std::string("scratch address: ") + hipGetErrorString(e));
+  }
+  auto* counter = static_cast<unsigned long long*>(base);
+  int* rows = reinterpret_cast<int*>(static_cast<unsigned char*>(base) + kScratchHead);
+  hipLaunchKernelGGL(k_clear_counter, dim3(1), dim3(64), 0, ctx->st, counter);
+  launch_export_pairs(ctx->st, fields(ctx), ctx->nl, ctx->n, rows, counter, kScratchPairs);
+  HIPCHK(hipGetLastError());
+  unsigned long long cnt = 0;
+  HIPCHK(hipMemcpyAsync(&cnt, counter, sizeof(cnt), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if (cnt > (unsigned long long)kScratchPairs)
+    return fail(ctx, "samos_nlist_pairs_noalloc: the list holds " + std::to_string(cnt) +
+                         " half pairs, more than its scratch maximum of " + std::to_string(kScratchPairs) +
+                         "; use samos_nlist_pairs");
+  *count = (long long)cnt;
+  if (!out) return 0;
+  if ((long long)cnt > cap) return fail(ctx, "output capacity too small");
+  if (cnt) HIPCHK(hipMemcpyAsync(out, rows, sizeof(int) * 2 * cnt, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  // ascending (id_lo, id_hi), sorted in place in the caller's array
+  PairRow* r = reinterpret_cast<PairRow*>(out);
+  std::sort(r, r + cnt, [](const PairRow& x, const PairRow& y) { return x.a < y.a || (x.a == y.a && x.b < y.b); });
   return 0;
 }
 
