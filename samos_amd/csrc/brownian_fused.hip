@@ -41,6 +41,7 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
   } else if (blockIdx.x == 0 && threadIdx.x == 0) {
     nl.flags[1 - ba.parity] = 0;
   }
+  const int head_enforce = ck ? (ck->base + ba.slot + 1 <= ck->steps ? 1 : 0) : ba.head_enforce;
   // ba.lanes (a power of two dividing 64) consecutive threads share one slot:
   // each sums every lanes-th batch of the list, the partial sums are added
   // in a fixed butterfly order, and lane 0 integrates and writes.
@@ -144,7 +145,7 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
       age_dt = ba.pos_dt;
     }
     const bool writer = lane == 0;
-    if (ba.head_enforce) enforce<kSet>(b, c, p);
+    if (head_enforce) enforce<kSet>(b, c, p);
     if (writer) {
       cur.fx[s] = fx; cur.fy[s] = fy; cur.fz[s] = fz0 ? 0.0 : fz;
       if (AL) { cur.tx[s] = tq[0]; cur.ty[s] = tq[1]; cur.tz[s] = tq[2]; }

@@ -292,47 +292,58 @@ int cached_graph(samos_ctx* ctx, const std::string& key, const std::function<int
   return 0;
 }
 
-// Slots per captured chunk for a range needing `need` slots: the configured
-// count, or the smallest power of two (at least 2) that covers a shorter range.
-int chunk_size(int configured, int need) {
-  int m = 2;
-  while (m < need && m < configured) m <<= 1;
-  return std::min(m, configured);
+// Instantiated chunk graphs of one step sequence, one per size, ascending:
+// 2, 4, ... below the configured size, then the configured size.
+using ChunkSet = std::vector<std::pair<int, hipGraphExec_t>>;
+
+// Every size of the sequence keyed by `base`, instantiated together on first
+// use, so that a range of any length finds its chunks ready.
+int chunk_graphs(samos_ctx* ctx, const std::string& base, const std::function<int(int)>& record, ChunkSet* out) {
+  out->clear();
+  std::vector<int> sizes;
+  for (int m = 2; m < ctx->chunk_slots; m <<= 1) sizes.push_back(m);
+  sizes.push_back(ctx->chunk_slots);
+  for (int m : sizes) {
+    KeyBuf k;
+    k << m;
+    hipGraphExec_t ge = nullptr;
+    if (int r = cached_graph(ctx, base + k.s, [&]() { return record(m); }, &ge)) return r;
+    out->push_back({m, ge});
+  }
+  return 0;
 }
 
-// The chunk graph of M slots for the sequence keyed by `base`.  When the
-// sequence has no instantiated chunk yet, every power-of-two size up to the
-// configured one is instantiated together, so a later range of another length
-// finds its chunk ready instead of paying for an instantiation.
-int chunk_graphs(samos_ctx* ctx, const std::string& base, int M, const std::function<int(int)>& record,
-                 hipGraphExec_t* out) {
-  auto key = [&](int m) { KeyBuf k; k << m; return base + k.s; };
-  const std::string want = key(M);
-  bool have = false;
-  for (auto& g : ctx->graphs) have |= g.key == want;
-  if (!have)
-    for (int m = 2; m <= ctx->chunk_slots; m <<= 1)
-      if (m != M) {
-        hipGraphExec_t ge = nullptr;
-        if (int r = cached_graph(ctx, key(m), [&]() { return record(m); }, &ge)) return r;
-      }
-  return cached_graph(ctx, want, [&]() { return record(M); }, out);
-}
-
-// Launches chunks of the graph until the chunk base reaches `end`: each chunk
-// runs at least its first slot, and launches go in batches sized by the slots
-// per chunk seen so far.  Returns the base reached in *t.
-int replay_chunks(samos_ctx* ctx, hipGraphExec_t ge, int M, int end, int* t) {
-  double per = 0.5 * M;
+// Launches chunks until the chunk base reaches `end` (each chunk runs at
+// least its first slot, and a slot whose positions need a rebuild stops its
+// chunk).  A stretch longer than the largest chunk is covered by a batch of
+// largest chunks, as many as the slots per chunk seen so far suggest; a
+// shorter remainder by chunks whose sizes add up to it exactly (one spare
+// slot for an odd count), so no slot is launched only to return.  The host
+// reads the base after each batch.  Returns the base reached in *t.
+int replay_chunks(samos_ctx* ctx, const ChunkSet& set, int end, int* t) {
+  const int Mmax = set.back().first;
+  double per = Mmax;
   int* h_base = reinterpret_cast<int*>(ctx->h_out);
   while (*t < end) {
-    const int k = std::max(1, std::min(256, (int)((end - *t) / per)));
-    for (int i = 0; i < k; ++i) HIPCHK(hipGraphLaunch(ge, ctx->st));
+    const int r = end - *t;
+    int k = 0;
+    if (r > Mmax) {
+      k = std::max(1, std::min(256, (int)(r / per)));
+      for (int i = 0; i < k; ++i) HIPCHK(hipGraphLaunch(set.back().second, ctx->st));
+    } else {
+      for (int left = r; left > 0;) {
+        auto it = set.rbegin();
+        while (it != set.rend() && it->first > left) ++it;
+        const auto& g = it != set.rend() ? *it : set.front();
+        HIPCHK(hipGraphLaunch(g.second, ctx->st));
+        left -= g.first;
+      }
+    }
     HIPCHK(hipMemcpyAsync(h_base, &ctx->d_chunk->base, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     const int tn = *h_base;
     if (tn <= *t || tn > end) return fail(ctx, "captured step chunks made no progress");
-    per = std::max(1.0, (double)(tn - *t) / k);
+    if (k) per = std::max(1.0, (double)(tn - *t) / k);
     *t = tn;
   }
   return 0;
@@ -683,8 +694,7 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   // iterations up to b - 2 (the range's last iteration has its own loop-head
   // flag and is launched directly).  Instantiated chunks are kept per context.
   const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
-  if (graph && b - 1 - t >= 2) {
-    const int M = chunk_size(ctx->chunk_slots, b - 1 - t);
+  if (graph && b - t >= 2) {
     Fields f0 = fields(ctx, 0), f1 = fields(ctx, 1);
     if (planar) { f0.xy = ctx->xy[0]; f1.xy = ctx->xy[1]; }
     KeyBuf base;
@@ -692,7 +702,8 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     key_context(base, ctx);
     key_integ(base, I);
     auto record = [&](int m) -> int {
-      // head_enforce of every slot: its iteration t' <= b - 2 < steps
+      // the loop-head enforce of each slot's next iteration is decided on the
+      // device from the run's last iteration
       for (int j = 0; j < m; ++j)
         if (int r = fused_step(ctx, I, 1, b - 2, j, j, steps, f32, planar, mass_u, false, ctx->d_chunk, j)) return r;
       launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, ctx->n, planar ? 1 : 0, 0);
@@ -700,8 +711,8 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
       launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
       return 0;
     };
-    hipGraphExec_t ge = nullptr;
-    if (int r = chunk_graphs(ctx, base.s, M, record, &ge)) return r;
+    ChunkSet set;
+    if (int r = chunk_graphs(ctx, base.s, record, &set)) return r;
     // slots start at buffer parity 0
     if (planar) {
       HIPCHK(hipMemcpyAsync(ctx->xy[0], ctx->xy[1], sizeof(double2) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
@@ -710,15 +721,15 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
         HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->alt[k], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
     }
     buf = 0;
-    launch_chunk_init(ctx->st, ctx->d_chunk, t, b - 1);
-    if (int r = replay_chunks(ctx, ge, M, b - 1, &t)) return r;
-    // the last chunk served any request; the direct steps below use the parity flags
+    launch_chunk_init(ctx->st, ctx->d_chunk, t, b, steps);
+    if (int r = replay_chunks(ctx, set, b, &t)) return r;
+    // the last chunk served any request; direct steps use the parity flags
     HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   }
   const int t0 = t;
   for (; t < b; ++t)
     if (int r = fused_step(ctx, I, 1, t, buf + (t - t0), t - t0, steps, f32, planar, mass_u, true)) return r;
-  launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
+  if (b > t0) launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
   if (int r = fused_step(ctx, I, 2, b, buf + (b - t0), 0, steps, f32, planar, mass_u, true)) return r;
   const int pf = (buf + (b - t0)) & 1;  // buffer holding x_b
   if (planar) {
@@ -807,8 +818,7 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
   };
   int t = a, buf = 0;
   const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
-  if (graph && b - 1 - t >= 2) {
-    const int M = chunk_size(ctx->chunk_slots, b - 1 - t);
+  if (graph && b - t >= 2) {
     Fields f0 = buffer(0), f1 = buffer(1);
     KeyBuf base;
     base << 'B' << f32 << planar << pot_mask(ctx) << al_mask(ctx);
@@ -816,22 +826,21 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
     key_integ(base, I);
     if (P) key_integ(base, *P);
     auto record = [&](int m) -> int {
-      // head_enforce of every slot: its iteration t' <= b - 2 < steps
       for (int j = 0; j < m; ++j) step(b - 2, j & 1, j, ctx->d_chunk, j);
       launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, n, planar ? 1 : 0, 1);
       launch_rebuild(ctx->st, f0, ctx->nl, ctx->box, n, &ctx->d_chunk->rebuild, planar ? 1 : 0);
       launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
       return 0;
     };
-    hipGraphExec_t ge = nullptr;
-    if (int r = chunk_graphs(ctx, base.s, M, record, &ge)) return r;
-    launch_chunk_init(ctx->st, ctx->d_chunk, t, b - 1);
-    if (int r = replay_chunks(ctx, ge, M, b - 1, &t)) return r;
+    ChunkSet set;
+    if (int r = chunk_graphs(ctx, base.s, record, &set)) return r;
+    launch_chunk_init(ctx->st, ctx->d_chunk, t, b, steps);
+    if (int r = replay_chunks(ctx, set, b, &t)) return r;
     HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   }
   const int t0 = t;
   for (; t < b; ++t) step(t, (buf + t - t0) & 1, t - t0, nullptr, 0);
-  launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
+  if (b > t0) launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
   const int pf = (buf + b - t0) & 1;  // buffer holding x_b and n_b
   if (planar) {
     Fields f0 = fields(ctx, 0);

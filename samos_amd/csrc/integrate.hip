@@ -332,7 +332,8 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
   const TempD temp = fa.temp;
   const uint32_t seed = fa.seed;
   const ClockD* clk = fa.clock;
-  const int slot = fa.clock_slot, t_off = fa.t_off, parity = fa.parity, head_enforce = fa.head_enforce;
+  const int slot = fa.clock_slot, t_off = fa.t_off, parity = fa.parity;
+  int head_enforce = fa.head_enforce;
   const int s = xcd_block() * blockDim.x + threadIdx.x;
   ChunkCtl* const ck = fa.chunk;
   if (ck) {
@@ -345,6 +346,7 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
     }
     if (ck->base + j >= ck->limit) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) ck->last = j + 1;
+    head_enforce = ck->base + j + 1 <= ck->steps ? 1 : 0;
   } else if (blockIdx.x == 0 && threadIdx.x == 0 && PHASE != 2) {
     nl.flags[1 - parity] = 0;
   }
@@ -472,10 +474,10 @@ __global__ void k_chunk_finish(ChunkCtl* ck, ClockD* clk) {
   }
 }
 
-__global__ void k_chunk_init(ChunkCtl* ck, int base, int limit) {
+__global__ void k_chunk_init(ChunkCtl* ck, int base, int limit, int steps) {
   if (threadIdx.x == 0) {
     ck->halt = 0; ck->req[0] = 0; ck->req[1] = 0; ck->last = 0; ck->rebuild = 0;
-    ck->base = base; ck->limit = limit;
+    ck->base = base; ck->limit = limit; ck->steps = steps;
   }
 }
 
@@ -635,8 +637,8 @@ void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock) {
   hipLaunchKernelGGL(k_chunk_finish, dim3(1), dim3(64), 0, st, ck, clock);
 }
 
-void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit) {
-  hipLaunchKernelGGL(k_chunk_init, dim3(1), dim3(64), 0, st, ck, base, limit);
+void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit, int steps) {
+  hipLaunchKernelGGL(k_chunk_init, dim3(1), dim3(64), 0, st, ck, base, limit, steps);
 }
 
 void launch_clock_advance(hipStream_t st, ClockD* clock, int nsteps) {

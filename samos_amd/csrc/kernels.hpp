@@ -139,10 +139,11 @@ void launch_brownian_fused(hipStream_t st, int pot_mask, int al_mask, Fields cur
 // copies positions (and, with `directors`, directors) left in buffer 1 back
 // to buffer 0 and forms the rebuild
 // request; the finish step advances the clock and the chunk base by the slots
-// that ran and clears the control words; init sets base and limit.
+// that ran and clears the control words; init sets base, limit and the run's
+// last iteration.
 void launch_chunk_settle(hipStream_t st, ChunkCtl* ck, Fields f0, Fields f1, int n, int planar, int directors);
 void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock);
-void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit);
+void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit, int steps);
 // Common-age bookkeeping of fused ranges (AgeCtl): `check` records whether
 // every age equals age[0]; `finish` advances the common value by `count`
 // additions of dt and writes it to every slot (nothing when not uniform).

@@ -168,7 +168,8 @@ struct ChunkCtl {
   int base;      // run step of slot 0
   int limit;     // slot j runs only while base + j < limit
   int rebuild;   // rebuild request read by the chunk's rebuild stages
-  int pad_;
+  int steps;     // last iteration of the run: slot j applies the next loop
+                 // head's enforce while base + j + 1 <= steps
 };
 
 struct ClockD {
