@@ -176,6 +176,21 @@ int samos_integrator_nve(samos_ctx* ctx, double limit, double angle_limit);
 /* `disable <type> { group = all }` (src/samos.cpp, disable command): removes
  * the integrator registered under type. */
 int samos_integrator_remove(samos_ctx* ctx, const char* type);
+/* `disable type { group = g }` (src/samos.cpp:1141-1170): removes the
+ * integrator registered under type_group. */
+int samos_integrator_disable(samos_ctx* ctx, const char* type, const char* group);
+/* `group name { type = ...; ... }` (src/samos.cpp:1108-1140, System::make_group
+ * src/system/system.cpp:462-570): member[n] (id order) non-zero puts that
+ * particle into the group; the caller evaluates the criteria (type, not_type,
+ * r_eq, r_le, r_ge, x/y/z_le, x/y/z_ge, all of those given) on the current
+ * state.  A group defined again gains the new members.  "all" (every particle)
+ * always exists; "boundary" and "tissue" are reserved; at most 31 groups. */
+int samos_group_define(samos_ctx* ctx, const char* name, const int* member);
+int samos_group_size(samos_ctx* ctx, const char* name, int* size);
+/* The `group` parameter of the next integrator created (src/samos.cpp:654-690):
+ * it acts on that group's particles only (Integrator::m_group_name,
+ * src/integrators/integrator.hpp:98-108) and is keyed type_group. */
+int samos_set_integrator_group(samos_ctx* ctx, const char* group);
 /* FIRE state out[8]: dt, alpha, P, |F|, |v|, E, converged, last_neg. */
 int samos_fire_state(samos_ctx* ctx, double* out);
 

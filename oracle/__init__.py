@@ -75,6 +75,9 @@ def load(fast=False):
         "orc_add_brownian": (I, [P, I, D, D, D, D, I, I, D, I, I, D, D, I]),
         "orc_add_langevin": (I, [P, D, I, I, I, D, D, I]),
         "orc_add_fire": (I, [P, D, D, D, D, D, D, D, I]),
+        "orc_group_define": (I, [P, ctypes.c_char_p, P]),
+        "orc_set_integrator_group": (I, [P, ctypes.c_char_p]),
+        "orc_integrator_disable": (I, [P, ctypes.c_char_p, ctypes.c_char_p]),
         "orc_fire_state": (I, [P, dp]),
         "orc_compute_forces": (I, [P, dp]),
         "orc_integrate_once": (I, [P]),
@@ -235,23 +238,40 @@ class OracleSystem:
         MT19937 + polar Gaussian, drawn in the reference's per-particle order)."""
         self._chk(self.L.orc_set_rng_stream(self.h, RNG_STREAM[kind]))
 
+    def group(self, name, member):
+        """`group name {...}`: member[i] true puts particle i into the group."""
+        m = np.ascontiguousarray(np.asarray(member, dtype=bool).astype(np.int32))
+        assert m.shape == (self.n,)
+        self._chk(self.L.orc_group_define(self.h, name.encode(), m.ctypes.data))
+
+    def _group(self, group):
+        if group != "all":
+            self._chk(self.L.orc_set_integrator_group(self.h, group.encode()))
+
+    def disable(self, kind, group="all"):
+        self._chk(self.L.orc_integrator_disable(self.h, kind.encode(), group.encode()))
+
     def integrator_brownian(self, which="brownian", v0=1.0, nu=1.0, mu=1.0, mur=1.0, seed=0, nematic=False,
-                            tau=1.0, velocity_align=False, temp=(0, 0.0, 1.0, 1000)):
+                            tau=1.0, velocity_align=False, temp=(0, 0.0, 1.0, 1000), group="all"):
+        self._group(group)
         w = {"brownian": 0, "brownian_pos": 1, "brownian_align": 2}[which]
         self._chk(self.L.orc_add_brownian(self.h, w, v0, nu, mu, mur, int(seed), int(bool(nematic)), float(tau),
                                           int(bool(velocity_align)), int(temp[0]), float(temp[1]), float(temp[2]),
                                           int(temp[3])))
 
-    def integrator_langevin(self, gamma=1.0, seed=0, method="baoab", temp=(0, 0.0, 1.0, 1000)):
+    def integrator_langevin(self, gamma=1.0, seed=0, method="baoab", temp=(0, 0.0, 1.0, 1000), group="all"):
+        self._group(group)
         self._chk(self.L.orc_add_langevin(self.h, float(gamma), int(seed), LANGEVIN_METHOD[method], int(temp[0]),
                                           float(temp[1]), float(temp[2]), int(temp[3])))
 
     def integrator_fire(self, alpha=0.1, alpha_scale=0.99, f_tol=1e-4, e_tol=1e-6, dt_inc=1.1, dt_dec=0.5,
-                        dt_max=1.0, min_neg_steps=5):
+                        dt_max=1.0, min_neg_steps=5, group="all"):
+        self._group(group)
         self._chk(self.L.orc_add_fire(self.h, alpha, alpha_scale, f_tol, e_tol, dt_inc, dt_dec, dt_max,
                                       int(min_neg_steps)))
 
-    def integrator_nve(self, limit=None, angle_limit=None):
+    def integrator_nve(self, limit=None, angle_limit=None, group="all"):
+        self._group(group)
         self._chk(self.L.orc_add_nve(self.h, -1.0 if limit is None else float(limit),
                                      -1.0 if angle_limit is None else float(angle_limit)))
 

@@ -210,6 +210,7 @@ struct Constraint {
 
 struct Integ {
   std::string name;     // "brownian", "brownian_pos", "brownian_align", "langevin", "fire"
+  std::string group = "all";  // Integrator::m_group_name (src/integrators/integrator.hpp:98-108)
   double dt = 0.0;
   // temperature control (src/system/value.hpp:115-170): kind 0 constant, 1 linear
   int tkind = 0; double tmin = 0.0, tmax = 1.0; int tsteps = 1000;
@@ -258,6 +259,10 @@ struct Sim {
   bool has_self_prop = false; double sp_alpha = 1.0;
   Constraint con;
   std::map<std::string, Integ> integs;   // key: type + "_" + group (src/samos.cpp:666-690)
+  // Particle groups (System::make_group, src/system/system.cpp:462-570):
+  // member indices in the order the reference adds them; "all" holds everyone.
+  std::map<std::string, std::vector<int>> groups;
+  std::string next_group = "all";
   int ntypes = 1;
   int rng_stream = 0;   // noise stream of integrators added from now on
 };
@@ -289,6 +294,9 @@ void reset_forces(Sim& s) {
 void reset_torques(Sim& s) {
   for (auto& q : s.p) { q.tau_x = q.tau_y = q.tau_z = 0.0; }
 }
+
+// Group::get_particles of an integrator's group.
+const std::vector<int>& members(const Sim& s, const Integ& I) { return s.groups.at(I.group); }
 
 // ---------------------------------------------------------------------------
 // Cell list (src/system/cell_list.cpp:37-104)
@@ -916,13 +924,13 @@ inline double noise_flip(Integ& I, uint32_t tag, uint32_t pid, uint64_t call) {
 
 // IntegratorBrownian::integrate (src/integrators/integrator_brownian.cpp:42-124)
 void brownian(Sim& s, Integ& I) {
+  const std::vector<int>& G = members(s, I);
   double T = I.Tval(s.run_step);
   double B = std::sqrt(2.0 * I.mu * T);
   double sqrt_dt = std::sqrt(I.dt);
   uint64_t call = I.calls++;
-  int N = (int)s.p.size();
   if (I.nematic) {
-    for (int i = 0; i < N; i++) {
+    for (int i : G) {
       Particle& p = s.p[i];
       if (noise_flip(I, TAG_BROWNIAN, (uint32_t)p.id, call) < I.tau) {
         p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz;
@@ -930,7 +938,7 @@ void brownian(Sim& s, Integ& I) {
       }
     }
   }
-  for (int i = 0; i < N; i++) {
+  for (int i : G) {
     Particle& p = s.p[i];
     double fdx = I.v0 * p.nx + I.mu * p.fx;
     double fdy = I.v0 * p.ny + I.mu * p.fy;
@@ -955,11 +963,13 @@ void brownian(Sim& s, Integ& I) {
 
 // IntegratorBrownianPos::integrate (src/integrators/integrator_brownian_pos.cpp:37-84)
 void brownian_pos(Sim& s, Integ& I) {
+  const std::vector<int>& G = members(s, I);
   double T = I.Tval(s.run_step);
   double B = std::sqrt(2.0 * I.mu * T);
   double sqrt_dt = std::sqrt(I.dt);
   uint64_t call = I.calls++;
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     double fdx = I.mu * p.fx, fdy = I.mu * p.fy, fdz = I.mu * p.fz;
     p.vx = fdx; p.vy = fdy; p.vz = fdz;
     p.x += I.dt * p.vx; p.y += I.dt * p.vy; p.z += I.dt * p.vz;
@@ -976,13 +986,16 @@ void brownian_pos(Sim& s, Integ& I) {
 
 // IntegratorBrownianAlign::integrate (src/integrators/integrator_brownian_align.cpp:41-76)
 void brownian_align(Sim& s, Integ& I) {
+  const std::vector<int>& G = members(s, I);
   uint64_t call = I.calls++;
   if (I.nematic) {
-    for (auto& p : s.p) {
+    for (int gi : G) {
+    Particle& p = s.p[gi];
       if (noise_flip(I, TAG_BROWNIAN_ALIGN, (uint32_t)p.id, call) < I.tau) { p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz; }
     }
   }
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     p.omega = I.mur * project_torque(s, p);
     double g = noise_rot(I, TAG_BROWNIAN_ALIGN, (uint32_t)p.id, call);
     double dtheta = I.dt * p.omega + I.stoch * g;
@@ -992,12 +1005,14 @@ void brownian_align(Sim& s, Integ& I) {
 
 // IntegratorLangevin::integrate_baoab (src/integrators/integrator_langevin.cpp:54-118)
 void langevin_baoab(Sim& s, Integ& I) {
+  const std::vector<int>& G = members(s, I);
   double T = I.Tval(s.run_step);
   double B = std::sqrt(T * (1.0 - std::exp(-2.0 * I.gamma * I.dt)));
   double exp_dt = std::exp(-I.gamma * I.dt);
   double dt2 = 0.5 * I.dt;
   uint64_t call = I.calls++;
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     double fact = dt2 / p.mass;
     p.vx += fact * p.fx; p.vy += fact * p.fy; p.vz += fact * p.fz;
     p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
@@ -1013,7 +1028,8 @@ void langevin_baoab(Sim& s, Integ& I) {
   }
   reset_forces(s);
   pot_compute(s);
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     double fact = dt2 / p.mass;
     p.vx += fact * p.fx; p.vy += fact * p.fy; p.vz += fact * p.fz;
     p.age += I.dt;
@@ -1022,19 +1038,22 @@ void langevin_baoab(Sim& s, Integ& I) {
 
 // IntegratorLangevin::integrate_spv (src/integrators/integrator_langevin.cpp:128-183)
 void langevin_spv(Sim& s, Integ& I) {
+  const std::vector<int>& G = members(s, I);
   double T = I.Tval(s.run_step);
   double zeta = std::sqrt(T * (1.0 - std::exp(-2.0 * I.gamma * I.dt)));
   double eta = (I.gamma == 0.0) ? 0.0 : (1.0 - std::exp(-I.dt * I.gamma)) / I.gamma;
   double exp_dt = std::exp(-I.dt * I.gamma);
   double dt2 = 0.5 * I.dt;
   uint64_t call = I.calls++;
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
     enforce(s, p);
   }
   reset_forces(s);
   pot_compute(s);
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     double fact = eta / p.mass;
     p.vx = exp_dt * p.vx + fact * p.fx;
     p.vy = exp_dt * p.vy + fact * p.fy;
@@ -1054,13 +1073,15 @@ void langevin_spv(Sim& s, Integ& I) {
 // The stored random vectors R are indexed by particle id; the initial draw
 // (integrator_langevin.hpp:106-116) uses call counter 2^63.
 void langevin_bbk(Sim& s, Integ& I) {
+  const std::vector<int>& G = members(s, I);
   double T = I.Tval(s.run_step);
   double B = std::sqrt(2.0 * T * I.dt * I.gamma);
   double dt2 = 0.5 * I.dt;
   double one_m_dt2 = 1.0 - I.gamma * dt2;
   double one_div = 1.0 / (1.0 + I.gamma * dt2);
   uint64_t call = I.calls++;
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     double dm = dt2 / p.mass;
     p.vx = one_m_dt2 * p.vx + dm * p.fx;
     p.vy = one_m_dt2 * p.vy + dm * p.fy;
@@ -1074,7 +1095,8 @@ void langevin_bbk(Sim& s, Integ& I) {
   }
   reset_forces(s);
   pot_compute(s);
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     double dm = dt2 / p.mass;
     p.vx = one_div * (p.vx + dm * p.fx);
     p.vy = one_div * (p.vy + dm * p.fy);
@@ -1092,11 +1114,13 @@ void langevin_bbk(Sim& s, Integ& I) {
 // IntegratorFIRE::integrate (src/integrators/integrator_fire.cpp:34-148).  The
 // per-iteration stdout line is not printed; its three numbers are kept instead.
 void fire(Sim& s, Integ& I) {
-  int N = (int)s.p.size();
+  const std::vector<int>& G = members(s, I);
+  int N = (int)G.size();
   double sqrt_ndof = std::sqrt(3 * N);
   double dt_2 = 0.5 * I.dt;
   double P = 0.0, Fn = 0.0, Vn = 0.0;
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     p.vx += dt_2 * p.fx; p.vy += dt_2 * p.fy; p.vz += dt_2 * p.fz;
     p.x += I.dt * p.vx; p.y += I.dt * p.vy; p.z += I.dt * p.vz;
     enforce(s, p);
@@ -1108,7 +1132,8 @@ void fire(Sim& s, Integ& I) {
   reset_torques(s);
   pot_compute(s);
   align_compute(s);
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     p.vx += dt_2 * p.fx; p.vy += dt_2 * p.fy; p.vz += dt_2 * p.fz;
     P += p.vx * p.fx + p.vy * p.fy + p.vz * p.fz;
     Fn += p.fx * p.fx + p.fy * p.fy + p.fz * p.fz;
@@ -1127,7 +1152,8 @@ void fire(Sim& s, Integ& I) {
   double inv_F = 1.0 / Fn;
   double f1 = 1.0 - I.alpha;
   double f2 = I.alpha * inv_F * Vn;
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     p.vx = f1 * p.vx + f2 * p.fx;
     p.vy = f1 * p.vy + f2 * p.fy;
     p.vz = f1 * p.vz + f2 * p.fz;
@@ -1139,7 +1165,8 @@ void fire(Sim& s, Integ& I) {
     I.dt *= I.f_dec;
     I.alpha = I.alpha_init;
     I.last_neg = 0;
-    for (auto& p : s.p) { p.vx = 0.0; p.vy = 0.0; p.vz = 0.0; }
+    for (int gi : G) {
+    Particle& p = s.p[gi]; p.vx = 0.0; p.vy = 0.0; p.vz = 0.0; }
   }
   I.converged = false;
   I.old_energy = E;
@@ -1152,13 +1179,16 @@ void fire(Sim& s, Integ& I) {
 inline double sign_of(double x) { return x >= 0.0 ? 1.0 : -1.0; }   // integrator_nve.hpp:38
 
 void nve(Sim& s, Integ& I) {
+  const std::vector<int>& G = members(s, I);
   double dt_2 = 0.5 * I.dt;
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     p.vx += dt_2 * p.fx; p.vy += dt_2 * p.fy; p.vz += dt_2 * p.fz;
     enforce(s, p);
     p.omega += dt_2 * project_torque(s, p);
   }
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     double dx = I.dt * p.vx, dy = I.dt * p.vy, dz = I.dt * p.vz;
     if (I.has_limit) {
       double dr = std::sqrt(dx * dx + dy * dy + dz * dz);
@@ -1171,7 +1201,8 @@ void nve(Sim& s, Integ& I) {
     p.x += dx; p.y += dy; p.z += dz;
     enforce(s, p);
   }
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     double dtheta = I.dt * p.omega;
     if (I.has_theta_limit && std::fabs(dtheta) > I.theta_limit) dtheta = sign_of(dtheta) * I.theta_limit;
     rotate_director(s, p, dtheta);
@@ -1180,7 +1211,8 @@ void nve(Sim& s, Integ& I) {
   reset_torques(s);
   pot_compute(s);
   align_compute(s);
-  for (auto& p : s.p) {
+  for (int gi : G) {
+    Particle& p = s.p[gi];
     p.vx += dt_2 * p.fx; p.vy += dt_2 * p.fy; p.vz += dt_2 * p.fz;
     if (I.has_limit) {
       double v = std::sqrt(p.vx * p.vx + p.vy * p.vy + p.vz * p.vz);
@@ -1256,7 +1288,40 @@ void* orc_create(int n, double lx, double ly, double lz, int periodic) {
   }
   s->box.set(lx, ly, lz);
   s->periodic = periodic != 0;
+  std::vector<int>& all = s->groups["all"];
+  for (int i = 0; i < n; i++) all.push_back(i);
   return s;
+}
+
+// `group name { ... }` (src/samos.cpp:1108-1140): member[i] != 0 puts particle i
+// into the group; a group defined again gains the new members (the reference
+// appends them to its list; a particle listed twice is kept once here).
+int orc_group_define(void* h, const char* name, const int* member) {
+  Sim* s = static_cast<Sim*>(h);
+  const std::string nm = name;
+  if (nm == "all") { g_err = "group all always holds every particle"; return -1; }
+  std::vector<int>& g = s->groups[nm];
+  std::vector<char> in(s->p.size(), 0);
+  for (int i : g) in[i] = 1;
+  for (int i = 0; i < (int)s->p.size(); i++) if (member[i] && !in[i]) g.push_back(i);
+  return 0;
+}
+
+// The `group` parameter of the next integrator added (src/samos.cpp:654-690).
+int orc_set_integrator_group(void* h, const char* group) {
+  Sim* s = static_cast<Sim*>(h);
+  if (!s->groups.count(group)) { g_err = std::string("Group ") + group + " does not exist."; return -1; }
+  s->next_group = group;
+  return 0;
+}
+
+// `disable type { group = g }`: removes that integrator.
+int orc_integrator_disable(void* h, const char* type, const char* group) {
+  Sim* s = static_cast<Sim*>(h);
+  auto it = s->integs.find(std::string(type) + "_" + group);
+  if (it == s->integs.end()) { g_err = "Integrator does not exist."; return -1; }
+  s->integs.erase(it);
+  return 0;
 }
 
 void orc_destroy(void* h) { delete static_cast<Sim*>(h); }
@@ -1484,7 +1549,20 @@ int orc_project_torques(void* h, double* out) {
 static Integ make_integ(Sim* s, const char* name, double dt) {
   Integ I; I.name = name; I.dt = dt > 0.0 ? dt : s->dt;
   I.stream = s->rng_stream;
+  I.group = s->next_group;
+  s->next_group = "all";
   return I;
+}
+
+// Registers under type_group (src/samos.cpp:666-690); an existing key is an error.
+static int put_integ(Sim* s, const Integ& I) {
+  const std::string key = I.name + "_" + I.group;
+  if (s->integs.count(key)) {
+    g_err = "Integrator of type " + I.name + " (for group " + I.group + ") already exists.";
+    return -1;
+  }
+  s->integs[key] = I;
+  return 0;
 }
 
 // Noise stream for integrators added after this call: 0 counter-based Philox
@@ -1516,8 +1594,7 @@ int orc_add_brownian(void* h, int which, double v0, double nu, double mu, double
   I.nematic = nematic != 0; I.tau = I.dt / tau; I.velocity = velocity_align != 0;
   I.tkind = tkind; I.tmin = tmin; I.tmax = tmax; I.tsteps = tsteps;
   I.stoch = std::sqrt(I.nu * I.dt);
-  s->integs[std::string(nm) + "_all"] = I;
-  return 0;
+  return put_integ(s, I);
 }
 
 int orc_add_langevin(void* h, double gamma, int seed, int method, int tkind, double tmin, double tmax, int tsteps) {
@@ -1535,8 +1612,7 @@ int orc_add_langevin(void* h, double gamma, int seed, int method, int tkind, dou
       I.Rx[i] = g[0]; I.Ry[i] = g[1]; I.Rz[i] = g[2];
     }
   }
-  s->integs["langevin_all"] = I;
-  return 0;
+  return put_integ(s, I);
 }
 
 int orc_add_fire(void* h, double alpha, double alpha_scale, double f_tol, double e_tol,
@@ -1546,8 +1622,7 @@ int orc_add_fire(void* h, double alpha, double alpha_scale, double f_tol, double
   if (I.dt <= 0.0) { g_err = "Integrator step has not been set."; return -1; }
   I.alpha = alpha; I.alpha_init = alpha; I.f_alpha = alpha_scale; I.F_tol = f_tol; I.E_tol = e_tol;
   I.f_inc = dt_inc; I.f_dec = dt_dec; I.dt_max = dt_max; I.N_min = min_neg;
-  s->integs["fire_all"] = I;
-  return 0;
+  return put_integ(s, I);
 }
 
 // NVE; a negative limit or angle_limit means the parameter is not given.
@@ -1557,14 +1632,14 @@ int orc_add_nve(void* h, double limit, double angle_limit) {
   if (I.dt <= 0.0) { g_err = "Integrator step has not been set."; return -1; }
   I.has_limit = limit >= 0.0; I.limit = limit;
   I.has_theta_limit = angle_limit >= 0.0; I.theta_limit = angle_limit;
-  s->integs["nve_all"] = I;
-  return 0;
+  return put_integ(s, I);
 }
 
 // FIRE state: dt, alpha, P, |F|, |v|, E, converged, last_neg
 int orc_fire_state(void* h, double* out) {
   Sim* s = static_cast<Sim*>(h);
-  auto it = s->integs.find("fire_all");
+  auto it = s->integs.begin();
+  while (it != s->integs.end() && it->second.name != "fire") ++it;
   if (it == s->integs.end()) return -1;
   Integ& I = it->second;
   out[0] = I.dt; out[1] = I.alpha; out[2] = I.last_P; out[3] = I.last_Fnorm; out[4] = I.last_Vnorm;

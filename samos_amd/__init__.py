@@ -89,6 +89,10 @@ def load_library(path: str = LIB_PATH):
         "samos_run_range": (I, [P, I, I, I, I]),
         "samos_energies": (I, [P, dp]),
         "samos_integrator_remove": (I, [P, ctypes.c_char_p]),
+        "samos_integrator_disable": (I, [P, ctypes.c_char_p, ctypes.c_char_p]),
+        "samos_group_define": (I, [P, ctypes.c_char_p, P]),
+        "samos_group_size": (I, [P, ctypes.c_char_p, ip]),
+        "samos_set_integrator_group": (I, [P, ctypes.c_char_p]),
         "samos_reduce": (I, [P, I, dp]),
         "samos_profile": (I, [P, I]),
         "samos_profile_read": (I, [P, ctypes.c_char_p, dp, llp]),
@@ -281,19 +285,41 @@ class System:
     def enforce_all(self):
         self._chk(self.L.samos_enforce_all(self.h))
 
+    def group(self, name, member):
+        """`group name {...}` (src/samos.cpp:1108-1140): member[i] true puts
+        particle i (id order) into the group."""
+        m = np.ascontiguousarray(np.asarray(member, dtype=bool).astype(np.int32))
+        assert m.shape == (self.n,)
+        self._chk(self.L.samos_group_define(self.h, name.encode(), m.ctypes.data))
+
+    def group_size(self, name):
+        v = ctypes.c_int()
+        self._chk(self.L.samos_group_size(self.h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def _group(self, group):
+        if group != "all":
+            self._chk(self.L.samos_set_integrator_group(self.h, group.encode()))
+
+    def disable(self, kind, group="all"):
+        self._chk(self.L.samos_integrator_disable(self.h, kind.encode(), group.encode()))
+
     def integrator_brownian(self, which="brownian", v0=1.0, nu=1.0, mu=1.0, mur=1.0, seed=0, nematic=False,
-                            tau=1.0, velocity_align=False, temp=(0, 0.0, 1.0, 1000)):
+                            tau=1.0, velocity_align=False, temp=(0, 0.0, 1.0, 1000), group="all"):
+        self._group(group)
         w = {"brownian": 0, "brownian_pos": 1, "brownian_align": 2}[which]
         self._chk(self.L.samos_integrator_brownian(self.h, w, v0, nu, mu, mur, int(seed), int(bool(nematic)),
                                                    float(tau), int(bool(velocity_align)), int(temp[0]),
                                                    float(temp[1]), float(temp[2]), int(temp[3])))
 
-    def integrator_langevin(self, gamma=1.0, seed=0, method="baoab", temp=(0, 0.0, 1.0, 1000)):
+    def integrator_langevin(self, gamma=1.0, seed=0, method="baoab", temp=(0, 0.0, 1.0, 1000), group="all"):
+        self._group(group)
         self._chk(self.L.samos_integrator_langevin(self.h, float(gamma), int(seed), LANGEVIN_METHOD[method],
                                                    int(temp[0]), float(temp[1]), float(temp[2]), int(temp[3])))
 
     def integrator_fire(self, alpha=0.1, alpha_scale=0.99, f_tol=1e-4, e_tol=1e-6, dt_inc=1.1, dt_dec=0.5,
-                        dt_max=1.0, min_neg_steps=5):
+                        dt_max=1.0, min_neg_steps=5, group="all"):
+        self._group(group)
         self._chk(self.L.samos_integrator_fire(self.h, alpha, alpha_scale, f_tol, e_tol, dt_inc, dt_dec, dt_max,
                                                int(min_neg_steps)))
 
@@ -310,8 +336,9 @@ class System:
         self._chk(self.L.samos_pair_extra_energy(self.h, ctypes.byref(e)))
         return e.value
 
-    def integrator_nve(self, limit=None, angle_limit=None):
+    def integrator_nve(self, limit=None, angle_limit=None, group="all"):
         """IntegratorNVE {limit; angle_limit} (src/integrators/integrator_nve.hpp:59-86)."""
+        self._group(group)
         self._chk(self.L.samos_integrator_nve(self.h, -1.0 if limit is None else float(limit),
                                               -1.0 if angle_limit is None else float(angle_limit)))
 

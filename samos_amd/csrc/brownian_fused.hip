@@ -89,35 +89,56 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
     constexpr int kSet = PLANAR ? kSurfPlane : kSurfAll;
     bool fz0 = false;
     double om, age_dt;
-    if (ba.mode == 0) {
+    // The integrator of this slot: with groups, the one whose group holds it
+    // (the groups are disjoint); a slot in none of them is not integrated.
+    bool member = true;
+    double q_dt = ba.dt, q_v0 = ba.v0, q_mu = ba.mu, q_mur = ba.mur, q_stoch = ba.stoch, q_flip = ba.flip_p;
+    int q_nem = ba.nematic, q_vel = ba.velocity, q_cs = ba.clock_slot;
+    TempD q_temp = ba.temp;
+    uint32_t q_seed = ba.seed;
+    if (MT && ba.ngroups > 0) {
+      const unsigned gm = (unsigned)cur.groups[s];
+      int k = 0;
+      while (k < ba.ngroups && !(gm & ba.grp[k].gbit)) ++k;
+      member = k < ba.ngroups;
+      if (member) {
+        const BrownArgs::Group& g = ba.grp[k];
+        q_dt = g.dt; q_v0 = g.v0; q_mu = g.mu; q_mur = g.mur; q_stoch = g.stoch; q_flip = g.flip_p;
+        q_nem = g.nematic; q_vel = g.velocity; q_cs = g.clock_slot; q_temp = g.temp; q_seed = g.seed;
+      }
+    }
+    if (!member) {
+      om = cur.omega[s];
+      age_dt = 0.0;
+    } else if (ba.mode == 0) {
       // IntegratorBrownian::integrate for this particle, with the forces and
       // torques just computed
-      const unsigned long long call = eff_call(ba.clock, ba.clock_slot, 0ULL) + (unsigned long long)ba.t_off;
+      const unsigned long long call = eff_call(ba.clock, q_cs, 0ULL) + (unsigned long long)ba.t_off;
       double g_rot = 0.0, u_flip = 1.0;
-      gauss1_u1(ba.seed, TAG_BROWNIAN, pid, call, g_rot, u_flip);
-      if (ba.nematic && u_flip < ba.flip_p) {
+      gauss1_u1(q_seed, TAG_BROWNIAN, pid, call, g_rot, u_flip);
+      if (q_nem && u_flip < q_flip) {
         p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz;
-        if (ba.velocity) { p.vx = -p.vx; p.vy = -p.vy; p.vz = -p.vz; }
+        if (q_vel) { p.vx = -p.vx; p.vy = -p.vy; p.vz = -p.vz; }
       }
-      const double T = ba.temp.at(eff_step(ba.clock, ba.t_off));
-      const double B = sqrt(2.0 * ba.mu * T);
-      const double sqrt_dt = sqrt(ba.dt);
-      const double fdx = ba.v0 * p.nx + ba.mu * fx, fdy = ba.v0 * p.ny + ba.mu * fy, fdz = ba.v0 * p.nz + ba.mu * fz;
+      const double T = q_temp.at(eff_step(ba.clock, ba.t_off));
+      const double B = sqrt(2.0 * q_mu * T);
+      const double sqrt_dt = sqrt(q_dt);
+      const double fdx = q_v0 * p.nx + q_mu * fx, fdy = q_v0 * p.ny + q_mu * fy, fdz = q_v0 * p.nz + q_mu * fz;
       p.vx = fdx; p.vy = fdy; p.vz = fdz;
-      p.x += ba.dt * fdx; p.y += ba.dt * fdy; p.z += ba.dt * fdz;
+      p.x += q_dt * fdx; p.y += q_dt * fdy; p.z += q_dt * fdz;
       if (T > 0.0) {
         double g[3];
-        gauss3(ba.seed, TAG_BROWNIAN, pid, call, g);
+        gauss3(q_seed, TAG_BROWNIAN, pid, call, g);
         const double frx = B * g[0], fry = B * g[1], frz = B * g[2];
         p.vx += frx; p.vy += fry; p.vz += frz;
         p.x += sqrt_dt * frx; p.y += sqrt_dt * fry; p.z += sqrt_dt * frz;
       }
       fz0 = enforce<kSet>(b, c, p);
-      om = ba.mur * project_torque<kSet>(c, p, tq[0], tq[1], tq[2]);
-      const double dtheta = ba.dt * om + ba.stoch * g_rot;
+      om = q_mur * project_torque<kSet>(c, p, tq[0], tq[1], tq[2]);
+      const double dtheta = q_dt * om + q_stoch * g_rot;
       rotate_director<kSet>(c, p, dtheta);
-      if (ba.velocity) rotate_velocity<kSet>(c, p, dtheta);
-      age_dt = ba.dt;
+      if (q_vel) rotate_velocity<kSet>(c, p, dtheta);
+      age_dt = q_dt;
     } else {
       // IntegratorBrownianAlign (directors, at the loop-head position), then
       // IntegratorBrownianPos (positions), each with its own noise stream
@@ -150,7 +171,7 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
       cur.fx[s] = fx; cur.fy[s] = fy; cur.fz[s] = fz0 ? 0.0 : fz;
       if (AL) { cur.tx[s] = tq[0]; cur.ty[s] = tq[1]; cur.tz[s] = tq[2]; }
       cur.omega[s] = om;
-      if (!(ba.age && ba.age->uniform)) cur.age[s] += age_dt;
+      if (member && !(ba.age && ba.age->uniform)) cur.age[s] += age_dt;
       cur.vx[s] = p.vx; cur.vy[s] = p.vy;
       if (!PLANAR) cur.vz[s] = p.vz;
       if (PLANAR) {

@@ -21,7 +21,9 @@ namespace {
 thread_local std::string g_create_error;
 
 struct Integ {
-  std::string name;   // brownian, brownian_pos, brownian_align, langevin, fire
+  std::string name;     // brownian, brownian_pos, brownian_align, langevin, fire
+  std::string group = "all";  // Integrator::m_group_name (src/integrators/integrator.hpp:98-108)
+  unsigned gbit = 1u;         // that group's bit in the per-slot group column
   int slot = 0;       // clock slot
   double dt = 0.0;
   TempD temp{0, 0.0, 1.0, 1000};
@@ -51,11 +53,17 @@ struct samos_ctx {
   BoxD box{};
   std::string err;
   double* dcol[kNumDoubleFields] = {};
-  int* icol[kNumIntFields] = {};      // type, ix, iy, iz, coord, id
+  int* icol[kNumIntFields] = {};      // type, ix, iy, iz, coord, id, groups
   double* alt[3] = {};                // second position buffer for fused steps
   double* altn[3] = {};               // second director buffer for fused Brownian steps
   double2* xy[2] = {};                // interleaved (x, y) of the two position buffers
   std::vector<int> h_id;              // slot -> id
+  // particle groups (System::make_group): bit index per name ("all" = 0) and
+  // each particle's bits in id order; the device holds them per slot
+  std::map<std::string, int> group_index{{"all", 0}};
+  std::vector<unsigned> h_groups;
+  std::string next_group = "all";     // group of the next integrator created
+  std::string last_key;               // map key of the integrator created last
   double dt = 0.0;
   int ntypes = 1;
   // neighbour list
@@ -163,7 +171,7 @@ Fields fields(samos_ctx* ctx, int parity = 0) {
   f.fx = d[9]; f.fy = d[10]; f.fz = d[11]; f.tx = d[12]; f.ty = d[13]; f.tz = d[14];
   f.omega = d[15]; f.age = d[16]; f.radius = d[17]; f.mass = d[18];
   f.type = ctx->icol[0]; f.ix = ctx->icol[1]; f.iy = ctx->icol[2]; f.iz = ctx->icol[3];
-  f.coord = ctx->icol[4]; f.id = ctx->icol[5];
+  f.coord = ctx->icol[4]; f.id = ctx->icol[5]; f.groups = ctx->icol[6];
   f.xy = nullptr;
   return f;
 }
@@ -256,8 +264,8 @@ void key_context(KeyBuf& k, const samos_ctx* c) {
 void key_integ(KeyBuf& k, const Integ& I) {
   k << I.name.size();
   k.s += I.name;
-  k << I.slot << I.dt << I.temp << I.v0 << I.nu << I.mu << I.mur << I.stoch << I.flip_p << I.nematic << I.velocity
-    << I.seed << I.gamma << I.method << I.R << I.limit << I.theta_limit;
+  k << I.slot << I.gbit << I.dt << I.temp << I.v0 << I.nu << I.mu << I.mur << I.stoch << I.flip_p << I.nematic
+    << I.velocity << I.seed << I.gamma << I.method << I.R << I.limit << I.theta_limit;
 }
 
 // The instantiated graph of `key`, recorded by `record` on first use.  The
@@ -496,6 +504,25 @@ void check_and_rebuild(samos_ctx* ctx) {
 }
 
 // ---- integrators ----------------------------------------------------------
+int group_size(const samos_ctx* ctx, unsigned gbit) {
+  if (gbit == 1u) return ctx->n;
+  int c = 0;
+  for (unsigned g : ctx->h_groups) c += (g & gbit) ? 1 : 0;
+  return c;
+}
+
+// True when no particle belongs to two of the integrators' groups.
+bool groups_disjoint(const samos_ctx* ctx, const std::vector<const Integ*>& v) {
+  for (size_t a = 0; a < v.size(); ++a)
+    for (size_t b = a + 1; b < v.size(); ++b) {
+      if (v[a]->gbit == 1u || v[b]->gbit == 1u) return false;
+      const unsigned both = v[a]->gbit | v[b]->gbit;
+      for (unsigned g : ctx->h_groups)
+        if ((g & both) == both) return false;
+    }
+  return true;
+}
+
 // j >= 0: the step is the j-th of a captured chunk; its noise call and run
 // step are read from the device clock plus j instead of being baked in.
 int integrate_one(samos_ctx* ctx, Integ& I, int run_step, int j = -1) {
@@ -510,36 +537,36 @@ int integrate_one(samos_ctx* ctx, Integ& I, int run_step, int j = -1) {
     int which = I.name == "brownian" ? 0 : (I.name == "brownian_pos" ? 1 : 2);
     ProfScope ps(ctx, "integrate");
     launch_brownian(ctx->st, which, f, ctx->box, ctx->con, n, I.dt, I.v0, I.mu, I.mur, I.stoch, I.flip_p, I.nematic,
-                    I.velocity, I.temp, I.seed, call, clk, I.slot, t_off);
+                    I.velocity, I.temp, I.seed, call, clk, I.slot, t_off, I.gbit);
   } else if (I.name == "langevin") {
     {
       ProfScope ps(ctx, "integrate");
       launch_langevin_a(ctx->st, I.method, f, ctx->box, ctx->con, n, I.dt, I.gamma, I.temp, I.seed, call, clk,
-                        I.slot, t_off, I.R);
+                        I.slot, t_off, I.R, I.gbit);
     }
     interact(ctx, pot_mask(ctx), 0, 0, nullptr);
     ProfScope ps(ctx, "integrate");
     launch_langevin_b(ctx->st, I.method, f, n, I.dt, I.gamma, I.temp, I.seed, call, clk, I.slot, t_off, I.R,
-                      ctx->box, ctx->con);
+                      ctx->box, ctx->con, I.gbit);
   } else if (I.name == "nve") {
     {
       ProfScope ps(ctx, "integrate");
-      launch_nve_a(ctx->st, f, ctx->box, ctx->con, n, I.dt, I.limit, I.theta_limit);
+      launch_nve_a(ctx->st, f, ctx->box, ctx->con, n, I.dt, I.limit, I.theta_limit, I.gbit);
     }
     // reset forces and torques, Potential::compute, Aligner::compute
     loop_head_forces(ctx, nullptr);
     ProfScope ps(ctx, "integrate");
-    launch_nve_b(ctx->st, f, ctx->box, ctx->con, n, I.dt, I.limit, I.theta_limit);
+    launch_nve_b(ctx->st, f, ctx->box, ctx->con, n, I.dt, I.limit, I.theta_limit, I.gbit);
   } else if (I.name == "fire") {
     {
       ProfScope ps(ctx, "integrate");
-      launch_fire_a(ctx->st, f, ctx->box, ctx->con, n, I.dt);
+      launch_fire_a(ctx->st, f, ctx->box, ctx->con, n, I.dt, I.gbit);
     }
     // reset forces and torques, Potential::compute, Aligner::compute
     loop_head_forces(ctx, ctx->d_en);
     {
       ProfScope ps(ctx, "integrate");
-      launch_fire_b(ctx->st, f, ctx->box, ctx->con, n, I.dt, ctx->d_cols, ctx->np);
+      launch_fire_b(ctx->st, f, ctx->box, ctx->con, n, I.dt, ctx->d_cols, ctx->np, I.gbit);
     }
     launch_reduce_sum(ctx->st, ctx->d_cols, 3, ctx->np, n, ctx->d_partial, ctx->d_out);
     launch_reduce_sum(ctx->st, ctx->d_en, 2, ctx->np, n, ctx->d_partial, ctx->d_out + 3);
@@ -553,7 +580,7 @@ int integrate_one(samos_ctx* ctx, Integ& I, int run_step, int j = -1) {
     const double Vn = std::sqrt(ctx->h_out[2]);
     double E = 0.5 * ctx->h_out[3] + 0.5 * ctx->h_out[4];
     if (ctx->extra.count > 0) E += 0.5 * ctx->h_out[5];
-    const double sqrt_ndof = std::sqrt((double)(3 * n));
+    const double sqrt_ndof = std::sqrt((double)(3 * group_size(ctx, I.gbit)));
     I.last[0] = P; I.last[1] = Fn; I.last[2] = Vn; I.last[3] = E;
     if (Fn / sqrt_ndof < I.F_tol && std::fabs(E - I.old_energy) < I.E_tol) {
       I.converged = true;
@@ -572,7 +599,7 @@ int integrate_one(samos_ctx* ctx, Integ& I, int run_step, int j = -1) {
       I.last_neg = 0;
       zero_v = 1;
     }
-    launch_fire_mix(ctx->st, f, n, f1, f2, zero_v);
+    launch_fire_mix(ctx->st, f, n, f1, f2, zero_v, I.gbit);
     I.converged = false;
     I.old_energy = E;
   }
@@ -611,7 +638,9 @@ int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int rel, int t_off, i
     fa.exp_dt = std::exp(-I.gamma * I.dt);
     fa.Bc = I.temp.kind == SAMOS_VALUE_CONSTANT ? std::sqrt(I.temp.tmin * (1.0 - std::exp(-2.0 * I.gamma * I.dt))) : -1.0;
     fa.batch = ctx->gather_batch;
-    fa.age = ctx->d_age;
+    // with a group other than "all" only members age: no common age then
+    fa.age = I.gbit == 1u ? ctx->d_age : nullptr;
+    fa.gbit = I.gbit;
     fa.chunk = chunk;
     fa.slot = slot;
     launch_baoab_fused(ctx->st, pot_mask(ctx), phase, cur, nxt, ctx->nl, ctx->box, ctx->con, interact_params(ctx),
@@ -656,11 +685,13 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   double mass_u = -1.0;
   if (int r = uniform_mass(ctx, &mass_u)) return r;
-  if (!ctx->age_checked) {
+  if (I.gbit != 1u) {
+    ctx->age_checked = false;  // each member's age advances in the step kernel
+  } else if (!ctx->age_checked) {
     launch_age_check(ctx->st, ctx->d_age, ctx->dcol[16], ctx->n);
     ctx->age_checked = true;
   }
-  if (tile_eligible(ctx, flags, mass_u)) {
+  if (I.gbit == 1u && tile_eligible(ctx, flags, mass_u)) {
     const int r = run_tiled_langevin(ctx, I, a, b, steps, flags, mass_u);
     if (r <= 0) return r;
   }
@@ -741,7 +772,7 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
       HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->alt[k], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
   }
   // one closing half-kick, and one age step, per iteration
-  launch_age_finish(ctx->st, ctx->d_age, ctx->dcol[16], ctx->n, I.dt, b - a);
+  if (I.gbit == 1u) launch_age_finish(ctx->st, ctx->d_age, ctx->dcol[16], ctx->n, I.dt, b - a);
   HIPCHK(hipGetLastError());
   I.calls += (unsigned long long)(b - a);
   return 0;
@@ -756,17 +787,35 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
 // primary columns hold the loop-head state of a; on return that of b.
 // Either one "brownian" integrator, or exactly "brownian_align" and
 // "brownian_pos" (both over all particles, the reference's example confs).
+// Or up to four "brownian" integrators on pairwise disjoint groups (each
+// particle is integrated by at most one of them, so their map order does not
+// matter), such as the active and passive groups of a two-component conf.
+bool brownian_groups(const samos_ctx* ctx) {
+  if (ctx->integs.empty() || ctx->integs.size() > 4) return false;
+  std::vector<const Integ*> v;
+  for (auto& kv : ctx->integs) {
+    if (kv.second.name != "brownian") return false;
+    v.push_back(&kv.second);
+  }
+  return v.size() == 1 || groups_disjoint(ctx, v);
+}
+
 bool fused_brownian_eligible(samos_ctx* ctx, int flags) {
   if ((flags & SAMOS_RUN_FAITHFUL) || ctx->extra.count != 0) return false;
   if (!needs_nlist(ctx) || !brownian_fused_supported(pot_mask(ctx), al_mask(ctx))) return false;
-  if (ctx->integs.size() == 1) return ctx->integs.begin()->second.name == "brownian";
+  if (brownian_groups(ctx)) return true;
   return ctx->integs.size() == 2 && ctx->integs.count("brownian_align_all") && ctx->integs.count("brownian_pos_all");
 }
 
 int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
-  const bool pair = ctx->integs.size() == 2;
+  const bool pair = !brownian_groups(ctx);
   Integ& I = pair ? ctx->integs["brownian_align_all"] : ctx->integs.begin()->second;
   Integ* P = pair ? &ctx->integs["brownian_pos_all"] : nullptr;
+  // integrators on groups other than "all": per-slot parameters (multi-type kernels)
+  std::vector<Integ*> grouped;
+  if (!pair && !(ctx->integs.size() == 1 && I.gbit == 1u))
+    for (auto& kv : ctx->integs) grouped.push_back(&kv.second);
+  const bool multi = ctx->ntypes > 1 || !grouped.empty();
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   const bool planar = ctx->con.kind == 1 && ctx->nl.use_cell;
   const int n = ctx->n;
@@ -776,7 +825,9 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
     if (planar) f.xy = ctx->xy[par];
     return f;
   };
-  if (!ctx->age_checked) {
+  if (!grouped.empty()) {
+    ctx->age_checked = false;  // only members age
+  } else if (!ctx->age_checked) {
     launch_age_check(ctx->st, ctx->d_age, ctx->dcol[16], n);
     ctx->age_checked = true;
   }
@@ -796,7 +847,15 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
     ba.temp = I.temp; ba.seed = I.seed; ba.clock = ctx->d_clock; ba.clock_slot = I.slot; ba.t_off = t_off;
     ba.parity = par; ba.head_enforce = t + 1 <= steps ? 1 : 0;
     ba.chunk = chunk; ba.slot = slot;
-    ba.age = ctx->d_age;
+    ba.age = grouped.empty() ? ctx->d_age : nullptr;
+    ba.ngroups = (int)grouped.size();
+    for (int k = 0; k < ba.ngroups; ++k) {
+      const Integ& G = *grouped[k];
+      BrownArgs::Group& q = ba.grp[k];
+      q.dt = G.dt; q.v0 = G.v0; q.mu = G.mu; q.mur = G.mur; q.stoch = G.stoch; q.flip_p = G.flip_p;
+      q.nematic = G.nematic ? 1 : 0; q.velocity = G.velocity ? 1 : 0; q.temp = G.temp; q.seed = G.seed;
+      q.clock_slot = G.slot; q.gbit = G.gbit;
+    }
     if (pair) {
       ba.mode = 1;
       ba.pos_dt = P->dt; ba.pos_mu = P->mu; ba.pos_temp = P->temp; ba.pos_seed = P->seed; ba.pos_slot = P->slot;
@@ -809,7 +868,7 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
     {
       ProfScope ps(ctx, "fused");
       launch_brownian_fused(ctx->st, pot_mask(ctx), al_mask(ctx), cur, nxt, ctx->nl, ctx->box, ctx->con,
-                            interact_params(ctx), n, ba, f32 ? 1 : 0, planar ? 1 : 0, ctx->ntypes > 1 ? 1 : 0);
+                            interact_params(ctx), n, ba, f32 ? 1 : 0, planar ? 1 : 0, multi ? 1 : 0);
     }
     if (!chunk) {
       ProfScope ps(ctx, "rebuild");
@@ -825,6 +884,7 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
     key_context(base, ctx);
     key_integ(base, I);
     if (P) key_integ(base, *P);
+    for (const Integ* G : grouped) key_integ(base, *G);
     auto record = [&](int m) -> int {
       for (int j = 0; j < m; ++j) step(b - 2, j & 1, j, ctx->d_chunk, j);
       launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, n, planar ? 1 : 0, 1);
@@ -853,8 +913,9 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
   if (pf)
     for (int k = 0; k < (planar ? 2 : 3); ++k)
       HIPCHK(hipMemcpyAsync(ctx->dcol[6 + k], ctx->altn[k], sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
-  launch_age_finish(ctx->st, ctx->d_age, ctx->dcol[16], n, P ? P->dt : I.dt, b - a);
-  I.calls += (unsigned long long)(b - a);
+  if (grouped.empty()) launch_age_finish(ctx->st, ctx->d_age, ctx->dcol[16], n, P ? P->dt : I.dt, b - a);
+  if (grouped.empty()) I.calls += (unsigned long long)(b - a);
+  for (Integ* G : grouped) G->calls += (unsigned long long)(b - a);
   if (P) P->calls += (unsigned long long)(b - a);
   // loop head of b: its enforce was applied with the positions; forces and torques
   if (b <= steps) loop_head_forces(ctx, nullptr, f32);
@@ -1163,6 +1224,7 @@ int samos_create(samos_ctx** out, int device, int n, double lx, double ly, doubl
   for (int k = 0; k < kNumIntFields; ++k) hipMemsetAsync(ctx->icol[k], 0, sizeof(int) * np, ctx->st);
   ctx->h_id.resize(n);
   for (int i = 0; i < n; ++i) ctx->h_id[i] = i;
+  ctx->h_groups.assign(n, 1u);  // every particle is in group "all"
   std::vector<double> ones(n, 1.0);
   std::vector<int> ione(n, 1);
   if (n > 0) {
@@ -1171,6 +1233,7 @@ int samos_create(samos_ctx** out, int device, int n, double lx, double ly, doubl
     hipMemcpyAsync(ctx->dcol[18], ones.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st);
     hipMemcpyAsync(ctx->icol[0], ione.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->st);
     hipMemcpyAsync(ctx->icol[5], ctx->h_id.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->st);
+    hipMemcpyAsync(ctx->icol[6], ione.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->st);
   }
   if (hipStreamSynchronize(ctx->st) != hipSuccess) return bad("initialisation");
   *out = ctx;
@@ -1521,13 +1584,18 @@ int samos_enforce_all(samos_ctx* ctx) {
 }
 
 static int add_integ(samos_ctx* ctx, Integ& I, const std::string& name) {
+  const std::string group = ctx->next_group;
+  ctx->next_group = "all";
   if (ctx->dt <= 0.0)
     return fail(ctx, "Integrator step size has to be larger than 0. Did you forget to use \"timestep\" command?");
-  const std::string key = name + "_all";
+  const std::string key = name + "_" + group;
   if (ctx->integs.count(key))
-    return fail(ctx, "Integrator of type " + name + " (for group all) already exists. You need to disable the old one first.");
+    return fail(ctx, "Integrator of type " + name + " (for group " + group +
+                         ") already exists. You need to disable the old one first.");
   if (ctx->integs.size() >= 8) return fail(ctx, "too many integrators");
   I.name = name;
+  I.group = group;
+  I.gbit = 1u << ctx->group_index.at(group);
   I.dt = ctx->dt;
   int slot = 0;
   for (;; ++slot) {
@@ -1537,8 +1605,12 @@ static int add_integ(samos_ctx* ctx, Integ& I, const std::string& name) {
   }
   I.slot = slot;
   ctx->integs[key] = I;
+  ctx->last_key = key;
   return 0;
 }
+
+// The integrator that add_integ registered last (its map key holds the group).
+static Integ& last_integ(samos_ctx* ctx, const char*) { return ctx->integs.at(ctx->last_key); }
 
 int samos_integrator_brownian(samos_ctx* ctx, int which, double v0, double nu, double mu, double mur, int seed,
                               int nematic, double tau, int velocity_align, int tkind, double tmin, double tmax,
@@ -1550,7 +1622,7 @@ int samos_integrator_brownian(samos_ctx* ctx, int which, double v0, double nu, d
   I.temp = TempD{tkind, tmin, tmax, tsteps};
   const char* nm = which == 0 ? "brownian" : (which == 1 ? "brownian_pos" : "brownian_align");
   if (int r = add_integ(ctx, I, nm)) return r;
-  Integ& J = ctx->integs[std::string(nm) + "_all"];
+  Integ& J = last_integ(ctx, nm);
   J.flip_p = J.dt / tau;
   J.stoch = std::sqrt(J.nu * J.dt);
   return 0;
@@ -1563,7 +1635,7 @@ int samos_integrator_langevin(samos_ctx* ctx, double gamma, int seed, int method
   I.gamma = gamma; I.seed = (uint32_t)seed; I.method = method;
   I.temp = TempD{tkind, tmin, tmax, tsteps};
   if (int r = add_integ(ctx, I, "langevin")) return r;
-  Integ& J = ctx->integs["langevin_all"];
+  Integ& J = last_integ(ctx, "langevin");
   if (method == 2) {
     HIPCHK(hipMalloc(&J.R, sizeof(double) * 3 * (size_t)std::max(ctx->n, 1)));
     launch_bbk_init(ctx->st, fields(ctx), ctx->n, J.seed, J.R);
@@ -1631,7 +1703,8 @@ int samos_pair_extra_energy(samos_ctx* ctx, double* energy) {
 }
 
 int samos_fire_state(samos_ctx* ctx, double* out) {
-  auto it = ctx->integs.find("fire_all");
+  auto it = ctx->integs.begin();
+  while (it != ctx->integs.end() && it->second.name != "fire") ++it;
   if (it == ctx->integs.end()) return fail(ctx, "no FIRE integrator");
   Integ& I = it->second;
   out[0] = I.dt; out[1] = I.alpha; out[2] = I.last[0]; out[3] = I.last[1]; out[4] = I.last[2];
@@ -1714,10 +1787,9 @@ int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags
   // Fused schedule: one Langevin BAOAB integrator, pair potentials only.  The
   // reference's loop-head force evaluation repeats the evaluation that ended
   // the previous step on the same positions, so it is done once.
-  const bool fused = !(flags & SAMOS_RUN_FAITHFUL) && ctx->extra.count == 0 && ctx->integs.size() == 1 &&
-                     ctx->integs.count("langevin_all") &&
-                     ctx->integs["langevin_all"].method == 0 && al_mask(ctx) == 0 && pot_mask(ctx) != 0 &&
-                     needs_nlist(ctx);
+  Integ* only = ctx->integs.size() == 1 ? &ctx->integs.begin()->second : nullptr;
+  const bool fused = !(flags & SAMOS_RUN_FAITHFUL) && ctx->extra.count == 0 && only && only->name == "langevin" &&
+                     only->method == 0 && al_mask(ctx) == 0 && pot_mask(ctx) != 0 && needs_nlist(ctx);
   const bool fused_b = !fused && fused_brownian_eligible(ctx, flags);
   if (t_begin == 0 && !ctx->run_begun) {
     // Loop head of t = 0 (src/samos.cpp:887-892): enforce, reset, forces and
@@ -1735,7 +1807,7 @@ int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags
   }
   if (t_end == steps + 1) ctx->run_begun = false;
   if (fused) {
-    if (int r = run_fused_langevin(ctx, ctx->integs["langevin_all"], t_begin, t_end, steps, flags)) return r;
+    if (int r = run_fused_langevin(ctx, *only, t_begin, t_end, steps, flags)) return r;
     ctx->time_step += t_end - t_begin;
     return check_overflow(ctx);
   }
@@ -1794,6 +1866,54 @@ int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags
   }
   HIPCHK(hipGetLastError());
   return check_overflow(ctx);
+}
+
+int samos_integrator_disable(samos_ctx* ctx, const char* type, const char* group) {
+  const std::string key = std::string(type) + "_" + group;
+  auto it = ctx->integs.find(key);
+  if (it == ctx->integs.end())
+    return fail(ctx, std::string("Integrator ") + type + " (for group " + group + ") does not exist.");
+  hipFree(it->second.R);
+  ctx->integs.erase(it);
+  return 0;
+}
+
+int samos_group_define(samos_ctx* ctx, const char* name, const int* member) {
+  const std::string nm = name ? name : "";
+  if (nm.empty() || nm == "all") return fail(ctx, "group all always holds every particle");
+  if (nm == "boundary" || nm == "tissue")
+    return fail(ctx, "Groups \"tissue\" and \"boundary\" are reserved and cannot be used.");
+  auto it = ctx->group_index.find(nm);
+  int idx;
+  if (it == ctx->group_index.end()) {
+    idx = (int)ctx->group_index.size();
+    if (idx >= 32) return fail(ctx, "too many particle groups (at most 31 besides all)");
+    ctx->group_index[nm] = idx;
+  } else {
+    idx = it->second;
+  }
+  const int n = ctx->n;
+  for (int i = 0; i < n; ++i)
+    if (member[i]) ctx->h_groups[i] |= 1u << idx;
+  std::vector<int> tmp(n);
+  for (int s = 0; s < n; ++s) tmp[s] = (int)ctx->h_groups[ctx->h_id[s]];
+  if (n) HIPCHK(hipMemcpyAsync(ctx->icol[6], tmp.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int samos_group_size(samos_ctx* ctx, const char* name, int* size) {
+  auto it = ctx->group_index.find(name ? name : "");
+  if (it == ctx->group_index.end()) return fail(ctx, std::string("Group ") + (name ? name : "") + " does not exist.");
+  *size = group_size(ctx, 1u << it->second);
+  return 0;
+}
+
+int samos_set_integrator_group(samos_ctx* ctx, const char* group) {
+  const std::string g = group ? group : "all";
+  if (!ctx->group_index.count(g)) return fail(ctx, "Group " + g + " does not exist.");
+  ctx->next_group = g;
+  return 0;
 }
 
 int samos_integrator_remove(samos_ctx* ctx, const char* type) {

@@ -56,9 +56,9 @@ __global__ void k_enforce(Fields f, BoxD b, ConstraintD c, int n) {
 template <int WHICH>
 __global__ void k_brownian(Fields f, BoxD b, ConstraintD c, int n, double dt, double v0, double mu, double mur,
                            double stoch, double flip_p, int nematic, int velocity, TempD temp, uint32_t seed,
-                           unsigned long long call0, const ClockD* clk, int slot, int t_off) {
+                           unsigned long long call0, const ClockD* clk, int slot, int t_off, unsigned gbit) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  if (s >= n || !in_group(f, s, gbit)) return;
   const unsigned long long call = eff_call(clk, slot, call0);
   const uint32_t pid = (uint32_t)f.id[s];
   const uint32_t tag = WHICH == 0 ? TAG_BROWNIAN : (WHICH == 1 ? TAG_BROWNIAN_POS : TAG_BROWNIAN_ALIGN);
@@ -104,9 +104,9 @@ __global__ void k_brownian(Fields f, BoxD b, ConstraintD c, int n, double dt, do
 template <int METHOD>
 __global__ void k_langevin_a(Fields f, BoxD b, ConstraintD c, int n, double dt, double gamma, TempD temp,
                              uint32_t seed, unsigned long long call0, const ClockD* clk, int slot, int t_off,
-                             const double* R) {
+                             const double* R, unsigned gbit) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  if (s >= n || !in_group(f, s, gbit)) return;
   PS p; load_ps(f, s, p);
   const double dt2 = 0.5 * dt;
   const double m = f.mass[s];
@@ -154,9 +154,9 @@ __global__ void k_langevin_a(Fields f, BoxD b, ConstraintD c, int n, double dt, 
 template <int METHOD>
 __global__ void k_langevin_b(Fields f, int n, double dt, double gamma, TempD temp, uint32_t seed,
                              unsigned long long call0, const ClockD* clk, int slot, int t_off, double* R, BoxD b,
-                             ConstraintD c) {
+                             ConstraintD c, unsigned gbit) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  if (s >= n || !in_group(f, s, gbit)) return;
   const double dt2 = 0.5 * dt;
   const double m = f.mass[s];
   if (METHOD == 0) {
@@ -211,9 +211,9 @@ __global__ void k_bbk_init(Fields f, int n, uint32_t seed, double* R) {
 }
 
 // FIRE first pass: half kick, drift, enforce, director half kick and rotation.
-__global__ void k_fire_a(Fields f, BoxD b, ConstraintD c, int n, double dt) {
+__global__ void k_fire_a(Fields f, BoxD b, ConstraintD c, int n, double dt, unsigned gbit) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  if (s >= n || !in_group(f, s, gbit)) return;
   PS p; load_ps(f, s, p);
   const double dt_2 = 0.5 * dt;
   p.vx += dt_2 * f.fx[s]; p.vy += dt_2 * f.fy[s]; p.vz += dt_2 * f.fz[s];
@@ -233,9 +233,9 @@ __global__ void k_fire_a(Fields f, BoxD b, ConstraintD c, int n, double dt) {
 // (limited to `theta` when theta_on).  limit < 0 / theta < 0: not given.
 __device__ __forceinline__ double sign_of(double x) { return x >= 0.0 ? 1.0 : -1.0; }
 
-__global__ void k_nve_a(Fields f, BoxD b, ConstraintD c, int n, double dt, double limit, double theta) {
+__global__ void k_nve_a(Fields f, BoxD b, ConstraintD c, int n, double dt, double limit, double theta, unsigned gbit) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  if (s >= n || !in_group(f, s, gbit)) return;
   PS p; load_ps(f, s, p);
   const double dt_2 = 0.5 * dt;
   p.vx += dt_2 * f.fx[s]; p.vy += dt_2 * f.fy[s]; p.vz += dt_2 * f.fz[s];
@@ -263,9 +263,9 @@ __global__ void k_nve_a(Fields f, BoxD b, ConstraintD c, int n, double dt, doubl
 // IntegratorNVE::integrate, last loop (after the force and torque evaluation):
 // half kick, speed limit, enforce, omega half kick and its limit (applied when
 // both limits are given; the reference reads an unset member otherwise), age.
-__global__ void k_nve_b(Fields f, BoxD b, ConstraintD c, int n, double dt, double limit, double theta) {
+__global__ void k_nve_b(Fields f, BoxD b, ConstraintD c, int n, double dt, double limit, double theta, unsigned gbit) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  if (s >= n || !in_group(f, s, gbit)) return;
   PS p; load_ps(f, s, p);
   const double dt_2 = 0.5 * dt;
   p.vx += dt_2 * f.fx[s]; p.vy += dt_2 * f.fy[s]; p.vz += dt_2 * f.fz[s];
@@ -288,9 +288,13 @@ __global__ void k_nve_b(Fields f, BoxD b, ConstraintD c, int n, double dt, doubl
 
 // FIRE second pass: half kick, per-slot P = v.F, |F|^2, |v|^2 (before the
 // enforce, as in the reference), enforce, director half kick.
-__global__ void k_fire_b(Fields f, BoxD b, ConstraintD c, int n, double dt, double* cols, long ld) {
+__global__ void k_fire_b(Fields f, BoxD b, ConstraintD c, int n, double dt, double* cols, long ld, unsigned gbit) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
+  if (!in_group(f, s, gbit)) {
+    cols[s] = 0.0; cols[ld + s] = 0.0; cols[2 * ld + s] = 0.0;
+    return;
+  }
   PS p; load_ps(f, s, p);
   const double dt_2 = 0.5 * dt;
   const double fx = f.fx[s], fy = f.fy[s], fz = f.fz[s];
@@ -304,9 +308,9 @@ __global__ void k_fire_b(Fields f, BoxD b, ConstraintD c, int n, double dt, doub
   store_ps(f, s, p);
 }
 
-__global__ void k_fire_mix(Fields f, int n, double f1, double f2, int zero_v) {
+__global__ void k_fire_mix(Fields f, int n, double f1, double f2, int zero_v, unsigned gbit) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  if (s >= n || !in_group(f, s, gbit)) return;
   double vx = f1 * f.vx[s] + f2 * f.fx[s];
   double vy = f1 * f.vy[s] + f2 * f.fy[s];
   double vz = f1 * f.vz[s] + f2 * f.fz[s];
@@ -372,8 +376,11 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
     const double dt2 = 0.5 * dt;
     const double m = mass_u > 0.0 ? mass_u : cur.mass[s];
     const double fact = dt2 / m;
+    // outside the integrator's group a slot is not integrated; it only takes
+    // the next loop head's enforce
+    const bool member = in_group(cur, s, fa.gbit);
     double vx = cur.vx[s], vy = cur.vy[s], vz = PLANAR ? 0.0 : cur.vz[s];
-    if (PHASE >= 1) {  // closing B of step t-1
+    if (PHASE >= 1 && member) {  // closing B of step t-1
       vx += fact * fx; vy += fact * fy; vz += fact * fz;
       if (!(fa.age && fa.age->uniform)) cur.age[s] += dt;
     }
@@ -389,33 +396,36 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
       p.nx = nx0; p.ny = ny0; p.nz = nz0;
       p.dix = p.diy = p.diz = 0;
       constexpr int kSet = PLANAR ? kSurfPlane : kSurfAll;
-      if (!PLANAR && PHASE == 1) {
-        // The loop-head enforce of step t acts on the velocity after step t-1's
-        // closing kick.  The positions already carry it (applied when written);
-        // the velocity projection is repeated here on the kicked velocity.  On a
-        // plane the kick has no normal component, so this is skipped there.
-        PS q = p;
-        enforce<kSet>(b, c, q);
-        p.vx = q.vx; p.vy = q.vy; p.vz = q.vz;
-      }
-      p.vx += fact * fx; p.vy += fact * fy; p.vz += fact * fz;
-      p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
-      bool fz0 = enforce<kSet>(b, c, p);
-      p.vx *= exp_dt; p.vy *= exp_dt; p.vz *= exp_dt;
-      if (B != 0.0) {
-        const double sf = B / sqrt(m);
-        const unsigned long long call = eff_call(clk, slot, 0ULL) + (unsigned long long)t_off;
-        double g[3];
-        if (PLANAR) {  // the z draw only reaches v_z, which the next enforce resets
-          gauss2(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], call, g);
-          g[2] = 0.0;
-        } else {
-          gauss3(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], call, g);
+      bool fz0 = false;
+      if (member) {
+        if (!PLANAR && PHASE == 1) {
+          // The loop-head enforce of step t acts on the velocity after step t-1's
+          // closing kick.  The positions already carry it (applied when written);
+          // the velocity projection is repeated here on the kicked velocity.  On a
+          // plane the kick has no normal component, so this is skipped there.
+          PS q = p;
+          enforce<kSet>(b, c, q);
+          p.vx = q.vx; p.vy = q.vy; p.vz = q.vz;
         }
-        p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
+        p.vx += fact * fx; p.vy += fact * fy; p.vz += fact * fz;
+        p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+        fz0 = enforce<kSet>(b, c, p);
+        p.vx *= exp_dt; p.vy *= exp_dt; p.vz *= exp_dt;
+        if (B != 0.0) {
+          const double sf = B / sqrt(m);
+          const unsigned long long call = eff_call(clk, slot, 0ULL) + (unsigned long long)t_off;
+          double g[3];
+          if (PLANAR) {  // the z draw only reaches v_z, which the next enforce resets
+            gauss2(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], call, g);
+            g[2] = 0.0;
+          } else {
+            gauss3(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], call, g);
+          }
+          p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
+        }
+        p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+        fz0 |= enforce<kSet>(b, c, p);
       }
-      p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
-      fz0 |= enforce<kSet>(b, c, p);
       if (head_enforce) fz0 |= enforce<kSet>(b, c, p);
       if (PLANAR) {
         nxt.xy[s] = make_double2(p.x, p.y);
@@ -567,65 +577,69 @@ void launch_enforce(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n) 
 
 void launch_brownian(hipStream_t st, int which, Fields f, BoxD box, ConstraintD con, int n, double dt, double v0,
                      double mu, double mur, double stoch, double flip_p, int nematic, int velocity, TempD temp,
-                     uint32_t seed, unsigned long long call, const ClockD* clock, int clock_slot, int t_off) {
+                     uint32_t seed, unsigned long long call, const ClockD* clock, int clock_slot, int t_off,
+                     unsigned gbit) {
   dim3 g(grid1(n)), blk(kBlock);
   if (which == 0)
     hipLaunchKernelGGL(k_brownian<0>, g, blk, 0, st, f, box, con, n, dt, v0, mu, mur, stoch, flip_p, nematic, velocity,
-                       temp, seed, call, clock, clock_slot, t_off);
+                       temp, seed, call, clock, clock_slot, t_off, gbit);
   else if (which == 1)
     hipLaunchKernelGGL(k_brownian<1>, g, blk, 0, st, f, box, con, n, dt, v0, mu, mur, stoch, flip_p, nematic, velocity,
-                       temp, seed, call, clock, clock_slot, t_off);
+                       temp, seed, call, clock, clock_slot, t_off, gbit);
   else
     hipLaunchKernelGGL(k_brownian<2>, g, blk, 0, st, f, box, con, n, dt, v0, mu, mur, stoch, flip_p, nematic, velocity,
-                       temp, seed, call, clock, clock_slot, t_off);
+                       temp, seed, call, clock, clock_slot, t_off, gbit);
 }
 
 void launch_langevin_a(hipStream_t st, int method, Fields f, BoxD box, ConstraintD con, int n, double dt, double gamma,
                        TempD temp, uint32_t seed, unsigned long long call, const ClockD* clock, int clock_slot,
-                       int t_off, double* R) {
+                       int t_off, double* R, unsigned gbit) {
   dim3 g(grid1(n)), blk(kBlock);
   if (method == 0)
-    hipLaunchKernelGGL(k_langevin_a<0>, g, blk, 0, st, f, box, con, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R);
+    hipLaunchKernelGGL(k_langevin_a<0>, g, blk, 0, st, f, box, con, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R, gbit);
   else if (method == 1)
-    hipLaunchKernelGGL(k_langevin_a<1>, g, blk, 0, st, f, box, con, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R);
+    hipLaunchKernelGGL(k_langevin_a<1>, g, blk, 0, st, f, box, con, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R, gbit);
   else
-    hipLaunchKernelGGL(k_langevin_a<2>, g, blk, 0, st, f, box, con, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R);
+    hipLaunchKernelGGL(k_langevin_a<2>, g, blk, 0, st, f, box, con, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R, gbit);
 }
 
 void launch_langevin_b(hipStream_t st, int method, Fields f, int n, double dt, double gamma, TempD temp, uint32_t seed,
                        unsigned long long call, const ClockD* clock, int clock_slot, int t_off, double* R, BoxD box,
-                       ConstraintD con) {
+                       ConstraintD con, unsigned gbit) {
   dim3 g(grid1(n)), blk(kBlock);
   if (method == 0)
-    hipLaunchKernelGGL(k_langevin_b<0>, g, blk, 0, st, f, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R, box, con);
+    hipLaunchKernelGGL(k_langevin_b<0>, g, blk, 0, st, f, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R, box, con, gbit);
   else if (method == 1)
-    hipLaunchKernelGGL(k_langevin_b<1>, g, blk, 0, st, f, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R, box, con);
+    hipLaunchKernelGGL(k_langevin_b<1>, g, blk, 0, st, f, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R, box, con, gbit);
   else
-    hipLaunchKernelGGL(k_langevin_b<2>, g, blk, 0, st, f, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R, box, con);
+    hipLaunchKernelGGL(k_langevin_b<2>, g, blk, 0, st, f, n, dt, gamma, temp, seed, call, clock, clock_slot, t_off, R, box, con, gbit);
 }
 
 void launch_bbk_init(hipStream_t st, Fields f, int n, uint32_t seed, double* R) {
   hipLaunchKernelGGL(k_bbk_init, dim3(grid1(n)), dim3(kBlock), 0, st, f, n, seed, R);
 }
 
-void launch_fire_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt) {
-  hipLaunchKernelGGL(k_fire_a, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt);
+void launch_fire_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, unsigned gbit) {
+  hipLaunchKernelGGL(k_fire_a, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, gbit);
 }
 
-void launch_fire_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double* cols, long ld) {
-  hipLaunchKernelGGL(k_fire_b, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, cols, ld);
+void launch_fire_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double* cols, long ld,
+                   unsigned gbit) {
+  hipLaunchKernelGGL(k_fire_b, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, cols, ld, gbit);
 }
 
-void launch_nve_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta) {
-  hipLaunchKernelGGL(k_nve_a, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, limit, theta);
+void launch_nve_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta,
+                  unsigned gbit) {
+  hipLaunchKernelGGL(k_nve_a, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, limit, theta, gbit);
 }
 
-void launch_nve_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta) {
-  hipLaunchKernelGGL(k_nve_b, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, limit, theta);
+void launch_nve_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta,
+                  unsigned gbit) {
+  hipLaunchKernelGGL(k_nve_b, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, limit, theta, gbit);
 }
 
-void launch_fire_mix(hipStream_t st, Fields f, int n, double f1, double f2, int zero_v) {
-  hipLaunchKernelGGL(k_fire_mix, dim3(grid1(n)), dim3(kBlock), 0, st, f, n, f1, f2, zero_v);
+void launch_fire_mix(hipStream_t st, Fields f, int n, double f1, double f2, int zero_v, unsigned gbit) {
+  hipLaunchKernelGGL(k_fire_mix, dim3(grid1(n)), dim3(kBlock), 0, st, f, n, f1, f2, zero_v, gbit);
 }
 
 void launch_chunk_settle(hipStream_t st, ChunkCtl* ck, Fields f0, Fields f1, int n, int planar, int directors) {

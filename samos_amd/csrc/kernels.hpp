@@ -56,23 +56,30 @@ void launch_reduce_sum(hipStream_t st, const double* cols, int ncols, long ld, i
 
 // ---- integrators and constraints (integrate.hip) --------------------------
 void launch_enforce(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n);
+// The integrator kernels act on the slots of the group with bit mask gbit only
+// (Group::get_particles of the integrator's group; 1u: every slot).
 void launch_brownian(hipStream_t st, int which, Fields f, BoxD box, ConstraintD con, int n, double dt,
                      double v0, double mu, double mur, double stoch, double flip_p, int nematic, int velocity,
-                     TempD temp, uint32_t seed, unsigned long long call, const ClockD* clock, int clock_slot, int t_off);
+                     TempD temp, uint32_t seed, unsigned long long call, const ClockD* clock, int clock_slot, int t_off,
+                     unsigned gbit = 1u);
 void launch_langevin_a(hipStream_t st, int method, Fields f, BoxD box, ConstraintD con, int n, double dt,
                        double gamma, TempD temp, uint32_t seed, unsigned long long call, const ClockD* clock,
-                       int clock_slot, int t_off, double* R);
+                       int clock_slot, int t_off, double* R, unsigned gbit = 1u);
 void launch_langevin_b(hipStream_t st, int method, Fields f, int n, double dt, double gamma, TempD temp,
                        uint32_t seed, unsigned long long call, const ClockD* clock, int clock_slot, int t_off,
-                       double* R, BoxD box, ConstraintD con);
+                       double* R, BoxD box, ConstraintD con, unsigned gbit = 1u);
 void launch_bbk_init(hipStream_t st, Fields f, int n, uint32_t seed, double* R);
-void launch_fire_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt);
-void launch_fire_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double* cols, long ld);
-void launch_fire_mix(hipStream_t st, Fields f, int n, double f1, double f2, int zero_v);
+void launch_fire_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, unsigned gbit = 1u);
+// Slots outside the group write zeros to the reduction columns.
+void launch_fire_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double* cols, long ld,
+                   unsigned gbit = 1u);
+void launch_fire_mix(hipStream_t st, Fields f, int n, double f1, double f2, int zero_v, unsigned gbit = 1u);
 // IntegratorNVE (src/integrators/integrator_nve.cpp:37-141): the part before the
 // force evaluation and the part after it; limit / theta < 0: not given.
-void launch_nve_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta);
-void launch_nve_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta);
+void launch_nve_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta,
+                  unsigned gbit = 1u);
+void launch_nve_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta,
+                  unsigned gbit = 1u);
 void launch_clock_advance(hipStream_t st, ClockD* clock, int nsteps);
 
 // Fused Langevin BAOAB step over the neighbour list (one launch per step):
@@ -97,6 +104,7 @@ struct FusedArgs {
   const AgeCtl* age = nullptr;  // common-age state (null or not uniform: each slot adds dt)
   ChunkCtl* chunk = nullptr;  // captured-chunk mode (null: direct launch with the parity flags)
   int slot = 0;               // slot index inside the chunk
+  unsigned gbit = 1u;         // the integrator's group: other slots only take the loop-head enforce
 };
 void launch_baoab_fused(hipStream_t st, int pot_mask, int phase, Fields cur, Fields nxt, NListD nl, BoxD box,
                         ConstraintD con, InteractD ip, int n, const FusedArgs& fa, int fp32, int planar,
@@ -129,6 +137,19 @@ struct BrownArgs {
   int slot = 0;
   int lanes = 1, lane_shift = 0;  // threads per slot (power of two up to 64) and its log2
   const AgeCtl* age = nullptr;    // common-age state (null or not uniform: each slot adds dt)
+  // Several "brownian" integrators on disjoint groups (the type_group keys of
+  // src/samos.cpp:654-690): with ngroups > 0 (multi-type kernels only) each
+  // slot takes the parameters of the group that holds it, and a slot in none
+  // of them is not integrated; the top-level fields are then unused.
+  int ngroups = 0;
+  struct Group {
+    double dt, v0, mu, mur, stoch, flip_p;
+    int nematic, velocity;
+    TempD temp;
+    uint32_t seed;
+    int clock_slot;
+    unsigned gbit;
+  } grp[4];
 };
 bool brownian_fused_supported(int pot_mask, int al_mask);
 void launch_brownian_fused(hipStream_t st, int pot_mask, int al_mask, Fields cur, Fields nxt, NListD nl, BoxD box,

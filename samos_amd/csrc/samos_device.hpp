@@ -74,10 +74,18 @@ struct Fields {
   double *x, *y, *z, *vx, *vy, *vz, *nx, *ny, *nz;
   double *fx, *fy, *fz, *tx, *ty, *tz, *omega, *age, *radius, *mass;
   int *type, *ix, *iy, *iz, *coord, *id;
+  int* groups;        // bit g set: the slot belongs to particle group g (bit 0: "all")
   double2* xy;        // interleaved (x, y) for neighbour gathers on planar systems; may be null
 };
 constexpr int kNumDoubleFields = 19;
-constexpr int kNumIntFields = 6;
+constexpr int kNumIntFields = 7;
+
+// Membership of slot s in the group whose bit mask is gbit (System::make_group,
+// src/system/system.cpp:462-570): every particle is in "all" (bit 0), and for
+// that group the column is not read.
+__device__ __forceinline__ bool in_group(const Fields& f, int s, unsigned gbit) {
+  return gbit == 1u || ((unsigned)f.groups[s] & gbit) != 0u;
+}
 
 // Neighbour-list state.  The list is a full (both directions) Verlet list in a
 // transposed layout: neighbour k of slot s is nbr[k * np + s], so that the
