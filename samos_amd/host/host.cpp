@@ -223,6 +223,50 @@ System::System(const std::string& input, MessengerPtr msg, const Box& box, bool 
   check(samos_set_field(ctx_, SAMOS_IY, iy.data()));
   check(samos_set_field(ctx_, SAMOS_IZ, iz.data()));
   msg_->msg(Messenger::INFO, "Read data for " + std::to_string(n_) + " particles.");
+  std::vector<int>& all = groups_["all"];
+  for (int i = 0; i < n_; ++i) all.push_back(i);
+}
+
+void System::make_group(const std::string& name, pairs_type& p) {
+  if (name == "boundary" || name == "tissue")
+    fail(msg_, "Groups \"tissue\" and \"boundary\" are reserved and cannot be used.",
+         "Trying to redefine reserved group name.");
+  std::vector<char> add(n_, 1);
+  auto crit = [&](const char* key, int field, bool is_int, int op) {
+    // op: 0 equal, 1 not equal, 2 <=, 3 >=
+    if (!p.count(key)) return;
+    msg_->msg(Messenger::INFO, "Adding particle to group " + name + ".");
+    if (is_int) {
+      const int v = to_int(p[key], key);
+      const std::vector<int> c = icolumn(field);
+      for (int i = 0; i < n_; ++i) add[i] &= (op == 0 ? c[i] == v : c[i] != v) ? 1 : 0;
+    } else {
+      const double v = to_double(p[key], key);
+      const std::vector<double> c = column(field);
+      for (int i = 0; i < n_; ++i) add[i] &= (op == 0 ? c[i] == v : op == 2 ? c[i] <= v : c[i] >= v) ? 1 : 0;
+    }
+  };
+  crit("type", SAMOS_TYPE, true, 0);
+  crit("not_type", SAMOS_TYPE, true, 1);
+  crit("r_eq", SAMOS_RADIUS, false, 0);
+  crit("r_le", SAMOS_RADIUS, false, 2);
+  crit("r_ge", SAMOS_RADIUS, false, 3);
+  crit("x_le", SAMOS_X, false, 2);
+  crit("x_ge", SAMOS_X, false, 3);
+  crit("y_le", SAMOS_Y, false, 2);
+  crit("y_ge", SAMOS_Y, false, 3);
+  crit("z_le", SAMOS_Z, false, 2);
+  crit("z_ge", SAMOS_Z, false, 3);
+  std::vector<int> member(n_);
+  std::vector<int>& g = groups_[name];
+  std::vector<char> in(n_, 0);
+  for (int i : g) in[i] = 1;
+  for (int i = 0; i < n_; ++i) {
+    member[i] = add[i];
+    if (add[i] && !in[i]) g.push_back(i);
+  }
+  check(samos_group_define(ctx_, name.c_str(), member.data()));
+  msg_->msg(Messenger::INFO, "Added " + std::to_string(g.size()) + " particles to group " + name + ".");
 }
 
 System::~System() {
@@ -537,8 +581,13 @@ Integrator::Integrator(SystemPtr sys, MessengerPtr msg, ValuePtr temp, pairs_typ
   if (dt_ <= 0.0)
     fail(msg_, "Integrator step size has to be larger than 0. Did you forget to use \"timestep\" command?",
          "Integrator step has not been set.");
-  if (has(p, "group") && p["group"] != "all")
-    fail(msg_, "Integrators on groups other than \"all\" are not supported by this build.", "Unsupported group.");
+  if (has(p, "group") && p["group"] != "all") {
+    const std::string g = p["group"];
+    if (!sys_->has_group(g))
+      fail(msg_, "Group " + g + " does not exist.", "Integrator group does not exist.");
+    msg_->msg(Messenger::INFO, "Applying integrator to group " + g + ".");
+    sys_->check(samos_set_integrator_group(sys_->ctx(), g.c_str()));
+  }
 }
 
 std::string Integrator::params_ok(pairs_type& params) const {
@@ -737,38 +786,155 @@ void Logger::log(int step) {
 }
 
 // ---------------------------------------------------------------------------
-// Dump (src/dump/dump.cpp:60-413, 490-690): xyz, full and input formats.
+// Dump (src/dump/dump.cpp:60-413, 419-800): xyz, full, input, velocity,
+// director, xyzv and dcd formats; vtp is accepted and writes nothing, as in a
+// reference build without VTK (dump.cpp:403-406).
 // ---------------------------------------------------------------------------
 Dump::Dump(SystemPtr sys, MessengerPtr msg, const std::string& name, pairs_type& params)
     : sys_(sys), msg_(msg), name_(name), params_(params) {
-  type_ = has(params, "type") ? params["type"] : "xyz";
-  static const std::map<std::string, std::string> ext = {{"xyz", "xyz"}, {"full", "dat"}, {"input", "inp"}};
+  if (!has(params, "type")) fail(msg_, "No dump type specified.", "No dump type specified.");
+  type_ = params["type"];
+  static const std::map<std::string, std::string> ext = {{"xyz", "xyz"},      {"full", "dat"}, {"input", "inp"},
+                                                         {"velocity", "vel"}, {"dcd", "dcd"},  {"director", "dir"},
+                                                         {"xyzv", "xyzv"},    {"vtp", "vtp"}};
   auto it = ext.find(type_);
   if (it == ext.end())
-    fail(msg_, "Dump type " + type_ + " is not supported by this build (xyz, full, input).", "Unsupported dump type.");
+    fail(msg_, "Dump type " + type_ + " is not supported by this build (xyz, full, input, velocity, director, xyzv, "
+                                      "dcd, vtp).", "Unsupported dump type");
   ext_ = it->second;
   if (has(params, "directory")) dir_ = params["directory"];
   if (has(params, "start")) start_ = to_int(params["start"], "start");
   freq_ = has(params, "freq") ? to_int(params["freq"], "freq") : 100;
+  if (has(params, "scale")) scale_ = to_double(params["scale"], "scale");
   multi_ = has(params, "multi");
   header_ = has(params, "header");
   keys_ = has(params, "keys");
+  if (has(params, "group")) group_ = params["group"];
+  if (!sys_->has_group(group_))
+    fail(msg_, "Dump: Group " + group_ + " is not defined.", "Attempted to dump group that does not exist.");
   if (type_ == "input") {
     for (const char* k : {"id", "tp", "radius", "coordinate", "velocity", "director", "omega"}) params_[k] = "";
     keys_ = true;
   }
+  if (type_ == "dcd") {
+    if (group_ != "all") fail(msg_, "DCD files does not group specific dump.", "Attempted to dump groups into a DCD output.");
+    if (multi_)
+      fail(msg_, "DCD files cannot be split. mulifile flag is not permitted.",
+           "Attempted to spread DCD output into multiple files.");
+    single_.open(dir_ + "/" + name_ + "." + ext_, std::ios::binary);
+    write_dcd_header();
+    return;
+  }
+  if (type_ == "vtp") {
+    msg_->msg(Messenger::WARNING, "vtp dumps need VTK, which this build does not use: no file is written.");
+    return;
+  }
   if (!multi_) single_.open(dir_ + "/" + name_ + "." + ext_);
 }
 
-void Dump::write(std::ostream& o) {
+namespace {
+void put_int(std::ostream& o, int v) { o.write(reinterpret_cast<const char*>(&v), 4); }
+}  // namespace
+
+// The CHARMM-style header of the reference (dump.cpp:296-341): 84-byte block
+// with "CORD", start and frequency, unit-cell flag and version 24; two
+// 80-byte title lines; the particle count.
+void Dump::write_dcd_header() {
+  std::ostream& f = single_;
+  put_int(f, 84);
+  f.write("CORD", 4);
+  put_int(f, 0);
+  put_int(f, start_);
+  put_int(f, freq_);
+  for (int k = 0; k < 6; ++k) put_int(f, 0);
+  put_int(f, 0);
+  put_int(f, 1);
+  for (int k = 0; k < 8; ++k) put_int(f, 0);
+  put_int(f, 24);
+  put_int(f, 84);
+  put_int(f, 164);
+  put_int(f, 2);
+  char title[80] = {0};
+  std::snprintf(title, sizeof title, "Created by SAMoS");
+  f.write(title, 80);
+  char when[81] = {0};
+  const std::time_t now = std::time(nullptr);
+  std::strftime(when, 80, "REMARKS Created  %d %B, %Y at %H:%M", std::localtime(&now));
+  f.write(when, 80);
+  put_int(f, 164);
+  put_int(f, 4);
+  put_int(f, sys_->size());
+  put_int(f, 4);
+  if (!f.good()) fail(msg_, "Error writing DCD header", "Error writing DCD file");
+}
+
+// One frame (dump.cpp:419-488): the unit-cell record (six doubles, 100 on the
+// diagonal), then x, y and z as FP32 records, unwrapped with `unwrap`.
+void Dump::write_dcd_frame() {
+  std::ostream& f = single_;
   const int n = sys_->size();
+  const double cell[6] = {100.0, 0.0, 100.0, 0.0, 0.0, 100.0};
+  put_int(f, 48);
+  f.write(reinterpret_cast<const char*>(cell), 48);
+  put_int(f, 48);
+  const bool unwrap = params_.count("unwrap") != 0;
+  const Box& b = sys_->box();
+  const int fc[3] = {SAMOS_X, SAMOS_Y, SAMOS_Z}, ic[3] = {SAMOS_IX, SAMOS_IY, SAMOS_IZ};
+  const double L[3] = {b.Lx, b.Ly, b.Lz};
+  std::vector<float> buf(n);
+  for (int k = 0; k < 3; ++k) {
+    const std::vector<double> c = sys_->column(fc[k]);
+    std::vector<int> im;
+    if (unwrap) im = sys_->icolumn(ic[k]);
+    for (int i = 0; i < n; ++i) buf[i] = static_cast<float>(unwrap ? c[i] + im[i] * L[k] : c[i]);
+    put_int(f, n * 4);
+    f.write(reinterpret_cast<const char*>(buf.data()), 4L * n);
+    put_int(f, n * 4);
+  }
+  if (!f.good()) fail(msg_, "Error writing into DCD file.", "Error writing DCD file");
+}
+
+void Dump::write(std::ostream& o) {
+  const std::vector<int>& ids = sys_->group(group_);
+  const int n = (int)ids.size();
   char buf[256];
   if (type_ == "xyz") {
     auto x = sys_->column(SAMOS_X), y = sys_->column(SAMOS_Y), z = sys_->column(SAMOS_Z);
     auto t = sys_->icolumn(SAMOS_TYPE);
     o << n << "\nGenerated by SAMoS code.\n";
-    for (int i = 0; i < n; ++i) {
+    for (int i : ids) {
       std::snprintf(buf, sizeof buf, "%3d\t%10.6f\t%10.6f\t%10.6f", t[i], x[i], y[i], z[i]);
+      o << buf << "\n";
+    }
+    return;
+  }
+  if (type_ == "velocity" || type_ == "director" || type_ == "xyzv") {
+    // dump.cpp:710-782: vector columns scaled by `scale`
+    const bool vel = type_ == "velocity" || (type_ == "xyzv" && params_.count("velocity"));
+    auto x = sys_->column(SAMOS_X), y = sys_->column(SAMOS_Y), z = sys_->column(SAMOS_Z);
+    auto ax = sys_->column(vel ? SAMOS_VX : SAMOS_NX), ay = sys_->column(vel ? SAMOS_VY : SAMOS_NY),
+         az = sys_->column(vel ? SAMOS_VZ : SAMOS_NZ);
+    const double a = scale_;
+    if (type_ == "xyzv") {
+      auto t = sys_->icolumn(SAMOS_TYPE);
+      o << n << "\n";
+      o << "Generated by SAMoS code. Vectors are " << (vel ? "velocities" : "directors") << " scaled by " << a << "\n";
+      for (int i : ids) {
+        std::snprintf(buf, sizeof buf, "%c\t%12.7e\t%12.7e\t%12.7e\t%12.7e\t%12.7e\t%12.7e", (char)(64 + t[i]), x[i],
+                      y[i], z[i], a * ax[i], a * ay[i], a * az[i]);
+        o << buf << "\n";
+      }
+      return;
+    }
+    if (header_) {
+      const char c = vel ? 'v' : 'n';
+      o << "# scale a = " << a << "\n";
+      o << "# x-0.5*a*" << c << "x  y-0.5*a*" << c << "y z-0.5*a*" << c << "z  a*" << c << "x  a*" << c << "y  a*" << c
+        << "z\n";
+    }
+    for (int i : ids) {
+      std::snprintf(buf, sizeof buf, "%12.7e\t%12.7e\t%12.7e\t%12.7e\t%12.7e\t%12.7e", x[i] - 0.5 * a * ax[i],
+                    y[i] - 0.5 * a * ay[i], z[i] - 0.5 * a * az[i], a * ax[i], a * ay[i], a * az[i]);
       o << buf << "\n";
     }
     return;
@@ -799,7 +965,7 @@ void Dump::write(std::ostream& o) {
   if (on("omega")) om = sys_->column(SAMOS_OMEGA);
   if (on("image_flags") && ix.empty()) { ix = sys_->icolumn(SAMOS_IX); iy = sys_->icolumn(SAMOS_IY); iz = sys_->icolumn(SAMOS_IZ); }
   const Box& b = sys_->box();
-  for (int i = 0; i < n; ++i) {
+  for (int i : ids) {
     if (on("id")) { std::snprintf(buf, sizeof buf, "%5d ", i); o << buf; }
     if (on("tp")) { std::snprintf(buf, sizeof buf, "%2d ", tp[i]); o << buf; }
     if (on("radius")) { std::snprintf(buf, sizeof buf, "%8.5f ", r[i]); o << buf; }
@@ -820,7 +986,12 @@ void Dump::write(std::ostream& o) {
 }
 
 void Dump::dump(int step) {
-  if (!due(step)) return;
+  if (!due(step) || type_ == "vtp") return;
+  if (type_ == "dcd") {
+    write_dcd_frame();
+    single_.flush();
+    return;
+  }
   if (multi_) {
     char num[32];
     std::snprintf(num, sizeof num, "%010d", step);

@@ -81,8 +81,16 @@ class System {
   int time_step = 0;  // cumulative across runs (System::get_step)
   std::vector<double> column(int field) const;
   std::vector<int> icolumn(int field) const;
+  // System::make_group (src/system/system.cpp:462-570): criteria type,
+  // not_type, r_eq, r_le, r_ge, x_le, x_ge, y_le, y_ge, z_le, z_ge on the
+  // current state, all of the given ones; the members go to the device.
+  void make_group(const std::string& name, pairs_type& params);
+  bool has_group(const std::string& name) const { return groups_.count(name) != 0; }
+  // Group::get_particles: member ids in the order they were added.
+  const std::vector<int>& group(const std::string& name) const { return groups_.at(name); }
 
  private:
+  std::map<std::string, std::vector<int>> groups_;
   samos_ctx* ctx_ = nullptr;
   MessengerPtr msg_;
   Box box_;
@@ -352,11 +360,14 @@ class Dump {
 
  private:
   void write(std::ostream& out);
+  void write_dcd_header();
+  void write_dcd_frame();
   SystemPtr sys_;
   MessengerPtr msg_;
-  std::string name_, type_, ext_, dir_ = ".";
+  std::string name_, type_, ext_, dir_ = ".", group_ = "all";
   pairs_type params_;
   int start_ = 0, freq_ = 1;
+  double scale_ = 1.0;
   bool multi_ = false, header_ = false, keys_ = false;
   std::ofstream single_;
 };

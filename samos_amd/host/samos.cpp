@@ -164,7 +164,7 @@ struct Driver {
       const std::string group = p.count("group") ? p["group"] : "all";
       if (!integs.count(attrib + "_" + group))
         fail(msg, "Integrator of type " + attrib + " (for group " + group + ") does not exist.", "Integrator does not exist.");
-      sys->check(samos_integrator_remove(sys->ctx(), attrib.c_str()));
+      sys->check(samos_integrator_disable(sys->ctx(), attrib.c_str(), group.c_str()));
       integs.erase(attrib + "_" + group);
     } else if (cmd == "log") {
       need_system("adding any logs");
@@ -175,9 +175,12 @@ struct Driver {
       parse();
       dumps.push_back(std::make_shared<Dump>(sys, msg, attrib, p));
     } else if (cmd == "group") {
+      if (!sys)
+        fail(msg, "System has not been defined. Please define system using \"input\" command before adding any "
+                  "particle groups.", "System not defined.");
       parse();
-      if (attrib != "all")
-        fail(msg, "Particle groups other than \"all\" are not supported by this build.", "Unsupported group.");
+      sys->make_group(attrib, p);
+      msg->msg(Messenger::INFO, "Adding group " + attrib + ".");
     } else if (cmd == "ntypes") {
       msg->msg(Messenger::INFO, "ntypes: per-type tables hold 16 types.");
     } else if (cmd == "run") {
