@@ -152,6 +152,16 @@ int samos_constraint(samos_ctx* ctx, int kind, double param, int flag);
  * (the reference's defaults: maxiter 100, tol 1e-6). Fails when the tube
  * radius a is not smaller than the ring radius c. */
 int samos_constraint_surface(samos_ctx* ctx, int kind, const double* params, int nparams, int max_iter, double tol);
+/* Sphere radius schedule {rescale; rescale_steps; rescale_freq} (Constraint
+ * constructor, src/constraints/constraint.hpp:85-108): at every loop head whose
+ * time step is a multiple of rescale_freq and below rescale_steps the radius is
+ * multiplied by rescale^(rescale_freq / rescale_steps), every particle is
+ * enforced and the neighbour list rebuilt (ConstraintSphere::rescale,
+ * src/constraints/constraint_sphere.cpp:87-104; src/samos.cpp:882-886).
+ * Sphere only. */
+int samos_constraint_rescale(samos_ctx* ctx, double rescale, int rescale_steps, int rescale_freq);
+/* The sphere's current radius. */
+int samos_constraint_radius(samos_ctx* ctx, double* r);
 /* Constrainer::enforce on every particle (src/samos.cpp:887-888). */
 int samos_enforce_all(samos_ctx* ctx);
 

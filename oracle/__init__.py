@@ -76,6 +76,8 @@ def load(fast=False):
         "orc_add_langevin": (I, [P, D, I, I, I, D, D, I]),
         "orc_add_fire": (I, [P, D, D, D, D, D, D, D, I]),
         "orc_group_define": (I, [P, ctypes.c_char_p, P]),
+        "orc_set_constraint_rescale": (I, [P, D, I, I]),
+        "orc_constraint_radius": (D, [P]),
         "orc_set_integrator_group": (I, [P, ctypes.c_char_p]),
         "orc_integrator_disable": (I, [P, ctypes.c_char_p, ctypes.c_char_p]),
         "orc_fire_state": (I, [P, dp]),
@@ -219,6 +221,13 @@ class OracleSystem:
         q[:len(params)] = params
         self._chk(self.L.orc_set_constraint_surface(self.h, CONSTRAINT[kind], q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                     int(max_iter), float(tol)))
+
+    def constraint_rescale(self, rescale, steps=1000, freq=10):
+        """Sphere radius schedule {rescale; rescale_steps; rescale_freq}."""
+        self._chk(self.L.orc_set_constraint_rescale(self.h, float(rescale), int(steps), int(freq)))
+
+    def constraint_radius(self):
+        return float(self.L.orc_constraint_radius(self.h))
 
     def enforce_all(self):
         self._chk(self.L.orc_enforce_all(self.h))

@@ -206,6 +206,9 @@ struct Constraint {
   double tc = 1.0, ta = 0.5;                 // torus: ring radius c, tube radius a
   double ea = 1.0, eb = 1.0, ec = 1.0;       // ellipsoid semi-axes
   int max_iter = 100; double tol = 1e-6;     // SHAKE iteration (constraint.hpp:62-84)
+  // radius schedule of the sphere (constraint.hpp:85-108): total factor,
+  // steps it is spread over, steps between rescales, factor per rescale
+  double rescale = 1.0; int rescale_steps = 1000, rescale_freq = 10; double scale = 1.0;
 };
 
 struct Integ {
@@ -1242,12 +1245,28 @@ void integrate(Sim& s, Integ& I) {
 
 bool needs_nlist(const Sim& s) { return s.has_nlist && (!s.pots.empty() || !s.als.empty()); }
 
+// ConstraintSphere::rescale (src/constraints/constraint_sphere.cpp:87-104): at
+// time steps that are multiples of rescale_freq and below rescale_steps the
+// radius is multiplied by the per-rescale factor and every particle enforced.
+bool sphere_rescale(Sim& s) {
+  Constraint& c = s.con;
+  if (c.kind != 2 || c.rescale == 1.0) return false;
+  const int step = s.time_step;
+  if (step % c.rescale_freq == 0 && step < c.rescale_steps) {
+    c.r *= c.scale;
+    for (auto& p : s.p) enforce(s, p);
+    return true;
+  }
+  return false;
+}
+
 // Run loop (src/samos.cpp:872-970): N+1 iterations.
 void run(Sim& s, int steps) {
   pot_compute(s);
   align_compute(s);
   for (int t = 0; t <= steps; t++) {
     s.run_step = t;
+    if (sphere_rescale(s) && s.has_nlist) nlist_build(s);
     for (auto& p : s.p) enforce(s, p);
     reset_forces(s);
     reset_torques(s);
@@ -1513,6 +1532,18 @@ int orc_set_constraint(void* h, int kind, double param, int flag) {
   for (auto& p : s->p) enforce(*s, p);
   return 0;
 }
+
+// Sphere radius schedule {rescale; rescale_steps; rescale_freq}
+// (constraint.hpp:85-108): the factor per rescale is rescale^(freq / steps).
+int orc_set_constraint_rescale(void* h, double rescale, int steps, int freq) {
+  Sim* s = static_cast<Sim*>(h);
+  if (s->con.kind != 2) { g_err = "only the sphere constraint rescales"; return -1; }
+  s->con.rescale = rescale; s->con.rescale_steps = steps; s->con.rescale_freq = freq;
+  s->con.scale = std::pow(rescale, static_cast<double>(freq) / static_cast<double>(steps));
+  return 0;
+}
+
+double orc_constraint_radius(void* h) { return static_cast<Sim*>(h)->con.r; }
 
 // Cylinder {r}, torus {c, a}, ellipsoid {a, b, c}; enforced at once.
 int orc_set_constraint_surface(void* h, int kind, const double* q, int max_iter, double tol) {

@@ -74,6 +74,8 @@ def load_library(path: str = LIB_PATH):
         "samos_constraint": (I, [P, I, D, I]),
         "samos_constraint_surface": (I, [P, I, dp, I, I, D]),
         "samos_enforce_all": (I, [P]),
+        "samos_constraint_rescale": (I, [P, D, I, I]),
+        "samos_constraint_radius": (I, [P, dp]),
         "samos_integrator_brownian": (I, [P, I, D, D, D, D, I, I, D, I, I, D, D, I]),
         "samos_integrator_langevin": (I, [P, D, I, I, I, D, D, I]),
         "samos_integrator_fire": (I, [P, D, D, D, D, D, D, D, I]),
@@ -281,6 +283,15 @@ class System:
         q = np.ascontiguousarray(params, dtype=np.float64)
         self._chk(self.L.samos_constraint_surface(self.h, CONSTRAINT[kind], q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                   len(q), int(max_iter), float(tol)))
+
+    def constraint_rescale(self, rescale, steps=1000, freq=10):
+        """Sphere radius schedule {rescale; rescale_steps; rescale_freq}."""
+        self._chk(self.L.samos_constraint_rescale(self.h, float(rescale), int(steps), int(freq)))
+
+    def constraint_radius(self):
+        r = ctypes.c_double()
+        self._chk(self.L.samos_constraint_radius(self.h, ctypes.byref(r)))
+        return r.value
 
     def enforce_all(self):
         self._chk(self.L.samos_enforce_all(self.h))

@@ -85,6 +85,9 @@ struct samos_ctx {
   std::vector<double> soft_tab, lj_tab, pol_tab, nem_tab;  // host copies [3][16][16]
   double* d_tab = nullptr;                                  // 4 tables x 3 x 256
   ConstraintD con{-1, 1.0, 0.0, 0, 0.0, 0.0, 0.0, 100, 1e-6};
+  // sphere radius schedule (Constraint rescale parameters)
+  double rescale = 1.0, rescale_scale = 1.0;
+  int rescale_steps = 1000, rescale_freq = 10;
   std::map<std::string, Integ> integs;
   ClockD* d_clock = nullptr;
   ChunkCtl* d_chunk = nullptr;  // control words of the captured step chunks
@@ -1774,14 +1777,29 @@ int samos_integrate(samos_ctx* ctx) {
   return 0;
 }
 
-int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags) {
-  if (ctx->con.kind < 0)
-    return fail(ctx, "Constraint has not been defined. Please define it using \"constraint\" command before running simulation.");
-  if (ctx->integs.empty())
-    return fail(ctx, "Integrator has not been defined. Please define it using \"integrator\" command before running simulation.");
-  if (steps < 0 || t_begin < 0 || t_end > steps + 1 || t_begin > t_end || (t_begin == t_end && t_begin != 0))
-    return fail(ctx, "invalid run range");
-  if (t_begin > 0 && !ctx->run_begun) return fail(ctx, "run range does not continue a started run");
+}  // extern "C"
+
+namespace {
+
+// ConstraintSphere::rescale (src/constraints/constraint_sphere.cpp:87-104): due
+// at loop heads whose time step is a multiple of rescale_freq below rescale_steps.
+bool rescale_due(const samos_ctx* ctx, int time_step) {
+  return ctx->con.kind == 2 && ctx->rescale != 1.0 && time_step % ctx->rescale_freq == 0 &&
+         time_step < ctx->rescale_steps;
+}
+
+// The radius grows by its factor, every particle is enforced on the new sphere,
+// and the neighbour list is rebuilt (src/samos.cpp:882-886).
+int apply_rescale(samos_ctx* ctx) {
+  ctx->con.r *= ctx->rescale_scale;
+  launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+  HIPCHK(hipGetLastError());
+  if (ctx->has_nlist) return build_sync(ctx);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int run_range_core(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags) {
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
   // Fused schedule: one Langevin BAOAB integrator, pair potentials only.  The
@@ -1866,6 +1884,63 @@ int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags
   }
   HIPCHK(hipGetLastError());
   return check_overflow(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+int samos_run_range(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags) {
+  if (ctx->con.kind < 0)
+    return fail(ctx, "Constraint has not been defined. Please define it using \"constraint\" command before running simulation.");
+  if (ctx->integs.empty())
+    return fail(ctx, "Integrator has not been defined. Please define it using \"integrator\" command before running simulation.");
+  if (steps < 0 || t_begin < 0 || t_end > steps + 1 || t_begin > t_end || (t_begin == t_end && t_begin != 0))
+    return fail(ctx, "invalid run range");
+  if (t_begin > 0 && !ctx->run_begun) return fail(ctx, "run range does not continue a started run");
+  const int ts0 = ctx->time_step - t_begin;  // time step of the run's iteration 0
+  bool rescaling = false;
+  for (int t = t_begin; t <= std::min(t_end, steps) && !rescaling; ++t) rescaling = rescale_due(ctx, ts0 + t);
+  if (!rescaling) return run_range_core(ctx, t_begin, t_end, steps, flags);
+  // The range is split at the loop heads where the sphere rescales: each piece
+  // ends in the loop-head state of its last iteration, the rescale follows
+  // there, and the loop-head enforce and forces are redone on the new sphere.
+  // The radius changes between pieces, so they launch step by step.
+  flags |= SAMOS_RUN_NO_GRAPH;
+  if (t_begin == 0 && !ctx->run_begun && rescale_due(ctx, ts0))
+    if (int r = apply_rescale(ctx)) return r;
+  int a = t_begin;
+  if (a == t_end) return run_range_core(ctx, a, t_end, steps, flags);
+  while (a < t_end) {
+    int r = -1;
+    for (int t = a + 1; t <= t_end && t <= steps && r < 0; ++t)
+      if (rescale_due(ctx, ts0 + t)) r = t;
+    if (r < 0) return run_range_core(ctx, a, t_end, steps, flags);
+    if (int rc = run_range_core(ctx, a, r, steps, flags)) return rc;
+    if (int rc = apply_rescale(ctx)) return rc;
+    launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+    loop_head_forces(ctx, nullptr, (flags & SAMOS_RUN_FP32_FORCE) != 0);
+    HIPCHK(hipGetLastError());
+    a = r;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int samos_constraint_rescale(samos_ctx* ctx, double rescale, int rescale_steps, int rescale_freq) {
+  if (ctx->con.kind != 2) return fail(ctx, "only the sphere constraint rescales");
+  if (rescale_steps <= 0 || rescale_freq <= 0) return fail(ctx, "rescale_steps and rescale_freq must be positive");
+  ctx->rescale = rescale;
+  ctx->rescale_steps = rescale_steps;
+  ctx->rescale_freq = rescale_freq;
+  // Constraint constructor (src/constraints/constraint.hpp:103)
+  ctx->rescale_scale = std::pow(rescale, static_cast<double>(rescale_freq) / static_cast<double>(rescale_steps));
+  return 0;
+}
+
+int samos_constraint_radius(samos_ctx* ctx, double* r) {
+  *r = ctx->con.r;
+  return 0;
 }
 
 int samos_integrator_disable(samos_ctx* ctx, const char* type, const char* group) {

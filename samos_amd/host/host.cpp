@@ -503,8 +503,14 @@ void PairNematicAlign::compute() { sys_->check(samos_align_compute(sys_->ctx(), 
 // Constraints (src/constraints/constraint.hpp:56-120 and the per-surface headers)
 // ---------------------------------------------------------------------------
 Constraint::Constraint(SystemPtr sys, MessengerPtr msg, pairs_type& p) : sys_(sys), msg_(msg) {
-  if (has(p, "rescale") && to_double(p["rescale"], "rescale") != 1.0)
-    fail(msg_, "Constraint rescaling is not supported by this build.", "Unsupported parameter rescale.");
+  // rescale, rescale_steps, rescale_freq (constraint.hpp:85-108); the steps and
+  // frequency are read as numbers and stored as ints, as there
+  if (has(p, "rescale")) {
+    rescale_ = to_double(p["rescale"], "rescale");
+    msg_->msg(Messenger::INFO, "Constraint. Rescaling constraint by factor " + p["rescale"] + ".");
+  }
+  if (has(p, "rescale_steps")) rescale_steps_ = (int)to_double(p["rescale_steps"], "rescale_steps");
+  if (has(p, "rescale_freq")) rescale_freq_ = (int)to_double(p["rescale_freq"], "rescale_freq");
   if (has(p, "group") && p["group"] != "all")
     fail(msg_, "Constraints on groups other than \"all\" are not supported by this build.", "Unsupported group.");
   // The reference reads maxiter and tol for every constraint; both go through an

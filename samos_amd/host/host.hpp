@@ -237,6 +237,8 @@ class Constraint {
   MessengerPtr msg_;
   int max_iter_ = 100;   // maxiter (constraint.hpp:62-73)
   double tol_ = 1e-6;    // tol (constraint.hpp:74-84)
+  double rescale_ = 1.0; // rescale, rescale_steps, rescale_freq (constraint.hpp:85-108)
+  int rescale_steps_ = 1000, rescale_freq_ = 10;
 };
 using ConstraintPtr = std::shared_ptr<Constraint>;
 
@@ -263,6 +265,12 @@ class ConstraintSphere : public Constraint {
   ConstraintSphere(SystemPtr sys, MessengerPtr msg, pairs_type& param);
   int kind() const override { return SAMOS_CONSTRAINT_SPHERE; }
   double param() const override { return r_; }
+  // ConstraintSphere::rescale (constraint_sphere.cpp:87-104) runs on the device
+  int install(samos_ctx* ctx) override {
+    int rc = samos_constraint(ctx, kind(), param(), flag());
+    if (rc == 0 && rescale_ != 1.0) rc = samos_constraint_rescale(ctx, rescale_, rescale_steps_, rescale_freq_);
+    return rc;
+  }
 
  private:
   double r_ = 1.0;
