@@ -250,6 +250,15 @@ int samos_sync(samos_ctx* ctx);
  * rebuild; "tile_cells" 0 = automatic, 1..80 = cells per workgroup tile of the
  * cell-tiled planar engine. */
 int samos_set_option(samos_ctx* ctx, const char* name, double value);
+/* "neighbour_capacity" n (no counterpart in the reference, whose lists are
+ * vectors): neighbour slots per particle, at least the current largest count;
+ * the list keeps its pairs.  A device rebuild that finds more neighbours than
+ * the capacity stops the run's captured steps before any of them reads the
+ * list; the capacity then grows and the list is rebuilt from the same
+ * positions, so the run continues exactly as with room to spare.
+ * Run statistics, no counterpart in the reference: out[4] = step-chunk graphs
+ * instantiated, chunk graphs reused, list overflows recovered, capacity. */
+int samos_run_stats(samos_ctx* ctx, long long* out);
 /* Cell-tiled planar engine (no counterpart in the reference): planar fused
  * Langevin runs keep the state in cell order and the Verlet list as stencil
  * bitmasks (see SAMOS_RUN_TILES).  info[6]: tiled runs, rebuilds inside

@@ -99,6 +99,7 @@ def load_library(path: str = LIB_PATH):
         "samos_profile": (I, [P, I]),
         "samos_profile_read": (I, [P, ctypes.c_char_p, dp, llp]),
         "samos_sync": (I, [P]),
+        "samos_run_stats": (I, [P, llp]),
         "samos_set_option": (I, [P, ctypes.c_char_p, D]),
         "samos_ens_last_error": (ctypes.c_char_p, [P]),
         "samos_ens_create": (I, [ctypes.POINTER(P), I, I, ip, dp, ip, I, dp]),
@@ -392,6 +393,11 @@ class System:
         v = ctypes.c_double()
         self._chk(self.L.samos_reduce(self.h, OBSERVABLE[what], ctypes.byref(v)))
         return v.value
+
+    def run_stats(self):
+        a = (ctypes.c_longlong * 4)()
+        self._chk(self.L.samos_run_stats(self.h, a))
+        return dict(zip(["graphs_instantiated", "graphs_reused", "overflow_recoveries", "capacity"], list(a)))
 
     def sync(self):
         self._chk(self.L.samos_sync(self.h))

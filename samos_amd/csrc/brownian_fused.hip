@@ -32,6 +32,7 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
   ChunkCtl* const ck = ba.chunk;
   if (ck) {
     const int j = ba.slot;
+    if (nl.flags[2]) return;  // the list overflowed at a rebuild: no step may read it
     if (ck->halt | ck->req[(j + 1) & 1]) {
       if (blockIdx.x == 0 && threadIdx.x == 0) ck->halt = 1;
       return;

@@ -119,6 +119,7 @@ struct samos_ctx {
   std::vector<GraphEntry> graphs;
   unsigned long long graph_tick = 0;
   long long graph_stats[2] = {0, 0};  // instantiations, reuses
+  long long overflow_recoveries = 0;  // list overflows recovered inside runs
   // cell-tiled engine of planar fused runs (allocated on first use)
   TileD td{};
   void* tile_mem = nullptr;
@@ -330,7 +331,8 @@ int chunk_graphs(samos_ctx* ctx, const std::string& base, const std::function<in
 // largest chunks, as many as the slots per chunk seen so far suggest; a
 // shorter remainder by chunks whose sizes add up to it exactly (one spare
 // slot for an odd count), so no slot is launched only to return.  The host
-// reads the base after each batch.  Returns the base reached in *t.
+// reads the base after each batch.  Returns the base reached in *t, and 1
+// when the batch stopped at a neighbour-list overflow.
 int replay_chunks(samos_ctx* ctx, const ChunkSet& set, int end, int* t) {
   const int Mmax = set.back().first;
   double per = Mmax;
@@ -351,8 +353,13 @@ int replay_chunks(samos_ctx* ctx, const ChunkSet& set, int end, int* t) {
       }
     }
     HIPCHK(hipMemcpyAsync(h_base, &ctx->d_chunk->base, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(h_base + 1, ctx->d_flags + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
-    const int tn = *h_base;
+    const int tn = h_base[0];
+    if (h_base[1]) {  // a rebuild overflowed: the slots after it stopped
+      *t = tn;
+      return 1;
+    }
     if (tn <= *t || tn > end) return fail(ctx, "captured step chunks made no progress");
     if (k) per = std::max(1.0, (double)(tn - *t) / k);
     *t = tn;
@@ -410,14 +417,15 @@ int read_flags(samos_ctx* ctx, int* h) {
 
 // Synchronous build with neighbour capacity growth: rebuilds until no slot
 // overflowed, then keeps 1.5x headroom for the device-side rebuilds of a run.
-// `at` (optional) replaces the slot columns the build reads positions from.
-int build_sync(samos_ctx* ctx, const Fields* at = nullptr) {
+// `at` (optional) replaces the slot columns the build reads positions from
+// (planar: from its interleaved (x, y) column).
+int build_sync(samos_ctx* ctx, const Fields* at = nullptr, bool planar = false) {
   for (int attempt = 0; attempt < 4; ++attempt) {
     HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
     HIPCHK(hipMemsetAsync(ctx->d_flags + 5, 0, sizeof(int), ctx->st));
     {
       ProfScope ps(ctx, "rebuild");
-      launch_rebuild(ctx->st, at ? *at : fields(ctx), ctx->nl, ctx->box, ctx->n, nullptr);
+      launch_rebuild(ctx->st, at ? *at : fields(ctx), ctx->nl, ctx->box, ctx->n, nullptr, planar ? 1 : 0);
     }
     HIPCHK(hipGetLastError());
     int h[8];
@@ -427,6 +435,26 @@ int build_sync(samos_ctx* ctx, const Fields* at = nullptr) {
     if (int r = alloc_nbr(ctx, (int)std::ceil(need * 1.5) + 16)) return r;
   }
   return fail(ctx, "neighbour list capacity could not be established");
+}
+
+// A device rebuild of this run found more neighbours than the list holds and
+// stored a truncated list (flags[2]); no step has read it (chunk slots stop at
+// the flag, direct steps are checked before the next one).  The capacity
+// grows and the list is rebuilt from the same positions, which gives the
+// rebuild's pair set, count and snapshot exactly; the run then continues.
+// Returns 1 when a recovery happened.
+int recover_overflow(samos_ctx* ctx, const Fields& at, bool planar) {
+  int h[8];
+  if (int r = read_flags(ctx, h)) return r;
+  if (!h[2]) return 0;
+  const int builds = h[4];
+  if (int r = alloc_nbr(ctx, (int)std::ceil(h[5] * 1.5) + 16)) return r;
+  if (int r = build_sync(ctx, &at, planar)) return r;
+  // the rebuild that overflowed has been counted; its repetition is not
+  HIPCHK(hipMemcpyAsync(ctx->d_flags + 4, &builds, sizeof(int), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->overflow_recoveries++;
+  return 1;
 }
 
 int resort(samos_ctx* ctx) {
@@ -713,7 +741,18 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     // the binning of either buffer reads z (= zpos after the enforce)
     HIPCHK(hipMemcpyAsync(ctx->alt[2], ctx->dcol[2], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
   }
+  auto at_buffer = [&](int par) {
+    Fields f = fields(ctx, par & 1);
+    if (planar) f.xy = ctx->xy[par & 1];
+    return f;
+  };
+  const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
+  const bool chunks = graph && b - (a + 1) >= 2;
   if (int r = fused_step(ctx, I, 0, a, 0, 0, steps, f32, planar, mass_u, true)) return r;
+  // a list overflow at this rebuild is caught by the chunks (their slots stop
+  // at it) or here, before the next step reads the list
+  if (!chunks)
+    if (int r = recover_overflow(ctx, at_buffer(1), planar); r < 0) return r;
   launch_clock_advance(ctx->st, ctx->d_clock, 1);
   int t = a + 1;
   int buf = 1;  // buffer holding x_t
@@ -724,17 +763,10 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   // past the chunk's limit return at once as well, so one chunk serves any
   // range; short ranges use a smaller power-of-two chunk.  The rebuild
   // decisions and the steps are exactly those of one launch per step; only
-  // the skipped-rebuild dispatches between steps are gone.  Slots cover
-  // iterations up to b - 2 (the range's last iteration has its own loop-head
-  // flag and is launched directly).  Instantiated chunks are kept per context.
-  const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
-  if (graph && b - t >= 2) {
-    Fields f0 = fields(ctx, 0), f1 = fields(ctx, 1);
-    if (planar) { f0.xy = ctx->xy[0]; f1.xy = ctx->xy[1]; }
-    KeyBuf base;
-    base << 'L' << f32 << planar << mass_u << pot_mask(ctx);
-    key_context(base, ctx);
-    key_integ(base, I);
+  // the skipped-rebuild dispatches between steps are gone.  Slots cover the
+  // iterations up to b - 1.  Instantiated chunks are kept per context.
+  if (chunks) {
+    Fields f0 = at_buffer(0), f1 = at_buffer(1);
     auto record = [&](int m) -> int {
       // the loop-head enforce of each slot's next iteration is decided on the
       // device from the run's last iteration
@@ -745,8 +777,6 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
       launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
       return 0;
     };
-    ChunkSet set;
-    if (int r = chunk_graphs(ctx, base.s, record, &set)) return r;
     // slots start at buffer parity 0
     if (planar) {
       HIPCHK(hipMemcpyAsync(ctx->xy[0], ctx->xy[1], sizeof(double2) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
@@ -755,14 +785,29 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
         HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->alt[k], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
     }
     buf = 0;
-    launch_chunk_init(ctx->st, ctx->d_chunk, t, b, steps);
-    if (int r = replay_chunks(ctx, set, b, &t)) return r;
+    for (;;) {
+      // the list's arrays (and so the chunks' key) change when it grows
+      KeyBuf base;
+      base << 'L' << f32 << planar << mass_u << pot_mask(ctx);
+      key_context(base, ctx);
+      key_integ(base, I);
+      ChunkSet set;
+      if (int r = chunk_graphs(ctx, base.s, record, &set)) return r;
+      launch_chunk_init(ctx->st, ctx->d_chunk, t, b, steps);
+      const int r = replay_chunks(ctx, set, b, &t);
+      if (r < 0) return r;
+      if (r == 0) break;
+      // stopped at an overflowed rebuild of the positions now in buffer 0
+      if (int rc = recover_overflow(ctx, f0, planar); rc < 0) return rc;
+    }
     // the last chunk served any request; direct steps use the parity flags
     HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   }
   const int t0 = t;
-  for (; t < b; ++t)
+  for (; t < b; ++t) {
     if (int r = fused_step(ctx, I, 1, t, buf + (t - t0), t - t0, steps, f32, planar, mass_u, true)) return r;
+    if (int r = recover_overflow(ctx, at_buffer(buf + (t - t0) + 1), planar); r < 0) return r;
+  }
   if (b > t0) launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
   if (int r = fused_step(ctx, I, 2, b, buf + (b - t0), 0, steps, f32, planar, mass_u, true)) return r;
   const int pf = (buf + (b - t0)) & 1;  // buffer holding x_b
@@ -882,12 +927,6 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
   const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
   if (graph && b - t >= 2) {
     Fields f0 = buffer(0), f1 = buffer(1);
-    KeyBuf base;
-    base << 'B' << f32 << planar << pot_mask(ctx) << al_mask(ctx);
-    key_context(base, ctx);
-    key_integ(base, I);
-    if (P) key_integ(base, *P);
-    for (const Integ* G : grouped) key_integ(base, *G);
     auto record = [&](int m) -> int {
       for (int j = 0; j < m; ++j) step(b - 2, j & 1, j, ctx->d_chunk, j);
       launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, n, planar ? 1 : 0, 1);
@@ -895,14 +934,31 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
       launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
       return 0;
     };
-    ChunkSet set;
-    if (int r = chunk_graphs(ctx, base.s, record, &set)) return r;
-    launch_chunk_init(ctx->st, ctx->d_chunk, t, b, steps);
-    if (int r = replay_chunks(ctx, set, b, &t)) return r;
+    for (;;) {
+      // the list's arrays (and so the chunks' key) change when it grows
+      KeyBuf base;
+      base << 'B' << f32 << planar << pot_mask(ctx) << al_mask(ctx);
+      key_context(base, ctx);
+      key_integ(base, I);
+      if (P) key_integ(base, *P);
+      for (const Integ* G : grouped) key_integ(base, *G);
+      ChunkSet set;
+      if (int r = chunk_graphs(ctx, base.s, record, &set)) return r;
+      launch_chunk_init(ctx->st, ctx->d_chunk, t, b, steps);
+      const int r = replay_chunks(ctx, set, b, &t);
+      if (r < 0) return r;
+      if (r == 0) break;
+      // stopped at an overflowed rebuild of the positions now in buffer 0
+      if (int rc = recover_overflow(ctx, f0, planar); rc < 0) return rc;
+    }
     HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   }
   const int t0 = t;
-  for (; t < b; ++t) step(t, (buf + t - t0) & 1, t - t0, nullptr, 0);
+  for (; t < b; ++t) {
+    step(t, (buf + t - t0) & 1, t - t0, nullptr, 0);
+    // the next step reads the list this step's rebuild may have overflowed
+    if (int r = recover_overflow(ctx, buffer((buf + t - t0 + 1) & 1), planar); r < 0) return r;
+  }
   if (b > t0) launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
   const int pf = (buf + b - t0) & 1;  // buffer holding x_b and n_b
   if (planar) {
@@ -1841,6 +1897,20 @@ int run_range_core(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags)
   // replayed; K is the largest power of two up to 32 that fits what is left.
   bool graphable = !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof && ctx->coop == 0;
   for (auto& kv : ctx->integs) graphable = graphable && kv.second.name != "fire";
+  if (graphable && needs_nlist(ctx)) {
+    // Captured iterations cannot stop at a list overflow: keep headroom (a
+    // quarter of the capacity) before replaying them.
+    int h[8];
+    if (int r = read_flags(ctx, h)) return r;
+    if (4 * h[5] > 3 * ctx->nl.stride) {
+      if (int r = alloc_nbr(ctx, (int)std::ceil(h[5] * 1.5) + 16)) return r;
+      Fields at = fields(ctx);
+      at.x = ctx->nl.ox; at.y = ctx->nl.oy; at.z = ctx->nl.oz;
+      const int builds = h[4];
+      if (int r = build_sync(ctx, &at)) return r;  // the same list, from the last build's snapshot
+      HIPCHK(hipMemcpyAsync(ctx->d_flags + 4, &builds, sizeof(int), hipMemcpyHostToDevice, ctx->st));
+    }
+  }
   if (graphable) {
     ctx->age_checked = false;
     // every captured iteration ends with the next loop head: t + K <= steps
@@ -1869,6 +1939,18 @@ int run_range_core(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags)
         t += K;
         ctx->time_step += K;
         for (auto& kv : ctx->integs) kv.second.calls += K;
+        int h[8];
+        if (int r = read_flags(ctx, h)) return r;
+        if (h[2]) {
+          // Iterations after the overflowed rebuild read a truncated list:
+          // stop here, with the capacity grown for a rerun.
+          alloc_nbr(ctx, (int)std::ceil(h[5] * 1.5) + 16);
+          build_sync(ctx);
+          return fail(ctx, "Neighbour list overflow inside the captured step sequence ending at run iteration " +
+                               std::to_string(t) + " (a particle had " + std::to_string(h[5]) +
+                               " neighbours); the capacity has grown, rerun from a saved state or use "
+                               "SAMOS_RUN_NO_GRAPH, whose steps recover from an overflow.");
+        }
       }
     }
   }
@@ -1876,6 +1958,9 @@ int run_range_core(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags)
     for (auto& kv : ctx->integs)
       if (int r = integrate_one(ctx, kv.second, t)) return r;
     check_and_rebuild(ctx);
+    // the loop-head forces below read the list this rebuild may have overflowed
+    if (needs_nlist(ctx))
+      if (int r = recover_overflow(ctx, fields(ctx), false); r < 0) return r;
     ctx->time_step++;
     if (t + 1 <= steps) {  // loop head of t + 1: enforce, reset, forces, torques
       launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
@@ -2096,6 +2181,14 @@ int samos_profile_read(samos_ctx* ctx, const char* cls, double* ms_total, long l
   return 0;
 }
 
+int samos_run_stats(samos_ctx* ctx, long long* out) {
+  out[0] = ctx->graph_stats[0];
+  out[1] = ctx->graph_stats[1];
+  out[2] = ctx->overflow_recoveries;
+  out[3] = ctx->nl.stride;
+  return 0;
+}
+
 int samos_sync(samos_ctx* ctx) {
   HIPCHK(hipStreamSynchronize(ctx->st));
   HIPCHK(hipGetLastError());
@@ -2120,6 +2213,21 @@ int samos_set_option(samos_ctx* ctx, const char* name, double value) {
     if (value < 2.0 || value > 256.0 || value != std::floor(value) || ((int)value & 1))
       return fail(ctx, "chunk_slots must be an even integer in 2..256");
     ctx->chunk_slots = (int)value;
+  } else if (k == "neighbour_capacity") {
+    // capacity per slot: max(value, the current largest count); the list is
+    // rebuilt from the last build's snapshot, so it holds the same pairs
+    if (!ctx->has_nlist) return fail(ctx, "no neighbour list");
+    if (value < 0.0 || value != std::floor(value)) return fail(ctx, "neighbour_capacity must be a non-negative integer");
+    int h[8];
+    if (int r = read_flags(ctx, h)) return r;
+    if (int r = alloc_nbr(ctx, std::max((int)value, h[5]))) return r;
+    Fields at = fields(ctx);
+    at.x = ctx->nl.ox; at.y = ctx->nl.oy; at.z = ctx->nl.oz;
+    HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
+    launch_rebuild(ctx->st, at, ctx->nl, ctx->box, ctx->n, nullptr);
+    HIPCHK(hipMemcpyAsync(ctx->d_flags + 4, &h[4], sizeof(int), hipMemcpyHostToDevice, ctx->st));
+    if (int r = read_flags(ctx, h)) return r;
+    if (h[2]) return fail(ctx, "neighbour_capacity below the current largest count");
   } else if (k == "tile_cells") {
     if (value < 0.0 || value > kTileMaxCells || value != std::floor(value))
       return fail(ctx, "tile_cells must be 0 (automatic) or an integer in 1.." + std::to_string(kTileMaxCells));

@@ -342,8 +342,10 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
   ChunkCtl* const ck = fa.chunk;
   if (ck) {
     // Every wavefront reads the same words, written by earlier kernels only:
-    // the whole grid either returns here or runs the step.
+    // the whole grid either returns here or runs the step.  A list that
+    // overflowed at a rebuild stops every later slot before any step reads it.
     const int j = fa.slot;
+    if (nl.flags[2]) return;
     if (ck->halt | ck->req[(j + 1) & 1]) {
       if (blockIdx.x == 0 && threadIdx.x == 0) ck->halt = 1;
       return;
