@@ -37,6 +37,8 @@ struct Integ {
   int N_min = 5, last_neg = 0; bool converged = false; double old_energy = 1e15;
   double last[4] = {0, 0, 0, 0};
   double limit = -1.0, theta_limit = -1.0;  // nve; negative: not given
+  FireD* fire = nullptr;  // fire: scalar state on the device
+  int fire_ndof3 = -1;    // 3 N of the group as last uploaded
 };
 
 struct ProfAcc {
@@ -269,7 +271,7 @@ void key_integ(KeyBuf& k, const Integ& I) {
   k << I.name.size();
   k.s += I.name;
   k << I.slot << I.gbit << I.dt << I.temp << I.v0 << I.nu << I.mu << I.mur << I.stoch << I.flip_p << I.nematic
-    << I.velocity << I.seed << I.gamma << I.method << I.R << I.limit << I.theta_limit;
+    << I.velocity << I.seed << I.gamma << I.method << I.R << I.limit << I.theta_limit << I.fire;
 }
 
 // The instantiated graph of `key`, recorded by `record` on first use.  The
@@ -591,50 +593,39 @@ int integrate_one(samos_ctx* ctx, Integ& I, int run_step, int j = -1) {
   } else if (I.name == "fire") {
     {
       ProfScope ps(ctx, "integrate");
-      launch_fire_a(ctx->st, f, ctx->box, ctx->con, n, I.dt, I.gbit);
+      launch_fire_a(ctx->st, f, ctx->box, ctx->con, n, I.fire, I.gbit);
     }
     // reset forces and torques, Potential::compute, Aligner::compute
     loop_head_forces(ctx, ctx->d_en);
     {
       ProfScope ps(ctx, "integrate");
-      launch_fire_b(ctx->st, f, ctx->box, ctx->con, n, I.dt, ctx->d_cols, ctx->np, I.gbit);
+      launch_fire_b(ctx->st, f, ctx->box, ctx->con, n, I.fire, ctx->d_cols, ctx->np, I.gbit);
     }
+    // P, |F|^2, |v|^2 and the pair energies in fixed-order sums, then the
+    // scalar update (src/integrators/integrator_fire.cpp:97-147) on the device
     launch_reduce_sum(ctx->st, ctx->d_cols, 3, ctx->np, n, ctx->d_partial, ctx->d_out);
     launch_reduce_sum(ctx->st, ctx->d_en, 2, ctx->np, n, ctx->d_partial, ctx->d_out + 3);
     if (ctx->extra.count > 0)
       launch_reduce_sum(ctx->st, ctx->d_en + 4L * ctx->np, 1, ctx->np, n, ctx->d_partial, ctx->d_out + 5);
-    HIPCHK(hipMemcpyAsync(ctx->h_out, ctx->d_out, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
-    HIPCHK(hipStreamSynchronize(ctx->st));
-    // IntegratorFIRE::integrate, scalar part (src/integrators/integrator_fire.cpp:97-147)
-    const double P = ctx->h_out[0];
-    const double Fn = std::sqrt(ctx->h_out[1]);
-    const double Vn = std::sqrt(ctx->h_out[2]);
-    double E = 0.5 * ctx->h_out[3] + 0.5 * ctx->h_out[4];
-    if (ctx->extra.count > 0) E += 0.5 * ctx->h_out[5];
-    const double sqrt_ndof = std::sqrt((double)(3 * group_size(ctx, I.gbit)));
-    I.last[0] = P; I.last[1] = Fn; I.last[2] = Vn; I.last[3] = E;
-    if (Fn / sqrt_ndof < I.F_tol && std::fabs(E - I.old_energy) < I.E_tol) {
-      I.converged = true;
-      return 0;
-    }
-    const double inv_F = 1.0 / Fn;
-    const double f1 = 1.0 - I.alpha;
-    const double f2 = I.alpha * inv_F * Vn;
-    int zero_v = 0;
-    if (P > 0.0) {
-      I.last_neg++;
-      if (I.last_neg > I.N_min) { I.dt = std::min(I.dt * I.f_inc, I.dt_max); I.alpha *= I.f_alpha; }
-    } else {
-      I.dt *= I.f_dec;
-      I.alpha = I.alpha_init;
-      I.last_neg = 0;
-      zero_v = 1;
-    }
-    launch_fire_mix(ctx->st, f, n, f1, f2, zero_v, I.gbit);
-    I.converged = false;
-    I.old_energy = E;
+    launch_fire_update(ctx->st, I.fire, ctx->d_out, ctx->extra.count > 0 ? 1 : 0);
+    launch_fire_mix(ctx->st, f, n, I.fire, I.gbit);
   }
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// The group's size enters FIRE's sqrt(3 N); groups may gain members after the
+// integrator exists, so it is refreshed before each run range (outside any
+// capture).
+int sync_fire_groups(samos_ctx* ctx) {
+  for (auto& kv : ctx->integs) {
+    Integ& I = kv.second;
+    if (I.name != "fire") continue;
+    const int ndof3 = 3 * group_size(ctx, I.gbit);
+    if (ndof3 == I.fire_ndof3) continue;
+    HIPCHK(hipMemcpy(&I.fire->ndof3, &ndof3, sizeof(int), hipMemcpyHostToDevice));
+    I.fire_ndof3 = ndof3;
+  }
   return 0;
 }
 
@@ -1307,7 +1298,7 @@ int samos_destroy(samos_ctx* ctx) {
   for (auto& g : ctx->graphs) hipGraphExecDestroy(g.exec);
   ctx->graphs.clear();
   free_nlist(ctx);
-  for (auto& kv : ctx->integs) hipFree(kv.second.R);
+  for (auto& kv : ctx->integs) { hipFree(kv.second.R); hipFree(kv.second.fire); }
   for (int k = 0; k < kNumDoubleFields; ++k) hipFree(ctx->dcol[k]);
   for (int k = 0; k < kNumIntFields; ++k) hipFree(ctx->icol[k]);
   for (int k = 0; k < 3; ++k) { hipFree(ctx->alt[k]); hipFree(ctx->altn[k]); }
@@ -1708,7 +1699,17 @@ int samos_integrator_fire(samos_ctx* ctx, double alpha, double alpha_scale, doub
   Integ I;
   I.alpha = alpha; I.alpha_init = alpha; I.f_alpha = alpha_scale; I.F_tol = f_tol; I.E_tol = e_tol;
   I.f_inc = dt_inc; I.f_dec = dt_dec; I.dt_max = dt_max; I.N_min = min_neg_steps;
-  return add_integ(ctx, I, "fire");
+  if (int r = add_integ(ctx, I, "fire")) return r;
+  Integ& J = last_integ(ctx, "fire");
+  FireD q{};
+  q.dt = J.dt; q.alpha = J.alpha; q.alpha_init = J.alpha_init; q.f_alpha = J.f_alpha; q.F_tol = J.F_tol;
+  q.E_tol = J.E_tol; q.f_inc = J.f_inc; q.f_dec = J.f_dec; q.dt_max = J.dt_max; q.old_energy = 1e15;
+  q.N_min = J.N_min;
+  q.ndof3 = 3 * group_size(ctx, J.gbit);
+  J.fire_ndof3 = q.ndof3;
+  HIPCHK(hipMalloc(&J.fire, sizeof(FireD)));
+  HIPCHK(hipMemcpy(J.fire, &q, sizeof(FireD), hipMemcpyHostToDevice));
+  return 0;
 }
 
 int samos_integrator_nve(samos_ctx* ctx, double limit, double angle_limit) {
@@ -1765,9 +1766,11 @@ int samos_fire_state(samos_ctx* ctx, double* out) {
   auto it = ctx->integs.begin();
   while (it != ctx->integs.end() && it->second.name != "fire") ++it;
   if (it == ctx->integs.end()) return fail(ctx, "no FIRE integrator");
-  Integ& I = it->second;
-  out[0] = I.dt; out[1] = I.alpha; out[2] = I.last[0]; out[3] = I.last[1]; out[4] = I.last[2];
-  out[5] = I.last[3]; out[6] = I.converged ? 1.0 : 0.0; out[7] = I.last_neg;
+  FireD q{};
+  HIPCHK(hipMemcpyAsync(&q, it->second.fire, sizeof(FireD), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  out[0] = q.dt; out[1] = q.alpha; out[2] = q.last[0]; out[3] = q.last[1]; out[4] = q.last[2];
+  out[5] = q.last[3]; out[6] = q.converged ? 1.0 : 0.0; out[7] = q.last_neg;
   return 0;
 }
 
@@ -1827,6 +1830,7 @@ int samos_align_compute(samos_ctx* ctx, int kind, double* energy) {
 
 int samos_integrate(samos_ctx* ctx) {
   if (ctx->con.kind < 0) return fail(ctx, "Constraint has not been defined. Please define it using \"constraint\" command before running simulation.");
+  if (int r = sync_fire_groups(ctx)) return r;
   for (auto& kv : ctx->integs)
     if (int r = integrate_one(ctx, kv.second, 0)) return r;
   HIPCHK(hipStreamSynchronize(ctx->st));
@@ -1856,6 +1860,7 @@ int apply_rescale(samos_ctx* ctx) {
 }
 
 int run_range_core(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags) {
+  if (int r = sync_fire_groups(ctx)) return r;
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
   // Fused schedule: one Langevin BAOAB integrator, pair potentials only.  The
@@ -1891,12 +1896,11 @@ int run_range_core(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags)
     return check_overflow(ctx);
   }
   int t = t_begin;
-  // Without FIRE (whose scalar update runs on the host) the step sequence is
-  // launch-only: chunks of K iterations are captured once, with the noise call
-  // and run step read from the device clock plus the iteration's offset, and
-  // replayed; K is the largest power of two up to 32 that fits what is left.
-  bool graphable = !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof && ctx->coop == 0;
-  for (auto& kv : ctx->integs) graphable = graphable && kv.second.name != "fire";
+  // The step sequence is launch-only (FIRE's scalar update included): chunks of
+  // K iterations are captured once, with the noise call and run step read from
+  // the device clock plus the iteration's offset, and replayed; K is the
+  // largest power of two up to 32 that fits what is left.
+  const bool graphable = !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof && ctx->coop == 0;
   if (graphable && needs_nlist(ctx)) {
     // Captured iterations cannot stop at a list overflow: keep headroom (a
     // quarter of the capacity) before replaying them.
@@ -2034,6 +2038,7 @@ int samos_integrator_disable(samos_ctx* ctx, const char* type, const char* group
   if (it == ctx->integs.end())
     return fail(ctx, std::string("Integrator ") + type + " (for group " + group + ") does not exist.");
   hipFree(it->second.R);
+  hipFree(it->second.fire);
   ctx->integs.erase(it);
   return 0;
 }
@@ -2080,6 +2085,7 @@ int samos_integrator_remove(samos_ctx* ctx, const char* type) {
   auto it = ctx->integs.find(std::string(type) + "_all");
   if (it == ctx->integs.end()) return fail(ctx, std::string("Integrator ") + type + " (for group all) does not exist.");
   hipFree(it->second.R);
+  hipFree(it->second.fire);
   ctx->integs.erase(it);
   return 0;
 }

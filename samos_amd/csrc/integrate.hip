@@ -211,9 +211,10 @@ __global__ void k_bbk_init(Fields f, int n, uint32_t seed, double* R) {
 }
 
 // FIRE first pass: half kick, drift, enforce, director half kick and rotation.
-__global__ void k_fire_a(Fields f, BoxD b, ConstraintD c, int n, double dt, unsigned gbit) {
+__global__ void k_fire_a(Fields f, BoxD b, ConstraintD c, int n, const FireD* fs, unsigned gbit) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n || !in_group(f, s, gbit)) return;
+  const double dt = fs->dt;
   PS p; load_ps(f, s, p);
   const double dt_2 = 0.5 * dt;
   p.vx += dt_2 * f.fx[s]; p.vy += dt_2 * f.fy[s]; p.vz += dt_2 * f.fz[s];
@@ -288,9 +289,11 @@ __global__ void k_nve_b(Fields f, BoxD b, ConstraintD c, int n, double dt, doubl
 
 // FIRE second pass: half kick, per-slot P = v.F, |F|^2, |v|^2 (before the
 // enforce, as in the reference), enforce, director half kick.
-__global__ void k_fire_b(Fields f, BoxD b, ConstraintD c, int n, double dt, double* cols, long ld, unsigned gbit) {
+__global__ void k_fire_b(Fields f, BoxD b, ConstraintD c, int n, const FireD* fs, double* cols, long ld,
+                         unsigned gbit) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
+  const double dt = fs->dt;
   if (!in_group(f, s, gbit)) {
     cols[s] = 0.0; cols[ld + s] = 0.0; cols[2 * ld + s] = 0.0;
     return;
@@ -308,9 +311,48 @@ __global__ void k_fire_b(Fields f, BoxD b, ConstraintD c, int n, double dt, doub
   store_ps(f, s, p);
 }
 
-__global__ void k_fire_mix(Fields f, int n, double f1, double f2, int zero_v, unsigned gbit) {
+// IntegratorFIRE::integrate, scalar part (src/integrators/integrator_fire.cpp:97-147),
+// in one thread with the reference's operation order.
+__global__ void k_fire_update(FireD* fs, const double* red, int has_extra) {
+  if (threadIdx.x != 0) return;
+  FireD q = *fs;
+  const double P = red[0];
+  const double Fn = sqrt(red[1]);
+  const double Vn = sqrt(red[2]);
+  double E = 0.5 * red[3] + 0.5 * red[4];
+  if (has_extra) E += 0.5 * red[5];
+  const double sqrt_ndof = sqrt((double)q.ndof3);
+  q.last[0] = P; q.last[1] = Fn; q.last[2] = Vn; q.last[3] = E;
+  if (Fn / sqrt_ndof < q.F_tol && fabs(E - q.old_energy) < q.E_tol) {
+    q.converged = 1;
+    q.mix = 0;
+    *fs = q;
+    return;
+  }
+  const double inv_F = 1.0 / Fn;
+  q.f1 = 1.0 - q.alpha;
+  q.f2 = q.alpha * inv_F * Vn;
+  if (P > 0.0) {
+    q.last_neg++;
+    if (q.last_neg > q.N_min) { q.dt = fmin(q.dt * q.f_inc, q.dt_max); q.alpha *= q.f_alpha; }
+    q.zero_v = 0;
+  } else {
+    q.dt *= q.f_dec;
+    q.alpha = q.alpha_init;
+    q.last_neg = 0;
+    q.zero_v = 1;
+  }
+  q.mix = 1;
+  q.converged = 0;
+  q.old_energy = E;
+  *fs = q;
+}
+
+__global__ void k_fire_mix(Fields f, int n, const FireD* fs, unsigned gbit) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n || !in_group(f, s, gbit)) return;
+  if (s >= n || !in_group(f, s, gbit) || !fs->mix) return;
+  const double f1 = fs->f1, f2 = fs->f2;
+  const int zero_v = fs->zero_v;
   double vx = f1 * f.vx[s] + f2 * f.fx[s];
   double vy = f1 * f.vy[s] + f2 * f.fy[s];
   double vz = f1 * f.vz[s] + f2 * f.fz[s];
@@ -621,13 +663,17 @@ void launch_bbk_init(hipStream_t st, Fields f, int n, uint32_t seed, double* R) 
   hipLaunchKernelGGL(k_bbk_init, dim3(grid1(n)), dim3(kBlock), 0, st, f, n, seed, R);
 }
 
-void launch_fire_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, unsigned gbit) {
-  hipLaunchKernelGGL(k_fire_a, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, gbit);
+void launch_fire_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, const FireD* fs, unsigned gbit) {
+  hipLaunchKernelGGL(k_fire_a, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, fs, gbit);
 }
 
-void launch_fire_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double* cols, long ld,
+void launch_fire_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, const FireD* fs, double* cols, long ld,
                    unsigned gbit) {
-  hipLaunchKernelGGL(k_fire_b, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, cols, ld, gbit);
+  hipLaunchKernelGGL(k_fire_b, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, fs, cols, ld, gbit);
+}
+
+void launch_fire_update(hipStream_t st, FireD* fs, const double* red, int has_extra) {
+  hipLaunchKernelGGL(k_fire_update, dim3(1), dim3(64), 0, st, fs, red, has_extra);
 }
 
 void launch_nve_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta,
@@ -640,8 +686,8 @@ void launch_nve_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, do
   hipLaunchKernelGGL(k_nve_b, dim3(grid1(n)), dim3(kBlock), 0, st, f, box, con, n, dt, limit, theta, gbit);
 }
 
-void launch_fire_mix(hipStream_t st, Fields f, int n, double f1, double f2, int zero_v, unsigned gbit) {
-  hipLaunchKernelGGL(k_fire_mix, dim3(grid1(n)), dim3(kBlock), 0, st, f, n, f1, f2, zero_v, gbit);
+void launch_fire_mix(hipStream_t st, Fields f, int n, const FireD* fs, unsigned gbit) {
+  hipLaunchKernelGGL(k_fire_mix, dim3(grid1(n)), dim3(kBlock), 0, st, f, n, fs, gbit);
 }
 
 void launch_chunk_settle(hipStream_t st, ChunkCtl* ck, Fields f0, Fields f1, int n, int planar, int directors) {

@@ -69,11 +69,16 @@ void launch_langevin_b(hipStream_t st, int method, Fields f, int n, double dt, d
                        uint32_t seed, unsigned long long call, const ClockD* clock, int clock_slot, int t_off,
                        double* R, BoxD box, ConstraintD con, unsigned gbit = 1u);
 void launch_bbk_init(hipStream_t st, Fields f, int n, uint32_t seed, double* R);
-void launch_fire_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, unsigned gbit = 1u);
-// Slots outside the group write zeros to the reduction columns.
-void launch_fire_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double* cols, long ld,
+// FIRE iteration (src/integrators/integrator_fire.cpp:34-148) with its scalar
+// state on the device (dt read from `fs`).  fire_b writes P, |F|^2, |v|^2 per
+// slot to `cols` (zeros outside the group); after their sums and the energy
+// sums are in red[0..2] and red[3..5], fire_update applies the convergence
+// test and the dt / alpha adaptation, and fire_mix mixes the velocities.
+void launch_fire_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, const FireD* fs, unsigned gbit = 1u);
+void launch_fire_b(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, const FireD* fs, double* cols, long ld,
                    unsigned gbit = 1u);
-void launch_fire_mix(hipStream_t st, Fields f, int n, double f1, double f2, int zero_v, unsigned gbit = 1u);
+void launch_fire_update(hipStream_t st, FireD* fs, const double* red, int has_extra);
+void launch_fire_mix(hipStream_t st, Fields f, int n, const FireD* fs, unsigned gbit = 1u);
 // IntegratorNVE (src/integrators/integrator_nve.cpp:37-141): the part before the
 // force evaluation and the part after it; limit / theta < 0: not given.
 void launch_nve_a(hipStream_t st, Fields f, BoxD box, ConstraintD con, int n, double dt, double limit, double theta,

@@ -186,6 +186,18 @@ struct ClockD {
   int pad_;
 };
 
+// IntegratorFIRE's scalar state (src/integrators/integrator_fire.hpp:58-154),
+// kept on the device so that an iteration is launches only: the step size
+// and mixing factors the particle kernels use, the reduced P, |F|, |v|, E of
+// the last iteration and the adaptation counters.
+struct FireD {
+  double dt, alpha, alpha_init, f_alpha, F_tol, E_tol, f_inc, f_dec, dt_max, old_energy;
+  double last[4];   // P, |F|, |v|, E of the last iteration
+  double f1, f2;    // velocity mixing factors of the last iteration
+  int N_min, last_neg, converged, zero_v, mix;  // mix: the last iteration mixed velocities
+  int ndof3;        // 3 N of the integrator's group
+};
+
 // Particle ages only grow by the same dt per step for every integrated
 // particle (src/integrators/integrator_langevin.cpp:115,
 // src/integrators/integrator_brownian.cpp:118).  While every age is equal the

@@ -29,6 +29,11 @@ def setup(s, kind):
         s.align_polar(0.1, 2.4)
         s.set_timestep(0.01)
         s.integrator_brownian(v0=0.3, nu=0.1, seed=9, temp=(0, 0.02, 0.02, 1))
+    elif kind == "fire":  # captured FIRE iterations with the scalar state on the device
+        s.nlist(2.4, 0.3)
+        s.pair_soft(1.0, 2.0, use_radii=True)
+        s.set_timestep(0.01)
+        s.integrator_fire(dt_max=0.1, f_tol=0.0, e_tol=0.0)
     else:  # generic captured loop (Langevin SPV)
         s.nlist(2.4, 0.3)
         s.pair_soft(1.0, 2.0)
@@ -37,10 +42,12 @@ def setup(s, kind):
 
 
 def state(kind):
+    if kind == "fire":
+        return sc.plane_disks(1500, phi=0.88, seed=41, polydisperse=(0.8, 1.2))
     return sc.lj_lattice(36, rho=0.7, seed=41) if kind == "langevin" else sc.plane_disks(1800, phi=0.7, seed=41)
 
 
-@pytest.mark.parametrize("kind", ["langevin", "brownian", "spv"])
+@pytest.mark.parametrize("kind", ["langevin", "brownian", "spv", "fire"])
 def test_short_ranges_equal_step_by_step(gpu, kind):
     st = state(kind)
     steps = 160
@@ -61,6 +68,8 @@ def test_short_ranges_equal_step_by_step(gpu, kind):
     for k in KEYS:
         assert np.array_equal(ref.get(k), got.get(k)), k
     assert np.array_equal(ref.pairs(), got.pairs())
+    if kind == "fire":
+        assert ref.fire_state() == got.fire_state()
     ref.close()
     got.close()
 
