@@ -522,6 +522,10 @@ void loop_head_forces(samos_ctx* ctx, double* e_out, bool f32 = false) {
   interact(ctx, pm, am, 0, e_out, f32);
 }
 
+// The general pair-sum form (per-type tables, per-particle radii) instead of
+// the single-type form with loop-constant parameters.
+bool general_pairs(const samos_ctx* c) { return c->ntypes > 1 || c->soft_radii || c->lj_radii; }
+
 bool needs_nlist(const samos_ctx* c) {
   return c->has_nlist && (c->has_soft || c->has_lj || c->has_pol || c->has_nem || c->extra.count > 0);
 }
@@ -666,7 +670,7 @@ int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int rel, int t_off, i
     fa.chunk = chunk;
     fa.slot = slot;
     launch_baoab_fused(ctx->st, pot_mask(ctx), phase, cur, nxt, ctx->nl, ctx->box, ctx->con, interact_params(ctx),
-                       ctx->n, fa, f32 ? 1 : 0, planar ? 1 : 0, ctx->ntypes > 1 ? 1 : 0);
+                       ctx->n, fa, f32 ? 1 : 0, planar ? 1 : 0, general_pairs(ctx) ? 1 : 0);
   }
   if (phase != 2 && !chunk) {
     ProfScope ps(ctx, "rebuild");
@@ -854,7 +858,7 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
   std::vector<Integ*> grouped;
   if (!pair && !(ctx->integs.size() == 1 && I.gbit == 1u))
     for (auto& kv : ctx->integs) grouped.push_back(&kv.second);
-  const bool multi = ctx->ntypes > 1 || !grouped.empty();
+  const bool multi = general_pairs(ctx) || !grouped.empty();
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   const bool planar = ctx->con.kind == 1 && ctx->nl.use_cell;
   const int n = ctx->n;

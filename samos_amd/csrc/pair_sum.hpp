@@ -11,7 +11,9 @@
 // PLANAR: every particle lies on the same plane z = const (plane constraint),
 // so neighbour positions come from the interleaved (x, y) column and dz = 0
 // exactly, which leaves r^2 bit-identical to the 3-D expression.
-// MT: more than one particle type is present (per-type table lookups).
+// MT: the general form: per-type table lookups and per-particle radii
+// (use_particle_radii).  Without MT the tables' (1, 1) entries hold for every
+// pair and no radius is read.
 #pragma once
 #include "samos_device.hpp"
 
@@ -39,7 +41,7 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
                                                 double& fx, double& fy, double& fz, bool wrap_i,
                                                 double nxi = 0.0, double nyi = 0.0, double nzi = 0.0,
                                                 double* tq = nullptr, int lane = 0, int lanes = 1) {
-  const bool radii = ((POT & POT_SOFT) && ip.soft_radii) || ((POT & POT_LJ) && ip.lj_radii);
+  const bool radii = MT && (((POT & POT_SOFT) && ip.soft_radii) || ((POT & POT_LJ) && ip.lj_radii));
   // One particle type: the table entries of the pair (1, 1) are loop constants,
   // and so are the first products of each expression (sigma^2, rcut^2,
   // 48 epsilon), which round exactly as when formed per pair.
@@ -97,14 +99,15 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
       if (POT & POT_SOFT) {
         const int tt = MT ? ti * ip.soft_nt + tj[u] : 0;
         const double kk = MT ? ip.soft_tab[tt] : c_kk;
-        const double a = ip.soft_radii ? ri + rj[u] : (MT ? ip.soft_tab[nt2s + tt] : c_a);
+        const double a = (MT && ip.soft_radii) ? ri + rj[u] : (MT ? ip.soft_tab[nt2s + tt] : c_a);
         if (F32) {
-          const float r = sqrtf((float)r_sq);
+          // hardware square root and reciprocal (each within one float ulp)
+          const float r = __builtin_amdgcn_sqrtf((float)r_sq);
           if (r < (float)a) {
             const float diff = (float)a - r;
-            const float ff = (r > 0.0f) ? (float)kk * diff / r : (float)kk * diff;
-            fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy);
-            if (!PLANAR) fz -= (double)(ff * (float)dz);
+            const float ff = (r > 0.0f) ? (float)kk * diff * __builtin_amdgcn_rcpf(r) : (float)kk * diff;
+            fx = fma(-(double)ff, dx, fx); fy = fma(-(double)ff, dy, fy);
+            if (!PLANAR) fz = fma(-(double)ff, dz, fz);
           }
         } else {
           const double r = sqrt(r_sq);
@@ -124,14 +127,16 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
           const double eps = MT ? ip.lj_tab[tt] : c_eps;
           const double e48 = MT ? 48.0 * eps : c_e48;
           double sig2 = c_sig2;
-          if (ip.lj_radii) { const double sg = ri + rj[u]; sig2 = sg * sg; }
+          if (MT && ip.lj_radii) { const double sg = ri + rj[u]; sig2 = sg * sg; }
           else if (MT) { const double sg = ip.lj_tab[nt2l + tt]; sig2 = sg * sg; }
           if (F32) {
-            const float s2 = (float)sig2 / (float)r_sq;
+            // sigma^2 / r^2 from the hardware reciprocal (within one float ulp);
+            // the factor in FP32, each product accumulated in FP64
+            const float s2 = (float)sig2 * __builtin_amdgcn_rcpf((float)r_sq);
             const float s6 = s2 * s2 * s2;
             const float ff = 48.0f * (float)eps * s6 * (s6 - 0.5f) * s2;
-            fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy);
-            if (!PLANAR) fz -= (double)(ff * (float)dz);
+            fx = fma(-(double)ff, dx, fx); fy = fma(-(double)ff, dy, fy);
+            if (!PLANAR) fz = fma(-(double)ff, dz, fz);
           } else {
             const double inv_r_sq = lj_inv_r_sq(sig2, r_sq);
             const double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
