@@ -86,6 +86,7 @@ def load(fast=False):
         "orc_run": (I, [P, I]),
         "orc_philox": (None, [ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "orc_gauss3": (None, [u32, u32, u32, u64, dp]),
+        "orc_bm_eval": (None, [dp, I, dp]),
         "orc_set_rng_stream": (I, [P, I]),
         "orc_add_nve": (I, [P, D, D]),
         "orc_add_pair_extra": (I, [P, I, dp, I, I]),
@@ -315,6 +316,16 @@ def gauss3(seed, tag, pid, call):
     g = (ctypes.c_double * 3)()
     L.orc_gauss3(seed, tag, pid, call, g)
     return list(g)
+
+
+def bm_eval(u):
+    """(log u, sin 2 pi u, cos 2 pi u) as the noise transform evaluates them."""
+    L = load()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.empty(3 * len(u))
+    L.orc_bm_eval(u.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(u),
+                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out[0::3], out[1::3], out[2::3]
 
 
 def gsl_stream_sample(seed, n_uniform, n_gauss=0):

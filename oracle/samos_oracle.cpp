@@ -84,13 +84,62 @@ U4 philox4x32_10(U4 ctr, uint32_t k0, uint32_t k1) {
 // Uniform in (0,1): (u + 0.5) * 2^-32.
 inline double u01(uint32_t u) { return ((double)u + 0.5) * 2.3283064365386963e-10; }
 
+// Natural log of u in (0, 1): u = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// log m = 2 artanh(s), s = (m - 1) / (m + 1), summed as the series
+// 2 s (1 + s^2/3 + ... + s^20/21) (Abramowitz & Stegun 4.1.27), plus e ln 2
+// with ln 2 split into a 33-bit head and a tail.  Within about 1.5 ulp of
+// log(u) (pinned against the libm log in the tests); written with explicit
+// fma and no contraction so that the device's evaluation gives the same bits.
+inline double bm_log(double u) {
+  int e;
+  double m = std::frexp(u, &e);
+  if (m < 0x1.6a09e667f3bcdp-1) { m = 2.0 * m; e -= 1; }
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  static const double c[10] = {0x1.8618618618618p-5, 0x1.af286bca1af28p-5, 0x1.e1e1e1e1e1e1ep-5,
+                               0x1.1111111111111p-4, 0x1.3b13b13b13b14p-4, 0x1.745d1745d1746p-4,
+                               0x1.c71c71c71c71cp-4, 0x1.2492492492492p-3, 0x1.999999999999ap-3,
+                               0x1.5555555555555p-2};  // 1/21, 1/19, ..., 1/3
+  double p = c[0];
+  for (int k = 1; k < 10; ++k) p = std::fma(p, z, c[k]);
+  const double s2 = 2.0 * s;
+  const double lm = std::fma(s2, z * p, s2);
+  const double de = (double)e;
+  const double lo = std::fma(de, 0x1.473de6af278edp-34, lm);
+  return de * 0x1.62e42fef00000p-1 + lo;
+}
+
+// sin and cos of 2 pi u, u in (0, 1): t = 4u, q = rint(t), f = t - q, the
+// Taylor polynomials of sin(pi f / 2) and cos(pi f / 2) in f with
+// coefficients (pi/2)^k / k!, and the quadrant q mod 4.
+inline void bm_sincos_2pi(double u, double& sn, double& cs) {
+  const double t = 4.0 * u;
+  const double q = std::rint(t);
+  const double f = t - q;
+  const double z = f * f;
+  static const double cs_s[9] = {0x1.aaec32af93359p-38, -0x1.6fadb9f155744p-31, 0x1.e8f434d018d63p-25,
+                                 -0x1.e3074fde8871fp-19, 0x1.50783487ee782p-13, -0x1.32d2cce62bd86p-8,
+                                 0x1.466bc6775aae2p-4, -0x1.4abbce625be53p-1, 0x1.921fb54442d18p+0};
+  static const double cs_c[9] = {0x1.20c62c2f2d7f5p-34, -0x1.b6e24f44b128fp-28, 0x1.f9d38a3763cc3p-22,
+                                 -0x1.a6d1f2a204a8cp-16, 0x1.e1f506891babbp-11, -0x1.55d3c7e3cbffap-6,
+                                 0x1.03c1f081b5ac4p-2, -0x1.3bd3cc9be45dep+0, 1.0};
+  double ps = cs_s[0], pc = cs_c[0];
+  for (int k = 1; k < 9; ++k) { ps = std::fma(ps, z, cs_s[k]); pc = std::fma(pc, z, cs_c[k]); }
+  const double S = f * ps, C = pc;
+  const int k = (int)q & 3;
+  sn = (k == 0) ? S : (k == 1) ? C : (k == 2) ? -S : -C;
+  cs = (k == 0) ? C : (k == 1) ? -S : (k == 2) ? -C : S;
+}
+
 // Box-Muller pair from two 32-bit words.
 inline void box_muller(uint32_t a, uint32_t b, double& g0, double& g1) {
   double u1 = u01(a), u2 = u01(b);
-  double r = std::sqrt(-2.0 * std::log(u1));
-  double th = 6.283185307179586 * u2;
-  g0 = r * std::cos(th);
-  g1 = r * std::sin(th);
+  double r = std::sqrt(-2.0 * bm_log(u1));
+  double s, c;
+  bm_sincos_2pi(u2, s, c);
+  g0 = r * c;
+  g1 = r * s;
 }
 
 // Stream tags: one per integrator family, so that two integrators with the
@@ -1723,6 +1772,15 @@ void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
 
 void orc_gauss3(uint32_t seed, uint32_t tag, uint32_t pid, uint64_t call, double* g) {
   gauss3(seed, tag, pid, call, g);
+}
+
+// The Box-Muller transcendentals on n arguments in (0, 1): out[3i] = log u,
+// out[3i+1] = sin 2 pi u, out[3i+2] = cos 2 pi u.
+void orc_bm_eval(const double* u, int n, double* out) {
+  for (int i = 0; i < n; ++i) {
+    out[3 * i] = bm_log(u[i]);
+    bm_sincos_2pi(u[i], out[3 * i + 1], out[3 * i + 2]);
+  }
 }
 
 }  // extern "C"

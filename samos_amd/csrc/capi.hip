@@ -407,7 +407,9 @@ int alloc_nbr(samos_ctx* ctx, int stride) {
   hipFree(nl.nbr);
   nl.nbr = nullptr;
   nl.stride = stride;
-  HIPCHK(hipMalloc(&nl.nbr, sizeof(int) * (size_t)stride * (size_t)ctx->np + 64));
+  // 8 rows beyond the capacity: the builds pad each list to a whole number
+  // of 8-entry batches
+  HIPCHK(hipMalloc(&nl.nbr, sizeof(int) * (size_t)(stride + 8) * (size_t)ctx->np + 64));
   return 0;
 }
 
@@ -663,6 +665,10 @@ int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int rel, int t_off, i
     // BAOAB O-step constants (src/integrators/integrator_langevin.cpp:57-58)
     fa.exp_dt = std::exp(-I.gamma * I.dt);
     fa.Bc = I.temp.kind == SAMOS_VALUE_CONSTANT ? std::sqrt(I.temp.tmin * (1.0 - std::exp(-2.0 * I.gamma * I.dt))) : -1.0;
+    // the per-particle quotients of a common mass, formed once (IEEE division
+    // and square root round the same on both sides)
+    fa.fact_u = mass_u > 0.0 ? (0.5 * I.dt) / mass_u : 0.0;
+    fa.sf_u = (mass_u > 0.0 && fa.Bc >= 0.0) ? fa.Bc / std::sqrt(mass_u) : -1.0;
     fa.batch = ctx->gather_batch;
     // with a group other than "all" only members age: no common age then
     fa.age = I.gbit == 1u ? ctx->d_age : nullptr;
@@ -843,8 +849,11 @@ bool brownian_groups(const samos_ctx* ctx) {
   return v.size() == 1 || groups_disjoint(ctx, v);
 }
 
+// The fused step kernels gather with 32-bit byte offsets (16 np < 2^32).
+bool fused_size_ok(const samos_ctx* ctx) { return ctx->np < (1 << 28); }
+
 bool fused_brownian_eligible(samos_ctx* ctx, int flags) {
-  if ((flags & SAMOS_RUN_FAITHFUL) || ctx->extra.count != 0) return false;
+  if ((flags & SAMOS_RUN_FAITHFUL) || ctx->extra.count != 0 || !fused_size_ok(ctx)) return false;
   if (!needs_nlist(ctx) || !brownian_fused_supported(pot_mask(ctx), al_mask(ctx))) return false;
   if (brownian_groups(ctx)) return true;
   return ctx->integs.size() == 2 && ctx->integs.count("brownian_align_all") && ctx->integs.count("brownian_pos_all");
@@ -1070,6 +1079,8 @@ void tile_fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int t_off, int 
   // BAOAB O-step constants (src/integrators/integrator_langevin.cpp:57-58)
   fa.exp_dt = std::exp(-I.gamma * I.dt);
   fa.Bc = I.temp.kind == SAMOS_VALUE_CONSTANT ? std::sqrt(I.temp.tmin * (1.0 - std::exp(-2.0 * I.gamma * I.dt))) : -1.0;
+  fa.fact_u = (0.5 * I.dt) / mass_u;  // the tiled engine runs with a common mass only
+  fa.sf_u = fa.Bc >= 0.0 ? fa.Bc / std::sqrt(mass_u) : -1.0;
   fa.batch = 4;
   fa.age = ctx->d_age;
   const Fields f = fields(ctx);
@@ -1872,7 +1883,8 @@ int run_range_core(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags)
   // the previous step on the same positions, so it is done once.
   Integ* only = ctx->integs.size() == 1 ? &ctx->integs.begin()->second : nullptr;
   const bool fused = !(flags & SAMOS_RUN_FAITHFUL) && ctx->extra.count == 0 && only && only->name == "langevin" &&
-                     only->method == 0 && al_mask(ctx) == 0 && pot_mask(ctx) != 0 && needs_nlist(ctx);
+                     only->method == 0 && al_mask(ctx) == 0 && pot_mask(ctx) != 0 && needs_nlist(ctx) &&
+                     fused_size_ok(ctx);
   const bool fused_b = !fused && fused_brownian_eligible(ctx, flags);
   if (t_begin == 0 && !ctx->run_begun) {
     // Loop head of t = 0 (src/samos.cpp:887-892): enforce, reset, forces and

@@ -74,9 +74,12 @@ __device__ __forceinline__ void surf_normal(const ConstraintD& c, const PS& p, d
 }
 
 // Constraint::enforce of the one active constraint; returns true when the
-// plane constraint zeroed the particle's f_z.
+// plane constraint zeroed the particle's f_z.  plane_dirs = false leaves out
+// the plane's director normalisation, for a caller that has already
+// normalised the same director in this step (on a plane the normal is fixed,
+// so a second normalisation of a unit director returns it to within an ulp).
 template <int SET = kSurfAll>
-__device__ __forceinline__ bool enforce(const BoxD& b, const ConstraintD& c, PS& p) {
+__device__ __forceinline__ bool enforce(const BoxD& b, const ConstraintD& c, PS& p, bool plane_dirs = true) {
   if ((SET & 0x01) && c.kind == 0) {
     enforce_periodic(b, p);
   } else if ((SET & 0x02) && c.kind == 1) {
@@ -88,10 +91,12 @@ __device__ __forceinline__ bool enforce(const BoxD& b, const ConstraintD& c, PS&
       if (p.y < ylo) { p.y = ylo; p.vy = -p.vy; } else if (p.y > yhi) { p.y = yhi; p.vy = -p.vy; }
     }
     p.nz = 0.0;
-    const double len_n = sqrt(p.nx * p.nx + p.ny * p.ny);
-    double inv_len = 1.0;
-    if (len_n != 0.0) inv_len = 1.0 / sqrt(p.nx * p.nx + p.ny * p.ny);
-    p.nx *= inv_len; p.ny *= inv_len;
+    if (plane_dirs) {
+      const double len_n = sqrt(p.nx * p.nx + p.ny * p.ny);
+      double inv_len = 1.0;
+      if (len_n != 0.0) inv_len = 1.0 / sqrt(p.nx * p.nx + p.ny * p.ny);
+      p.nx *= inv_len; p.ny *= inv_len;
+    }
     return true;
   } else if ((SET & 0x04) && c.kind == 2) {
     const double R0 = c.r;

@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
     // streamed.
     const double dt2 = 0.5 * dt;
     const double m = mass_u > 0.0 ? mass_u : cur.mass[s];
-    const double fact = dt2 / m;
+    const double fact = mass_u > 0.0 ? fa.fact_u : dt2 / m;
     // outside the integrator's group a slot is not integrated; it only takes
     // the next loop head's enforce
     const bool member = in_group(cur, s, fa.gbit);
@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
         fz0 = enforce<kSet>(b, c, p);
         p.vx *= exp_dt; p.vy *= exp_dt; p.vz *= exp_dt;
         if (B != 0.0) {
-          const double sf = B / sqrt(m);
+          const double sf = fa.sf_u >= 0.0 ? fa.sf_u : B / sqrt(m);
           const unsigned long long call = eff_call(clk, slot, 0ULL) + (unsigned long long)t_off;
           double g[3];
           if (PLANAR) {  // the z draw only reaches v_z, which the next enforce resets
@@ -468,9 +468,10 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
           p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
         }
         p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
-        fz0 |= enforce<kSet>(b, c, p);
+        // on a plane the director was normalised by the first enforce
+        fz0 |= enforce<kSet>(b, c, p, !PLANAR);
       }
-      if (head_enforce) fz0 |= enforce<kSet>(b, c, p);
+      if (head_enforce) fz0 |= enforce<kSet>(b, c, p, !PLANAR || !member);
       if (PLANAR) {
         nxt.xy[s] = make_double2(p.x, p.y);
       } else {

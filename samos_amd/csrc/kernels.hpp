@@ -105,6 +105,8 @@ struct FusedArgs {
   double mass_u;   // common mass, or -1 to read each particle's
   double exp_dt;   // exp(-gamma dt)
   double Bc;       // O-step noise amplitude at constant temperature, or -1
+  double fact_u;   // (dt / 2) / mass_u when the mass is common
+  double sf_u;     // Bc / sqrt(mass_u) when both are known, or -1
   int batch;       // neighbour gathers in flight per thread (4 or 8)
   const AgeCtl* age = nullptr;  // common-age state (null or not uniform: each slot adds dt)
   ChunkCtl* chunk = nullptr;  // captured-chunk mode (null: direct launch with the parity flags)

@@ -335,6 +335,9 @@ __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
     if (k > nl.stride) overflow = true;
     kmax = max(kmax, k);
     nl.nnb[s] = min(k, nl.stride);
+    // padding to a multiple of 8 entries with the own index (the list has 8
+    // rows beyond its capacity), read by the batched pair loops, never summed
+    for (int q = min(k, nl.stride); q & 7; ++q) nl.nbr[(long)q * nl.np + s] = s;
     f.coord[s] = coord;
     nl.ox[s] = xi; nl.oy[s] = yi; nl.oz[s] = zi;
   }
@@ -421,6 +424,9 @@ __global__ void k_build_nsq(Fields f, NListD nl, BoxD b, int n, const int* req) 
     if (k > nl.stride) overflow = true;
     kmax = max(kmax, k);
     nl.nnb[s] = min(k, nl.stride);
+    // padding to a multiple of 8 entries with the own index (the list has 8
+    // rows beyond its capacity), read by the batched pair loops, never summed
+    for (int q = min(k, nl.stride); q & 7; ++q) nl.nbr[(long)q * nl.np + s] = s;
     f.coord[s] = coord;
     nl.ox[s] = xi; nl.oy[s] = yi; nl.oz[s] = zi;
   }

@@ -19,6 +19,12 @@
 
 namespace samos {
 
+// base[byte_off / sizeof(T)] with a 32-bit byte offset.
+template <typename T>
+__device__ __forceinline__ T at32(const T* base, unsigned byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
 // wrap_i: apply the minimum image to each pair.  A slot farther than
 // 2 (rcut + pad) from every periodic face has all its listed neighbours
 // (within rcut + pad at the build, each moved by less than pad / 2 since)
@@ -62,28 +68,41 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
   }
   double tx = 0.0, ty = 0.0, tz = 0.0;
   const int kn = nl.nnb[s];
-  const int* nb = nl.nbr + s;
-  for (int k0 = lane * kBatch; k0 < kn; k0 += lanes * kBatch) {
-    int t[kBatch];
+  // A list is padded with the slot's own index to a multiple of 8 entries
+  // (the builds write the padding), so whole batches load without bounds
+  // tests; the arithmetic below skips the padding.  The next batch's indices
+  // are loaded while the current batch is gathered and summed.  Gather offsets are 32-bit byte offsets from the column bases (np < 2^28,
+  // which the host checks before choosing these kernels), so each gather
+  // takes its address from a scalar base and one vector register.
+  const int step = lanes * kBatch;
+  const unsigned s4 = (unsigned)s << 2;
+  unsigned t[kBatch];  // neighbour slots
+  if (lane * kBatch < kn) {
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) t[u] = (k0 + u < kn) ? nb[(long)(k0 + u) * nl.np] : s;
+    for (int u = 0; u < kBatch; ++u) t[u] = (unsigned)at32(nl.nbr + (long)(lane * kBatch + u) * nl.np, s4);
+  }
+  for (int k0 = lane * kBatch; k0 < kn; k0 += step) {
     double xj[kBatch], yj[kBatch], zj[kBatch], rj[kBatch];
     double nxj[kBatch], nyj[kBatch], nzj[kBatch];
     int tj[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       if (PLANAR) {
-        const double2 p = f.xy[t[u]];
+        const double2 p = at32(f.xy, t[u] << 4);
         xj[u] = p.x; yj[u] = p.y; zj[u] = zi;
       } else {
-        xj[u] = f.x[t[u]]; yj[u] = f.y[t[u]]; zj[u] = f.z[t[u]];
+        xj[u] = at32(f.x, t[u] << 3); yj[u] = at32(f.y, t[u] << 3); zj[u] = at32(f.z, t[u] << 3);
       }
-      tj[u] = MT ? f.type[t[u]] - 1 : 0;
+      tj[u] = MT ? at32(f.type, t[u] << 2) - 1 : 0;
       if (AL) {
-        nxj[u] = f.nx[t[u]]; nyj[u] = f.ny[t[u]];
-        nzj[u] = PLANAR ? 0.0 : f.nz[t[u]];
+        nxj[u] = at32(f.nx, t[u] << 3); nyj[u] = at32(f.ny, t[u] << 3);
+        nzj[u] = PLANAR ? 0.0 : at32(f.nz, t[u] << 3);
       }
-      rj[u] = radii ? f.radius[t[u]] : 0.0;
+      rj[u] = radii ? at32(f.radius, t[u] << 3) : 0.0;
+    }
+    if (k0 + step < kn) {
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) t[u] = (unsigned)at32(nl.nbr + (long)(k0 + step + u) * nl.np, s4);
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {

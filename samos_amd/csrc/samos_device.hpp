@@ -103,7 +103,7 @@ struct NListD {
   int* start;         // [ncell + 1] exclusive scan of count
   int* members;       // [n] slots grouped by cell, ascending slot within a cell
   int* bsum;          // [scan blocks] per-block sums for the scan
-  int* nbr;           // [stride * np]
+  int* nbr;           // [(stride + 8) * np]: entry k of slot s at k * np + s
   int* nnb;           // [np]
   double *ox, *oy, *oz;  // positions at the last build
   int* flags;         // [0],[1] rebuild request by step parity, [2] overflow, [3] largest cell
@@ -224,18 +224,18 @@ __device__ __forceinline__ int eff_step(const ClockD* clk, int t_off) { return c
 struct U4 { uint32_t a, b, c, d; };
 
 __device__ __forceinline__ U4 philox4x32_10(U4 ctr, uint32_t k0, uint32_t k1) {
-  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  // the round's two 32x32 -> 64-bit products, each one v_mad_u64_u32
+  const uint64_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
   const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     if (r > 0) { k0 += W0; k1 += W1; }
-    uint32_t hi0 = __umulhi(M0, ctr.a), lo0 = M0 * ctr.a;
-    uint32_t hi1 = __umulhi(M1, ctr.c), lo1 = M1 * ctr.c;
+    const uint64_t p0 = M0 * ctr.a, p1 = M1 * ctr.c;
     U4 n;
-    n.a = hi1 ^ ctr.b ^ k0;
-    n.b = lo1;
-    n.c = hi0 ^ ctr.d ^ k1;
-    n.d = lo0;
+    n.a = (uint32_t)(p1 >> 32) ^ ctr.b ^ k0;
+    n.b = (uint32_t)p1;
+    n.c = (uint32_t)(p0 >> 32) ^ ctr.d ^ k1;
+    n.d = (uint32_t)p0;
     ctr = n;
   }
   return ctr;
@@ -243,12 +243,78 @@ __device__ __forceinline__ U4 philox4x32_10(U4 ctr, uint32_t k0, uint32_t k1) {
 
 __device__ __forceinline__ double u01(uint32_t u) { return ((double)u + 0.5) * 2.3283064365386963e-10; }
 
+// Natural log of u in (0, 1) (normal, non-zero), within about 1.5 ulp:
+// u = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 artanh(s) with
+// s = (m - 1) / (m + 1), |s| <= 0.1716, summed as the series
+// 2 s (1 + s^2/3 + s^4/5 + ... + s^20/21) (Abramowitz & Stegun 4.1.27;
+// the next term is below 2^-55 relative), and e ln 2 with ln 2 split into a
+// 33-bit head (e ln2_hi exact) and a tail.  Every operation is IEEE (division
+// included) or an explicit fma, so a C compilation of the same expression
+// without contraction returns the same bits.
+__device__ __forceinline__ double bm_log(double u) {
+  int e;
+  double m = frexp(u, &e);
+  if (m < 0x1.6a09e667f3bcdp-1) { m = 2.0 * m; e -= 1; }
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  double p = 0x1.8618618618618p-5;            // 1/21
+  p = fma(p, z, 0x1.af286bca1af28p-5);        // 1/19
+  p = fma(p, z, 0x1.e1e1e1e1e1e1ep-5);        // 1/17
+  p = fma(p, z, 0x1.1111111111111p-4);        // 1/15
+  p = fma(p, z, 0x1.3b13b13b13b14p-4);        // 1/13
+  p = fma(p, z, 0x1.745d1745d1746p-4);        // 1/11
+  p = fma(p, z, 0x1.c71c71c71c71cp-4);        // 1/9
+  p = fma(p, z, 0x1.2492492492492p-3);        // 1/7
+  p = fma(p, z, 0x1.999999999999ap-3);        // 1/5
+  p = fma(p, z, 0x1.5555555555555p-2);        // 1/3
+  const double s2 = 2.0 * s;
+  const double lm = fma(s2, z * p, s2);
+  const double de = (double)e;
+  const double lo = fma(de, 0x1.473de6af278edp-34, lm);  // ln2 - ln2_hi
+  return de * 0x1.62e42fef00000p-1 + lo;                // e ln2_hi is exact
+}
+
+// sin and cos of 2 pi u for u in (0, 1), within about 1.5 ulp: t = 4u,
+// q = rint(t), f = t - q in [-1/2, 1/2] (both exact), then the Taylor
+// polynomials of sin(pi f / 2) (to f^17) and cos(pi f / 2) (to f^16), whose
+// coefficients (pi/2)^k / k! are rounded from exact values; truncation is
+// below 2^-60.  The quadrant q mod 4 swaps and negates them.
+__device__ __forceinline__ void bm_sincos_2pi(double u, double& sn, double& cs) {
+  const double t = 4.0 * u;
+  const double q = rint(t);
+  const double f = t - q;
+  const double z = f * f;
+  double ps = 0x1.aaec32af93359p-38;
+  ps = fma(ps, z, -0x1.6fadb9f155744p-31);
+  ps = fma(ps, z, 0x1.e8f434d018d63p-25);
+  ps = fma(ps, z, -0x1.e3074fde8871fp-19);
+  ps = fma(ps, z, 0x1.50783487ee782p-13);
+  ps = fma(ps, z, -0x1.32d2cce62bd86p-8);
+  ps = fma(ps, z, 0x1.466bc6775aae2p-4);
+  ps = fma(ps, z, -0x1.4abbce625be53p-1);
+  ps = fma(ps, z, 0x1.921fb54442d18p+0);
+  double pc = 0x1.20c62c2f2d7f5p-34;
+  pc = fma(pc, z, -0x1.b6e24f44b128fp-28);
+  pc = fma(pc, z, 0x1.f9d38a3763cc3p-22);
+  pc = fma(pc, z, -0x1.a6d1f2a204a8cp-16);
+  pc = fma(pc, z, 0x1.e1f506891babbp-11);
+  pc = fma(pc, z, -0x1.55d3c7e3cbffap-6);
+  pc = fma(pc, z, 0x1.03c1f081b5ac4p-2);
+  pc = fma(pc, z, -0x1.3bd3cc9be45dep+0);
+  pc = fma(pc, z, 1.0);
+  const double S = f * ps, C = pc;
+  const int k = (int)q & 3;
+  sn = (k == 0) ? S : (k == 1) ? C : (k == 2) ? -S : -C;
+  cs = (k == 0) ? C : (k == 1) ? -S : (k == 2) ? -C : S;
+}
+
+// Box-Muller pair from two 32-bit words (both uniforms in (0, 1)).
 __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& g0, double& g1) {
-  double u1 = u01(a), u2 = u01(b);
-  double r = sqrt(-2.0 * log(u1));
-  double th = 6.283185307179586 * u2;
+  const double u1 = u01(a), u2 = u01(b);
+  const double r = sqrt(-2.0 * bm_log(u1));
   double s, c;
-  sincos(th, &s, &c);
+  bm_sincos_2pi(u2, s, c);
   g0 = r * c;
   g1 = r * s;
 }

@@ -516,7 +516,7 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_step(TileD td, SlotOut so
     const int row1 = g.nc + 2;
     const int kb = s_hcs[row1 + 1], ke = s_hcs[row1 + g.nc + 1];
     const int qa = s_hgs[row1 + 1];
-    const double dt = fa.dt, dt2 = 0.5 * dt, m = fa.mass_u, fact = dt2 / m;
+    const double dt = fa.dt, dt2 = 0.5 * dt, m = fa.mass_u, fact = fa.fact_u;
     for (int k = kb + (int)threadIdx.x; k < ke; k += kTileThreads) {
       const int q = qa + (k - kb);
       unsigned long long mw[4];
@@ -567,12 +567,13 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_step(TileD td, SlotOut so
         enforce<kSurfPlane>(b, c, p);
         p.vx *= fa.exp_dt; p.vy *= fa.exp_dt; p.vz *= fa.exp_dt;
         if (B != 0.0) {
-          const double sfac = B / sqrt(m);
+          const double sfac = fa.sf_u >= 0.0 ? fa.sf_u : B / sqrt(m);
           p.vx += sfac * gn[0]; p.vy += sfac * gn[1]; p.vz += sfac * 0.0;
         }
         p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
-        enforce<kSurfPlane>(b, c, p);
-        if (fa.head_enforce) enforce<kSurfPlane>(b, c, p);
+        // the director was normalised by the first enforce of the step
+        enforce<kSurfPlane>(b, c, p, false);
+        if (fa.head_enforce) enforce<kSurfPlane>(b, c, p, false);
         A.xy[par ^ 1][q] = make_double2(p.x, p.y);
         A.v[q] = make_double2(p.vx, p.vy);
         if (p.nx != n0.x || p.ny != n0.y) A.nd[q] = make_double2(p.nx, p.ny);

@@ -58,6 +58,36 @@ def test_gaussian_moments():
     assert abs(np.mean(g ** 4) - 3.0) < 0.15
 
 
+def test_box_muller_transcendentals_pinned_to_libm():
+    """The noise transform's own log and sin/cos of 2 pi u (the device
+    evaluates the same expression) against the x87 extended-precision libm:
+    log within 2 ulp, sin and cos within 2 ulp where |value| > 1/64 and within
+    2^-53 absolute everywhere (the reduction is exact, so values near the
+    zeros keep their relative accuracy too)."""
+    rng = np.random.default_rng(11)
+    words = np.concatenate([rng.integers(0, 2 ** 32, 400000), [0, 1, 2 ** 31, 2 ** 32 - 1],
+                            np.arange(2 ** 29 - 40, 2 ** 29 + 40), np.arange(3 * 2 ** 29 - 40, 3 * 2 ** 29 + 40),
+                            np.arange(2 ** 31 - 40, 2 ** 31 + 40), np.arange(int(0.70710678 * 2 ** 32) - 400,
+                                                                             int(0.70710678 * 2 ** 32) + 400)])
+    u = (words.astype(np.float64) + 0.5) * 2.0 ** -32
+    lg, sn, cs = oracle.bm_eval(u)
+    ul = u.astype(np.longdouble)
+    two_pi = np.longdouble("6.283185307179586476925286766559005768394")
+    ref_log = np.log(ul)
+    ref_sin, ref_cos = np.sin(two_pi * ul), np.cos(two_pi * ul)
+
+    def ulps(got, ref):
+        return np.abs(got.astype(np.longdouble) - ref) / np.spacing(np.abs(ref.astype(np.float64)))
+
+    assert ulps(lg, ref_log).max() <= 2.0
+    for got, ref in ((sn, ref_sin), (cs, ref_cos)):
+        big = np.abs(ref) > 1.0 / 64
+        assert ulps(got[big], ref[big]).max() <= 2.0
+        assert float(np.abs(got.astype(np.longdouble) - ref).max()) <= 2.0 ** -52
+    # quadrant signs and the exact reduction: u and 1/2 + u, 1/4 + u
+    assert np.array_equal(oracle.bm_eval([0.5 + 2.0 ** -33])[1], -oracle.bm_eval([2.0 ** -33])[1])
+
+
 # ---------------------------------------------------------- pair sets ----
 @pytest.mark.parametrize("seed", [0, 1, 2])
 @pytest.mark.parametrize("periodic", [True, False])
