@@ -436,7 +436,12 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
       }
       PS p;
       p.x = xi; p.y = yi; p.z = zi; p.vx = vx; p.vy = vy; p.vz = vz;
-      const double nx0 = cur.nx[s], ny0 = cur.ny[s], nz0 = PLANAR ? 0.0 : cur.nz[s];
+      // On a plane only the enforce's normalisation changes a director, and
+      // the first step of every range (phase 0) has normalised them all; the
+      // later steps of the range neither read nor write them.
+      constexpr bool kDirs = !(PLANAR && PHASE == 1);
+      const double nx0 = kDirs ? cur.nx[s] : 0.0, ny0 = kDirs ? cur.ny[s] : 0.0;
+      const double nz0 = PLANAR ? 0.0 : cur.nz[s];
       p.nx = nx0; p.ny = ny0; p.nz = nz0;
       p.dix = p.diy = p.diz = 0;
       constexpr int kSet = PLANAR ? kSurfPlane : kSurfAll;
@@ -453,7 +458,7 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
         }
         p.vx += fact * fx; p.vy += fact * fy; p.vz += fact * fz;
         p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
-        fz0 = enforce<kSet>(b, c, p);
+        fz0 = enforce<kSet>(b, c, p, kDirs);
         p.vx *= exp_dt; p.vy *= exp_dt; p.vz *= exp_dt;
         if (B != 0.0) {
           const double sf = fa.sf_u >= 0.0 ? fa.sf_u : B / sqrt(m);
@@ -471,7 +476,7 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
         // on a plane the director was normalised by the first enforce
         fz0 |= enforce<kSet>(b, c, p, !PLANAR);
       }
-      if (head_enforce) fz0 |= enforce<kSet>(b, c, p, !PLANAR || !member);
+      if (head_enforce) fz0 |= enforce<kSet>(b, c, p, !PLANAR || (kDirs && !member));
       if (PLANAR) {
         nxt.xy[s] = make_double2(p.x, p.y);
       } else {
@@ -480,7 +485,7 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
       cur.vx[s] = p.vx; cur.vy[s] = p.vy;
       if (!PLANAR) cur.vz[s] = p.vz;
       // the enforce renormalises the director; store only what changed
-      if (p.nx != nx0 || p.ny != ny0) { cur.nx[s] = p.nx; cur.ny[s] = p.ny; }
+      if (kDirs && (p.nx != nx0 || p.ny != ny0)) { cur.nx[s] = p.nx; cur.ny[s] = p.ny; }
       if (!PLANAR && p.nz != nz0) cur.nz[s] = p.nz;
       if (p.dix) cur.ix[s] += p.dix;
       if (p.diy) cur.iy[s] += p.diy;
