@@ -331,11 +331,13 @@ int chunk_graphs(samos_ctx* ctx, const std::string& base, const std::function<in
 // least its first slot, and a slot whose positions need a rebuild stops its
 // chunk).  A stretch longer than the largest chunk is covered by a batch of
 // largest chunks, as many as the slots per chunk seen so far suggest; a
-// shorter remainder by chunks whose sizes add up to it exactly (one spare
-// slot for an odd count), so no slot is launched only to return.  The host
-// reads the base after each batch.  Returns the base reached in *t, and 1
+// shorter remainder by the largest chunks that fit while more than 8 slots
+// are left and then by one chunk that covers the rest.  The host reads the
+// base after each batch.  Returns the base reached in *t, and 1
 // when the batch stopped at a neighbour-list overflow.
-int replay_chunks(samos_ctx* ctx, const ChunkSet& set, int end, int* t) {
+// stall: rounds that may end without progress (a request left before the
+// chunks stops their first slot; the chunk's rebuild serves it).
+int replay_chunks(samos_ctx* ctx, const ChunkSet& set, int end, int* t, int stall = 0) {
   const int Mmax = set.back().first;
   double per = Mmax;
   int* h_base = reinterpret_cast<int*>(ctx->h_out);
@@ -346,22 +348,33 @@ int replay_chunks(samos_ctx* ctx, const ChunkSet& set, int end, int* t) {
       k = std::max(1, std::min(256, (int)(r / per)));
       for (int i = 0; i < k; ++i) HIPCHK(hipGraphLaunch(set.back().second, ctx->st));
     } else {
+      // largest chunks while more than 8 slots are left, then the smallest
+      // chunk that covers the rest: a chunk boundary (5 dispatches) costs more
+      // than the up to 7 slots that return at once past the range's end
       for (int left = r; left > 0;) {
-        auto it = set.rbegin();
-        while (it != set.rend() && it->first > left) ++it;
-        const auto& g = it != set.rend() ? *it : set.front();
-        HIPCHK(hipGraphLaunch(g.second, ctx->st));
-        left -= g.first;
+        const std::pair<int, hipGraphExec_t>* g = nullptr;
+        if (left <= 8) {
+          for (const auto& e : set)
+            if (e.first >= left) { g = &e; break; }
+        }
+        if (!g) {
+          auto it = set.rbegin();
+          while (it != set.rend() && it->first > left) ++it;
+          g = it != set.rend() ? &*it : &set.front();
+        }
+        HIPCHK(hipGraphLaunch(g->second, ctx->st));
+        left -= g->first;
       }
     }
-    HIPCHK(hipMemcpyAsync(h_base, &ctx->d_chunk->base, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
-    HIPCHK(hipMemcpyAsync(h_base + 1, ctx->d_flags + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    // the next run step and the list's overflow flag, adjacent in ChunkCtl
+    HIPCHK(hipMemcpyAsync(h_base, &ctx->d_chunk->base, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     const int tn = h_base[0];
     if (h_base[1]) {  // a rebuild overflowed: the slots after it stopped
       *t = tn;
       return 1;
     }
+    if (tn == *t && stall > 0) { --stall; continue; }
     if (tn <= *t || tn > end) return fail(ctx, "captured step chunks made no progress");
     if (k) per = std::max(1.0, (double)(tn - *t) / k);
     *t = tn;
@@ -650,8 +663,13 @@ int upload_clock(samos_ctx* ctx, int run_step) {
 // (coop), else the stage kernels (bin, scan, scatter, sort, build).  In a
 // chunk (chunk != null) the step is slot `slot` and the chunk's own rebuild
 // follows the slots.
+// rebuild = false leaves the step's rebuild request in the parity flag for the
+// captured chunks that follow to serve.
+// call0 >= 0: a launch outside the chunks, which takes noise call call0 and
+// run step t directly instead of the device clock.
 int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int rel, int t_off, int steps, bool f32, bool planar,
-               double mass_u, bool coop, ChunkCtl* chunk = nullptr, int slot = 0) {
+               double mass_u, bool coop, ChunkCtl* chunk = nullptr, int slot = 0, bool rebuild = true,
+               long long call0 = -1) {
   const int par = rel & 1;
   Fields cur = fields(ctx, par), nxt = fields(ctx, par ^ 1);
   if (planar) { cur.xy = ctx->xy[par]; nxt.xy = ctx->xy[par ^ 1]; }
@@ -660,6 +678,7 @@ int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int rel, int t_off, i
     FusedArgs fa;
     fa.dt = I.dt; fa.gamma = I.gamma; fa.temp = I.temp; fa.seed = I.seed;
     fa.clock = ctx->d_clock; fa.clock_slot = I.slot; fa.t_off = t_off; fa.parity = par;
+    if (call0 >= 0) { fa.clock = nullptr; fa.call0 = (unsigned long long)call0; fa.t_off = t; }
     fa.head_enforce = t + 1 <= steps ? 1 : 0;
     fa.mass_u = mass_u;
     // BAOAB O-step constants (src/integrators/integrator_langevin.cpp:57-58)
@@ -675,10 +694,11 @@ int fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int rel, int t_off, i
     fa.gbit = I.gbit;
     fa.chunk = chunk;
     fa.slot = slot;
+    if (phase == 2 && planar) { fa.out_x = ctx->dcol[0]; fa.out_y = ctx->dcol[1]; }
     launch_baoab_fused(ctx->st, pot_mask(ctx), phase, cur, nxt, ctx->nl, ctx->box, ctx->con, interact_params(ctx),
                        ctx->n, fa, f32 ? 1 : 0, planar ? 1 : 0, general_pairs(ctx) ? 1 : 0);
   }
-  if (phase != 2 && !chunk) {
+  if (phase != 2 && !chunk && rebuild) {
     ProfScope ps(ctx, "rebuild");
     if (coop && ctx->coop != 0) {
       hipError_t e = launch_rebuild_coop(ctx->st, nxt, ctx->nl, ctx->box, ctx->n, ctx->d_flags + par, planar ? 1 : 0);
@@ -713,7 +733,9 @@ int uniform_mass(samos_ctx* ctx, double* out) {
 bool tile_eligible(samos_ctx* ctx, int flags, double mass_u);
 int run_tiled_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int flags, double mass_u);
 
-int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int flags) {
+// packed: the interleaved (x, y) column of buffer 1 already holds the
+// positions (written by the run's first enforce).
+int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int flags, bool packed) {
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   double mass_u = -1.0;
   if (int r = uniform_mass(ctx, &mass_u)) return r;
@@ -727,36 +749,46 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     const int r = run_tiled_langevin(ctx, I, a, b, steps, flags, mass_u);
     if (r <= 0) return r;
   }
-  if (int r = upload_clock(ctx, a)) return r;
   HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   // Planar layout: the plane constraint puts every particle on z = zpos, so
   // neighbour gathers read the interleaved (x, y) column.  The N^2 build reads
   // the x, y, z columns, so lists without a cell grid keep the 3-D layout.
   const bool planar = ctx->con.kind == 1 && ctx->nl.use_cell;
+  const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
+  const bool chunks = graph && b - (a + 1) >= 2;
+  // x_a is read from buffer r0: planar runs keep it in the interleaved
+  // column of buffer 1, so that x_{a+1} lands in buffer 0, where captured
+  // chunks start
+  const int r0 = planar ? 1 : 0;
   // the loop-head enforce of iteration a was applied by the run's begin or by
   // the last kernel of the previous range
   if (planar) {
-    Fields f0 = fields(ctx, 0);
-    f0.xy = ctx->xy[0];
-    launch_pack_xy(ctx->st, f0, ctx->n);
-    // the binning of either buffer reads z (= zpos after the enforce)
-    HIPCHK(hipMemcpyAsync(ctx->alt[2], ctx->dcol[2], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
+    Fields f1 = fields(ctx, 0);
+    f1.xy = ctx->xy[1];
+    if (!packed) launch_pack_xy(ctx->st, f1, ctx->n);
+    // the direct steps' rebuilds bin either buffer and read z (= zpos after
+    // the enforce); chunks bin buffer 0 only
+    if (!chunks)
+      HIPCHK(hipMemcpyAsync(ctx->alt[2], ctx->dcol[2], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
   }
   auto at_buffer = [&](int par) {
     Fields f = fields(ctx, par & 1);
     if (planar) f.xy = ctx->xy[par & 1];
     return f;
   };
-  const bool graph = ctx->coop == 0 && !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
-  const bool chunks = graph && b - (a + 1) >= 2;
-  if (int r = fused_step(ctx, I, 0, a, 0, 0, steps, f32, planar, mass_u, true)) return r;
-  // a list overflow at this rebuild is caught by the chunks (their slots stop
-  // at it) or here, before the next step reads the list
+  // Launches outside the chunks take their noise call and run step from the
+  // host: iteration t draws call c0 + (t - a).
+  const unsigned long long c0 = I.calls;
+  auto call_of = [&](int t) { return (long long)(c0 + (unsigned long long)(t - a)); };
+  // With chunks to follow, this step's rebuild request is served by the first
+  // chunk (its slot 0 stops at once, the chunk's rebuild runs); otherwise the
+  // rebuild runs here and an overflow of it is caught before the next step.
+  if (int r = fused_step(ctx, I, 0, a, r0, 0, steps, f32, planar, mass_u, true, nullptr, 0, !chunks, call_of(a)))
+    return r;
   if (!chunks)
-    if (int r = recover_overflow(ctx, at_buffer(1), planar); r < 0) return r;
-  launch_clock_advance(ctx->st, ctx->d_clock, 1);
+    if (int r = recover_overflow(ctx, at_buffer(r0 ^ 1), planar); r < 0) return r;
   int t = a + 1;
-  int buf = 1;  // buffer holding x_t
+  int buf = r0 ^ 1;  // buffer holding x_t
   // Captured chunks: M fused steps ("slots") followed by one device-conditional
   // rebuild.  A slot whose new positions need a rebuild stops the chunk: the
   // later slots return at once, the positions settle into buffer 0, the
@@ -765,7 +797,9 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   // range; short ranges use a smaller power-of-two chunk.  The rebuild
   // decisions and the steps are exactly those of one launch per step; only
   // the skipped-rebuild dispatches between steps are gone.  Slots cover the
-  // iterations up to b - 1.  Instantiated chunks are kept per context.
+  // iterations up to b - 1 and read the noise call and run step from the
+  // device clock, which each chunk's bookkeeping advances.  Instantiated
+  // chunks are kept per context.
   if (chunks) {
     Fields f0 = at_buffer(0), f1 = at_buffer(1);
     auto record = [&](int m) -> int {
@@ -773,19 +807,15 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
       // device from the run's last iteration
       for (int j = 0; j < m; ++j)
         if (int r = fused_step(ctx, I, 1, b - 2, j, j, steps, f32, planar, mass_u, false, ctx->d_chunk, j)) return r;
-      launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, ctx->n, planar ? 1 : 0, 0);
-      launch_rebuild(ctx->st, f0, ctx->nl, ctx->box, ctx->n, &ctx->d_chunk->rebuild, planar ? 1 : 0);
-      launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
+      launch_chunk_rebuild(ctx->st, ctx->d_chunk, ctx->d_clock, f0, f1, ctx->nl, ctx->box, ctx->n, planar ? 1 : 0, 0);
       return 0;
     };
     // slots start at buffer parity 0
-    if (planar) {
-      HIPCHK(hipMemcpyAsync(ctx->xy[0], ctx->xy[1], sizeof(double2) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
-    } else {
+    if (buf == 1)
       for (int k = 0; k < 3; ++k)
         HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->alt[k], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
-    }
     buf = 0;
+    bool pending = true;
     for (;;) {
       // the list's arrays (and so the chunks' key) change when it grows
       KeyBuf base;
@@ -794,8 +824,12 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
       key_integ(base, I);
       ChunkSet set;
       if (int r = chunk_graphs(ctx, base.s, record, &set)) return r;
-      launch_chunk_init(ctx->st, ctx->d_chunk, t, b, steps);
-      const int r = replay_chunks(ctx, set, b, &t);
+      // the first chunks take over the first step's request (its parity
+      // flag); the device clock starts at iteration t
+      launch_chunk_init(ctx->st, ctx->d_chunk, t, b, steps, pending ? ctx->d_flags + r0 : nullptr, ctx->d_clock,
+                        I.slot, (unsigned long long)call_of(t));
+      const int r = replay_chunks(ctx, set, b, &t, pending ? 1 : 0);
+      pending = false;
       if (r < 0) return r;
       if (r == 0) break;
       // stopped at an overflowed rebuild of the positions now in buffer 0
@@ -806,17 +840,17 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   }
   const int t0 = t;
   for (; t < b; ++t) {
-    if (int r = fused_step(ctx, I, 1, t, buf + (t - t0), t - t0, steps, f32, planar, mass_u, true)) return r;
+    if (int r = fused_step(ctx, I, 1, t, buf + (t - t0), 0, steps, f32, planar, mass_u, true, nullptr, 0, true,
+                           call_of(t)))
+      return r;
     if (int r = recover_overflow(ctx, at_buffer(buf + (t - t0) + 1), planar); r < 0) return r;
   }
-  if (b > t0) launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
-  if (int r = fused_step(ctx, I, 2, b, buf + (b - t0), 0, steps, f32, planar, mass_u, true)) return r;
+  if (int r = fused_step(ctx, I, 2, b, buf + (b - t0), 0, steps, f32, planar, mass_u, true, nullptr, 0, true,
+                         call_of(b)))
+    return r;
   const int pf = (buf + (b - t0)) & 1;  // buffer holding x_b
-  if (planar) {
-    Fields f0 = fields(ctx, 0);
-    f0.xy = ctx->xy[pf];
-    launch_unpack_xy(ctx->st, f0, ctx->n);
-  } else if (pf) {
+  // planar: the closing step wrote x_b back to the x, y columns
+  if (!planar && pf) {
     for (int k = 0; k < 3; ++k)
       HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->alt[k], sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->st));
   }
@@ -933,9 +967,7 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
     Fields f0 = buffer(0), f1 = buffer(1);
     auto record = [&](int m) -> int {
       for (int j = 0; j < m; ++j) step(b - 2, j & 1, j, ctx->d_chunk, j);
-      launch_chunk_settle(ctx->st, ctx->d_chunk, f0, f1, n, planar ? 1 : 0, 1);
-      launch_rebuild(ctx->st, f0, ctx->nl, ctx->box, n, &ctx->d_chunk->rebuild, planar ? 1 : 0);
-      launch_chunk_finish(ctx->st, ctx->d_chunk, ctx->d_clock);
+      launch_chunk_rebuild(ctx->st, ctx->d_chunk, ctx->d_clock, f0, f1, ctx->nl, ctx->box, n, planar ? 1 : 0, 1);
       return 0;
     };
     for (;;) {
@@ -1886,13 +1918,21 @@ int run_range_core(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags)
                      only->method == 0 && al_mask(ctx) == 0 && pot_mask(ctx) != 0 && needs_nlist(ctx) &&
                      fused_size_ok(ctx);
   const bool fused_b = !fused && fused_brownian_eligible(ctx, flags);
+  bool packed = false;
   if (t_begin == 0 && !ctx->run_begun) {
     // Loop head of t = 0 (src/samos.cpp:887-892): enforce, reset, forces and
     // torques.  The pre-run evaluation before it (:872-875) is overwritten by
     // this reset and leaves no trace, so it is not repeated here.  The fused
     // schedules evaluate the loop-head forces of every step they integrate,
     // so they need the evaluation only when the range stops at t = 0.
-    launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
+    Fields f = fields(ctx);
+    // the fused planar Langevin step reads positions from the interleaved
+    // column, which this enforce writes as it goes
+    if (fused && t_end > t_begin && ctx->con.kind == 1 && ctx->nl.use_cell) {
+      f.xy = ctx->xy[1];
+      packed = true;
+    }
+    launch_enforce(ctx->st, f, ctx->box, ctx->con, ctx->n);
     if (!(fused || fused_b) || t_end == t_begin) loop_head_forces(ctx, nullptr, f32);
     ctx->run_begun = true;
   }
@@ -1902,7 +1942,7 @@ int run_range_core(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags)
   }
   if (t_end == steps + 1) ctx->run_begun = false;
   if (fused) {
-    if (int r = run_fused_langevin(ctx, *only, t_begin, t_end, steps, flags)) return r;
+    if (int r = run_fused_langevin(ctx, *only, t_begin, t_end, steps, flags, packed)) return r;
     ctx->time_step += t_end - t_begin;
     return check_overflow(ctx);
   }

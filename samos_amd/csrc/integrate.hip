@@ -44,12 +44,30 @@ __device__ __forceinline__ void store_ps(const Fields& f, int s, const PS& p) {
 }
 
 
+__device__ __forceinline__ bool differs(double a, double b) { return __double_as_longlong(a) != __double_as_longlong(b); }
+
+// Constraint::enforce over every particle.  Only the values the enforce
+// changed are stored (bit comparison), and with f.xy set the interleaved
+// (x, y) column is written as well (the planar step's layout).
 __global__ void k_enforce(Fields f, BoxD b, ConstraintD c, int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   PS p; load_ps(f, s, p);
+  const PS q = p;
   if (enforce(b, c, p)) f.fz[s] = 0.0;
-  store_ps(f, s, p);
+  if (differs(p.x, q.x)) f.x[s] = p.x;
+  if (differs(p.y, q.y)) f.y[s] = p.y;
+  if (differs(p.z, q.z)) f.z[s] = p.z;
+  if (differs(p.vx, q.vx)) f.vx[s] = p.vx;
+  if (differs(p.vy, q.vy)) f.vy[s] = p.vy;
+  if (differs(p.vz, q.vz)) f.vz[s] = p.vz;
+  if (differs(p.nx, q.nx)) f.nx[s] = p.nx;
+  if (differs(p.ny, q.ny)) f.ny[s] = p.ny;
+  if (differs(p.nz, q.nz)) f.nz[s] = p.nz;
+  if (p.dix) f.ix[s] += p.dix;
+  if (p.diy) f.iy[s] += p.diy;
+  if (p.diz) f.iz[s] += p.diz;
+  if (f.xy) f.xy[s] = make_double2(p.x, p.y);
 }
 
 // which: 0 brownian, 1 brownian_pos, 2 brownian_align
@@ -462,7 +480,7 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
         p.vx *= exp_dt; p.vy *= exp_dt; p.vz *= exp_dt;
         if (B != 0.0) {
           const double sf = fa.sf_u >= 0.0 ? fa.sf_u : B / sqrt(m);
-          const unsigned long long call = eff_call(clk, slot, 0ULL) + (unsigned long long)t_off;
+          const unsigned long long call = clk ? clk->calls[slot] + (unsigned long long)t_off : fa.call0;
           double g[3];
           if (PLANAR) {  // the z draw only reaches v_z, which the next enforce resets
             gauss2(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], call, g);
@@ -494,7 +512,9 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
       min_image(b, ddx, ddy, ddz);
       need = !(ddx * ddx + ddy * ddy + ddz * ddz < nl.pad2q);
     } else {
-      // closing phase: forces F(x_t) are the state the reference leaves behind
+      // closing phase: forces F(x_t) are the state the reference leaves behind;
+      // a planar run's positions go back to the x, y columns here
+      if (PLANAR && fa.out_x) { fa.out_x[s] = xi; fa.out_y[s] = yi; }
       cur.vx[s] = vx; cur.vy[s] = vy;
       if (!PLANAR) cur.vz[s] = vz;
       cur.fx[s] = fx; cur.fy[s] = fy; cur.fz[s] = fz;
@@ -523,20 +543,26 @@ __global__ void __launch_bounds__(kBlock) k_chunk_settle(ChunkCtl* ck, Fields f0
   }
 }
 
-__global__ void k_chunk_finish(ChunkCtl* ck, ClockD* clk) {
+__global__ void k_chunk_finish(ChunkCtl* ck, ClockD* clk, const int* nl_flags) {
   const int last = ck->last;
   __syncthreads();
   if (threadIdx.x < 8) clk->calls[threadIdx.x] += (unsigned long long)last;
   if (threadIdx.x == 0) {
     clk->run_step += last;
     ck->base += last;
+    ck->overflow = nl_flags[2];
     ck->halt = 0; ck->req[0] = 0; ck->req[1] = 0; ck->last = 0; ck->rebuild = 0;
   }
 }
 
-__global__ void k_chunk_init(ChunkCtl* ck, int base, int limit, int steps) {
+// pending: a rebuild request left by the step before the chunks (slot 0 then
+// stops at once and the chunk's rebuild serves it), or null.
+__global__ void k_chunk_init(ChunkCtl* ck, int base, int limit, int steps, const int* pending, ClockD* clk, int slot,
+                             unsigned long long calls) {
   if (threadIdx.x == 0) {
-    ck->halt = 0; ck->req[0] = 0; ck->req[1] = 0; ck->last = 0; ck->rebuild = 0;
+    if (clk) { clk->calls[slot] = calls; clk->run_step = base; }
+    ck->halt = 0; ck->req[0] = 0; ck->req[1] = pending ? *pending : 0; ck->last = 0; ck->rebuild = 0;
+    ck->overflow = 0;
     ck->base = base; ck->limit = limit; ck->steps = steps;
   }
 }
@@ -701,12 +727,13 @@ void launch_chunk_settle(hipStream_t st, ChunkCtl* ck, Fields f0, Fields f1, int
   hipLaunchKernelGGL(k_chunk_settle, dim3(g), dim3(kBlock), 0, st, ck, f0, f1, n, planar, directors);
 }
 
-void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock) {
-  hipLaunchKernelGGL(k_chunk_finish, dim3(1), dim3(64), 0, st, ck, clock);
+void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock, const int* nl_flags) {
+  hipLaunchKernelGGL(k_chunk_finish, dim3(1), dim3(64), 0, st, ck, clock, nl_flags);
 }
 
-void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit, int steps) {
-  hipLaunchKernelGGL(k_chunk_init, dim3(1), dim3(64), 0, st, ck, base, limit, steps);
+void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit, int steps, const int* pending, ClockD* clk,
+                       int slot, unsigned long long calls) {
+  hipLaunchKernelGGL(k_chunk_init, dim3(1), dim3(64), 0, st, ck, base, limit, steps, pending, clk, slot, calls);
 }
 
 void launch_clock_advance(hipStream_t st, ClockD* clock, int nsteps) {

@@ -112,6 +112,9 @@ struct FusedArgs {
   ChunkCtl* chunk = nullptr;  // captured-chunk mode (null: direct launch with the parity flags)
   int slot = 0;               // slot index inside the chunk
   unsigned gbit = 1u;         // the integrator's group: other slots only take the loop-head enforce
+  unsigned long long call0 = 0;  // noise call of a launch without the device clock (clock null)
+  double* out_x = nullptr;    // closing phase of a planar run: x, y columns to write (or null)
+  double* out_y = nullptr;
 };
 void launch_baoab_fused(hipStream_t st, int pot_mask, int phase, Fields cur, Fields nxt, NListD nl, BoxD box,
                         ConstraintD con, InteractD ip, int n, const FusedArgs& fa, int fp32, int planar,
@@ -170,8 +173,16 @@ void launch_brownian_fused(hipStream_t st, int pot_mask, int al_mask, Fields cur
 // that ran and clears the control words; init sets base, limit and the run's
 // last iteration.
 void launch_chunk_settle(hipStream_t st, ChunkCtl* ck, Fields f0, Fields f1, int n, int planar, int directors);
-void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock);
-void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit, int steps);
+void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock, const int* nl_flags);
+// The end of a captured step chunk in five dispatches: settle + binning, the
+// scan, scatter and cell-sort stages, build + the chunk's bookkeeping (the
+// stages of the rebuild do nothing unless a slot asked for one).
+void launch_chunk_rebuild(hipStream_t st, ChunkCtl* ck, ClockD* clk, Fields f0, Fields f1, NListD nl, BoxD box,
+                          int n, int planar, int directors);
+// clk (optional): the device clock is set to noise call `calls` in slot
+// `slot` and run step `base`.
+void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit, int steps, const int* pending = nullptr,
+                       ClockD* clk = nullptr, int slot = 0, unsigned long long calls = 0);
 // Common-age bookkeeping of fused ranges (AgeCtl): `check` records whether
 // every age equals age[0]; `finish` advances the common value by `count`
 // additions of dt and writes it to every slot (nothing when not uniform).

@@ -244,11 +244,15 @@ __device__ void sort_phase(NListD& nl) {
 
 // Flags: [2] overflow, [4] builds, [5] largest neighbour count.  One atomic per
 // wavefront.
-__device__ __forceinline__ void build_epilogue(NListD& nl, bool overflow, int kmax) {
+// ovf2 (optional): a second overflow flag (the captured chunk's).
+__device__ __forceinline__ void build_epilogue(NListD& nl, bool overflow, int kmax, int* ovf2 = nullptr) {
   for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o, 64));
   const bool any_over = __ballot(overflow) != 0;
   if ((threadIdx.x & 63) == 0) {
-    if (any_over) atomicOr(&nl.flags[2], 1);
+    if (any_over) {
+      atomicOr(&nl.flags[2], 1);
+      if (ovf2) atomicOr(ovf2, 1);
+    }
     if (kmax > 0) atomic_max_if_above(&nl.flags[5], kmax);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&nl.flags[4], 1);
@@ -258,7 +262,7 @@ __device__ __forceinline__ void build_epilogue(NListD& nl, bool overflow, int km
 // Cell members are consumed in batches so several candidate loads are in
 // flight per wavefront.
 template <bool PLANAR>
-__device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
+__device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n, int* ovf2 = nullptr) {
   bool overflow = false;
   int kmax = 0;
   constexpr int U = 8;
@@ -341,7 +345,7 @@ __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n) {
     f.coord[s] = coord;
     nl.ox[s] = xi; nl.oy[s] = yi; nl.oz[s] = zi;
   }
-  build_epilogue(nl, overflow, kmax);
+  build_epilogue(nl, overflow, kmax, ovf2);
 }
 
 // ---- standalone stage kernels ---------------------------------------------
@@ -377,6 +381,52 @@ template <bool PLANAR>
 __global__ void k_build(Fields f, NListD nl, BoxD b, int n, const int* req) {
   if (req_off(req)) return;
   build_phase<PLANAR>(f, nl, b, n);
+}
+
+// ---- the end of a captured step chunk ----------------------------------------
+// First kernel: the chunk's positions (and directors) settle into buffer 0
+// when its last slot left them in buffer 1, the rebuild request of its slots
+// is recorded, and when there is one the slots are binned, each by the thread
+// that has just copied it.
+template <bool PLANAR>
+__global__ void k_settle_bin(ChunkCtl* ck, Fields f0, Fields f1, NListD nl, BoxD b, int n, int directors) {
+  const bool odd = (ck->last & 1) != 0;
+  const int rb = ck->halt | ck->req[0] | ck->req[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) ck->rebuild = rb;
+  if (!odd && !rb) return;
+  if (odd) {
+    for (int i = gtid(); i < n; i += gsize()) {
+      if (PLANAR) {
+        f0.xy[i] = f1.xy[i];
+      } else {
+        f0.x[i] = f1.x[i]; f0.y[i] = f1.y[i]; f0.z[i] = f1.z[i];
+      }
+      if (directors) {
+        f0.nx[i] = f1.nx[i]; f0.ny[i] = f1.ny[i];
+        if (!PLANAR) f0.nz[i] = f1.nz[i];
+      }
+    }
+  }
+  if (rb) bin_phase<PLANAR>(f0, nl, b, n);
+}
+
+// Last kernel: the chunk's bookkeeping (run step and noise calls advanced by
+// the slots that ran, requests cleared), then the list build when requested;
+// an overflow of it is recorded in the chunk as well.
+template <bool PLANAR>
+__global__ void k_build_finish(Fields f, NListD nl, BoxD b, int n, ChunkCtl* ck, ClockD* clk) {
+  if (blockIdx.x == 0) {
+    const int last = ck->last;
+    __syncthreads();
+    if (threadIdx.x < 8) clk->calls[threadIdx.x] += (unsigned long long)last;
+    if (threadIdx.x == 0) {
+      clk->run_step += last;
+      ck->base += last;
+      ck->halt = 0; ck->req[0] = 0; ck->req[1] = 0; ck->last = 0;
+    }
+  }
+  if (ck->rebuild == 0) return;
+  build_phase<PLANAR>(f, nl, b, n, &ck->overflow);
 }
 
 // ---- the whole rebuild in one cooperative launch ----------------------------
@@ -537,6 +587,24 @@ void launch_rebuild(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const 
   launch_scatter(st, nl, n, req);
   launch_cell_sort(st, nl, req);
   launch_build(st, f, nl, box, n, req, planar);
+}
+
+void launch_chunk_rebuild(hipStream_t st, ChunkCtl* ck, ClockD* clk, Fields f0, Fields f1, NListD nl, BoxD box,
+                          int n, int planar, int directors) {
+  if (!nl.use_cell) {
+    launch_chunk_settle(st, ck, f0, f1, n, planar, directors);
+    launch_build_nsq(st, f0, nl, box, n, &ck->rebuild);
+    launch_chunk_finish(st, ck, clk, nl.flags);
+    return;
+  }
+  const int* req = &ck->rebuild;
+  if (planar) hipLaunchKernelGGL(k_settle_bin<true>, dim3(grid_for(n)), dim3(kBlock), 0, st, ck, f0, f1, nl, box, n, directors);
+  else hipLaunchKernelGGL(k_settle_bin<false>, dim3(grid_for(n)), dim3(kBlock), 0, st, ck, f0, f1, nl, box, n, directors);
+  launch_scan(st, nl, req);
+  launch_scatter(st, nl, n, req);
+  launch_cell_sort(st, nl, req);
+  if (planar) hipLaunchKernelGGL(k_build_finish<true>, dim3(grid_for(n)), dim3(kBlock), 0, st, f0, nl, box, n, ck, clk);
+  else hipLaunchKernelGGL(k_build_finish<false>, dim3(grid_for(n)), dim3(kBlock), 0, st, f0, nl, box, n, ck, clk);
 }
 
 hipError_t launch_rebuild_coop(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const int* req, int planar) {

@@ -174,6 +174,7 @@ struct ChunkCtl {
   int req[2];    // rebuild request of the slot with this parity
   int last;      // slots executed in this chunk
   int base;      // run step of slot 0
+  int overflow;  // the list's overflow flag as the chunk ended (next to base: one read-back)
   int limit;     // slot j runs only while base + j < limit
   int rebuild;   // rebuild request read by the chunk's rebuild stages
   int steps;     // last iteration of the run: slot j applies the next loop
