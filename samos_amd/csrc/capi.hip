@@ -749,7 +749,7 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     const int r = run_tiled_langevin(ctx, I, a, b, steps, flags, mass_u);
     if (r <= 0) return r;
   }
-  HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
+  // (the rebuild-request flags were cleared at the range's start)
   // Planar layout: the plane constraint puts every particle on z = zpos, so
   // neighbour gathers read the interleaved (x, y) column.  The N^2 build reads
   // the x, y, z columns, so lists without a cell grid keep the 3-D layout.
@@ -836,7 +836,7 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
       if (int rc = recover_overflow(ctx, f0, planar); rc < 0) return rc;
     }
     // the last chunk served any request; direct steps use the parity flags
-    HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
+    if (t < b) HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   }
   const int t0 = t;
   for (; t < b; ++t) {
@@ -1909,7 +1909,8 @@ int apply_rescale(samos_ctx* ctx) {
 int run_range_core(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags) {
   if (int r = sync_fire_groups(ctx)) return r;
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
-  HIPCHK(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->st));
+  // rebuild requests of both step parities and the overflow flag
+  HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 3 * sizeof(int), ctx->st));
   // Fused schedule: one Langevin BAOAB integrator, pair potentials only.  The
   // reference's loop-head force evaluation repeats the evaluation that ended
   // the previous step on the same positions, so it is done once.
@@ -1942,9 +1943,12 @@ int run_range_core(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags)
   }
   if (t_end == steps + 1) ctx->run_begun = false;
   if (fused) {
+    // every rebuild of the fused schedule is checked for overflow as the run
+    // goes (and a truncated list is rebuilt before any step reads it)
     if (int r = run_fused_langevin(ctx, *only, t_begin, t_end, steps, flags, packed)) return r;
     ctx->time_step += t_end - t_begin;
-    return check_overflow(ctx);
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
   }
   if (fused_b) {
     if (int r = run_fused_brownian(ctx, t_begin, t_end, steps, flags)) return r;
