@@ -256,8 +256,14 @@ int samos_set_option(samos_ctx* ctx, const char* name, double value);
  * the capacity stops the run's captured steps before any of them reads the
  * list; the capacity then grows and the list is rebuilt from the same
  * positions, so the run continues exactly as with room to spare.
- * Run statistics, no counterpart in the reference: out[4] = step-chunk graphs
- * instantiated, chunk graphs reused, list overflows recovered, capacity. */
+ * "resort_every" k (no counterpart in the reference): inside a run, re-sort
+ * the particles' storage along the Morton order of their cells after every k
+ * rebuilds of the captured steps (default 200; 0 = never), keeping the
+ * neighbour gathers local over long runs; the pair set and the trajectory
+ * are unchanged up to the order of the force sums.
+ * Run statistics, no counterpart in the reference: out[5] = step-chunk graphs
+ * instantiated, chunk graphs reused, list overflows recovered, capacity,
+ * Morton re-sorts inside runs. */
 int samos_run_stats(samos_ctx* ctx, long long* out);
 /* Cell-tiled planar engine (no counterpart in the reference): planar fused
  * Langevin runs keep the state in cell order and the Verlet list as stencil

@@ -395,9 +395,10 @@ class System:
         return v.value
 
     def run_stats(self):
-        a = (ctypes.c_longlong * 4)()
+        a = (ctypes.c_longlong * 5)()
         self._chk(self.L.samos_run_stats(self.h, a))
-        return dict(zip(["graphs_instantiated", "graphs_reused", "overflow_recoveries", "capacity"], list(a)))
+        return dict(zip(["graphs_instantiated", "graphs_reused", "overflow_recoveries", "capacity", "resorts"],
+                        list(a)))
 
     def sync(self):
         self._chk(self.L.samos_sync(self.h))

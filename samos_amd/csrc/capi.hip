@@ -122,6 +122,9 @@ struct samos_ctx {
   unsigned long long graph_tick = 0;
   long long graph_stats[2] = {0, 0};  // instantiations, reuses
   long long overflow_recoveries = 0;  // list overflows recovered inside runs
+  int resort_every = 200;             // list builds between Morton re-sorts inside runs (0: never)
+  int builds_at_sort = 0;             // the builds count after the last re-sort
+  long long resorts = 0;              // re-sorts done inside runs
   // cell-tiled engine of planar fused runs (allocated on first use)
   TileD td{};
   void* tile_mem = nullptr;
@@ -336,7 +339,8 @@ int chunk_graphs(samos_ctx* ctx, const std::string& base, const std::function<in
 // base after each batch.  Returns the base reached in *t, and 1
 // when the batch stopped at a neighbour-list overflow.
 // stall: rounds that may end without progress (a request left before the
-// chunks stops their first slot; the chunk's rebuild serves it).
+// chunks stops their first slot; the chunk's rebuild serves it).  Returns 2
+// when the slots stopped because a Morton re-sort is due.
 int replay_chunks(samos_ctx* ctx, const ChunkSet& set, int end, int* t, int stall = 0) {
   const int Mmax = set.back().first;
   double per = Mmax;
@@ -366,13 +370,17 @@ int replay_chunks(samos_ctx* ctx, const ChunkSet& set, int end, int* t, int stal
         left -= g->first;
       }
     }
-    // the next run step and the list's overflow flag, adjacent in ChunkCtl
+    // the next run step and the stop word, adjacent in ChunkCtl
     HIPCHK(hipMemcpyAsync(h_base, &ctx->d_chunk->base, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     const int tn = h_base[0];
-    if (h_base[1]) {  // a rebuild overflowed: the slots after it stopped
+    if (h_base[1] & 1) {  // a rebuild overflowed: the slots after it stopped
       *t = tn;
       return 1;
+    }
+    if (h_base[1] & 2) {  // the slots stopped for a re-sort
+      *t = tn;
+      return 2;
     }
     if (tn == *t && stall > 0) { --stall; continue; }
     if (tn <= *t || tn > end) return fail(ctx, "captured step chunks made no progress");
@@ -460,15 +468,17 @@ int build_sync(samos_ctx* ctx, const Fields* at = nullptr, bool planar = false) 
 // grows and the list is rebuilt from the same positions, which gives the
 // rebuild's pair set, count and snapshot exactly; the run then continues.
 // Returns 1 when a recovery happened.
-int recover_overflow(samos_ctx* ctx, const Fields& at, bool planar) {
+// builds (optional): the builds count after the call.
+int recover_overflow(samos_ctx* ctx, const Fields& at, bool planar, int* builds = nullptr) {
   int h[8];
   if (int r = read_flags(ctx, h)) return r;
+  if (builds) *builds = h[4];
   if (!h[2]) return 0;
-  const int builds = h[4];
+  const int counted = h[4];
   if (int r = alloc_nbr(ctx, (int)std::ceil(h[5] * 1.5) + 16)) return r;
   if (int r = build_sync(ctx, &at, planar)) return r;
   // the rebuild that overflowed has been counted; its repetition is not
-  HIPCHK(hipMemcpyAsync(ctx->d_flags + 4, &builds, sizeof(int), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(ctx->d_flags + 4, &counted, sizeof(int), hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->overflow_recoveries++;
   return 1;
@@ -488,10 +498,66 @@ int resort(samos_ctx* ctx) {
     launch_gather_i(ctx->st, ctx->itmp, ctx->icol[k], ctx->perm2, n);
     HIPCHK(hipMemcpyAsync(ctx->icol[k], ctx->itmp, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->st));
   }
+  // the list's snapshot follows its slots
+  double* snap[3] = {ctx->nl.ox, ctx->nl.oy, ctx->nl.oz};
+  for (double* col : snap) {
+    launch_gather_d(ctx->st, ctx->dtmp, col, ctx->perm2, n);
+    HIPCHK(hipMemcpyAsync(col, ctx->dtmp, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
+  }
   HIPCHK(hipMemcpyAsync(ctx->h_id.data(), ctx->icol[5], sizeof(int) * n, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// Morton re-sort between captured chunks of a run (particles drift away from
+// their storage neighbours over long runs, and the gathers lose locality).
+// The positions are in buffer 0 then; every per-slot column, the list's
+// snapshot among them, is permuted, and the list is rebuilt from the
+// permuted snapshot: the same pair set as the last build, which stays the
+// one counted.  Only storage order changes (and with it the order of the
+// force sums).
+// par: the buffer holding the positions (and, with directors, the directors
+// of a schedule that double-buffers them); they are in buffer 0 afterwards.
+int resort_in_run(samos_ctx* ctx, bool planar, int par, bool directors) {
+  const size_t bytes = sizeof(double) * ctx->n;
+  if (par) {
+    if (planar) {
+      HIPCHK(hipMemcpyAsync(ctx->xy[0], ctx->xy[1], 2 * bytes, hipMemcpyDeviceToDevice, ctx->st));
+    } else {
+      for (int k = 0; k < 3; ++k)
+        HIPCHK(hipMemcpyAsync(ctx->dcol[k], ctx->alt[k], bytes, hipMemcpyDeviceToDevice, ctx->st));
+    }
+    if (directors)
+      for (int k = 0; k < (planar ? 2 : 3); ++k)
+        HIPCHK(hipMemcpyAsync(ctx->dcol[6 + k], ctx->altn[k], bytes, hipMemcpyDeviceToDevice, ctx->st));
+  }
+  if (planar) {
+    Fields f0 = fields(ctx, 0);
+    f0.xy = ctx->xy[0];
+    launch_unpack_xy(ctx->st, f0, ctx->n);
+  }
+  int h[8];
+  if (int r = read_flags(ctx, h)) return r;
+  if (int r = resort(ctx)) return r;
+  Fields at = fields(ctx);
+  at.x = ctx->nl.ox; at.y = ctx->nl.oy; at.z = ctx->nl.oz;
+  if (int r = build_sync(ctx, &at, false)) return r;
+  HIPCHK(hipMemcpyAsync(ctx->d_flags + 4, &h[4], sizeof(int), hipMemcpyHostToDevice, ctx->st));
+  if (planar) {
+    Fields f0 = fields(ctx, 0);
+    f0.xy = ctx->xy[0];
+    launch_pack_xy(ctx->st, f0, ctx->n);
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->builds_at_sort = h[4];
+  ctx->resorts++;
+  return 0;
+}
+
+// The builds count at which a run re-sorts next.
+int resort_at(const samos_ctx* ctx) {
+  return ctx->resort_every > 0 ? ctx->builds_at_sort + ctx->resort_every : INT_MAX;
 }
 
 // ---- forces ---------------------------------------------------------------
@@ -785,10 +851,22 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   // rebuild runs here and an overflow of it is caught before the next step.
   if (int r = fused_step(ctx, I, 0, a, r0, 0, steps, f32, planar, mass_u, true, nullptr, 0, !chunks, call_of(a)))
     return r;
-  if (!chunks)
-    if (int r = recover_overflow(ctx, at_buffer(r0 ^ 1), planar); r < 0) return r;
   int t = a + 1;
   int buf = r0 ^ 1;  // buffer holding x_t
+  // steps outside the chunks: an overflow of the step's rebuild is recovered,
+  // and a due re-sort done, before the next step reads the list
+  auto after_direct = [&](int par) -> int {
+    int builds = 0;
+    if (int r = recover_overflow(ctx, at_buffer(par), planar, &builds); r < 0) return r;
+    if (builds < resort_at(ctx)) return 0;
+    if (int r = resort_in_run(ctx, planar, par, false)) return r;
+    return 1;
+  };
+  if (!chunks) {
+    const int r = after_direct(buf);
+    if (r < 0) return r;
+    if (r == 1) buf = 0;
+  }
   // Captured chunks: M fused steps ("slots") followed by one device-conditional
   // rebuild.  A slot whose new positions need a rebuild stops the chunk: the
   // later slots return at once, the positions settle into buffer 0, the
@@ -826,29 +904,33 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
       if (int r = chunk_graphs(ctx, base.s, record, &set)) return r;
       // the first chunks take over the first step's request (its parity
       // flag); the device clock starts at iteration t
-      launch_chunk_init(ctx->st, ctx->d_chunk, t, b, steps, pending ? ctx->d_flags + r0 : nullptr, ctx->d_clock,
-                        I.slot, (unsigned long long)call_of(t));
+      launch_chunk_init(ctx->st, ctx->d_chunk, t, b, steps, resort_at(ctx), pending ? ctx->d_flags + r0 : nullptr,
+                        ctx->d_clock, I.slot, (unsigned long long)call_of(t));
       const int r = replay_chunks(ctx, set, b, &t, pending ? 1 : 0);
       pending = false;
       if (r < 0) return r;
       if (r == 0) break;
+      if (r == 2) {
+        if (int rc = resort_in_run(ctx, planar, 0, false)) return rc;
+        continue;
+      }
       // stopped at an overflowed rebuild of the positions now in buffer 0
       if (int rc = recover_overflow(ctx, f0, planar); rc < 0) return rc;
     }
     // the last chunk served any request; direct steps use the parity flags
     if (t < b) HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
   }
-  const int t0 = t;
   for (; t < b; ++t) {
-    if (int r = fused_step(ctx, I, 1, t, buf + (t - t0), 0, steps, f32, planar, mass_u, true, nullptr, 0, true,
-                           call_of(t)))
+    if (int r = fused_step(ctx, I, 1, t, buf, 0, steps, f32, planar, mass_u, true, nullptr, 0, true, call_of(t)))
       return r;
-    if (int r = recover_overflow(ctx, at_buffer(buf + (t - t0) + 1), planar); r < 0) return r;
+    buf ^= 1;
+    const int r = after_direct(buf);
+    if (r < 0) return r;
+    if (r == 1) buf = 0;
   }
-  if (int r = fused_step(ctx, I, 2, b, buf + (b - t0), 0, steps, f32, planar, mass_u, true, nullptr, 0, true,
-                         call_of(b)))
+  if (int r = fused_step(ctx, I, 2, b, buf, 0, steps, f32, planar, mass_u, true, nullptr, 0, true, call_of(b)))
     return r;
-  const int pf = (buf + (b - t0)) & 1;  // buffer holding x_b
+  const int pf = buf;  // buffer holding x_b
   // planar: the closing step wrote x_b back to the x, y columns
   if (!planar && pf) {
     for (int k = 0; k < 3; ++k)
@@ -980,10 +1062,14 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
       for (const Integ* G : grouped) key_integ(base, *G);
       ChunkSet set;
       if (int r = chunk_graphs(ctx, base.s, record, &set)) return r;
-      launch_chunk_init(ctx->st, ctx->d_chunk, t, b, steps);
+      launch_chunk_init(ctx->st, ctx->d_chunk, t, b, steps, resort_at(ctx));
       const int r = replay_chunks(ctx, set, b, &t);
       if (r < 0) return r;
       if (r == 0) break;
+      if (r == 2) {
+        if (int rc = resort_in_run(ctx, planar, 0, true)) return rc;
+        continue;
+      }
       // stopped at an overflowed rebuild of the positions now in buffer 0
       if (int rc = recover_overflow(ctx, f0, planar); rc < 0) return rc;
     }
@@ -991,12 +1077,19 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
   }
   const int t0 = t;
   for (; t < b; ++t) {
-    step(t, (buf + t - t0) & 1, t - t0, nullptr, 0);
-    // the next step reads the list this step's rebuild may have overflowed
-    if (int r = recover_overflow(ctx, buffer((buf + t - t0 + 1) & 1), planar); r < 0) return r;
+    step(t, buf, t - t0, nullptr, 0);
+    buf ^= 1;
+    // the next step reads the list this step's rebuild may have overflowed;
+    // a due re-sort happens here as well
+    int builds = 0;
+    if (int r = recover_overflow(ctx, buffer(buf), planar, &builds); r < 0) return r;
+    if (builds >= resort_at(ctx)) {
+      if (int r = resort_in_run(ctx, planar, buf, true)) return r;
+      buf = 0;
+    }
   }
   if (b > t0) launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
-  const int pf = (buf + b - t0) & 1;  // buffer holding x_b and n_b
+  const int pf = buf;  // buffer holding x_b and n_b
   if (planar) {
     Fields f0 = fields(ctx, 0);
     f0.xy = ctx->xy[pf];
@@ -1316,6 +1409,7 @@ int samos_create(samos_ctx** out, int device, int n, double lx, double ly, doubl
   if (hipMalloc(&ctx->sort_tmp, ctx->sort_tmp_bytes + 256) != hipSuccess) return bad("sort");
   // defaults: type 1, radius 1, mass 1, everything else zero; id = slot
   hipMemsetAsync(ctx->d_flags, 0, 16 * sizeof(int), ctx->st);
+  hipMemsetAsync(ctx->d_chunk, 0, sizeof(ChunkCtl), ctx->st);
   hipMemsetAsync(ctx->d_tab, 0, 4 * kTab * sizeof(double), ctx->st);
   for (int k = 0; k < kNumDoubleFields; ++k) hipMemsetAsync(ctx->dcol[k], 0, sizeof(double) * np, ctx->st);
   for (int k = 0; k < kNumIntFields; ++k) hipMemsetAsync(ctx->icol[k], 0, sizeof(int) * np, ctx->st);
@@ -2252,6 +2346,7 @@ int samos_run_stats(samos_ctx* ctx, long long* out) {
   out[1] = ctx->graph_stats[1];
   out[2] = ctx->overflow_recoveries;
   out[3] = ctx->nl.stride;
+  out[4] = ctx->resorts;
   return 0;
 }
 
@@ -2294,6 +2389,9 @@ int samos_set_option(samos_ctx* ctx, const char* name, double value) {
     HIPCHK(hipMemcpyAsync(ctx->d_flags + 4, &h[4], sizeof(int), hipMemcpyHostToDevice, ctx->st));
     if (int r = read_flags(ctx, h)) return r;
     if (h[2]) return fail(ctx, "neighbour_capacity below the current largest count");
+  } else if (k == "resort_every") {
+    if (value < 0.0 || value != std::floor(value)) return fail(ctx, "resort_every must be a non-negative integer");
+    ctx->resort_every = (int)value;
   } else if (k == "tile_cells") {
     if (value < 0.0 || value > kTileMaxCells || value != std::floor(value))
       return fail(ctx, "tile_cells must be 0 (automatic) or an integer in 1.." + std::to_string(kTileMaxCells));

@@ -406,6 +406,10 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
     // overflowed at a rebuild stops every later slot before any step reads it.
     const int j = fa.slot;
     if (nl.flags[2]) return;
+    if (nl.flags[4] >= ck->resort_at) {  // a Morton re-sort is due before this slot
+      if (blockIdx.x == 0 && threadIdx.x == 0) ck->stop |= 2;
+      return;
+    }
     if (ck->halt | ck->req[(j + 1) & 1]) {
       if (blockIdx.x == 0 && threadIdx.x == 0) ck->halt = 1;
       return;
@@ -550,7 +554,7 @@ __global__ void k_chunk_finish(ChunkCtl* ck, ClockD* clk, const int* nl_flags) {
   if (threadIdx.x == 0) {
     clk->run_step += last;
     ck->base += last;
-    ck->overflow = nl_flags[2];
+    if (nl_flags[2]) ck->stop |= 1;
     ck->halt = 0; ck->req[0] = 0; ck->req[1] = 0; ck->last = 0; ck->rebuild = 0;
   }
 }
@@ -558,11 +562,11 @@ __global__ void k_chunk_finish(ChunkCtl* ck, ClockD* clk, const int* nl_flags) {
 // pending: a rebuild request left by the step before the chunks (slot 0 then
 // stops at once and the chunk's rebuild serves it), or null.
 __global__ void k_chunk_init(ChunkCtl* ck, int base, int limit, int steps, const int* pending, ClockD* clk, int slot,
-                             unsigned long long calls) {
+                             unsigned long long calls, int resort_at) {
   if (threadIdx.x == 0) {
     if (clk) { clk->calls[slot] = calls; clk->run_step = base; }
     ck->halt = 0; ck->req[0] = 0; ck->req[1] = pending ? *pending : 0; ck->last = 0; ck->rebuild = 0;
-    ck->overflow = 0;
+    ck->stop = 0; ck->resort_at = resort_at;
     ck->base = base; ck->limit = limit; ck->steps = steps;
   }
 }
@@ -731,9 +735,10 @@ void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock, const int*
   hipLaunchKernelGGL(k_chunk_finish, dim3(1), dim3(64), 0, st, ck, clock, nl_flags);
 }
 
-void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit, int steps, const int* pending, ClockD* clk,
-                       int slot, unsigned long long calls) {
-  hipLaunchKernelGGL(k_chunk_init, dim3(1), dim3(64), 0, st, ck, base, limit, steps, pending, clk, slot, calls);
+void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit, int steps, int resort_at,
+                       const int* pending, ClockD* clk, int slot, unsigned long long calls) {
+  hipLaunchKernelGGL(k_chunk_init, dim3(1), dim3(64), 0, st, ck, base, limit, steps, pending, clk, slot, calls,
+                     resort_at);
 }
 
 void launch_clock_advance(hipStream_t st, ClockD* clock, int nsteps) {

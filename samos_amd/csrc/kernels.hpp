@@ -179,10 +179,12 @@ void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock, const int*
 // stages of the rebuild do nothing unless a slot asked for one).
 void launch_chunk_rebuild(hipStream_t st, ChunkCtl* ck, ClockD* clk, Fields f0, Fields f1, NListD nl, BoxD box,
                           int n, int planar, int directors);
+// resort_at: builds count at which the slots stop for a Morton re-sort.
 // clk (optional): the device clock is set to noise call `calls` in slot
 // `slot` and run step `base`.
-void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit, int steps, const int* pending = nullptr,
-                       ClockD* clk = nullptr, int slot = 0, unsigned long long calls = 0);
+void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit, int steps, int resort_at,
+                       const int* pending = nullptr, ClockD* clk = nullptr, int slot = 0,
+                       unsigned long long calls = 0);
 // Common-age bookkeeping of fused ranges (AgeCtl): `check` records whether
 // every age equals age[0]; `finish` advances the common value by `count`
 // additions of dt and writes it to every slot (nothing when not uniform).

@@ -412,7 +412,7 @@ __global__ void k_settle_bin(ChunkCtl* ck, Fields f0, Fields f1, NListD nl, BoxD
 
 // Last kernel: the chunk's bookkeeping (run step and noise calls advanced by
 // the slots that ran, requests cleared), then the list build when requested;
-// an overflow of it is recorded in the chunk as well.
+// an overflow of it is recorded in the chunk's stop word as well.
 template <bool PLANAR>
 __global__ void k_build_finish(Fields f, NListD nl, BoxD b, int n, ChunkCtl* ck, ClockD* clk) {
   if (blockIdx.x == 0) {
@@ -426,7 +426,7 @@ __global__ void k_build_finish(Fields f, NListD nl, BoxD b, int n, ChunkCtl* ck,
     }
   }
   if (ck->rebuild == 0) return;
-  build_phase<PLANAR>(f, nl, b, n, &ck->overflow);
+  build_phase<PLANAR>(f, nl, b, n, &ck->stop);
 }
 
 // ---- the whole rebuild in one cooperative launch ----------------------------

@@ -174,7 +174,9 @@ struct ChunkCtl {
   int req[2];    // rebuild request of the slot with this parity
   int last;      // slots executed in this chunk
   int base;      // run step of slot 0
-  int overflow;  // the list's overflow flag as the chunk ended (next to base: one read-back)
+  int stop;      // why the chunk's slots stopped early (next to base: one read-back):
+                 // bit 0 the list overflowed at a rebuild, bit 1 a Morton re-sort is due
+  int resort_at; // builds count (NListD::flags[4]) at which slots stop for a re-sort
   int limit;     // slot j runs only while base + j < limit
   int rebuild;   // rebuild request read by the chunk's rebuild stages
   int steps;     // last iteration of the run: slot j applies the next loop
