@@ -14,9 +14,8 @@ either stream, so the checks here are:
   * oracle(Philox) against oracle(MT19937), on the CPU, and device (Philox)
     against oracle(MT19937), on the GPU: velocity autocorrelation at three
     lags, the polar or nematic order and <|v|^2>, averaged per seed over M
-    seeds from one start configuration, must agree within 2 standard errors
-    of the difference, allowing one statistic of a case in [2, 3) sigma for
-    sampling noise (tests/noise_stats.py);
+    seeds from one start configuration, must agree within 2 sigma as a
+    vector (Hotelling's T^2 turned into a normal deviate, tests/noise_stats.py);
   * the comparison has power: a run with the rotational noise variance nu or
     the friction gamma doubled fails it by a wide margin.
 """
@@ -138,18 +137,18 @@ def test_philox_stream_statistics_match_reference_stream(scenario):
     else:
         a = oracle_runs(lj_langevin, range(100, 100 + m), "philox", nside=16)
         b = oracle_runs(lj_langevin, range(200, 200 + m), "mt19937", nside=16)
-    z = ns.zscores(a, b)
-    assert ns.same_distribution(z), z
+    zj = ns.joint_z(a, b)
+    assert ns.same_distribution(a, b), (zj, ns.zscores(a, b))
 
 
 def test_comparison_detects_wrong_noise():
     m = 48
     a = oracle_runs(active_plane, range(100, 100 + m), "philox", n=256)
     b = oracle_runs(active_plane, range(300, 300 + m), "mt19937", n=256, nu=0.2)
-    assert ns.zscores(a, b).max() > 8.0
+    assert ns.zscores(a, b).max() > 8.0 and ns.joint_z(a, b) > 8.0
     a = oracle_runs(lj_langevin, range(100, 100 + m), "philox", nside=16)
     b = oracle_runs(lj_langevin, range(300, 300 + m), "mt19937", nside=16, gamma=2.0)
-    assert ns.zscores(a, b).max() > 4.0
+    assert ns.zscores(a, b).max() > 4.0 and ns.joint_z(a, b) > 4.0
 
 
 # ------------------------------------------------------------------ GPU ---
@@ -169,6 +168,6 @@ def test_device_noise_statistics_match_reference_stream(gpu, case):
     print("noise statistics: %s, %d seeds per side" % (case, m), flush=True)
     dev = np.array([fn(gpu.System, sd, **kw) for sd in range(500, 500 + m)])
     ref = oracle_runs(fn, range(600, 600 + m), "mt19937", **kw)
-    z = ns.zscores(dev, ref)
-    print("  device means %s\n  oracle means %s\n  z %s" % (dev.mean(0), ref.mean(0), z), flush=True)
-    assert ns.same_distribution(z), z
+    z, zj = ns.zscores(dev, ref), ns.joint_z(dev, ref)
+    print("  device means %s\n  oracle means %s\n  z %s, joint %.3f" % (dev.mean(0), ref.mean(0), z, zj), flush=True)
+    assert ns.same_distribution(dev, ref), (zj, z)
