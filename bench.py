@@ -476,11 +476,31 @@ def bench_fire(samos_amd, sc, nrep=512, n=4096, max_iter=400000, weights=WARM_WE
     e.close()
     avg_s = ms / 1e3 / max(launches, 1)
     achieved = FIRE_BYTES_PER_PARTICLE_ITER * nrep * n / avg_s / 1e9
+    # Control for any warm start: FIRE restarted (v = 0, fresh dt and alpha)
+    # from the exact positions of plain FIRE's own path after one iteration.
+    e = fire_batch(samos_amd, sc, states)
+    e.fire_setup(dt=0.01, alpha=0.1, dt_max=0.1, f_tol=0.0, stop_fmax=1e-10)
+    e.fire_run(1, check=1)
+    zeros = np.zeros(nrep * n)
+    e.set("vx", zeros)
+    e.set("vy", zeros)
+    e.fire_setup(dt=0.01, alpha=0.1, dt_max=0.1, f_tol=0.0, stop_fmax=1e-10)
+    e.fire_run(max_iter, check=100)
+    fr = e.fire_state()
+    e.close()
+    restart_same = float(np.mean((fr["stopped"] == 1) & (np.abs(fr["energy"] - fs["energy"]) <= 1e-12 * np.abs(fs["energy"]))))
+    print("bench: FIRE restarted from its own positions after one iteration: same minimum %.3f" % restart_same,
+          file=sys.stderr, flush=True)
     warm = None
     if weights and os.path.exists(weights):
         warm = warm_start_ab(samos_amd, sc, states, fs, secs, max_iter, weights)
     return {
         "warm_start": warm,
+        "restart_control": {
+            "what": "FIRE restarted (v=0, dt0, alpha0) from plain FIRE's own positions after 1 iteration",
+            "same_minimum_fraction": restart_same,
+            "iterations_per_replica_median": float(np.median(fr["iters"]) + 1),
+        },
         "workload": "fire_polydisperse: %d replicas x N=%d, radii U[0.8, 1.2], periodic, packing 0.88; soft k=1 "
                     "use_particle_radii; FIRE dt0 0.01 dt_max 0.1 alpha 0.1, stop at max|F| < 1e-10" % (nrep, n),
         "time_to_converge_s": secs,
