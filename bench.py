@@ -137,7 +137,7 @@ def oracle_timed(body):
     return json.loads(out.stdout.strip().splitlines()[-1])
 
 
-def cpu_baseline_ensemble(sample_every=32, steps=100):
+def cpu_baseline_ensemble(sample_every=32, steps=400):
     """The ensemble's replicas as the reference runs them, one `samos` per
     replica: the oracle's reference-flag build on one core, over every
     `sample_every`-th replica of the same workload for `steps` run-loop
@@ -169,7 +169,7 @@ print(json.dumps({"ps": tot_ps, "seconds": tot_t, "counts": picked}))
                                                                      r["counts"], steps, r["seconds"], host_cpu())}
 
 
-def cpu_baseline_fire(iterations=6000, n=4096):
+def cpu_baseline_fire(iterations=40000, n=4096):
     """One configuration of the FIRE workload relaxed by the oracle's
     reference-flag build on one core (IntegratorFIRE, which never stops by
     itself): `iterations` run-loop iterations."""
@@ -608,7 +608,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--nside", type=int, default=NSIDE)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--variants", action="store_true", help="also time the FP32-force and faithful schedules")
     # skin: the reference's default padding (src/system/defaults.hpp:37); the
     # fastest of 0.3-0.8 in a sweep on MI355X (0.4-0.6 within 1%)
@@ -664,9 +664,9 @@ def main():
                                     cpu=not args.no_cpu)
     ng = samos_amd.RUN_NO_GRAPH if args.no_graph else 0
     if "c1" in parts:
-        extras["polar_disks_plane_n4096"] = bench_single(samos_amd, sc, "c1", 10000, 0 if args.no_cpu else 1000, ng)
+        extras["polar_disks_plane_n4096"] = bench_single(samos_amd, sc, "c1", 10000, 0 if args.no_cpu else 25000, ng)
     if "c2" in parts:
-        extras["nematic_sphere_n100000"] = bench_single(samos_amd, sc, "c2", 5000, 0 if args.no_cpu else 40, ng)
+        extras["nematic_sphere_n100000"] = bench_single(samos_amd, sc, "c2", 5000, 0 if args.no_cpu else 280, ng)
     if "main" not in parts:
         print(json.dumps(extras))
         return
