@@ -11,6 +11,10 @@ oracle's own expressions:
   BAOAB, gamma = 0, soft pair         energy conservation: exit speed
                                       (a - r0) sqrt(k / (2 m)) within O(dt^2)
   BAOAB noise with mass               equipartition <m v^2> = T per degree of freedom
+  SPV, constant force                 v_k = (a / gamma)(1 - exp(-gamma k dt)) exactly;
+  (integrator_langevin.cpp:126-182)   the reference's eta = 0 at gamma = 0 ignores forces
+  BBK, constant force                 v' = r v + dt a / (1 + g), r = (1 - g) / (1 + g),
+  (integrator_langevin.cpp:192-258)   g = gamma dt / 2; at gamma = 0 velocity Verlet
   plane rotate_director               (n0 x n1) . z = sin(phi), n0 . n1 = cos(phi)
   (constraint_plane.cpp:96-113)
   Brownian mu (integrator_brownian    drift (v0 + mu alpha) n t under self propulsion;
@@ -84,6 +88,62 @@ def test_baoab_constant_force_with_friction():
     assert s.get("vx")[0] == pytest.approx(vK, rel=1e-11)
     assert s.get("x")[0] == pytest.approx(xK, rel=1e-11)
     assert s.get("vy")[0] == 0.0
+
+
+# ------------------------------------------------------------ SPV, BBK -----
+def langevin_constant_force(method, gamma, m=2.0, alpha=0.8, dt=0.02, K=150):
+    s = one_particle()
+    director(s, 0.0)
+    s.set("mass", [m])
+    s.self_propulsion(alpha)
+    s.constraint("plane")
+    s.set_timestep(dt)
+    s.integrator_langevin(gamma=gamma, method=method, temp=(0, 0.0, 1.0, 1))
+    s.run(K - 1)
+    assert s.get("vy")[0] == 0.0
+    return s.get("vx")[0], s.get("x")[0]
+
+
+def test_spv_constant_force_velocity_is_the_exact_exponential():
+    # integrator_langevin.cpp:126-182: v' = c v + eta a with c = exp(-gamma dt),
+    # eta = (1 - c) / gamma, so v_k = (a / gamma)(1 - c^k) is the continuous
+    # solution at t = k dt; x' = x + h (v + v') with h = dt / 2
+    gamma, m, alpha, dt, K = 0.7, 2.0, 0.8, 0.02, 150
+    v, x = langevin_constant_force("spv", gamma, m, alpha, dt, K)
+    a, c, h = alpha / m, math.exp(-gamma * dt), 0.5 * dt
+    vK = a / gamma * (1.0 - c ** K)
+    S = a / gamma * ((K + 1) - (1.0 - c ** (K + 1)) / (1.0 - c))  # sum of v_k, k = 0..K
+    assert v == pytest.approx(a / gamma * (1.0 - math.exp(-gamma * K * dt)), rel=1e-12)
+    assert v == pytest.approx(vK, rel=1e-12)
+    assert x == pytest.approx(h * (2.0 * S - vK), rel=1e-11)
+
+
+def test_spv_without_friction_ignores_the_force():
+    # the reference sets eta = 0 at gamma = 0 (integrator_langevin.cpp:132-133),
+    # so a constant force never reaches the velocity
+    v, x = langevin_constant_force("spv", 0.0)
+    assert v == 0.0 and x == 0.0
+
+
+@pytest.mark.parametrize("gamma", [0.7, 0.0])
+def test_bbk_constant_force_geometric_velocity(gamma):
+    # integrator_langevin.cpp:192-258: v_half = (1 - g) v + h a, x' = x + dt v_half,
+    # v' = (v_half + h a) / (1 + g) with g = gamma dt / 2, h = dt / 2:
+    # v' = r v + dt a / (1 + g), r = (1 - g) / (1 + g)
+    m, alpha, dt, K = 2.0, 0.8, 0.02, 150
+    v, x = langevin_constant_force("bbk", gamma, m, alpha, dt, K)
+    a, h, g = alpha / m, 0.5 * dt, 0.5 * gamma * dt
+    if gamma == 0.0:
+        t = K * dt
+        assert v == pytest.approx(a * t, rel=1e-12)
+        assert x == pytest.approx(0.5 * a * t * t, rel=1e-11)
+        return
+    r = (1.0 - g) / (1.0 + g)
+    vs = a / gamma
+    vK = vs * (1.0 - r ** K)
+    sum_v = vs * (K - (1.0 - r ** K) / (1.0 - r))  # sum of v_k, k < K
+    assert v == pytest.approx(vK, rel=1e-12)
+    assert x == pytest.approx(dt * ((1.0 - g) * sum_v + K * h * a), rel=1e-11)
 
 
 def soft_pair_exit_speed(dt, m):
