@@ -574,7 +574,7 @@ def traffic_from_profiles():
 
 def bench_tiled_line(samos_amd, sc, args):
     """The same workload on the cell-tiled engine (state in cell order, LDS-staged
-    halos, stencil-bitmask lists): whole-step rate and its step kernel."""
+    halos, lists of 16-bit stage indices): whole-step rate and its step kernel."""
     s, st = build_system(samos_amd, sc, args.nside, args.pad, args.rebuild, args.batch)
     s.set_option("tile_cells", args.tile_cells)
     s.set_option("chunk_slots", args.chunk_slots)
@@ -597,7 +597,7 @@ def bench_tiled_line(samos_amd, sc, args):
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "kernel": "k_tile_step<LJ>", "avg_launch_us": avg_s * 1e6,
                          "launches_timed": launches},
-            "cells_per_tile": t["tile_cells"], "mask_words": t["mask_words"], "max_candidates": t["max_candidates"],
+            "cells_per_tile": t["tile_cells"], "list_capacity": t["list_capacity"], "max_neighbours": t["max_neighbours"],
             "max_halo": t["max_halo"]}
 
 
@@ -751,8 +751,9 @@ def main():
         },
         "nlist": {"builds": info["builds"], "stride": info["stride"], "rebuild_ms_total": rb_ms,
                   "rebuild_stage_launches": rb_launches},
-        "tiles": None if not args.tiles else {"cells_per_tile": tiles["tile_cells"], "mask_words": tiles["mask_words"],
-                                              "max_candidates": tiles["max_candidates"],
+        "tiles": None if not args.tiles else {"cells_per_tile": tiles["tile_cells"],
+                                              "list_capacity": tiles["list_capacity"],
+                                              "max_neighbours": tiles["max_neighbours"],
                                               "max_halo": tiles["max_halo"], "rebuilds": tiles["rebuilds"]},
     }
     out.update(extra)

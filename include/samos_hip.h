@@ -266,10 +266,10 @@ int samos_set_option(samos_ctx* ctx, const char* name, double value);
  * Morton re-sorts inside runs. */
 int samos_run_stats(samos_ctx* ctx, long long* out);
 /* Cell-tiled planar engine (no counterpart in the reference): planar fused
- * Langevin runs keep the state in cell order and the Verlet list as stencil
- * bitmasks (see SAMOS_RUN_TILES).  info[6]: tiled runs, rebuilds inside
- * them, largest stencil candidate count, largest staged halo, cells per tile
- * and mask words of the last run. */
+ * Langevin runs keep the state in cell order and the Verlet list as 16-bit
+ * indices into each tile's LDS-staged halo (see SAMOS_RUN_TILES).  info[6]:
+ * tiled runs, rebuilds inside them, largest neighbour count, largest staged
+ * halo, cells per tile and list capacity of the last run. */
 int samos_tile_info(samos_ctx* ctx, long long* info);
 
 /* ---- replica ensembles ---------------------------------------------------

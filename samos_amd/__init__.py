@@ -232,7 +232,7 @@ class System:
     def tile_info(self):
         a = (ctypes.c_longlong * 6)()
         self._chk(self.L.samos_tile_info(self.h, a))
-        keys = ["runs", "rebuilds", "max_candidates", "max_halo", "tile_cells", "mask_words"]
+        keys = ["runs", "rebuilds", "max_neighbours", "max_halo", "tile_cells", "list_capacity"]
         return dict(zip(keys, list(a)))
 
     def pairs(self):

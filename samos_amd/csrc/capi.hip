@@ -128,8 +128,10 @@ struct samos_ctx {
   // cell-tiled engine of planar fused runs (allocated on first use)
   TileD td{};
   void* tile_mem = nullptr;
+  TileStepArgs* td_args = nullptr;  // the step kernels' constant arguments (inside tile_mem)
+  TileStepArgs h_tile_args{};       // their host copy
   size_t tile_bytes = 0;
-  int tile_key[4] = {-1, -1, -1, -1};  // n, ncx, ncy, T of the allocation
+  int tile_key[5] = {-1, -1, -1, -1, -1};  // n, ncx, ncy, T, K of the allocation
   long long tile_stats[4] = {0, 0, 0, 0};  // tiled runs, rebuilds, largest candidate count, largest halo
   int tile_cells = 0;                      // cells per tile along x; 0: from the density
   // further pair potentials (morse, gaussian, yukawa, soft_attractive), by name
@@ -1139,11 +1141,15 @@ int tile_prepare(samos_ctx* ctx) {
   // about 224 particles per tile, so most tiles fill one 256-thread workgroup
   int T = ctx->tile_cells > 0 ? ctx->tile_cells : (int)std::lround(224.0 / std::max(mean, 1e-9));
   T = std::max(1, std::min(T, std::min(kTileMaxCells, ncx)));
-  // mask words: room for twice the mean stencil population
-  const int W = std::max(2, std::min(4, (int)std::ceil(2.0 * 9.0 * mean / 64.0)));
+  // automatic width: the fewest tiles per row of at most T cells, of even width
+  if (ctx->tile_cells <= 0) T = (ncx + (ncx + T - 1) / T - 1) / ((ncx + T - 1) / T);
+  // list capacity: twice the mean count within rcut + pad, plus 8, in rows of 4
+  const double area = ctx->box.Lx * ctx->box.Ly;
+  const double mean_nb = area > 0.0 ? M_PI * nl.cut2 * (double)n / area : 0.0;
+  const int K = 4 * std::max(2, (int)std::ceil((2.0 * mean_nb + 8.0) / 4.0));
   TileD& td = ctx->td;
-  const int key[4] = {n, ncx, ncy, T};
-  if (!(ctx->tile_mem && std::equal(key, key + 4, ctx->tile_key))) {
+  const int key[5] = {n, ncx, ncy, T, K};
+  if (!(ctx->tile_mem && std::equal(key, key + 5, ctx->tile_key))) {
     hipFree(ctx->tile_mem);
     ctx->tile_mem = nullptr;
     const int ntx = (ncx + T - 1) / T;
@@ -1151,9 +1157,10 @@ int tile_prepare(samos_ctx* ctx) {
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t d2 = al(sizeof(double2) * n), i4 = al(sizeof(int) * n);
     const size_t bytes = 2 * (2 * d2 + 3 * d2 + 2 * i4) + 2 * al(sizeof(int) * (ncell + 1)) +
-                         al(sizeof(unsigned long long) * 4 * (size_t)n) + al(sizeof(int) * ncell) +
+                         al(sizeof(ushort4) * (size_t)(K / 4) * (size_t)n) + i4 +
+                         al(sizeof(int) * kTileDesc * (size_t)ntiles) + al(sizeof(int) * ncell) +
                          al(sizeof(int) * ntiles) + al(sizeof(int) * (ntiles + 1)) + 2 * i4 + al(sizeof(TileCtl)) +
-                         al(2 * sizeof(TileSet));
+                         al(2 * sizeof(TileSet)) + al(sizeof(TileStepArgs));
     HIPCHK(hipMalloc(&ctx->tile_mem, bytes));
     ctx->tile_bytes = bytes;
     char* p = static_cast<char*>(ctx->tile_mem);
@@ -1169,17 +1176,20 @@ int tile_prepare(samos_ctx* ctx) {
     TileSet* d_sets = (TileSet*)take(al(2 * sizeof(TileSet)));
     HIPCHK(hipMemcpy(d_sets, sets, 2 * sizeof(TileSet), hipMemcpyHostToDevice));
     td.sets = d_sets;
-    td.mask = (unsigned long long*)take(al(sizeof(unsigned long long) * 4 * (size_t)n));
+    td.lst = (ushort4*)take(al(sizeof(ushort4) * (size_t)(K / 4) * (size_t)n));
+    td.len = (int*)take(i4);
+    td.desc = (int*)take(al(sizeof(int) * kTileDesc * (size_t)ntiles));
     td.count = (int*)take(al(sizeof(int) * ncell));
     td.tcount = (int*)take(al(sizeof(int) * ntiles));
     td.tstart = (int*)take(al(sizeof(int) * (ntiles + 1)));
     td.cell_of = (int*)take(i4);
     td.members = (int*)take(i4);
     td.ctl = (TileCtl*)take(al(sizeof(TileCtl)));
+    ctx->td_args = (TileStepArgs*)take(al(sizeof(TileStepArgs)));
     td.n = n; td.ncx = ncx; td.ncy = ncy; td.ncell = (int)ncell; td.T = T; td.ntx = ntx; td.ntiles = (int)ntiles;
-    std::copy(key, key + 4, ctx->tile_key);
+    td.K = K;
+    std::copy(key, key + 5, ctx->tile_key);
   }
-  td.W = W;
   HIPCHK(hipMemsetAsync(td.count, 0, sizeof(int) * (size_t)td.ncell, ctx->st));
   HIPCHK(hipMemsetAsync(td.tcount, 0, sizeof(int) * (size_t)td.ntiles, ctx->st));
   td.wx = nl.wx; td.wy = nl.wy; td.xlo = ctx->box.xlo; td.ylo = ctx->box.ylo;
@@ -1193,13 +1203,22 @@ int read_tile_ctl(samos_ctx* ctx, TileCtl* h) {
   return 0;
 }
 
-// par: the position buffer step t reads (its offset from the range start, mod 2).
-void tile_fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int t_off, int steps, bool f32, double mass_u,
-                     int par) {
-  FusedArgs fa;
+// The constant arguments of a tiled run's step kernels, uploaded to the
+// context's device copy before the run's first launch.
+int tile_upload_args(samos_ctx* ctx, Integ& I, double mass_u) {
+  // the host copy lives in the context, so the asynchronous upload never reads
+  // a released stack frame
+  TileStepArgs& A = ctx->h_tile_args;
+  A = TileStepArgs{};
+  A.td = ctx->td;
+  const Fields f = fields(ctx);
+  A.so = SlotOut{f.fx, f.fy, f.fz, f.ix, f.iy, f.iz};
+  A.b = ctx->box;
+  A.c = ctx->con;
+  FusedArgs& fa = A.fa;
   fa.dt = I.dt; fa.gamma = I.gamma; fa.temp = I.temp; fa.seed = I.seed;
-  fa.clock = ctx->d_clock; fa.clock_slot = I.slot; fa.t_off = t_off; fa.parity = 0;
-  fa.head_enforce = t + 1 <= steps ? 1 : 0;
+  fa.clock = ctx->d_clock; fa.clock_slot = I.slot; fa.t_off = 0; fa.parity = 0;
+  fa.head_enforce = 0;
   fa.mass_u = mass_u;
   // BAOAB O-step constants (src/integrators/integrator_langevin.cpp:57-58)
   fa.exp_dt = std::exp(-I.gamma * I.dt);
@@ -1208,9 +1227,7 @@ void tile_fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int t_off, int 
   fa.sf_u = fa.Bc >= 0.0 ? fa.Bc / std::sqrt(mass_u) : -1.0;
   fa.batch = 4;
   fa.age = ctx->d_age;
-  const Fields f = fields(ctx);
-  const SlotOut so{f.fx, f.fy, f.fz, f.ix, f.iy, f.iz};
-  PairConst pc{};
+  PairConst& pc = A.pc;
   const int nn = kMaxTypes * kMaxTypes;
   if (ctx->has_soft) { pc.soft_k = ctx->soft_tab[0]; pc.soft_a = ctx->soft_tab[nn]; }
   if (ctx->has_lj) {
@@ -1220,9 +1237,16 @@ void tile_fused_step(samos_ctx* ctx, Integ& I, int phase, int t, int t_off, int 
   }
   pc.has_sp = ctx->has_sp ? 1 : 0;
   pc.sp_alpha = ctx->sp_alpha;
+  HIPCHK(hipMemcpyAsync(ctx->td_args, &A, sizeof(A), hipMemcpyHostToDevice, ctx->st));
+  return 0;
+}
+
+// par: the position buffer step t reads (its offset from the range start, mod 2).
+void tile_fused_step(samos_ctx* ctx, int phase, int t, int t_off, int steps, bool f32, int par) {
   {
     ProfScope ps(ctx, "fused");
-    launch_tile_step(ctx->st, pot_mask(ctx), phase, f32 ? 1 : 0, ctx->td, so, ctx->box, ctx->con, pc, fa, par);
+    launch_tile_step(ctx->st, pot_mask(ctx), phase, f32 ? 1 : 0, ctx->td, ctx->td_args, par, t_off,
+                     t + 1 <= steps ? 1 : 0);
   }
   if (phase != 2) {
     ProfScope ps(ctx, "rebuild");
@@ -1242,9 +1266,10 @@ int run_tiled_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   launch_tile_build(ctx->st, ctx->td, ctx->box, 1, 1);
   TileCtl h;
   if (int r = read_tile_ctl(ctx, &h)) return r;
-  if (h.overflow || !(h.word & 2) || h.maxcand > 48 * ctx->td.W) return 1;  // too dense for the masks: list path
+  if (h.overflow || !(h.word & 2)) return 1;  // too dense for the lists: list path
   if (int r = upload_clock(ctx, a)) return r;
-  tile_fused_step(ctx, I, 0, a, 0, steps, f32, mass_u, 0);
+  if (int r = tile_upload_args(ctx, I, mass_u)) return r;
+  tile_fused_step(ctx, 0, a, 0, steps, f32, 0);
   launch_clock_advance(ctx->st, ctx->d_clock, 1);
   int t = a + 1;
   const int K = 64;  // steps per captured chunk
@@ -1253,7 +1278,7 @@ int run_tiled_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < K; ++j) tile_fused_step(ctx, I, 1, t + j, j, steps, f32, mass_u, (t + j - a) & 1);
+    for (int j = 0; j < K; ++j) tile_fused_step(ctx, 1, t + j, j, steps, f32, (t + j - a) & 1);
     launch_clock_advance(ctx->st, ctx->d_clock, K);
     HIPCHK(hipStreamEndCapture(ctx->st, &g));
     HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -1266,9 +1291,9 @@ int run_tiled_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     HIPCHK(hipGraphDestroy(g));
   }
   const int t0 = t;
-  for (; t < b; ++t) tile_fused_step(ctx, I, 1, t, t - t0, steps, f32, mass_u, (t - a) & 1);
+  for (; t < b; ++t) tile_fused_step(ctx, 1, t, t - t0, steps, f32, (t - a) & 1);
   launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
-  tile_fused_step(ctx, I, 2, b, 0, steps, f32, mass_u, (b - a) & 1);
+  tile_fused_step(ctx, 2, b, 0, steps, f32, (b - a) & 1);
   launch_tile_store(ctx->st, ctx->td, f, ctx->nl, (b - a) & 1);
   HIPCHK(hipGetLastError());
   if (int r = read_tile_ctl(ctx, &h)) return r;
@@ -1278,8 +1303,8 @@ int run_tiled_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   ctx->tile_stats[3] = std::max<long long>(ctx->tile_stats[3], h.maxhalo);
   if (h.far) return fail(ctx, "tiled run: a particle moved more than one cell width between neighbour-list builds");
   if (h.overflow)
-    return fail(ctx, "tiled run: a particle had more than " + std::to_string(64 * ctx->td.W) +
-                         " stencil candidates (largest " + std::to_string(h.maxcand) + ")");
+    return fail(ctx, "tiled run: a particle had more than " + std::to_string(ctx->td.K) +
+                         " neighbours (largest " + std::to_string(h.maxcand) + ")");
   if (h.rebuilds > 0) {
     // The slot-ordered list is rebuilt at the last build's snapshot, so it
     // holds exactly the pair set (and coordination) of that build.
@@ -2359,7 +2384,7 @@ int samos_sync(samos_ctx* ctx) {
 int samos_tile_info(samos_ctx* ctx, long long* info) {
   for (int k = 0; k < 4; ++k) info[k] = ctx->tile_stats[k];
   info[4] = ctx->tile_mem ? ctx->td.T : 0;
-  info[5] = ctx->tile_mem ? ctx->td.W : 0;
+  info[5] = ctx->tile_mem ? ctx->td.K : 0;
   return 0;
 }
 

@@ -12,10 +12,11 @@
 //     members of a cell contiguous), in two sets that a build swaps between;
 //   * a workgroup owns T consecutive cells of one row and stages the positions
 //     of the 3 x (T + 2) cells around them in LDS with coalesced loads;
-//   * a particle's Verlet list is a bitmask over its 3 x 3 stencil candidates
-//     in that staging order (bit k set: candidate k was within rcut + pad at the
-//     last build), so a step reads 8 * W bytes of list instead of 4 bytes per
-//     neighbour.
+//   * a particle's Verlet list holds the 16-bit stage indices of the
+//     candidates of its 3 x 3 stencil that were within rcut + pad at the last
+//     build (the stage's layout follows the cell starts, fixed between
+//     builds), so a step reads 2 bytes of list per neighbour and takes each
+//     neighbour position from LDS.
 // Launch order and control: the position buffer a step reads alternates with
 // the step and is a launch argument; which state set is current and whether a
 // build is requested live in one control word changed only by atomics (see
@@ -146,16 +147,60 @@ __device__ __forceinline__ int staged_global(const TileGeom& g, const int* s_hcs
   return s_hgs[h] + (k - s_hcs[h]);
 }
 
-// Own column (1 .. nc) of staged index k of the middle row: the last column
-// whose first staged index is <= k.
-__device__ __forceinline__ int own_column(const TileGeom& g, const int* s_hcs, int k) {
-  const int base = g.nc + 2;
-  int lo = 1, hi = g.nc;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (s_hcs[base + mid] <= k) lo = mid; else hi = mid - 1;
+// The same for a tile whose halo rows are contiguous in the set's cell order
+// (no periodic wrap along x inside the halo): row r is one run.
+__device__ __forceinline__ int staged_global_rows(const TileGeom& g, const int* s_hcs, const int* s_hgs, int k) {
+  const int row1 = g.nc + 2;
+  const int r = (k >= s_hcs[row1] ? 1 : 0) + (k >= s_hcs[2 * row1] ? 1 : 0);
+  return s_hgs[r * row1] + (k - s_hcs[r * row1]);
+}
+
+// Loads and stores through the global address space (the set's arrays are
+// reached through pointers read from memory, which the compiler would
+// otherwise access with flat instructions).
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)(p);
+}
+typedef double gdouble2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 gld(const double2* p) {
+  const gdouble2 v = *(const __attribute__((address_space(1))) gdouble2*)(p);
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void gst(double2* p, const double2& a) {
+  gdouble2 v;
+  v.x = a.x;
+  v.y = a.y;
+  *(__attribute__((address_space(1))) gdouble2*)(p) = v;
+}
+
+// A tile's stage descriptor, read with scalar loads (uniform over the
+// workgroup, written by an earlier kernel).
+typedef const __attribute__((address_space(4))) int* DescPtr;
+__device__ __forceinline__ DescPtr tile_desc(const TileD& td, int t) {
+  return (DescPtr)(td.desc + (long)t * kTileDesc);
+}
+
+// Set index of stage entry k: the last run starting at or before k.  The
+// runs past the last hold INT_MAX as their first stage entry (the builds
+// write every slot), so a fixed, unrolled search needs no run count.
+__device__ __forceinline__ int desc_global(DescPtr D, int k) {
+  int so = D[5], gs = D[6];
+#pragma unroll
+  for (int r = 1; r < kTileRuns; ++r) {
+    const int o = D[5 + 2 * r];
+    if (k >= o) { so = o; gs = D[6 + 2 * r]; }
   }
-  return lo;
+  return gs + (k - so);
+}
+
+// The same for a stage of at most three runs (a tile whose halo does not wrap
+// along x: one run per row).
+__device__ __forceinline__ int desc_global3(DescPtr D, int k) {
+  int so = D[5], gs = D[6];
+  if (k >= D[7]) { so = D[7]; gs = D[8]; }
+  if (k >= D[9]) { so = D[9]; gs = D[10]; }
+  return gs + (k - so);
 }
 
 // ---------------------------------------------------------------------------
@@ -311,8 +356,8 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_sort(TileD td, int par) {
 }
 
 // Build, kernel 3 (per tile): stage the snapshot positions of the halo in new
-// order, write the new set for the tile's own particles and their stencil
-// masks (NeighbourList::build_cell's test, src/system/neighbour_list.cpp:
+// order, write the new set for the tile's own particles and their lists of
+// stage indices (NeighbourList::build_cell's test, src/system/neighbour_list.cpp:
 // 240-262: dx = x_i - x_j, single-wrap image, d2 = dx*dx + dy*dy + dz*dz,
 // kept when d2 < (rcut + pad)^2).  The current positions are in buffer
 // par ^ 1, and the new set's go there too (the next step reads it).  Tiles
@@ -333,6 +378,30 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_build(TileD td, BoxD b, i
   const int* csn = td.sets[N].cstart;
   const int total = stage_index(td, g, csn, s_hcs, s_hgs, s_scan);
   const bool staged = total <= kTileHaloMax;
+  over |= g.active && total > kTileStageMax;
+  if (g.active && threadIdx.x == 0) {
+    // the stage as runs of consecutive set indices (empty cells skipped,
+    // neighbouring cells of the set's order merged)
+    int* D = td.desc + (long)t * kTileDesc;
+    const int row1 = g.nc + 2;
+    D[0] = total;
+    D[1] = s_hcs[row1 + 1];
+    D[2] = s_hcs[row1 + g.nc + 1];
+    D[3] = s_hgs[row1 + 1];
+    int nr = 0, end = -1;
+    for (int h = 0; h < g.nh; ++h) {
+      const int cnt = s_hcs[h + 1] - s_hcs[h];
+      if (cnt == 0) continue;
+      if (nr > 0 && s_hgs[h] == end) { end += cnt; continue; }
+      D[5 + 2 * nr] = s_hcs[h];
+      D[6 + 2 * nr] = s_hgs[h];
+      end = s_hgs[h] + cnt;
+      ++nr;
+    }
+    if (nr == 0) { D[5] = 0; D[6] = 0; nr = 1; }
+    D[4] = nr;
+    for (int r = nr; r < kTileRuns; ++r) { D[5 + 2 * r] = 0x7fffffff; D[6 + 2 * r] = 0; }
+  }
   const double2* snap_src = init ? td.sets[S].o : td.sets[S].xy[par];
   if (g.active && staged) {
     for (int k = threadIdx.x; k < total; k += kTileThreads)
@@ -345,6 +414,7 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_build(TileD td, BoxD b, i
     const int kb = s_hcs[row1 + 1], ke = s_hcs[row1 + g.nc + 1];
     const int qa = s_hgs[row1 + 1];
     const double zero = 0.0;
+    over |= ke - kb > kTileMaxOwn;  // the step kernels take at most this many per tile
     for (int k = kb + (int)threadIdx.x; k < ke; k += kTileThreads) {
       const int q = qa + (k - kb);
       const int mem = td.members[q];
@@ -356,8 +426,10 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_build(TileD td, BoxD b, i
       td.sets[N].id[q] = td.sets[S].id[mem];
       td.sets[N].slot[q] = td.sets[S].slot[mem];
       const int lc = (td.cell_of[mem] % td.ncx) - g.i0 + 1;
-      unsigned long long m[4] = {0ULL, 0ULL, 0ULL, 0ULL};
-      int base = 0;
+      // entries in ascending stage order, four per 8-byte row
+      unsigned long long* L = reinterpret_cast<unsigned long long*>(td.lst) + q;
+      unsigned long long acc = 0ULL;
+      int cnt = 0;
       for (int r = 0; r < 3; ++r) {
         const int a = s_hcs[r * row1 + lc - 1], e = s_hcs[r * row1 + lc + 2];
         for (int kk = a; kk < e; ++kk) {
@@ -366,14 +438,27 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_build(TileD td, BoxD b, i
           double dx = si.x - sj.x, dy = si.y - sj.y, dz = zero;
           min_image(b, dx, dy, dz);
           const double d2 = dx * dx + dy * dy + dz * dz;
-          const int bit = base + (kk - a);
-          if (d2 < td.cut2 && bit < 64 * td.W) m[bit >> 6] |= 1ULL << (bit & 63);
+          if (d2 < td.cut2) {
+            if (cnt < td.K) {
+              acc |= (unsigned long long)kk << (16 * (cnt & 3));
+              if ((cnt & 3) == 3) {
+                L[(long)(cnt >> 2) * td.n] = acc;
+                acc = 0ULL;
+              }
+            }
+            ++cnt;
+          }
         }
-        base += e - a;
       }
-      over |= base > 64 * td.W;
-      cmax = max(cmax, base);
-      for (int w = 0; w < td.W; ++w) td.mask[(long)w * td.n + q] = m[w];
+      const int c = min(cnt, td.K);
+      if (c & 3) {
+        // pad the last row with the sentinel (the stage's slot after the halo)
+        for (int u = c & 3; u < 4; ++u) acc |= (unsigned long long)total << (16 * u);
+        L[(long)(c >> 2) * td.n] = acc;
+      }
+      td.len[q] = c;
+      over |= cnt > td.K;
+      cmax = max(cmax, cnt);
     }
     if ((int)threadIdx.x < g.nc) td.count[g.ty * td.ncx + g.i0 + threadIdx.x] = 0;
     if (threadIdx.x == 0) td.tcount[t] = 0;
@@ -393,208 +478,261 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_build(TileD td, BoxD b, i
 // Per-pair force terms of a planar system (dz = 0), as pair_force_sum in
 // pair_sum.hpp with one particle type: d = x_j - x_i.
 // ---------------------------------------------------------------------------
+// Constants of the pair loop: the pair parameters and the box's image
+// thresholds.  list_forces keeps them in vector registers (an empty asm with
+// a "+v" operand): they are uniform, but held in scalar registers across the
+// pair loop they would be spilled and reloaded lane by lane.
+struct PairK {
+  double soft_k, soft_a, lj_rc2, lj_sig2, lj_e48;
+  double xlo, xhi, lx, ylo, yhi, ly;
+};
+
+__device__ __forceinline__ PairK pair_k(const PairConst& pc, const BoxD& b) {
+  PairK k;
+  k.soft_k = pc.soft_k; k.soft_a = pc.soft_a;
+  k.lj_rc2 = pc.lj_rc2; k.lj_sig2 = pc.lj_sig2; k.lj_e48 = 48.0 * pc.lj_eps;
+  k.xlo = b.xlo; k.xhi = b.xhi; k.lx = b.Lx; k.ylo = b.ylo; k.yhi = b.yhi; k.ly = b.Ly;
+  return k;
+}
+
 template <int POT, bool F32>
-__device__ __forceinline__ void pair_term(const PairConst& pc, double dx, double dy, double& fx, double& fy) {
+__device__ __forceinline__ void pair_term(const PairK& pk, double dx, double dy, double& fx, double& fy) {
   // dz = 0: adding +0 to a sum of squares changes no bit
   const double r_sq = dx * dx + dy * dy;
   if (POT & POT_SOFT) {
     if (F32) {
       const float r = sqrtf((float)r_sq);
-      if (r < (float)pc.soft_a) {
-        const float diff = (float)pc.soft_a - r;
-        const float ff = (r > 0.0f) ? (float)pc.soft_k * diff / r : (float)pc.soft_k * diff;
+      if (r < (float)pk.soft_a) {
+        const float diff = (float)pk.soft_a - r;
+        const float ff = (r > 0.0f) ? (float)pk.soft_k * diff / r : (float)pk.soft_k * diff;
         fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy);
       }
     } else {
       const double r = sqrt(r_sq);
-      if (r < pc.soft_a) {
-        const double diff = pc.soft_a - r;
-        const double ff = (r > 0.0) ? pc.soft_k * diff / r : pc.soft_k * diff;
+      if (r < pk.soft_a) {
+        const double diff = pk.soft_a - r;
+        const double ff = (r > 0.0) ? pk.soft_k * diff / r : pk.soft_k * diff;
         fx -= ff * dx; fy -= ff * dy;
       }
     }
   }
   if (POT & POT_LJ) {
-    if (r_sq <= pc.lj_rc2) {
+    if (r_sq <= pk.lj_rc2) {
       if (F32) {
-        const float s2 = (float)pc.lj_sig2 / (float)r_sq;
+        const float s2 = (float)pk.lj_sig2 / (float)r_sq;
         const float s6 = s2 * s2 * s2;
-        const float ff = 48.0f * (float)pc.lj_eps * s6 * (s6 - 0.5f) * s2;
+        const float ff = (float)pk.lj_e48 * s6 * (s6 - 0.5f) * s2;
         fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy);
       } else {
-        const double inv_r_sq = lj_inv_r_sq(pc.lj_sig2, r_sq);
+        const double inv_r_sq = lj_inv_r_sq(pk.lj_sig2, r_sq);
         const double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
-        const double ff = 48.0 * pc.lj_eps * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
+        const double ff = pk.lj_e48 * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
         fx -= ff * dx; fy -= ff * dy;
       }
     }
   }
 }
 
-// Candidate ranges of a particle in staged order: row r's three stencil
-// cells are staged indices [a_r, e_r); mask bit b is candidate b of the
-// concatenation (rows 0, 1, 2).
-struct Stencil {
-  int a0, a1, a2, c0, c1;
-  __device__ __forceinline__ int staged(int bit) const {
-    int off = a0;
-    off = bit >= c0 ? a1 - c0 : off;
-    off = bit >= c1 ? a2 - c1 : off;
-    return off + bit;
-  }
-};
+// Single-wrap minimum image of a planar pair (src/system/system.cpp:1083-1095).
+__device__ __forceinline__ void min_image_k(const PairK& pk, double& dx, double& dy) {
+  if (dx > pk.xhi) dx -= pk.lx; else if (dx < pk.xlo) dx += pk.lx;
+  if (dy > pk.yhi) dy -= pk.ly; else if (dy < pk.ylo) dy += pk.ly;
+}
 
-// Forces on one particle from its masked candidates, kB candidates per batch
-// so that kB staged reads are in flight together.  A batch slot without a
-// candidate reads the particle's own row start and is moved out of range.
-// WRAP: the tile's halo crosses the periodic boundary, so the single-wrap image
-// is applied (inside the box it never changes a difference and is skipped).
-// STAGED: candidates come from LDS, else from the set's positions (L2).
-template <int POT, bool F32, bool WRAP, bool STAGED>
-__device__ __forceinline__ void masked_forces(const BoxD& b, const PairConst& pc, const Stencil& sc,
-                                              const unsigned long long* mw, int W, const double2* src,
-                                              const TileGeom& g, const int* s_hcs, const int* s_hgs, double xi,
-                                              double yi, double& fx, double& fy) {
-  constexpr int kB = 4;
-  for (int w = 0; w < W; ++w) {
-    unsigned long long m = mw[w];
-    while (m) {
-      int k[kB];
-      bool ok[kB];
+// Forces on particle q from its list: one 8-byte load brings four stage
+// indices (the next row's load is issued before the current row's reads),
+// then four staged positions are read from LDS together.  The padding of the
+// last row is the stage's sentinel slot, a position so far away that no pair
+// term is formed with it.  wrap: the tile's halo crosses the periodic
+// boundary, so the single-wrap image is applied (inside the box it never
+// changes a difference and is skipped); uniform over the workgroup.
+template <int POT, bool F32>
+__device__ __forceinline__ void list_forces(PairK pk, const TileD& td, int q, int kn, const double2* s_pos, bool wrap,
+                                            double xi, double yi, double& fx, double& fy) {
+  asm volatile("" : "+v"(pk.lj_rc2), "+v"(pk.lj_sig2), "+v"(pk.lj_e48), "+v"(pk.soft_k), "+v"(pk.soft_a));
+  asm volatile("" : "+v"(pk.xlo), "+v"(pk.xhi), "+v"(pk.lx), "+v"(pk.ylo), "+v"(pk.yhi), "+v"(pk.ly));
+  const unsigned long long* L = reinterpret_cast<const unsigned long long*>(td.lst) + q;
+  const long n = td.n;
+  const int rows = (kn + 3) >> 2;
+  unsigned long long nxt = rows > 0 ? gld(L) : 0ULL;
+  for (int r = 0; r < rows; ++r) {
+    const unsigned long long cur = nxt;
+    if (r + 1 < rows) nxt = gld(L + (long)(r + 1) * n);
+    double2 pj[4];
 #pragma unroll
-      for (int u = 0; u < kB; ++u) {
-        ok[u] = m != 0ULL;
-        const int bit = 64 * w + __builtin_ctzll(m | (1ULL << 63));
-        m &= m - 1;
-        k[u] = ok[u] ? sc.staged(bit) : sc.a1;
-      }
-      double2 pj[kB];
+    for (int u = 0; u < 4; ++u) pj[u] = s_pos[(int)((cur >> (16 * u)) & 0xffffULL)];
 #pragma unroll
-      for (int u = 0; u < kB; ++u) pj[u] = STAGED ? src[k[u]] : src[staged_global(g, s_hcs, s_hgs, k[u])];
-#pragma unroll
-      for (int u = 0; u < kB; ++u) {
-        double dx = ok[u] ? pj[u].x - xi : 1.0e30, dy = pj[u].y - yi;
-        if (WRAP) {
-          double dz = 0.0;
-          min_image(b, dx, dy, dz);
-        }
-        pair_term<POT, F32>(pc, dx, dy, fx, fy);
-      }
+    for (int u = 0; u < 4; ++u) {
+      double dx = pj[u].x - xi, dy = pj[u].y - yi;
+      if (wrap) min_image_k(pk, dx, dy);
+      pair_term<POT, F32>(pk, dx, dy, fx, fy);
     }
   }
 }
 
-// One fused BAOAB step on the current set: phases as k_baoab_fused.
+// The same for a tile whose halo is larger than the LDS stage: each listed
+// position is read from the set through L2, one entry at a time.
+template <int POT, bool F32>
+__device__ __forceinline__ void list_forces_l2(const PairK& pk, const TileD& td, int q, int kn, const double2* cur,
+                                               DescPtr D, int nr, double xi, double yi, double& fx, double& fy) {
+  const unsigned short* L = reinterpret_cast<const unsigned short*>(td.lst);
+#pragma unroll 1
+  for (int k = 0; k < kn; ++k) {
+    const int e = L[((long)(k >> 2) * td.n + q) * 4 + (k & 3)];
+    const double2 pj = gld(cur + desc_global(D, e));
+    double dx = pj.x - xi, dy = pj.y - yi;
+    min_image_k(pk, dx, dy);
+    pair_term<POT, F32>(pk, dx, dy, fx, fy);
+  }
+}
+
+// One particle of a fused BAOAB step (phases as k_baoab_fused): k is its
+// stage index, q its index in the set.  Returns the rebuild request.
 template <int POT, int PHASE, bool F32>
-__global__ void __launch_bounds__(kTileThreads) k_tile_step(TileD td, SlotOut so, BoxD b, ConstraintD c,
-                                                            PairConst pc, FusedArgs fa, int par) {
+__device__ __forceinline__ bool tile_particle(const TileD& td, const TileSet& A, const SlotOut& so, const BoxD& b,
+                                              const ConstraintD& c, const PairConst& pc, const FusedArgs& fa, int par,
+                                              const double2* cur, const double2* s_pos, bool staged, bool wrap,
+                                              DescPtr D, int nr, int k, int q) {
+  const double dt = fa.dt, dt2 = 0.5 * dt, m = fa.mass_u, fact = fa.fact_u;
+  const int kn = gld(td.len + q);
+  double2 pi;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  if (staged) {
+    pi = s_pos[k];
+    list_forces<POT, F32>(pair_k(pc, b), td, q, kn, s_pos, wrap, pi.x, pi.y, fx, fy);
+  } else {
+    pi = gld(cur + q);
+    list_forces_l2<POT, F32>(pair_k(pc, b), td, q, kn, cur, D, nr, pi.x, pi.y, fx, fy);
+  }
+  // On a plane only the enforce's normalisation changes a director in a
+  // Langevin run, and the first step of a run (phase 0) normalises them all:
+  // later steps read a director only for self propulsion.
+  constexpr bool kDirs = PHASE == 0;
+  const double2 n0 = (kDirs || pc.has_sp) ? gld(A.nd + q) : make_double2(0.0, 0.0);
+  if (pc.has_sp) { fx += pc.sp_alpha * n0.x; fy += pc.sp_alpha * n0.y; fz += pc.sp_alpha * 0.0; }
+  const double2 v0 = gld(A.v + q);
+  double vx = v0.x, vy = v0.y, vz = 0.0;
+  if (PHASE >= 1) {  // closing B of step t-1
+    vx += fact * fx; vy += fact * fy; vz += fact * fz;
+  }
+  if (PHASE <= 1) {
+    // the O-step noise; the z draw only reaches v_z, which the plane enforce
+    // resets, so only the first pair of the block is formed
+    double gn[2] = {0.0, 0.0};
+    double B = fa.Bc;
+    if (B < 0.0) {
+      const double T = fa.temp.at(eff_step(fa.clock, fa.t_off));
+      B = sqrt(T * (1.0 - exp(-2.0 * fa.gamma * dt)));
+    }
+    if (B != 0.0)
+      gauss2(fa.seed, TAG_LANGEVIN, (uint32_t)gld(A.id + q),
+             eff_call(fa.clock, fa.clock_slot, 0ULL) + (unsigned long long)fa.t_off, gn);
+    PS p;
+    p.x = pi.x; p.y = pi.y; p.z = c.zpos; p.vx = vx; p.vy = vy; p.vz = vz;
+    p.nx = n0.x; p.ny = n0.y; p.nz = 0.0;
+    p.dix = p.diy = p.diz = 0;
+    p.vx += fact * fx; p.vy += fact * fy; p.vz += fact * fz;
+    p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+    enforce<kSurfPlane>(b, c, p, kDirs);
+    p.vx *= fa.exp_dt; p.vy *= fa.exp_dt; p.vz *= fa.exp_dt;
+    if (B != 0.0) {
+      const double sfac = fa.sf_u >= 0.0 ? fa.sf_u : B / sqrt(m);
+      p.vx += sfac * gn[0]; p.vy += sfac * gn[1]; p.vz += sfac * 0.0;
+    }
+    p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
+    // the director was normalised by the first enforce of the step
+    enforce<kSurfPlane>(b, c, p, false);
+    if (fa.head_enforce) enforce<kSurfPlane>(b, c, p, false);
+    gst(A.xy[par ^ 1] + q, make_double2(p.x, p.y));
+    gst(A.v + q, make_double2(p.vx, p.vy));
+    if (kDirs && (p.nx != n0.x || p.ny != n0.y)) gst(A.nd + q, make_double2(p.nx, p.ny));
+    if (p.dix | p.diy | p.diz) {
+      const int s = A.slot[q];
+      if (p.dix) so.ix[s] += p.dix;
+      if (p.diy) so.iy[s] += p.diy;
+      if (p.diz) so.iz[s] += p.diz;
+    }
+    const double2 o = gld(A.o + q);
+    double ddx = o.x - p.x, ddy = o.y - p.y, ddz = 0.0;
+    min_image(b, ddx, ddy, ddz);
+    return !(ddx * ddx + ddy * ddy + ddz * ddz < td.pad2q);
+  }
+  // closing phase: F(x_t) is the state the reference leaves behind
+  A.v[q] = make_double2(vx, vy);
+  const int s = A.slot[q];
+  so.fx[s] = fx; so.fy[s] = fy; so.fz[s] = fz;
+  return false;
+}
+
+// One fused BAOAB step on the current set: phases as k_baoab_fused.
+// The run's constant arguments come from device memory, read with scalar
+// loads where they are used: as kernel arguments they would occupy scalar
+// registers for the whole kernel and be spilled and reloaded lane by lane.
+template <int POT, int PHASE, bool F32>
+__global__ void __launch_bounds__(kTileThreads, 8) k_tile_step(const TileStepArgs* __restrict__ P, int par,
+                                                               int t_off, int head_enforce) {
+  const TileD& td = P->td;
+  const SlotOut& so = P->so;
+  const BoxD& b = P->b;
+  const ConstraintD& c = P->c;
+  const PairConst& pc = P->pc;
+  FusedArgs fa = P->fa;
+  fa.t_off = t_off;
+  fa.head_enforce = head_enforce;
   TileCtl* ctl = td.ctl;
   const int w = ctl_word(ctl);
   const int S = (w ^ (w >> 1)) & 1;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
     // commit a finished build, clear the request it answered
     if (w & 2) atomicXor(&ctl->word, 3);
     atomicAnd(&ctl->word, ~(4 << (par ^ 1)));
     if (ctl->init) ctl->init = 0;
   }
   const TileSet& A = td.sets[S];
-  __shared__ double2 s_pos[kTileHaloMax];
-  __shared__ int s_hcs[3 * (kTileMaxCells + 2) + 1];
-  __shared__ int s_hgs[3 * (kTileMaxCells + 2)];
-  __shared__ int s_scan[kTileThreads / 64];
-  const TileGeom g = tile_geom(td, xcd_tile(td));
+  __shared__ double2 s_pos[kTileHaloMax + 1];
+  const int t = xcd_tile(td);
+  const TileGeom g = tile_geom(td, t);
+  if (!g.active) return;
+  const DescPtr D = tile_desc(td, t);
+  const int total = D[0], kb = D[1], ke = D[2], qa = D[3], nr = D[4];
+  // Workgroup (t, r) takes the tile's particles r * kTileThreads .. + kTileThreads - 1
+  // (one per thread, no loop: a loop would keep the step's loop-invariant
+  // values in registers across the pair loop).  A later-round workgroup
+  // whose tile holds no particle of its round returns at once.
+  const int round = (int)blockIdx.y;
+  if (round > 0 && kb + round * kTileThreads >= ke) return;
   const double2* cur = A.xy[par];
-  const int total = stage_index(td, g, A.cstart, s_hcs, s_hgs, s_scan);
   const bool staged = total <= kTileHaloMax;
-  if (g.active && staged) {
-    for (int k = threadIdx.x; k < total; k += kTileThreads) s_pos[k] = cur[staged_global(g, s_hcs, s_hgs, k)];
+  if (staged) {
+    // every load of the stage issued before the first LDS store
+    double2 v[kTileHaloMax / kTileThreads];
+    if (nr <= 3) {
+#pragma unroll
+      for (int u = 0; u < kTileHaloMax / kTileThreads; ++u) {
+        const int k = (int)threadIdx.x + u * kTileThreads;
+        if (k < total) v[u] = gld(cur + desc_global3(D, k));
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kTileHaloMax / kTileThreads; ++u) {
+        const int k = (int)threadIdx.x + u * kTileThreads;
+        if (k < total) v[u] = gld(cur + desc_global(D, k));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kTileHaloMax / kTileThreads; ++u) {
+      const int k = (int)threadIdx.x + u * kTileThreads;
+      if (k < total) s_pos[k] = v[u];
+    }
+    if (threadIdx.x == 0) s_pos[total] = make_double2(1.0e30, 1.0e30);  // the lists' padding
   }
   __syncthreads();
   const bool wrap = b.periodic && (g.i0 == 0 || g.i0 + g.nc >= td.ncx || g.ty == 0 || g.ty == td.ncy - 1);
   bool need = false;
-  if (g.active) {
-    const int row1 = g.nc + 2;
-    const int kb = s_hcs[row1 + 1], ke = s_hcs[row1 + g.nc + 1];
-    const int qa = s_hgs[row1 + 1];
-    const double dt = fa.dt, dt2 = 0.5 * dt, m = fa.mass_u, fact = fa.fact_u;
-    for (int k = kb + (int)threadIdx.x; k < ke; k += kTileThreads) {
-      const int q = qa + (k - kb);
-      unsigned long long mw[4];
-#pragma unroll
-      for (int w = 0; w < 4; ++w) mw[w] = w < td.W ? td.mask[(long)w * td.n + q] : 0ULL;
-      const double2 pi = staged ? s_pos[k] : cur[q];
-      const int lc = own_column(g, s_hcs, k);
-      Stencil sc;
-      sc.a0 = s_hcs[lc - 1];
-      sc.a1 = s_hcs[row1 + lc - 1];
-      sc.a2 = s_hcs[2 * row1 + lc - 1];
-      sc.c0 = s_hcs[lc + 2] - sc.a0;
-      sc.c1 = sc.c0 + (s_hcs[row1 + lc + 2] - sc.a1);
-      // noise first, so its temporaries are dead before the force loop; the z
-      // draw only reaches v_z, which the plane enforce resets
-      double gn[2] = {0.0, 0.0};
-      double B = fa.Bc;
-      if (PHASE <= 1) {
-        if (B < 0.0) {
-          const double T = fa.temp.at(eff_step(fa.clock, fa.t_off));
-          B = sqrt(T * (1.0 - exp(-2.0 * fa.gamma * dt)));
-        }
-        if (B != 0.0)
-          gauss2(fa.seed, TAG_LANGEVIN, (uint32_t)A.id[q],
-                 eff_call(fa.clock, fa.clock_slot, 0ULL) + (unsigned long long)fa.t_off, gn);
-      }
-      double fx = 0.0, fy = 0.0, fz = 0.0;
-      if (staged) {
-        if (wrap) masked_forces<POT, F32, true, true>(b, pc, sc, mw, td.W, s_pos, g, s_hcs, s_hgs, pi.x, pi.y, fx, fy);
-        else masked_forces<POT, F32, false, true>(b, pc, sc, mw, td.W, s_pos, g, s_hcs, s_hgs, pi.x, pi.y, fx, fy);
-      } else {
-        masked_forces<POT, F32, true, false>(b, pc, sc, mw, td.W, cur, g, s_hcs, s_hgs, pi.x, pi.y, fx, fy);
-      }
-      const double2 n0 = A.nd[q];
-      if (pc.has_sp) { fx += pc.sp_alpha * n0.x; fy += pc.sp_alpha * n0.y; fz += pc.sp_alpha * 0.0; }
-      const double2 v0 = A.v[q];
-      double vx = v0.x, vy = v0.y, vz = 0.0;
-      if (PHASE >= 1) {  // closing B of step t-1
-        vx += fact * fx; vy += fact * fy; vz += fact * fz;
-      }
-      if (PHASE <= 1) {
-        PS p;
-        p.x = pi.x; p.y = pi.y; p.z = c.zpos; p.vx = vx; p.vy = vy; p.vz = vz;
-        p.nx = n0.x; p.ny = n0.y; p.nz = 0.0;
-        p.dix = p.diy = p.diz = 0;
-        p.vx += fact * fx; p.vy += fact * fy; p.vz += fact * fz;
-        p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
-        enforce<kSurfPlane>(b, c, p);
-        p.vx *= fa.exp_dt; p.vy *= fa.exp_dt; p.vz *= fa.exp_dt;
-        if (B != 0.0) {
-          const double sfac = fa.sf_u >= 0.0 ? fa.sf_u : B / sqrt(m);
-          p.vx += sfac * gn[0]; p.vy += sfac * gn[1]; p.vz += sfac * 0.0;
-        }
-        p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
-        // the director was normalised by the first enforce of the step
-        enforce<kSurfPlane>(b, c, p, false);
-        if (fa.head_enforce) enforce<kSurfPlane>(b, c, p, false);
-        A.xy[par ^ 1][q] = make_double2(p.x, p.y);
-        A.v[q] = make_double2(p.vx, p.vy);
-        if (p.nx != n0.x || p.ny != n0.y) A.nd[q] = make_double2(p.nx, p.ny);
-        if (p.dix | p.diy | p.diz) {
-          const int s = A.slot[q];
-          if (p.dix) so.ix[s] += p.dix;
-          if (p.diy) so.iy[s] += p.diy;
-          if (p.diz) so.iz[s] += p.diz;
-        }
-        const double2 o = A.o[q];
-        double ddx = o.x - p.x, ddy = o.y - p.y, ddz = 0.0;
-        min_image(b, ddx, ddy, ddz);
-        need |= !(ddx * ddx + ddy * ddy + ddz * ddz < td.pad2q);
-      } else {
-        // closing phase: F(x_t) is the state the reference leaves behind
-        A.v[q] = make_double2(vx, vy);
-        const int s = A.slot[q];
-        so.fx[s] = fx; so.fy[s] = fy; so.fz[s] = fz;
-      }
-    }
-  }
+  const int k = kb + round * kTileThreads + (int)threadIdx.x;
+  if (k < ke)
+    need = tile_particle<POT, PHASE, F32>(td, A, so, b, c, pc, fa, par, cur, s_pos, staged, wrap, D, nr, k,
+                                          qa + (k - kb));
   if (PHASE != 2) {
     if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomicOr(&ctl->word, 4 << par);
   }
@@ -644,19 +782,19 @@ int capped_grid(long n) {
 int tile_grid(const TileD& td) { return 8 * ((td.ntiles + 7) / 8); }
 
 template <int POT, int PHASE>
-void step_dispatch(hipStream_t st, int fp32, TileD td, SlotOut so, BoxD b, ConstraintD c, PairConst pc,
-                   const FusedArgs& fa, int par) {
-  const dim3 g(tile_grid(td)), blk(kTileThreads);
-  if (fp32) hipLaunchKernelGGL((k_tile_step<POT, PHASE, true>), g, blk, 0, st, td, so, b, c, pc, fa, par);
-  else hipLaunchKernelGGL((k_tile_step<POT, PHASE, false>), g, blk, 0, st, td, so, b, c, pc, fa, par);
+void step_dispatch(hipStream_t st, int fp32, const TileStepArgs* args, int grid, int par, int t_off,
+                   int head_enforce) {
+  const dim3 g(grid, kTileMaxOwn / kTileThreads), blk(kTileThreads);
+  if (fp32) hipLaunchKernelGGL((k_tile_step<POT, PHASE, true>), g, blk, 0, st, args, par, t_off, head_enforce);
+  else hipLaunchKernelGGL((k_tile_step<POT, PHASE, false>), g, blk, 0, st, args, par, t_off, head_enforce);
 }
 
 template <int POT>
-void step_phase(hipStream_t st, int phase, int fp32, TileD td, SlotOut so, BoxD b, ConstraintD c, PairConst pc,
-                const FusedArgs& fa, int par) {
-  if (phase == 0) step_dispatch<POT, 0>(st, fp32, td, so, b, c, pc, fa, par);
-  else if (phase == 1) step_dispatch<POT, 1>(st, fp32, td, so, b, c, pc, fa, par);
-  else step_dispatch<POT, 2>(st, fp32, td, so, b, c, pc, fa, par);
+void step_phase(hipStream_t st, int phase, int fp32, const TileStepArgs* args, int grid, int par, int t_off,
+                int head_enforce) {
+  if (phase == 0) step_dispatch<POT, 0>(st, fp32, args, grid, par, t_off, head_enforce);
+  else if (phase == 1) step_dispatch<POT, 1>(st, fp32, args, grid, par, t_off, head_enforce);
+  else step_dispatch<POT, 2>(st, fp32, args, grid, par, t_off, head_enforce);
 }
 
 }  // namespace
@@ -677,12 +815,13 @@ void launch_tile_build(hipStream_t st, TileD td, BoxD box, int init, int par) {
   hipLaunchKernelGGL(k_tile_build, dim3(g), dim3(kTileThreads), 0, st, td, box, par);
 }
 
-void launch_tile_step(hipStream_t st, int pot_mask, int phase, int fp32, TileD td, SlotOut out, BoxD box,
-                      ConstraintD con, PairConst pc, const FusedArgs& fa, int par) {
+void launch_tile_step(hipStream_t st, int pot_mask, int phase, int fp32, const TileD& td, const TileStepArgs* args,
+                      int par, int t_off, int head_enforce) {
+  const int grid = tile_grid(td);
   switch (pot_mask) {
-    case 1: step_phase<1>(st, phase, fp32, td, out, box, con, pc, fa, par); break;
-    case 2: step_phase<2>(st, phase, fp32, td, out, box, con, pc, fa, par); break;
-    default: step_phase<3>(st, phase, fp32, td, out, box, con, pc, fa, par); break;
+    case 1: step_phase<1>(st, phase, fp32, args, grid, par, t_off, head_enforce); break;
+    case 2: step_phase<2>(st, phase, fp32, args, grid, par, t_off, head_enforce); break;
+    default: step_phase<3>(st, phase, fp32, args, grid, par, t_off, head_enforce); break;
   }
 }
 

@@ -1,8 +1,10 @@
 // Cell-tiled engine for planar single-system runs: the state lives in
 // cell order (row-major over a 2-D cell grid), each workgroup owns a run of
 // cells in one row and stages the positions of the 3 x (run + 2) surrounding
-// cells in LDS, and the Verlet list is a per-particle bitmask over that
-// stencil's candidates instead of a list of global indices.
+// cells in LDS, and the Verlet list of a particle holds 16-bit indices into
+// that stage instead of global indices.  The stage's layout depends only on
+// the cell starts of the current set, which change only at builds, so an
+// index written by a build stays valid for every step until the next one.
 #pragma once
 #include "kernels.hpp"
 
@@ -10,7 +12,18 @@ namespace samos {
 
 constexpr int kTileThreads = 256;
 constexpr int kTileMaxCells = 80;     // cells per tile along x (3 * (80 + 2) halo cells <= 256)
-constexpr int kTileHaloMax = 2048;    // halo positions staged in LDS; larger halos read L2
+constexpr int kTileHaloMax = 1024;    // halo positions staged in LDS; larger halos read L2
+constexpr int kTileStageMax = 65535;  // staged indices are 16-bit (the largest is the sentinel)
+constexpr int kTileMaxOwn = 2 * kTileThreads;  // particles of one tile (two per step-kernel thread)
+static_assert(kTileHaloMax % kTileThreads == 0, "the stage is loaded in whole rounds of the workgroup");
+// Stage descriptor of a tile, written by each build for the new set:
+//   [0] halo size (stage entries)  [1], [2] stage range of the tile's own
+//   particles  [3] set index of the first of them  [4] runs
+//   [5 + 2r], [6 + 2r] first stage entry and first set index of run r: the
+//   stage is the concatenation of at most 9 runs of consecutive set indices
+//   (each halo row is one run, or up to three where it wraps along x).
+constexpr int kTileRuns = 9;
+constexpr int kTileDesc = 5 + 2 * kTileRuns + 1;
 
 // Control block in device memory.  `word` holds the current set and the
 // build handshake, changed only by atomics:
@@ -27,9 +40,11 @@ struct TileCtl {
   int word;
   int init;      // the pending build is a run's first (snapshot from the slot list)
   int rebuilds;  // builds during the run (the first one not counted)
-  int overflow;  // some particle had more stencil candidates than the mask holds
+  int overflow;  // some particle had more neighbours than the list holds, a
+                 // halo more than kTileStageMax particles or a tile more than
+                 // kTileMaxOwn
   int far;       // some particle changed cell by more than one between builds
-  int maxcand;   // largest stencil candidate count of the builds
+  int maxcand;   // largest neighbour count of the builds
   int maxhalo;   // largest staged halo of the builds
   int pad_;
 };
@@ -50,12 +65,17 @@ struct TileD {
   int ncx, ncy, ncell;   // 2-D cell grid (the reference's n_x, n_y; one z layer)
   int T;                 // cells per tile along x
   int ntx, ntiles;       // tiles per row, tiles
-  int W;                 // 64-bit mask words per particle
+  int K;                 // list capacity per particle (a multiple of 4)
   double wx, wy, xlo, ylo;
   double cut2;           // (rcut + pad)^2
   double pad2q;          // pad^2 / 4
   const TileSet* sets;   // [2], in device memory
-  unsigned long long* mask;  // [W][n] candidate bitmasks of the current set
+  // Verlet lists of the current set: row r of particle q holds its entries
+  // 4r .. 4r + 3 (staged indices, ascending staging order; the last row is
+  // padded with the stage's sentinel index, the halo size), lst[r * n + q].
+  ushort4* lst;          // [K / 4][n]
+  int* len;              // [n] entries of each list
+  int* desc;             // [ntiles][kTileDesc] stage descriptors of the current set
   int* count;            // [ncell] particles per cell of the build in progress
   int* tcount;           // [ntiles] particles per tile of the build in progress
   int* tstart;           // [ntiles + 1] exclusive scan of tcount
@@ -88,10 +108,21 @@ void launch_tile_load(hipStream_t st, TileD td, Fields f, NListD nl);
 // the current positions are in buffer par ^ 1, and so are the new set's.
 // init: the first build of a run (scatter + sort).
 void launch_tile_build(hipStream_t st, TileD td, BoxD box, int init, int par);
+// The constant arguments of a run's step kernels, kept in device memory
+// (the FusedArgs fields t_off and head_enforce are taken from the launch).
+struct TileStepArgs {
+  TileD td;
+  SlotOut so;
+  BoxD b;
+  ConstraintD c;
+  PairConst pc;
+  FusedArgs fa;
+};
 // One fused BAOAB step (phases as launch_baoab_fused) on the current set,
-// reading positions from buffer par and writing buffer par ^ 1.
-void launch_tile_step(hipStream_t st, int pot_mask, int phase, int fp32, TileD td, SlotOut out, BoxD box,
-                      ConstraintD con, PairConst pc, const FusedArgs& fa, int par);
+// reading positions from buffer par and writing buffer par ^ 1; args: the
+// run's TileStepArgs in device memory (its td equal to the td given).
+void launch_tile_step(hipStream_t st, int pot_mask, int phase, int fp32, const TileD& td, const TileStepArgs* args,
+                      int par, int t_off, int head_enforce);
 // Writes the current set (positions in buffer par) back into the slot-ordered
 // columns (x, y, xy, v, n) and the last build's snapshot into nl.ox / nl.oy.
 void launch_tile_store(hipStream_t st, TileD td, Fields f, NListD nl, int par);
