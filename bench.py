@@ -326,6 +326,48 @@ def c2_system(samos_amd, sc):
     return s, st
 
 
+def c1_large_system(samos_amd, sc):
+    """Active polar soft disks in a periodic 2-D box (N = 1,048,576, packing 0.8): the physics of the
+    4096-particle case at the headline size."""
+    st = sc.plane_disks(1 << 20, phi=0.8, seed=103)
+    s = sc.load(samos_amd.System, st)
+    s.nlist(2.4, 0.3)
+    s.pair_soft(1.0, 2.0)
+    s.align_polar(0.1, 2.4)
+    s.constraint("plane")
+    s.set_timestep(0.01)
+    s.integrator_brownian(v0=0.1, nu=0.1, mu=1.0, seed=13)
+    return s, st
+
+
+def bench_tiled_brownian(samos_amd, sc, steps):
+    """The fused Brownian step with polar alignment on the cell-tiled engine (positions and directors
+    of each tile's halo staged in LDS) against the neighbour-list kernels, same system and steps."""
+    out = {}
+    for label, flags, kernel in (("tiles", samos_amd.RUN_TILES, "k_tile_brownian<soft, polar>"),
+                                 ("list", 0, "k_brownian_fused")):
+        s, st = c1_large_system(samos_amd, sc)
+        n = st["n"]
+        timed_run(s, 100, flags)
+        secs = timed_run(s, steps, flags)
+        s.profile(True)
+        timed_run(s, 200, flags)
+        ms, launches = s.profile_read("fused")
+        s.profile(False)
+        avg = ms / 1e3 / max(launches, 1)
+        achieved = 80 * n / avg / 1e9
+        out[label] = {"value": n * steps / secs, "ms_per_step": secs * 1e3 / steps,
+                      "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": achieved / HBM_PEAK_GBS, "kernel": kernel, "avg_launch_us": avg * 1e6,
+                                   "launches_timed": launches, "alg_bytes_per_launch": 80 * n}}
+        if label == "tiles":
+            out["tiles_used"] = s.tile_info()
+        s.close()
+    print("bench: tiled Brownian %s" % out, file=sys.stderr, flush=True)
+    return {"workload": c1_large_system.__doc__.strip(), "particles": 1 << 20, "steps": steps,
+            "unit": "particle-steps/s", **out}
+
+
 def bench_single(samos_amd, sc, which, steps, cpu_steps, extra_flags=0):
     """One system through the drop-in single-system path (the `run` loop).
     extra_flags: e.g. RUN_NO_GRAPH for a profiler run (step-by-step launches)."""
@@ -625,10 +667,11 @@ def main():
     ap.add_argument("--tile-cells", type=int, default=0, help="cells per tile (0: automatic)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this size (HBM counter calibration) and exit")
-    ap.add_argument("--parts", default="main,ensemble,fire,c1,c2",
+    ap.add_argument("--parts", default="main,ensemble,fire,c1,c2,tiles_brownian",
                     help="comma list: main (the N=1M metric), ensemble (batched active spheres), "
                          "fire (batched FIRE relaxation and the warm start), c1 (polar disks, N=4096), "
-                         "c2 (nematic sphere, N=100000)")
+                         "c2 (nematic sphere, N=100000), tiles_brownian (polar disks, N=1M, cell-tiled "
+                         "Brownian step against the list kernels)")
     ap.add_argument("--profile-run", choices=["ensemble", "mpnn"], default=None,
                     help="only run a fixed workload for rocprofv3 and exit")
     ap.add_argument("--ensemble-steps", type=int, default=5000)
@@ -667,6 +710,8 @@ def main():
         extras["polar_disks_plane_n4096"] = bench_single(samos_amd, sc, "c1", 10000, 0 if args.no_cpu else 25000, ng)
     if "c2" in parts:
         extras["nematic_sphere_n100000"] = bench_single(samos_amd, sc, "c2", 5000, 0 if args.no_cpu else 280, ng)
+    if "tiles_brownian" in parts:
+        extras["polar_disks_plane_n1048576_tiled"] = bench_tiled_brownian(samos_amd, sc, 500)
     if "main" not in parts:
         print(json.dumps(extras))
         return

@@ -130,6 +130,8 @@ struct samos_ctx {
   void* tile_mem = nullptr;
   TileStepArgs* td_args = nullptr;  // the step kernels' constant arguments (inside tile_mem)
   TileStepArgs h_tile_args{};       // their host copy
+  TileBrownArgs* tb_args = nullptr; // the same for tiled Brownian runs
+  TileBrownArgs h_tb_args{};
   size_t tile_bytes = 0;
   int tile_key[5] = {-1, -1, -1, -1, -1};  // n, ncx, ncy, T, K of the allocation
   long long tile_stats[4] = {0, 0, 0, 0};  // tiled runs, rebuilds, largest candidate count, largest halo
@@ -957,6 +959,9 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
 // Or up to four "brownian" integrators on pairwise disjoint groups (each
 // particle is integrated by at most one of them, so their map order does not
 // matter), such as the active and passive groups of a two-component conf.
+bool tile_brownian_eligible(samos_ctx* ctx, int flags);
+int run_tiled_brownian(samos_ctx* ctx, Integ& I, int a, int b, int steps, int flags);
+
 bool brownian_groups(const samos_ctx* ctx) {
   if (ctx->integs.empty() || ctx->integs.size() > 4) return false;
   std::vector<const Integ*> v;
@@ -1000,6 +1005,10 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
   } else if (!ctx->age_checked) {
     launch_age_check(ctx->st, ctx->d_age, ctx->dcol[16], n);
     ctx->age_checked = true;
+  }
+  if (!pair && grouped.empty() && tile_brownian_eligible(ctx, flags)) {
+    const int r = run_tiled_brownian(ctx, I, a, b, steps, flags);
+    if (r <= 0) return r;
   }
   if (int r = upload_clock(ctx, a)) return r;
   HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
@@ -1155,12 +1164,12 @@ int tile_prepare(samos_ctx* ctx) {
     const int ntx = (ncx + T - 1) / T;
     const long ncell = (long)ncx * ncy, ntiles = (long)ntx * ncy;
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t d2 = al(sizeof(double2) * n), i4 = al(sizeof(int) * n);
-    const size_t bytes = 2 * (2 * d2 + 3 * d2 + 2 * i4) + 2 * al(sizeof(int) * (ncell + 1)) +
+    const size_t d2 = al(sizeof(double2) * n), i4 = al(sizeof(int) * n), d8 = al(sizeof(double) * n);
+    const size_t bytes = 2 * (2 * d2 + 4 * d2 + d8 + 2 * i4) + 2 * al(sizeof(int) * (ncell + 1)) +
                          al(sizeof(ushort4) * (size_t)(K / 4) * (size_t)n) + i4 +
                          al(sizeof(int) * kTileDesc * (size_t)ntiles) + al(sizeof(int) * ncell) +
                          al(sizeof(int) * ntiles) + al(sizeof(int) * (ntiles + 1)) + 2 * i4 + al(sizeof(TileCtl)) +
-                         al(2 * sizeof(TileSet)) + al(sizeof(TileStepArgs));
+                         al(2 * sizeof(TileSet)) + al(sizeof(TileStepArgs)) + al(sizeof(TileBrownArgs));
     HIPCHK(hipMalloc(&ctx->tile_mem, bytes));
     ctx->tile_bytes = bytes;
     char* p = static_cast<char*>(ctx->tile_mem);
@@ -1169,7 +1178,8 @@ int tile_prepare(samos_ctx* ctx) {
     TileSet sets[2];
     for (int s = 0; s < 2; ++s) {
       sets[s].xy[0] = (double2*)take(d2); sets[s].xy[1] = (double2*)take(d2);
-      sets[s].v = (double2*)take(d2); sets[s].nd = (double2*)take(d2); sets[s].o = (double2*)take(d2);
+      sets[s].v = (double2*)take(d2); sets[s].nd[0] = (double2*)take(d2); sets[s].nd[1] = (double2*)take(d2);
+      sets[s].o = (double2*)take(d2); sets[s].om = (double*)take(d8);
       sets[s].id = (int*)take(i4); sets[s].slot = (int*)take(i4);
       sets[s].cstart = (int*)take(al(sizeof(int) * (ncell + 1)));
     }
@@ -1186,6 +1196,7 @@ int tile_prepare(samos_ctx* ctx) {
     td.members = (int*)take(i4);
     td.ctl = (TileCtl*)take(al(sizeof(TileCtl)));
     ctx->td_args = (TileStepArgs*)take(al(sizeof(TileStepArgs)));
+    ctx->tb_args = (TileBrownArgs*)take(al(sizeof(TileBrownArgs)));
     td.n = n; td.ncx = ncx; td.ncy = ncy; td.ncell = (int)ncell; td.T = T; td.ntx = ntx; td.ntiles = (int)ntiles;
     td.K = K;
     std::copy(key, key + 5, ctx->tile_key);
@@ -1194,6 +1205,7 @@ int tile_prepare(samos_ctx* ctx) {
   HIPCHK(hipMemsetAsync(td.tcount, 0, sizeof(int) * (size_t)td.ntiles, ctx->st));
   td.wx = nl.wx; td.wy = nl.wy; td.xlo = ctx->box.xlo; td.ylo = ctx->box.ylo;
   td.cut2 = nl.cut2; td.pad2q = nl.pad2q;
+  td.dirs2 = 0;
   return 0;
 }
 
@@ -1212,7 +1224,7 @@ int tile_upload_args(samos_ctx* ctx, Integ& I, double mass_u) {
   A = TileStepArgs{};
   A.td = ctx->td;
   const Fields f = fields(ctx);
-  A.so = SlotOut{f.fx, f.fy, f.fz, f.ix, f.iy, f.iz};
+  A.so = SlotOut{f.fx, f.fy, f.fz, f.ix, f.iy, f.iz, f.tx, f.ty, f.tz};
   A.b = ctx->box;
   A.c = ctx->con;
   FusedArgs& fa = A.fa;
@@ -1321,6 +1333,138 @@ int run_tiled_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
   I.calls += (unsigned long long)(b - a);
   // one closing half-kick, and one age step, per iteration
   launch_age_finish(ctx->st, ctx->d_age, ctx->dcol[16], ctx->n, I.dt, b - a);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---- cell-tiled fused Brownian run (planar systems) ------------------------
+// One "brownian" integrator on every particle of a planar single-type system
+// without per-particle radii, asked for with SAMOS_RUN_TILES: the fused
+// Brownian step (forces, aligner torques and IntegratorBrownian) on the tiled
+// engine, with positions and directors staged in LDS.  Same contract as
+// run_fused_brownian; returns 1 (nothing changed) when the system does not fit.
+bool tile_brownian_eligible(samos_ctx* ctx, int flags) {
+  const NListD& nl = ctx->nl;
+  if (!((flags & SAMOS_RUN_TILES) && ctx->integs.size() == 1 && ctx->con.kind == 1 && nl.use_cell &&
+        nl.ncx >= 3 && nl.ncy >= 3 && ctx->ntypes == 1 && !ctx->soft_radii && !ctx->lj_radii && ctx->n > 0))
+    return false;
+  const Integ& I = ctx->integs.begin()->second;
+  if (I.name != "brownian" || I.gbit != 1u) return false;
+  AgeCtl h{};
+  if (hipMemcpyAsync(&h, ctx->d_age, sizeof(AgeCtl), hipMemcpyDeviceToHost, ctx->st) != hipSuccess ||
+      hipStreamSynchronize(ctx->st) != hipSuccess)
+    return false;
+  return h.uniform != 0;
+}
+
+int tile_upload_brownian_args(samos_ctx* ctx, const Integ& I) {
+  TileBrownArgs& A = ctx->h_tb_args;
+  A = TileBrownArgs{};
+  A.td = ctx->td;
+  const Fields f = fields(ctx);
+  A.so = SlotOut{f.fx, f.fy, f.fz, f.ix, f.iy, f.iz, f.tx, f.ty, f.tz};
+  A.b = ctx->box;
+  A.c = ctx->con;
+  PairConst& pc = A.pc;
+  const int nn = kMaxTypes * kMaxTypes;
+  if (ctx->has_soft) { pc.soft_k = ctx->soft_tab[0]; pc.soft_a = ctx->soft_tab[nn]; }
+  if (ctx->has_lj) {
+    pc.lj_eps = ctx->lj_tab[0];
+    pc.lj_sig2 = ctx->lj_tab[nn] * ctx->lj_tab[nn];
+    pc.lj_rc2 = ctx->lj_tab[2 * nn] * ctx->lj_tab[2 * nn];
+  }
+  pc.has_sp = ctx->has_sp ? 1 : 0;
+  pc.sp_alpha = ctx->sp_alpha;
+  A.dt = I.dt; A.v0 = I.v0; A.mu = I.mu; A.mur = I.mur; A.stoch = I.stoch; A.flip_p = I.flip_p;
+  A.nematic = I.nematic ? 1 : 0; A.velocity = I.velocity ? 1 : 0;
+  A.temp = I.temp; A.seed = I.seed; A.clock = ctx->d_clock; A.clock_slot = I.slot;
+  // the aligner's coupling as the pair loop applies it: the global nematic J
+  // is doubled, a per-type J is used as given
+  // (src/aligner/pair/pair_nematic_aligner.cpp:37,57-61)
+  if (ctx->has_pol) { A.al_J = ctx->pol_tab[0]; A.al_rc2 = ctx->pol_tab[nn] * ctx->pol_tab[nn]; }
+  if (ctx->has_nem) {
+    A.al_J = ctx->nem_pp ? ctx->nem_tab[0] : 2.0 * ctx->nem_J;
+    A.al_rc2 = ctx->nem_tab[nn] * ctx->nem_tab[nn];
+  }
+  HIPCHK(hipMemcpyAsync(ctx->tb_args, &A, sizeof(A), hipMemcpyHostToDevice, ctx->st));
+  return 0;
+}
+
+// store: the range's last step, whose loop-head forces and torques are the
+// state left behind
+void tile_brownian_step(samos_ctx* ctx, int t, int t_off, int steps, bool f32, int par, bool store = false) {
+  {
+    ProfScope ps(ctx, "fused");
+    launch_tile_brownian(ctx->st, pot_mask(ctx), al_mask(ctx), f32 ? 1 : 0, ctx->td, ctx->tb_args, par, t_off,
+                         t + 1 <= steps ? 1 : 0, store ? 1 : 0);
+  }
+  ProfScope ps(ctx, "rebuild");
+  launch_tile_build(ctx->st, ctx->td, ctx->box, 0, par);
+}
+
+int run_tiled_brownian(samos_ctx* ctx, Integ& I, int a, int b, int steps, int flags) {
+  const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
+  if (int r = tile_prepare(ctx)) return r;
+  ctx->td.dirs2 = 1;
+  Fields f = fields(ctx);
+  f.xy = ctx->xy[0];
+  launch_tile_load(ctx->st, ctx->td, f, ctx->nl);
+  launch_tile_build(ctx->st, ctx->td, ctx->box, 1, 1);
+  TileCtl h;
+  if (int r = read_tile_ctl(ctx, &h)) return r;
+  if (h.overflow || !(h.word & 2)) return 1;  // too dense for the lists: list path
+  if (int r = upload_clock(ctx, a)) return r;
+  if (int r = tile_upload_brownian_args(ctx, I)) return r;
+  int t = a;
+  const int K = 64;  // steps per captured chunk
+  const bool graph = !(flags & SAMOS_RUN_NO_GRAPH) && !ctx->prof;
+  if (graph && b - t >= 2 * K) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < K; ++j) tile_brownian_step(ctx, t + j, j, steps, f32, (t + j - a) & 1);
+    launch_clock_advance(ctx->st, ctx->d_clock, K);
+    HIPCHK(hipStreamEndCapture(ctx->st, &g));
+    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    while (b - t > K) {
+      HIPCHK(hipGraphLaunch(ge, ctx->st));
+      t += K;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    HIPCHK(hipGraphExecDestroy(ge));
+    HIPCHK(hipGraphDestroy(g));
+  }
+  const int t0 = t;
+  for (; t < b; ++t) tile_brownian_step(ctx, t, t - t0, steps, f32, (t - a) & 1, t == b - 1);
+  launch_clock_advance(ctx->st, ctx->d_clock, b - t0);
+  launch_tile_store(ctx->st, ctx->td, f, ctx->nl, (b - a) & 1);
+  HIPCHK(hipGetLastError());
+  if (int r = read_tile_ctl(ctx, &h)) return r;
+  ctx->tile_stats[0] += 1;
+  ctx->tile_stats[1] += h.rebuilds;
+  ctx->tile_stats[2] = std::max<long long>(ctx->tile_stats[2], h.maxcand);
+  ctx->tile_stats[3] = std::max<long long>(ctx->tile_stats[3], h.maxhalo);
+  if (h.far) return fail(ctx, "tiled run: a particle moved more than one cell width between neighbour-list builds");
+  if (h.overflow)
+    return fail(ctx, "tiled run: a particle had more than " + std::to_string(ctx->td.K) +
+                         " neighbours (largest " + std::to_string(h.maxcand) + ")");
+  if (h.rebuilds > 0) {
+    // the slot-ordered list is rebuilt at the last build's snapshot, so it
+    // holds exactly the pair set (and coordination) of that build
+    int before[8], after[8];
+    if (int r = read_flags(ctx, before)) return r;
+    Fields at = fields(ctx);
+    at.x = ctx->nl.ox; at.y = ctx->nl.oy;
+    if (int r = build_sync(ctx, &at)) return r;
+    if (int r = read_flags(ctx, after)) return r;
+    after[4] = before[4] + h.rebuilds;
+    HIPCHK(hipMemcpy(ctx->d_flags + 4, &after[4], sizeof(int), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
+  launch_age_finish(ctx->st, ctx->d_age, ctx->dcol[16], ctx->n, I.dt, b - a);
+  I.calls += (unsigned long long)(b - a);
+  // loop head of b: its enforce was applied with the positions; forces and torques
+  if (b <= steps) loop_head_forces(ctx, nullptr, f32);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -115,20 +115,26 @@ __device__ __forceinline__ TileGeom tile_geom(const TileD& td, int t) {
 // index, s_hgs[h] its first index in the set's cell order; s_hcs[nh] = total.
 __device__ __forceinline__ int stage_index(const TileD& td, const TileGeom& g, const int* cs, int* s_hcs, int* s_hgs,
                                            int* s_scan) {
-  int cnt = 0;
-  const int h = threadIdx.x;
-  if (g.active && h < g.nh) {
-    const int r = h / (g.nc + 2), k = h - r * (g.nc + 2);
-    const int ci = wrapi(g.i0 - 1 + k, td.ncx), cj = wrapi(g.ty - 1 + r, td.ncy);
-    const int c = cj * td.ncx + ci;
-    const int a = cs[c];
-    cnt = cs[c + 1] - a;
-    s_hgs[h] = a;
+  // thread t takes halo cells 2t and 2t + 1 (at most 2 * kTileThreads)
+  int cnt[2] = {0, 0};
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int h = 2 * (int)threadIdx.x + u;
+    if (g.active && h < g.nh) {
+      const int r = h / (g.nc + 2), k = h - r * (g.nc + 2);
+      const int ci = wrapi(g.i0 - 1 + k, td.ncx), cj = wrapi(g.ty - 1 + r, td.ncy);
+      const int c = cj * td.ncx + ci;
+      const int a = cs[c];
+      cnt[u] = cs[c + 1] - a;
+      s_hgs[h] = a;
+    }
   }
   int total;
-  const int ex = block_exscan(cnt, s_scan, total);
+  const int ex = block_exscan(cnt[0] + cnt[1], s_scan, total);
+  const int h = 2 * (int)threadIdx.x;
   if (h < g.nh) s_hcs[h] = ex;
-  if (h == 0) s_hcs[g.nh] = total;
+  if (h + 1 < g.nh) s_hcs[h + 1] = ex + cnt[0];
+  if (threadIdx.x == 0) s_hcs[g.nh] = total;
   __syncthreads();
   return total;
 }
@@ -422,7 +428,9 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_build(TileD td, BoxD b, i
       td.sets[N].xy[par][q] = td.sets[S].xy[par][mem];
       td.sets[N].o[q] = si;
       td.sets[N].v[q] = td.sets[S].v[mem];
-      td.sets[N].nd[q] = td.sets[S].nd[mem];
+      const int dp = td.dirs2 ? par : 0;  // the buffer holding the current directors
+      td.sets[N].nd[dp][q] = td.sets[S].nd[dp][mem];
+      td.sets[N].om[q] = td.sets[S].om[mem];
       td.sets[N].id[q] = td.sets[S].id[mem];
       td.sets[N].slot[q] = td.sets[S].slot[mem];
       const int lc = (td.cell_of[mem] % td.ncx) - g.i0 + 1;
@@ -485,6 +493,7 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_build(TileD td, BoxD b, i
 struct PairK {
   double soft_k, soft_a, lj_rc2, lj_sig2, lj_e48;
   double xlo, xhi, lx, ylo, yhi, ly;
+  double al_J, al_rc2;  // aligner coupling and cutoff^2 (Brownian runs)
 };
 
 __device__ __forceinline__ PairK pair_k(const PairConst& pc, const BoxD& b) {
@@ -492,13 +501,14 @@ __device__ __forceinline__ PairK pair_k(const PairConst& pc, const BoxD& b) {
   k.soft_k = pc.soft_k; k.soft_a = pc.soft_a;
   k.lj_rc2 = pc.lj_rc2; k.lj_sig2 = pc.lj_sig2; k.lj_e48 = 48.0 * pc.lj_eps;
   k.xlo = b.xlo; k.xhi = b.xhi; k.lx = b.Lx; k.ylo = b.ylo; k.yhi = b.yhi; k.ly = b.Ly;
+  k.al_J = 0.0; k.al_rc2 = 0.0;
   return k;
 }
 
+// r_sq = dx * dx + dy * dy (dz = 0: adding +0 to a sum of squares changes no bit)
 template <int POT, bool F32>
-__device__ __forceinline__ void pair_term(const PairK& pk, double dx, double dy, double& fx, double& fy) {
-  // dz = 0: adding +0 to a sum of squares changes no bit
-  const double r_sq = dx * dx + dy * dy;
+__device__ __forceinline__ void pair_term(const PairK& pk, double dx, double dy, double r_sq, double& fx,
+                                          double& fy) {
   if (POT & POT_SOFT) {
     if (F32) {
       const float r = sqrtf((float)r_sq);
@@ -565,7 +575,7 @@ __device__ __forceinline__ void list_forces(PairK pk, const TileD& td, int q, in
     for (int u = 0; u < 4; ++u) {
       double dx = pj[u].x - xi, dy = pj[u].y - yi;
       if (wrap) min_image_k(pk, dx, dy);
-      pair_term<POT, F32>(pk, dx, dy, fx, fy);
+      pair_term<POT, F32>(pk, dx, dy, dx * dx + dy * dy, fx, fy);
     }
   }
 }
@@ -582,7 +592,7 @@ __device__ __forceinline__ void list_forces_l2(const PairK& pk, const TileD& td,
     const double2 pj = gld(cur + desc_global(D, e));
     double dx = pj.x - xi, dy = pj.y - yi;
     min_image_k(pk, dx, dy);
-    pair_term<POT, F32>(pk, dx, dy, fx, fy);
+    pair_term<POT, F32>(pk, dx, dy, dx * dx + dy * dy, fx, fy);
   }
 }
 
@@ -608,7 +618,7 @@ __device__ __forceinline__ bool tile_particle(const TileD& td, const TileSet& A,
   // Langevin run, and the first step of a run (phase 0) normalises them all:
   // later steps read a director only for self propulsion.
   constexpr bool kDirs = PHASE == 0;
-  const double2 n0 = (kDirs || pc.has_sp) ? gld(A.nd + q) : make_double2(0.0, 0.0);
+  const double2 n0 = (kDirs || pc.has_sp) ? gld(A.nd[0] + q) : make_double2(0.0, 0.0);
   if (pc.has_sp) { fx += pc.sp_alpha * n0.x; fy += pc.sp_alpha * n0.y; fz += pc.sp_alpha * 0.0; }
   const double2 v0 = gld(A.v + q);
   double vx = v0.x, vy = v0.y, vz = 0.0;
@@ -645,7 +655,7 @@ __device__ __forceinline__ bool tile_particle(const TileD& td, const TileSet& A,
     if (fa.head_enforce) enforce<kSurfPlane>(b, c, p, false);
     gst(A.xy[par ^ 1] + q, make_double2(p.x, p.y));
     gst(A.v + q, make_double2(p.vx, p.vy));
-    if (kDirs && (p.nx != n0.x || p.ny != n0.y)) gst(A.nd + q, make_double2(p.nx, p.ny));
+    if (kDirs && (p.nx != n0.x || p.ny != n0.y)) gst(A.nd[0] + q, make_double2(p.nx, p.ny));
     if (p.dix | p.diy | p.diz) {
       const int s = A.slot[q];
       if (p.dix) so.ix[s] += p.dix;
@@ -738,6 +748,230 @@ __global__ void __launch_bounds__(kTileThreads, 8) k_tile_step(const TileStepArg
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused Brownian step on the tiles: loop-head forces and aligner torques of
+// step t from the staged positions and directors, then
+// IntegratorBrownian::integrate (src/integrators/integrator_brownian.cpp:
+// 42-117) on the plane.  Pair terms as the list-based k_brownian_fused
+// (pair_sum.hpp: soft, LJ; polar src/aligner/pair/pair_polar_aligner.cpp:
+// 58-103, nematic src/aligner/pair/pair_nematic_aligner.cpp:53-92, with
+// J the effective coupling).  Positions and directors are read from buffer
+// par and written to par ^ 1.
+// ---------------------------------------------------------------------------
+template <int POT, int AL, bool F32>
+__device__ __forceinline__ void align_term(const PairK& pk, double r_sq, double nxi, double nyi, double nxj,
+                                           double nyj, double& tz) {
+  // n_i x n_j: on a plane only its z component is non-zero
+  const double cz = nxi * nyj - nyi * nxj;
+  if (AL & AL_POLAR) {
+    if (r_sq <= pk.al_rc2) tz += pk.al_J * cz;
+  }
+  if (AL & AL_NEMATIC) {
+    if (r_sq <= pk.al_rc2) {
+      const double nn = nxi * nxj + nyi * nyj;
+      tz += pk.al_J * nn * cz;
+    }
+  }
+}
+
+template <int POT, int AL, bool F32>
+__device__ __forceinline__ void list_forces_al(PairK pk, const TileD& td, int q, int kn, const double2* s_pos,
+                                               const double2* s_dir, bool wrap, double xi, double yi, double nxi,
+                                               double nyi, double& fx, double& fy, double& tz) {
+  asm volatile("" : "+v"(pk.lj_rc2), "+v"(pk.lj_sig2), "+v"(pk.lj_e48), "+v"(pk.soft_k), "+v"(pk.soft_a));
+  asm volatile("" : "+v"(pk.xlo), "+v"(pk.xhi), "+v"(pk.lx), "+v"(pk.ylo), "+v"(pk.yhi), "+v"(pk.ly));
+  asm volatile("" : "+v"(pk.al_J), "+v"(pk.al_rc2));
+  const unsigned long long* L = reinterpret_cast<const unsigned long long*>(td.lst) + q;
+  const long n = td.n;
+  const int rows = (kn + 3) >> 2;
+  unsigned long long nxt = rows > 0 ? gld(L) : 0ULL;
+  for (int r = 0; r < rows; ++r) {
+    const unsigned long long cur = nxt;
+    if (r + 1 < rows) nxt = gld(L + (long)(r + 1) * n);
+    double2 pj[4], nj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = (int)((cur >> (16 * u)) & 0xffffULL);
+      pj[u] = s_pos[e];
+      if (AL) nj[u] = s_dir[e];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double dx = pj[u].x - xi, dy = pj[u].y - yi;
+      if (wrap) min_image_k(pk, dx, dy);
+      const double r_sq = dx * dx + dy * dy;
+      pair_term<POT, F32>(pk, dx, dy, r_sq, fx, fy);
+      if (AL) align_term<POT, AL, F32>(pk, r_sq, nxi, nyi, nj[u].x, nj[u].y, tz);
+    }
+  }
+}
+
+// The same through L2 for a tile whose halo is larger than the LDS stage.
+template <int POT, int AL, bool F32>
+__device__ __forceinline__ void list_forces_al_l2(const PairK& pk, const TileD& td, int q, int kn,
+                                                  const double2* cur, const double2* dir, DescPtr D, double xi,
+                                                  double yi, double nxi, double nyi, double& fx, double& fy,
+                                                  double& tz) {
+  const unsigned short* L = reinterpret_cast<const unsigned short*>(td.lst);
+#pragma unroll 1
+  for (int k = 0; k < kn; ++k) {
+    const int e = L[((long)(k >> 2) * td.n + q) * 4 + (k & 3)];
+    const int j = desc_global(D, e);
+    const double2 pj = gld(cur + j);
+    double dx = pj.x - xi, dy = pj.y - yi;
+    min_image_k(pk, dx, dy);
+    const double r_sq = dx * dx + dy * dy;
+    pair_term<POT, F32>(pk, dx, dy, r_sq, fx, fy);
+    if (AL) {
+      const double2 nj = gld(dir + j);
+      align_term<POT, AL, F32>(pk, r_sq, nxi, nyi, nj.x, nj.y, tz);
+    }
+  }
+}
+
+template <int POT, int AL, bool F32>
+__device__ __forceinline__ bool tile_brownian_particle(const TileBrownArgs& P, const TileSet& A, int par, int t_off,
+                                                       int head_enforce, int store, const double2* cur,
+                                                       const double2* dir,
+                                                       const double2* s_pos, const double2* s_dir, bool staged,
+                                                       bool wrap, DescPtr D, int k, int q) {
+  const TileD& td = P.td;
+  const BoxD& b = P.b;
+  const ConstraintD& c = P.c;
+  const PairConst& pc = P.pc;
+  const int kn = gld(td.len + q);
+  PairK pk = pair_k(pc, b);
+  pk.al_J = P.al_J;
+  pk.al_rc2 = P.al_rc2;
+  double2 pi, ni;
+  double fx = 0.0, fy = 0.0, fz = 0.0, tz = 0.0;
+  if (staged) {
+    pi = s_pos[k];
+    ni = AL ? s_dir[k] : gld(dir + q);  // directors are staged only for an aligner
+    list_forces_al<POT, AL, F32>(pk, td, q, kn, s_pos, s_dir, wrap, pi.x, pi.y, ni.x, ni.y, fx, fy, tz);
+  } else {
+    pi = gld(cur + q);
+    ni = gld(dir + q);
+    list_forces_al_l2<POT, AL, F32>(pk, td, q, kn, cur, dir, D, pi.x, pi.y, ni.x, ni.y, fx, fy, tz);
+  }
+  if (pc.has_sp) { fx += pc.sp_alpha * ni.x; fy += pc.sp_alpha * ni.y; fz += pc.sp_alpha * 0.0; }
+  const uint32_t pid = (uint32_t)gld(A.id + q);
+  const double2 v0 = gld(A.v + q);
+  PS p;
+  p.x = pi.x; p.y = pi.y; p.z = c.zpos;
+  p.vx = v0.x; p.vy = v0.y; p.vz = 0.0;
+  p.nx = ni.x; p.ny = ni.y; p.nz = 0.0;
+  p.dix = p.diy = p.diz = 0;
+  // IntegratorBrownian::integrate for this particle (k_brownian_fused, mode 0)
+  const unsigned long long call = eff_call(P.clock, P.clock_slot, 0ULL) + (unsigned long long)t_off;
+  double g_rot = 0.0, u_flip = 1.0;
+  gauss1_u1(P.seed, TAG_BROWNIAN, pid, call, g_rot, u_flip);
+  if (P.nematic && u_flip < P.flip_p) {
+    p.nx = -p.nx; p.ny = -p.ny; p.nz = -p.nz;
+    if (P.velocity) { p.vx = -p.vx; p.vy = -p.vy; p.vz = -p.vz; }
+  }
+  const double T = P.temp.at(eff_step(P.clock, t_off));
+  const double B = sqrt(2.0 * P.mu * T);
+  const double sqrt_dt = sqrt(P.dt);
+  const double fdx = P.v0 * p.nx + P.mu * fx, fdy = P.v0 * p.ny + P.mu * fy, fdz = P.v0 * p.nz + P.mu * fz;
+  p.vx = fdx; p.vy = fdy; p.vz = fdz;
+  p.x += P.dt * fdx; p.y += P.dt * fdy; p.z += P.dt * fdz;
+  if (T > 0.0) {
+    double g[3];
+    gauss3(P.seed, TAG_BROWNIAN, pid, call, g);
+    const double frx = B * g[0], fry = B * g[1], frz = B * g[2];
+    p.vx += frx; p.vy += fry; p.vz += frz;
+    p.x += sqrt_dt * frx; p.y += sqrt_dt * fry; p.z += sqrt_dt * frz;
+  }
+  const bool fz0 = enforce<kSurfPlane>(b, c, p);
+  const double om = P.mur * project_torque<kSurfPlane>(c, p, 0.0, 0.0, tz);
+  if (store) {
+    const int s = A.slot[q];
+    P.so.fx[s] = fx; P.so.fy[s] = fy; P.so.fz[s] = fz0 ? 0.0 : fz;
+    if (AL) { P.so.tx[s] = 0.0; P.so.ty[s] = 0.0; P.so.tz[s] = tz; }
+  }
+  const double dtheta = P.dt * om + P.stoch * g_rot;
+  rotate_director<kSurfPlane>(c, p, dtheta);
+  if (P.velocity) rotate_velocity<kSurfPlane>(c, p, dtheta);
+  if (head_enforce) enforce<kSurfPlane>(b, c, p);
+  gst(A.xy[par ^ 1] + q, make_double2(p.x, p.y));
+  gst(A.nd[par ^ 1] + q, make_double2(p.nx, p.ny));
+  gst(A.v + q, make_double2(p.vx, p.vy));
+  A.om[q] = om;
+  if (p.dix | p.diy | p.diz) {
+    const int s = A.slot[q];
+    if (p.dix) P.so.ix[s] += p.dix;
+    if (p.diy) P.so.iy[s] += p.diy;
+    if (p.diz) P.so.iz[s] += p.diz;
+  }
+  const double2 o = gld(A.o + q);
+  double ddx = o.x - p.x, ddy = o.y - p.y, ddz = 0.0;
+  min_image(b, ddx, ddy, ddz);
+  return !(ddx * ddx + ddy * ddy + ddz * ddz < td.pad2q);
+}
+
+// One fused Brownian step on the current set (launch layout as k_tile_step;
+// the stage holds positions and, with an aligner, directors).
+template <int POT, int AL, bool F32>
+__global__ void __launch_bounds__(kTileThreads, 5) k_tile_brownian(const TileBrownArgs* __restrict__ P, int par,
+                                                                   int t_off, int head_enforce, int store) {
+  const TileD& td = P->td;
+  TileCtl* ctl = td.ctl;
+  const int w = ctl_word(ctl);
+  const int S = (w ^ (w >> 1)) & 1;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    // commit a finished build, clear the request it answered
+    if (w & 2) atomicXor(&ctl->word, 3);
+    atomicAnd(&ctl->word, ~(4 << (par ^ 1)));
+    if (ctl->init) ctl->init = 0;
+  }
+  const TileSet& A = td.sets[S];
+  __shared__ double2 s_pos[kTileHaloMax + 1];
+  __shared__ double2 s_dir[AL ? kTileHaloMax + 1 : 1];
+  const int t = xcd_tile(td);
+  const TileGeom g = tile_geom(td, t);
+  if (!g.active) return;
+  const DescPtr D = tile_desc(td, t);
+  const int total = D[0], kb = D[1], ke = D[2], qa = D[3], nr = D[4];
+  const int round = (int)blockIdx.y;
+  if (round > 0 && kb + round * kTileThreads >= ke) return;
+  const double2* cur = A.xy[par];
+  const double2* dir = A.nd[par];
+  const bool staged = total <= kTileHaloMax;
+  if (staged) {
+    double2 v[kTileHaloMax / kTileThreads], d[kTileHaloMax / kTileThreads];
+#pragma unroll
+    for (int u = 0; u < kTileHaloMax / kTileThreads; ++u) {
+      const int k = (int)threadIdx.x + u * kTileThreads;
+      if (k < total) {
+        const int j = nr <= 3 ? desc_global3(D, k) : desc_global(D, k);
+        v[u] = gld(cur + j);
+        if (AL) d[u] = gld(dir + j);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kTileHaloMax / kTileThreads; ++u) {
+      const int k = (int)threadIdx.x + u * kTileThreads;
+      if (k < total) {
+        s_pos[k] = v[u];
+        if (AL) s_dir[k] = d[u];
+      }
+    }
+    if (threadIdx.x == 0) {
+      s_pos[total] = make_double2(1.0e30, 1.0e30);  // the lists' padding
+      if (AL) s_dir[total] = make_double2(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+  const bool wrap = P->b.periodic && (g.i0 == 0 || g.i0 + g.nc >= td.ncx || g.ty == 0 || g.ty == td.ncy - 1);
+  bool need = false;
+  const int k = kb + round * kTileThreads + (int)threadIdx.x;
+  if (k < ke)
+    need = tile_brownian_particle<POT, AL, F32>(*P, A, par, t_off, head_enforce, store, cur, dir, s_pos, s_dir,
+                                                staged, wrap, D, k, qa + (k - kb));
+  if (__ballot(need) != 0 && (threadIdx.x & 63) == 0) atomicOr(&ctl->word, 4 << par);
+}
+
 __global__ void k_tile_load(TileD td, Fields f, NListD nl) {
   const int p = gtid();
   if (p == 0) {
@@ -750,7 +984,8 @@ __global__ void k_tile_load(TileD td, Fields f, NListD nl) {
     td.sets[1].xy[0][p] = make_double2(f.x[p], f.y[p]);
     td.sets[1].o[p] = make_double2(nl.ox[p], nl.oy[p]);
     td.sets[1].v[p] = make_double2(f.vx[p], f.vy[p]);
-    td.sets[1].nd[p] = make_double2(f.nx[p], f.ny[p]);
+    td.sets[1].nd[0][p] = make_double2(f.nx[p], f.ny[p]);
+    td.sets[1].om[p] = f.omega[p];
     td.sets[1].id[p] = f.id[p];
     td.sets[1].slot[p] = p;
   }
@@ -767,7 +1002,8 @@ __global__ void k_tile_store(TileD td, Fields f, NListD nl, int par) {
     if (f.xy) f.xy[s] = p;
     const double2 v = td.sets[S].v[q];
     f.vx[s] = v.x; f.vy[s] = v.y;
-    const double2 d = td.sets[S].nd[q];
+    const double2 d = td.sets[S].nd[td.dirs2 ? par : 0][q];
+    f.omega[s] = td.sets[S].om[q];
     f.nx[s] = d.x; f.ny[s] = d.y;
     const double2 o = td.sets[S].o[q];
     nl.ox[s] = o.x; nl.oy[s] = o.y;
@@ -822,6 +1058,37 @@ void launch_tile_step(hipStream_t st, int pot_mask, int phase, int fp32, const T
     case 1: step_phase<1>(st, phase, fp32, args, grid, par, t_off, head_enforce); break;
     case 2: step_phase<2>(st, phase, fp32, args, grid, par, t_off, head_enforce); break;
     default: step_phase<3>(st, phase, fp32, args, grid, par, t_off, head_enforce); break;
+  }
+}
+
+template <int POT, int AL>
+void brownian_dispatch(hipStream_t st, int fp32, const TileBrownArgs* args, int grid, int par, int t_off,
+                       int head_enforce, int store) {
+  const dim3 g(grid, kTileMaxOwn / kTileThreads), blk(kTileThreads);
+  if (fp32)
+    hipLaunchKernelGGL((k_tile_brownian<POT, AL, true>), g, blk, 0, st, args, par, t_off, head_enforce, store);
+  else
+    hipLaunchKernelGGL((k_tile_brownian<POT, AL, false>), g, blk, 0, st, args, par, t_off, head_enforce, store);
+}
+
+template <int POT>
+void brownian_al(hipStream_t st, int al_mask, int fp32, const TileBrownArgs* args, int grid, int par, int t_off,
+                 int head_enforce, int store) {
+  if (al_mask == AL_POLAR)
+    brownian_dispatch<POT, AL_POLAR>(st, fp32, args, grid, par, t_off, head_enforce, store);
+  else if (al_mask == AL_NEMATIC)
+    brownian_dispatch<POT, AL_NEMATIC>(st, fp32, args, grid, par, t_off, head_enforce, store);
+  else
+    brownian_dispatch<POT, 0>(st, fp32, args, grid, par, t_off, head_enforce, store);
+}
+
+void launch_tile_brownian(hipStream_t st, int pot_mask, int al_mask, int fp32, const TileD& td,
+                          const TileBrownArgs* args, int par, int t_off, int head_enforce, int store) {
+  const int grid = tile_grid(td);
+  switch (pot_mask) {
+    case 1: brownian_al<1>(st, al_mask, fp32, args, grid, par, t_off, head_enforce, store); break;
+    case 2: brownian_al<2>(st, al_mask, fp32, args, grid, par, t_off, head_enforce, store); break;
+    default: brownian_al<3>(st, al_mask, fp32, args, grid, par, t_off, head_enforce, store); break;
   }
 }
 

@@ -11,11 +11,12 @@
 namespace samos {
 
 constexpr int kTileThreads = 256;
-constexpr int kTileMaxCells = 80;     // cells per tile along x (3 * (80 + 2) halo cells <= 256)
+constexpr int kTileMaxCells = 168;    // cells per tile along x (3 * (168 + 2) halo cells <= 2 * 256)
 constexpr int kTileHaloMax = 1024;    // halo positions staged in LDS; larger halos read L2
 constexpr int kTileStageMax = 65535;  // staged indices are 16-bit (the largest is the sentinel)
 constexpr int kTileMaxOwn = 2 * kTileThreads;  // particles of one tile (two per step-kernel thread)
 static_assert(kTileHaloMax % kTileThreads == 0, "the stage is loaded in whole rounds of the workgroup");
+static_assert(3 * (kTileMaxCells + 2) <= 2 * kTileThreads, "two halo cells per thread in stage_index");
 // Stage descriptor of a tile, written by each build for the new set:
 //   [0] halo size (stage entries)  [1], [2] stage range of the tile's own
 //   particles  [3] set index of the first of them  [4] runs
@@ -53,7 +54,9 @@ struct TileCtl {
 struct TileSet {
   double2* xy[2];        // positions, double-buffered by step
   double2* v;            // velocities
-  double2* nd;           // directors
+  double2* nd[2];        // directors: Langevin runs keep them in nd[0]; Brownian
+                         // runs double-buffer them by step like the positions
+  double* om;            // angular velocity omega (Brownian runs)
   double2* o;            // positions at the last build
   int* id;               // particle ids (noise keys)
   int* slot;             // slot of each particle in the slot-ordered columns
@@ -66,6 +69,7 @@ struct TileD {
   int T;                 // cells per tile along x
   int ntx, ntiles;       // tiles per row, tiles
   int K;                 // list capacity per particle (a multiple of 4)
+  int dirs2;             // directors double-buffered by step (Brownian runs)
   double wx, wy, xlo, ylo;
   double cut2;           // (rcut + pad)^2
   double pad2q;          // pad^2 / 4
@@ -84,10 +88,12 @@ struct TileD {
   TileCtl* ctl;
 };
 
-// Slot-ordered columns a step writes: closing forces, image-flag changes.
+// Slot-ordered columns a step writes: closing forces (and torques), image-flag
+// changes.
 struct SlotOut {
   double *fx, *fy, *fz;
   int *ix, *iy, *iz;
+  double *tx, *ty, *tz;
 };
 
 // Pair and self-propulsion constants of a one-type planar system.
@@ -123,8 +129,33 @@ struct TileStepArgs {
 // run's TileStepArgs in device memory (its td equal to the td given).
 void launch_tile_step(hipStream_t st, int pot_mask, int phase, int fp32, const TileD& td, const TileStepArgs* args,
                       int par, int t_off, int head_enforce);
-// Writes the current set (positions in buffer par) back into the slot-ordered
-// columns (x, y, xy, v, n) and the last build's snapshot into nl.ox / nl.oy.
+// The constant arguments of a tiled Brownian run (one IntegratorBrownian on
+// every particle of a planar system), kept in device memory.
+struct TileBrownArgs {
+  TileD td;
+  SlotOut so;
+  BoxD b;
+  ConstraintD c;
+  PairConst pc;
+  double dt, v0, mu, mur, stoch, flip_p;  // IntegratorBrownian: dt, v0, mu, mur, sqrt(nu dt), dt / tau
+  int nematic, velocity;                  //   nematic flips, velocity_align
+  TempD temp;
+  uint32_t seed;
+  const ClockD* clock;
+  int clock_slot;
+  double al_J, al_rc2;                    // aligner: effective J and rcut^2 (al_mask 0: unused)
+};
+// One fused Brownian step (k_brownian_fused's physics, mode 0) on the current
+// set: positions and directors from buffer par, written to par ^ 1.  td.dirs2
+// must be set for the run.
+// store: also write the loop-head forces and torques of this step to the
+// slot-ordered columns (the last step of a range: the state the reference
+// leaves behind).
+void launch_tile_brownian(hipStream_t st, int pot_mask, int al_mask, int fp32, const TileD& td,
+                          const TileBrownArgs* args, int par, int t_off, int head_enforce, int store);
+// Writes the current set (positions, and with dirs2 directors, in buffer par)
+// back into the slot-ordered columns (x, y, xy, v, n, omega) and the last
+// build's snapshot into nl.ox / nl.oy.
 void launch_tile_store(hipStream_t st, TileD td, Fields f, NListD nl, int par);
 
 }  // namespace samos
