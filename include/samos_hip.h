@@ -247,8 +247,9 @@ int samos_sync(samos_ctx* ctx);
  * (default); "gather_batch" 4 or 8 = neighbour gathers in flight per thread in
  * the fused step; "chunk_slots" (even, 2..256, default 32) = fused Langevin
  * steps per captured chunk, each chunk ending in one device-conditional
- * rebuild; "tile_cells" 0 = automatic, 1..80 = cells per workgroup tile of the
- * cell-tiled planar engine. */
+ * rebuild; "tile_cells" 0 = automatic, 1..168 = cells per workgroup tile of the
+ * cell-tiled planar engine; "brownian_lanes" 0 = automatic, 1, 2, 4, 8 or 16 =
+ * threads sharing one particle's neighbour list in the fused Brownian step. */
 int samos_set_option(samos_ctx* ctx, const char* name, double value);
 /* "neighbour_capacity" n (no counterpart in the reference, whose lists are
  * vectors): neighbour slots per particle, at least the current largest count;

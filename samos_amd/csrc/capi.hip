@@ -136,6 +136,7 @@ struct samos_ctx {
   int tile_key[5] = {-1, -1, -1, -1, -1};  // n, ncx, ncy, T, K of the allocation
   long long tile_stats[4] = {0, 0, 0, 0};  // tiled runs, rebuilds, largest candidate count, largest halo
   int tile_cells = 0;                      // cells per tile along x; 0: from the density
+  int brownian_lanes = 0;                  // threads per particle in the fused Brownian step; 0: automatic
   // further pair potentials (morse, gaussian, yukawa, soft_attractive), by name
   std::map<std::string, std::pair<int, std::vector<double>>> extras;
   std::map<std::string, std::pair<int, int>> extra_flags;  // use_particle_radii, shifted
@@ -1040,10 +1041,15 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
       ba.pos_dt = P->dt; ba.pos_mu = P->mu; ba.pos_temp = P->temp; ba.pos_seed = P->seed; ba.pos_slot = P->slot;
     }
     // small systems (the step is then latency-bound): several threads per
-    // slot, up to 16, while the launch stays under 64 Ki threads; larger
-    // systems measured faster with one thread per slot
+    // slot, up to 16, while the launch stays under 16 Ki threads; larger
+    // systems measured faster with one thread per slot (a sweep of 1-16
+    // threads on MI355X: 4096 particles fastest with 2, 100,000 with 1)
     ba.lanes = 1; ba.lane_shift = 0;
-    while (ba.lanes < 16 && (long)n * ba.lanes * 2 <= 65536L) { ba.lanes <<= 1; ++ba.lane_shift; }
+    if (ctx->brownian_lanes > 0) {
+      while (ba.lanes < ctx->brownian_lanes) { ba.lanes <<= 1; ++ba.lane_shift; }
+    } else {
+      while (ba.lanes < 16 && (long)n * ba.lanes * 2 <= 16384L) { ba.lanes <<= 1; ++ba.lane_shift; }
+    }
     {
       ProfScope ps(ctx, "fused");
       launch_brownian_fused(ctx->st, pot_mask(ctx), al_mask(ctx), cur, nxt, ctx->nl, ctx->box, ctx->con,
@@ -1066,7 +1072,7 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
     for (;;) {
       // the list's arrays (and so the chunks' key) change when it grows
       KeyBuf base;
-      base << 'B' << f32 << planar << pot_mask(ctx) << al_mask(ctx);
+      base << 'B' << f32 << planar << pot_mask(ctx) << al_mask(ctx) << ctx->brownian_lanes;
       key_context(base, ctx);
       key_integ(base, I);
       if (P) key_integ(base, *P);
@@ -2561,6 +2567,11 @@ int samos_set_option(samos_ctx* ctx, const char* name, double value) {
   } else if (k == "resort_every") {
     if (value < 0.0 || value != std::floor(value)) return fail(ctx, "resort_every must be a non-negative integer");
     ctx->resort_every = (int)value;
+  } else if (k == "brownian_lanes") {
+    const int v = (int)value;
+    if (value != v || !(v == 0 || v == 1 || v == 2 || v == 4 || v == 8 || v == 16))
+      return fail(ctx, "brownian_lanes must be 0 (automatic), 1, 2, 4, 8 or 16");
+    ctx->brownian_lanes = v;
   } else if (k == "tile_cells") {
     if (value < 0.0 || value > kTileMaxCells || value != std::floor(value))
       return fail(ctx, "tile_cells must be 0 (automatic) or an integer in 1.." + std::to_string(kTileMaxCells));

@@ -541,7 +541,7 @@ def test_fused_brownian_cases(gpu, case):
 
 
 def test_fused_brownian_one_thread_per_slot(gpu):
-    # above 32 Ki particles the fused Brownian step uses one thread per slot
+    # above 4 Ki particles the fused Brownian step uses one thread per slot
     st = sc.plane_disks(40000, phi=0.8, seed=34)
     d, o = both(gpu, st)
     for s in (d, o):
