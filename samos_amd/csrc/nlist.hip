@@ -258,24 +258,194 @@ __device__ __forceinline__ void build_epilogue(NListD& nl, bool overflow, int km
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&nl.flags[4], 1);
 }
 
-// NeighbourList::build_cell (src/system/neighbour_list.cpp:216-273), full list.
-// Cell members are consumed in batches so several candidate loads are in
-// flight per wavefront.
+// NeighbourList::build_cell (src/system/neighbour_list.cpp:216-273), full list:
+// candidate t of slot s is kept when d2 < (rcut + pad)^2; the coordination
+// counts the kept candidates closer than r_s + r_t.
+template <bool PLANAR>
+__device__ __forceinline__ void build_candidate(const NListD& nl, const BoxD& b, int s, double xi, double yi,
+                                                double zi, double ri, bool wrap, int t, double xj, double yj,
+                                                double zj, double rj, int& k, int& coord) {
+  if (t == s) return;
+  double dx = xi - xj, dy = yi - yj, dz = PLANAR ? 0.0 : zi - zj;
+  if (wrap) {
+    if (PLANAR) min_image_xy(b, dx, dy);
+    else min_image(b, dx, dy, dz);
+  }
+  // planar: dz = 0, and adding +0 to a sum of squares changes no bit
+  const double d2 = PLANAR ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
+  if (d2 < nl.cut2) {
+    if (k < nl.stride) nl.nbr[(long)k * nl.np + s] = t;
+    ++k;
+    const double r = ri + rj;
+    if (d2 < r * r) ++coord;
+  }
+}
+
+// Candidates from the 3 x 3 (x 3) cells around the particle's own, read from
+// global memory: cell members are consumed in batches so several candidate
+// loads are in flight per thread.
+template <bool PLANAR>
+__device__ __forceinline__ void build_cells_global(const Fields& f, const NListD& nl, const BoxD& b, int s, int c,
+                                                   double xi, double yi, double zi, double ri, bool runi, bool wrap,
+                                                   int& k, int& coord) {
+  constexpr int U = 8;
+  const int ci = c / (nl.ncy * nl.ncz), cj = (c / nl.ncz) % nl.ncy, ck = c % nl.ncz;
+  // Planar: every particle is in the same z-layer ck, and a used cell grid
+  // has ncz >= 2, so the stencil's other z-layers hold nobody and the own
+  // layer appears exactly once; only it is visited.
+  const int iz_lo = PLANAR ? 0 : -1, iz_hi = PLANAR ? 0 : 1;
+  for (int ix = -1; ix <= 1; ix++) {
+    int iix = ci + ix;
+    if (iix < 0) iix = nl.ncx - 1; else if (iix == nl.ncx) iix = 0;
+    for (int iy = -1; iy <= 1; iy++) {
+      int iiy = cj + iy;
+      if (iiy < 0) iiy = nl.ncy - 1; else if (iiy == nl.ncy) iiy = 0;
+      for (int iz = iz_lo; iz <= iz_hi; iz++) {
+        int iiz = ck + iz;
+        if (iiz < 0) iiz = nl.ncz - 1; else if (iiz == nl.ncz) iiz = 0;
+        const int cc = nl.ncy * nl.ncz * iix + nl.ncz * iiy + iiz;
+        const int a = nl.start[cc], e = nl.start[cc + 1];
+        for (int m0 = a; m0 < e; m0 += U) {
+          int t[U];
+          double xj[U], yj[U], zj[U], rj[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) t[u] = (m0 + u < e) ? nl.members[m0 + u] : s;
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            if (PLANAR) { const double2 p = f.xy[t[u]]; xj[u] = p.x; yj[u] = p.y; zj[u] = zi; }
+            else { xj[u] = f.x[t[u]]; yj[u] = f.y[t[u]]; zj[u] = f.z[t[u]]; }
+            rj[u] = runi ? nl.r_uni : f.radius[t[u]];
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            if (m0 + u >= e) break;
+            build_candidate<PLANAR>(nl, b, s, xi, yi, zi, ri, wrap, t[u], xj[u], yj[u], zj[u], rj[u], k, coord);
+          }
+        }
+      }
+    }
+  }
+}
+
+// Planar builds stage candidates per wavefront: the 64 slots of a wavefront
+// (consecutive along the Morton order) lie in a small block of cells, so the
+// members and positions of that block plus one cell around it are loaded into
+// LDS once, coalesced, and every lane reads its 3 x 3 stencil from there in
+// the same order as the global path (the same list, entry for entry).  A
+// wavefront whose block touches the grid's edge (stencil cells that wrap) or
+// does not fit the stage takes the global path.
+constexpr int kStageCells = 64;   // cells of a wavefront's block, halo included (one lane each)
+constexpr int kStageCand = 448;   // staged candidates per wavefront
+static_assert(kStageCells <= 64, "the cell table is filled and scanned by the lanes of one wavefront");
+struct WaveStage {
+  int cstart[kStageCells + 1];    // first staged entry of each block cell; [nq] = total
+  int gstart[kStageCells];        // its first index in members[]
+  int mem[kStageCand];            // staged slots
+  double2 pos[kStageCand];        // their (x, y)
+};
+
+// LDS written by some lanes of a wavefront and read by others of the same one.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Every lane of the wavefront calls it (lanes past n with active false).
+// Returns false (nothing staged) when the wavefront takes the global path.
+__device__ bool stage_wave_block(const Fields& f, const NListD& nl, WaveStage& S, bool active, int c, int& bx0,
+                                 int& by0, int& bw) {
+  const int lane = threadIdx.x & 63;
+  const int ci = c / (nl.ncy * nl.ncz), cj = (c / nl.ncz) % nl.ncy, ck = c % nl.ncz;
+  const int imin = wave_min_i(active ? ci : INT_MAX), imax = wave_max_i(active ? ci : INT_MIN);
+  const int jmin = wave_min_i(active ? cj : INT_MAX), jmax = wave_max_i(active ? cj : INT_MIN);
+  const int kmin = wave_min_i(active ? ck : INT_MAX), kmax = wave_max_i(active ? ck : INT_MIN);
+  if (imin > imax) return false;  // no active lane
+  bx0 = imin - 1; by0 = jmin - 1;
+  bw = imax - imin + 3;
+  const int bh = jmax - jmin + 3, nq = bw * bh;
+  if (kmin != kmax || bx0 < 0 || imax + 1 >= nl.ncx || by0 < 0 || jmax + 1 >= nl.ncy || nq > kStageCells)
+    return false;
+  int cnt = 0, gs = 0;
+  if (lane < nq) {
+    const int cc = nl.ncy * nl.ncz * (bx0 + lane % bw) + nl.ncz * (by0 + lane / bw) + kmin;
+    gs = nl.start[cc];
+    cnt = nl.start[cc + 1] - gs;
+  }
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
+  }
+  const int total = __shfl(incl, 63, 64);
+  if (total > kStageCand) return false;
+  if (lane < nq) { S.cstart[lane] = incl - cnt; S.gstart[lane] = gs; }
+  if (lane == 0) S.cstart[nq] = total;
+  wave_lds_sync();
+  for (int q = lane; q < total; q += 64) {
+    int lo = 0, hi = nq - 1;  // the last block cell starting at or before q
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (S.cstart[mid] <= q) lo = mid; else hi = mid - 1;
+    }
+    const int t = nl.members[S.gstart[lo] + (q - S.cstart[lo])];
+    S.mem[q] = t;
+    S.pos[q] = f.xy[t];
+  }
+  wave_lds_sync();
+  return true;
+}
+
+// The same stencil walk as build_cells_global, from the wavefront's stage.
+__device__ __forceinline__ void build_cells_staged(const NListD& nl, const BoxD& b, const WaveStage& S, int bx0,
+                                                   int by0, int bw, int s, int c, double xi, double yi, double ri,
+                                                   double rj_u, bool wrap, int& k, int& coord) {
+  const int ci = c / (nl.ncy * nl.ncz), cj = (c / nl.ncz) % nl.ncy;
+  for (int ix = -1; ix <= 1; ix++) {
+    for (int iy = -1; iy <= 1; iy++) {
+      const int q = (ci + ix - bx0) + (cj + iy - by0) * bw;
+      const int a = S.cstart[q], e = S.cstart[q + 1];
+      for (int m = a; m < e; ++m) {
+        const double2 p = S.pos[m];
+        build_candidate<true>(nl, b, s, xi, yi, 0.0, ri, wrap, S.mem[m], p.x, p.y, 0.0, rj_u, k, coord);
+      }
+    }
+  }
+}
+
 template <bool PLANAR>
 __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n, int* ovf2 = nullptr) {
   bool overflow = false;
   int kmax = 0;
-  constexpr int U = 8;
-  for (int s = xcd_block() * blockDim.x + threadIdx.x; s < n; s += gsize()) {
+  __shared__ WaveStage stage[PLANAR ? kBlock / 64 : 1];
+  const int lane = threadIdx.x & 63;
+  // whole wavefronts step together (the planar stage is per wavefront)
+  for (int s0 = xcd_block() * blockDim.x + threadIdx.x - lane; s0 < n; s0 += gsize()) {
+    const int s = s0 + lane;
+    const bool active = s < n;
+    const int c = active ? nl.cell_of[s] : 0;
+    const bool runi = nl.r_uni >= 0.0;
+    int bx0 = 0, by0 = 0, bw = 0;
+    // planar with one common radius: candidates from the wavefront's stage
+    const bool staged = PLANAR && runi && stage_wave_block(f, nl, stage[PLANAR ? threadIdx.x >> 6 : 0], active, c,
+                                                           bx0, by0, bw);
+    if (!active) continue;
     double xi, yi, zi;
     if (PLANAR) { const double2 p = f.xy[s]; xi = p.x; yi = p.y; zi = f.z[s]; }
     else { xi = f.x[s]; yi = f.y[s]; zi = f.z[s]; }
     // one common radius (the usual case): the coordination test needs no
     // per-candidate radius gather
-    const bool runi = nl.r_uni >= 0.0;
     const double ri = runi ? nl.r_uni : f.radius[s];
-    const int c = nl.cell_of[s];
-    const int ci = c / (nl.ncy * nl.ncz), cj = (c / nl.ncz) % nl.ncy, ck = c % nl.ncz;
     int k = 0, coord = 0;
     // Candidates come from the 3 x 3 (x 3) cells around the particle's own,
     // all within two cell widths of it per axis.  Farther than that from every
@@ -288,54 +458,11 @@ __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n, int* ov
       if (!PLANAR) inner = inner && zi - b.zlo > Rz && b.zhi - zi > Rz;
       wrap = !inner;
     }
-    // Planar: every particle is in the same z-layer ck, and a used cell grid
-    // has ncz >= 2, so the stencil's other z-layers hold nobody and the own
-    // layer appears exactly once; only it is visited.
-    const int iz_lo = PLANAR ? 0 : -1, iz_hi = PLANAR ? 0 : 1;
-    for (int ix = -1; ix <= 1; ix++) {
-      int iix = ci + ix;
-      if (iix < 0) iix = nl.ncx - 1; else if (iix == nl.ncx) iix = 0;
-      for (int iy = -1; iy <= 1; iy++) {
-        int iiy = cj + iy;
-        if (iiy < 0) iiy = nl.ncy - 1; else if (iiy == nl.ncy) iiy = 0;
-        for (int iz = iz_lo; iz <= iz_hi; iz++) {
-          int iiz = ck + iz;
-          if (iiz < 0) iiz = nl.ncz - 1; else if (iiz == nl.ncz) iiz = 0;
-          const int cc = nl.ncy * nl.ncz * iix + nl.ncz * iiy + iiz;
-          const int a = nl.start[cc], e = nl.start[cc + 1];
-          for (int m0 = a; m0 < e; m0 += U) {
-            int t[U];
-            double xj[U], yj[U], zj[U], rj[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) t[u] = (m0 + u < e) ? nl.members[m0 + u] : s;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-              if (PLANAR) { const double2 p = f.xy[t[u]]; xj[u] = p.x; yj[u] = p.y; zj[u] = zi; }
-              else { xj[u] = f.x[t[u]]; yj[u] = f.y[t[u]]; zj[u] = f.z[t[u]]; }
-              rj[u] = runi ? nl.r_uni : f.radius[t[u]];
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-              if (m0 + u >= e) break;
-              if (t[u] == s) continue;
-              double dx = xi - xj[u], dy = yi - yj[u], dz = PLANAR ? 0.0 : zi - zj[u];
-              if (wrap) {
-                if (PLANAR) min_image_xy(b, dx, dy);
-                else min_image(b, dx, dy, dz);
-              }
-              // planar: dz = 0, and adding +0 to a sum of squares changes no bit
-              const double d2 = PLANAR ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
-              if (d2 < nl.cut2) {
-                if (k < nl.stride) nl.nbr[(long)k * nl.np + s] = t[u];
-                ++k;
-                const double r = ri + rj[u];
-                if (d2 < r * r) ++coord;
-              }
-            }
-          }
-        }
-      }
-    }
+    if (PLANAR && staged)
+      build_cells_staged(nl, b, stage[PLANAR ? threadIdx.x >> 6 : 0], bx0, by0, bw, s, c, xi, yi, ri, nl.r_uni, wrap,
+                         k, coord);
+    else
+      build_cells_global<PLANAR>(f, nl, b, s, c, xi, yi, zi, ri, runi, wrap, k, coord);
     if (k > nl.stride) overflow = true;
     kmax = max(kmax, k);
     nl.nnb[s] = min(k, nl.stride);
