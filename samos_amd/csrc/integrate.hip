@@ -432,6 +432,30 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
     const int ti = MT ? cur.type[s] - 1 : 0;
     double ri = 0.0;
     if (((POT & POT_SOFT) && ip.soft_radii) || ((POT & POT_LJ) && ip.lj_radii)) ri = cur.radius[s];
+    // outside the integrator's group a slot is not integrated; it only takes
+    // the next loop head's enforce
+    const bool member = in_group(cur, s, fa.gbit);
+    // The O step's Gaussian draw does not depend on the forces: it is made
+    // before the pair sum, so its arithmetic overlaps the neighbour gathers.
+    double B = 0.0, gn0 = 0.0, gn1 = 0.0, gn2 = 0.0;
+    if (PHASE <= 1) {
+      B = Bc;
+      if (B < 0.0) {
+        const double T = temp.at(eff_step(clk, t_off));
+        B = sqrt(T * (1.0 - exp(-2.0 * gamma * dt)));
+      }
+      if (member && B != 0.0) {
+        const unsigned long long call = clk ? clk->calls[slot] + (unsigned long long)t_off : fa.call0;
+        double g[3];
+        if (PLANAR) {  // the z draw only reaches v_z, which the next enforce resets
+          gauss2(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], call, g);
+          g[2] = 0.0;
+        } else {
+          gauss3(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], call, g);
+        }
+        gn0 = g[0]; gn1 = g[1]; gn2 = g[2];
+      }
+    }
     double fx = 0.0, fy = 0.0, fz = 0.0;
     pair_force_sum<POT, F32, PLANAR, MT, BATCH>(cur, nl, b, ip, s, xi, yi, zi, ti, ri, fx, fy, fz);
     if (ip.has_sp) { fx += ip.sp_alpha * cur.nx[s]; fy += ip.sp_alpha * cur.ny[s]; fz += ip.sp_alpha * cur.nz[s]; }
@@ -442,20 +466,12 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
     const double dt2 = 0.5 * dt;
     const double m = mass_u > 0.0 ? mass_u : cur.mass[s];
     const double fact = mass_u > 0.0 ? fa.fact_u : dt2 / m;
-    // outside the integrator's group a slot is not integrated; it only takes
-    // the next loop head's enforce
-    const bool member = in_group(cur, s, fa.gbit);
     double vx = cur.vx[s], vy = cur.vy[s], vz = PLANAR ? 0.0 : cur.vz[s];
     if (PHASE >= 1 && member) {  // closing B of step t-1
       vx += fact * fx; vy += fact * fy; vz += fact * fz;
       if (!(fa.age && fa.age->uniform)) cur.age[s] += dt;
     }
     if (PHASE <= 1) {
-      double B = Bc;
-      if (B < 0.0) {
-        const double T = temp.at(eff_step(clk, t_off));
-        B = sqrt(T * (1.0 - exp(-2.0 * gamma * dt)));
-      }
       PS p;
       p.x = xi; p.y = yi; p.z = zi; p.vx = vx; p.vy = vy; p.vz = vz;
       // On a plane only the enforce's normalisation changes a director, and
@@ -484,15 +500,7 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
         p.vx *= exp_dt; p.vy *= exp_dt; p.vz *= exp_dt;
         if (B != 0.0) {
           const double sf = fa.sf_u >= 0.0 ? fa.sf_u : B / sqrt(m);
-          const unsigned long long call = clk ? clk->calls[slot] + (unsigned long long)t_off : fa.call0;
-          double g[3];
-          if (PLANAR) {  // the z draw only reaches v_z, which the next enforce resets
-            gauss2(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], call, g);
-            g[2] = 0.0;
-          } else {
-            gauss3(seed, TAG_LANGEVIN, (uint32_t)cur.id[s], call, g);
-          }
-          p.vx += sf * g[0]; p.vy += sf * g[1]; p.vz += sf * g[2];
+          p.vx += sf * gn0; p.vy += sf * gn1; p.vz += sf * gn2;
         }
         p.x += dt2 * p.vx; p.y += dt2 * p.vy; p.z += dt2 * p.vz;
         // on a plane the director was normalised by the first enforce
