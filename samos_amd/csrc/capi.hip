@@ -493,7 +493,7 @@ int resort(samos_ctx* ctx) {
   if (!ctx->has_nlist || !ctx->nl.use_cell || ctx->n < 2) return 0;
   const int n = ctx->n;
   Fields f = fields(ctx);
-  launch_morton_keys(ctx->st, f, ctx->nl, ctx->box, n, ctx->keys, ctx->perm);
+  launch_morton_keys(ctx->st, f, ctx->nl, ctx->box, n, ctx->keys, ctx->perm, ctx->con.kind == 1 ? 1 : 0);
   radix_sort_pairs(ctx->st, ctx->sort_tmp, ctx->sort_tmp_bytes, ctx->keys, ctx->keys2, ctx->perm, ctx->perm2, n);
   for (int k = 0; k < kNumDoubleFields; ++k) {
     launch_gather_d(ctx->st, ctx->dtmp, ctx->dcol[k], ctx->perm2, n);
@@ -512,6 +512,21 @@ int resort(samos_ctx* ctx) {
   HIPCHK(hipMemcpyAsync(ctx->h_id.data(), ctx->icol[5], sizeof(int) * n, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Re-sorts the storage and rebuilds the list from its permuted snapshot: the
+// same pair set as the last build, which stays the one counted.
+int resort_at_snapshot(samos_ctx* ctx) {
+  int h[8];
+  if (int r = read_flags(ctx, h)) return r;
+  if (int r = resort(ctx)) return r;
+  Fields at = fields(ctx);
+  at.x = ctx->nl.ox; at.y = ctx->nl.oy; at.z = ctx->nl.oz;
+  if (int r = build_sync(ctx, &at, false)) return r;
+  HIPCHK(hipMemcpyAsync(ctx->d_flags + 4, &h[4], sizeof(int), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->builds_at_sort = h[4];
   return 0;
 }
 
@@ -542,20 +557,13 @@ int resort_in_run(samos_ctx* ctx, bool planar, int par, bool directors) {
     f0.xy = ctx->xy[0];
     launch_unpack_xy(ctx->st, f0, ctx->n);
   }
-  int h[8];
-  if (int r = read_flags(ctx, h)) return r;
-  if (int r = resort(ctx)) return r;
-  Fields at = fields(ctx);
-  at.x = ctx->nl.ox; at.y = ctx->nl.oy; at.z = ctx->nl.oz;
-  if (int r = build_sync(ctx, &at, false)) return r;
-  HIPCHK(hipMemcpyAsync(ctx->d_flags + 4, &h[4], sizeof(int), hipMemcpyHostToDevice, ctx->st));
+  if (int r = resort_at_snapshot(ctx)) return r;
   if (planar) {
     Fields f0 = fields(ctx, 0);
     f0.xy = ctx->xy[0];
     launch_pack_xy(ctx->st, f0, ctx->n);
   }
   HIPCHK(hipStreamSynchronize(ctx->st));
-  ctx->builds_at_sort = h[4];
   ctx->resorts++;
   return 0;
 }
@@ -1912,12 +1920,17 @@ int samos_external_self_propulsion(samos_ctx* ctx, double alpha) {
 
 int samos_constraint(samos_ctx* ctx, int kind, double param, int flag) {
   if (kind < 0 || kind > 2) return fail(ctx, "unknown constraint kind");
+  const int was = ctx->con.kind;
   ctx->con.kind = kind;
   if (kind == 1) { ctx->con.zpos = param; ctx->con.unlimited = flag ? 1 : 0; }
   if (kind == 2) ctx->con.r = param;
   launch_enforce(ctx->st, fields(ctx), ctx->box, ctx->con, ctx->n);
   HIPCHK(hipStreamSynchronize(ctx->st));
   HIPCHK(hipGetLastError());
+  // a plane keeps its slots in Hilbert order (resort); a list made before
+  // the constraint was set is re-sorted into it (the same pair set)
+  if (kind != was && (kind == 1 || was == 1) && ctx->has_nlist && ctx->nl.use_cell && ctx->n >= 2)
+    return resort_at_snapshot(ctx);
   return 0;
 }
 

@@ -22,7 +22,11 @@ void launch_rebuild(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const 
 // when any particle moved by pad/2 or more.
 void launch_need_update(hipStream_t st, Fields f, NListD nl, BoxD box, int n, int* req_out);
 // Morton keys of each slot's cell, for the locality re-sort.
-void launch_morton_keys(hipStream_t st, Fields f, NListD nl, BoxD box, int n, unsigned long long* keys, int* slots);
+// Storage-order keys of every slot: plane = 1 orders each z layer of cells
+// along a 2-D Hilbert curve (layer-major), plane = 0 the grid along a 3-D
+// Morton curve.
+void launch_morton_keys(hipStream_t st, Fields f, NListD nl, BoxD box, int n, unsigned long long* keys, int* slots,
+                        int plane);
 void launch_gather_d(hipStream_t st, double* dst, const double* src, const int* perm, int n);
 void launch_gather_i(hipStream_t st, int* dst, const int* src, const int* perm, int n);
 // Half pair export: for slot s, neighbours t with id[t] > id[s]; writes (id_lo, id_hi) rows.

@@ -633,7 +633,32 @@ __device__ __forceinline__ unsigned long long spread3(unsigned long long v) {
   return v;
 }
 
-__global__ void k_morton(Fields f, NListD nl, BoxD b, int n, unsigned long long* keys, int* slots) {
+// Position of cell (x, y) along the Hilbert curve over a 2^order x 2^order
+// grid: the iterative form that reads one bit of each coordinate per level,
+// from the top, adds the quadrant's offset and rotates or reflects the
+// coordinates into the quadrant's frame (Hilbert curve, "xy2d").
+__device__ __forceinline__ unsigned long long hilbert2(unsigned x, unsigned y, int order) {
+  const unsigned side = 1u << order;
+  unsigned long long d = 0;
+  for (unsigned h = side >> 1; h > 0; h >>= 1) {
+    const unsigned rx = (x & h) ? 1u : 0u, ry = (y & h) ? 1u : 0u;
+    d += (unsigned long long)h * h * ((3u * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) { x = side - 1 - x; y = side - 1 - y; }
+      const unsigned t = x; x = y; y = t;
+    }
+  }
+  return d;
+}
+
+// Storage keys.  Plane-constrained systems (every particle in one z layer of
+// cells) are ordered along a 2-D Hilbert curve per layer, whose consecutive
+// cells are always adjacent, so the 64 slots of a wavefront cover a compact
+// block of cells (the planar build stages such blocks); other systems along
+// a 3-D Morton curve of the grid.
+__global__ void k_morton(Fields f, NListD nl, BoxD b, int n, unsigned long long* keys, int* slots, int plane) {
+  const int m = max(nl.ncx, nl.ncy);
+  const int order = m > 1 ? 32 - __clz(m - 1) : 0;
   for (int s = gtid(); s < n; s += gsize()) {
     int i = (int)((f.x[s] - b.xlo) / nl.wx);
     int j = (int)((f.y[s] - b.ylo) / nl.wy);
@@ -641,7 +666,8 @@ __global__ void k_morton(Fields f, NListD nl, BoxD b, int n, unsigned long long*
     i = min(max(i, 0), nl.ncx - 1);
     j = min(max(j, 0), nl.ncy - 1);
     k = min(max(k, 0), nl.ncz - 1);
-    keys[s] = spread3(i) | (spread3(j) << 1) | (spread3(k) << 2);
+    keys[s] = plane ? ((unsigned long long)k << 42) | hilbert2((unsigned)i, (unsigned)j, order)
+                    : spread3(i) | (spread3(j) << 1) | (spread3(k) << 2);
     slots[s] = s;
   }
 }
@@ -759,8 +785,9 @@ void launch_need_update(hipStream_t st, Fields f, NListD nl, BoxD box, int n, in
   hipLaunchKernelGGL(k_need_update, dim3(grid_for(n)), dim3(kBlock), 0, st, f, nl, box, n, req_out);
 }
 
-void launch_morton_keys(hipStream_t st, Fields f, NListD nl, BoxD box, int n, unsigned long long* keys, int* slots) {
-  hipLaunchKernelGGL(k_morton, dim3(grid_for(n)), dim3(kBlock), 0, st, f, nl, box, n, keys, slots);
+void launch_morton_keys(hipStream_t st, Fields f, NListD nl, BoxD box, int n, unsigned long long* keys, int* slots,
+                        int plane) {
+  hipLaunchKernelGGL(k_morton, dim3(grid_for(n)), dim3(kBlock), 0, st, f, nl, box, n, keys, slots, plane);
 }
 
 void launch_gather_d(hipStream_t st, double* dst, const double* src, const int* perm, int n) {
