@@ -79,7 +79,10 @@ int samos_set_timestep(samos_ctx* ctx, double dt);
 /* ---- neighbour list (src/system/neighbour_list.hpp:124-272) -------------- */
 /* NeighbourList(sys, msg, rcut, pad, param): chooses the cell build when every
  * box side exceeds 2 (rcut + pad) (neighbour_list.hpp:138-150), else N^2,
- * re-sorts particles along a Morton curve of the cell grid, and builds. */
+ * re-sorts particles along a curve of the cell grid (Hilbert per cell layer
+ * under the plane constraint, else Morton), and builds.  Setting the plane
+ * constraint (or leaving it) with a list present re-sorts into that order and
+ * rebuilds from the list's snapshot (the same pair set). */
 int samos_nlist_create(samos_ctx* ctx, double rcut, double pad);
 /* NeighbourList::build (src/system/neighbour_list.cpp:117-149). */
 int samos_nlist_build(samos_ctx* ctx);
@@ -101,7 +104,8 @@ int samos_nlist_pairs(samos_ctx* ctx, int* out, long long cap, long long* count)
  * array, so calls that overlap on one GPU (several host threads or contexts
  * at once) race; calls one after another are independent. */
 int samos_nlist_pairs_noalloc(samos_ctx* ctx, int* out, long long cap, long long* count);
-/* Re-sort particle storage along the Morton curve of the cell grid and rebuild. */
+/* Re-sort particle storage along the curve of the cell grid (Hilbert per
+ * layer under the plane constraint, else Morton) and rebuild. */
 int samos_resort(samos_ctx* ctx);
 
 /* ---- pair potentials (src/potentials/pair/pair_potential.hpp:52-158) ----- */
@@ -140,7 +144,10 @@ int samos_external_self_propulsion(samos_ctx* ctx, double alpha);
 
 /* ---- constraint (src/constraints/constraint.hpp:47-161) ------------------ */
 /* kind NONE; PLANE with param = zpos, flag = unlimited; SPHERE with param = r.
- * Like the `constraint` command (src/samos.cpp:510-522) it enforces at once. */
+ * Like the `constraint` command (src/samos.cpp:510-522) it enforces at once.
+ * Entering or leaving the plane with a cell list present re-sorts the storage
+ * (see samos_nlist_create); ids, the pair set and the trajectory are
+ * unchanged up to the order of the force sums. */
 int samos_constraint(samos_ctx* ctx, int kind, double param, int flag);
 /* Further surfaces, enforced at once like samos_constraint:
  *   CYLINDER  params {r}        (src/constraints/constraint_cylinder.hpp:42-67, .cpp:30-66)
@@ -258,13 +265,13 @@ int samos_set_option(samos_ctx* ctx, const char* name, double value);
  * list; the capacity then grows and the list is rebuilt from the same
  * positions, so the run continues exactly as with room to spare.
  * "resort_every" k (no counterpart in the reference): inside a run, re-sort
- * the particles' storage along the Morton order of their cells after every k
+ * the particles' storage along the curve order of their cells after every k
  * rebuilds of the captured steps (default 200; 0 = never), keeping the
  * neighbour gathers local over long runs; the pair set and the trajectory
  * are unchanged up to the order of the force sums.
  * Run statistics, no counterpart in the reference: out[5] = step-chunk graphs
  * instantiated, chunk graphs reused, list overflows recovered, capacity,
- * Morton re-sorts inside runs. */
+ * storage re-sorts inside runs. */
 int samos_run_stats(samos_ctx* ctx, long long* out);
 /* Cell-tiled planar engine (no counterpart in the reference): planar fused
  * Langevin runs keep the state in cell order and the Verlet list as 16-bit

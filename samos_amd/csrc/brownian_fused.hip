@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nx
   if (ck) {
     const int j = ba.slot;
     if (nl.flags[2]) return;  // the list overflowed at a rebuild: no step may read it
-    if (nl.flags[4] >= ck->resort_at) {  // a Morton re-sort is due before this slot
+    if (nl.flags[4] >= ck->resort_at) {  // a storage re-sort is due before this slot
       if (blockIdx.x == 0 && threadIdx.x == 0) ck->stop |= 2;
       return;
     }

@@ -122,7 +122,7 @@ struct samos_ctx {
   unsigned long long graph_tick = 0;
   long long graph_stats[2] = {0, 0};  // instantiations, reuses
   long long overflow_recoveries = 0;  // list overflows recovered inside runs
-  int resort_every = 200;             // list builds between Morton re-sorts inside runs (0: never)
+  int resort_every = 200;             // list builds between storage re-sorts inside runs (0: never)
   int builds_at_sort = 0;             // the builds count after the last re-sort
   long long resorts = 0;              // re-sorts done inside runs
   // cell-tiled engine of planar fused runs (allocated on first use)
@@ -345,7 +345,7 @@ int chunk_graphs(samos_ctx* ctx, const std::string& base, const std::function<in
 // when the batch stopped at a neighbour-list overflow.
 // stall: rounds that may end without progress (a request left before the
 // chunks stops their first slot; the chunk's rebuild serves it).  Returns 2
-// when the slots stopped because a Morton re-sort is due.
+// when the slots stopped because a storage re-sort is due.
 int replay_chunks(samos_ctx* ctx, const ChunkSet& set, int end, int* t, int stall = 0) {
   const int Mmax = set.back().first;
   double per = Mmax;
@@ -530,7 +530,7 @@ int resort_at_snapshot(samos_ctx* ctx) {
   return 0;
 }
 
-// Morton re-sort between captured chunks of a run (particles drift away from
+// Storage re-sort between captured chunks of a run (particles drift away from
 // their storage neighbours over long runs, and the gathers lose locality).
 // The positions are in buffer 0 then; every per-slot column, the list's
 // snapshot among them, is permuted, and the list is rebuilt from the

@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
     // overflowed at a rebuild stops every later slot before any step reads it.
     const int j = fa.slot;
     if (nl.flags[2]) return;
-    if (nl.flags[4] >= ck->resort_at) {  // a Morton re-sort is due before this slot
+    if (nl.flags[4] >= ck->resort_at) {  // a storage re-sort is due before this slot
       if (blockIdx.x == 0 && threadIdx.x == 0) ck->stop |= 2;
       return;
     }

@@ -21,7 +21,7 @@ void launch_rebuild(hipStream_t st, Fields f, NListD nl, BoxD box, int n, const 
 // Displacement test of every particle against the last build; sets *req_out to 1
 // when any particle moved by pad/2 or more.
 void launch_need_update(hipStream_t st, Fields f, NListD nl, BoxD box, int n, int* req_out);
-// Morton keys of each slot's cell, for the locality re-sort.
+// Curve keys of each slot's cell, for the locality re-sort.
 // Storage-order keys of every slot: plane = 1 orders each z layer of cells
 // along a 2-D Hilbert curve (layer-major), plane = 0 the grid along a 3-D
 // Morton curve.
@@ -183,7 +183,7 @@ void launch_chunk_finish(hipStream_t st, ChunkCtl* ck, ClockD* clock, const int*
 // stages of the rebuild do nothing unless a slot asked for one).
 void launch_chunk_rebuild(hipStream_t st, ChunkCtl* ck, ClockD* clk, Fields f0, Fields f1, NListD nl, BoxD box,
                           int n, int planar, int directors);
-// resort_at: builds count at which the slots stop for a Morton re-sort.
+// resort_at: builds count at which the slots stop for a storage re-sort.
 // clk (optional): the device clock is set to noise call `calls` in slot
 // `slot` and run step `base`.
 void launch_chunk_init(hipStream_t st, ChunkCtl* ck, int base, int limit, int steps, int resort_at,

@@ -38,7 +38,7 @@ __device__ __forceinline__ int gsize() { return gridDim.x * blockDim.x; }
 // The index is clamped to the grid: the reference indexes one past the grid for
 // a particle exactly on the high face.
 // Adds 1 to counter[key] for every active lane, one atomic per distinct key
-// in the wavefront (slots in Morton order share cells in runs).  Every lane
+// in the wavefront (slots in curve order share cells in runs).  Every lane
 // of the wavefront must call it.
 __device__ __forceinline__ void wave_count(int* counter, int key, bool active) {
   unsigned long long todo = __ballot(active);
@@ -328,7 +328,7 @@ __device__ __forceinline__ void build_cells_global(const Fields& f, const NListD
 }
 
 // Planar builds stage candidates per wavefront: the 64 slots of a wavefront
-// (consecutive along the Morton order) lie in a small block of cells, so the
+// (consecutive along the storage curve) lie in a small block of cells, so the
 // members and positions of that block plus one cell around it are loaded into
 // LDS once, coalesced, and every lane reads its 3 x 3 stencil from there in
 // the same order as the global path (the same list, entry for entry).  A

@@ -1,7 +1,8 @@
 // Device-side data model and shared helpers for the SAMoS particle pipeline on
 // CDNA4 (gfx950).  All particle state lives in HBM as structure-of-arrays FP64
-// columns in "slot" order: slots are a Morton (Z-order) ordering of the cell
-// grid, so particles that are close in space are close in memory.  The
+// columns in "slot" order: slots follow a space-filling curve over the cell
+// grid (Morton / Z-order in 3-D, Hilbert per cell layer on a plane), so
+// particles that are close in space are close in memory.  The
 // original particle id of each slot is kept in `id`, so every export to the
 // host is in the reference's id order.
 #pragma once
@@ -175,7 +176,7 @@ struct ChunkCtl {
   int last;      // slots executed in this chunk
   int base;      // run step of slot 0
   int stop;      // why the chunk's slots stopped early (next to base: one read-back):
-                 // bit 0 the list overflowed at a rebuild, bit 1 a Morton re-sort is due
+                 // bit 0 the list overflowed at a rebuild, bit 1 a storage re-sort is due
   int resort_at; // builds count (NListD::flags[4]) at which slots stop for a re-sort
   int limit;     // slot j runs only while base + j < limit
   int rebuild;   // rebuild request read by the chunk's rebuild stages
