@@ -138,6 +138,84 @@ __global__ void __launch_bounds__(kThreads) k_gemm(const bf16_t* __restrict__ A,
 // Rows are staged 32 at a time in LDS; each wavefront owns a set of 16 x 16
 // output tiles and reads its MFMA operands transposed from LDS.
 // ---------------------------------------------------------------------------
+// Edge products C[E x H] = X[E x KE] * B with the operand rows gathered inside
+// the kernel: row e of X is [bf(h_src[e]), bf(h_col[e]), bf(f_e), 0 ...]
+// (round to nearest even, zero padding to 160 columns), assembled per lane
+// from the FP32 node features (8 columns = two 16-byte loads) and edge
+// features, so no packed operand matrix is written and read back.  Lane mapping as in k_gemm (chunk c = 4 kk
+// + lk of 8 columns: c < 8 source node, c < 16 target node, c = 16 the four
+// edge features).  PACK also stores the rows (training's weight gradients read
+// them).
+//   EPI 1: Z = C + bias, Y = silu(Z)
+//   EPI 3: Z = C + bias (the aggregation applies silu itself)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t bf2pack(float lo, float hi) {
+  return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+}
+
+template <int EPI, bool PACK>
+__global__ void __launch_bounds__(kThreads) k_gemm_edges(const float* __restrict__ h, const float* __restrict__ fe,
+                                                         const int* __restrict__ src, const int* __restrict__ col,
+                                                         int M, const bf16_t* __restrict__ Bt,
+                                                         const float* __restrict__ bias, float* __restrict__ Z,
+                                                         float* __restrict__ Y, bf16_t* __restrict__ Xout) {
+  constexpr int KI = KE, NO = H;
+  static_assert(KI == 160 && NO == 64, "edge operand layout");
+  __shared__ __attribute__((aligned(16))) bf16_t sB[NO * KI];
+  for (int i = threadIdx.x * 8; i < NO * KI; i += kThreads * 8)
+    *reinterpret_cast<uint4*>(sB + i) = *reinterpret_cast<const uint4*>(Bt + i);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int ntile = (M + 15) / 16;
+  for (int tile = blockIdx.x * 4 + wave; tile < ntile; tile += gridDim.x * 4) {
+    const int r0 = tile * 16;
+    f4 acc[NO / 16];
+#pragma unroll
+    for (int t = 0; t < NO / 16; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    const int ra = r0 + li;
+    const bool live = ra < M;
+    const int is = live ? src[ra] : 0, ic = live ? col[ra] : 0;
+#pragma unroll
+    for (int kk = 0; kk < KI / 32; ++kk) {
+      uint4 w{0u, 0u, 0u, 0u};
+      if (live) {
+        if (kk < 4) {
+          const float* p = h + (size_t)(kk < 2 ? is : ic) * H + (kk & 1) * 32 + lk * 8;
+          const float4 q0 = *reinterpret_cast<const float4*>(p);
+          const float4 q1 = *reinterpret_cast<const float4*>(p + 4);
+          w.x = bf2pack(q0.x, q0.y); w.y = bf2pack(q0.z, q0.w);
+          w.z = bf2pack(q1.x, q1.y); w.w = bf2pack(q1.z, q1.w);
+        } else if (lk == 0) {
+          const float4 q = *reinterpret_cast<const float4*>(fe + 4 * (size_t)ra);
+          w.x = bf2pack(q.x, q.y); w.y = bf2pack(q.z, q.w);
+        }
+        if (PACK) *reinterpret_cast<uint4*>(Xout + (size_t)ra * KI + kk * 32 + lk * 8) = w;
+      }
+      const bf16x8 a = __builtin_bit_cast(bf16x8, w);
+#pragma unroll
+      for (int t = 0; t < NO / 16; ++t) {
+        const bf16x8 b = load8(sB + (16 * t + li) * KI + kk * 32 + lk * 8);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NO / 16; ++t) {
+      const int c = 16 * t + li;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = r0 + 4 * lk + e;
+        if (row >= M) continue;
+        const size_t o = (size_t)row * NO + c;
+        const float z = acc[t][e] + bias[c];
+        Z[o] = z;
+        if (EPI == 1) Y[o] = silu(z);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 template <int KI, int NO>
 __global__ void __launch_bounds__(kThreads) k_gemm_tn(const bf16_t* __restrict__ A, const bf16_t* __restrict__ G,
                                                       int M, float* __restrict__ partial) {
@@ -269,31 +347,6 @@ __global__ void k_geom(EState e, int b, const RepD* __restrict__ reps, const int
   for (int k = 3; k < KN; ++k) o[k] = 0;
 }
 
-// Edge operand rows [bf(h_i), bf(h_j), bf(f_e), 0 ...]: one thread per 8 columns.
-__global__ void k_pack_edges(const float* h, const float* fe, const int* src, const int* col, bf16_t* X, int E) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)E * (KE / 8)) return;
-  const int e = (int)(t / (KE / 8)), c = (int)(t % (KE / 8));
-  bf16_t v[8];
-  if (c < 16) {
-    const float* hs = h + (size_t)(c < 8 ? src[e] : col[e]) * H + (c % 8) * 8;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = f2bf(hs[q]);
-  } else if (c == 16) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = f2bf(fe[4 * e + q]);
-#pragma unroll
-    for (int q = 4; q < 8; ++q) v[q] = 0;
-  } else {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = 0;
-  }
-  uint4 w;
-  w.x = v[0] | ((uint32_t)v[1] << 16); w.y = v[2] | ((uint32_t)v[3] << 16);
-  w.z = v[4] | ((uint32_t)v[5] << 16); w.w = v[6] | ((uint32_t)v[7] << 16);
-  *reinterpret_cast<uint4*>(X + (size_t)e * KE + c * 8) = w;
-}
-
 // Node operand rows [bf(h_i), bf(A_i)].
 __global__ void k_pack_nodes(const float* h, const float* A, bf16_t* X, int n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -302,13 +355,13 @@ __global__ void k_pack_nodes(const float* h, const float* A, bf16_t* X, int n) {
   X[t] = f2bf(c < H ? h[(size_t)i * H + c] : A[(size_t)i * H + c - H]);
 }
 
-// A_i = sum over row i of M_e (row order).
-__global__ void k_aggregate(const float* M, const int* row, float* A, int n) {
+// A_i = kAgg * sum over row i of silu(Z_e) (row order).
+__global__ void k_aggregate(const float* Z, const int* row, float* A, int n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)n * H) return;
   const int i = (int)(t / H), c = (int)(t % H);
   float s = 0.f;
-  for (int e = row[i]; e < row[i + 1]; ++e) s += M[(size_t)e * H + c];
+  for (int e = row[i]; e < row[i + 1]; ++e) s += silu(Z[(size_t)e * H + c]);
   A[t] = kAgg * s;
 }
 
@@ -534,7 +587,7 @@ struct samos_mpnn {
   bf16_t* u = nullptr;
   float *Z0 = nullptr, *h[NLAYER + 1] = {}, *Zn[NLAYER] = {}, *A = nullptr, *delta = nullptr, *target = nullptr;
   bf16_t *Xn[NLAYER] = {}, *Xe[NLAYER + 1] = {};
-  float *Ze[NLAYER + 1] = {}, *Mb = nullptr, *U = nullptr, *w = nullptr;
+  float *Ze[NLAYER + 1] = {}, *U = nullptr, *w = nullptr;
   // backward
   float *gdelta = nullptr, *gw = nullptr, *gX = nullptr, *gh = nullptr, *gXn = nullptr, *gA = nullptr;
   bf16_t *gZ = nullptr, *gZn = nullptr;
@@ -643,7 +696,7 @@ int ensure(samos_mpnn* M, int n, int E, bool train) {
   const long N = std::max<long>(n, M->capN), EE = std::max<long>(E, M->capE) + E / 8 + 64;
   m_free(M, M->row); m_free(M, M->col); m_free(M, M->src); m_free(M, M->rev);
   m_free(M, M->dh); m_free(M, M->fe); m_free(M, M->u); m_free(M, M->Z0); m_free(M, M->A);
-  m_free(M, M->delta); m_free(M, M->target); m_free(M, M->Mb); m_free(M, M->U); m_free(M, M->w);
+  m_free(M, M->delta); m_free(M, M->target); m_free(M, M->U); m_free(M, M->w);
   m_free(M, M->gdelta); m_free(M, M->gw); m_free(M, M->gX); m_free(M, M->gh); m_free(M, M->gXn);
   m_free(M, M->gA); m_free(M, M->gZ); m_free(M, M->gZn); m_free(M, M->loss_part);
   for (int l = 0; l <= NLAYER; ++l) { m_free(M, M->h[l]); m_free(M, M->Xe[l]); m_free(M, M->Ze[l]); }
@@ -659,13 +712,12 @@ int ensure(samos_mpnn* M, int n, int E, bool train) {
   MCHK(m_alloc(M, &M->A, N * H));
   MCHK(m_alloc(M, &M->delta, 2 * N));
   MCHK(m_alloc(M, &M->target, 2 * N));
-  MCHK(m_alloc(M, &M->Mb, EE * H));
   MCHK(m_alloc(M, &M->U, EE * H));
   MCHK(m_alloc(M, &M->w, EE));
   for (int l = 0; l <= NLAYER; ++l) MCHK(m_alloc(M, &M->h[l], N * H));
   const int layers = train ? NLAYER + 1 : 1;
   for (int l = 0; l < layers; ++l) {
-    MCHK(m_alloc(M, &M->Xe[l], EE * KE));
+    if (train) MCHK(m_alloc(M, &M->Xe[l], EE * KE));  // operand rows kept for the weight gradients
     MCHK(m_alloc(M, &M->Ze[l], EE * H));
   }
   for (int l = 0; l < (train ? NLAYER : 1); ++l) {
@@ -721,6 +773,23 @@ void gemm(samos_mpnn* M, hipStream_t st, const bf16_t* A, int rows, const bf16_t
   auto ev = gprof_begin(M, st);
   hipLaunchKernelGGL((k_gemm<KI, NO, EPI>), dim3(grid), dim3(kThreads), 0, st, A, rows, Bt, bias, Z, Y);
   gprof_end(M, st, ev, 2.0 * rows * KI * NO);
+}
+
+// Edge products with operand rows gathered in the kernel (k_gemm_edges);
+// Xout (training) receives the rows.
+template <int EPI>
+void gemm_edges(samos_mpnn* M, hipStream_t st, const float* h, int rows, const bf16_t* Bt, const float* bias,
+                float* Z, float* Y, bf16_t* Xout) {
+  const int tiles = (rows + 15) / 16;
+  const int grid = std::max(1, std::min((tiles + 3) / 4, 4096));
+  auto ev = gprof_begin(M, st);
+  if (Xout)
+    hipLaunchKernelGGL((k_gemm_edges<EPI, true>), dim3(grid), dim3(kThreads), 0, st, h, M->fe, M->src, M->col, rows,
+                       Bt, bias, Z, Y, Xout);
+  else
+    hipLaunchKernelGGL((k_gemm_edges<EPI, false>), dim3(grid), dim3(kThreads), 0, st, h, M->fe, M->src, M->col, rows,
+                       Bt, bias, Z, Y, Xout);
+  gprof_end(M, st, ev, 2.0 * rows * KE * H);
 }
 
 // out[0:len] = sum over the nblk rows of M->partial, in row order (two levels
@@ -795,23 +864,17 @@ int forward(samos_mpnn* M, samos_ens* E, bool train) {
   float* P = M->p;
   gemm<KN, H, 1>(M, st, M->u, n, M->W0t, P + M->b0.off, M->Z0, M->h[0]);
   for (int l = 0; l < NLAYER; ++l) {
-    bf16_t* Xe = M->Xe[train ? l : 0];
     float* Ze = M->Ze[train ? l : 0];
     bf16_t* Xn = M->Xn[train ? l : 0];
     float* Zn = M->Zn[train ? l : 0];
-    hipLaunchKernelGGL(k_pack_edges, dim3(blocks((long)Etot * (KE / 8))), dim3(kThreads), 0, st, M->h[l], M->fe,
-                       M->src, M->col, Xe, Etot);
-    gemm<KE, H, 1>(M, st, Xe, Etot, M->Wet[l], P + M->be[l].off, Ze, M->Mb);
-    hipLaunchKernelGGL(k_aggregate, dim3(blocks((long)n * H)), dim3(kThreads), 0, st, M->Mb, M->row, M->A, n);
+    gemm_edges<3>(M, st, M->h[l], Etot, M->Wet[l], P + M->be[l].off, Ze, nullptr, train ? M->Xe[l] : nullptr);
+    hipLaunchKernelGGL(k_aggregate, dim3(blocks((long)n * H)), dim3(kThreads), 0, st, Ze, M->row, M->A, n);
     hipLaunchKernelGGL(k_pack_nodes, dim3(blocks((long)n * KU)), dim3(kThreads), 0, st, M->h[l], M->A, Xn, n);
     MCHK(hipMemcpyAsync(M->h[l + 1], M->h[l], sizeof(float) * (size_t)n * H, hipMemcpyDeviceToDevice, st));
     gemm<KU, H, 2>(M, st, Xn, n, M->Wnt[l], P + M->bn[l].off, Zn, M->h[l + 1]);
   }
-  bf16_t* Xo = M->Xe[train ? NLAYER : 0];
   float* Zo = M->Ze[train ? NLAYER : 0];
-  hipLaunchKernelGGL(k_pack_edges, dim3(blocks((long)Etot * (KE / 8))), dim3(kThreads), 0, st, M->h[NLAYER], M->fe,
-                     M->src, M->col, Xo, Etot);
-  gemm<KE, H, 1>(M, st, Xo, Etot, M->Wot, P + M->bo.off, Zo, M->U);
+  gemm_edges<1>(M, st, M->h[NLAYER], Etot, M->Wot, P + M->bo.off, Zo, M->U, train ? M->Xe[NLAYER] : nullptr);
   hipLaunchKernelGGL(k_head_out, dim3(blocks(Etot)), dim3(kThreads), 0, st, M->U, P + M->vv.off, P + M->cc.off,
                      M->w, Etot);
   hipLaunchKernelGGL(k_delta, dim3(blocks(n)), dim3(kThreads), 0, st, M->w, M->dh, M->row, M->delta, n);
