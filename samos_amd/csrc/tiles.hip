@@ -505,39 +505,42 @@ __device__ __forceinline__ PairK pair_k(const PairConst& pc, const BoxD& b) {
   return k;
 }
 
-// r_sq = dx * dx + dy * dy (dz = 0: adding +0 to a sum of squares changes no bit)
+// r_sq = dx * dx + dy * dy (dz = 0: adding +0 to a sum of squares changes no
+// bit).  The arithmetic of pair_force_loop (pair_sum.hpp) for one type: each
+// product accumulated with one rounding (fused multiply-add); FP32 factors
+// from the hardware square root and reciprocal, accumulated in FP64.
 template <int POT, bool F32>
 __device__ __forceinline__ void pair_term(const PairK& pk, double dx, double dy, double r_sq, double& fx,
                                           double& fy) {
   if (POT & POT_SOFT) {
     if (F32) {
-      const float r = sqrtf((float)r_sq);
+      const float r = __builtin_amdgcn_sqrtf((float)r_sq);
       if (r < (float)pk.soft_a) {
         const float diff = (float)pk.soft_a - r;
-        const float ff = (r > 0.0f) ? (float)pk.soft_k * diff / r : (float)pk.soft_k * diff;
-        fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy);
+        const float ff = (r > 0.0f) ? (float)pk.soft_k * diff * __builtin_amdgcn_rcpf(r) : (float)pk.soft_k * diff;
+        fx = fma(-(double)ff, dx, fx); fy = fma(-(double)ff, dy, fy);
       }
     } else {
       const double r = sqrt(r_sq);
       if (r < pk.soft_a) {
         const double diff = pk.soft_a - r;
         const double ff = (r > 0.0) ? pk.soft_k * diff / r : pk.soft_k * diff;
-        fx -= ff * dx; fy -= ff * dy;
+        fx = fma(-ff, dx, fx); fy = fma(-ff, dy, fy);
       }
     }
   }
   if (POT & POT_LJ) {
     if (r_sq <= pk.lj_rc2) {
       if (F32) {
-        const float s2 = (float)pk.lj_sig2 / (float)r_sq;
+        const float s2 = (float)pk.lj_sig2 * __builtin_amdgcn_rcpf((float)r_sq);
         const float s6 = s2 * s2 * s2;
         const float ff = (float)pk.lj_e48 * s6 * (s6 - 0.5f) * s2;
-        fx -= (double)(ff * (float)dx); fy -= (double)(ff * (float)dy);
+        fx = fma(-(double)ff, dx, fx); fy = fma(-(double)ff, dy, fy);
       } else {
         const double inv_r_sq = lj_inv_r_sq(pk.lj_sig2, r_sq);
         const double inv_r_6 = inv_r_sq * inv_r_sq * inv_r_sq;
         const double ff = pk.lj_e48 * inv_r_6 * (inv_r_6 - 0.5) * inv_r_sq;
-        fx -= ff * dx; fy -= ff * dy;
+        fx = fma(-ff, dx, fx); fy = fma(-ff, dy, fy);
       }
     }
   }
