@@ -433,9 +433,10 @@ int alloc_nbr(samos_ctx* ctx, int stride) {
   hipFree(nl.nbr);
   nl.nbr = nullptr;
   nl.stride = stride;
-  // 8 rows beyond the capacity: the builds pad each list to a whole number
-  // of 8-entry batches
-  HIPCHK(hipMalloc(&nl.nbr, sizeof(int) * (size_t)(stride + 8) * (size_t)ctx->np + 64));
+  // 8 entries beyond the capacity, in groups of four per slot: the builds pad
+  // each list to a whole number of 8-entry batches
+  const size_t groups = (size_t)(stride + 8 + 3) / 4;
+  HIPCHK(hipMalloc(&nl.nbr, sizeof(int) * 4 * groups * (size_t)ctx->np + 64));
   return 0;
 }
 

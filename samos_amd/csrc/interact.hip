@@ -31,9 +31,8 @@ __global__ void __launch_bounds__(kBlock) k_interact(int mode, Fields f, NListD 
   double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
   double e_soft = 0.0, e_lj = 0.0, e_pol = 0.0, e_nem = 0.0;
   const int kn = nl.nnb[s];
-  const int* nb = nl.nbr + s;
   for (int k = 0; k < kn; ++k) {
-    const int t = nb[(long)k * nl.np];
+    const int t = nl.nbr[nbr_at(nl, k, s)];
     // d = x_j - x_i (the soft and aligner convention); LJ uses -d.
     double dx = f.x[t] - xi, dy = f.y[t] - yi, dz = f.z[t] - zi;
     min_image(b, dx, dy, dz);
@@ -164,9 +163,8 @@ __global__ void __launch_bounds__(kBlock) k_interact_extra(Fields f, NListD nl, 
   double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
   int coord = 0;
   const int kn = nl.nnb[s];
-  const int* nb = nl.nbr + s;
   for (int k = 0; k < kn; ++k) {
-    const int t = nb[(long)k * nl.np];
+    const int t = nl.nbr[nbr_at(nl, k, s)];
     double dx = f.x[t] - xi, dy = f.y[t] - yi, dz = f.z[t] - zi;   // x_j - x_i
     min_image(b, dx, dy, dz);
     const double r_sq = dx * dx + dy * dy + dz * dz;

@@ -242,6 +242,35 @@ __device__ void sort_phase(NListD& nl) {
   if ((threadIdx.x & 63) == 0 && local_max > 0) atomic_max_if_above(&nl.flags[3], local_max);
 }
 
+// A slot's list is written four entries at a time: the group being filled
+// is a shift register (the newest entry enters at w) stored whole, 16 bytes,
+// when its fourth entry arrives.
+__device__ __forceinline__ void group_push(uint4& qb, unsigned t) {
+  qb.x = qb.y; qb.y = qb.z; qb.z = qb.w; qb.w = t;
+}
+__device__ __forceinline__ void group_store(const NListD& nl, int g, int s, const uint4& qb) {
+  *reinterpret_cast<uint4*>(nl.nbr + nbr_at(nl, 4 * g, s)) = qb;
+}
+// Entry t of slot s at position k of its list (nothing past the capacity).
+__device__ __forceinline__ void list_push(const NListD& nl, int s, int k, unsigned t, uint4& qb) {
+  if (k < nl.stride) {
+    group_push(qb, t);
+    if ((k & 3) == 3) group_store(nl, k >> 2, s, qb);
+  }
+}
+// After a slot's last entry (k of them found): the last group completed with
+// the own index, and one more group of it when the list would end inside an
+// 8-entry batch (the pair loops load whole batches and skip the padding).
+__device__ __forceinline__ void list_finish(const NListD& nl, int s, int k, uint4& qb) {
+  const int kk = min(k, nl.stride);
+  if (kk & 3) {
+    for (int q = kk & 3; q < 4; ++q) group_push(qb, (unsigned)s);
+    group_store(nl, kk >> 2, s, qb);
+  }
+  const int groups = (kk + 3) >> 2;
+  if (groups & 1) group_store(nl, groups, s, uint4{(unsigned)s, (unsigned)s, (unsigned)s, (unsigned)s});
+}
+
 // Flags: [2] overflow, [4] builds, [5] largest neighbour count.  One atomic per
 // wavefront.
 // ovf2 (optional): a second overflow flag (the captured chunk's).
@@ -264,7 +293,7 @@ __device__ __forceinline__ void build_epilogue(NListD& nl, bool overflow, int km
 template <bool PLANAR>
 __device__ __forceinline__ void build_candidate(const NListD& nl, const BoxD& b, int s, double xi, double yi,
                                                 double zi, double ri, bool wrap, int t, double xj, double yj,
-                                                double zj, double rj, int& k, int& coord) {
+                                                double zj, double rj, int& k, int& coord, uint4& qb) {
   if (t == s) return;
   double dx = xi - xj, dy = yi - yj, dz = PLANAR ? 0.0 : zi - zj;
   if (wrap) {
@@ -274,7 +303,7 @@ __device__ __forceinline__ void build_candidate(const NListD& nl, const BoxD& b,
   // planar: dz = 0, and adding +0 to a sum of squares changes no bit
   const double d2 = PLANAR ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
   if (d2 < nl.cut2) {
-    if (k < nl.stride) nl.nbr[(long)k * nl.np + s] = t;
+    list_push(nl, s, k, (unsigned)t, qb);
     ++k;
     const double r = ri + rj;
     if (d2 < r * r) ++coord;
@@ -287,7 +316,7 @@ __device__ __forceinline__ void build_candidate(const NListD& nl, const BoxD& b,
 template <bool PLANAR>
 __device__ __forceinline__ void build_cells_global(const Fields& f, const NListD& nl, const BoxD& b, int s, int c,
                                                    double xi, double yi, double zi, double ri, bool runi, bool wrap,
-                                                   int& k, int& coord) {
+                                                   int& k, int& coord, uint4& qb) {
   constexpr int U = 8;
   const int ci = c / (nl.ncy * nl.ncz), cj = (c / nl.ncz) % nl.ncy, ck = c % nl.ncz;
   // Planar: every particle is in the same z-layer ck, and a used cell grid
@@ -319,7 +348,8 @@ __device__ __forceinline__ void build_cells_global(const Fields& f, const NListD
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             if (m0 + u >= e) break;
-            build_candidate<PLANAR>(nl, b, s, xi, yi, zi, ri, wrap, t[u], xj[u], yj[u], zj[u], rj[u], k, coord);
+            build_candidate<PLANAR>(nl, b, s, xi, yi, zi, ri, wrap, t[u], xj[u], yj[u], zj[u], rj[u], k, coord,
+                                    qb);
           }
         }
       }
@@ -409,7 +439,7 @@ __device__ bool stage_wave_block(const Fields& f, const NListD& nl, WaveStage& S
 // The same stencil walk as build_cells_global, from the wavefront's stage.
 __device__ __forceinline__ void build_cells_staged(const NListD& nl, const BoxD& b, const WaveStage& S, int bx0,
                                                    int by0, int bw, int s, int c, double xi, double yi, double ri,
-                                                   double rj_u, bool wrap, int& k, int& coord) {
+                                                   double rj_u, bool wrap, int& k, int& coord, uint4& qb) {
   const int ci = c / (nl.ncy * nl.ncz), cj = (c / nl.ncz) % nl.ncy;
   for (int ix = -1; ix <= 1; ix++) {
     for (int iy = -1; iy <= 1; iy++) {
@@ -417,7 +447,7 @@ __device__ __forceinline__ void build_cells_staged(const NListD& nl, const BoxD&
       const int a = S.cstart[q], e = S.cstart[q + 1];
       for (int m = a; m < e; ++m) {
         const double2 p = S.pos[m];
-        build_candidate<true>(nl, b, s, xi, yi, 0.0, ri, wrap, S.mem[m], p.x, p.y, 0.0, rj_u, k, coord);
+        build_candidate<true>(nl, b, s, xi, yi, 0.0, ri, wrap, S.mem[m], p.x, p.y, 0.0, rj_u, k, coord, qb);
       }
     }
   }
@@ -447,6 +477,7 @@ __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n, int* ov
     // per-candidate radius gather
     const double ri = runi ? nl.r_uni : f.radius[s];
     int k = 0, coord = 0;
+    uint4 qb{0u, 0u, 0u, 0u};
     // Candidates come from the 3 x 3 (x 3) cells around the particle's own,
     // all within two cell widths of it per axis.  Farther than that from every
     // periodic face, none of them is across a face, the raw difference is the
@@ -460,15 +491,13 @@ __device__ void build_phase(Fields& f, NListD& nl, const BoxD& b, int n, int* ov
     }
     if (PLANAR && staged)
       build_cells_staged(nl, b, stage[PLANAR ? threadIdx.x >> 6 : 0], bx0, by0, bw, s, c, xi, yi, ri, nl.r_uni, wrap,
-                         k, coord);
+                         k, coord, qb);
     else
-      build_cells_global<PLANAR>(f, nl, b, s, c, xi, yi, zi, ri, runi, wrap, k, coord);
+      build_cells_global<PLANAR>(f, nl, b, s, c, xi, yi, zi, ri, runi, wrap, k, coord, qb);
     if (k > nl.stride) overflow = true;
     kmax = max(kmax, k);
     nl.nnb[s] = min(k, nl.stride);
-    // padding to a multiple of 8 entries with the own index (the list has 8
-    // rows beyond its capacity), read by the batched pair loops, never summed
-    for (int q = min(k, nl.stride); q & 7; ++q) nl.nbr[(long)q * nl.np + s] = s;
+    list_finish(nl, s, k, qb);
     f.coord[s] = coord;
     nl.ox[s] = xi; nl.oy[s] = yi; nl.oz[s] = zi;
   }
@@ -586,13 +615,14 @@ __global__ void k_build_nsq(Fields f, NListD nl, BoxD b, int n, const int* req) 
   for (int s = gtid(); s < n; s += gsize()) {
     const double xi = f.x[s], yi = f.y[s], zi = f.z[s], ri = f.radius[s];
     int k = 0, coord = 0;
+    uint4 qb{0u, 0u, 0u, 0u};
     for (int t = 0; t < n; ++t) {
       if (t == s) continue;
       double dx = xi - f.x[t], dy = yi - f.y[t], dz = zi - f.z[t];
       min_image(b, dx, dy, dz);
       const double d2 = dx * dx + dy * dy + dz * dz;
       if (d2 < nl.cut2) {
-        if (k < nl.stride) nl.nbr[(long)k * nl.np + s] = t;
+        list_push(nl, s, k, (unsigned)t, qb);
         ++k;
         const double r = ri + f.radius[t];
         if (d2 < r * r) ++coord;
@@ -601,9 +631,7 @@ __global__ void k_build_nsq(Fields f, NListD nl, BoxD b, int n, const int* req) 
     if (k > nl.stride) overflow = true;
     kmax = max(kmax, k);
     nl.nnb[s] = min(k, nl.stride);
-    // padding to a multiple of 8 entries with the own index (the list has 8
-    // rows beyond its capacity), read by the batched pair loops, never summed
-    for (int q = min(k, nl.stride); q & 7; ++q) nl.nbr[(long)q * nl.np + s] = s;
+    list_finish(nl, s, k, qb);
     f.coord[s] = coord;
     nl.ox[s] = xi; nl.oy[s] = yi; nl.oz[s] = zi;
   }
@@ -684,7 +712,7 @@ __global__ void k_export_pairs(Fields f, NListD nl, int n, int* out, unsigned lo
     const int is = f.id[s];
     const int kn = nl.nnb[s];
     for (int k = 0; k < kn; ++k) {
-      const int t = nl.nbr[(long)k * nl.np + s];
+      const int t = nl.nbr[nbr_at(nl, k, s)];
       const int it = f.id[t];
       if (it > is) {
         unsigned long long p = atomicAdd(counter, 1ULL);

@@ -74,12 +74,17 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
   // are loaded while the current batch is gathered and summed.  Gather offsets are 32-bit byte offsets from the column bases (np < 2^28,
   // which the host checks before choosing these kernels), so each gather
   // takes its address from a scalar base and one vector register.
+  static_assert(kBatch % 4 == 0, "batches of whole four-entry groups");
   const int step = lanes * kBatch;
-  const unsigned s4 = (unsigned)s << 2;
+  const unsigned s16 = (unsigned)s << 4;
+  const uint4* G = reinterpret_cast<const uint4*>(nl.nbr);
   unsigned t[kBatch];  // neighbour slots
   if (lane * kBatch < kn) {
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) t[u] = (unsigned)at32(nl.nbr + (long)(lane * kBatch + u) * nl.np, s4);
+    for (int g = 0; g < kBatch / 4; ++g) {
+      const uint4 q = at32(G + (long)((lane * kBatch >> 2) + g) * nl.np, s16);
+      t[4 * g] = q.x; t[4 * g + 1] = q.y; t[4 * g + 2] = q.z; t[4 * g + 3] = q.w;
+    }
   }
   for (int k0 = lane * kBatch; k0 < kn; k0 += step) {
     double xj[kBatch], yj[kBatch], zj[kBatch], rj[kBatch];
@@ -102,7 +107,10 @@ __device__ __forceinline__ void pair_force_loop(const Fields& f, const NListD& n
     }
     if (k0 + step < kn) {
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) t[u] = (unsigned)at32(nl.nbr + (long)(k0 + step + u) * nl.np, s4);
+      for (int g = 0; g < kBatch / 4; ++g) {
+        const uint4 q = at32(G + (long)(((k0 + step) >> 2) + g) * nl.np, s16);
+        t[4 * g] = q.x; t[4 * g + 1] = q.y; t[4 * g + 2] = q.z; t[4 * g + 3] = q.w;
+      }
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {

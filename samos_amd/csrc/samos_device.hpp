@@ -89,8 +89,11 @@ __device__ __forceinline__ bool in_group(const Fields& f, int s, unsigned gbit) 
 }
 
 // Neighbour-list state.  The list is a full (both directions) Verlet list in a
-// transposed layout: neighbour k of slot s is nbr[k * np + s], so that the
-// k-th loads of consecutive slots are coalesced.
+// transposed layout of groups of four: entries 4g .. 4g + 3 of slot s are the
+// 16 bytes at nbr[(g * np + s) * 4] (nbr_at), so one 16-byte load brings a
+// slot four entries, the g-th loads of consecutive slots are contiguous, and
+// a build stores each group whole.  (A replica ensemble's list, built and read
+// in ensemble.hip, keeps one entry per row: entry k of slot s at k * np + s.)
 struct NListD {
   int ncx, ncy, ncz, ncell;
   double wx, wy, wz;
@@ -104,13 +107,18 @@ struct NListD {
   int* start;         // [ncell + 1] exclusive scan of count
   int* members;       // [n] slots grouped by cell, ascending slot within a cell
   int* bsum;          // [scan blocks] per-block sums for the scan
-  int* nbr;           // [(stride + 8) * np]: entry k of slot s at k * np + s
+  int* nbr;           // [ceil((stride + 8) / 4) * np * 4]: entry k of slot s at nbr_at(k, s)
   int* nnb;           // [np]
   double *ox, *oy, *oz;  // positions at the last build
   int* flags;         // [0],[1] rebuild request by step parity, [2] overflow, [3] largest cell
                       // count, [4] builds, [5] largest neighbour count
   double r_uni = -1.0;  // the common particle radius, or -1: the build then reads each radius
 };
+
+// Index of entry k of slot s in NListD::nbr.
+__device__ __forceinline__ long nbr_at(const NListD& nl, int k, int s) {
+  return (((long)(k >> 2) * nl.np + s) << 2) + (k & 3);
+}
 
 // Pair interactions evaluated in one pass over the neighbour list.
 struct InteractD {
