@@ -324,6 +324,7 @@ __global__ void __launch_bounds__(kBlock) k_ens_build(EState e, int b, const Rep
       }
     }
     int k = 0;
+    uint4 qb{0u, 0u, 0u, 0u};
     for (int u = 0; u < nbk; ++u) {
       const int a0 = nl.start[bk[u]], a1 = nl.start[bk[u] + 1];
       for (int m = a0; m < a1; ++m) {
@@ -337,11 +338,12 @@ __global__ void __launch_bounds__(kBlock) k_ens_build(EState e, int b, const Rep
         double dx = pi.x - pt.x, dy = pi.y - pt.y, dz = pi.z - pt.z;
         min_image(R.box, dx, dy, dz);
         if (dx * dx + dy * dy + dz * dz < R.cut2) {
-          if (k < nl.stride) nl.nbr[(long)k * nl.np + s] = t;
+          list_push(nl, s, k, (unsigned)t, qb);
           ++k;
         }
       }
     }
+    list_close(nl, s, k, qb);
     overflow |= k > nl.stride;
     kmax = max(kmax, k);
     nl.nnb[s] = min(k, nl.stride);
@@ -392,14 +394,21 @@ template <int BATCH, bool AL, bool EN>
 __device__ __forceinline__ void ens_pair_sum(const EState& e, int b, const NListD& nl, const RepD& R, int s,
                                              const PS& p, double ri, double& fx, double& fy, double& fz,
                                              double& tx, double& ty, double& tz, double& en) {
+  static_assert(BATCH % 4 == 0, "batches of whole four-entry groups");
   const int kn = nl.nnb[s];
-  const int* nb = nl.nbr + s;
+  const uint4* G = reinterpret_cast<const uint4*>(nl.nbr);
   const D4* P = e.p[b];
   const D4* Dn = e.d[b];
   for (int k0 = 0; k0 < kn; k0 += BATCH) {
     int t[BATCH];
 #pragma unroll
-    for (int u = 0; u < BATCH; ++u) t[u] = (k0 + u < kn) ? nb[(long)(k0 + u) * nl.np] : s;
+    for (int g = 0; g < BATCH / 4; ++g) {
+      const uint4 q = G[(long)((k0 >> 2) + g) * nl.np + s];
+      t[4 * g] = (int)q.x; t[4 * g + 1] = (int)q.y; t[4 * g + 2] = (int)q.z; t[4 * g + 3] = (int)q.w;
+    }
+#pragma unroll
+    for (int u = 0; u < BATCH; ++u)
+      if (k0 + u >= kn) t[u] = s;
     D4 pj[BATCH];
 #pragma unroll
     for (int u = 0; u < BATCH; ++u) pj[u] = P[t[u]];
@@ -754,6 +763,10 @@ thread_local std::string g_ens_err;
     }                                                                             \
   } while (0)
 
+// Ints of a list of the given capacity: groups of four entries per slot, one
+// spare group (a batch of eight may load the group after the last written).
+size_t ens_nbr_ints(int stride, int np) { return ((size_t)(stride + 3) / 4 + 1) * 4 * (size_t)np; }
+
 template <class T>
 hipError_t ens_alloc(samos_ens* E, T** p, size_t count) {
   void* q = nullptr;
@@ -847,7 +860,7 @@ int samos::ens_build_all(samos_ens* E) {
     ens_free(E, E->nl.nbr);
     E->nl.nbr = nullptr;
     E->nl.stride = (int)std::ceil(h[5] * 1.5) + 16;
-    ECHK(ens_alloc(E, &E->nl.nbr, (size_t)E->nl.stride * E->np));
+    ECHK(ens_alloc(E, &E->nl.nbr, ens_nbr_ints(E->nl.stride, E->np)));
   }
   E->err = "neighbour list capacity could not be established";
   return -1;
@@ -1179,7 +1192,7 @@ int samos_ens_nlist(samos_ens* E, double rcut, double pad) {
   ECHK(hipMemsetAsync(nl.count, 0, sizeof(int) * nb_total, E->st));
   ECHK(hipMemsetAsync(nl.nnb, 0, sizeof(int) * np, E->st));
   nl.stride = 32;
-  ECHK(ens_alloc(E, &nl.nbr, (size_t)nl.stride * np));
+  ECHK(ens_alloc(E, &nl.nbr, ens_nbr_ints(nl.stride, np)));
   if (int rc = ens_upload(E)) return rc;
   // Morton re-sort inside each replica: key = (replica, Morton code of the cell)
   {
@@ -1239,15 +1252,17 @@ int samos_ens_pairs(samos_ens* E, int rep, int* out, long long cap, long long* c
   const RepD& R = E->h_rep[rep];
   std::vector<int> nnb(R.n);
   ECHK(hipMemcpy(nnb.data(), E->nl.nnb + R.n0, sizeof(int) * R.n, hipMemcpyDeviceToHost));
-  std::vector<int> nbr((size_t)E->nl.stride * R.n);
-  for (int k = 0; k < E->nl.stride; ++k)
-    ECHK(hipMemcpy(nbr.data() + (size_t)k * R.n, E->nl.nbr + (size_t)k * E->np + R.n0, sizeof(int) * R.n,
-                   hipMemcpyDeviceToHost));
+  // the replica's rows of each four-entry group
+  const int groups = (E->nl.stride + 3) / 4;
+  std::vector<int> nbr((size_t)groups * 4 * R.n);
+  for (int g = 0; g < groups; ++g)
+    ECHK(hipMemcpy(nbr.data() + (size_t)g * 4 * R.n, E->nl.nbr + ((size_t)g * E->np + R.n0) * 4,
+                   sizeof(int) * 4 * R.n, hipMemcpyDeviceToHost));
   std::vector<std::pair<int, int>> pr;
   for (int i = 0; i < R.n; ++i) {
     const int gi = E->h_gid[R.n0 + i] - R.n0;
     for (int k = 0; k < nnb[i]; ++k) {
-      const int t = nbr[(size_t)k * R.n + i];
+      const int t = nbr[((size_t)(k >> 2) * R.n + i) * 4 + (k & 3)];
       const int gt = E->h_gid[t] - R.n0;
       if (gt > gi) pr.push_back({gi, gt});
     }

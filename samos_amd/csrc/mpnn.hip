@@ -299,7 +299,7 @@ __global__ void k_csr_fill(NListD nl, const int* row, int* col, int* src, int n)
   if (s >= n) return;
   const int a = row[s], k = nl.nnb[s];
   for (int q = 0; q < k; ++q) {
-    col[a + q] = nl.nbr[(long)q * nl.np + s];
+    col[a + q] = nl.nbr[nbr_at(nl, q, s)];
     src[a + q] = s;
   }
 }

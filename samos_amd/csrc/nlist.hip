@@ -242,32 +242,12 @@ __device__ void sort_phase(NListD& nl) {
   if ((threadIdx.x & 63) == 0 && local_max > 0) atomic_max_if_above(&nl.flags[3], local_max);
 }
 
-// A slot's list is written four entries at a time: the group being filled
-// is a shift register (the newest entry enters at w) stored whole, 16 bytes,
-// when its fourth entry arrives.
-__device__ __forceinline__ void group_push(uint4& qb, unsigned t) {
-  qb.x = qb.y; qb.y = qb.z; qb.z = qb.w; qb.w = t;
-}
-__device__ __forceinline__ void group_store(const NListD& nl, int g, int s, const uint4& qb) {
-  *reinterpret_cast<uint4*>(nl.nbr + nbr_at(nl, 4 * g, s)) = qb;
-}
-// Entry t of slot s at position k of its list (nothing past the capacity).
-__device__ __forceinline__ void list_push(const NListD& nl, int s, int k, unsigned t, uint4& qb) {
-  if (k < nl.stride) {
-    group_push(qb, t);
-    if ((k & 3) == 3) group_store(nl, k >> 2, s, qb);
-  }
-}
 // After a slot's last entry (k of them found): the last group completed with
 // the own index, and one more group of it when the list would end inside an
 // 8-entry batch (the pair loops load whole batches and skip the padding).
 __device__ __forceinline__ void list_finish(const NListD& nl, int s, int k, uint4& qb) {
-  const int kk = min(k, nl.stride);
-  if (kk & 3) {
-    for (int q = kk & 3; q < 4; ++q) group_push(qb, (unsigned)s);
-    group_store(nl, kk >> 2, s, qb);
-  }
-  const int groups = (kk + 3) >> 2;
+  list_close(nl, s, k, qb);
+  const int groups = (min(k, nl.stride) + 3) >> 2;
   if (groups & 1) group_store(nl, groups, s, uint4{(unsigned)s, (unsigned)s, (unsigned)s, (unsigned)s});
 }
 

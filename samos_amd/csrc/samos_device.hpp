@@ -92,8 +92,7 @@ __device__ __forceinline__ bool in_group(const Fields& f, int s, unsigned gbit) 
 // transposed layout of groups of four: entries 4g .. 4g + 3 of slot s are the
 // 16 bytes at nbr[(g * np + s) * 4] (nbr_at), so one 16-byte load brings a
 // slot four entries, the g-th loads of consecutive slots are contiguous, and
-// a build stores each group whole.  (A replica ensemble's list, built and read
-// in ensemble.hip, keeps one entry per row: entry k of slot s at k * np + s.)
+// a build stores each group whole (single systems and replica ensembles).
 struct NListD {
   int ncx, ncy, ncz, ncell;
   double wx, wy, wz;
@@ -118,6 +117,32 @@ struct NListD {
 // Index of entry k of slot s in NListD::nbr.
 __device__ __forceinline__ long nbr_at(const NListD& nl, int k, int s) {
   return (((long)(k >> 2) * nl.np + s) << 2) + (k & 3);
+}
+
+// A list being built is written four entries at a time: the group being
+// filled is a shift register (the newest entry enters at w), stored whole,
+// 16 bytes, when its fourth entry arrives.
+__device__ __forceinline__ void group_push(uint4& qb, unsigned t) {
+  qb.x = qb.y; qb.y = qb.z; qb.z = qb.w; qb.w = t;
+}
+__device__ __forceinline__ void group_store(const NListD& nl, int g, int s, const uint4& qb) {
+  *reinterpret_cast<uint4*>(nl.nbr + nbr_at(nl, 4 * g, s)) = qb;
+}
+// Entry t of slot s at position k of its list (nothing past the capacity).
+__device__ __forceinline__ void list_push(const NListD& nl, int s, int k, unsigned t, uint4& qb) {
+  if (k < nl.stride) {
+    group_push(qb, t);
+    if ((k & 3) == 3) group_store(nl, k >> 2, s, qb);
+  }
+}
+// The last, partly filled group of a list of k entries, completed with the
+// own index.
+__device__ __forceinline__ void list_close(const NListD& nl, int s, int k, uint4& qb) {
+  const int kk = min(k, nl.stride);
+  if (kk & 3) {
+    for (int q = kk & 3; q < 4; ++q) group_push(qb, (unsigned)s);
+    group_store(nl, kk >> 2, s, qb);
+  }
 }
 
 // Pair interactions evaluated in one pass over the neighbour list.
