@@ -30,23 +30,28 @@ template <int POT, int AL, bool F32, bool PLANAR, bool MT>
 __global__ void __launch_bounds__(kBlock) k_brownian_fused(Fields cur, Fields nxt, NListD nl, BoxD b, ConstraintD c,
                                                            InteractD ip, int n, BrownArgs ba) {
   ChunkCtl* const ck = ba.chunk;
+  int head_enforce = ba.head_enforce;
   if (ck) {
     const int j = ba.slot;
-    if (nl.flags[2]) return;  // the list overflowed at a rebuild: no step may read it
-    if (nl.flags[4] >= ck->resort_at) {  // a storage re-sort is due before this slot
-      if (blockIdx.x == 0 && threadIdx.x == 0) ck->stop |= 2;
-      return;
+    const SlotGate g = slot_gate(ck, nl.flags, j);
+    if (g.stop) {
+      // the first test that holds decides, in this order
+      if (g.stop & kGateOverflow) return;  // the list overflowed at a rebuild: no step may read it
+      if (g.stop & kGateResort) {  // a storage re-sort is due before this slot
+        if (blockIdx.x == 0 && threadIdx.x == 0) ck->stop |= 2;
+        return;
+      }
+      if (g.stop & kGateHalt) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ck->halt = 1;
+        return;
+      }
+      return;  // past the range
     }
-    if (ck->halt | ck->req[(j + 1) & 1]) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) ck->halt = 1;
-      return;
-    }
-    if (ck->base + j >= ck->limit) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) ck->last = j + 1;
+    head_enforce = g.head;
   } else if (blockIdx.x == 0 && threadIdx.x == 0) {
     nl.flags[1 - ba.parity] = 0;
   }
-  const int head_enforce = ck ? (ck->base + ba.slot + 1 <= ck->steps ? 1 : 0) : ba.head_enforce;
   // ba.lanes (a power of two dividing 64) consecutive threads share one slot:
   // each sums every lanes-th batch of the list, the partial sums are added
   // in a fixed butterfly order, and lane 0 integrates and writes.

@@ -405,18 +405,22 @@ __global__ void __launch_bounds__(kBlock, (PLANAR && !MT) ? 7 : 1) k_baoab_fused
     // the whole grid either returns here or runs the step.  A list that
     // overflowed at a rebuild stops every later slot before any step reads it.
     const int j = fa.slot;
-    if (nl.flags[2]) return;
-    if (nl.flags[4] >= ck->resort_at) {  // a storage re-sort is due before this slot
-      if (blockIdx.x == 0 && threadIdx.x == 0) ck->stop |= 2;
-      return;
+    const SlotGate g = slot_gate(ck, nl.flags, j);
+    if (g.stop) {
+      // the first test that holds decides, in this order
+      if (g.stop & kGateOverflow) return;  // the list overflowed at a rebuild: no step may read it
+      if (g.stop & kGateResort) {  // a storage re-sort is due before this slot
+        if (blockIdx.x == 0 && threadIdx.x == 0) ck->stop |= 2;
+        return;
+      }
+      if (g.stop & kGateHalt) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ck->halt = 1;
+        return;
+      }
+      return;  // past the range
     }
-    if (ck->halt | ck->req[(j + 1) & 1]) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) ck->halt = 1;
-      return;
-    }
-    if (ck->base + j >= ck->limit) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) ck->last = j + 1;
-    head_enforce = ck->base + j + 1 <= ck->steps ? 1 : 0;
+    head_enforce = g.head;
   } else if (blockIdx.x == 0 && threadIdx.x == 0 && PHASE != 2) {
     nl.flags[1 - parity] = 0;
   }

@@ -217,6 +217,29 @@ struct ChunkCtl {
                  // head's enforce while base + j + 1 <= steps
 };
 
+// The tests a captured slot j makes before its step, from words all loaded
+// up front and combined before the first branch (one memory latency instead
+// of one per test), so a slot that returns at once costs little more than
+// its launch.  Bits: kGateOverflow the list overflowed at a rebuild,
+// kGateResort a storage re-sort is due, kGateHalt an earlier slot asked for a
+// rebuild, kGateEnd the slot lies past the range; head: the slot applies the
+// next loop head's enforce.
+constexpr int kGateOverflow = 1, kGateResort = 2, kGateHalt = 4, kGateEnd = 8;
+struct SlotGate {
+  int stop;  // kGate* bits
+  int head;
+};
+__device__ __forceinline__ SlotGate slot_gate(const ChunkCtl* ck, const int* flags, int j) {
+  const int overflow = flags[2], builds = flags[4];
+  const int halt = ck->halt, req = ck->req[(j + 1) & 1];
+  const int base = ck->base, resort_at = ck->resort_at, limit = ck->limit, steps = ck->steps;
+  SlotGate g;
+  g.stop = (overflow ? kGateOverflow : 0) | (builds >= resort_at ? kGateResort : 0) | ((halt | req) ? kGateHalt : 0) |
+           (base + j >= limit ? kGateEnd : 0);
+  g.head = base + j + 1 <= steps ? 1 : 0;
+  return g;
+}
+
 struct ClockD {
   unsigned long long calls[8];   // per-integrator call counters
   int run_step;                  // run-loop index t
