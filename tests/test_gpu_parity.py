@@ -428,6 +428,47 @@ def test_chunked_fused_steps_equal_direct_launches(gpu, slots):
     assert ba == bc and ba > 10
 
 
+def test_fused_langevin_gather_batch_8_equals_4(gpu):
+    # the planar single-type step gathers its neighbours in batches of 4 or 8
+    # list entries (two 16-byte list groups); the entries and the order of
+    # the sums are the same, so the trajectories agree bit for bit
+    st = sc.lj_lattice(40, rho=0.7, seed=36)
+    out = []
+    for batch in (4, 8):
+        d = sc.load(gpu.System, st)
+        d.nlist(2.5, 0.3)
+        d.pair_lj(1.0, 1.0, 2.5, shifted=True)
+        d.constraint("plane")
+        d.set_timestep(0.004)
+        d.integrator_langevin(gamma=1.0, seed=12, temp=(0, 1.0, 1.0, 1))
+        d.set_option("gather_batch", batch)
+        d.run(299)
+        out.append(({k: d.get(k) for k in ("x", "y", "vx", "vy", "fx", "fy")}, d.pairs()))
+    (a, pa), (c, pc) = out
+    assert all(np.array_equal(a[k], c[k]) for k in a)
+    assert np.array_equal(pa, pc)
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 16])
+def test_fused_brownian_lanes_match_oracle(gpu, lanes):
+    # each slot's list split over 1, 4 or 16 threads (partial sums added in a
+    # fixed butterfly order) against the oracle's sequential sums
+    st = sc.plane_disks(2000, phi=0.8, seed=37)
+    d, o = both(gpu, st)
+    d.set_option("brownian_lanes", lanes)
+    for s in (d, o):
+        s.nlist(2.4, 0.3)
+        s.pair_soft(1.0, 2.0)
+        s.align_polar(0.2, 2.4)
+        s.constraint("plane")
+        s.set_timestep(0.01)
+        s.integrator_brownian(v0=0.2, nu=0.1, seed=13, temp=(0, 0.0, 0.0, 1))
+        s.run(199)
+    assert pos_err(positions(d), positions(o), st["box"], True) <= 1e-9
+    assert np.abs(directors(d) - directors(o)).max() <= 1e-9
+    assert sc.rel_force_err(torques(d), torques(o)) <= 1e-9
+
+
 @pytest.mark.parametrize("method", ["spv", "bbk"])
 def test_langevin_methods_noise_on(gpu, method):
     st = sc.lj_lattice(30, rho=0.6, seed=19)
