@@ -426,8 +426,19 @@ __device__ __forceinline__ void build_cells_staged(const NListD& nl, const BoxD&
       const int q = (ci + ix - bx0) + (cj + iy - by0) * bw;
       const int a = S.cstart[q], e = S.cstart[q + 1];
       for (int m = a; m < e; ++m) {
+        // both LDS reads issued together; the own slot (at distance 0) is
+        // excluded after the distance test, as build_candidate does first
         const double2 p = S.pos[m];
-        build_candidate<true>(nl, b, s, xi, yi, 0.0, ri, wrap, S.mem[m], p.x, p.y, 0.0, rj_u, k, coord, qb);
+        const int t = S.mem[m];
+        double dx = xi - p.x, dy = yi - p.y;
+        if (wrap) min_image_xy(b, dx, dy);
+        const double d2 = dx * dx + dy * dy;
+        if (d2 < nl.cut2 && t != s) {
+          list_push(nl, s, k, (unsigned)t, qb);
+          ++k;
+          const double r = ri + rj_u;
+          if (d2 < r * r) ++coord;
+        }
       }
     }
   }
