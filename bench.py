@@ -625,7 +625,7 @@ def bench_tiled_line(samos_amd, sc, args):
     timed_run(s, max(args.warmup, 20), flags)
     secs = timed_run(s, args.steps, flags)
     s.profile(True)
-    timed_run(s, min(args.steps, 500), flags)
+    timed_run(s, min(max(args.steps, 200), 500), flags)
     ms, launches = s.profile_read("fused")
     s.profile(False)
     avg_s = ms / 1e3 / max(launches, 1)
@@ -730,9 +730,11 @@ def main():
     print("bench: timed %.3f s -> %.4e particle-steps/s" % (secs, value), file=sys.stderr, flush=True)
 
     # Dominant kernel: the fused LJ force + BAOAB step.  HIP events bracket each
-    # launch on the context's stream (step-by-step launches, same kernels).
+    # launch on the context's stream (step-by-step launches, same kernels), over
+    # at least 200 launches so that a short timed run still gets a steady
+    # average (each run's first launch is the heavier phase-0 step).
     s.profile(True)
-    prof_steps = min(args.steps, 500)
+    prof_steps = min(max(args.steps, 200), 500)
     timed_run(s, prof_steps, run_flags)
     ms, launches = s.profile_read("fused")
     rb_ms, rb_launches = s.profile_read("rebuild")
