@@ -687,8 +687,8 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: the pipeline runs one particle system per GPU and this build has no "
-              "multi-GPU path (the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: each particle system runs on one GPU, and there is no multi-GPU "
+              "path to time", file=sys.stderr)
         sys.exit(2)
 
     import samos_amd

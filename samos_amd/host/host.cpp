@@ -328,7 +328,7 @@ NeighbourList::NeighbourList(SystemPtr sys, MessengerPtr msg, double cutoff, dou
     : sys_(sys), cut_(cutoff), pad_(pad) {
   for (const char* k : {"triangulation", "disable_nlist"})
     if (has(params, k))
-      fail(msg, std::string("Neighbour list option ") + k + " (tissue models) is not supported by this build.",
+      fail(msg, std::string("Neighbour list option ") + k + " (tissue models) is not supported by this implementation.",
            "Unsupported neighbour list option.");
   sys_->check(samos_nlist_create(sys_->ctx(), cutoff, pad));
   long long info[8];
@@ -377,7 +377,7 @@ PairSoftPotential::PairSoftPotential(SystemPtr sys, MessengerPtr msg, NeighbourL
     fail(msg_, "Parameter \"" + bad + "\" is not a valid parameter for soft pair potential.",
          "Unknown parameter \"" + bad + "\" in soft potential.");
   if (has(p, "phase_in"))
-    fail(msg_, "Soft pair potential: phase_in (population growth) is not supported by this build.",
+    fail(msg_, "Soft pair potential: phase_in (population growth) is not supported by this implementation.",
          "Unsupported parameter phase_in.");
   if (!has(p, "k")) msg_->msg(Messenger::WARNING, "No potential strength (k) specified for soft pair potential. Setting it to 1.");
   else k_ = to_double(p["k"], "k");
@@ -411,7 +411,7 @@ PairLJPotential::PairLJPotential(SystemPtr sys, MessengerPtr msg, NeighbourListP
     fail(msg_, "Parameter \"" + bad + "\" is not a valid parameter for Lennard-Jones pair potential.",
          "Unknown parameter \"" + bad + "\" in Lennard-Jones potential.");
   if (has(p, "phase_in"))
-    fail(msg_, "Lennard Jones pair potential: phase_in (population growth) is not supported by this build.",
+    fail(msg_, "Lennard Jones pair potential: phase_in (population growth) is not supported by this implementation.",
          "Unsupported parameter phase_in.");
   if (has(p, "epsilon")) eps_ = to_double(p["epsilon"], "epsilon");
   else msg_->msg(Messenger::WARNING, "No potential depth (epsilon) specified for the Lennard Jones pair potential. Setting it to 1.");
@@ -462,7 +462,7 @@ PairPolarAlign::PairPolarAlign(SystemPtr sys, MessengerPtr msg, NeighbourListPtr
     : PairAlign(sys, msg, nlist, p) {
   read_J_rcut(msg_, p, nlist_, J_, rcut_, "polar alignment");
   if (has(p, "exclude_boundary"))
-    fail(msg_, "Polar alignment: exclude_boundary (tissue boundaries) is not supported by this build.",
+    fail(msg_, "Polar alignment: exclude_boundary (tissue boundaries) is not supported by this implementation.",
          "Unsupported parameter exclude_boundary.");
   sys_->check(samos_align_polar(sys_->ctx(), J_, rcut_));
 }
@@ -512,7 +512,7 @@ Constraint::Constraint(SystemPtr sys, MessengerPtr msg, pairs_type& p) : sys_(sy
   if (has(p, "rescale_steps")) rescale_steps_ = (int)to_double(p["rescale_steps"], "rescale_steps");
   if (has(p, "rescale_freq")) rescale_freq_ = (int)to_double(p["rescale_freq"], "rescale_freq");
   if (has(p, "group") && p["group"] != "all")
-    fail(msg_, "Constraints on groups other than \"all\" are not supported by this build.", "Unsupported group.");
+    fail(msg_, "Constraints on groups other than \"all\" are not supported by this implementation.", "Unsupported group.");
   // The reference reads maxiter and tol for every constraint; both go through an
   // integer conversion (constraint.hpp:71,82), so a fractional tolerance such as
   // 1e-8 is a conversion error there as here.
@@ -805,7 +805,7 @@ Dump::Dump(SystemPtr sys, MessengerPtr msg, const std::string& name, pairs_type&
                                                          {"xyzv", "xyzv"},    {"vtp", "vtp"}};
   auto it = ext.find(type_);
   if (it == ext.end())
-    fail(msg_, "Dump type " + type_ + " is not supported by this build (xyz, full, input, velocity, director, xyzv, "
+    fail(msg_, "Dump type " + type_ + " is not supported by this implementation (xyz, full, input, velocity, director, xyzv, "
                                       "dcd, vtp).", "Unsupported dump type");
   ext_ = it->second;
   if (has(params, "directory")) dir_ = params["directory"];
@@ -1041,7 +1041,7 @@ PairExtraPotential::PairExtraPotential(SystemPtr sys, MessengerPtr msg, Neighbou
            "Unknown parameter \"" + bad + "\" in " + sp.label + " potential.");
   }
   if (has(p, "phase_in"))
-    fail(msg_, std::string(sp.label) + " pair potential: phase_in (population growth) is not supported by this build.",
+    fail(msg_, std::string(sp.label) + " pair potential: phase_in (population growth) is not supported by this implementation.",
          "Unsupported parameter phase_in.");
   std::vector<double> q;
   for (auto& kv : sp.params) {
@@ -1058,7 +1058,7 @@ PairExtraPotential::PairExtraPotential(SystemPtr sys, MessengerPtr msg, Neighbou
 }
 
 void PairExtraPotential::set_pair_parameters(pairs_type&) {
-  fail(msg_, "Per-type parameters of the " + name_ + " pair potential are not supported by this build.",
+  fail(msg_, "Per-type parameters of the " + name_ + " pair potential are not supported by this implementation.",
        "Unsupported pair_param for " + name_ + ".");
 }
 

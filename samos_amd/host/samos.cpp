@@ -16,7 +16,7 @@ constexpr double kDefaultCutoff = 3.0;  // src/system/defaults.hpp:36-37
 constexpr double kDefaultPadding = 0.5;
 
 // Commands of the reference grammar (src/parser/parse_command.hpp:66-102) that
-// belong to models outside this build.
+// belong to models outside this implementation.
 const char* kOutOfScope[] = {"external_param", "external_align", "external_align_param", "population",
                              "zero_momentum", "bond", "bond_param", "angle", "angle_param", "read_bonds",
                              "read_angles", "read_cell_boundary", "pair_type_param", "disable_population"};
@@ -63,7 +63,7 @@ struct Driver {
       parse();
       msg->set_file(attrib);
     } else if (cmd == "config") {
-      msg->msg(Messenger::WARNING, "config: the parameter record file is not written by this build.");
+      msg->msg(Messenger::WARNING, "config: the parameter record file is not written by this implementation.");
     } else if (cmd == "box") {
       parse();
       if (attrib == "periodic") periodic = true;
@@ -88,7 +88,7 @@ struct Driver {
       need_system("adding constraint");
       parse();
       if (constraint)
-        fail(msg, "More than one constraint is not supported by this build.", "Unsupported constraint combination.");
+        fail(msg, "More than one constraint is not supported by this implementation.", "Unsupported constraint combination.");
       auto it = constraints().find(attrib);
       if (it == constraints().end())
         fail(msg, "Constraint " + attrib + " is not available (sphere, plane, none, cylinder, torus, ellipsoid)" + where,
@@ -188,7 +188,7 @@ struct Driver {
     } else {
       for (const char* o : kOutOfScope)
         if (cmd == o)
-          fail(msg, "Command " + cmd + " belongs to a model outside this build" + where, "Unsupported command.");
+          fail(msg, "Command " + cmd + " belongs to a model outside this implementation" + where, "Unsupported command.");
       std::cerr << "Error parsing line : " << line_no << std::endl;
       std::cerr << "Unknown command : " << cmd << std::endl;
       throw std::runtime_error("Error parsing command script.");
