@@ -832,7 +832,7 @@ Dump::Dump(SystemPtr sys, MessengerPtr msg, const std::string& name, pairs_type&
     return;
   }
   if (type_ == "vtp") {
-    msg_->msg(Messenger::WARNING, "vtp dumps need VTK, which this build does not use: no file is written.");
+    msg_->msg(Messenger::WARNING, "vtp dumps need VTK, which this implementation does not use: no file is written.");
     return;
   }
   if (!multi_) single_.open(dir_ + "/" + name_ + "." + ext_);

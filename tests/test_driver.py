@@ -58,7 +58,7 @@ def test_missing_input_file(driver, tmp_path):
 
 def test_out_of_scope_command(driver, tmp_path):
     r = run_conf(tmp_path, "bond harmonic { k = 1 }\n")
-    assert r.returncode != 0 and "outside this build" in r.stderr
+    assert r.returncode != 0 and "outside this implementation" in r.stderr
 
 
 # ------------------------------------------------------------------ GPU ---
