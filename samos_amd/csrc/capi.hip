@@ -156,6 +156,7 @@ __global__ void k_fill_i(int* p, int v, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
 
+
 void fill_d(hipStream_t st, double* p, double v, long n) {
   const int g = (int)std::max<long>(1, std::min<long>((n + kBlock - 1) / kBlock, 1024));
   hipLaunchKernelGGL(k_fill_d, dim3(g), dim3(kBlock), 0, st, p, v, n);
@@ -728,6 +729,12 @@ int sync_fire_groups(samos_ctx* ctx) {
   return 0;
 }
 
+// Clears a few flag words on the run paths: one small kernel launch costs the
+// host less time than hipMemsetAsync, and the run's first kernel waits on it.
+void clear_flags(samos_ctx* ctx, int first, int count) {
+  hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, ctx->st, ctx->d_flags + first, 0, (long)count);
+}
+
 // ---- fused Langevin run ---------------------------------------------------
 int upload_clock(samos_ctx* ctx, int run_step) {
   ClockD c{};
@@ -932,7 +939,7 @@ int run_fused_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
       if (int rc = recover_overflow(ctx, f0, planar); rc < 0) return rc;
     }
     // the last chunk served any request; direct steps use the parity flags
-    if (t < b) HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
+    if (t < b) clear_flags(ctx, 0, 2);
   }
   for (; t < b; ++t) {
     if (int r = fused_step(ctx, I, 1, t, buf, 0, steps, f32, planar, mass_u, true, nullptr, 0, true, call_of(t)))
@@ -1021,7 +1028,7 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
     if (r <= 0) return r;
   }
   if (int r = upload_clock(ctx, a)) return r;
-  HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
+  clear_flags(ctx, 0, 2);
   if (planar) {
     Fields f0 = buffer(0);
     launch_pack_xy(ctx->st, f0, n);
@@ -1099,7 +1106,7 @@ int run_fused_brownian(samos_ctx* ctx, int a, int b, int steps, int flags) {
       // stopped at an overflowed rebuild of the positions now in buffer 0
       if (int rc = recover_overflow(ctx, f0, planar); rc < 0) return rc;
     }
-    HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
+    clear_flags(ctx, 0, 2);
   }
   const int t0 = t;
   for (; t < b; ++t) {
@@ -1344,7 +1351,7 @@ int run_tiled_langevin(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     after[4] = before[4] + h.rebuilds;
     HIPCHK(hipMemcpy(ctx->d_flags + 4, &after[4], sizeof(int), hipMemcpyHostToDevice));
   }
-  HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
+  clear_flags(ctx, 0, 2);
   I.calls += (unsigned long long)(b - a);
   // one closing half-kick, and one age step, per iteration
   launch_age_finish(ctx->st, ctx->d_age, ctx->dcol[16], ctx->n, I.dt, b - a);
@@ -1475,7 +1482,7 @@ int run_tiled_brownian(samos_ctx* ctx, Integ& I, int a, int b, int steps, int fl
     after[4] = before[4] + h.rebuilds;
     HIPCHK(hipMemcpy(ctx->d_flags + 4, &after[4], sizeof(int), hipMemcpyHostToDevice));
   }
-  HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->st));
+  clear_flags(ctx, 0, 2);
   launch_age_finish(ctx->st, ctx->d_age, ctx->dcol[16], ctx->n, I.dt, b - a);
   I.calls += (unsigned long long)(b - a);
   // loop head of b: its enforce was applied with the positions; forces and torques
@@ -2193,7 +2200,7 @@ int run_range_core(samos_ctx* ctx, int t_begin, int t_end, int steps, int flags)
   if (int r = sync_fire_groups(ctx)) return r;
   const bool f32 = (flags & SAMOS_RUN_FP32_FORCE) != 0;
   // rebuild requests of both step parities and the overflow flag
-  HIPCHK(hipMemsetAsync(ctx->d_flags, 0, 3 * sizeof(int), ctx->st));
+  clear_flags(ctx, 0, 3);
   // Fused schedule: one Langevin BAOAB integrator, pair potentials only.  The
   // reference's loop-head force evaluation repeats the evaluation that ended
   // the previous step on the same positions, so it is done once.
